@@ -1,0 +1,169 @@
+"""Benchmark of the heat-operator hot path on one MI355X.
+
+Workload (one "step"): one FP64 application of the heat tangent
+J = rho_cp/dt M + K_k on the unit cube, 64^3 hex cells, FE_Q(4) on
+Gauss-Lobatto points with QGauss(5) (N = 257^3 = 16,974,593 DoFs), homogeneous
+Dirichlet on all faces, k(x) = 1 + 0.5 sin(2 pi x) sin(2 pi y) sin(2 pi z) at
+the quadrature points, rho_cp = 1, dt = 1e-3; input uniform[-1,1] on the free
+DoFs from SplitMix64 seed 0. Inputs are resident in HBM before timing.
+
+Printed: one JSON line with value = GDoF/s (N / time per vmult), the
+roofline of the vmult kernel, the CPU baseline (the oracle's C++ restatement
+timed on this host), and the CG + Jacobi time-to-1e-8 of the FP64 and the
+FP32-mixed solver on the same operator.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N_CELLS = 64
+DEGREE = 4
+DT = 1e-3
+K_AMP = 0.5
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def algorithmic_counts(n_cells, degree, bytes_per=8):
+    """Bytes and flops of one vmult (DoF vectors read + written once; analytic
+    coefficients and the implicit lattice index map cost no bytes; flops by the
+    even-odd sum-factorisation count 12 n_q^4 + 36 n_q^3 per cell)."""
+    n1 = n_cells * degree + 1
+    N = n1**3
+    nq = degree + 1
+    cells = n_cells**3
+    return {"N": N, "N_free": (n1 - 2) ** 3, "cells": cells, "bytes": 2 * bytes_per * N,
+            "flops": cells * (12 * nq**4 + 36 * nq**3)}
+
+
+def cpu_baseline(sample_vmults=3, threads=None):
+    """Times the oracle's cell-wise sum-factorised FP64 vmult on the host cores
+    (the reference cannot be built here: it needs deal.II + p4est + Trilinos + MPI)."""
+    from oracle import cpu
+    threads = threads or min(16, os.cpu_count() or 1)
+    cpu.set_threads(threads)
+    pb = cpu.Problem(N_CELLS, DEGREE, dt=DT, k_amplitude=K_AMP)
+    u = pb.random(0)
+    pb.vmult(u)  # warm-up
+    t0 = time.perf_counter()
+    for _ in range(sample_vmults):
+        pb.vmult(u)
+    t = (time.perf_counter() - t0) / sample_vmults
+    return {"value": round(pb.N / t / 1e9, 4), "unit": "GDoF/s", "cores": threads, "kind": "port",
+            "sample": f"{sample_vmults} FP64 vmults of the 64^3 Q4 operator (C++ restatement of the "
+                      f"reference cell loop, OpenMP {threads} threads), {t*1e3:.1f} ms each"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-solve", action="store_true", help="skip the CG time-to-solution legs")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU design is "
+              "described in DESIGN.md and is not built", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from meltpooldg_amd import HeatDiffuseOperator, LinearSolver, LinearSolverData
+    from oracle.rng import random_vector
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    op = HeatDiffuseOperator(N_CELLS, DEGREE, conductivity_amplitude=K_AMP)
+    op.reset_time_increment(DT)
+    cnt = algorithmic_counts(N_CELLS, DEGREE)
+    N = op.n_dofs()
+    assert N == cnt["N"]
+    src = torch.tensor(random_vector(N_CELLS, DEGREE, 0), dtype=torch.float64, device=dev)
+    dst = torch.empty_like(src)
+
+    stream = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        op.vmult(dst, src)
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        op.vmult(dst, src)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_event = ev0.elapsed_time(ev1) / args.steps
+    ms_step = wall * 1e3 / args.steps
+    gdofs = N / (ms_step * 1e-3) / 1e9
+
+    # vmult = 8 launches of the cell kernel (one per colour); per-launch bytes
+    # and flops are 1/8 of the vmult's, per-launch time 1/8 of the vmult's
+    t_k = ms_event * 1e-3
+    ach_flops = cnt["flops"] / t_k / 1e12
+    ach_bytes = cnt["bytes"] / t_k / 1e9
+    roofline = {"bound": "mfma", "achieved": round(ach_flops, 3), "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(ach_flops / FP64_PEAK_TFLOPS, 4), "traffic": None,
+                "kernel": "vmult_kernel<4,double,sine> x 8 colour passes",
+                "launch_ms": round(ms_event / 8, 5),
+                "hbm_achieved_GBs": round(ach_bytes, 1),
+                "hbm_frac": round(ach_bytes / HBM_PEAK_GBS, 4),
+                "algorithmic_bytes_per_vmult": cnt["bytes"],
+                "algorithmic_flops_per_vmult": cnt["flops"]}
+
+    solve = None
+    if not args.no_solve:
+        b = torch.tensor(random_vector(N_CELLS, DEGREE, 1), dtype=torch.float64, device=dev)
+        solve = {}
+        for name, mixed in (("fp64", False), ("fp32_mixed", True)):
+            x = torch.zeros_like(b)
+            data = LinearSolverData(solver_type="CG", preconditioner_type="Diagonal",
+                                    rel_tolerance=1e-8, mixed_precision=mixed)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            LinearSolver.solve(op, x, b, data)
+            torch.cuda.synchronize()
+            t = time.perf_counter() - t0
+            rep = LinearSolver.last_report
+            solve[name] = {"seconds": round(t, 4), "iterations": rep["iterations"],
+                           "outer_iterations": rep["outer_iterations"],
+                           "rel_residual": rep["final_residual"] / rep["initial_residual"]}
+            if mixed:
+                solve[name]["speedup_vs_fp64"] = round(solve["fp64"]["seconds"] / t, 3)
+
+    cpu = None if args.no_cpu else cpu_baseline()
+
+    line = {
+        "metric": "GDoF/s heat vmult (Q4, 64^3); CG time-to-1e-8: FP32-mixed vs FP64, 1 GPU",
+        "value": round(gdofs, 3),
+        "unit": "GDoF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (SplitMix64 seed 0 uniform[-1,1] on free DoFs)",
+        "config": {"workload": "heat vmult, unit cube 64^3 hex cells, FE_Q(4) GLL / QGauss(5), "
+                               "k(x)=1+0.5 sin sin sin, rho_cp=1, dt=1e-3, Dirichlet all faces",
+                   "n_dofs": N, "n_free_dofs": cnt["N_free"], "cells": cnt["cells"],
+                   "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "solve_time_to_1e-8": solve,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

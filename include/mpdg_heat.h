@@ -1,0 +1,157 @@
+/*
+ * mpdg_heat.h - C ABI of the MI355X matrix-free heat-operator engine.
+ *
+ * Drop-in for the hot path of MeltPoolDG's diffuse heat operator:
+ *   HeatDiffuseMultiPhaseOperator<3,double>          (include/meltpooldg/heat/heat_diffuse_operator.hpp:109-366)
+ *   ::vmult(dst, src)                                 (source/heat/heat_diffuse_operator.cpp:179-197)
+ *   ::compute_inverse_diagonal_from_matrixfree(diag)  (source/heat/heat_diffuse_operator.cpp:271-284)
+ *   OperatorBase::reset_time_increment(dt)            (include/meltpooldg/core/operator_base.hpp:41-46)
+ *   LinearSolver::solve(op, x, b, data, prec)         (include/meltpooldg/linear_algebra/linear_solver.hpp:26-134)
+ *   JacobiPreconditioner::vmult/update                (include/meltpooldg/linear_algebra/preconditioner_jacobi.hpp:51-69)
+ *
+ * Conventions
+ *   - Plain C types only. Vector arguments are DEVICE pointers (HBM) of
+ *     n_dofs() entries in lexicographic lattice numbering
+ *     I = ix + N1x*(iy + N1y*iz), N1d = n_cells[d]*degree + 1.
+ *   - Homogeneous Dirichlet DoFs (lattice nodes on the marked box faces) follow
+ *     the reference's constraint semantics (include/meltpooldg/utilities/
+ *     constraints.hpp:49-108): vmult ignores src there and writes dst = 0;
+ *     the inverse diagonal is 1 there.
+ *   - Every call returns an mpdg_status; 0 is success. No C++ exception crosses
+ *     the boundary. mpdg_last_error() returns a thread-local message.
+ *   - A handle is single-caller: calls on one handle are serialised on its
+ *     stream (the reference's vmult is likewise not re-entrant).
+ */
+#ifndef MPDG_HEAT_H
+#define MPDG_HEAT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  MPDG_OK = 0,
+  MPDG_ERR_ARGUMENT = 1,      /* invalid descriptor, size or pointer */
+  MPDG_ERR_HIP = 2,           /* a HIP runtime call or kernel launch failed */
+  MPDG_ERR_NOT_CONVERGED = 3, /* SolverControl::NoConvergence (linear_solver.hpp:124-129) */
+  MPDG_ERR_UNSUPPORTED = 4,   /* degree / mode combination not built */
+  MPDG_ERR_ZERO_TIME_INCREMENT = 5 /* AssertThrowZeroTimeIncrement (core/exceptions.hpp:6) */
+} mpdg_status;
+
+typedef enum { MPDG_FP64 = 0, MPDG_FP32 = 1 } mpdg_precision;
+
+/* Material model (include/meltpooldg/core/material.hpp:46-55). */
+typedef enum {
+  MPDG_MATERIAL_GAS = 0,        /* constant rho, cp, k */
+  MPDG_MATERIAL_GAS_LIQUID = 1  /* blend by a nodal heaviside field, mpdg_heat_set_heaviside */
+} mpdg_material_type;
+
+/* Extra conductivity field on top of the material's conductivity. */
+typedef enum {
+  MPDG_CONDUCTIVITY_MATERIAL = 0,
+  MPDG_CONDUCTIVITY_SEPARABLE_SINE = 1 /* k += amp * sin(2 pi x) sin(2 pi y) sin(2 pi z) at q */
+} mpdg_conductivity_field;
+
+/* Prescribed advection velocity evaluated at the quadrature points. */
+typedef enum {
+  MPDG_VELOCITY_NONE = 0,
+  MPDG_VELOCITY_VORTEX = 1 /* (-sin^2(pi x) sin(2 pi y), sin(2 pi x) sin^2(pi y), w0) */
+} mpdg_velocity_field;
+
+typedef struct {
+  double density;
+  double specific_heat_capacity;
+  double thermal_conductivity;
+} mpdg_phase;
+
+typedef struct {
+  int n_cells[3];              /* cells per direction of the box mesh */
+  int degree;                  /* FE_Q degree p, 1..6; QGauss(p+1) */
+  double box_lo[3], box_hi[3];
+  int precision;               /* mpdg_precision of the cell loop */
+  int material;                /* mpdg_material_type */
+  mpdg_phase gas, liquid;      /* MaterialData gas / liquid */
+  int two_phase_sharp;         /* TwoPhaseFluidPropertiesTransitionType::sharp */
+  int volume_specific_heat;    /* use_volume_specific_thermal_capacity_for_phase_interpolation */
+  int conductivity_field;      /* mpdg_conductivity_field */
+  double conductivity_amplitude;
+  int velocity;                /* mpdg_velocity_field */
+  double velocity_w0;
+  unsigned dirichlet_faces;    /* bit 2d: face x_d = lo, bit 2d+1: x_d = hi; 0x3F = all */
+  void *stream;                /* hipStream_t the handle launches on; NULL = default stream */
+} mpdg_heat_desc;
+
+typedef struct mpdg_heat mpdg_heat;
+
+/* Krylov method (include/meltpooldg/linear_algebra/linear_solver_data.hpp:16-25). */
+typedef enum { MPDG_SOLVER_CG = 0, MPDG_SOLVER_GMRES = 1 } mpdg_solver_type;
+typedef enum { MPDG_PRECONDITIONER_IDENTITY = 0, MPDG_PRECONDITIONER_DIAGONAL = 3 } mpdg_preconditioner_type;
+
+typedef struct {
+  int solver_type;          /* mpdg_solver_type; LinearSolverData::solver_type */
+  int preconditioner_type;  /* mpdg_preconditioner_type; Diagonal = JacobiPreconditioner */
+  int max_iterations;       /* LinearSolverData::max_iterations (default 10000) */
+  double rel_tolerance;     /* LinearSolverData::rel_tolerance */
+  double abs_tolerance;     /* LinearSolverData::abs_tolerance (default 1e-20) */
+  int gmres_restart;        /* GMRES basis size (30) */
+  int mixed_precision;      /* 1: FP32 cell loop inside FP64 defect correction */
+  double inner_rel_tolerance; /* mixed: relative reduction of each inner FP32 solve */
+  int check_interval;       /* iterations between host convergence checks (0 = auto) */
+} mpdg_solver_params;
+
+typedef struct {
+  int iterations;           /* SolverControl::last_step(); mixed: total inner iterations */
+  int outer_iterations;     /* mixed precision: defect-correction steps */
+  double initial_residual;  /* ||b - A x0||_2 */
+  double final_residual;    /* ||b - A x||_2 at exit (FP64, recomputed for mixed) */
+  double seconds;           /* wall time of the solve including the diagonal update */
+} mpdg_solver_report;
+
+/* ---- handle ------------------------------------------------------------ */
+/* Replaces the operator construction + ScratchData::build (source/core/scratch_data.cpp:190-268). */
+mpdg_status mpdg_heat_create(const mpdg_heat_desc *desc, mpdg_heat **out);
+mpdg_status mpdg_heat_destroy(mpdg_heat *h);
+/* Total lattice DoFs N (deal.II n_dofs()) and free DoFs. */
+int64_t mpdg_heat_n_dofs(const mpdg_heat *h);
+int64_t mpdg_heat_n_free_dofs(const mpdg_heat *h);
+
+/* OperatorBase::reset_time_increment (operator_base.hpp:41-46); dt = 0 is rejected. */
+mpdg_status mpdg_heat_set_dt(mpdg_heat *h, double dt);
+/* level_set_as_heaviside registration (heat_diffuse_operator.hpp:137-143): device
+ * pointer to n_dofs() doubles, read plain (no constraints) and interpolated at q.
+ * The engine copies it (and converts to the cell-loop precision). */
+mpdg_status mpdg_heat_set_heaviside(mpdg_heat *h, const double *heaviside);
+
+/* HeatDiffuseMultiPhaseOperator::vmult (heat_diffuse_operator.cpp:179-197), FP64 vectors. */
+mpdg_status mpdg_heat_vmult(mpdg_heat *h, double *dst, const double *src);
+/* Same operator with FP32 vectors and FP32 cell loop (inner operator of the mixed solve). */
+mpdg_status mpdg_heat_vmult_f32(mpdg_heat *h, float *dst, const float *src);
+/* compute_inverse_diagonal_from_matrixfree (heat_diffuse_operator.cpp:271-284), FP64. */
+mpdg_status mpdg_heat_inverse_diagonal(mpdg_heat *h, double *inv_diag);
+/* JacobiPreconditioner::vmult (preconditioner_jacobi.hpp:51-56): dst = inv_diag .* src. */
+mpdg_status mpdg_heat_jacobi_vmult(mpdg_heat *h, double *dst, const double *src,
+                                   const double *inv_diag);
+
+/* LinearSolver::solve (linear_solver.hpp:26-134) with JacobiPreconditioner::update
+ * (preconditioner_jacobi.hpp:65-69) done inside. x holds the initial guess on entry.
+ * Returns MPDG_ERR_NOT_CONVERGED (report filled) where the reference throws. */
+mpdg_status mpdg_heat_solve(mpdg_heat *h, double *x, const double *b,
+                            const mpdg_solver_params *params, mpdg_solver_report *report);
+
+/* Default parameters: LinearSolverData defaults with CG + Diagonal. */
+void mpdg_solver_params_default(mpdg_solver_params *p);
+
+/* Thread-local description of the last error on this thread. */
+const char *mpdg_last_error(void);
+
+/* Kernel timing hook for measurement: average device time of the last
+ * n_launches vmult launches (ms), recorded with HIP events on the handle's stream. */
+mpdg_status mpdg_heat_time_vmult(mpdg_heat *h, double *dst, const double *src, int repeats,
+                                 int precision, double *ms_per_vmult);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,122 @@
+"""ctypes binding of the C ABI in include/mpdg_heat.h (libmpdg_heat.so).
+
+The library is the only compute path: if it is missing or fails to load the
+import raises, there is no CPU fallback.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmpdg_heat.so")
+
+MPDG_OK = 0
+MPDG_ERR_ARGUMENT = 1
+MPDG_ERR_HIP = 2
+MPDG_ERR_NOT_CONVERGED = 3
+MPDG_ERR_UNSUPPORTED = 4
+MPDG_ERR_ZERO_TIME_INCREMENT = 5
+
+MPDG_FP64 = 0
+MPDG_FP32 = 1
+
+
+class Phase(ctypes.Structure):
+    _fields_ = [("density", ctypes.c_double), ("specific_heat_capacity", ctypes.c_double),
+                ("thermal_conductivity", ctypes.c_double)]
+
+
+class HeatDesc(ctypes.Structure):
+    _fields_ = [
+        ("n_cells", ctypes.c_int * 3),
+        ("degree", ctypes.c_int),
+        ("box_lo", ctypes.c_double * 3),
+        ("box_hi", ctypes.c_double * 3),
+        ("precision", ctypes.c_int),
+        ("material", ctypes.c_int),
+        ("gas", Phase),
+        ("liquid", Phase),
+        ("two_phase_sharp", ctypes.c_int),
+        ("volume_specific_heat", ctypes.c_int),
+        ("conductivity_field", ctypes.c_int),
+        ("conductivity_amplitude", ctypes.c_double),
+        ("velocity", ctypes.c_int),
+        ("velocity_w0", ctypes.c_double),
+        ("dirichlet_faces", ctypes.c_uint),
+        ("stream", ctypes.c_void_p),
+    ]
+
+
+class SolverParams(ctypes.Structure):
+    _fields_ = [
+        ("solver_type", ctypes.c_int),
+        ("preconditioner_type", ctypes.c_int),
+        ("max_iterations", ctypes.c_int),
+        ("rel_tolerance", ctypes.c_double),
+        ("abs_tolerance", ctypes.c_double),
+        ("gmres_restart", ctypes.c_int),
+        ("mixed_precision", ctypes.c_int),
+        ("inner_rel_tolerance", ctypes.c_double),
+        ("check_interval", ctypes.c_int),
+    ]
+
+
+class SolverReport(ctypes.Structure):
+    _fields_ = [
+        ("iterations", ctypes.c_int),
+        ("outer_iterations", ctypes.c_int),
+        ("initial_residual", ctypes.c_double),
+        ("final_residual", ctypes.c_double),
+        ("seconds", ctypes.c_double),
+    ]
+
+
+# name -> (restype, argtypes); exactly the entry points include/mpdg_heat.h declares
+_VP = ctypes.c_void_p
+SIGNATURES = {
+    "mpdg_heat_create": (ctypes.c_int, [ctypes.POINTER(HeatDesc), ctypes.POINTER(_VP)]),
+    "mpdg_heat_destroy": (ctypes.c_int, [_VP]),
+    "mpdg_heat_n_dofs": (ctypes.c_int64, [_VP]),
+    "mpdg_heat_n_free_dofs": (ctypes.c_int64, [_VP]),
+    "mpdg_heat_set_dt": (ctypes.c_int, [_VP, ctypes.c_double]),
+    "mpdg_heat_set_heaviside": (ctypes.c_int, [_VP, _VP]),
+    "mpdg_heat_vmult": (ctypes.c_int, [_VP, _VP, _VP]),
+    "mpdg_heat_vmult_f32": (ctypes.c_int, [_VP, _VP, _VP]),
+    "mpdg_heat_inverse_diagonal": (ctypes.c_int, [_VP, _VP]),
+    "mpdg_heat_jacobi_vmult": (ctypes.c_int, [_VP, _VP, _VP, _VP]),
+    "mpdg_heat_solve": (ctypes.c_int, [_VP, _VP, _VP, ctypes.POINTER(SolverParams),
+                                       ctypes.POINTER(SolverReport)]),
+    "mpdg_solver_params_default": (None, [ctypes.POINTER(SolverParams)]),
+    "mpdg_last_error": (ctypes.c_char_p, []),
+    "mpdg_heat_time_vmult": (ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int, ctypes.c_int,
+                                            ctypes.POINTER(ctypes.c_double)]),
+}
+
+_lib = None
+
+
+def load():
+    """Load libmpdg_heat.so (built by meltpooldg_amd.build). Raises if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run `python -m meltpooldg_amd.build` "
+                          "(the HIP library is the only compute path)")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class MpdgError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"mpdg status {status}: {message}")
+        self.status = status
+
+
+def check(status):
+    if status != MPDG_OK:
+        raise MpdgError(status, load().mpdg_last_error().decode())

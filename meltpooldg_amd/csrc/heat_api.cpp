@@ -1,0 +1,648 @@
+// C ABI of the heat-operator engine (include/mpdg_heat.h): handle set-up
+// (the MatrixFree "plan" of a box mesh), vmult, inverse diagonal and the
+// device-resident Krylov drivers.
+#include "../../include/mpdg_heat.h"
+
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+#include "tables.hpp"
+
+using namespace mpdg;
+
+namespace
+{
+  thread_local std::string g_last_error;
+
+  mpdg_status fail(mpdg_status s, const std::string &msg)
+  {
+    g_last_error = msg;
+    return s;
+  }
+
+#define MPDG_HIP(expr)                                                                          \
+  do                                                                                            \
+    {                                                                                           \
+      const hipError_t e_ = (expr);                                                             \
+      if (e_ != hipSuccess)                                                                     \
+        return fail(MPDG_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));           \
+    }                                                                                           \
+  while (0)
+
+  template <typename T>
+  struct DevBuf
+  {
+    T *ptr = nullptr;
+    size_t n = 0;
+    hipError_t alloc(size_t count)
+    {
+      if (count <= n && ptr)
+        return hipSuccess;
+      if (ptr)
+        (void)hipFree(ptr);
+      ptr = nullptr;
+      n = 0;
+      const hipError_t e = hipMalloc(&ptr, count * sizeof(T) + 64);
+      if (e == hipSuccess)
+        n = count;
+      return e;
+    }
+    void release()
+    {
+      if (ptr)
+        (void)hipFree(ptr);
+      ptr = nullptr;
+      n = 0;
+    }
+  };
+} // namespace
+
+struct mpdg_heat
+{
+  mpdg_heat_desc desc{};
+  hipStream_t stream = nullptr;
+  int device = 0;
+  Tables1D t1d;
+  int64_t N = 0, N_free = 0;
+  double h[3]{};
+  double dt = 0.0;
+  bool dt_set = false;
+
+  CellParams<double> prm64{};
+  CellParams<float> prm32{};
+  VmultTables<double> vt64{};
+  VmultTables<float> vt32{};
+  DiagTables<double> dt64{};
+
+  DevBuf<double> sin_tab64, sin_tab32_src; // sin(2 pi x), sin^2(pi x) per direction (double)
+  DevBuf<float> sin_tab32;
+  DevBuf<double> heav64;
+  DevBuf<float> heav32;
+  bool has_heaviside = false;
+
+  // solver workspace
+  DevBuf<double> r, p, q, inv_diag, partials, history;
+  DevBuf<float> r32, p32, q32, x32, inv32;
+  DevBuf<SolverScalars> scal;
+  SolverScalars *host_scal = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+  ~mpdg_heat()
+  {
+    for (auto *b : {&sin_tab64, &sin_tab32_src, &heav64, &r, &p, &q, &inv_diag, &partials, &history})
+      b->release();
+    for (auto *b : {&sin_tab32, &heav32, &r32, &p32, &q32, &x32, &inv32})
+      b->release();
+    scal.release();
+    if (host_scal)
+      (void)hipHostFree(host_scal);
+    if (ev0)
+      (void)hipEventDestroy(ev0);
+    if (ev1)
+      (void)hipEventDestroy(ev1);
+  }
+};
+
+namespace
+{
+  template <typename Number>
+  void fill_params(mpdg_heat *H, CellParams<Number> &prm, const Number *tab)
+  {
+    const mpdg_heat_desc &d = H->desc;
+    const int n = H->t1d.n;
+    prm.nx = d.n_cells[0];
+    prm.ny = d.n_cells[1];
+    prm.nz = d.n_cells[2];
+    prm.n1x = d.n_cells[0] * d.degree + 1;
+    prm.n1y = d.n_cells[1] * d.degree + 1;
+    prm.n1z = d.n_cells[2] * d.degree + 1;
+    prm.faces = d.dirichlet_faces;
+    const double vol = H->h[0] * H->h[1] * H->h[2];
+    const double g_rhocp = d.gas.density * d.gas.specific_heat_capacity;
+    const double l_rhocp = d.liquid.density * d.liquid.specific_heat_capacity;
+    const double dt_inv = H->dt_set ? 1.0 / H->dt : 0.0; // OperatorBase::reset_time_increment
+    prm.mass = (Number)(dt_inv * g_rhocp * vol);
+    prm.dtinv_vol = (Number)(dt_inv * vol);
+    for (int k = 0; k < 3; ++k)
+      {
+        prm.kf[k] = (Number)(vol / (H->h[k] * H->h[k]));
+        prm.vf[k] = (Number)(vol / H->h[k]);
+      }
+    prm.k0 = (Number)d.gas.thermal_conductivity;
+    prm.kamp = (Number)(d.conductivity_field == MPDG_CONDUCTIVITY_SEPARABLE_SINE ?
+                          d.conductivity_amplitude :
+                          0.0);
+    prm.rho_cp0 = (Number)g_rhocp;
+    prm.w0 = (Number)d.velocity_w0;
+    prm.g_rho = (Number)d.gas.density;
+    prm.g_cp = (Number)d.gas.specific_heat_capacity;
+    prm.g_k = (Number)d.gas.thermal_conductivity;
+    prm.l_rho = (Number)d.liquid.density;
+    prm.l_cp = (Number)d.liquid.specific_heat_capacity;
+    prm.l_k = (Number)d.liquid.thermal_conductivity;
+    prm.g_rhocp = (Number)g_rhocp;
+    prm.l_rhocp = (Number)l_rhocp;
+    prm.sharp = d.two_phase_sharp;
+    prm.volume_specific = d.volume_specific_heat;
+    // per-direction tables: [sin2pi dir 0 | dir 1 | dir 2 | sinpi2 dir 0 | dir 1 | dir 2]
+    size_t off = 0;
+    for (int k = 0; k < 3; ++k)
+      {
+        prm.sin2pi[k] = tab + off;
+        off += (size_t)d.n_cells[k] * n;
+      }
+    for (int k = 0; k < 3; ++k)
+      {
+        prm.sinpi2[k] = tab + off;
+        off += (size_t)d.n_cells[k] * n;
+      }
+  }
+
+  int coef_mode(const mpdg_heat *H)
+  {
+    if (H->desc.material == MPDG_MATERIAL_GAS_LIQUID)
+      return kCoefHeaviside;
+    return H->desc.conductivity_field == MPDG_CONDUCTIVITY_SEPARABLE_SINE ? kCoefSine :
+                                                                           kCoefConstant;
+  }
+
+  mpdg_status check_ready(const mpdg_heat *H)
+  {
+    if (!H)
+      return fail(MPDG_ERR_ARGUMENT, "null handle");
+    if (!H->dt_set)
+      return fail(MPDG_ERR_ZERO_TIME_INCREMENT,
+                  "time increment not set: call mpdg_heat_set_dt (reset_time_increment) first");
+    if (H->desc.material == MPDG_MATERIAL_GAS_LIQUID && !H->has_heaviside)
+      return fail(MPDG_ERR_ARGUMENT, "gas_liquid material needs mpdg_heat_set_heaviside");
+    return MPDG_OK;
+  }
+
+  mpdg_status ensure_workspace(mpdg_heat *H, bool mixed)
+  {
+    const ColourGrid cg = colour_grid(H->desc.n_cells[0], H->desc.n_cells[1], H->desc.n_cells[2],
+                                      cells_per_block(H->desc.degree, false));
+    const ColourGrid cg32 = colour_grid(H->desc.n_cells[0], H->desc.n_cells[1],
+                                        H->desc.n_cells[2], cells_per_block(H->desc.degree, true));
+    const size_t np = std::max<size_t>({(size_t)cg.total, (size_t)cg32.total, 2 * (size_t)kVecBlocks});
+    MPDG_HIP(H->partials.alloc(np));
+    MPDG_HIP(H->r.alloc(H->N));
+    MPDG_HIP(H->p.alloc(H->N));
+    MPDG_HIP(H->q.alloc(H->N));
+    MPDG_HIP(H->inv_diag.alloc(H->N));
+    MPDG_HIP(H->scal.alloc(1));
+    if (!H->host_scal)
+      MPDG_HIP(hipHostMalloc((void **)&H->host_scal, sizeof(SolverScalars), hipHostMallocDefault));
+    if (mixed)
+      {
+        MPDG_HIP(H->r32.alloc(H->N));
+        MPDG_HIP(H->p32.alloc(H->N));
+        MPDG_HIP(H->q32.alloc(H->N));
+        MPDG_HIP(H->x32.alloc(H->N));
+        MPDG_HIP(H->inv32.alloc(H->N));
+      }
+    return MPDG_OK;
+  }
+
+  mpdg_status read_scalars(mpdg_heat *H)
+  {
+    MPDG_HIP(hipMemcpyAsync(H->host_scal, H->scal.ptr, sizeof(SolverScalars),
+                            hipMemcpyDeviceToHost, H->stream));
+    MPDG_HIP(hipStreamSynchronize(H->stream));
+    return MPDG_OK;
+  }
+
+  mpdg_status vmult64(mpdg_heat *H, double *dst, const double *src, double *partials,
+                      SolverScalars *scal)
+  {
+    MPDG_HIP(launch_vmult<double>(H->desc.degree, coef_mode(H), H->desc.velocity != 0, H->prm64,
+                                  H->vt64, src, dst, H->heav64.ptr, partials, scal, H->stream));
+    return MPDG_OK;
+  }
+
+  mpdg_status vmult32(mpdg_heat *H, float *dst, const float *src, double *partials,
+                      SolverScalars *scal)
+  {
+    MPDG_HIP(launch_vmult<float>(H->desc.degree, coef_mode(H), H->desc.velocity != 0, H->prm32,
+                                 H->vt32, src, dst, H->heav32.ptr, partials, scal, H->stream));
+    return MPDG_OK;
+  }
+
+  mpdg_status diagonal_inverse(mpdg_heat *H, double *inv)
+  {
+    MPDG_HIP(launch_diagonal<double>(H->desc.degree, coef_mode(H), H->desc.velocity != 0,
+                                     H->prm64, H->dt64, inv, H->heav64.ptr, H->stream));
+    MPDG_HIP(launch_invert_diagonal(inv, H->N, H->stream));
+    return MPDG_OK;
+  }
+
+  // Device PCG on precision Number; x and the workspace vectors in Number,
+  // b in double. Returns through scal (iteration, status, res).
+  template <typename Number>
+  mpdg_status pcg(mpdg_heat *H, Number *x, const double *b, Number *r, Number *p, Number *q,
+                  const Number *inv, double rel_tol, double abs_tol, int max_it, int check,
+                  bool x_is_zero)
+  {
+    double *part = H->partials.ptr;
+    SolverScalars *sc = H->scal.ptr;
+    if (x_is_zero)
+      {
+        MPDG_HIP(launch_cg_init<Number>(b, nullptr, r, p, inv, H->N, rel_tol, abs_tol, max_it,
+                                        part, sc, H->stream));
+      }
+    else
+      {
+        mpdg_status st;
+        if constexpr (sizeof(Number) == 8)
+          st = vmult64(H, q, x, nullptr, nullptr);
+        else
+          st = vmult32(H, q, x, nullptr, nullptr);
+        if (st)
+          return st;
+        MPDG_HIP(launch_cg_init<Number>(b, q, r, p, inv, H->N, rel_tol, abs_tol, max_it, part, sc,
+                                        H->stream));
+      }
+    for (int done_it = 0;;)
+      {
+        for (int k = 0; k < check; ++k)
+          {
+            mpdg_status st;
+            if constexpr (sizeof(Number) == 8)
+              st = vmult64(H, q, p, part, sc);
+            else
+              st = vmult32(H, q, p, part, sc);
+            if (st)
+              return st;
+            MPDG_HIP(launch_cg_update<Number>(x, r, p, q, inv, H->N, part, sc, H->history.ptr,
+                                              (int)H->history.n, H->stream));
+            MPDG_HIP(launch_cg_direction<Number>(p, r, inv, H->N, sc, H->stream));
+          }
+        done_it += check;
+        if (mpdg_status st = read_scalars(H))
+          return st;
+        if (H->host_scal->done)
+          break;
+        (void)done_it;
+      }
+    return MPDG_OK;
+  }
+
+  int auto_check(const mpdg_heat *H)
+  {
+    // poll less often when an iteration is cheap
+    return H->N < (1 << 20) ? 16 : 8;
+  }
+} // namespace
+
+extern "C" {
+
+const char *mpdg_last_error(void) { return g_last_error.c_str(); }
+
+void mpdg_solver_params_default(mpdg_solver_params *p)
+{
+  if (!p)
+    return;
+  std::memset(p, 0, sizeof(*p));
+  p->solver_type = MPDG_SOLVER_CG;
+  p->preconditioner_type = MPDG_PRECONDITIONER_DIAGONAL;
+  p->max_iterations = 10000;
+  p->rel_tolerance = 1e-12;
+  p->abs_tolerance = 1e-20;
+  p->gmres_restart = 30;
+  p->mixed_precision = 0;
+  p->inner_rel_tolerance = 1e-2;
+  p->check_interval = 0;
+}
+
+mpdg_status mpdg_heat_create(const mpdg_heat_desc *desc, mpdg_heat **out)
+{
+  if (!desc || !out)
+    return fail(MPDG_ERR_ARGUMENT, "null descriptor or output pointer");
+  *out = nullptr;
+  const mpdg_heat_desc &d = *desc;
+  if (d.degree < 1 || d.degree > kMaxDegree)
+    return fail(MPDG_ERR_UNSUPPORTED, "degree must be in 1.." + std::to_string(kMaxDegree));
+  int64_t N = 1, Nf = 1;
+  for (int k = 0; k < 3; ++k)
+    {
+      if (d.n_cells[k] < 1)
+        return fail(MPDG_ERR_ARGUMENT, "n_cells must be >= 1");
+      if (!(d.box_hi[k] > d.box_lo[k]))
+        return fail(MPDG_ERR_ARGUMENT, "box_hi must exceed box_lo");
+      const int64_t n1 = (int64_t)d.n_cells[k] * d.degree + 1;
+      N *= n1;
+      const int64_t drop = ((d.dirichlet_faces >> (2 * k)) & 1u) + ((d.dirichlet_faces >> (2 * k + 1)) & 1u);
+      Nf *= std::max<int64_t>(0, n1 - drop);
+    }
+  if (N >= ((int64_t)1 << 40))
+    return fail(MPDG_ERR_ARGUMENT, "mesh too large");
+  if (d.precision != MPDG_FP64 && d.precision != MPDG_FP32)
+    return fail(MPDG_ERR_ARGUMENT, "unknown precision");
+  if (d.material != MPDG_MATERIAL_GAS && d.material != MPDG_MATERIAL_GAS_LIQUID)
+    return fail(MPDG_ERR_UNSUPPORTED, "material type not supported");
+  if (d.velocity != MPDG_VELOCITY_NONE && d.velocity != MPDG_VELOCITY_VORTEX)
+    return fail(MPDG_ERR_UNSUPPORTED, "velocity field not supported");
+  if (d.conductivity_field != MPDG_CONDUCTIVITY_MATERIAL &&
+      d.conductivity_field != MPDG_CONDUCTIVITY_SEPARABLE_SINE)
+    return fail(MPDG_ERR_UNSUPPORTED, "conductivity field not supported");
+  if (d.material == MPDG_MATERIAL_GAS_LIQUID && d.conductivity_field != MPDG_CONDUCTIVITY_MATERIAL)
+    return fail(MPDG_ERR_UNSUPPORTED, "separable conductivity applies to the gas material only");
+
+  mpdg_heat *H = new (std::nothrow) mpdg_heat();
+  if (!H)
+    return fail(MPDG_ERR_ARGUMENT, "out of host memory");
+  H->desc = d;
+  H->stream = (hipStream_t)d.stream;
+  H->N = N;
+  H->N_free = Nf;
+  H->t1d = make_tables(d.degree);
+  for (int k = 0; k < 3; ++k)
+    H->h[k] = (d.box_hi[k] - d.box_lo[k]) / d.n_cells[k];
+  (void)hipGetDevice(&H->device);
+
+  const Tables1D &t = H->t1d;
+  const int n = t.n;
+  H->vt64.S = even_odd<double>(t.S, n, false);
+  H->vt64.St = even_odd<double>(t.S, n, true);
+  H->vt64.C = even_odd<double>(t.C, n, false);
+  H->vt64.Ct = even_odd<double>(t.C, n, true);
+  H->vt32.S = even_odd<float>(t.S, n, false);
+  H->vt32.St = even_odd<float>(t.S, n, true);
+  H->vt32.C = even_odd<float>(t.C, n, false);
+  H->vt32.Ct = even_odd<float>(t.C, n, true);
+  double S2[kMaxN][kMaxN]{}, D2[kMaxN][kMaxN]{}, SD[kMaxN][kMaxN]{};
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j)
+      {
+        S2[i][j] = t.S[i][j] * t.S[i][j];
+        D2[i][j] = t.D[i][j] * t.D[i][j];
+        SD[i][j] = t.S[i][j] * t.D[i][j];
+      }
+  H->dt64.S = even_odd<double>(t.S, n, false);
+  H->dt64.S2t = even_odd<double>(S2, n, true);
+  H->dt64.D2t = even_odd<double>(D2, n, true);
+  H->dt64.SDt = even_odd<double>(SD, n, true);
+  for (int q = 0; q < n; ++q)
+    {
+      H->vt64.W[q] = t.wq[q];
+      H->vt32.W[q] = (float)t.wq[q];
+      H->dt64.W[q] = t.wq[q];
+    }
+  if (symmetry_defect(t.S, n, 1) > 1e-13 || symmetry_defect(t.C, n, -1) > 1e-12 ||
+      symmetry_defect(t.D, n, -1) > 1e-12)
+    {
+      delete H;
+      return fail(MPDG_ERR_ARGUMENT, "1D tables lost their even-odd symmetry");
+    }
+
+  // analytic coefficient tables at the quadrature coordinates
+  std::vector<double> tab;
+  const double two_pi = 2.0 * M_PI;
+  for (int pass = 0; pass < 2; ++pass)
+    for (int k = 0; k < 3; ++k)
+      for (int c = 0; c < d.n_cells[k]; ++c)
+        for (int qq = 0; qq < n; ++qq)
+          {
+            const double x = d.box_lo[k] + (c + t.xq[qq]) * H->h[k];
+            if (pass == 0)
+              tab.push_back(std::sin(two_pi * x));
+            else
+              {
+                const double s = std::sin(M_PI * x);
+                tab.push_back(s * s);
+              }
+          }
+  std::vector<float> tabf(tab.begin(), tab.end());
+  hipError_t e = H->sin_tab64.alloc(tab.size());
+  if (e == hipSuccess)
+    e = H->sin_tab32.alloc(tabf.size());
+  if (e == hipSuccess)
+    e = hipMemcpy(H->sin_tab64.ptr, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(H->sin_tab32.ptr, tabf.data(), tabf.size() * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipEventCreate(&H->ev0);
+  if (e == hipSuccess)
+    e = hipEventCreate(&H->ev1);
+  if (e != hipSuccess)
+    {
+      delete H;
+      return fail(MPDG_ERR_HIP, std::string("set-up: ") + hipGetErrorString(e));
+    }
+  fill_params<double>(H, H->prm64, H->sin_tab64.ptr);
+  fill_params<float>(H, H->prm32, H->sin_tab32.ptr);
+  *out = H;
+  return MPDG_OK;
+}
+
+mpdg_status mpdg_heat_destroy(mpdg_heat *h)
+{
+  if (!h)
+    return MPDG_OK;
+  if (h->stream)
+    (void)hipStreamSynchronize(h->stream);
+  else
+    (void)hipDeviceSynchronize();
+  delete h;
+  return MPDG_OK;
+}
+
+int64_t mpdg_heat_n_dofs(const mpdg_heat *h) { return h ? h->N : -1; }
+int64_t mpdg_heat_n_free_dofs(const mpdg_heat *h) { return h ? h->N_free : -1; }
+
+mpdg_status mpdg_heat_set_dt(mpdg_heat *h, double dt)
+{
+  if (!h)
+    return fail(MPDG_ERR_ARGUMENT, "null handle");
+  if (dt == 0.0 || !std::isfinite(dt))
+    return fail(MPDG_ERR_ZERO_TIME_INCREMENT, "time increment must be finite and nonzero");
+  h->dt = dt;
+  h->dt_set = true;
+  fill_params<double>(h, h->prm64, h->sin_tab64.ptr);
+  fill_params<float>(h, h->prm32, h->sin_tab32.ptr);
+  return MPDG_OK;
+}
+
+mpdg_status mpdg_heat_set_heaviside(mpdg_heat *h, const double *heaviside)
+{
+  if (!h || !heaviside)
+    return fail(MPDG_ERR_ARGUMENT, "null handle or heaviside");
+  MPDG_HIP(h->heav64.alloc(h->N));
+  MPDG_HIP(h->heav32.alloc(h->N));
+  MPDG_HIP(hipMemcpyAsync(h->heav64.ptr, heaviside, h->N * sizeof(double),
+                          hipMemcpyDeviceToDevice, h->stream));
+  MPDG_HIP(launch_to_f32(h->heav32.ptr, h->heav64.ptr, h->N, h->stream));
+  h->has_heaviside = true;
+  return MPDG_OK;
+}
+
+mpdg_status mpdg_heat_vmult(mpdg_heat *h, double *dst, const double *src)
+{
+  if (mpdg_status s = check_ready(h))
+    return s;
+  if (!dst || !src)
+    return fail(MPDG_ERR_ARGUMENT, "null vector");
+  if (dst == src)
+    return fail(MPDG_ERR_ARGUMENT, "vmult cannot run in place");
+  return vmult64(h, dst, src, nullptr, nullptr);
+}
+
+mpdg_status mpdg_heat_vmult_f32(mpdg_heat *h, float *dst, const float *src)
+{
+  if (mpdg_status s = check_ready(h))
+    return s;
+  if (!dst || !src)
+    return fail(MPDG_ERR_ARGUMENT, "null vector");
+  if ((const void *)dst == (const void *)src)
+    return fail(MPDG_ERR_ARGUMENT, "vmult cannot run in place");
+  return vmult32(h, dst, src, nullptr, nullptr);
+}
+
+mpdg_status mpdg_heat_inverse_diagonal(mpdg_heat *h, double *inv_diag)
+{
+  if (mpdg_status s = check_ready(h))
+    return s;
+  if (!inv_diag)
+    return fail(MPDG_ERR_ARGUMENT, "null vector");
+  return diagonal_inverse(h, inv_diag);
+}
+
+mpdg_status mpdg_heat_jacobi_vmult(mpdg_heat *h, double *dst, const double *src,
+                                   const double *inv_diag)
+{
+  if (!h || !dst || !src || !inv_diag)
+    return fail(MPDG_ERR_ARGUMENT, "null argument");
+  MPDG_HIP(launch_jacobi(dst, src, inv_diag, h->N, h->stream));
+  return MPDG_OK;
+}
+
+mpdg_status mpdg_heat_solve(mpdg_heat *h, double *x, const double *b,
+                            const mpdg_solver_params *params, mpdg_solver_report *report)
+{
+  if (mpdg_status s = check_ready(h))
+    return s;
+  if (!x || !b)
+    return fail(MPDG_ERR_ARGUMENT, "null vector");
+  mpdg_solver_params prm;
+  if (params)
+    prm = *params;
+  else
+    mpdg_solver_params_default(&prm);
+  if (prm.solver_type != MPDG_SOLVER_CG)
+    return fail(MPDG_ERR_UNSUPPORTED, "only CG is built in this engine version");
+  if (prm.max_iterations < 1 || !(prm.rel_tolerance >= 0) || !(prm.abs_tolerance >= 0))
+    return fail(MPDG_ERR_ARGUMENT, "invalid solver parameters");
+  const bool mixed = prm.mixed_precision != 0;
+  if (mpdg_status s = ensure_workspace(h, mixed))
+    return s;
+  const int check = prm.check_interval > 0 ? prm.check_interval : auto_check(h);
+  mpdg_solver_report rep{};
+  const auto t0 = std::chrono::steady_clock::now();
+
+  // JacobiPreconditioner::update -> compute_inverse_diagonal_from_matrixfree
+  if (prm.preconditioner_type == MPDG_PRECONDITIONER_DIAGONAL)
+    {
+      if (mpdg_status s = diagonal_inverse(h, h->inv_diag.ptr))
+        return s;
+    }
+  else
+    MPDG_HIP(launch_fill(h->inv_diag.ptr, 1.0, h->N, h->stream));
+
+  mpdg_status status = MPDG_OK;
+  if (!mixed)
+    {
+      if (mpdg_status s = pcg<double>(h, x, b, h->r.ptr, h->p.ptr, h->q.ptr, h->inv_diag.ptr,
+                                      prm.rel_tolerance, prm.abs_tolerance, prm.max_iterations,
+                                      check, false))
+        return s;
+      rep.iterations = h->host_scal->iteration;
+      rep.outer_iterations = 0;
+      rep.initial_residual = h->host_scal->res0;
+      rep.final_residual = h->host_scal->res;
+      if (h->host_scal->status != 0)
+        status = fail(MPDG_ERR_NOT_CONVERGED, "iterative solver did not converge");
+    }
+  else
+    {
+      // Defect correction: r = b - A64 x in FP64, inner FP32 PCG for A d = r,
+      // x += d in FP64, until ||r|| <= max(abs, rel ||r0||).
+      MPDG_HIP(launch_to_f32(h->inv32.ptr, h->inv_diag.ptr, h->N, h->stream));
+      double tol = 0.0, res0 = 0.0;
+      int inner_total = 0, outer = 0;
+      for (;;)
+        {
+          if (mpdg_status s = vmult64(h, h->q.ptr, x, nullptr, nullptr))
+            return s;
+          MPDG_HIP(launch_residual(h->r.ptr, b, h->q.ptr, h->N, h->partials.ptr, h->scal.ptr,
+                                   h->stream));
+          if (mpdg_status s = read_scalars(h))
+            return s;
+          const double res = h->host_scal->res;
+          if (outer == 0)
+            {
+              res0 = res;
+              tol = std::max(prm.abs_tolerance, prm.rel_tolerance * res);
+            }
+          rep.final_residual = res;
+          if (res <= tol)
+            break;
+          if (inner_total >= prm.max_iterations || !std::isfinite(res))
+            {
+              status = fail(MPDG_ERR_NOT_CONVERGED, "defect correction did not converge");
+              break;
+            }
+          ++outer;
+          // inner solve: the relative reduction is capped so the last step
+          // does not oversolve past the outer tolerance
+          const double inner_rel = std::max(prm.inner_rel_tolerance, 0.5 * tol / res);
+          MPDG_HIP(launch_fill_f32(h->x32.ptr, 0.0f, h->N, h->stream));
+          if (mpdg_status s = pcg<float>(h, h->x32.ptr, h->r.ptr, h->r32.ptr, h->p32.ptr,
+                                         h->q32.ptr, h->inv32.ptr, inner_rel, 0.0,
+                                         prm.max_iterations - inner_total, check, true))
+            return s;
+          inner_total += h->host_scal->iteration;
+          MPDG_HIP(launch_add_f32(x, h->x32.ptr, h->N, h->stream));
+        }
+      rep.iterations = inner_total;
+      rep.outer_iterations = outer;
+      rep.initial_residual = res0;
+    }
+  MPDG_HIP(hipStreamSynchronize(h->stream));
+  rep.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (report)
+    *report = rep;
+  return status;
+}
+
+mpdg_status mpdg_heat_time_vmult(mpdg_heat *h, double *dst, const double *src, int repeats,
+                                 int precision, double *ms_per_vmult)
+{
+  if (mpdg_status s = check_ready(h))
+    return s;
+  if (!dst || !src || repeats < 1 || !ms_per_vmult)
+    return fail(MPDG_ERR_ARGUMENT, "invalid timing arguments");
+  MPDG_HIP(hipEventRecord(h->ev0, h->stream));
+  for (int i = 0; i < repeats; ++i)
+    {
+      mpdg_status s = precision == MPDG_FP32 ?
+                        vmult32(h, (float *)dst, (const float *)src, nullptr, nullptr) :
+                        vmult64(h, dst, src, nullptr, nullptr);
+      if (s)
+        return s;
+    }
+  MPDG_HIP(hipEventRecord(h->ev1, h->stream));
+  MPDG_HIP(hipEventSynchronize(h->ev1));
+  float ms = 0.0f;
+  MPDG_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  *ms_per_vmult = ms / repeats;
+  return MPDG_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,908 @@
+// CDNA4 cell kernels of the matrix-free heat operator.
+//
+// One workgroup = one row segment of CPB consecutive cells along x at a fixed
+// (cy, cz). Thread (l, s) owns cell l and "line" s = s0 + N*s1 of that cell.
+// All 1D contractions are done by a single thread over a whole line held in
+// registers (even-odd split), and the three directions are reached by moving
+// the cell tensor through LDS: x-lines (gather), y-lines, z-lines, and back.
+// The global gather/scatter uses x-lines with the cell index fastest across
+// lanes, so a wavefront touches contiguous runs of the lattice rows.
+//
+// Scatter determinism: cells of a segment add their shared x-faces inside
+// the workgroup (own value + left neighbour's value), segments are coloured by
+// (sx%2, cy%2, cz%2) and the eight colour passes run in order on the stream.
+// A lattice node is stored by the first pass that touches it (even parity in
+// every direction where it is shared) and read-modify-written by the later
+// ones, so dst never needs clearing and every node's sum has a fixed order.
+//
+// Reference behaviour reproduced (source/heat/heat_diffuse_operator.cpp):
+//   vmult / tangent_cell_loop :179-241 (gather with homogeneous constraints
+//   resolved to 0, distribute_local_to_global dropping constrained rows),
+//   tangent_local_cell_operation :955-1086 (evaluate values|gradients,
+//   val = dt^-1 rho_cp u, val_grad = k grad u, + rho_cp grad u . v,
+//   integrate values|gradients), material lookup :1133-1210.
+#include "kernels.hpp"
+
+namespace mpdg
+{
+  namespace
+  {
+    template <int N>
+    constexpr int lds_stride()
+    {
+      // odd per-cell stride keeps lanes of consecutive cells on distinct banks
+      return (N * N * N) % 2 == 0 ? N * N * N + 1 : N * N * N;
+    }
+
+    template <int P, bool FP32>
+    constexpr int cpb()
+    {
+      return cells_per_block_for(P, FP32);
+    }
+
+    template <int P, bool FP32>
+    constexpr int block_threads()
+    {
+      return ((cpb<P, FP32>() * (P + 1) * (P + 1) + 63) / 64) * 64;
+    }
+
+    // y = M x with M in even-odd form, SIGN the symmetry sign of M.
+    template <int N, int SIGN, typename Number>
+    __device__ __forceinline__ void eo_apply(const EvenOdd<Number> &M, const Number (&x)[N],
+                                             Number (&y)[N])
+    {
+      constexpr int H = N / 2;
+      Number E[H], O[H];
+#pragma unroll
+      for (int j = 0; j < H; ++j)
+        {
+          E[j] = x[j] + x[N - 1 - j];
+          O[j] = x[j] - x[N - 1 - j];
+        }
+#pragma unroll
+      for (int i = 0; i < H; ++i)
+        {
+          Number a = M.e[i][0] * E[0];
+          Number b = M.o[i][0] * O[0];
+#pragma unroll
+          for (int j = 1; j < H; ++j)
+            {
+              a += M.e[i][j] * E[j];
+              b += M.o[i][j] * O[j];
+            }
+          if constexpr (N % 2 == 1)
+            a += M.mc[i] * x[H];
+          y[i] = a + b;
+          y[N - 1 - i] = SIGN > 0 ? a - b : b - a;
+        }
+      if constexpr (N % 2 == 1)
+        {
+          Number m = M.mm * x[H];
+#pragma unroll
+          for (int j = 0; j < H; ++j)
+            m += M.mr[j] * (SIGN > 0 ? E[j] : O[j]);
+          y[H] = m;
+        }
+    }
+
+    template <typename Number>
+    __device__ __forceinline__ bool on_dirichlet(const CellParams<Number> &prm, int i, int d)
+    {
+      const int n1 = d == 0 ? prm.n1x : (d == 1 ? prm.n1y : prm.n1z);
+      return (((prm.faces >> (2 * d)) & 1u) && i == 0) ||
+             (((prm.faces >> (2 * d + 1)) & 1u) && i == n1 - 1);
+    }
+
+    // Material::compute_parameters for gas_liquid with the blend of
+    // LevelSet::Tools::interpolate (material.templates.hpp:515-524,
+    // level_set_tools.hpp:40-43); rho*cp as in heat_diffuse_operator.cpp:1167-1210.
+    template <typename Number>
+    __device__ __forceinline__ void two_phase(const CellParams<Number> &prm, Number H, Number &rc,
+                                              Number &k)
+    {
+      const Number w = prm.sharp ? (H > Number(0.5) ? Number(1) : Number(0)) : H;
+      const Number one_w = Number(1) - w;
+      k = one_w * prm.g_k + w * prm.l_k;
+      if (prm.volume_specific)
+        rc = one_w * prm.g_rhocp + w * prm.l_rhocp;
+      else
+        rc = (one_w * prm.g_rho + w * prm.l_rho) * (one_w * prm.g_cp + w * prm.l_cp);
+    }
+
+    // Deterministic block sum (fixed butterfly + fixed-order wave sum).
+    template <int THREADS>
+    __device__ __forceinline__ double block_sum(double v, double *sh)
+    {
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1)
+        v += __shfl_xor(v, off, 64);
+      const int wave = threadIdx.x >> 6;
+      if ((threadIdx.x & 63) == 0)
+        sh[wave] = v;
+      __syncthreads();
+      double s = 0.0;
+      if (threadIdx.x == 0)
+        {
+          for (int w = 0; w < THREADS / 64; ++w)
+            s += sh[w];
+        }
+      return s; // valid in thread 0
+    }
+
+    // Sum of partials[0..count) in a fixed order by one block.
+    template <int THREADS>
+    __device__ __forceinline__ double ordered_sum(const double *partials, int count, double *sh)
+    {
+      double v = 0.0;
+      for (int i = threadIdx.x; i < count; i += THREADS)
+        v += __hip_atomic_load(partials + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __syncthreads();
+      return block_sum<THREADS>(v, sh);
+    }
+
+    struct Segment
+    {
+      int sx, cy, cz, cx0, ncell;
+    };
+
+    template <typename Number, int CPB>
+    __device__ __forceinline__ Segment segment_of(const CellParams<Number> &prm, int colour)
+    {
+      const int nsx = (prm.nx + CPB - 1) / CPB;
+      const int ox = colour & 1, oy = (colour >> 1) & 1, oz = (colour >> 2) & 1;
+      const int msx = (nsx - ox + 1) / 2;
+      const int my = (prm.ny - oy + 1) / 2;
+      int b = blockIdx.x;
+      Segment s;
+      s.sx = ox + 2 * (b % msx);
+      b /= msx;
+      s.cy = oy + 2 * (b % my);
+      s.cz = oz + 2 * (b / my);
+      s.cx0 = s.sx * CPB;
+      s.ncell = min(CPB, prm.nx - s.cx0);
+      return s;
+    }
+
+    // Coloured first-touch scatter of one cell x-line (b = s0, c = s1) after the
+    // in-segment x-face exchange (see the file header).
+    template <int P, typename Number, int CPB>
+    __device__ __forceinline__ void scatter_xline(const CellParams<Number> &prm, const Segment &g,
+                                                  int l, int s0, int s1, const Number (&Y)[P + 1],
+                                                  Number *__restrict__ dst)
+    {
+      constexpr int N = P + 1;
+      const int Yi = g.cy * P + s0, Zi = g.cz * P + s1;
+      const bool ysh = (s0 == 0 && g.cy > 0) || (s0 == P && g.cy < prm.ny - 1);
+      const bool zsh = (s1 == 0 && g.cz > 0) || (s1 == P && g.cz < prm.nz - 1);
+      const bool yz_first = !(ysh && (g.cy & 1)) && !(zsh && (g.cz & 1));
+      const bool rowc = on_dirichlet(prm, Yi, 1) || on_dirichlet(prm, Zi, 2);
+      const int X0 = (g.cx0 + l) * P;
+      const int64_t base = X0 + (int64_t)prm.n1x * (Yi + (int64_t)prm.n1y * Zi);
+      const bool last = l == g.ncell - 1;
+#pragma unroll
+      for (int a = 0; a < N; ++a)
+        {
+          if (a == N - 1 && !last)
+            continue;
+          const bool xsh = (a == 0 && l == 0 && g.cx0 > 0) ||
+                           (a == P && last && g.cx0 + g.ncell < prm.nx);
+          const bool first = yz_first && !(xsh && (g.sx & 1));
+          const bool con = rowc || on_dirichlet(prm, X0 + a, 0);
+          Number *ptr = dst + base + a;
+          if (con)
+            {
+              if (first)
+                *ptr = Number(0);
+            }
+          else if (first)
+            *ptr = Y[a];
+          else
+            *ptr = *ptr + Y[a];
+        }
+    }
+
+    template <typename Number, int COEF, int N>
+    __device__ __forceinline__ Number conductivity(const CellParams<Number> &prm,
+                                                   const Number *KBl, int cx, int cy, int cz,
+                                                   int qx, int qy, int qz)
+    {
+      if constexpr (COEF == kCoefConstant)
+        return prm.k0;
+      else if constexpr (COEF == kCoefSine)
+        return prm.k0 + prm.kamp * prm.sin2pi[0][cx * N + qx] * prm.sin2pi[1][cy * N + qy] *
+                          prm.sin2pi[2][cz * N + qz];
+      else
+        return KBl[(qz * N + qy) * N + qx];
+    }
+
+    template <int P, typename Number, int COEF, bool VEL, int CPB, int THREADS>
+    __global__ void __launch_bounds__(THREADS)
+      vmult_kernel(const Number *__restrict__ src, Number *__restrict__ dst,
+                   const Number *__restrict__ heav, const VmultTables<Number> tab,
+                   const CellParams<Number> prm, const int colour, double *__restrict__ partials,
+                   const int partial_offset, const int partial_total, const int finalize,
+                   SolverScalars *__restrict__ scal)
+    {
+      constexpr int N = P + 1;
+      constexpr int NL = N * N;
+      constexpr int STR = lds_stride<N>();
+      constexpr bool H2 = COEF == kCoefHeaviside;
+      if (scal != nullptr && scal->done)
+        return;
+
+      __shared__ Number A[CPB * STR];
+      __shared__ Number B[CPB * STR];
+      __shared__ Number KB[H2 ? CPB * STR : 1];
+      __shared__ Number RB[(H2 && VEL) ? CPB * STR : 1];
+      __shared__ double red[THREADS / 64];
+
+      const Segment g = segment_of<Number, CPB>(prm, colour);
+      const int t = threadIdx.x;
+      const bool active = t < CPB * NL;
+      const int l = t % CPB;
+      const int s = t / CPB;
+      const int s0 = s % N, s1 = s / N;
+      const bool cell_ok = active && l < g.ncell;
+      const int cx = min(g.cx0 + l, prm.nx - 1);
+      Number *Al = A + l * STR;
+      Number *Bl = B + l * STR;
+      Number *KBl = KB + (H2 ? l * STR : 0);
+      Number *RBl = RB + ((H2 && VEL) ? l * STR : 0);
+
+      // ---- phase 0: gather x-line (y = s0, z = s1), interpolate in x -------
+      Number u[N];
+      if (active)
+        {
+          const int Yi = g.cy * P + s0, Zi = g.cz * P + s1;
+          const int X0 = (g.cx0 + l) * P;
+          const bool rowc = on_dirichlet(prm, Yi, 1) || on_dirichlet(prm, Zi, 2);
+          const int64_t base = X0 + (int64_t)prm.n1x * (Yi + (int64_t)prm.n1y * Zi);
+#pragma unroll
+          for (int a = 0; a < N; ++a)
+            u[a] = (cell_ok && !rowc && !on_dirichlet(prm, X0 + a, 0)) ? src[base + a] : Number(0);
+          Number t1[N];
+          eo_apply<N, 1>(tab.S, u, t1);
+#pragma unroll
+          for (int q = 0; q < N; ++q)
+            Al[(s1 * N + s0) * N + q] = t1[q];
+          if constexpr (H2)
+            {
+              Number hl[N], h1[N];
+#pragma unroll
+              for (int a = 0; a < N; ++a)
+                hl[a] = cell_ok ? heav[base + a] : Number(0); // read_dof_values_plain
+              eo_apply<N, 1>(tab.S, hl, h1);
+#pragma unroll
+              for (int q = 0; q < N; ++q)
+                Bl[(s1 * N + s0) * N + q] = h1[q];
+            }
+        }
+      else
+        {
+#pragma unroll
+          for (int a = 0; a < N; ++a)
+            u[a] = Number(0);
+        }
+      __syncthreads();
+
+      // ---- phase 1: y-lines (qx = s0, z = s1), interpolate in y -------------
+      if (active)
+        {
+          Number v[N], w[N];
+#pragma unroll
+          for (int b = 0; b < N; ++b)
+            v[b] = Al[(s1 * N + b) * N + s0];
+          eo_apply<N, 1>(tab.S, v, w);
+#pragma unroll
+          for (int b = 0; b < N; ++b)
+            Al[(s1 * N + b) * N + s0] = w[b];
+          if constexpr (H2)
+            {
+#pragma unroll
+              for (int b = 0; b < N; ++b)
+                v[b] = Bl[(s1 * N + b) * N + s0];
+              eo_apply<N, 1>(tab.S, v, w);
+#pragma unroll
+              for (int b = 0; b < N; ++b)
+                Bl[(s1 * N + b) * N + s0] = w[b];
+            }
+        }
+      __syncthreads();
+
+      // ---- phase 2: z-lines (qx = s0, qy = s1): values, z-gradient --------
+      Number U[N], Gz[N], RC[N];
+      if (active)
+        {
+          Number v[N];
+#pragma unroll
+          for (int c = 0; c < N; ++c)
+            v[c] = Al[(c * N + s1) * N + s0];
+          eo_apply<N, 1>(tab.S, v, U);
+          eo_apply<N, -1>(tab.C, U, Gz);
+#pragma unroll
+          for (int q = 0; q < N; ++q)
+            Al[(q * N + s1) * N + s0] = U[q];
+          if constexpr (H2)
+            {
+              Number Hq[N];
+#pragma unroll
+              for (int c = 0; c < N; ++c)
+                v[c] = Bl[(c * N + s1) * N + s0];
+              eo_apply<N, 1>(tab.S, v, Hq);
+#pragma unroll
+              for (int q = 0; q < N; ++q)
+                {
+                  Number rc, k;
+                  two_phase(prm, Hq[q], rc, k);
+                  KBl[(q * N + s1) * N + s0] = k;
+                  if constexpr (VEL)
+                    RBl[(q * N + s1) * N + s0] = rc;
+                  RC[q] = rc;
+                }
+            }
+        }
+      __syncthreads();
+
+      // ---- phase 3a: x-lines (qy = s0, qz = s1): x-gradient flux ----------
+      if (active)
+        {
+          Number v[N], gx[N], f[N], o[N];
+#pragma unroll
+          for (int r = 0; r < N; ++r)
+            v[r] = Al[(s1 * N + s0) * N + r];
+          eo_apply<N, -1>(tab.C, v, gx);
+          const Number wyz = tab.W[s0] * tab.W[s1];
+          Number cv[N];
+#pragma unroll
+          for (int q = 0; q < N; ++q)
+            {
+              const Number w3 = tab.W[q] * wyz;
+              const Number k =
+                conductivity<Number, COEF, N>(prm, KBl, cx, g.cy, g.cz, q, s0, s1);
+              f[q] = k * w3 * prm.kf[0] * gx[q];
+              if constexpr (VEL)
+                {
+                  const Number rc = H2 ? RBl[(s1 * N + s0) * N + q] : prm.rho_cp0;
+                  const Number vx = -prm.sinpi2[0][cx * N + q] * prm.sin2pi[1][g.cy * N + s0];
+                  cv[q] = rc * w3 * prm.vf[0] * vx * gx[q];
+                }
+            }
+          eo_apply<N, -1>(tab.Ct, f, o);
+#pragma unroll
+          for (int q = 0; q < N; ++q)
+            Bl[(s1 * N + s0) * N + q] = VEL ? o[q] + cv[q] : o[q];
+        }
+      __syncthreads();
+
+      // ---- phase 3b: y-lines (qx = s0, qz = s1): y-gradient flux ----------
+      if (active)
+        {
+          Number v[N], gy[N], f[N], o[N], cv[N];
+#pragma unroll
+          for (int r = 0; r < N; ++r)
+            v[r] = Al[(s1 * N + r) * N + s0];
+          eo_apply<N, -1>(tab.C, v, gy);
+          const Number wxz = tab.W[s0] * tab.W[s1];
+#pragma unroll
+          for (int q = 0; q < N; ++q)
+            {
+              const Number w3 = tab.W[q] * wxz;
+              const Number k =
+                conductivity<Number, COEF, N>(prm, KBl, cx, g.cy, g.cz, s0, q, s1);
+              f[q] = k * w3 * prm.kf[1] * gy[q];
+              if constexpr (VEL)
+                {
+                  const Number rc = H2 ? RBl[(s1 * N + q) * N + s0] : prm.rho_cp0;
+                  const Number vy = prm.sin2pi[0][cx * N + s0] * prm.sinpi2[1][g.cy * N + q];
+                  cv[q] = rc * w3 * prm.vf[1] * vy * gy[q];
+                }
+            }
+          eo_apply<N, -1>(tab.Ct, f, o);
+#pragma unroll
+          for (int q = 0; q < N; ++q)
+            {
+              const int idx = (s1 * N + q) * N + s0;
+              Bl[idx] = Bl[idx] + (VEL ? o[q] + cv[q] : o[q]);
+            }
+        }
+      __syncthreads();
+
+      // ---- phase 4: z-lines (qx = s0, qy = s1): q-point physics, integrate z
+      if (active)
+        {
+          Number f[N], o[N], V[N], y1[N];
+          const Number wxy = tab.W[s0] * tab.W[s1];
+#pragma unroll
+          for (int q = 0; q < N; ++q)
+            {
+              const Number w3 = tab.W[q] * wxy;
+              const Number k =
+                conductivity<Number, COEF, N>(prm, KBl, cx, g.cy, g.cz, s0, s1, q);
+              f[q] = k * w3 * prm.kf[2] * Gz[q];
+            }
+          eo_apply<N, -1>(tab.Ct, f, o);
+#pragma unroll
+          for (int q = 0; q < N; ++q)
+            {
+              const Number w3 = tab.W[q] * wxy;
+              const Number mass = H2 ? RC[q] * prm.dtinv_vol : prm.mass;
+              Number val = mass * w3 * U[q] + o[q] + Bl[(q * N + s1) * N + s0];
+              if constexpr (VEL)
+                {
+                  const Number rc = H2 ? RC[q] : prm.rho_cp0;
+                  val += rc * w3 * prm.vf[2] * prm.w0 * Gz[q];
+                }
+              V[q] = val;
+            }
+          eo_apply<N, 1>(tab.St, V, y1);
+#pragma unroll
+          for (int c = 0; c < N; ++c)
+            Al[(c * N + s1) * N + s0] = y1[c];
+        }
+      __syncthreads();
+
+      // ---- phase 5: y-lines (qx = s0, c = s1): integrate y ----------------
+      if (active)
+        {
+          Number v[N], w[N];
+#pragma unroll
+          for (int q = 0; q < N; ++q)
+            v[q] = Al[(s1 * N + q) * N + s0];
+          eo_apply<N, 1>(tab.St, v, w);
+#pragma unroll
+          for (int b = 0; b < N; ++b)
+            Al[(s1 * N + b) * N + s0] = w[b];
+        }
+      __syncthreads();
+
+      // ---- phase 6: x-lines (b = s0, c = s1): integrate x, scatter ---------
+      Number Y[N];
+      double acc = 0.0;
+      if (active)
+        {
+          Number v[N];
+#pragma unroll
+          for (int q = 0; q < N; ++q)
+            v[q] = Al[(s1 * N + s0) * N + q];
+          eo_apply<N, 1>(tab.St, v, Y);
+#pragma unroll
+          for (int a = 0; a < N; ++a)
+            acc += (double)u[a] * (double)Y[a];
+          B[s * CPB + l] = Y[N - 1];
+        }
+      __syncthreads();
+      if (cell_ok)
+        {
+          if (l > 0)
+            Y[0] += B[s * CPB + l - 1];
+          scatter_xline<P, Number, CPB>(prm, g, l, s0, s1, Y, dst);
+        }
+
+      if (partials != nullptr)
+        {
+          // p . Ap: sum over cells of u_cell . y_cell (constrained entries are 0)
+          const double bs = block_sum<THREADS>(acc, red);
+          __shared__ bool is_last;
+          if (threadIdx.x == 0)
+            {
+              partials[partial_offset + blockIdx.x] = bs;
+              bool lst = false;
+              if (finalize)
+                {
+                  __threadfence();
+                  const unsigned prev = atomicAdd(&scal->counter, 1u);
+                  lst = prev == gridDim.x - 1;
+                }
+              is_last = lst;
+            }
+          __syncthreads();
+          if (is_last)
+            {
+              __threadfence();
+              const double pap = ordered_sum<THREADS>(partials, partial_total, red);
+              if (threadIdx.x == 0)
+                {
+                  scal->pap = pap;
+                  scal->alpha = scal->rz / pap;
+                  scal->counter = 0u;
+                }
+            }
+        }
+    }
+
+    // Closed-form diagonal of the cell matrix, summed like vmult's scatter.
+    // diag_i = sum_q JxW [ c phi_i^2 + k sum_d (d_d phi_i)^2 + rho_cp phi_i v . grad phi_i ]
+    // evaluated with squared 1D matrices: S o S, D o D, S o D.
+    template <int P, typename Number, int COEF, bool VEL, int CPB, int THREADS>
+    __global__ void __launch_bounds__(THREADS)
+      diag_kernel(Number *__restrict__ dst, const Number *__restrict__ heav,
+                  const DiagTables<Number> tab, const CellParams<Number> prm, const int colour)
+    {
+      constexpr int N = P + 1;
+      constexpr int NL = N * N;
+      constexpr int STR = lds_stride<N>();
+      constexpr bool H2 = COEF == kCoefHeaviside;
+      // KB/RB (material at q) alias B/C: the x-line phase reads them for the
+      // same positions it then overwrites, within one thread.
+      __shared__ Number A[CPB * STR];
+      __shared__ Number B[CPB * STR];
+      __shared__ Number C[CPB * STR];
+      __shared__ Number D3[VEL ? CPB * STR : 1];
+      __shared__ Number E3[VEL ? CPB * STR : 1];
+      Number *KB = B;
+      Number *RB = C;
+
+      const Segment g = segment_of<Number, CPB>(prm, colour);
+      const int t = threadIdx.x;
+      const bool active = t < CPB * NL;
+      const int l = t % CPB;
+      const int s = t / CPB;
+      const int s0 = s % N, s1 = s / N;
+      const bool cell_ok = active && l < g.ncell;
+      const int cx = min(g.cx0 + l, prm.nx - 1);
+      Number *Al = A + l * STR;
+      Number *Bl = B + l * STR;
+      Number *Cl = C + l * STR;
+      Number *Dl = D3 + (VEL ? l * STR : 0);
+      Number *El = E3 + (VEL ? l * STR : 0);
+      Number *KBl = KB + (H2 ? l * STR : 0);
+      Number *RBl = RB + (H2 ? l * STR : 0);
+
+      if constexpr (H2)
+        {
+          // heaviside at q: plain gather + three interpolation sweeps
+          if (active)
+            {
+              const int Yi = g.cy * P + s0, Zi = g.cz * P + s1;
+              const int64_t base = (g.cx0 + l) * P + (int64_t)prm.n1x * (Yi + (int64_t)prm.n1y * Zi);
+              Number hl[N], h1[N];
+#pragma unroll
+              for (int a = 0; a < N; ++a)
+                hl[a] = cell_ok ? heav[base + a] : Number(0);
+              eo_apply<N, 1>(tab.S, hl, h1);
+#pragma unroll
+              for (int q = 0; q < N; ++q)
+                Al[(s1 * N + s0) * N + q] = h1[q];
+            }
+          __syncthreads();
+          if (active)
+            {
+              Number v[N], w[N];
+#pragma unroll
+              for (int b = 0; b < N; ++b)
+                v[b] = Al[(s1 * N + b) * N + s0];
+              eo_apply<N, 1>(tab.S, v, w);
+#pragma unroll
+              for (int b = 0; b < N; ++b)
+                Al[(s1 * N + b) * N + s0] = w[b];
+            }
+          __syncthreads();
+          if (active)
+            {
+              Number v[N], Hq[N];
+#pragma unroll
+              for (int c = 0; c < N; ++c)
+                v[c] = Al[(c * N + s1) * N + s0];
+              eo_apply<N, 1>(tab.S, v, Hq);
+#pragma unroll
+              for (int q = 0; q < N; ++q)
+                {
+                  Number rc, k;
+                  two_phase(prm, Hq[q], rc, k);
+                  KBl[(q * N + s1) * N + s0] = k;
+                  RBl[(q * N + s1) * N + s0] = rc;
+                }
+            }
+          __syncthreads();
+        }
+
+      // x-lines (qy = s0, qz = s1): coefficient fields, contract x
+      if (active)
+        {
+          Number Fm[N], Fkx[N], Fky[N], Fkz[N], Fvx[N], Fvy[N], Fvz[N];
+          const Number wyz = tab.W[s0] * tab.W[s1];
+#pragma unroll
+          for (int q = 0; q < N; ++q)
+            {
+              const Number w3 = tab.W[q] * wyz;
+              const Number k = conductivity<Number, COEF, N>(prm, KBl, cx, g.cy, g.cz, q, s0, s1);
+              const Number rc = H2 ? RBl[(s1 * N + s0) * N + q] : prm.rho_cp0;
+              Fm[q] = (H2 ? rc * prm.dtinv_vol : prm.mass) * w3;
+              Fkx[q] = k * w3 * prm.kf[0];
+              Fky[q] = k * w3 * prm.kf[1];
+              Fkz[q] = k * w3 * prm.kf[2];
+              if constexpr (VEL)
+                {
+                  const Number vx = -prm.sinpi2[0][cx * N + q] * prm.sin2pi[1][g.cy * N + s0];
+                  const Number vy = prm.sin2pi[0][cx * N + q] * prm.sinpi2[1][g.cy * N + s0];
+                  Fvx[q] = rc * w3 * prm.vf[0] * vx;
+                  Fvy[q] = rc * w3 * prm.vf[1] * vy;
+                  Fvz[q] = rc * w3 * prm.vf[2] * prm.w0;
+                }
+            }
+          Number g1[N], t1[N], g2[N], g4[N];
+          eo_apply<N, 1>(tab.S2t, Fm, g1);
+          eo_apply<N, 1>(tab.D2t, Fkx, t1);
+#pragma unroll
+          for (int a = 0; a < N; ++a)
+            g1[a] += t1[a];
+          if constexpr (VEL)
+            {
+              eo_apply<N, -1>(tab.SDt, Fvx, t1);
+#pragma unroll
+              for (int a = 0; a < N; ++a)
+                g1[a] += t1[a];
+            }
+          eo_apply<N, 1>(tab.S2t, Fky, g2);
+          eo_apply<N, 1>(tab.S2t, Fkz, g4);
+#pragma unroll
+          for (int a = 0; a < N; ++a)
+            {
+              const int idx = (s1 * N + s0) * N + a;
+              Al[idx] = g1[a];
+              Bl[idx] = g2[a];
+              Cl[idx] = g4[a];
+            }
+          if constexpr (VEL)
+            {
+              Number g3[N], g5[N];
+              eo_apply<N, 1>(tab.S2t, Fvy, g3);
+              eo_apply<N, 1>(tab.S2t, Fvz, g5);
+#pragma unroll
+              for (int a = 0; a < N; ++a)
+                {
+                  const int idx = (s1 * N + s0) * N + a;
+                  Dl[idx] = g3[a];
+                  El[idx] = g5[a];
+                }
+            }
+        }
+      __syncthreads();
+
+      // y-lines (a = s0, qz = s1)
+      if (active)
+        {
+          Number v1[N], v2[N], v4[N], h1[N], t1[N], h2[N];
+#pragma unroll
+          for (int r = 0; r < N; ++r)
+            {
+              const int idx = (s1 * N + r) * N + s0;
+              v1[r] = Al[idx];
+              v2[r] = Bl[idx];
+              v4[r] = Cl[idx];
+            }
+          eo_apply<N, 1>(tab.S2t, v1, h1);
+          eo_apply<N, 1>(tab.D2t, v2, t1);
+#pragma unroll
+          for (int b = 0; b < N; ++b)
+            h1[b] += t1[b];
+          eo_apply<N, 1>(tab.S2t, v4, h2);
+          Number h3[N];
+          if constexpr (VEL)
+            {
+              Number v3[N], v5[N];
+#pragma unroll
+              for (int r = 0; r < N; ++r)
+                {
+                  const int idx = (s1 * N + r) * N + s0;
+                  v3[r] = Dl[idx];
+                  v5[r] = El[idx];
+                }
+              eo_apply<N, -1>(tab.SDt, v3, t1);
+#pragma unroll
+              for (int b = 0; b < N; ++b)
+                h1[b] += t1[b];
+              eo_apply<N, 1>(tab.S2t, v5, h3);
+            }
+#pragma unroll
+          for (int b = 0; b < N; ++b)
+            {
+              const int idx = (s1 * N + b) * N + s0;
+              Al[idx] = h1[b];
+              Bl[idx] = h2[b];
+              if constexpr (VEL)
+                Cl[idx] = h3[b];
+            }
+        }
+      __syncthreads();
+
+      // z-lines (a = s0, b = s1)
+      if (active)
+        {
+          Number v1[N], v2[N], d[N], t1[N];
+#pragma unroll
+          for (int r = 0; r < N; ++r)
+            {
+              const int idx = (r * N + s1) * N + s0;
+              v1[r] = Al[idx];
+              v2[r] = Bl[idx];
+            }
+          eo_apply<N, 1>(tab.S2t, v1, d);
+          eo_apply<N, 1>(tab.D2t, v2, t1);
+#pragma unroll
+          for (int c = 0; c < N; ++c)
+            d[c] += t1[c];
+          if constexpr (VEL)
+            {
+              Number v3[N];
+#pragma unroll
+              for (int r = 0; r < N; ++r)
+                v3[r] = Cl[(r * N + s1) * N + s0];
+              eo_apply<N, -1>(tab.SDt, v3, t1);
+#pragma unroll
+              for (int c = 0; c < N; ++c)
+                d[c] += t1[c];
+            }
+#pragma unroll
+          for (int c = 0; c < N; ++c)
+            Al[(c * N + s1) * N + s0] = d[c];
+        }
+      __syncthreads();
+
+      // x-lines (b = s0, c = s1): exchange shared x-face, scatter
+      Number Y[N];
+      if (active)
+        {
+#pragma unroll
+          for (int a = 0; a < N; ++a)
+            Y[a] = Al[(s1 * N + s0) * N + a];
+          B[s * CPB + l] = Y[N - 1];
+        }
+      __syncthreads();
+      if (cell_ok)
+        {
+          if (l > 0)
+            Y[0] += B[s * CPB + l - 1];
+          scatter_xline<P, Number, CPB>(prm, g, l, s0, s1, Y, dst);
+        }
+    }
+
+    template <typename Number, int P, int COEF, bool VEL>
+    hipError_t vmult_passes(const CellParams<Number> &prm, const VmultTables<Number> &tab,
+                            const Number *src, Number *dst, const Number *heav, double *partials,
+                            SolverScalars *scal, hipStream_t stream)
+    {
+      constexpr bool F32 = sizeof(Number) == 4;
+      constexpr int CPB = cpb<P, F32>();
+      constexpr int THREADS = block_threads<P, F32>();
+      const ColourGrid cg = colour_grid(prm.nx, prm.ny, prm.nz, CPB);
+      for (int c = 0; c < 8; ++c)
+        {
+          if (cg.blocks[c] == 0)
+            continue;
+          vmult_kernel<P, Number, COEF, VEL, CPB, THREADS><<<cg.blocks[c], THREADS, 0, stream>>>(
+            src, dst, heav, tab, prm, c, partials, cg.offset[c], cg.total, c == cg.last ? 1 : 0,
+            scal);
+        }
+      return hipGetLastError();
+    }
+
+    template <typename Number, int P, int COEF, bool VEL>
+    hipError_t diag_passes(const CellParams<Number> &prm, const DiagTables<Number> &tab,
+                           Number *dst, const Number *heav, hipStream_t stream)
+    {
+      constexpr bool F32 = sizeof(Number) == 4;
+      constexpr int CPB = cpb<P, F32>();
+      constexpr int THREADS = block_threads<P, F32>();
+      const ColourGrid cg = colour_grid(prm.nx, prm.ny, prm.nz, CPB);
+      for (int c = 0; c < 8; ++c)
+        {
+          if (cg.blocks[c] == 0)
+            continue;
+          diag_kernel<P, Number, COEF, VEL, CPB, THREADS><<<cg.blocks[c], THREADS, 0, stream>>>(
+            dst, heav, tab, prm, c);
+        }
+      return hipGetLastError();
+    }
+
+    template <typename Number, int P>
+    hipError_t vmult_deg(int coef, bool vel, const CellParams<Number> &prm,
+                         const VmultTables<Number> &tab, const Number *src, Number *dst,
+                         const Number *heav, double *partials, SolverScalars *scal, hipStream_t s)
+    {
+      switch (coef * 2 + (vel ? 1 : 0))
+        {
+          case 0: return vmult_passes<Number, P, 0, false>(prm, tab, src, dst, heav, partials, scal, s);
+          case 1: return vmult_passes<Number, P, 0, true>(prm, tab, src, dst, heav, partials, scal, s);
+          case 2: return vmult_passes<Number, P, 1, false>(prm, tab, src, dst, heav, partials, scal, s);
+          case 3: return vmult_passes<Number, P, 1, true>(prm, tab, src, dst, heav, partials, scal, s);
+          case 4: return vmult_passes<Number, P, 2, false>(prm, tab, src, dst, heav, partials, scal, s);
+          case 5: return vmult_passes<Number, P, 2, true>(prm, tab, src, dst, heav, partials, scal, s);
+          default: return hipErrorInvalidValue;
+        }
+    }
+
+    template <typename Number, int P>
+    hipError_t diag_deg(int coef, bool vel, const CellParams<Number> &prm,
+                        const DiagTables<Number> &tab, Number *dst, const Number *heav,
+                        hipStream_t s)
+    {
+      switch (coef * 2 + (vel ? 1 : 0))
+        {
+          case 0: return diag_passes<Number, P, 0, false>(prm, tab, dst, heav, s);
+          case 1: return diag_passes<Number, P, 0, true>(prm, tab, dst, heav, s);
+          case 2: return diag_passes<Number, P, 1, false>(prm, tab, dst, heav, s);
+          case 3: return diag_passes<Number, P, 1, true>(prm, tab, dst, heav, s);
+          case 4: return diag_passes<Number, P, 2, false>(prm, tab, dst, heav, s);
+          case 5: return diag_passes<Number, P, 2, true>(prm, tab, dst, heav, s);
+          default: return hipErrorInvalidValue;
+        }
+    }
+  } // namespace
+
+  int cells_per_block(int degree, bool fp32) { return cells_per_block_for(degree, fp32); }
+
+  int threads_per_block(int degree, bool fp32)
+  {
+    const int n = degree + 1;
+    return ((cells_per_block(degree, fp32) * n * n + 63) / 64) * 64;
+  }
+
+  ColourGrid colour_grid(int nx, int ny, int nz, int cpb_)
+  {
+    ColourGrid g{};
+    const int nsx = (nx + cpb_ - 1) / cpb_;
+    int off = 0;
+    g.last = -1;
+    for (int c = 0; c < 8; ++c)
+      {
+        const int ox = c & 1, oy = (c >> 1) & 1, oz = (c >> 2) & 1;
+        const int m = ((nsx - ox + 1) / 2) * ((ny - oy + 1) / 2) * ((nz - oz + 1) / 2);
+        g.blocks[c] = m;
+        g.offset[c] = off;
+        off += m;
+        if (m > 0)
+          g.last = c;
+      }
+    g.total = off;
+    return g;
+  }
+
+  template <typename Number>
+  hipError_t launch_vmult(int degree, int coef, bool vel, const CellParams<Number> &prm,
+                          const VmultTables<Number> &tab, const Number *src, Number *dst,
+                          const Number *heaviside, double *partials, SolverScalars *scal,
+                          hipStream_t stream)
+  {
+    switch (degree)
+      {
+        case 1: return vmult_deg<Number, 1>(coef, vel, prm, tab, src, dst, heaviside, partials, scal, stream);
+        case 2: return vmult_deg<Number, 2>(coef, vel, prm, tab, src, dst, heaviside, partials, scal, stream);
+        case 3: return vmult_deg<Number, 3>(coef, vel, prm, tab, src, dst, heaviside, partials, scal, stream);
+        case 4: return vmult_deg<Number, 4>(coef, vel, prm, tab, src, dst, heaviside, partials, scal, stream);
+        case 5: return vmult_deg<Number, 5>(coef, vel, prm, tab, src, dst, heaviside, partials, scal, stream);
+        case 6: return vmult_deg<Number, 6>(coef, vel, prm, tab, src, dst, heaviside, partials, scal, stream);
+        default: return hipErrorInvalidValue;
+      }
+  }
+
+  template <typename Number>
+  hipError_t launch_diagonal(int degree, int coef, bool vel, const CellParams<Number> &prm,
+                             const DiagTables<Number> &tab, Number *diag,
+                             const Number *heaviside, hipStream_t stream)
+  {
+    switch (degree)
+      {
+        case 1: return diag_deg<Number, 1>(coef, vel, prm, tab, diag, heaviside, stream);
+        case 2: return diag_deg<Number, 2>(coef, vel, prm, tab, diag, heaviside, stream);
+        case 3: return diag_deg<Number, 3>(coef, vel, prm, tab, diag, heaviside, stream);
+        case 4: return diag_deg<Number, 4>(coef, vel, prm, tab, diag, heaviside, stream);
+        case 5: return diag_deg<Number, 5>(coef, vel, prm, tab, diag, heaviside, stream);
+        case 6: return diag_deg<Number, 6>(coef, vel, prm, tab, diag, heaviside, stream);
+        default: return hipErrorInvalidValue;
+      }
+  }
+
+  template hipError_t launch_vmult<double>(int, int, bool, const CellParams<double> &,
+                                           const VmultTables<double> &, const double *, double *,
+                                           const double *, double *, SolverScalars *, hipStream_t);
+  template hipError_t launch_vmult<float>(int, int, bool, const CellParams<float> &,
+                                          const VmultTables<float> &, const float *, float *,
+                                          const float *, double *, SolverScalars *, hipStream_t);
+  template hipError_t launch_diagonal<double>(int, int, bool, const CellParams<double> &,
+                                              const DiagTables<double> &, double *, const double *,
+                                              hipStream_t);
+  template hipError_t launch_diagonal<float>(int, int, bool, const CellParams<float> &,
+                                             const DiagTables<float> &, float *, const float *,
+                                             hipStream_t);
+} // namespace mpdg

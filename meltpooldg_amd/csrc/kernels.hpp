@@ -1,0 +1,129 @@
+// Shared declarations between the C-ABI layer and the HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "tables.hpp"
+
+namespace mpdg
+{
+  // Per-handle operator constants passed by value to the cell kernels.
+  template <typename Number>
+  struct CellParams
+  {
+    int nx, ny, nz;       // cells per direction
+    int n1x, n1y, n1z;    // lattice nodes per direction
+    unsigned faces;       // Dirichlet face mask (bit 2d: lo, 2d+1: hi)
+    Number mass;          // rho_cp / dt * |J| for a constant material
+    Number dtinv_vol;     // |J| / dt (two-phase: multiplied by rho_cp(H) at q)
+    Number kf[3];         // |J| / h_d^2 (Laplace metric, affine Cartesian cells)
+    Number vf[3];         // |J| / h_d (convection metric)
+    Number k0;            // material conductivity (constant material)
+    Number kamp;          // separable-sine conductivity amplitude
+    Number rho_cp0;       // constant rho*cp (convection term)
+    Number w0;            // vortex axial velocity
+    Number g_rho, g_cp, g_k, l_rho, l_cp, l_k, g_rhocp, l_rhocp;
+    int sharp, volume_specific;
+    const Number *sin2pi[3]; // sin(2 pi x) at the quadrature coordinates of each direction
+    const Number *sinpi2[3]; // sin^2(pi x) at the same coordinates
+  };
+
+  // 1D tables of the vmult kernel in even-odd form (device copy by value).
+  template <typename Number>
+  struct VmultTables
+  {
+    EvenOdd<Number> S, St, C, Ct;
+    Number W[kMaxN];
+  };
+
+  // 1D tables of the closed-form diagonal kernel.
+  template <typename Number>
+  struct DiagTables
+  {
+    EvenOdd<Number> S, S2t, D2t, SDt;
+    Number W[kMaxN];
+  };
+
+  // Device-resident Krylov scalars; the kernels read and update them so the
+  // host only polls for convergence every few iterations.
+  struct SolverScalars
+  {
+    double rz, rz_old, pap, alpha, beta, rr, res, tol;
+    double res0;
+    int iteration, done, status, max_it;
+    unsigned counter;
+    unsigned pad[3];
+  };
+
+  enum CoefMode
+  {
+    kCoefConstant = 0,
+    kCoefSine = 1,
+    kCoefHeaviside = 2
+  };
+
+  // Launch shape of one colour pass.
+  struct ColourGrid
+  {
+    int blocks[8];
+    int offset[8];  // prefix sum of blocks (index of the pass's first partial)
+    int total;
+    int last;       // last colour with blocks (finalises the partial dot)
+  };
+
+  // Cells per workgroup (one row segment along x). Sized so that 64-lane
+  // wavefronts are mostly full and the LDS tensors fit the 160 KiB of a CU.
+  constexpr int cells_per_block_for(int degree, bool fp32)
+  {
+    return degree <= 2 ? 32 : (degree <= 4 ? 16 : (fp32 ? 16 : 8));
+  }
+  int cells_per_block(int degree, bool fp32);
+  int threads_per_block(int degree, bool fp32);
+  ColourGrid colour_grid(int nx, int ny, int nz, int cpb);
+
+  // Cell-loop vmult: dst = A src over all 8 colour passes. With scal != nullptr
+  // it also reduces p.Ap into scal->pap and sets scal->alpha = rz / pap, and it
+  // does nothing once scal->done is set.
+  template <typename Number>
+  hipError_t launch_vmult(int degree, int coef, bool vel, const CellParams<Number> &prm,
+                          const VmultTables<Number> &tab, const Number *src, Number *dst,
+                          const Number *heaviside, double *partials, SolverScalars *scal,
+                          hipStream_t stream);
+
+  template <typename Number>
+  hipError_t launch_diagonal(int degree, int coef, bool vel, const CellParams<Number> &prm,
+                             const DiagTables<Number> &tab, Number *diag,
+                             const Number *heaviside, hipStream_t stream);
+
+  // Vector kernels (solver_kernels.hip).
+  constexpr int kVecBlocks = 1024;
+  constexpr int kVecThreads = 256;
+
+  hipError_t launch_invert_diagonal(double *d, int64_t n, hipStream_t s);
+  hipError_t launch_jacobi(double *dst, const double *src, const double *inv, int64_t n, hipStream_t s);
+
+  template <typename Number>
+  hipError_t launch_cg_init(const double *b, const Number *ax, Number *r, Number *p,
+                            const Number *inv_diag, int64_t n, double rel_tol, double abs_tol,
+                            int max_it, double *partials, SolverScalars *scal, hipStream_t s);
+  template <typename Number>
+  hipError_t launch_cg_update(Number *x, Number *r, const Number *p, const Number *q,
+                              const Number *inv_diag, int64_t n, double *partials,
+                              SolverScalars *scal, double *history, int history_len,
+                              hipStream_t s);
+  template <typename Number>
+  hipError_t launch_cg_direction(Number *p, const Number *r, const Number *inv_diag, int64_t n,
+                                 const SolverScalars *scal, hipStream_t s);
+
+  // Mixed precision helpers.
+  hipError_t launch_residual(double *r, const double *b, const double *ax, int64_t n,
+                             double *partials, SolverScalars *scal, hipStream_t s);
+  hipError_t launch_to_f32(float *dst, const double *src, int64_t n, hipStream_t s);
+  hipError_t launch_add_f32(double *x, const float *d, int64_t n, hipStream_t s);
+  hipError_t launch_fill(double *x, double v, int64_t n, hipStream_t s);
+  hipError_t launch_fill_f32(float *x, float v, int64_t n, hipStream_t s);
+  hipError_t launch_dot(const double *a, const double *b, int64_t n, double *partials,
+                        double *out, unsigned *counter, hipStream_t s);
+} // namespace mpdg

@@ -1,0 +1,372 @@
+// Vector kernels of the device-resident Krylov solvers: fused Jacobi-PCG
+// updates with deterministic FP64 reductions, diagonal inversion, and the
+// precision conversions of the defect-correction loop.
+//
+// Reductions: each kernel runs a fixed grid (kVecBlocks x kVecThreads) with a
+// grid-stride loop, so every partial sum covers a fixed index set; the last
+// block to finish (atomic ticket) adds the partials in index order. Results
+// are therefore bitwise reproducible run to run.
+//
+// Reference semantics restated: deal.II SolverCG under ReductionControl
+// (include/meltpooldg/linear_algebra/linear_solver.hpp:36-38,106-111),
+// JacobiPreconditioner::vmult dst = src .* inverse_diag
+// (include/meltpooldg/linear_algebra/preconditioner_jacobi.hpp:51-56),
+// and the inversion guard of compute_inverse_diagonal_from_matrixfree
+// (source/heat/heat_diffuse_operator.cpp:282-283).
+#include "kernels.hpp"
+
+namespace mpdg
+{
+  namespace
+  {
+    constexpr int T = kVecThreads;
+
+    __device__ __forceinline__ double wave_sum(double v)
+    {
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1)
+        v += __shfl_xor(v, off, 64);
+      return v;
+    }
+
+    // Sums NV values per thread over the block; result valid in thread 0.
+    template <int NV>
+    __device__ __forceinline__ void block_sums(double (&v)[NV], double (*sh)[T / 64])
+    {
+#pragma unroll
+      for (int k = 0; k < NV; ++k)
+        v[k] = wave_sum(v[k]);
+      const int w = threadIdx.x >> 6;
+      if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+          sh[k][w] = v[k];
+      __syncthreads();
+      if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+          {
+            double s = 0.0;
+            for (int i = 0; i < T / 64; ++i)
+              s += sh[k][i];
+            v[k] = s;
+          }
+      __syncthreads();
+    }
+
+    // Writes this block's NV partials, and in the last block returns the
+    // ordered totals (in thread 0). Returns true in the last block.
+    template <int NV>
+    __device__ __forceinline__ bool finish_reduction(double (&v)[NV], double *partials,
+                                                     unsigned *counter, double (&tot)[NV])
+    {
+      __shared__ double sh[NV][T / 64];
+      __shared__ bool last;
+      block_sums<NV>(v, sh);
+      if (threadIdx.x == 0)
+        {
+#pragma unroll
+          for (int k = 0; k < NV; ++k)
+            partials[k * kVecBlocks + blockIdx.x] = v[k];
+          __threadfence();
+          const unsigned prev = atomicAdd(counter, 1u);
+          last = prev == gridDim.x - 1;
+        }
+      __syncthreads();
+      if (!last)
+        return false;
+      __threadfence();
+      double s[NV];
+#pragma unroll
+      for (int k = 0; k < NV; ++k)
+        {
+          s[k] = 0.0;
+          for (int i = threadIdx.x; i < (int)gridDim.x; i += T)
+            s[k] += __hip_atomic_load(partials + k * kVecBlocks + i, __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
+        }
+      block_sums<NV>(s, sh);
+      if (threadIdx.x == 0)
+        {
+#pragma unroll
+          for (int k = 0; k < NV; ++k)
+            tot[k] = s[k];
+          *counter = 0u;
+        }
+      return true;
+    }
+
+    __global__ void __launch_bounds__(T) invert_kernel(double *d, int64_t n)
+    {
+      for (int64_t i = blockIdx.x * (int64_t)T + threadIdx.x; i < n; i += (int64_t)gridDim.x * T)
+        {
+          const double v = d[i];
+          d[i] = fabs(v) > 1.0e-10 ? 1.0 / v : 1.0;
+        }
+    }
+
+    __global__ void __launch_bounds__(T)
+      jacobi_kernel(double *dst, const double *src, const double *inv, int64_t n)
+    {
+      for (int64_t i = blockIdx.x * (int64_t)T + threadIdx.x; i < n; i += (int64_t)gridDim.x * T)
+        dst[i] = src[i] * inv[i];
+    }
+
+    // r = b - Ax0, p = D r, rr = r.r, rz = r.p; sets tol, res0, done.
+    template <typename Number>
+    __global__ void __launch_bounds__(T)
+      cg_init_kernel(const double *b, const Number *ax, Number *r, Number *p, const Number *inv,
+                     int64_t n, double rel_tol, double abs_tol, int max_it, double *partials,
+                     SolverScalars *scal)
+    {
+      double v[2] = {0.0, 0.0};
+      for (int64_t i = blockIdx.x * (int64_t)T + threadIdx.x; i < n; i += (int64_t)gridDim.x * T)
+        {
+          const Number ri = (Number)(b[i] - (ax ? (double)ax[i] : 0.0));
+          const Number zi = ri * inv[i];
+          r[i] = ri;
+          p[i] = zi;
+          v[0] += (double)ri * (double)ri;
+          v[1] += (double)ri * (double)zi;
+        }
+      double tot[2];
+      if (finish_reduction<2>(v, partials, &scal->counter, tot) && threadIdx.x == 0)
+        {
+          const double res = sqrt(tot[0]);
+          scal->rr = tot[0];
+          scal->rz = tot[1];
+          scal->res = res;
+          scal->res0 = res;
+          scal->tol = fmax(abs_tol, rel_tol * res);
+          scal->iteration = 0;
+          scal->max_it = max_it;
+          scal->status = 1;
+          scal->done = 0;
+          if (res <= scal->tol)
+            {
+              scal->done = 1;
+              scal->status = 0;
+            }
+        }
+    }
+
+    // x += alpha p, r -= alpha q, rr = r.r, rz = r.(D r); convergence check.
+    template <typename Number>
+    __global__ void __launch_bounds__(T)
+      cg_update_kernel(Number *x, Number *r, const Number *p, const Number *q, const Number *inv,
+                       int64_t n, double *partials, SolverScalars *scal, double *history,
+                       int history_len)
+    {
+      if (scal->done)
+        return;
+      const double alpha = scal->alpha;
+      const Number a = (Number)alpha;
+      double v[2] = {0.0, 0.0};
+      for (int64_t i = blockIdx.x * (int64_t)T + threadIdx.x; i < n; i += (int64_t)gridDim.x * T)
+        {
+          const Number pi = p[i];
+          x[i] = x[i] + a * pi;
+          const Number ri = r[i] - a * q[i];
+          r[i] = ri;
+          v[0] += (double)ri * (double)ri;
+          v[1] += (double)ri * (double)(ri * inv[i]);
+        }
+      double tot[2];
+      if (finish_reduction<2>(v, partials, &scal->counter, tot) && threadIdx.x == 0)
+        {
+          const int it = scal->iteration + 1;
+          const double res = sqrt(fabs(tot[0]));
+          scal->iteration = it;
+          scal->rr = tot[0];
+          scal->res = res;
+          if (history && it < history_len)
+            history[it] = res;
+          scal->beta = tot[1] / scal->rz;
+          scal->rz_old = scal->rz;
+          scal->rz = tot[1];
+          if (res <= scal->tol)
+            {
+              scal->done = 1;
+              scal->status = 0;
+            }
+          else if (it >= scal->max_it || isnan(res))
+            {
+              scal->done = 1;
+              scal->status = 1;
+            }
+        }
+    }
+
+    // p = D r + beta p
+    template <typename Number>
+    __global__ void __launch_bounds__(T)
+      cg_direction_kernel(Number *p, const Number *r, const Number *inv, int64_t n,
+                          const SolverScalars *scal)
+    {
+      if (scal->done)
+        return;
+      const Number beta = (Number)scal->beta;
+      for (int64_t i = blockIdx.x * (int64_t)T + threadIdx.x; i < n; i += (int64_t)gridDim.x * T)
+        p[i] = r[i] * inv[i] + beta * p[i];
+    }
+
+    // r = b - ax (FP64), rr -> scal->rr / res.
+    __global__ void __launch_bounds__(T)
+      residual_kernel(double *r, const double *b, const double *ax, int64_t n, double *partials,
+                      SolverScalars *scal)
+    {
+      double v[1] = {0.0};
+      for (int64_t i = blockIdx.x * (int64_t)T + threadIdx.x; i < n; i += (int64_t)gridDim.x * T)
+        {
+          const double ri = b[i] - ax[i];
+          r[i] = ri;
+          v[0] += ri * ri;
+        }
+      double tot[1];
+      if (finish_reduction<1>(v, partials, &scal->counter, tot) && threadIdx.x == 0)
+        {
+          scal->rr = tot[0];
+          scal->res = sqrt(tot[0]);
+        }
+    }
+
+    __global__ void __launch_bounds__(T)
+      dot_kernel(const double *a, const double *b, int64_t n, double *partials, double *out,
+                 unsigned *counter)
+    {
+      double v[1] = {0.0};
+      for (int64_t i = blockIdx.x * (int64_t)T + threadIdx.x; i < n; i += (int64_t)gridDim.x * T)
+        v[0] += a[i] * b[i];
+      double tot[1];
+      if (finish_reduction<1>(v, partials, counter, tot) && threadIdx.x == 0)
+        *out = tot[0];
+    }
+
+    __global__ void __launch_bounds__(T) to_f32_kernel(float *dst, const double *src, int64_t n)
+    {
+      for (int64_t i = blockIdx.x * (int64_t)T + threadIdx.x; i < n; i += (int64_t)gridDim.x * T)
+        dst[i] = (float)src[i];
+    }
+
+    __global__ void __launch_bounds__(T) add_f32_kernel(double *x, const float *d, int64_t n)
+    {
+      for (int64_t i = blockIdx.x * (int64_t)T + threadIdx.x; i < n; i += (int64_t)gridDim.x * T)
+        x[i] += (double)d[i];
+    }
+
+    __global__ void __launch_bounds__(T) fill_kernel(double *x, double v, int64_t n)
+    {
+      for (int64_t i = blockIdx.x * (int64_t)T + threadIdx.x; i < n; i += (int64_t)gridDim.x * T)
+        x[i] = v;
+    }
+
+    __global__ void __launch_bounds__(T) fill_f32_kernel(float *x, float v, int64_t n)
+    {
+      for (int64_t i = blockIdx.x * (int64_t)T + threadIdx.x; i < n; i += (int64_t)gridDim.x * T)
+        x[i] = v;
+    }
+
+    inline int grid_for(int64_t n)
+    {
+      const int64_t g = (n + T - 1) / T;
+      return (int)(g < kVecBlocks ? (g < 1 ? 1 : g) : kVecBlocks);
+    }
+  } // namespace
+
+  hipError_t launch_invert_diagonal(double *d, int64_t n, hipStream_t s)
+  {
+    invert_kernel<<<grid_for(n), T, 0, s>>>(d, n);
+    return hipGetLastError();
+  }
+
+  hipError_t launch_jacobi(double *dst, const double *src, const double *inv, int64_t n,
+                           hipStream_t s)
+  {
+    jacobi_kernel<<<grid_for(n), T, 0, s>>>(dst, src, inv, n);
+    return hipGetLastError();
+  }
+
+  // Reductions always use the full fixed grid so each partial covers a fixed index set.
+  template <typename Number>
+  hipError_t launch_cg_init(const double *b, const Number *ax, Number *r, Number *p,
+                            const Number *inv_diag, int64_t n, double rel_tol, double abs_tol,
+                            int max_it, double *partials, SolverScalars *scal, hipStream_t s)
+  {
+    cg_init_kernel<Number><<<kVecBlocks, T, 0, s>>>(b, ax, r, p, inv_diag, n, rel_tol, abs_tol,
+                                                    max_it, partials, scal);
+    return hipGetLastError();
+  }
+
+  template <typename Number>
+  hipError_t launch_cg_update(Number *x, Number *r, const Number *p, const Number *q,
+                              const Number *inv_diag, int64_t n, double *partials,
+                              SolverScalars *scal, double *history, int history_len, hipStream_t s)
+  {
+    cg_update_kernel<Number><<<kVecBlocks, T, 0, s>>>(x, r, p, q, inv_diag, n, partials, scal,
+                                                      history, history_len);
+    return hipGetLastError();
+  }
+
+  template <typename Number>
+  hipError_t launch_cg_direction(Number *p, const Number *r, const Number *inv_diag, int64_t n,
+                                 const SolverScalars *scal, hipStream_t s)
+  {
+    cg_direction_kernel<Number><<<grid_for(n), T, 0, s>>>(p, r, inv_diag, n, scal);
+    return hipGetLastError();
+  }
+
+  hipError_t launch_residual(double *r, const double *b, const double *ax, int64_t n,
+                             double *partials, SolverScalars *scal, hipStream_t s)
+  {
+    residual_kernel<<<kVecBlocks, T, 0, s>>>(r, b, ax, n, partials, scal);
+    return hipGetLastError();
+  }
+
+  hipError_t launch_dot(const double *a, const double *b, int64_t n, double *partials, double *out,
+                        unsigned *counter, hipStream_t s)
+  {
+    dot_kernel<<<kVecBlocks, T, 0, s>>>(a, b, n, partials, out, counter);
+    return hipGetLastError();
+  }
+
+  hipError_t launch_to_f32(float *dst, const double *src, int64_t n, hipStream_t s)
+  {
+    to_f32_kernel<<<grid_for(n), T, 0, s>>>(dst, src, n);
+    return hipGetLastError();
+  }
+
+  hipError_t launch_add_f32(double *x, const float *d, int64_t n, hipStream_t s)
+  {
+    add_f32_kernel<<<grid_for(n), T, 0, s>>>(x, d, n);
+    return hipGetLastError();
+  }
+
+  hipError_t launch_fill(double *x, double v, int64_t n, hipStream_t s)
+  {
+    fill_kernel<<<grid_for(n), T, 0, s>>>(x, v, n);
+    return hipGetLastError();
+  }
+
+  hipError_t launch_fill_f32(float *x, float v, int64_t n, hipStream_t s)
+  {
+    fill_f32_kernel<<<grid_for(n), T, 0, s>>>(x, v, n);
+    return hipGetLastError();
+  }
+
+  template hipError_t launch_cg_init<double>(const double *, const double *, double *, double *,
+                                             const double *, int64_t, double, double, int,
+                                             double *, SolverScalars *, hipStream_t);
+  template hipError_t launch_cg_init<float>(const double *, const float *, float *, float *,
+                                            const float *, int64_t, double, double, int, double *,
+                                            SolverScalars *, hipStream_t);
+  template hipError_t launch_cg_update<double>(double *, double *, const double *, const double *,
+                                               const double *, int64_t, double *, SolverScalars *,
+                                               double *, int, hipStream_t);
+  template hipError_t launch_cg_update<float>(float *, float *, const float *, const float *,
+                                              const float *, int64_t, double *, SolverScalars *,
+                                              double *, int, hipStream_t);
+  template hipError_t launch_cg_direction<double>(double *, const double *, const double *, int64_t,
+                                                  const SolverScalars *, hipStream_t);
+  template hipError_t launch_cg_direction<float>(float *, const float *, const float *, int64_t,
+                                                 const SolverScalars *, hipStream_t);
+} // namespace mpdg

@@ -1,0 +1,239 @@
+"""Host-side mirror of MeltPoolDG's heat-operator interface over the C ABI.
+
+Names, argument meaning and error behaviour follow the reference:
+
+* ``HeatDiffuseOperator``  <- HeatDiffuseMultiPhaseOperator<3,double>
+  (include/meltpooldg/heat/heat_diffuse_operator.hpp:109-366) as an
+  OperatorMatrixFree (include/meltpooldg/core/operator_base.hpp:189-283):
+  ``vmult(dst, src)``, ``compute_inverse_diagonal_from_matrixfree(diag)``,
+  ``reset_time_increment(dt)``.
+* ``JacobiPreconditioner`` <- include/meltpooldg/linear_algebra/preconditioner_jacobi.hpp:25-89
+* ``LinearSolverData``     <- include/meltpooldg/linear_algebra/linear_solver_data.hpp:37-51
+* ``LinearSolver.solve``   <- include/meltpooldg/linear_algebra/linear_solver.hpp:26-134;
+  returns the iteration count, raises ``NoConvergence`` where deal.II throws
+  SolverControl::NoConvergence.
+
+Vectors are torch tensors on the GPU (torch is only the device-memory and
+stream plumbing); every arithmetic operation runs in libmpdg_heat.so.
+"""
+import ctypes
+from dataclasses import dataclass, field
+
+from . import _lib as L
+
+
+class NoConvergence(RuntimeError):
+    """dealii::SolverControl::NoConvergence (linear_solver.hpp:124-129)."""
+
+    def __init__(self, iterations, residual, message=""):
+        super().__init__(f"Iterative method reported convergence failure in step {iterations}. "
+                         f"The residual in the last step was {residual}. {message}")
+        self.last_step = iterations
+        self.last_residual = residual
+
+
+class ZeroTimeIncrement(ValueError):
+    """ExcZeroTimeIncrement (include/meltpooldg/core/exceptions.hpp:6)."""
+
+
+@dataclass
+class PhaseData:
+    """One phase of MaterialData (density, specific heat capacity, conductivity)."""
+    density: float = 1.0
+    specific_heat_capacity: float = 1.0
+    thermal_conductivity: float = 1.0
+
+
+@dataclass
+class MaterialData:
+    """Material model of include/meltpooldg/core/material.hpp:46-55 ('gas' or 'gas_liquid')."""
+    type: str = "gas"
+    gas: PhaseData = field(default_factory=PhaseData)
+    liquid: PhaseData = field(default_factory=PhaseData)
+    two_phase_fluid_properties_transition_type: str = "smooth"
+    use_volume_specific_thermal_capacity_for_phase_interpolation: bool = False
+
+
+@dataclass
+class LinearSolverData:
+    """include/meltpooldg/linear_algebra/linear_solver_data.hpp:37-51 (same defaults)."""
+    do_matrix_free: bool = True
+    preconditioner_type: str = "Identity"
+    solver_type: str = "GMRES"
+    max_iterations: int = 10000
+    rel_tolerance: float = 1e-12
+    abs_tolerance: float = 1e-20
+    monitor_type: str = "none"
+    # engine extensions
+    mixed_precision: bool = False
+    inner_rel_tolerance: float = 1e-2
+    check_interval: int = 0
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream_handle():
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class HeatDiffuseOperator:
+    """Matrix-free heat tangent J = rho_cp/dt M + K_k + rho_cp C_u on a box mesh.
+
+    Parameters mirror the reference's set-up: FE_Q(degree) on Gauss-Lobatto
+    points with QGauss(degree+1) on ``n_cells`` affine hex cells of the box,
+    homogeneous Dirichlet on ``dirichlet_faces`` (bit 2d: x_d = lo, 2d+1: hi).
+    """
+
+    def __init__(self, n_cells, degree, material=None, box=((0.0, 0.0, 0.0), (1.0, 1.0, 1.0)),
+                 precision="fp64", conductivity_amplitude=0.0, velocity=None, velocity_w0=0.2,
+                 dirichlet_faces=0x3F, stream=None):
+        self._lib = L.load()
+        n = (n_cells,) * 3 if isinstance(n_cells, int) else tuple(n_cells)
+        mat = material or MaterialData()
+        d = L.HeatDesc()
+        d.n_cells[:] = [int(v) for v in n]
+        d.degree = int(degree)
+        d.box_lo[:] = [float(v) for v in box[0]]
+        d.box_hi[:] = [float(v) for v in box[1]]
+        d.precision = L.MPDG_FP32 if precision == "fp32" else L.MPDG_FP64
+        d.material = 0 if mat.type == "gas" else 1
+        for dst, src in ((d.gas, mat.gas), (d.liquid, mat.liquid)):
+            dst.density = src.density
+            dst.specific_heat_capacity = src.specific_heat_capacity
+            dst.thermal_conductivity = src.thermal_conductivity
+        d.two_phase_sharp = 1 if mat.two_phase_fluid_properties_transition_type == "sharp" else 0
+        d.volume_specific_heat = int(mat.use_volume_specific_thermal_capacity_for_phase_interpolation)
+        d.conductivity_field = 1 if conductivity_amplitude else 0
+        d.conductivity_amplitude = float(conductivity_amplitude)
+        d.velocity = {None: 0, "none": 0, "vortex": 1}[velocity]
+        d.velocity_w0 = float(velocity_w0)
+        d.dirichlet_faces = int(dirichlet_faces)
+        d.stream = stream if stream is not None else _stream_handle()
+        self.desc = d
+        h = ctypes.c_void_p()
+        L.check(self._lib.mpdg_heat_create(ctypes.byref(d), ctypes.byref(h)))
+        self._h = h
+        self.n_cells = n
+        self.degree = degree
+        self.material = mat
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._lib.mpdg_heat_destroy(h)
+            self._h = None
+
+    # -- OperatorBase ---------------------------------------------------------
+    def reset_time_increment(self, dt):
+        st = self._lib.mpdg_heat_set_dt(self._h, float(dt))
+        if st == L.MPDG_ERR_ZERO_TIME_INCREMENT:
+            raise ZeroTimeIncrement(self._lib.mpdg_last_error().decode())
+        L.check(st)
+        self.time_increment = float(dt)
+
+    def n_dofs(self):
+        return int(self._lib.mpdg_heat_n_dofs(self._h))
+
+    def n_free_dofs(self):
+        return int(self._lib.mpdg_heat_n_free_dofs(self._h))
+
+    def set_heaviside(self, heaviside):
+        """Register the level-set heaviside field (read plain, interpolated at q)."""
+        self._check_vec(heaviside, "float64")
+        L.check(self._lib.mpdg_heat_set_heaviside(self._h, _ptr(heaviside)))
+        self._heaviside = heaviside
+
+    # -- OperatorMatrixFree ---------------------------------------------------
+    def vmult(self, dst, src):
+        if str(src.dtype).endswith("float32"):
+            self._check_vec(dst, "float32")
+            self._check_vec(src, "float32")
+            L.check(self._lib.mpdg_heat_vmult_f32(self._h, _ptr(dst), _ptr(src)))
+            return
+        self._check_vec(dst, "float64")
+        self._check_vec(src, "float64")
+        L.check(self._lib.mpdg_heat_vmult(self._h, _ptr(dst), _ptr(src)))
+
+    def compute_inverse_diagonal_from_matrixfree(self, diag):
+        self._check_vec(diag, "float64")
+        L.check(self._lib.mpdg_heat_inverse_diagonal(self._h, _ptr(diag)))
+
+    def time_vmult(self, dst, src, repeats=10):
+        ms = ctypes.c_double()
+        prec = L.MPDG_FP32 if str(src.dtype).endswith("float32") else L.MPDG_FP64
+        L.check(self._lib.mpdg_heat_time_vmult(self._h, _ptr(dst), _ptr(src), int(repeats), prec,
+                                               ctypes.byref(ms)))
+        return ms.value
+
+    def _check_vec(self, v, dtype):
+        if not getattr(v, "is_cuda", False):
+            raise ValueError("vectors must be GPU tensors")
+        if not str(v.dtype).endswith(dtype):
+            raise ValueError(f"expected {dtype}, got {v.dtype}")
+        if not v.is_contiguous() or v.numel() != self.n_dofs():
+            raise ValueError(f"expected a contiguous vector of {self.n_dofs()} entries")
+
+
+class JacobiPreconditioner:
+    """preconditioner_jacobi.hpp:25-89: update() -> inverse diagonal, vmult(dst, src)."""
+
+    def __init__(self, operator):
+        import torch
+        self.eq_operator = operator
+        self.inverse_diag = torch.empty(operator.n_dofs(), dtype=torch.float64, device="cuda")
+
+    def update(self, *_):
+        self.eq_operator.compute_inverse_diagonal_from_matrixfree(self.inverse_diag)
+
+    def reinit(self):
+        pass
+
+    def vmult(self, dst, src):
+        op = self.eq_operator
+        L.check(op._lib.mpdg_heat_jacobi_vmult(op._h, _ptr(dst), _ptr(src), _ptr(self.inverse_diag)))
+
+
+class LinearSolver:
+    """linear_solver.hpp:26-134."""
+
+    last_report = None
+
+    @staticmethod
+    def solve(system_matrix, solution, rhs, data, preconditioner=None, identifier=""):
+        """Solve system_matrix * solution = rhs; returns the iteration count.
+
+        ``preconditioner`` is a JacobiPreconditioner (or None / 'Identity').
+        The engine runs the whole Krylov loop on the device; with a Jacobi
+        preconditioner it recomputes the inverse diagonal first, as
+        HeatDiffuseOperation::solve does before every solve
+        (source/heat/heat_diffuse_operation.cpp:275-276).
+        """
+        op = system_matrix
+        prm = L.SolverParams()
+        op._lib.mpdg_solver_params_default(ctypes.byref(prm))
+        prm.solver_type = {"CG": 0, "GMRES": 1}[data.solver_type]
+        use_diag = isinstance(preconditioner, JacobiPreconditioner) or data.preconditioner_type == "Diagonal"
+        prm.preconditioner_type = 3 if use_diag else 0
+        prm.max_iterations = int(data.max_iterations)
+        prm.rel_tolerance = float(data.rel_tolerance)
+        prm.abs_tolerance = float(data.abs_tolerance)
+        prm.mixed_precision = 1 if data.mixed_precision else 0
+        prm.inner_rel_tolerance = float(data.inner_rel_tolerance)
+        prm.check_interval = int(data.check_interval)
+        op._check_vec(solution, "float64")
+        op._check_vec(rhs, "float64")
+        rep = L.SolverReport()
+        st = op._lib.mpdg_heat_solve(op._h, _ptr(solution), _ptr(rhs), ctypes.byref(prm),
+                                     ctypes.byref(rep))
+        LinearSolver.last_report = {"iterations": rep.iterations,
+                                    "outer_iterations": rep.outer_iterations,
+                                    "initial_residual": rep.initial_residual,
+                                    "final_residual": rep.final_residual,
+                                    "seconds": rep.seconds}
+        if st == L.MPDG_ERR_NOT_CONVERGED:
+            raise NoConvergence(rep.iterations, rep.final_residual, identifier)
+        L.check(st)
+        return rep.iterations

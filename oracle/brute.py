@@ -1,0 +1,140 @@
+"""Dense brute-force assembly of the heat tangent on tiny meshes (test infrastructure).
+
+Builds every cell matrix explicitly from 3D basis values/gradients at every
+quadrature point (no sum factorisation), assembles the global dense matrix and
+restricts it to the free DoFs. Implements the quadrature-point physics of
+tangent_local_cell_operation (source/heat/heat_diffuse_operator.cpp:1029-1042)
+with Material gas / gas_liquid (include/meltpooldg/core/material.templates.hpp:
+223-295, 515-524) and the nodal heaviside read plain and interpolated at q
+(:982-1002). The 1D basis comes from a monomial Vandermonde solve, a third
+independent construction besides oracle/tables.py and the C++ oracle.
+"""
+import numpy as np
+from numpy.polynomial import legendre as L
+
+from .rng import constrained_mask, lattice_shape
+
+
+def _basis_1d(p):
+    inner = np.sort(L.Legendre.basis(p).deriv().roots().real) if p > 1 else np.array([])
+    nodes = np.concatenate([[0.0], 0.5 * (inner + 1.0), [1.0]])
+    V = np.vander(nodes, p + 1, increasing=True)  # V[i, m] = x_i^m
+    C = np.linalg.solve(V, np.eye(p + 1))  # column i: monomial coefficients of l_i
+    gx, gw = L.leggauss(p + 1)
+    xq, wq = 0.5 * (gx + 1.0), 0.5 * gw
+    P = np.vander(xq, p + 1, increasing=True)
+    dP = np.zeros_like(P)
+    for m in range(1, p + 1):
+        dP[:, m] = m * xq ** (m - 1)
+    return xq, wq, P @ C, dP @ C
+
+
+def material(H, mat):
+    """(rho_cp, k) at a quadrature point for material dict ``mat``."""
+    if mat.get("type", "gas") == "gas":
+        return mat["gas_rho"] * mat["gas_cp"], mat["gas_k"] + 0.0 * H
+    w = (H > 0.5).astype(float) if mat.get("sharp") else H
+    k = (1.0 - w) * mat["gas_k"] + w * mat["liq_k"]
+    if mat.get("volume_specific"):
+        rc = (1.0 - w) * (mat["gas_rho"] * mat["gas_cp"]) + w * (mat["liq_rho"] * mat["liq_cp"])
+    else:
+        cp = (1.0 - w) * mat["gas_cp"] + w * mat["liq_cp"]
+        if mat.get("consistent_with_evaporation"):
+            rho = 1.0 / ((1.0 - H) / mat["gas_rho"] + H / mat["liq_rho"])
+        else:
+            rho = (1.0 - w) * mat["gas_rho"] + w * mat["liq_rho"]
+        rc = rho * cp
+    return rc, k
+
+
+def assemble(n, p, dt, mat, heaviside=None, k_amplitude=0.0, velocity=False, w0=0.2,
+             lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), dirichlet_faces=0x3F):
+    """Return (A_full, mask): dense lattice matrix without constraints and the
+    constrained-DoF mask. The constrained operator is A_full restricted to free
+    rows/columns, zero elsewhere."""
+    n = (n, n, n) if np.isscalar(n) else tuple(n)
+    n1 = lattice_shape(n, p)
+    N = n1[0] * n1[1] * n1[2]
+    xq, wq, S, D = _basis_1d(p)
+    nq = p + 1
+    h = [(hi[d] - lo[d]) / n[d] for d in range(3)]
+    # 3D basis at q: index q = (qz, qy, qx), i = (c, b, a)
+    Phi = np.einsum("zc,yb,xa->zyxcba", S, S, S).reshape(nq**3, nq**3)
+    Gx = np.einsum("zc,yb,xa->zyxcba", S, S, D).reshape(nq**3, nq**3) / h[0]
+    Gy = np.einsum("zc,yb,xa->zyxcba", S, D, S).reshape(nq**3, nq**3) / h[1]
+    Gz = np.einsum("zc,yb,xa->zyxcba", D, S, S).reshape(nq**3, nq**3) / h[2]
+    W = np.einsum("z,y,x->zyx", wq, wq, wq).reshape(-1) * h[0] * h[1] * h[2]
+    A = np.zeros((N, N))
+    loc = np.arange(nq)
+    for cz in range(n[2]):
+        for cy in range(n[1]):
+            for cx in range(n[0]):
+                ix = cx * p + loc
+                iy = cy * p + loc
+                iz = cz * p + loc
+                dofs = (ix[None, None, :] + n1[0] * (iy[None, :, None] + n1[1] * iz[:, None, None])).reshape(-1)
+                X = lo[0] + (cx + xq) * h[0]
+                Y = lo[1] + (cy + xq) * h[1]
+                Z = lo[2] + (cz + xq) * h[2]
+                Zq, Yq, Xq = [a.reshape(-1) for a in np.meshgrid(Z, Y, X, indexing="ij")]
+                Hq = Phi @ heaviside[dofs] if heaviside is not None else np.zeros(nq**3)
+                rc, k = material(Hq, mat)
+                rc = rc * np.ones(nq**3)
+                k = k * np.ones(nq**3)
+                if k_amplitude:
+                    k = k + k_amplitude * np.sin(2 * np.pi * Xq) * np.sin(2 * np.pi * Yq) * np.sin(2 * np.pi * Zq)
+                Aloc = Phi.T @ ((W * rc / dt)[:, None] * Phi)
+                for G in (Gx, Gy, Gz):
+                    Aloc += G.T @ ((W * k)[:, None] * G)
+                if velocity:
+                    vx = -np.sin(np.pi * Xq) ** 2 * np.sin(2 * np.pi * Yq)
+                    vy = np.sin(2 * np.pi * Xq) * np.sin(np.pi * Yq) ** 2
+                    vz = w0 * np.ones_like(Xq)
+                    adv = vx[:, None] * Gx + vy[:, None] * Gy + vz[:, None] * Gz
+                    Aloc += Phi.T @ ((W * rc)[:, None] * adv)
+                A[np.ix_(dofs, dofs)] += Aloc
+    mask = constrained_mask(n, p, dirichlet_faces)
+    return A, mask
+
+
+def constrained(A, mask):
+    Ac = A.copy()
+    Ac[mask, :] = 0.0
+    Ac[:, mask] = 0.0
+    return Ac
+
+
+def vmult(A, mask, src):
+    return constrained(A, mask) @ src
+
+
+def diagonal(A, mask):
+    d = np.diag(A).copy()
+    d[mask] = 0.0
+    return d
+
+
+def pcg_dense(Ac, b, inv_diag, rel_tol, abs_tol=1e-20, max_it=10000):
+    """Textbook Jacobi-PCG (Saad, Iterative Methods, Alg. 9.1) with the
+    ReductionControl rule ||r_k|| <= max(abs_tol, rel_tol ||r_0||), x0 = 0."""
+    x = np.zeros_like(b)
+    r = b.copy()
+    r0 = np.linalg.norm(r)
+    tol = max(abs_tol, rel_tol * r0)
+    if r0 <= tol:
+        return x, 0
+    z = inv_diag * r
+    p = z.copy()
+    rz = r @ z
+    for it in range(1, max_it + 1):
+        q = Ac @ p
+        alpha = rz / (p @ q)
+        x += alpha * p
+        r -= alpha * q
+        if np.linalg.norm(r) <= tol:
+            return x, it
+        z = inv_diag * r
+        rz_new = r @ z
+        p = z + (rz_new / rz) * p
+        rz = rz_new
+    return x, max_it

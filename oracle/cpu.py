@@ -1,0 +1,176 @@
+"""ctypes front end of the C++ oracle ``oracle/build/libheat_oracle.so`` (test infrastructure)."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "build", "libheat_oracle.so")
+_lib = None
+
+
+class OracleProblem(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int * 3),
+        ("degree", ctypes.c_int),
+        ("lo", ctypes.c_double * 3),
+        ("hi", ctypes.c_double * 3),
+        ("dt", ctypes.c_double),
+        ("material", ctypes.c_int),
+        ("gas_rho", ctypes.c_double),
+        ("gas_cp", ctypes.c_double),
+        ("gas_k", ctypes.c_double),
+        ("liq_rho", ctypes.c_double),
+        ("liq_cp", ctypes.c_double),
+        ("liq_k", ctypes.c_double),
+        ("sharp", ctypes.c_int),
+        ("volume_specific", ctypes.c_int),
+        ("consistent_with_evaporation", ctypes.c_int),
+        ("conductivity_field", ctypes.c_int),
+        ("k_amplitude", ctypes.c_double),
+        ("velocity", ctypes.c_int),
+        ("w0", ctypes.c_double),
+        ("dirichlet_faces", ctypes.c_uint),
+        ("heaviside", ctypes.POINTER(ctypes.c_double)),
+    ]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        P = ctypes.POINTER(OracleProblem)
+        L.oracle_tables.argtypes = [ctypes.c_int, dp, dp, dp, dp, dp]
+        L.oracle_n_dofs.argtypes = [P]
+        L.oracle_n_dofs.restype = ctypes.c_int64
+        L.oracle_constrained_mask.argtypes = [P, ctypes.POINTER(ctypes.c_ubyte)]
+        L.oracle_random_vector.argtypes = [P, ctypes.c_uint64, dp]
+        L.oracle_material.argtypes = [ctypes.c_int] * 4 + [ctypes.c_double] * 7 + [dp] * 4
+        L.oracle_vmult.argtypes = [P, dp, dp]
+        L.oracle_diagonal.argtypes = [P, dp]
+        L.oracle_inverse_diagonal.argtypes = [P, dp]
+        L.oracle_solve.argtypes = [P, ctypes.c_int, ctypes.c_int, dp, dp, dp, ctypes.c_double,
+                                   ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_int), dp,
+                                   dp, dp, ctypes.c_int]
+        L.oracle_set_threads.argtypes = [ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def set_threads(n):
+    lib().oracle_set_threads(int(n))
+
+
+def tables(p):
+    g = np.zeros(p + 1)
+    x = np.zeros(p + 1)
+    w = np.zeros(p + 1)
+    S = np.zeros((p + 1, p + 1))
+    D = np.zeros((p + 1, p + 1))
+    rc = lib().oracle_tables(p, _dp(g), _dp(x), _dp(w), _dp(S), _dp(D))
+    assert rc == 0
+    return g, x, w, S, D
+
+
+def material(heaviside, mat):
+    out = [ctypes.c_double() for _ in range(4)]
+    kind = 0 if mat.get("type", "gas") == "gas" else 1
+    lib().oracle_material(kind, int(mat.get("sharp", 0)), int(mat.get("volume_specific", 0)),
+                          int(mat.get("consistent_with_evaporation", 0)), mat["gas_rho"],
+                          mat["gas_cp"], mat["gas_k"], mat.get("liq_rho", 0.0),
+                          mat.get("liq_cp", 0.0), mat.get("liq_k", 0.0), heaviside,
+                          *[ctypes.byref(o) for o in out])
+    rho_cp, k, rho, cp = (o.value for o in out)
+    return {"rho_cp": rho_cp, "k": k, "rho": rho, "cp": cp}
+
+
+class Problem:
+    """Python mirror of ``oracle_problem``; keeps the heaviside array alive."""
+
+    def __init__(self, n, degree, dt=1e-3, material=None, k_amplitude=0.0, velocity=False,
+                 w0=0.2, heaviside=None, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0),
+                 dirichlet_faces=0x3F):
+        mat = material or {"type": "gas", "gas_rho": 1.0, "gas_cp": 1.0, "gas_k": 1.0}
+        n = (n, n, n) if np.isscalar(n) else tuple(n)
+        s = OracleProblem()
+        s.n[:] = [int(v) for v in n]
+        s.degree = degree
+        s.lo[:] = list(lo)
+        s.hi[:] = list(hi)
+        s.dt = dt
+        s.material = 0 if mat.get("type", "gas") == "gas" else 1
+        s.gas_rho, s.gas_cp, s.gas_k = mat["gas_rho"], mat["gas_cp"], mat["gas_k"]
+        s.liq_rho = mat.get("liq_rho", 0.0)
+        s.liq_cp = mat.get("liq_cp", 0.0)
+        s.liq_k = mat.get("liq_k", 0.0)
+        s.sharp = int(mat.get("sharp", 0))
+        s.volume_specific = int(mat.get("volume_specific", 0))
+        s.consistent_with_evaporation = int(mat.get("consistent_with_evaporation", 0))
+        s.conductivity_field = 1 if k_amplitude else 0
+        s.k_amplitude = k_amplitude
+        s.velocity = 1 if velocity else 0
+        s.w0 = w0
+        s.dirichlet_faces = dirichlet_faces
+        self._h = None
+        if heaviside is not None:
+            self._h = np.ascontiguousarray(heaviside, dtype=np.float64)
+            s.heaviside = _dp(self._h)
+        self.s = s
+        self.n = n
+        self.degree = degree
+        self.N = int(lib().oracle_n_dofs(ctypes.byref(s)))
+        assert self.N > 0, "invalid oracle problem"
+
+    def mask(self):
+        m = np.zeros(self.N, dtype=np.uint8)
+        lib().oracle_constrained_mask(ctypes.byref(self.s), m.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)))
+        return m.astype(bool)
+
+    def random(self, seed):
+        v = np.zeros(self.N)
+        lib().oracle_random_vector(ctypes.byref(self.s), seed, _dp(v))
+        return v
+
+    def vmult(self, src):
+        src = np.ascontiguousarray(src, dtype=np.float64)
+        dst = np.zeros(self.N)
+        assert lib().oracle_vmult(ctypes.byref(self.s), _dp(src), _dp(dst)) == 0
+        return dst
+
+    def diagonal(self):
+        d = np.zeros(self.N)
+        assert lib().oracle_diagonal(ctypes.byref(self.s), _dp(d)) == 0
+        return d
+
+    def inverse_diagonal(self):
+        d = np.zeros(self.N)
+        assert lib().oracle_inverse_diagonal(ctypes.byref(self.s), _dp(d)) == 0
+        return d
+
+    def solve(self, b, solver="CG", rel_tol=1e-8, abs_tol=1e-20, max_it=10000, inv_diag=None,
+              restart=30, x0=None, history=0):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros(self.N) if x0 is None else np.array(x0, dtype=np.float64)
+        it = ctypes.c_int()
+        r0 = ctypes.c_double()
+        rf = ctypes.c_double()
+        hist = np.zeros(max(history, 1))
+        dptr = _dp(inv_diag) if inv_diag is not None else None
+        status = lib().oracle_solve(ctypes.byref(self.s), 0 if solver == "CG" else 1, restart,
+                                    _dp(b), _dp(x), dptr, rel_tol, abs_tol, max_it,
+                                    ctypes.byref(it), ctypes.byref(r0), ctypes.byref(rf),
+                                    _dp(hist), history)
+        return {"x": x, "status": status, "iterations": it.value, "initial_res": r0.value,
+                "final_res": rf.value, "history": hist[: min(history, it.value + 1)]}

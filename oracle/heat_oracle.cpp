@@ -1,0 +1,786 @@
+// CPU oracle for the MeltPoolDG diffuse heat operator hot path.
+//
+// TEST INFRASTRUCTURE ONLY: the checker for the HIP path and the CPU baseline
+// timed beside it. The product library never links, loads or calls this file.
+// See heat_oracle.h for the list of reference functions restated here.
+//
+// The cell kernel is deliberately the textbook form: values and gradients are
+// interpolated straight from the nodal coefficients with the 1D value matrix S
+// and derivative matrix D (no collocation transform, no even-odd split), so it
+// is an independent code path from the GPU kernels.
+#include "heat_oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace
+{
+  constexpr long double PI_L = 3.141592653589793238462643383279502884L;
+
+  // Legendre P_n and P_n' on [-1,1] by the three-term recurrence.
+  void legendre(int n, long double x, long double &p, long double &dp)
+  {
+    long double p0 = 1.0L, p1 = x;
+    if (n == 0)
+      {
+        p = 1.0L;
+        dp = 0.0L;
+        return;
+      }
+    for (int k = 2; k <= n; ++k)
+      {
+        long double p2 = ((2.0L * k - 1.0L) * x * p1 - (k - 1.0L) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+      }
+    p = p1;
+    dp = n * (x * p1 - p0) / (x * x - 1.0L);
+  }
+
+  // Gauss-Legendre points/weights on [0,1], ascending (deal.II QGauss(n)).
+  void gauss_rule(int n, long double *x, long double *w)
+  {
+    for (int i = 0; i < n; ++i)
+      {
+        long double z = std::cos(PI_L * (i + 0.75L) / (n + 0.5L));
+        for (int it = 0; it < 100; ++it)
+          {
+            long double p, dp;
+            legendre(n, z, p, dp);
+            long double dz = p / dp;
+            z -= dz;
+            if (std::fabs(dz) < 1e-30L)
+              break;
+          }
+        long double p, dp;
+        legendre(n, z, p, dp);
+        // z runs from +1 down to -1 as i grows; store ascending on [0,1]
+        x[n - 1 - i] = 0.5L * (1.0L + z);
+        w[n - 1 - i] = 1.0L / ((1.0L - z * z) * dp * dp); // (2/((1-z^2)P'^2)) / 2
+      }
+  }
+
+  // Gauss-Lobatto points of degree p on [0,1]: 0, roots of P_p', 1.
+  void gll_nodes(int p, long double *x)
+  {
+    x[0] = 0.0L;
+    x[p] = 1.0L;
+    for (int i = 1; i < p; ++i)
+      {
+        long double z = -std::cos(PI_L * i / p);
+        for (int it = 0; it < 100; ++it)
+          {
+            long double P, dP;
+            legendre(p, z, P, dP);
+            // (1-z^2) P'' = 2 z P' - p(p+1) P
+            long double d2P = (2.0L * z * dP - p * (p + 1.0L) * P) / (1.0L - z * z);
+            long double dz = dP / d2P;
+            z -= dz;
+            if (std::fabs(dz) < 1e-30L)
+              break;
+          }
+        x[i] = 0.5L * (1.0L + z);
+      }
+  }
+
+  long double lagrange(int p, const long double *nodes, int i, long double x)
+  {
+    long double v = 1.0L;
+    for (int j = 0; j <= p; ++j)
+      if (j != i)
+        v *= (x - nodes[j]) / (nodes[i] - nodes[j]);
+    return v;
+  }
+
+  long double lagrange_deriv(int p, const long double *nodes, int i, long double x)
+  {
+    long double s = 0.0L;
+    for (int k = 0; k <= p; ++k)
+      {
+        if (k == i)
+          continue;
+        long double t = 1.0L / (nodes[i] - nodes[k]);
+        for (int j = 0; j <= p; ++j)
+          if (j != i && j != k)
+            t *= (x - nodes[j]) / (nodes[i] - nodes[j]);
+        s += t;
+      }
+    return s;
+  }
+
+  constexpr int MAXN = ORACLE_MAX_DEGREE + 1;
+
+  struct Tables
+  {
+    int p, n;
+    double S[MAXN][MAXN]; // S[q][i] = phi_i(x_q)
+    double D[MAXN][MAXN]; // D[q][i] = phi_i'(x_q)
+    double W[MAXN];       // Gauss weights on [0,1]
+    double X[MAXN];       // Gauss points on [0,1]
+  };
+
+  Tables make_tables(int p)
+  {
+    Tables t;
+    t.p = p;
+    t.n = p + 1;
+    long double nodes[MAXN], gx[MAXN], gw[MAXN];
+    gll_nodes(p, nodes);
+    gauss_rule(p + 1, gx, gw);
+    for (int q = 0; q <= p; ++q)
+      {
+        t.W[q] = (double)gw[q];
+        t.X[q] = (double)gx[q];
+        for (int i = 0; i <= p; ++i)
+          {
+            t.S[q][i] = (double)lagrange(p, nodes, i, gx[q]);
+            t.D[q][i] = (double)lagrange_deriv(p, nodes, i, gx[q]);
+          }
+      }
+    return t;
+  }
+
+  struct Mesh
+  {
+    int n[3];
+    int p;
+    int64_t N1[3];
+    int64_t N;
+    double h[3];
+    double lo[3];
+  };
+
+  Mesh make_mesh(const oracle_problem *pb)
+  {
+    Mesh m;
+    m.p = pb->degree;
+    m.N = 1;
+    for (int d = 0; d < 3; ++d)
+      {
+        m.n[d] = pb->n[d];
+        m.N1[d] = (int64_t)pb->n[d] * pb->degree + 1;
+        m.N *= m.N1[d];
+        m.h[d] = (pb->hi[d] - pb->lo[d]) / pb->n[d];
+        m.lo[d] = pb->lo[d];
+      }
+    return m;
+  }
+
+  bool valid(const oracle_problem *pb)
+  {
+    if (!pb || pb->degree < 1 || pb->degree > ORACLE_MAX_DEGREE)
+      return false;
+    for (int d = 0; d < 3; ++d)
+      if (pb->n[d] < 1 || !(pb->hi[d] > pb->lo[d]))
+        return false;
+    if (!(pb->dt != 0.0))
+      return false;
+    if (pb->material == 1 && !pb->heaviside)
+      return false;
+    return true;
+  }
+
+  inline bool is_constrained(const oracle_problem *pb, const Mesh &m, int64_t ix, int64_t iy,
+                             int64_t iz)
+  {
+    const unsigned f = pb->dirichlet_faces;
+    const int64_t c[3] = {ix, iy, iz};
+    for (int d = 0; d < 3; ++d)
+      {
+        if ((f >> (2 * d)) & 1u)
+          if (c[d] == 0)
+            return true;
+        if ((f >> (2 * d + 1)) & 1u)
+          if (c[d] == m.N1[d] - 1)
+            return true;
+      }
+    return false;
+  }
+
+  // Material::compute_parameters for gas / gas_liquid
+  // (include/meltpooldg/core/material.templates.hpp:223-295, 510-539;
+  //  LevelSet::Tools::interpolate include/meltpooldg/level_set/level_set_tools.hpp:40-43;
+  //  rho*cp combination: source/heat/heat_diffuse_operator.cpp:1167-1210).
+  inline void material_at(const oracle_problem *pb, double H, double &rho_cp, double &k)
+  {
+    if (pb->material == 0)
+      {
+        rho_cp = pb->gas_rho * pb->gas_cp;
+        k = pb->gas_k;
+        return;
+      }
+    const double w = pb->sharp ? (H > 0.5 ? 1.0 : 0.0) : H;
+    k = (1. - w) * pb->gas_k + w * pb->liq_k;
+    if (pb->volume_specific)
+      {
+        rho_cp = (1. - w) * (pb->gas_rho * pb->gas_cp) + w * (pb->liq_rho * pb->liq_cp);
+      }
+    else
+      {
+        const double cp = (1. - w) * pb->gas_cp + w * pb->liq_cp;
+        const double rho = pb->consistent_with_evaporation ?
+                             1. / ((1. - H) / pb->gas_rho + H / pb->liq_rho) :
+                             (1. - w) * pb->gas_rho + w * pb->liq_rho;
+        rho_cp = rho * cp;
+      }
+  }
+
+  // 1D contraction along one tensor direction of a local n^3 array
+  // (index (c*n+b)*n+a, a = x fastest). out = M in along `dir`, or M^T.
+  template <bool transpose>
+  void apply_dir(const double (*M)[MAXN], int n, int dir, const double *in, double *out)
+  {
+    const int stride = dir == 0 ? 1 : (dir == 1 ? n : n * n);
+    for (int c = 0; c < n; ++c)
+      for (int b = 0; b < n; ++b)
+        for (int a = 0; a < n; ++a)
+          {
+            const int idx[3] = {a, b, c};
+            const int o = idx[dir];
+            const int base = (c * n + b) * n + a - o * stride;
+            double s = 0.0;
+            for (int r = 0; r < n; ++r)
+              s += (transpose ? M[r][o] : M[o][r]) * in[base + r * stride];
+            out[(c * n + b) * n + a] = s;
+          }
+  }
+
+  struct CellGeom
+  {
+    int c[3];
+  };
+
+  // tangent_local_cell_operation for one cell (heat_diffuse_operator.cpp:955-1086):
+  // evaluate values|gradients, quadrature-point physics, integrate values|gradients.
+  void local_apply(const oracle_problem *pb, const Tables &t, const Mesh &m, const CellGeom &g,
+                   const double *u, const double *Hloc, double *y)
+  {
+    const int n = t.n;
+    const int n3 = n * n * n;
+    double tS[MAXN * MAXN * MAXN], tD[MAXN * MAXN * MAXN];
+    double tSS[MAXN * MAXN * MAXN], tSD[MAXN * MAXN * MAXN], tDS[MAXN * MAXN * MAXN];
+    double U[MAXN * MAXN * MAXN], Gx[MAXN * MAXN * MAXN], Gy[MAXN * MAXN * MAXN],
+      Gz[MAXN * MAXN * MAXN], Hq[MAXN * MAXN * MAXN];
+
+    // evaluate(values | gradients)
+    apply_dir<false>(t.S, n, 0, u, tS);
+    apply_dir<false>(t.D, n, 0, u, tD);
+    apply_dir<false>(t.S, n, 1, tS, tSS);
+    apply_dir<false>(t.D, n, 1, tS, tSD);
+    apply_dir<false>(t.S, n, 1, tD, tDS);
+    apply_dir<false>(t.S, n, 2, tSS, U);
+    apply_dir<false>(t.D, n, 2, tSS, Gz);
+    apply_dir<false>(t.S, n, 2, tSD, Gy);
+    apply_dir<false>(t.S, n, 2, tDS, Gx);
+
+    if (pb->material == 1)
+      {
+        // heaviside_eval.read_dof_values_plain + evaluate(values)
+        apply_dir<false>(t.S, n, 0, Hloc, tS);
+        apply_dir<false>(t.S, n, 1, tS, tSS);
+        apply_dir<false>(t.S, n, 2, tSS, Hq);
+      }
+
+    const double hx = m.h[0], hy = m.h[1], hz = m.h[2];
+    const double jac_det = hx * hy * hz;
+    const double dt_inv = 1. / pb->dt;
+    const double two_pi = 2.0 * (double)PI_L;
+    const double pi = (double)PI_L;
+
+    double V[MAXN * MAXN * MAXN], Fx[MAXN * MAXN * MAXN], Fy[MAXN * MAXN * MAXN],
+      Fz[MAXN * MAXN * MAXN];
+    for (int qz = 0; qz < n; ++qz)
+      for (int qy = 0; qy < n; ++qy)
+        for (int qx = 0; qx < n; ++qx)
+          {
+            const int q = (qz * n + qy) * n + qx;
+            const double xq = m.lo[0] + (g.c[0] + t.X[qx]) * hx;
+            const double yq = m.lo[1] + (g.c[1] + t.X[qy]) * hy;
+            const double zq = m.lo[2] + (g.c[2] + t.X[qz]) * hz;
+            const double JxW = t.W[qx] * t.W[qy] * t.W[qz] * jac_det;
+
+            double rho_cp, k;
+            material_at(pb, pb->material == 1 ? Hq[q] : 0.0, rho_cp, k);
+            if (pb->conductivity_field == 1)
+              k += pb->k_amplitude * std::sin(two_pi * xq) * std::sin(two_pi * yq) *
+                   std::sin(two_pi * zq);
+
+            const double gx = Gx[q] / hx, gy = Gy[q] / hy, gz = Gz[q] / hz;
+            double val = dt_inv * rho_cp * U[q];
+            if (pb->velocity == 1)
+              {
+                const double sx = std::sin(pi * xq), sy = std::sin(pi * yq);
+                const double vx = -sx * sx * std::sin(two_pi * yq);
+                const double vy = std::sin(two_pi * xq) * sy * sy;
+                const double vz = pb->w0;
+                val += rho_cp * (gx * vx + gy * vy + gz * vz);
+              }
+            V[q] = val * JxW;
+            Fx[q] = k * gx * JxW / hx;
+            Fy[q] = k * gy * JxW / hy;
+            Fz[q] = k * gz * JxW / hz;
+          }
+
+    // integrate(values | gradients)
+    double acc[MAXN * MAXN * MAXN];
+    double a1[MAXN * MAXN * MAXN], a2[MAXN * MAXN * MAXN];
+    // value part: S^T S^T S^T V
+    apply_dir<true>(t.S, n, 2, V, a1);
+    apply_dir<true>(t.S, n, 1, a1, a2);
+    apply_dir<true>(t.S, n, 0, a2, acc);
+    // x gradient: S^T(z) S^T(y) D^T(x)
+    apply_dir<true>(t.S, n, 2, Fx, a1);
+    apply_dir<true>(t.S, n, 1, a1, a2);
+    apply_dir<true>(t.D, n, 0, a2, a1);
+    for (int i = 0; i < n3; ++i)
+      acc[i] += a1[i];
+    apply_dir<true>(t.S, n, 2, Fy, a1);
+    apply_dir<true>(t.D, n, 1, a1, a2);
+    apply_dir<true>(t.S, n, 0, a2, a1);
+    for (int i = 0; i < n3; ++i)
+      acc[i] += a1[i];
+    apply_dir<true>(t.D, n, 2, Fz, a1);
+    apply_dir<true>(t.S, n, 1, a1, a2);
+    apply_dir<true>(t.S, n, 0, a2, a1);
+    for (int i = 0; i < n3; ++i)
+      y[i] = acc[i] + a1[i];
+  }
+
+  inline int64_t node_index(const Mesh &m, const CellGeom &g, int a, int b, int c)
+  {
+    const int64_t ix = (int64_t)g.c[0] * m.p + a;
+    const int64_t iy = (int64_t)g.c[1] * m.p + b;
+    const int64_t iz = (int64_t)g.c[2] * m.p + c;
+    return ix + m.N1[0] * (iy + m.N1[1] * iz);
+  }
+
+  template <typename Body>
+  void for_cells_coloured(const Mesh &m, Body body)
+  {
+    for (int colour = 0; colour < 8; ++colour)
+      {
+        const int ox = colour & 1, oy = (colour >> 1) & 1, oz = (colour >> 2) & 1;
+        const int mx = (m.n[0] - ox + 1) / 2, my = (m.n[1] - oy + 1) / 2,
+                  mz = (m.n[2] - oz + 1) / 2;
+        const int64_t total = (int64_t)mx * my * mz;
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < total; ++i)
+          {
+            CellGeom g;
+            g.c[0] = ox + 2 * (int)(i % mx);
+            g.c[1] = oy + 2 * (int)((i / mx) % my);
+            g.c[2] = oz + 2 * (int)(i / ((int64_t)mx * my));
+            body(g);
+          }
+      }
+  }
+
+  void vmult_impl(const oracle_problem *pb, const Tables &t, const Mesh &m, const double *src,
+                  double *dst)
+  {
+    const int n = t.n;
+    std::fill(dst, dst + m.N, 0.0);
+    for_cells_coloured(m, [&](const CellGeom &g) {
+      double u[MAXN * MAXN * MAXN], Hl[MAXN * MAXN * MAXN], y[MAXN * MAXN * MAXN];
+      for (int c = 0; c < n; ++c)
+        for (int b = 0; b < n; ++b)
+          for (int a = 0; a < n; ++a)
+            {
+              const int l = (c * n + b) * n + a;
+              const int64_t I = node_index(m, g, a, b, c);
+              const bool con = is_constrained(pb, m, (int64_t)g.c[0] * m.p + a,
+                                              (int64_t)g.c[1] * m.p + b, (int64_t)g.c[2] * m.p + c);
+              // read_dof_values: homogeneous constraints resolve to 0
+              u[l] = con ? 0.0 : src[I];
+              Hl[l] = pb->material == 1 ? pb->heaviside[I] : 0.0;
+            }
+      local_apply(pb, t, m, g, u, Hl, y);
+      for (int c = 0; c < n; ++c)
+        for (int b = 0; b < n; ++b)
+          for (int a = 0; a < n; ++a)
+            {
+              const bool con = is_constrained(pb, m, (int64_t)g.c[0] * m.p + a,
+                                              (int64_t)g.c[1] * m.p + b, (int64_t)g.c[2] * m.p + c);
+              // distribute_local_to_global drops constrained rows
+              if (!con)
+                dst[node_index(m, g, a, b, c)] += y[(c * n + b) * n + a];
+            }
+    });
+  }
+
+  double dot(const std::vector<double> &a, const std::vector<double> &b)
+  {
+    double s = 0.0;
+    const int64_t N = (int64_t)a.size();
+#pragma omp parallel for reduction(+ : s) schedule(static)
+    for (int64_t i = 0; i < N; ++i)
+      s += a[i] * b[i];
+    return s;
+  }
+
+  inline uint64_t splitmix64(uint64_t z)
+  {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  }
+} // namespace
+
+extern "C" {
+
+void oracle_set_threads(int n)
+{
+#ifdef _OPENMP
+  if (n > 0)
+    omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
+int oracle_tables(int p, double *gll, double *gauss_pts, double *gauss_w, double *S, double *D)
+{
+  if (p < 1 || p > ORACLE_MAX_DEGREE)
+    return -1;
+  long double nodes[MAXN], gx[MAXN], gw[MAXN];
+  gll_nodes(p, nodes);
+  gauss_rule(p + 1, gx, gw);
+  Tables t = make_tables(p);
+  for (int i = 0; i <= p; ++i)
+    {
+      if (gll)
+        gll[i] = (double)nodes[i];
+      if (gauss_pts)
+        gauss_pts[i] = (double)gx[i];
+      if (gauss_w)
+        gauss_w[i] = (double)gw[i];
+      for (int j = 0; j <= p; ++j)
+        {
+          if (S)
+            S[i * (p + 1) + j] = t.S[i][j];
+          if (D)
+            D[i * (p + 1) + j] = t.D[i][j];
+        }
+    }
+  return 0;
+}
+
+int64_t oracle_n_dofs(const oracle_problem *pb)
+{
+  if (!valid(pb))
+    return -1;
+  return make_mesh(pb).N;
+}
+
+int oracle_constrained_mask(const oracle_problem *pb, unsigned char *mask)
+{
+  if (!valid(pb))
+    return -1;
+  const Mesh m = make_mesh(pb);
+  for (int64_t iz = 0; iz < m.N1[2]; ++iz)
+    for (int64_t iy = 0; iy < m.N1[1]; ++iy)
+      for (int64_t ix = 0; ix < m.N1[0]; ++ix)
+        mask[ix + m.N1[0] * (iy + m.N1[1] * iz)] = is_constrained(pb, m, ix, iy, iz) ? 1 : 0;
+  return 0;
+}
+
+// Seeded uniform[-1,1] on free DoFs, 0 on constrained ones. Keyed by
+// (seed, lattice index) so it does not depend on thread order.
+void oracle_random_vector(const oracle_problem *pb, uint64_t seed, double *v)
+{
+  const Mesh m = make_mesh(pb);
+  const uint64_t key = splitmix64(seed);
+#pragma omp parallel for schedule(static)
+  for (int64_t iz = 0; iz < m.N1[2]; ++iz)
+    for (int64_t iy = 0; iy < m.N1[1]; ++iy)
+      for (int64_t ix = 0; ix < m.N1[0]; ++ix)
+        {
+          const int64_t I = ix + m.N1[0] * (iy + m.N1[1] * iz);
+          const uint64_t r = splitmix64(key ^ (uint64_t)I);
+          const double u = (double)(r >> 11) * 0x1.0p-53;
+          v[I] = is_constrained(pb, m, ix, iy, iz) ? 0.0 : 2.0 * u - 1.0;
+        }
+}
+
+int oracle_material(int material, int sharp, int volume_specific, int consistent_with_evaporation,
+                    double gas_rho, double gas_cp, double gas_k, double liq_rho, double liq_cp,
+                    double liq_k, double heaviside, double *rho_cp, double *k, double *rho,
+                    double *cp)
+{
+  oracle_problem pb;
+  std::memset(&pb, 0, sizeof(pb));
+  pb.material = material;
+  pb.sharp = sharp;
+  pb.volume_specific = volume_specific;
+  pb.consistent_with_evaporation = consistent_with_evaporation;
+  pb.gas_rho = gas_rho;
+  pb.gas_cp = gas_cp;
+  pb.gas_k = gas_k;
+  pb.liq_rho = liq_rho;
+  pb.liq_cp = liq_cp;
+  pb.liq_k = liq_k;
+  material_at(&pb, heaviside, *rho_cp, *k);
+  if (material == 0)
+    {
+      *rho = gas_rho;
+      *cp = gas_cp;
+    }
+  else
+    {
+      const double w = sharp ? (heaviside > 0.5 ? 1.0 : 0.0) : heaviside;
+      *cp = (1. - w) * gas_cp + w * liq_cp;
+      *rho = consistent_with_evaporation ? 1. / ((1. - heaviside) / gas_rho + heaviside / liq_rho) :
+                                           (1. - w) * gas_rho + w * liq_rho;
+    }
+  return 0;
+}
+
+int oracle_vmult(const oracle_problem *pb, const double *src, double *dst)
+{
+  if (!valid(pb))
+    return -1;
+  const Tables t = make_tables(pb->degree);
+  const Mesh m = make_mesh(pb);
+  vmult_impl(pb, t, m, src, dst);
+  return 0;
+}
+
+// MatrixFreeTools::compute_diagonal restated: apply the cell operator to each
+// local unit vector and keep the matching entry (heat_diffuse_operator.cpp:332-358).
+int oracle_diagonal(const oracle_problem *pb, double *diag)
+{
+  if (!valid(pb))
+    return -1;
+  const Tables t = make_tables(pb->degree);
+  const Mesh m = make_mesh(pb);
+  const int n = t.n;
+  const int n3 = n * n * n;
+  std::fill(diag, diag + m.N, 0.0);
+  for_cells_coloured(m, [&](const CellGeom &g) {
+    double u[MAXN * MAXN * MAXN], Hl[MAXN * MAXN * MAXN], y[MAXN * MAXN * MAXN];
+    double d[MAXN * MAXN * MAXN];
+    for (int c = 0; c < n; ++c)
+      for (int b = 0; b < n; ++b)
+        for (int a = 0; a < n; ++a)
+          Hl[(c * n + b) * n + a] = pb->material == 1 ? pb->heaviside[node_index(m, g, a, b, c)] : 0.0;
+    for (int j = 0; j < n3; ++j)
+      {
+        std::fill(u, u + n3, 0.0);
+        u[j] = 1.0;
+        local_apply(pb, t, m, g, u, Hl, y);
+        d[j] = y[j];
+      }
+    for (int c = 0; c < n; ++c)
+      for (int b = 0; b < n; ++b)
+        for (int a = 0; a < n; ++a)
+          {
+            const bool con = is_constrained(pb, m, (int64_t)g.c[0] * m.p + a,
+                                            (int64_t)g.c[1] * m.p + b, (int64_t)g.c[2] * m.p + c);
+            if (!con)
+              diag[node_index(m, g, a, b, c)] += d[(c * n + b) * n + a];
+          }
+  });
+  return 0;
+}
+
+int oracle_inverse_diagonal(const oracle_problem *pb, double *inv_diag)
+{
+  const int rc = oracle_diagonal(pb, inv_diag);
+  if (rc)
+    return rc;
+  const int64_t N = make_mesh(pb).N;
+  // heat_diffuse_operator.cpp:282-283
+  for (int64_t i = 0; i < N; ++i)
+    inv_diag[i] = std::abs(inv_diag[i]) > 1.0e-10 ? 1.0 / inv_diag[i] : 1.0;
+  return 0;
+}
+
+// LinearSolver::solve restated (linear_solver.hpp:26-134): deal.II SolverCG /
+// right-preconditioned SolverGMRES under ReductionControl(max_it, abs, rel),
+// Jacobi preconditioner dst = inv_diag .* src (preconditioner_jacobi.hpp:51-56).
+// Return 0 on convergence, 1 on NoConvergence, -1 on bad input.
+int oracle_solve(const oracle_problem *pb, int solver, int restart, const double *b, double *x,
+                 const double *inv_diag, double rel_tol, double abs_tol, int max_it,
+                 int *iterations, double *initial_res, double *final_res, double *history,
+                 int history_len)
+{
+  if (!valid(pb) || (solver == 1 && restart < 1))
+    return -1;
+  const Tables t = make_tables(pb->degree);
+  const Mesh m = make_mesh(pb);
+  const int64_t N = m.N;
+  std::vector<double> X(x, x + N), B(b, b + N), D(N, 1.0);
+  if (inv_diag)
+    D.assign(inv_diag, inv_diag + N);
+  std::vector<double> r(N), tmp(N);
+  auto record = [&](int it, double res) {
+    if (history && it < history_len)
+      history[it] = res;
+  };
+
+  vmult_impl(pb, t, m, X.data(), tmp.data());
+  for (int64_t i = 0; i < N; ++i)
+    r[i] = B[i] - tmp[i];
+  double res = std::sqrt(dot(r, r));
+  *initial_res = res;
+  const double tol = std::max(abs_tol, rel_tol * res);
+  record(0, res);
+  int it = 0;
+  int status = 1;
+  if (res <= tol)
+    status = 0;
+
+  if (status != 0 && solver == 0)
+    {
+      std::vector<double> z(N), p(N), q(N);
+      double rz = 0.0;
+      while (true)
+        {
+          // SolverCG IterationWorker::do_iteration (deal.II, restated)
+          ++it;
+          for (int64_t i = 0; i < N; ++i)
+            z[i] = D[i] * r[i];
+          const double rz_old = rz;
+          rz = dot(r, z);
+          if (it > 1)
+            {
+              const double beta = rz / rz_old;
+              for (int64_t i = 0; i < N; ++i)
+                p[i] = z[i] + beta * p[i];
+            }
+          else
+            p = z;
+          vmult_impl(pb, t, m, p.data(), q.data());
+          const double alpha = rz / dot(p, q);
+          for (int64_t i = 0; i < N; ++i)
+            {
+              X[i] += alpha * p[i];
+              r[i] -= alpha * q[i];
+            }
+          res = std::sqrt(std::abs(dot(r, r)));
+          record(it, res);
+          if (res <= tol)
+            {
+              status = 0;
+              break;
+            }
+          if (it >= max_it || std::isnan(res))
+            break;
+        }
+    }
+  else if (status != 0 && solver == 1)
+    {
+      // GMRES(restart), right preconditioning, classical Gram-Schmidt with one
+      // re-orthogonalisation pass; the inner check uses the Givens residual estimate.
+      const int mr = restart;
+      std::vector<std::vector<double>> V(mr + 1, std::vector<double>(N));
+      std::vector<double> H((size_t)(mr + 1) * mr), cs(mr), sn(mr), gv(mr + 1), w(N), zz(N),
+        hcol(mr + 1);
+      bool done = false;
+      while (!done)
+        {
+          const double beta = std::sqrt(dot(r, r));
+          for (int64_t i = 0; i < N; ++i)
+            V[0][i] = r[i] / beta;
+          std::fill(gv.begin(), gv.end(), 0.0);
+          gv[0] = beta;
+          int j = 0;
+          for (; j < mr; ++j)
+            {
+              ++it;
+              for (int64_t i = 0; i < N; ++i)
+                zz[i] = D[i] * V[j][i];
+              vmult_impl(pb, t, m, zz.data(), w.data());
+              std::fill(hcol.begin(), hcol.end(), 0.0);
+              for (int pass = 0; pass < 2; ++pass)
+                {
+                  std::vector<double> hh(j + 1);
+                  for (int l = 0; l <= j; ++l)
+                    hh[l] = dot(V[l], w);
+                  for (int l = 0; l <= j; ++l)
+                    {
+                      for (int64_t i = 0; i < N; ++i)
+                        w[i] -= hh[l] * V[l][i];
+                      hcol[l] += hh[l];
+                    }
+                }
+              const double hn = std::sqrt(dot(w, w));
+              hcol[j + 1] = hn;
+              if (hn > 0)
+                for (int64_t i = 0; i < N; ++i)
+                  V[j + 1][i] = w[i] / hn;
+              for (int l = 0; l < j; ++l)
+                {
+                  const double a = cs[l] * hcol[l] + sn[l] * hcol[l + 1];
+                  const double bb = -sn[l] * hcol[l] + cs[l] * hcol[l + 1];
+                  hcol[l] = a;
+                  hcol[l + 1] = bb;
+                }
+              const double den = std::hypot(hcol[j], hcol[j + 1]);
+              cs[j] = hcol[j] / den;
+              sn[j] = hcol[j + 1] / den;
+              hcol[j] = den;
+              hcol[j + 1] = 0.0;
+              gv[j + 1] = -sn[j] * gv[j];
+              gv[j] = cs[j] * gv[j];
+              for (int l = 0; l <= j; ++l)
+                H[(size_t)l * mr + j] = hcol[l];
+              res = std::abs(gv[j + 1]);
+              record(it, res);
+              if (res <= tol || it >= max_it || std::isnan(res))
+                {
+                  ++j;
+                  break;
+                }
+            }
+          // solve the upper triangular system and update x += P V y
+          std::vector<double> yv(j);
+          for (int l = j - 1; l >= 0; --l)
+            {
+              double s = gv[l];
+              for (int c = l + 1; c < j; ++c)
+                s -= H[(size_t)l * mr + c] * yv[c];
+              yv[l] = s / H[(size_t)l * mr + l];
+            }
+          std::fill(zz.begin(), zz.end(), 0.0);
+          for (int l = 0; l < j; ++l)
+            for (int64_t i = 0; i < N; ++i)
+              zz[i] += yv[l] * V[l][i];
+          for (int64_t i = 0; i < N; ++i)
+            X[i] += D[i] * zz[i];
+          if (res <= tol)
+            {
+              status = 0;
+              done = true;
+            }
+          else if (it >= max_it || std::isnan(res))
+            done = true;
+          else
+            {
+              vmult_impl(pb, t, m, X.data(), tmp.data());
+              for (int64_t i = 0; i < N; ++i)
+                r[i] = B[i] - tmp[i];
+              res = std::sqrt(dot(r, r));
+              if (res <= tol)
+                {
+                  status = 0;
+                  done = true;
+                }
+            }
+        }
+    }
+  std::copy(X.begin(), X.end(), x);
+  *iterations = it;
+  *final_res = res;
+  return status;
+}
+
+} // extern "C"

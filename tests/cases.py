@@ -1,0 +1,71 @@
+"""Shared problem definitions of the tests: golden fixtures and the
+oracle / engine objects built from the same parameters."""
+import glob
+import json
+import os
+
+import numpy as np
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def golden_cases():
+    out = []
+    for path in sorted(glob.glob(os.path.join(GOLDEN, "heat_*.npz"))):
+        z = np.load(path, allow_pickle=False)
+        case = {k: z[k] for k in z.files}
+        case["name"] = os.path.basename(path)[5:-4]
+        case["material"] = json.loads(str(case["material"]))
+        case["n"] = tuple(int(v) for v in case["n"])
+        case["degree"] = int(case["degree"])
+        case["dt"] = float(case["dt"])
+        case["k_amplitude"] = float(case["k_amplitude"])
+        case["velocity"] = bool(int(case["velocity"]))
+        case["faces"] = int(case["faces"])
+        if case["heaviside"].size == 0:
+            case["heaviside"] = None
+        out.append(case)
+    return out
+
+
+def oracle_problem(case):
+    from oracle import cpu
+    return cpu.Problem(case["n"], case["degree"], dt=case["dt"], material=case["material"],
+                       k_amplitude=case["k_amplitude"], velocity=case["velocity"],
+                       heaviside=case["heaviside"], dirichlet_faces=case["faces"])
+
+
+def engine_operator(case, precision="fp64"):
+    """HeatDiffuseOperator with the same parameters (needs a GPU)."""
+    import torch
+    from meltpooldg_amd import HeatDiffuseOperator, MaterialData, PhaseData
+    m = case["material"]
+    mat = MaterialData(
+        type=m.get("type", "gas"),
+        gas=PhaseData(m["gas_rho"], m["gas_cp"], m["gas_k"]),
+        liquid=PhaseData(m.get("liq_rho", 1.0), m.get("liq_cp", 1.0), m.get("liq_k", 1.0)),
+        two_phase_fluid_properties_transition_type="sharp" if m.get("sharp") else "smooth",
+        use_volume_specific_thermal_capacity_for_phase_interpolation=bool(m.get("volume_specific")))
+    op = HeatDiffuseOperator(case["n"], case["degree"], material=mat, precision=precision,
+                             conductivity_amplitude=case["k_amplitude"],
+                             velocity="vortex" if case["velocity"] else None,
+                             dirichlet_faces=case["faces"])
+    op.reset_time_increment(case["dt"])
+    if case["heaviside"] is not None:
+        op.set_heaviside(torch.tensor(case["heaviside"], dtype=torch.float64, device="cuda"))
+    return op
+
+
+def make_case(n, degree, dt=1e-3, material=None, k_amplitude=0.0, velocity=False, faces=0x3F,
+              heaviside=None):
+    n = (n,) * 3 if isinstance(n, int) else tuple(n)
+    return {"name": f"n{n}_p{degree}", "n": n, "degree": degree, "dt": dt,
+            "material": material or {"type": "gas", "gas_rho": 1.0, "gas_cp": 1.0, "gas_k": 1.0},
+            "k_amplitude": k_amplitude, "velocity": velocity, "faces": faces,
+            "heaviside": heaviside}
+
+
+def rel_err(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))

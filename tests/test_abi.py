@@ -1,0 +1,86 @@
+"""CPU checks of the C-ABI library: it loads, exports every entry point the
+header declares, and rejects bad arguments before touching the GPU. Also
+checks that the product package never reaches the oracle."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "mpdg_heat.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(mpdg_[a-z0-9_]+)\s*\(", text)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from meltpooldg_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        from meltpooldg_amd import build
+        build.build()
+    return _lib.load()
+
+
+def test_exports_every_declared_symbol(lib):
+    from meltpooldg_amd import _lib
+    names = header_functions()
+    assert len(names) >= 14
+    for name in names:
+        assert hasattr(lib, name), name
+        assert name in _lib.SIGNATURES, f"{name} missing from the ctypes binding"
+    assert set(_lib.SIGNATURES) == set(names)
+
+
+def test_argument_errors_without_gpu(lib):
+    from meltpooldg_amd import _lib as L
+    h = ctypes.c_void_p()
+    assert lib.mpdg_heat_create(None, ctypes.byref(h)) == L.MPDG_ERR_ARGUMENT
+    assert lib.mpdg_last_error()
+    d = L.HeatDesc()
+    d.n_cells[:] = [4, 4, 4]
+    d.box_hi[:] = [1.0, 1.0, 1.0]
+    d.degree = 9
+    assert lib.mpdg_heat_create(ctypes.byref(d), ctypes.byref(h)) == L.MPDG_ERR_UNSUPPORTED
+    d.degree = 2
+    d.n_cells[1] = 0
+    assert lib.mpdg_heat_create(ctypes.byref(d), ctypes.byref(h)) == L.MPDG_ERR_ARGUMENT
+    d.n_cells[1] = 4
+    d.box_hi[2] = 0.0
+    assert lib.mpdg_heat_create(ctypes.byref(d), ctypes.byref(h)) == L.MPDG_ERR_ARGUMENT
+    d.box_hi[2] = 1.0
+    d.material = 7
+    assert lib.mpdg_heat_create(ctypes.byref(d), ctypes.byref(h)) == L.MPDG_ERR_UNSUPPORTED
+    assert lib.mpdg_heat_set_dt(None, 1.0) == L.MPDG_ERR_ARGUMENT
+    assert lib.mpdg_heat_vmult(None, None, None) == L.MPDG_ERR_ARGUMENT
+    assert lib.mpdg_heat_n_dofs(None) == -1
+
+
+def test_solver_defaults_follow_linear_solver_data(lib):
+    from meltpooldg_amd import _lib as L
+    p = L.SolverParams()
+    lib.mpdg_solver_params_default(ctypes.byref(p))
+    # LinearSolverData defaults (linear_solver_data.hpp:37-51), CG + Diagonal
+    assert p.max_iterations == 10000
+    assert p.rel_tolerance == 1e-12
+    assert p.abs_tolerance == 1e-20
+    assert p.solver_type == 0 and p.preconditioner_type == 3
+
+
+def test_python_mirror_defaults():
+    from meltpooldg_amd import LinearSolverData
+    d = LinearSolverData()
+    assert d.solver_type == "GMRES" and d.preconditioner_type == "Identity"
+    assert d.max_iterations == 10000 and d.rel_tolerance == 1e-12 and d.abs_tolerance == 1e-20
+
+
+def test_product_never_imports_oracle():
+    pkg = os.path.join(ROOT, "meltpooldg_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith((".py", ".cpp", ".hip", ".hpp", ".h")):
+                text = open(os.path.join(dirpath, f)).read()
+                assert "oracle" not in text.lower(), os.path.join(dirpath, f)

@@ -1,0 +1,232 @@
+"""GPU parity of the HIP path (through the C ABI) against the oracle.
+
+Tolerances: FP64 vmult / inverse diagonal within 1e-12 relative (max-norm,
+scaled by the max entry) of the FP64 oracle; FP32 cell loop within 2e-5
+relative; CG iteration counts equal to the oracle's restatement of deal.II
+SolverCG; mixed-precision solutions within the stated tolerance of the FP64
+oracle solution.
+"""
+import numpy as np
+import pytest
+
+from tests.cases import engine_operator, golden_cases, make_case, oracle_problem, rel_err
+
+pytestmark = pytest.mark.gpu
+
+TWO = {"type": "gas_liquid", "gas_rho": 1.0, "gas_cp": 1.0, "gas_k": 1.0,
+       "liq_rho": 10.0, "liq_cp": 1.0, "liq_k": 100.0}
+
+
+def _t(a, dtype=None):
+    import torch
+    return torch.tensor(a, dtype=dtype or torch.float64, device="cuda")
+
+
+def _np(t):
+    import torch
+    torch.cuda.synchronize()
+    return t.double().cpu().numpy()
+
+
+def _vmult(op, src_np, fp32=False):
+    import torch
+    dt = torch.float32 if fp32 else torch.float64
+    src = _t(src_np, dt)
+    dst = torch.full_like(src, float("nan"))  # the engine must write every entry
+    op.vmult(dst, src)
+    return _np(dst)
+
+
+def _invdiag(op):
+    import torch
+    d = torch.full((op.n_dofs(),), float("nan"), dtype=torch.float64, device="cuda")
+    op.compute_inverse_diagonal_from_matrixfree(d)
+    return _np(d)
+
+
+@pytest.mark.parametrize("case", golden_cases(), ids=lambda c: c["name"])
+def test_golden_vmult_and_diagonal(case):
+    print("golden", case["name"], flush=True)
+    op = engine_operator(case)
+    assert op.n_dofs() == case["src"].size
+    assert rel_err(_vmult(op, case["src"]), case["dst"]) < 1e-12
+    inv = np.where(case["mask"], 1.0, 1.0 / np.where(case["mask"], 1.0, case["diag"]))
+    assert rel_err(_invdiag(op), inv) < 1e-12
+
+
+CASES = []
+for p in range(1, 7):
+    CASES.append(make_case((3, 4, 5), p))
+    CASES.append(make_case((5, 3, 2), p, k_amplitude=0.5, velocity=True, dt=5e-4))
+CASES += [
+    make_case((37, 3, 2), 2),                  # ragged: several x segments, last one partial
+    make_case((19, 2, 3), 4, k_amplitude=0.5),
+    make_case((1, 1, 1), 3),
+    make_case((2, 1, 1), 1, faces=0),
+    make_case((4, 5, 3), 3, faces=0b100101, velocity=True),
+    make_case((9, 2, 2), 5, faces=0b011010, k_amplitude=0.25),
+    make_case((10, 3, 3), 6, velocity=True),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: f"{c['name']}_k{c['k_amplitude']}_v{int(c['velocity'])}_f{c['faces']}")
+def test_vmult_diag_vs_oracle(case):
+    print("case", case["name"], case["k_amplitude"], case["velocity"], case["faces"], flush=True)
+    pb = oracle_problem(case)
+    op = engine_operator(case)
+    u = pb.random(0)
+    assert rel_err(_vmult(op, u), pb.vmult(u)) < 1e-12
+    assert rel_err(_invdiag(op), pb.inverse_diagonal()) < 1e-12
+
+
+@pytest.mark.parametrize("p", [1, 2, 3, 4, 5, 6])
+@pytest.mark.parametrize("mat", ["smooth", "sharp", "volspec"])
+def test_two_phase_vs_oracle(p, mat):
+    m = dict(TWO)
+    if mat == "sharp":
+        m["sharp"] = 1
+    if mat == "volspec":
+        m.update(volume_specific=1, liq_cp=3.0)
+    n = (4, 3, 3)
+    from oracle.rng import random_vector
+    H = 0.5 * (random_vector(n, p, 11, dirichlet_faces=0) + 1.0)
+    for vel in (False, True):
+        case = make_case(n, p, material=m, heaviside=H, velocity=vel)
+        print("two-phase", p, mat, vel, flush=True)
+        pb = oracle_problem(case)
+        op = engine_operator(case)
+        u = pb.random(0)
+        assert rel_err(_vmult(op, u), pb.vmult(u)) < 1e-12
+        assert rel_err(_invdiag(op), pb.inverse_diagonal()) < 1e-12
+
+
+@pytest.mark.parametrize("p", [1, 2, 3, 4, 5, 6])
+def test_fp32_cell_loop(p):
+    case = make_case((6, 5, 4), p, k_amplitude=0.5, velocity=True, dt=5e-4)
+    print("fp32", p, flush=True)
+    pb = oracle_problem(case)
+    op = engine_operator(case, precision="fp32")
+    u = pb.random(0)
+    ref = pb.vmult(u)
+    out = _vmult(op, u.astype(np.float32), fp32=True)
+    assert rel_err(out, ref) < 2e-5
+
+
+def test_deterministic_bitwise():
+    import torch
+    case = make_case((24, 20, 18), 4, k_amplitude=0.5)
+    op = engine_operator(case)
+    pb = oracle_problem(case)
+    src = _t(pb.random(0))
+    a = torch.empty_like(src)
+    b = torch.empty_like(src)
+    op.vmult(a, src)
+    for _ in range(3):
+        op.vmult(b, src)
+        torch.cuda.synchronize()
+        assert torch.equal(a, b)
+
+
+def test_cg_config0_iterations_match_oracle():
+    """16^3 Q2, rho_cp = k = 1, dt = 1e-3, CG + Jacobi to 1e-10."""
+    import torch
+    from meltpooldg_amd import JacobiPreconditioner, LinearSolver, LinearSolverData
+    case = make_case(16, 2)
+    pb = oracle_problem(case)
+    b = pb.random(1)
+    ref = pb.solve(b, rel_tol=1e-10, inv_diag=pb.inverse_diagonal())
+    op = engine_operator(case)
+    prec = JacobiPreconditioner(op)
+    prec.update()
+    x = torch.zeros(op.n_dofs(), dtype=torch.float64, device="cuda")
+    data = LinearSolverData(solver_type="CG", preconditioner_type="Diagonal", rel_tolerance=1e-10)
+    it = LinearSolver.solve(op, x, _t(b), data, prec)
+    print("iterations", it, "oracle", ref["iterations"], flush=True)
+    assert it == ref["iterations"]
+    assert rel_err(_np(x), ref["x"]) < 1e-9
+    rep = LinearSolver.last_report
+    assert rep["final_residual"] <= 1e-10 * rep["initial_residual"]
+
+
+def test_cg_nonzero_initial_guess_and_variable_k():
+    import torch
+    from meltpooldg_amd import LinearSolver, LinearSolverData
+    case = make_case((10, 9, 8), 3, k_amplitude=0.5)
+    pb = oracle_problem(case)
+    b = pb.random(1)
+    x0 = 0.1 * pb.random(2)
+    ref = pb.solve(b, rel_tol=1e-9, inv_diag=pb.inverse_diagonal(), x0=x0)
+    op = engine_operator(case)
+    x = _t(x0)
+    data = LinearSolverData(solver_type="CG", preconditioner_type="Diagonal", rel_tolerance=1e-9)
+    it = LinearSolver.solve(op, x, _t(b), data)
+    assert it == ref["iterations"]
+    assert rel_err(_np(x), ref["x"]) < 1e-8
+
+
+def test_mixed_precision_defect_correction():
+    import torch
+    from meltpooldg_amd import LinearSolver, LinearSolverData
+    from oracle.rng import random_vector
+    n, p = 12, 3
+    H = 0.5 * (random_vector(n, p, 3, dirichlet_faces=0) + 1.0)
+    case = make_case(n, p, material=TWO, heaviside=H)
+    pb = oracle_problem(case)
+    b = pb.random(1)
+    ref = pb.solve(b, rel_tol=1e-12, inv_diag=pb.inverse_diagonal())
+    op = engine_operator(case)
+    x = torch.zeros(op.n_dofs(), dtype=torch.float64, device="cuda")
+    data = LinearSolverData(solver_type="CG", preconditioner_type="Diagonal", rel_tolerance=1e-8,
+                            mixed_precision=True)
+    it = LinearSolver.solve(op, x, _t(b), data)
+    rep = LinearSolver.last_report
+    print("mixed", rep, "fp64 oracle iterations", ref["iterations"], flush=True)
+    assert rep["final_residual"] <= 1e-8 * rep["initial_residual"]
+    # error against the (converged) FP64 oracle solution, relative max-norm
+    assert rel_err(_np(x), ref["x"]) < 1e-6
+
+
+def test_no_convergence_and_zero_dt():
+    import torch
+    from meltpooldg_amd import (HeatDiffuseOperator, LinearSolver, LinearSolverData,
+                                NoConvergence, ZeroTimeIncrement)
+    case = make_case(6, 2, dt=1.0)
+    op = engine_operator(case)
+    pb = oracle_problem(case)
+    x = torch.zeros(op.n_dofs(), dtype=torch.float64, device="cuda")
+    data = LinearSolverData(solver_type="CG", preconditioner_type="Diagonal", rel_tolerance=1e-14,
+                            max_iterations=3)
+    with pytest.raises(NoConvergence) as e:
+        LinearSolver.solve(op, x, _t(pb.random(1)), data)
+    assert e.value.last_step == 3
+    op2 = HeatDiffuseOperator(4, 1)
+    with pytest.raises(ZeroTimeIncrement):
+        op2.reset_time_increment(0.0)
+    with pytest.raises(Exception):
+        op2.vmult(x[:10], x[:10])
+
+
+def test_config1_full_size_vs_oracle():
+    """64^3 Q4, k = 1 + 0.5 s(x)s(y)s(z) at full size: one vmult
+    against the oracle, plus symmetry and linearity of the operator."""
+    import torch
+    from oracle import cpu
+    case = make_case(64, 4, k_amplitude=0.5)
+    pb = oracle_problem(case)
+    cpu.set_threads(16)
+    u = pb.random(0)
+    ref = pb.vmult(u)
+    op = engine_operator(case)
+    ut = _t(u)
+    out = torch.empty_like(ut)
+    op.vmult(out, ut)
+    assert rel_err(_np(out), ref) < 1e-12
+    v = _t(pb.random(5))
+    Av = torch.empty_like(v)
+    op.vmult(Av, v)
+    uAv = float(torch.dot(ut, Av))
+    vAu = float(torch.dot(v, out))
+    assert abs(uAv - vAu) < 1e-11 * abs(uAv)
+    w = torch.empty_like(v)
+    op.vmult(w, 2.0 * ut - 3.0 * v)
+    assert float((w - (2.0 * out - 3.0 * Av)).abs().max()) < 1e-12 * float(out.abs().max())
