@@ -197,7 +197,13 @@ namespace
     MPDG_HIP(H->p.alloc(H->N));
     MPDG_HIP(H->q.alloc(H->N));
     MPDG_HIP(H->inv_diag.alloc(H->N));
-    MPDG_HIP(H->scal.alloc(1));
+    if (!H->scal.ptr)
+      {
+        // the reduction ticket counter must start at 0; the kernels reset it after each use
+        MPDG_HIP(H->scal.alloc(1));
+        const SolverScalars zero{};
+        MPDG_HIP(hipMemcpy(H->scal.ptr, &zero, sizeof(zero), hipMemcpyHostToDevice));
+      }
     if (!H->host_scal)
       MPDG_HIP(hipHostMalloc((void **)&H->host_scal, sizeof(SolverScalars), hipHostMallocDefault));
     if (mixed)
