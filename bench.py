@@ -61,6 +61,27 @@ def cpu_baseline(sample_vmults=3, threads=None):
                       f"reference cell loop, OpenMP {threads} threads), {t*1e3:.1f} ms each"}
 
 
+def copy_calibration(args, torch, dev):
+    """Plain copy of one FP64 DoF vector of the workload (read N*8 B, write N*8 B):
+    the known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters."""
+    N = algorithmic_counts(N_CELLS, DEGREE)["N"]
+    src = torch.rand(N, dtype=torch.float64, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(args.warmup):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(args.steps):
+        dst.copy_(src)
+    ev1.record()
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / args.steps
+    print(json.dumps({"mode": "copy", "bytes_read": 8 * N, "bytes_written": 8 * N, "ms": ms,
+                      "GBs": 16 * N / (ms * 1e-3) / 1e9}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -68,6 +89,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-solve", action="store_true", help="skip the CG time-to-solution legs")
+    ap.add_argument("--mode", choices=["vmult", "copy"], default="vmult",
+                    help="copy: time a plain device copy of one DoF vector (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU design is "
@@ -80,6 +103,8 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.mode == "copy":
+        return copy_calibration(args, torch, dev)
     op = HeatDiffuseOperator(N_CELLS, DEGREE, conductivity_amplitude=K_AMP)
     op.reset_time_increment(DT)
     cnt = algorithmic_counts(N_CELLS, DEGREE)

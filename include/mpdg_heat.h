@@ -146,6 +146,16 @@ void mpdg_solver_params_default(mpdg_solver_params *p);
 /* Thread-local description of the last error on this thread. */
 const char *mpdg_last_error(void);
 
+/* Launch tuning of the cell kernels (no counterpart in the reference, whose
+ * cell loop is tuned by deal.II's MatrixFree::AdditionalData). Used by the
+ * autotuner; results do not depend on it beyond summation order. */
+typedef struct {
+  int z_layers;       /* cell layers swept per vmult work item (>= 1) */
+  int blocks_per_cu;  /* persistent workgroups per CU, 0 = occupancy limit */
+} mpdg_tuning;
+mpdg_status mpdg_heat_get_tuning(const mpdg_heat *h, mpdg_tuning *t);
+mpdg_status mpdg_heat_set_tuning(mpdg_heat *h, const mpdg_tuning *t);
+
 /* Kernel timing hook for measurement: average device time of the last
  * n_launches vmult launches (ms), recorded with HIP events on the handle's stream. */
 mpdg_status mpdg_heat_time_vmult(mpdg_heat *h, double *dst, const double *src, int repeats,

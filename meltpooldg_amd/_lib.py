@@ -70,6 +70,10 @@ class SolverReport(ctypes.Structure):
     ]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("z_layers", ctypes.c_int), ("blocks_per_cu", ctypes.c_int)]
+
+
 # name -> (restype, argtypes); exactly the entry points include/mpdg_heat.h declares
 _VP = ctypes.c_void_p
 SIGNATURES = {
@@ -87,6 +91,8 @@ SIGNATURES = {
                                        ctypes.POINTER(SolverReport)]),
     "mpdg_solver_params_default": (None, [ctypes.POINTER(SolverParams)]),
     "mpdg_last_error": (ctypes.c_char_p, []),
+    "mpdg_heat_get_tuning": (ctypes.c_int, [_VP, ctypes.POINTER(Tuning)]),
+    "mpdg_heat_set_tuning": (ctypes.c_int, [_VP, ctypes.POINTER(Tuning)]),
     "mpdg_heat_time_vmult": (ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int, ctypes.c_int,
                                             ctypes.POINTER(ctypes.c_double)]),
 }

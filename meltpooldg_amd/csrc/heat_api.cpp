@@ -74,6 +74,8 @@ struct mpdg_heat
   double h[3]{};
   double dt = 0.0;
   bool dt_set = false;
+  int zlayers = 4;       // cell layers per vmult work item (z-chunk)
+  int blocks_per_cu = 0; // persistent workgroups per CU, 0 = occupancy limit
 
   CellParams<double> prm64{};
   CellParams<float> prm32{};
@@ -151,6 +153,8 @@ namespace
     prm.l_rhocp = (Number)l_rhocp;
     prm.sharp = d.two_phase_sharp;
     prm.volume_specific = d.volume_specific_heat;
+    prm.zlayers = std::max(1, std::min(H->zlayers, d.n_cells[2]));
+    prm.blocks_per_cu = H->blocks_per_cu;
     // per-direction tables: [sin2pi dir 0 | dir 1 | dir 2 | sinpi2 dir 0 | dir 1 | dir 2]
     size_t off = 0;
     for (int k = 0; k < 3; ++k)
@@ -191,7 +195,9 @@ namespace
                                       cells_per_block(H->desc.degree, false));
     const ColourGrid cg32 = colour_grid(H->desc.n_cells[0], H->desc.n_cells[1],
                                         H->desc.n_cells[2], cells_per_block(H->desc.degree, true));
-    const size_t np = std::max<size_t>({(size_t)cg.total, (size_t)cg32.total, 2 * (size_t)kVecBlocks});
+    const size_t vmult_slots = 8 * (size_t)kMaxResidentPerCu * device_cu_count();
+    const size_t np = std::max<size_t>({(size_t)cg.total, (size_t)cg32.total, vmult_slots,
+                                        2 * (size_t)kVecBlocks});
     MPDG_HIP(H->partials.alloc(np));
     MPDG_HIP(H->r.alloc(H->N));
     MPDG_HIP(H->p.alloc(H->N));
@@ -625,6 +631,30 @@ mpdg_status mpdg_heat_solve(mpdg_heat *h, double *x, const double *b,
   if (report)
     *report = rep;
   return status;
+}
+
+mpdg_status mpdg_heat_get_tuning(const mpdg_heat *h, mpdg_tuning *t)
+{
+  if (!h || !t)
+    return fail(MPDG_ERR_ARGUMENT, "null argument");
+  t->z_layers = h->zlayers;
+  t->blocks_per_cu = h->blocks_per_cu;
+  return MPDG_OK;
+}
+
+mpdg_status mpdg_heat_set_tuning(mpdg_heat *h, const mpdg_tuning *t)
+{
+  if (!h || !t)
+    return fail(MPDG_ERR_ARGUMENT, "null argument");
+  if (t->z_layers < 1 || t->blocks_per_cu < 0 || t->blocks_per_cu > kMaxResidentPerCu)
+    return fail(MPDG_ERR_ARGUMENT, "z_layers must be >= 1 and blocks_per_cu in 0..16");
+  if (h->stream)
+    MPDG_HIP(hipStreamSynchronize(h->stream));
+  h->zlayers = t->z_layers;
+  h->blocks_per_cu = t->blocks_per_cu;
+  fill_params<double>(h, h->prm64, h->sin_tab64.ptr);
+  fill_params<float>(h, h->prm32, h->sin_tab32.ptr);
+  return MPDG_OK;
 }
 
 mpdg_status mpdg_heat_time_vmult(mpdg_heat *h, double *dst, const double *src, int repeats,

@@ -21,6 +21,7 @@
 //   tangent_local_cell_operation :955-1086 (evaluate values|gradients,
 //   val = dt^-1 rho_cp u, val_grad = k grad u, + rho_cp grad u . v,
 //   integrate values|gradients), material lookup :1133-1210.
+#include "ctables.hpp"
 #include "kernels.hpp"
 
 namespace mpdg
@@ -45,6 +46,10 @@ namespace mpdg
     {
       return ((cpb<P, FP32>() * (P + 1) * (P + 1) + 63) / 64) * 64;
     }
+
+    // Minimum wavefronts per SIMD the register allocator must leave room for
+    // (HIP's second __launch_bounds__ argument): 4 caps the kernel at 128 VGPRs.
+    constexpr int kMinWavesPerSimd = 4;
 
     // y = M x with M in even-odd form, SIGN the symmetry sign of M.
     template <int N, int SIGN, typename Number>
@@ -85,12 +90,88 @@ namespace mpdg
         }
     }
 
+    // y = M x with a compile-time even-odd table (entries folded into the code).
+    template <int N, int SIGN, typename Number>
+    __device__ __forceinline__ void eo_apply(const ct::EO<N> &M, const Number (&x)[N], Number (&y)[N])
+    {
+      constexpr int H = N / 2;
+      Number E[H], O[H];
+#pragma unroll
+      for (int j = 0; j < H; ++j)
+        {
+          E[j] = x[j] + x[N - 1 - j];
+          O[j] = x[j] - x[N - 1 - j];
+        }
+#pragma unroll
+      for (int i = 0; i < H; ++i)
+        {
+          Number a = Number(M.e[i][0]) * E[0];
+          Number b = Number(M.o[i][0]) * O[0];
+#pragma unroll
+          for (int j = 1; j < H; ++j)
+            {
+              a += Number(M.e[i][j]) * E[j];
+              b += Number(M.o[i][j]) * O[j];
+            }
+          if constexpr (N % 2 == 1)
+            a += Number(M.mc[i]) * x[H];
+          y[i] = a + b;
+          y[N - 1 - i] = SIGN > 0 ? a - b : b - a;
+        }
+      if constexpr (N % 2 == 1)
+        {
+          Number m = Number(M.mm) * x[H];
+#pragma unroll
+          for (int j = 0; j < H; ++j)
+            m += Number(M.mr[j]) * (SIGN > 0 ? E[j] : O[j]);
+          y[H] = m;
+        }
+    }
+
     template <typename Number>
     __device__ __forceinline__ bool on_dirichlet(const CellParams<Number> &prm, int i, int d)
     {
       const int n1 = d == 0 ? prm.n1x : (d == 1 ? prm.n1y : prm.n1z);
       return (((prm.faces >> (2 * d)) & 1u) && i == 0) ||
              (((prm.faces >> (2 * d + 1)) & 1u) && i == n1 - 1);
+    }
+
+    // Raw buffer access: an offset at or past the buffer's byte size reads 0
+    // and drops the store, so masked lanes need no branch.
+    constexpr uint32_t kOutOfRange = 0xFFFFFF00u;
+
+    __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *p, uint32_t bytes)
+    {
+      return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes,
+                                               0x00020000);
+    }
+
+    typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+
+    __device__ __forceinline__ double buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off, double)
+    {
+      return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+    }
+    __device__ __forceinline__ float buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off, float)
+    {
+      return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+    }
+    __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t off, double v)
+    {
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, off, 0, 0);
+    }
+    __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t off, float v)
+    {
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, 0);
+    }
+
+    // on_dirichlet without short-circuit branches
+    template <typename Number>
+    __device__ __forceinline__ bool dirichlet_node(const CellParams<Number> &prm, int i, int n1, int d)
+    {
+      const unsigned lo = (prm.faces >> (2 * d)) & 1u;
+      const unsigned hi = (prm.faces >> (2 * d + 1)) & 1u;
+      return (lo & (unsigned)(i == 0)) | (hi & (unsigned)(i == n1 - 1));
     }
 
     // Material::compute_parameters for gas_liquid with the blend of
@@ -215,18 +296,119 @@ namespace mpdg
         return KBl[(qz * N + qy) * N + qx];
     }
 
-    template <int P, typename Number, int COEF, bool VEL, int CPB, int THREADS>
-    __global__ void __launch_bounds__(THREADS)
+    // Orders this wavefront's LDS accesses across lanes: LDS instructions of one
+    // wavefront execute in issue order, so a compiler barrier is enough.
+    __device__ __forceinline__ void wave_sync()
+    {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+
+    // Work item of the vmult: x segment sx (CPB cells), row cy, z chunk kz of
+    // up to `layers` cell layers.
+    struct Item
+    {
+      int sx, cy, kz, cx0, ncell, cz0, nl;
+    };
+
+    template <typename Number, int CPB>
+    __device__ __forceinline__ Item decode_item(const CellParams<Number> &prm, int colour,
+                                                int layers, int item)
+    {
+      const int nsx = (prm.nx + CPB - 1) / CPB;
+      const int ox = colour & 1, oy = (colour >> 1) & 1, oz = (colour >> 2) & 1;
+      const int msx = (nsx - ox + 1) / 2;
+      const int my = (prm.ny - oy + 1) / 2;
+      Item it;
+      it.sx = ox + 2 * (item % msx);
+      const int r = item / msx;
+      it.cy = oy + 2 * (r % my);
+      it.kz = oz + 2 * (r / my);
+      it.cx0 = it.sx * CPB;
+      it.ncell = min(CPB, prm.nx - it.cx0);
+      it.cz0 = it.kz * layers;
+      it.nl = min(layers, prm.nz - it.cz0);
+      return it;
+    }
+
+    // Gather of one x-line (y = s0, z = s1 of cell layer cz, block cell cb) with
+    // homogeneous constraints read as 0 (FEEvaluation::read_dof_values).
+    template <int P, typename Number>
+    __device__ __forceinline__ void gather_xline(const CellParams<Number> &prm,
+                                                 __amdgpu_buffer_rsrc_t src, const Item &it,
+                                                 int cz, int cb, int s0, int s1, bool ok,
+                                                 Number (&u)[P + 1])
+    {
+      constexpr int N = P + 1;
+      const int Yi = it.cy * P + s0, Zi = cz * P + s1;
+      const int X0 = (it.cx0 + cb) * P;
+      const bool take = ok & (cb < it.ncell) & !dirichlet_node(prm, Yi, prm.n1y, 1) &
+                        !dirichlet_node(prm, Zi, prm.n1z, 2);
+      const uint32_t base = (uint32_t)(X0 + prm.n1x * (Yi + prm.n1y * Zi)) * sizeof(Number);
+#pragma unroll
+      for (int a = 0; a < N; ++a)
+        {
+          const bool t = take & !dirichlet_node(prm, X0 + a, prm.n1x, 0);
+          u[a] = buf_load(src, t ? base + a * (uint32_t)sizeof(Number) : kOutOfRange, Number());
+        }
+    }
+
+    template <int P, typename Number>
+    __device__ __forceinline__ void gather_plain(const CellParams<Number> &prm,
+                                                 __amdgpu_buffer_rsrc_t v, const Item &it,
+                                                 int cz, int cb, int s0, int s1, bool ok,
+                                                 Number (&u)[P + 1])
+    {
+      constexpr int N = P + 1;
+      const uint32_t base =
+        (uint32_t)((it.cx0 + cb) * P + prm.n1x * (it.cy * P + s0 + prm.n1y * (cz * P + s1))) *
+        sizeof(Number);
+      const bool take = ok & (cb < it.ncell);
+#pragma unroll
+      for (int a = 0; a < N; ++a)
+        u[a] = buf_load(v, take ? base + a * (uint32_t)sizeof(Number) : kOutOfRange, Number());
+    }
+
+    // Cell-loop vmult, wavefront-segment form.
+    //
+    // A workgroup of WPB wavefronts owns a row segment of CPB = WPB*CW cells
+    // along x; wavefront w owns cells [w*CW, (w+1)*CW) of it. The CW*N^2 lines
+    // of those cells are spread over the 64 lanes x R slots of the wavefront,
+    // cell index fastest. Lanes beyond the last line repeat a real line: they
+    // load and compute exactly what its owner does (so their LDS writes carry
+    // the same values) and are masked off global stores and the dot product. Every 1D contraction is one lane working on a
+    // whole line in registers; moving between x-, y- and z-lines goes through
+    // the wavefront's own LDS region, ordered by wave_sync() only. The single
+    // workgroup barrier per cell layer is the exchange of shared x-faces between
+    // neighbouring wavefronts.
+    //
+    // Each workgroup loops over the work items of one colour and over the cell
+    // layers of each item (z-faces between layers carried through LDS, so only
+    // the chunk's outer z-faces take part in the colouring). The next layer's
+    // x-lines and the current layer's read-modify-write values are loaded into
+    // registers early so their latency overlaps the sum-factorisation work.
+    template <int P, typename Number, int COEF, bool VEL, int CW, int R, int WPB, int MINW>
+    __global__ void __launch_bounds__(64 * WPB, MINW)
       vmult_kernel(const Number *__restrict__ src, Number *__restrict__ dst,
                    const Number *__restrict__ heav, const VmultTables<Number> tab,
-                   const CellParams<Number> prm, const int colour, double *__restrict__ partials,
-                   const int partial_offset, const int partial_total, const int finalize,
-                   SolverScalars *__restrict__ scal)
+                   const CellParams<Number> prm, const int colour, const int layers,
+                   const int items, double *__restrict__ partials, const int partial_offset,
+                   const int partial_total, const int finalize, SolverScalars *__restrict__ scal)
     {
       constexpr int N = P + 1;
       constexpr int NL = N * N;
       constexpr int STR = lds_stride<N>();
+      constexpr int LT = CW * NL;
+      constexpr int CPB = CW * WPB;
+      constexpr int THREADS = 64 * WPB;
       constexpr bool H2 = COEF == kCoefHeaviside;
+      static_assert(LT <= 64 * R && LT > 64 * (R - 1), "lines must fill the last lane slot");
+      constexpr ct::EO<N> TS = ct::VTab<P>::S;
+      constexpr ct::EO<N> TSt = ct::VTab<P>::St;
+      constexpr ct::EO<N> TC = ct::VTab<P>::C;
+      constexpr ct::EO<N> TCt = ct::VTab<P>::Ct;
+      constexpr ct::Tab<N> T1 = ct::VTab<P>::t;
       if (scal != nullptr && scal->done)
         return;
 
@@ -234,248 +416,377 @@ namespace mpdg
       __shared__ Number B[CPB * STR];
       __shared__ Number KB[H2 ? CPB * STR : 1];
       __shared__ Number RB[(H2 && VEL) ? CPB * STR : 1];
-      __shared__ double red[THREADS / 64];
+      __shared__ Number FZ[2][CPB * NL]; // z-face carry [cell][b][a], by layer parity
+      __shared__ Number XE[2][CPB * NL]; // right x-face of each cell line [cell][line]
+      __shared__ double red[WPB];
 
-      const Segment g = segment_of<Number, CPB>(prm, colour);
-      const int t = threadIdx.x;
-      const bool active = t < CPB * NL;
-      const int l = t % CPB;
-      const int s = t / CPB;
-      const int s0 = s % N, s1 = s / N;
-      const bool cell_ok = active && l < g.ncell;
-      const int cx = min(g.cx0 + l, prm.nx - 1);
-      Number *Al = A + l * STR;
-      Number *Bl = B + l * STR;
-      Number *KBl = KB + (H2 ? l * STR : 0);
-      Number *RBl = RB + ((H2 && VEL) ? l * STR : 0);
-
-      // ---- phase 0: gather x-line (y = s0, z = s1), interpolate in x -------
-      Number u[N];
-      if (active)
+      const uint32_t vbytes = (uint32_t)prm.n1x * prm.n1y * prm.n1z * sizeof(Number);
+      const __amdgpu_buffer_rsrc_t rsrc = make_rsrc(src, vbytes);
+      const __amdgpu_buffer_rsrc_t rdst = make_rsrc(dst, vbytes);
+      const __amdgpu_buffer_rsrc_t rheav = make_rsrc(H2 ? heav : src, vbytes);
+      const int w = threadIdx.x >> 6;
+      const int lane = threadIdx.x & 63;
+      int cb[R], s0v[R], s1v[R], sv[R];
+      bool lv[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r)
         {
-          const int Yi = g.cy * P + s0, Zi = g.cz * P + s1;
-          const int X0 = (g.cx0 + l) * P;
-          const bool rowc = on_dirichlet(prm, Yi, 1) || on_dirichlet(prm, Zi, 2);
-          const int64_t base = X0 + (int64_t)prm.n1x * (Yi + (int64_t)prm.n1y * Zi);
+          // a spare lane (only in the last slot) repeats a line of that same slot,
+          // so both run in one instruction and see the same LDS contents
+          const int g0 = lane + 64 * r;
+          lv[r] = g0 < LT;
+          const int g = lv[r] ? g0 : 64 * (R - 1) + (g0 - LT) % (LT - 64 * (R - 1));
+          const int l = g % CW;
+          sv[r] = g / CW;
+          s0v[r] = sv[r] % N;
+          s1v[r] = sv[r] / N;
+          cb[r] = w * CW + l;
+        }
+
+      int item = blockIdx.x;
+      int j = 0;
+      Item it = decode_item<Number, CPB>(prm, colour, layers, item);
+      Number u[R][N], hl[R][N];
 #pragma unroll
-          for (int a = 0; a < N; ++a)
-            u[a] = (cell_ok && !rowc && !on_dirichlet(prm, X0 + a, 0)) ? src[base + a] : Number(0);
-          Number t1[N];
-          eo_apply<N, 1>(tab.S, u, t1);
-#pragma unroll
-          for (int q = 0; q < N; ++q)
-            Al[(s1 * N + s0) * N + q] = t1[q];
+      for (int r = 0; r < R; ++r)
+        {
+          gather_xline<P>(prm, rsrc, it, it.cz0, cb[r], s0v[r], s1v[r], true, u[r]);
           if constexpr (H2)
+            gather_plain<P>(prm, rheav, it, it.cz0, cb[r], s0v[r], s1v[r], true, hl[r]);
+        }
+      double acc = 0.0;
+      int step = 0; // layers done by this workgroup; selects the x-exchange buffer
+
+      while (item < items)
+        {
+          const int cz = it.cz0 + j;
+          int item_n = item, j_n = j + 1;
+          Item it_n = it;
+          if (j_n >= it.nl)
             {
-              Number hl[N], h1[N];
+              item_n = item + gridDim.x;
+              j_n = 0;
+              if (item_n < items)
+                it_n = decode_item<Number, CPB>(prm, colour, layers, item_n);
+            }
+
+          // ---- x-lines (y = s0, z = s1): interpolate in x ------------------------
+#pragma unroll
+          for (int r = 0; r < R; ++r)
+            {
+              Number *Al = A + cb[r] * STR;
+              Number t1[N];
+              eo_apply<N, 1>(TS, u[r], t1);
+#pragma unroll
+              for (int q = 0; q < N; ++q)
+                Al[(s1v[r] * N + s0v[r]) * N + q] = t1[q];
+              if constexpr (H2)
+                {
+                  Number *KBl = KB + cb[r] * STR;
+                  eo_apply<N, 1>(TS, hl[r], t1);
+#pragma unroll
+                  for (int q = 0; q < N; ++q)
+                    KBl[(s1v[r] * N + s0v[r]) * N + q] = t1[q];
+                }
+            }
+
+          // prefetch of the next layer's x-lines
+          Number un[R][N], hn[R][N];
+#pragma unroll
+          for (int r = 0; r < R; ++r)
+            {
+              gather_xline<P>(prm, rsrc, it_n, it_n.cz0 + j_n, cb[r], s0v[r], s1v[r],
+                              item_n < items, un[r]);
+              if constexpr (H2)
+                gather_plain<P>(prm, rheav, it_n, it_n.cz0 + j_n, cb[r], s0v[r], s1v[r],
+                                item_n < items, hn[r]);
+            }
+
+          // scatter plan of each output x-line (b = s0, c = s1): byte offset of
+          // each node written (out of range when not written here) and early
+          // load of the nodes it read-modify-writes (0 where stored first)
+          uint32_t woff[R][N];
+          Number dv[R][N];
+          bool zero[R][N];
+#pragma unroll
+          for (int r = 0; r < R; ++r)
+            {
+              const int b = s0v[r], c = s1v[r];
+              const int Yo = it.cy * P + b, Zo = cz * P + c;
+              const int X0 = (it.cx0 + cb[r]) * P;
+              const uint32_t obase = (uint32_t)(X0 + prm.n1x * (Yo + prm.n1y * Zo)) * sizeof(Number);
+              const bool ok = lv[r] & (cb[r] < it.ncell);
+              const bool last_cell = cb[r] == it.ncell - 1;
+              const bool defer = (c == P) & (j < it.nl - 1); // top face carried to the next layer
+              const bool ysh = ((b == 0) & (it.cy > 0)) | ((b == P) & (it.cy < prm.ny - 1));
+              const bool zsh = ((c == 0) & (j == 0) & (it.cz0 > 0)) |
+                               ((c == P) & (j == it.nl - 1) & (it.cz0 + it.nl < prm.nz));
+              const bool yz_first = !(ysh & (it.cy & 1)) & !(zsh & (it.kz & 1));
+              const bool rowc = dirichlet_node(prm, Yo, prm.n1y, 1) | dirichlet_node(prm, Zo, prm.n1z, 2);
 #pragma unroll
               for (int a = 0; a < N; ++a)
-                hl[a] = cell_ok ? heav[base + a] : Number(0); // read_dof_values_plain
-              eo_apply<N, 1>(tab.S, hl, h1);
-#pragma unroll
-              for (int q = 0; q < N; ++q)
-                Bl[(s1 * N + s0) * N + q] = h1[q];
+                {
+                  const bool xsh = ((a == 0) & (cb[r] == 0) & (it.cx0 > 0)) |
+                                   ((a == P) & last_cell & (it.cx0 + it.ncell < prm.nx));
+                  const bool first = yz_first & !(xsh & (it.sx & 1));
+                  const bool con = rowc | dirichlet_node(prm, X0 + a, prm.n1x, 0);
+                  const bool writes = ok & !defer & ((a < N - 1) | last_cell) & (!con | first);
+                  const uint32_t off = obase + a * (uint32_t)sizeof(Number);
+                  woff[r][a] = writes ? off : kOutOfRange;
+                  zero[r][a] = con;
+                  dv[r][a] = buf_load(rdst, (writes & !first & !con) ? off : kOutOfRange, Number());
+                }
             }
-        }
-      else
-        {
-#pragma unroll
-          for (int a = 0; a < N; ++a)
-            u[a] = Number(0);
-        }
-      __syncthreads();
+          wave_sync();
 
-      // ---- phase 1: y-lines (qx = s0, z = s1), interpolate in y -------------
-      if (active)
-        {
-          Number v[N], w[N];
+          // ---- y-lines (qx = s0, z = s1): interpolate in y ------------------------
 #pragma unroll
-          for (int b = 0; b < N; ++b)
-            v[b] = Al[(s1 * N + b) * N + s0];
-          eo_apply<N, 1>(tab.S, v, w);
-#pragma unroll
-          for (int b = 0; b < N; ++b)
-            Al[(s1 * N + b) * N + s0] = w[b];
-          if constexpr (H2)
+          for (int r = 0; r < R; ++r)
             {
+              Number *Al = A + cb[r] * STR;
+              Number v[N], o[N];
 #pragma unroll
               for (int b = 0; b < N; ++b)
-                v[b] = Bl[(s1 * N + b) * N + s0];
-              eo_apply<N, 1>(tab.S, v, w);
+                v[b] = Al[(s1v[r] * N + b) * N + s0v[r]];
+              eo_apply<N, 1>(TS, v, o);
 #pragma unroll
               for (int b = 0; b < N; ++b)
-                Bl[(s1 * N + b) * N + s0] = w[b];
+                Al[(s1v[r] * N + b) * N + s0v[r]] = o[b];
+              if constexpr (H2)
+                {
+                  Number *KBl = KB + cb[r] * STR;
+#pragma unroll
+                  for (int b = 0; b < N; ++b)
+                    v[b] = KBl[(s1v[r] * N + b) * N + s0v[r]];
+                  eo_apply<N, 1>(TS, v, o);
+#pragma unroll
+                  for (int b = 0; b < N; ++b)
+                    KBl[(s1v[r] * N + b) * N + s0v[r]] = o[b];
+                }
             }
-        }
-      __syncthreads();
+          wave_sync();
 
-      // ---- phase 2: z-lines (qx = s0, qy = s1): values, z-gradient --------
-      Number U[N], Gz[N], RC[N];
-      if (active)
-        {
-          Number v[N];
+          // ---- z-lines (qx = s0, qy = s1): values and z-gradient --------------
+          Number U[R][N], Gz[R][N], RC[R][N];
 #pragma unroll
-          for (int c = 0; c < N; ++c)
-            v[c] = Al[(c * N + s1) * N + s0];
-          eo_apply<N, 1>(tab.S, v, U);
-          eo_apply<N, -1>(tab.C, U, Gz);
-#pragma unroll
-          for (int q = 0; q < N; ++q)
-            Al[(q * N + s1) * N + s0] = U[q];
-          if constexpr (H2)
+          for (int r = 0; r < R; ++r)
             {
-              Number Hq[N];
+              Number *Al = A + cb[r] * STR;
+              Number v[N];
 #pragma unroll
               for (int c = 0; c < N; ++c)
-                v[c] = Bl[(c * N + s1) * N + s0];
-              eo_apply<N, 1>(tab.S, v, Hq);
+                v[c] = Al[(c * N + s1v[r]) * N + s0v[r]];
+              eo_apply<N, 1>(TS, v, U[r]);
+              eo_apply<N, -1>(TC, U[r], Gz[r]);
+#pragma unroll
+              for (int q = 0; q < N; ++q)
+                Al[(q * N + s1v[r]) * N + s0v[r]] = U[r][q];
+              if constexpr (H2)
+                {
+                  Number *KBl = KB + cb[r] * STR;
+                  Number *RBl = RB + ((H2 && VEL) ? cb[r] * STR : 0);
+                  Number Hq[N];
+#pragma unroll
+                  for (int c = 0; c < N; ++c)
+                    v[c] = KBl[(c * N + s1v[r]) * N + s0v[r]];
+                  eo_apply<N, 1>(TS, v, Hq);
+#pragma unroll
+                  for (int q = 0; q < N; ++q)
+                    {
+                      Number rc, k;
+                      two_phase(prm, Hq[q], rc, k);
+                      KBl[(q * N + s1v[r]) * N + s0v[r]] = k;
+                      if constexpr (VEL)
+                        RBl[(q * N + s1v[r]) * N + s0v[r]] = rc;
+                      RC[r][q] = rc;
+                    }
+                }
+            }
+          wave_sync();
+
+          // ---- x-lines (qy = s0, qz = s1): x-gradient flux --------------------
+#pragma unroll
+          for (int r = 0; r < R; ++r)
+            {
+              const int s0 = s0v[r], s1 = s1v[r];
+              const int cx = min(it.cx0 + cb[r], prm.nx - 1);
+              Number *Al = A + cb[r] * STR;
+              Number *Bl = B + cb[r] * STR;
+              const Number *KBl = KB + (H2 ? cb[r] * STR : 0);
+              const Number *RBl = RB + ((H2 && VEL) ? cb[r] * STR : 0);
+              Number v[N], gx[N], f[N], o[N], cv[N];
+#pragma unroll
+              for (int q = 0; q < N; ++q)
+                v[q] = Al[(s1 * N + s0) * N + q];
+              eo_apply<N, -1>(TC, v, gx);
+              const Number wyz = (Number)T1.w[s0] * (Number)T1.w[s1];
 #pragma unroll
               for (int q = 0; q < N; ++q)
                 {
-                  Number rc, k;
-                  two_phase(prm, Hq[q], rc, k);
-                  KBl[(q * N + s1) * N + s0] = k;
+                  const Number w3 = (Number)T1.w[q] * wyz;
+                  const Number k = conductivity<Number, COEF, N>(prm, KBl, cx, it.cy, cz, q, s0, s1);
+                  f[q] = k * w3 * prm.kf[0] * gx[q];
                   if constexpr (VEL)
-                    RBl[(q * N + s1) * N + s0] = rc;
-                  RC[q] = rc;
+                    {
+                      const Number rc = H2 ? RBl[(s1 * N + s0) * N + q] : prm.rho_cp0;
+                      const Number vx = -prm.sinpi2[0][cx * N + q] * prm.sin2pi[1][it.cy * N + s0];
+                      cv[q] = rc * w3 * prm.vf[0] * vx * gx[q];
+                    }
                 }
+              eo_apply<N, -1>(TCt, f, o);
+#pragma unroll
+              for (int q = 0; q < N; ++q)
+                Bl[(s1 * N + s0) * N + q] = VEL ? o[q] + cv[q] : o[q];
             }
-        }
-      __syncthreads();
+          wave_sync();
 
-      // ---- phase 3a: x-lines (qy = s0, qz = s1): x-gradient flux ----------
-      if (active)
-        {
-          Number v[N], gx[N], f[N], o[N];
+          // ---- y-lines (qx = s0, qz = s1): y-gradient flux ----------------------
 #pragma unroll
-          for (int r = 0; r < N; ++r)
-            v[r] = Al[(s1 * N + s0) * N + r];
-          eo_apply<N, -1>(tab.C, v, gx);
-          const Number wyz = tab.W[s0] * tab.W[s1];
-          Number cv[N];
-#pragma unroll
-          for (int q = 0; q < N; ++q)
+          for (int r = 0; r < R; ++r)
             {
-              const Number w3 = tab.W[q] * wyz;
-              const Number k =
-                conductivity<Number, COEF, N>(prm, KBl, cx, g.cy, g.cz, q, s0, s1);
-              f[q] = k * w3 * prm.kf[0] * gx[q];
-              if constexpr (VEL)
+              const int s0 = s0v[r], s1 = s1v[r];
+              const int cx = min(it.cx0 + cb[r], prm.nx - 1);
+              Number *Al = A + cb[r] * STR;
+              Number *Bl = B + cb[r] * STR;
+              const Number *KBl = KB + (H2 ? cb[r] * STR : 0);
+              const Number *RBl = RB + ((H2 && VEL) ? cb[r] * STR : 0);
+              Number v[N], gy[N], f[N], o[N], cv[N];
+#pragma unroll
+              for (int q = 0; q < N; ++q)
+                v[q] = Al[(s1 * N + q) * N + s0];
+              eo_apply<N, -1>(TC, v, gy);
+              const Number wxz = (Number)T1.w[s0] * (Number)T1.w[s1];
+#pragma unroll
+              for (int q = 0; q < N; ++q)
                 {
-                  const Number rc = H2 ? RBl[(s1 * N + s0) * N + q] : prm.rho_cp0;
-                  const Number vx = -prm.sinpi2[0][cx * N + q] * prm.sin2pi[1][g.cy * N + s0];
-                  cv[q] = rc * w3 * prm.vf[0] * vx * gx[q];
+                  const Number w3 = (Number)T1.w[q] * wxz;
+                  const Number k = conductivity<Number, COEF, N>(prm, KBl, cx, it.cy, cz, s0, q, s1);
+                  f[q] = k * w3 * prm.kf[1] * gy[q];
+                  if constexpr (VEL)
+                    {
+                      const Number rc = H2 ? RBl[(s1 * N + q) * N + s0] : prm.rho_cp0;
+                      const Number vy = prm.sin2pi[0][cx * N + s0] * prm.sinpi2[1][it.cy * N + q];
+                      cv[q] = rc * w3 * prm.vf[1] * vy * gy[q];
+                    }
                 }
-            }
-          eo_apply<N, -1>(tab.Ct, f, o);
+              eo_apply<N, -1>(TCt, f, o);
 #pragma unroll
-          for (int q = 0; q < N; ++q)
-            Bl[(s1 * N + s0) * N + q] = VEL ? o[q] + cv[q] : o[q];
-        }
-      __syncthreads();
-
-      // ---- phase 3b: y-lines (qx = s0, qz = s1): y-gradient flux ----------
-      if (active)
-        {
-          Number v[N], gy[N], f[N], o[N], cv[N];
-#pragma unroll
-          for (int r = 0; r < N; ++r)
-            v[r] = Al[(s1 * N + r) * N + s0];
-          eo_apply<N, -1>(tab.C, v, gy);
-          const Number wxz = tab.W[s0] * tab.W[s1];
-#pragma unroll
-          for (int q = 0; q < N; ++q)
-            {
-              const Number w3 = tab.W[q] * wxz;
-              const Number k =
-                conductivity<Number, COEF, N>(prm, KBl, cx, g.cy, g.cz, s0, q, s1);
-              f[q] = k * w3 * prm.kf[1] * gy[q];
-              if constexpr (VEL)
+              for (int q = 0; q < N; ++q)
                 {
-                  const Number rc = H2 ? RBl[(s1 * N + q) * N + s0] : prm.rho_cp0;
-                  const Number vy = prm.sin2pi[0][cx * N + s0] * prm.sinpi2[1][g.cy * N + q];
-                  cv[q] = rc * w3 * prm.vf[1] * vy * gy[q];
+                  const int idx = (s1 * N + q) * N + s0;
+                  Bl[idx] = Bl[idx] + (VEL ? o[q] + cv[q] : o[q]);
                 }
             }
-          eo_apply<N, -1>(tab.Ct, f, o);
-#pragma unroll
-          for (int q = 0; q < N; ++q)
-            {
-              const int idx = (s1 * N + q) * N + s0;
-              Bl[idx] = Bl[idx] + (VEL ? o[q] + cv[q] : o[q]);
-            }
-        }
-      __syncthreads();
+          wave_sync();
 
-      // ---- phase 4: z-lines (qx = s0, qy = s1): q-point physics, integrate z
-      if (active)
-        {
-          Number f[N], o[N], V[N], y1[N];
-          const Number wxy = tab.W[s0] * tab.W[s1];
+          // ---- z-lines (qx = s0, qy = s1): q-point physics, integrate z -------
 #pragma unroll
-          for (int q = 0; q < N; ++q)
+          for (int r = 0; r < R; ++r)
             {
-              const Number w3 = tab.W[q] * wxy;
-              const Number k =
-                conductivity<Number, COEF, N>(prm, KBl, cx, g.cy, g.cz, s0, s1, q);
-              f[q] = k * w3 * prm.kf[2] * Gz[q];
-            }
-          eo_apply<N, -1>(tab.Ct, f, o);
+              const int s0 = s0v[r], s1 = s1v[r];
+              const int cx = min(it.cx0 + cb[r], prm.nx - 1);
+              Number *Al = A + cb[r] * STR;
+              const Number *Bl = B + cb[r] * STR;
+              const Number *KBl = KB + (H2 ? cb[r] * STR : 0);
+              Number f[N], o[N], V[N], y1[N];
+              const Number wxy = (Number)T1.w[s0] * (Number)T1.w[s1];
 #pragma unroll
-          for (int q = 0; q < N; ++q)
-            {
-              const Number w3 = tab.W[q] * wxy;
-              const Number mass = H2 ? RC[q] * prm.dtinv_vol : prm.mass;
-              Number val = mass * w3 * U[q] + o[q] + Bl[(q * N + s1) * N + s0];
-              if constexpr (VEL)
+              for (int q = 0; q < N; ++q)
                 {
-                  const Number rc = H2 ? RC[q] : prm.rho_cp0;
-                  val += rc * w3 * prm.vf[2] * prm.w0 * Gz[q];
+                  const Number w3 = (Number)T1.w[q] * wxy;
+                  const Number k = conductivity<Number, COEF, N>(prm, KBl, cx, it.cy, cz, s0, s1, q);
+                  f[q] = k * w3 * prm.kf[2] * Gz[r][q];
                 }
-              V[q] = val;
+              eo_apply<N, -1>(TCt, f, o);
+#pragma unroll
+              for (int q = 0; q < N; ++q)
+                {
+                  const Number w3 = (Number)T1.w[q] * wxy;
+                  const Number mass = H2 ? RC[r][q] * prm.dtinv_vol : prm.mass;
+                  Number val = mass * w3 * U[r][q] + o[q] + Bl[(q * N + s1) * N + s0];
+                  if constexpr (VEL)
+                    {
+                      const Number rc = H2 ? RC[r][q] : prm.rho_cp0;
+                      val += rc * w3 * prm.vf[2] * prm.w0 * Gz[r][q];
+                    }
+                  V[q] = val;
+                }
+              eo_apply<N, 1>(TSt, V, y1);
+#pragma unroll
+              for (int c = 0; c < N; ++c)
+                Al[(c * N + s1) * N + s0] = y1[c];
             }
-          eo_apply<N, 1>(tab.St, V, y1);
-#pragma unroll
-          for (int c = 0; c < N; ++c)
-            Al[(c * N + s1) * N + s0] = y1[c];
-        }
-      __syncthreads();
+          wave_sync();
 
-      // ---- phase 5: y-lines (qx = s0, c = s1): integrate y ----------------
-      if (active)
-        {
-          Number v[N], w[N];
+          // ---- y-lines (qx = s0, c = s1): integrate y ------------------------------
 #pragma unroll
-          for (int q = 0; q < N; ++q)
-            v[q] = Al[(s1 * N + q) * N + s0];
-          eo_apply<N, 1>(tab.St, v, w);
+          for (int r = 0; r < R; ++r)
+            {
+              Number *Al = A + cb[r] * STR;
+              Number v[N], o[N];
 #pragma unroll
-          for (int b = 0; b < N; ++b)
-            Al[(s1 * N + b) * N + s0] = w[b];
-        }
-      __syncthreads();
+              for (int q = 0; q < N; ++q)
+                v[q] = Al[(s1v[r] * N + q) * N + s0v[r]];
+              eo_apply<N, 1>(TSt, v, o);
+#pragma unroll
+              for (int b = 0; b < N; ++b)
+                Al[(s1v[r] * N + b) * N + s0v[r]] = o[b];
+            }
+          wave_sync();
 
-      // ---- phase 6: x-lines (b = s0, c = s1): integrate x, scatter ---------
-      Number Y[N];
-      double acc = 0.0;
-      if (active)
-        {
-          Number v[N];
+          // ---- x-lines (b = s0, c = s1): integrate x, publish right faces ---------
+          Number Y[R][N];
 #pragma unroll
-          for (int q = 0; q < N; ++q)
-            v[q] = Al[(s1 * N + s0) * N + q];
-          eo_apply<N, 1>(tab.St, v, Y);
+          for (int r = 0; r < R; ++r)
+            {
+              const Number *Al = A + cb[r] * STR;
+              Number v[N];
 #pragma unroll
-          for (int a = 0; a < N; ++a)
-            acc += (double)u[a] * (double)Y[a];
-          B[s * CPB + l] = Y[N - 1];
-        }
-      __syncthreads();
-      if (cell_ok)
-        {
-          if (l > 0)
-            Y[0] += B[s * CPB + l - 1];
-          scatter_xline<P, Number, CPB>(prm, g, l, s0, s1, Y, dst);
+              for (int q = 0; q < N; ++q)
+                v[q] = Al[(s1v[r] * N + s0v[r]) * N + q];
+              eo_apply<N, 1>(TSt, v, Y[r]);
+              if (lv[r])
+#pragma unroll
+                for (int a = 0; a < N; ++a)
+                  acc += (double)u[r][a] * (double)Y[r][a];
+              XE[step & 1][cb[r] * NL + sv[r]] = Y[r][N - 1];
+            }
+          __syncthreads();
+
+          // ---- sum shared x- and z-faces, scatter -----------------------------------
+#pragma unroll
+          for (int r = 0; r < R; ++r)
+            {
+              const int b = s0v[r], c = s1v[r];
+              if (cb[r] > 0)
+                Y[r][0] += XE[step & 1][(cb[r] - 1) * NL + sv[r]];
+              if (c == 0 && j > 0)
+#pragma unroll
+                for (int a = 0; a < N; ++a)
+                  Y[r][a] += FZ[(j - 1) & 1][(cb[r] * N + b) * N + a];
+              if (c == P && j < it.nl - 1)
+#pragma unroll
+                for (int a = 0; a < N; ++a)
+                  FZ[j & 1][(cb[r] * N + b) * N + a] = Y[r][a];
+#pragma unroll
+              for (int a = 0; a < N; ++a)
+                buf_store(rdst, woff[r][a], zero[r][a] ? Number(0) : dv[r][a] + Y[r][a]);
+            }
+
+#pragma unroll
+          for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int a = 0; a < N; ++a)
+              {
+                u[r][a] = un[r][a];
+                if constexpr (H2)
+                  hl[r][a] = hn[r][a];
+              }
+          item = item_n;
+          j = j_n;
+          it = it_n;
+          ++step;
+          wave_sync();
         }
 
       if (partials != nullptr)
@@ -757,22 +1068,46 @@ namespace mpdg
         }
     }
 
+    // Wavefront shape of the vmult per degree: CW cells per wavefront, R line
+    // slots per lane (CW*(P+1)^2 lines fill the last slot), 4 wavefronts per
+    // workgroup.
+    template <int P, bool FP32>
+    struct VShape
+    {
+      static constexpr int CW = P == 1 ? 16 : P == 2 ? 7 : P == 3 ? 4 : P == 4 ? (FP32 ? 5 : 2) : P == 5 ? (FP32 ? 3 : 1) : 1;
+      static constexpr int R = (CW * (P + 1) * (P + 1) + 63) / 64;
+      static constexpr int WPB = 4;
+      // register budget: 128 VGPRs (4 waves per SIMD), 256 for FP64 from degree 5
+      static constexpr int MINW = (P >= 5 && !FP32) ? 2 : kMinWavesPerSimd;
+    };
+
     template <typename Number, int P, int COEF, bool VEL>
     hipError_t vmult_passes(const CellParams<Number> &prm, const VmultTables<Number> &tab,
                             const Number *src, Number *dst, const Number *heav, double *partials,
                             SolverScalars *scal, hipStream_t stream)
     {
-      constexpr bool F32 = sizeof(Number) == 4;
-      constexpr int CPB = cpb<P, F32>();
-      constexpr int THREADS = block_threads<P, F32>();
-      const ColourGrid cg = colour_grid(prm.nx, prm.ny, prm.nz, CPB);
+      using Shape = VShape<P, sizeof(Number) == 4>;
+      constexpr int THREADS = 64 * Shape::WPB;
+      constexpr int CPB = Shape::CW * Shape::WPB;
+      auto kernel = vmult_kernel<P, Number, COEF, VEL, Shape::CW, Shape::R, Shape::WPB, Shape::MINW>;
+      static int resident = 0; // workgroups per CU, from the occupancy calculator
+      if (resident == 0)
+        {
+          int nb = 0;
+          if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, THREADS, 0) != hipSuccess || nb < 1)
+            nb = 1;
+          resident = nb < kMaxResidentPerCu ? nb : kMaxResidentPerCu;
+        }
+      const int per_cu = prm.blocks_per_cu > 0 ? prm.blocks_per_cu : resident;
+      const VmultGrid vg = vmult_grid(prm.nx, prm.ny, prm.nz, CPB, prm.zlayers,
+                                      per_cu * device_cu_count());
       for (int c = 0; c < 8; ++c)
         {
-          if (cg.blocks[c] == 0)
+          if (vg.grid[c] == 0)
             continue;
-          vmult_kernel<P, Number, COEF, VEL, CPB, THREADS><<<cg.blocks[c], THREADS, 0, stream>>>(
-            src, dst, heav, tab, prm, c, partials, cg.offset[c], cg.total, c == cg.last ? 1 : 0,
-            scal);
+          kernel<<<vg.grid[c], THREADS, 0, stream>>>(src, dst, heav, tab, prm, c, prm.zlayers,
+                                                     vg.items[c], partials, vg.offset[c], vg.total,
+                                                     c == vg.last ? 1 : 0, scal);
         }
       return hipGetLastError();
     }
@@ -851,6 +1186,42 @@ namespace mpdg
         g.blocks[c] = m;
         g.offset[c] = off;
         off += m;
+        if (m > 0)
+          g.last = c;
+      }
+    g.total = off;
+    return g;
+  }
+
+  int device_cu_count()
+  {
+    static int cus = 0;
+    if (cus == 0)
+      {
+        int dev = 0, n = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n < 1)
+          n = 256;
+        cus = n;
+      }
+    return cus;
+  }
+
+  VmultGrid vmult_grid(int nx, int ny, int nz, int cpb_, int layers, int max_blocks)
+  {
+    VmultGrid g{};
+    const int nsx = (nx + cpb_ - 1) / cpb_;
+    const int nkz = (nz + layers - 1) / layers;
+    int off = 0;
+    g.last = -1;
+    for (int c = 0; c < 8; ++c)
+      {
+        const int ox = c & 1, oy = (c >> 1) & 1, oz = (c >> 2) & 1;
+        const int m = ((nsx - ox + 1) / 2) * ((ny - oy + 1) / 2) * ((nkz - oz + 1) / 2);
+        g.items[c] = m;
+        g.grid[c] = m < max_blocks ? m : max_blocks;
+        g.offset[c] = off;
+        off += g.grid[c];
         if (m > 0)
           g.last = c;
       }

@@ -26,6 +26,8 @@ namespace mpdg
     Number w0;            // vortex axial velocity
     Number g_rho, g_cp, g_k, l_rho, l_cp, l_k, g_rhocp, l_rhocp;
     int sharp, volume_specific;
+    int zlayers;              // cell layers per z-chunk of a vmult work item
+    int blocks_per_cu;        // persistent workgroups per CU (0: occupancy limit)
     const Number *sin2pi[3]; // sin(2 pi x) at the quadrature coordinates of each direction
     const Number *sinpi2[3]; // sin^2(pi x) at the same coordinates
   };
@@ -80,6 +82,21 @@ namespace mpdg
     return degree <= 2 ? 32 : (degree <= 4 ? 16 : (fp32 ? 16 : 8));
   }
   int cells_per_block(int degree, bool fp32);
+
+  // Persistent launch shape of the vmult: per colour the number of work items
+  // (segment x row x z-chunk) and of workgroups looping over them.
+  struct VmultGrid
+  {
+    int items[8];
+    int grid[8];
+    int offset[8]; // index of the colour's first partial dot
+    int total;
+    int last;
+  };
+  VmultGrid vmult_grid(int nx, int ny, int nz, int cpb, int layers, int max_blocks);
+  int device_cu_count();
+  // Upper bound on the partial-dot slots of one vmult.
+  constexpr int kMaxResidentPerCu = 16;
   int threads_per_block(int degree, bool fp32);
   ColourGrid colour_grid(int nx, int ny, int nz, int cpb);
 

@@ -161,6 +161,18 @@ class HeatDiffuseOperator:
         self._check_vec(diag, "float64")
         L.check(self._lib.mpdg_heat_inverse_diagonal(self._h, _ptr(diag)))
 
+    def get_tuning(self):
+        t = L.Tuning()
+        L.check(self._lib.mpdg_heat_get_tuning(self._h, ctypes.byref(t)))
+        return {"z_layers": t.z_layers, "blocks_per_cu": t.blocks_per_cu}
+
+    def set_tuning(self, z_layers=None, blocks_per_cu=None):
+        cur = self.get_tuning()
+        t = L.Tuning()
+        t.z_layers = int(cur["z_layers"] if z_layers is None else z_layers)
+        t.blocks_per_cu = int(cur["blocks_per_cu"] if blocks_per_cu is None else blocks_per_cu)
+        L.check(self._lib.mpdg_heat_set_tuning(self._h, ctypes.byref(t)))
+
     def time_vmult(self, dst, src, repeats=10):
         ms = ctypes.c_double()
         prec = L.MPDG_FP32 if str(src.dtype).endswith("float32") else L.MPDG_FP64
