@@ -92,16 +92,22 @@ struct mpdg_heat
   // solver workspace
   DevBuf<double> r, p, q, inv_diag, partials, history;
   DevBuf<float> r32, p32, q32, x32, inv32;
+  DevBuf<double> V64, gm;  // GMRES basis (FP64) and small coefficient block
+  DevBuf<float> V32;       // GMRES basis of the FP32 inner solves
+  double *host_gm = nullptr;
   DevBuf<SolverScalars> scal;
   SolverScalars *host_scal = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
   ~mpdg_heat()
   {
-    for (auto *b : {&sin_tab64, &sin_tab32_src, &heav64, &r, &p, &q, &inv_diag, &partials, &history})
+    for (auto *b : {&sin_tab64, &sin_tab32_src, &heav64, &r, &p, &q, &inv_diag, &partials, &history,
+                    &V64, &gm})
       b->release();
-    for (auto *b : {&sin_tab32, &heav32, &r32, &p32, &q32, &x32, &inv32})
+    for (auto *b : {&sin_tab32, &heav32, &r32, &p32, &q32, &x32, &inv32, &V32})
       b->release();
+    if (host_gm)
+      (void)hipHostFree(host_gm);
     scal.release();
     if (host_scal)
       (void)hipHostFree(host_scal);
@@ -197,7 +203,7 @@ namespace
                                         H->desc.n_cells[2], cells_per_block(H->desc.degree, true));
     const size_t vmult_slots = 8 * (size_t)kMaxResidentPerCu * device_cu_count();
     const size_t np = std::max<size_t>({(size_t)cg.total, (size_t)cg32.total, vmult_slots,
-                                        2 * (size_t)kVecBlocks});
+                                        (size_t)kMaxBasis * kVecBlocks});
     MPDG_HIP(H->partials.alloc(np));
     MPDG_HIP(H->r.alloc(H->N));
     MPDG_HIP(H->p.alloc(H->N));
@@ -220,6 +226,23 @@ namespace
         MPDG_HIP(H->x32.alloc(H->N));
         MPDG_HIP(H->inv32.alloc(H->N));
       }
+    return MPDG_OK;
+  }
+
+  // GMRES coefficient block layout (doubles): h1[32] | h2[32] | nrm2 | y[32]
+  constexpr int kGmH1 = 0, kGmH2 = kMaxBasis, kGmNrm = 2 * kMaxBasis, kGmY = 2 * kMaxBasis + 8,
+                kGmSize = 3 * kMaxBasis + 8;
+
+  mpdg_status ensure_gmres(mpdg_heat *H, int restart, bool fp32)
+  {
+    const size_t nb = (size_t)(restart + 1) * H->N;
+    if (fp32)
+      MPDG_HIP(H->V32.alloc(nb));
+    else
+      MPDG_HIP(H->V64.alloc(nb));
+    MPDG_HIP(H->gm.alloc(kGmSize));
+    if (!H->host_gm)
+      MPDG_HIP(hipHostMalloc((void **)&H->host_gm, kGmSize * sizeof(double), hipHostMallocDefault));
     return MPDG_OK;
   }
 
@@ -303,6 +326,132 @@ namespace
           break;
         (void)done_it;
       }
+    return MPDG_OK;
+  }
+
+  // Right-preconditioned GMRES(restart) with Jacobi, classical Gram-Schmidt
+  // plus one re-orthogonalisation pass, Givens rotations on the host; the
+  // inner check uses the least-squares residual estimate, each restart the
+  // true residual (deal.II SolverGMRES with right preconditioning,
+  // linear_solver.hpp:112-119). Number is the precision of the basis and of
+  // the operator; b is FP64.
+  template <typename Number>
+  mpdg_status gmres(mpdg_heat *H, Number *x, const double *b, Number *r, Number *z, Number *w,
+                    Number *V, const Number *inv, double rel_tol, double abs_tol, int max_it,
+                    int restart, bool x_is_zero, int *iterations, double *res0_out,
+                    double *res_out, bool *converged)
+  {
+    const int64_t n = H->N;
+    const int64_t ld = n;
+    double *part = H->partials.ptr;
+    SolverScalars *sc = H->scal.ptr;
+    unsigned *counter = &sc->counter;
+    double *gm = H->gm.ptr;
+    auto op = [&](Number *dst, const Number *src) -> mpdg_status {
+      if constexpr (sizeof(Number) == 8)
+        return vmult64(H, dst, src, nullptr, nullptr);
+      else
+        return vmult32(H, dst, src, nullptr, nullptr);
+    };
+    auto residual = [&](bool zero_x) -> mpdg_status {
+      if (!zero_x)
+        if (mpdg_status st = op(w, x))
+          return st;
+      // r = b - A x and ||r|| (z receives D r, unused)
+      MPDG_HIP(launch_cg_init<Number>(b, zero_x ? nullptr : w, r, z, inv, n, 0.0, 0.0, 1, part, sc,
+                                      H->stream));
+      return read_scalars(H);
+    };
+    if (mpdg_status st = residual(x_is_zero))
+      return st;
+    double res = H->host_scal->res;
+    const double res0 = res;
+    const double tol = std::max(abs_tol, rel_tol * res0);
+    int it = 0;
+    bool conv = res <= tol;
+    std::vector<double> Hm((size_t)(restart + 1) * restart), cs(restart), sn(restart),
+      g(restart + 1), y(restart);
+    while (!conv && it < max_it)
+      {
+        MPDG_HIP(launch_scale<Number>(V, r, &sc->rr, n, H->stream));
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = res;
+        int j = 0;
+        for (; j < restart; ++j)
+          {
+            ++it;
+            Number *vj = V + j * ld;
+            MPDG_HIP(launch_diag_scale<Number>(z, vj, inv, n, H->stream));
+            if (mpdg_status st = op(w, z))
+              return st;
+            MPDG_HIP(launch_multi_dot<Number>(V, ld, j + 1, w, n, part, gm + kGmH1, counter, H->stream));
+            MPDG_HIP(launch_multi_axpy<Number>(w, V, ld, j + 1, gm + kGmH1, n, part, nullptr, counter,
+                                               H->stream));
+            MPDG_HIP(launch_multi_dot<Number>(V, ld, j + 1, w, n, part, gm + kGmH2, counter, H->stream));
+            MPDG_HIP(launch_multi_axpy<Number>(w, V, ld, j + 1, gm + kGmH2, n, part, gm + kGmNrm,
+                                               counter, H->stream));
+            MPDG_HIP(hipMemcpyAsync(H->host_gm, gm, (kGmNrm + 1) * sizeof(double),
+                                    hipMemcpyDeviceToHost, H->stream));
+            MPDG_HIP(hipStreamSynchronize(H->stream));
+            std::vector<double> col(j + 2);
+            for (int l = 0; l <= j; ++l)
+              col[l] = H->host_gm[kGmH1 + l] + H->host_gm[kGmH2 + l];
+            const double hn = std::sqrt(H->host_gm[kGmNrm]);
+            col[j + 1] = hn;
+            if (hn > 0.0 && j + 1 <= restart)
+              MPDG_HIP(launch_scale<Number>(V + (j + 1) * ld, w, gm + kGmNrm, n, H->stream));
+            for (int l = 0; l < j; ++l)
+              {
+                const double a = cs[l] * col[l] + sn[l] * col[l + 1];
+                const double bb = -sn[l] * col[l] + cs[l] * col[l + 1];
+                col[l] = a;
+                col[l + 1] = bb;
+              }
+            const double den = std::hypot(col[j], col[j + 1]);
+            cs[j] = col[j] / den;
+            sn[j] = col[j + 1] / den;
+            col[j] = den;
+            col[j + 1] = 0.0;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            for (int l = 0; l <= j; ++l)
+              Hm[(size_t)l * restart + j] = col[l];
+            res = std::fabs(g[j + 1]);
+            if (res <= tol || it >= max_it || std::isnan(res) || hn == 0.0)
+              {
+                ++j;
+                break;
+              }
+          }
+        // x += D V y, y from the triangular system
+        for (int l = j - 1; l >= 0; --l)
+          {
+            double sum = g[l];
+            for (int c = l + 1; c < j; ++c)
+              sum -= Hm[(size_t)l * restart + c] * y[c];
+            y[l] = sum / Hm[(size_t)l * restart + l];
+          }
+        std::copy(y.begin(), y.begin() + j, H->host_gm + kGmY);
+        MPDG_HIP(hipMemcpyAsync(gm + kGmY, H->host_gm + kGmY, j * sizeof(double),
+                                hipMemcpyHostToDevice, H->stream));
+        MPDG_HIP((launch_gmres_update<Number, Number>(x, V, ld, j, gm + kGmY, inv, n, H->stream)));
+        if (res <= tol)
+          {
+            conv = true;
+            break;
+          }
+        if (it >= max_it || std::isnan(res))
+          break;
+        if (mpdg_status st = residual(false))
+          return st;
+        res = H->host_scal->res;
+        conv = res <= tol;
+      }
+    MPDG_HIP(hipStreamSynchronize(H->stream));
+    *iterations = it;
+    *res0_out = res0;
+    *res_out = res;
+    *converged = conv;
     return MPDG_OK;
   }
 
@@ -408,10 +557,10 @@ mpdg_status mpdg_heat_create(const mpdg_heat_desc *desc, mpdg_heat **out)
       H->dt64.W[q] = t.wq[q];
     }
   if (symmetry_defect(t.S, n, 1) > 1e-13 || symmetry_defect(t.C, n, -1) > 1e-12 ||
-      symmetry_defect(t.D, n, -1) > 1e-12)
+      symmetry_defect(t.D, n, -1) > 1e-12 || ctable_defect(d.degree, t) > 1e-13)
     {
       delete H;
-      return fail(MPDG_ERR_ARGUMENT, "1D tables lost their even-odd symmetry");
+      return fail(MPDG_ERR_ARGUMENT, "1D tables failed their symmetry / compile-time consistency check");
     }
 
   // analytic coefficient tables at the quadrature coordinates
@@ -547,13 +696,19 @@ mpdg_status mpdg_heat_solve(mpdg_heat *h, double *x, const double *b,
     prm = *params;
   else
     mpdg_solver_params_default(&prm);
-  if (prm.solver_type != MPDG_SOLVER_CG)
-    return fail(MPDG_ERR_UNSUPPORTED, "only CG is built in this engine version");
+  if (prm.solver_type != MPDG_SOLVER_CG && prm.solver_type != MPDG_SOLVER_GMRES)
+    return fail(MPDG_ERR_UNSUPPORTED, "solver type must be CG or GMRES");
+  const bool use_gmres = prm.solver_type == MPDG_SOLVER_GMRES;
+  if (use_gmres && (prm.gmres_restart < 1 || prm.gmres_restart > kMaxBasis - 1))
+    return fail(MPDG_ERR_ARGUMENT, "gmres_restart must be in 1..31");
   if (prm.max_iterations < 1 || !(prm.rel_tolerance >= 0) || !(prm.abs_tolerance >= 0))
     return fail(MPDG_ERR_ARGUMENT, "invalid solver parameters");
   const bool mixed = prm.mixed_precision != 0;
   if (mpdg_status s = ensure_workspace(h, mixed))
     return s;
+  if (use_gmres)
+    if (mpdg_status s = ensure_gmres(h, prm.gmres_restart, mixed))
+      return s;
   const int check = prm.check_interval > 0 ? prm.check_interval : auto_check(h);
   mpdg_solver_report rep{};
   const auto t0 = std::chrono::steady_clock::now();
@@ -568,7 +723,23 @@ mpdg_status mpdg_heat_solve(mpdg_heat *h, double *x, const double *b,
     MPDG_HIP(launch_fill(h->inv_diag.ptr, 1.0, h->N, h->stream));
 
   mpdg_status status = MPDG_OK;
-  if (!mixed)
+  if (!mixed && use_gmres)
+    {
+      int its = 0;
+      double r0 = 0.0, rf = 0.0;
+      bool conv = false;
+      if (mpdg_status s = gmres<double>(h, x, b, h->r.ptr, h->p.ptr, h->q.ptr, h->V64.ptr,
+                                        h->inv_diag.ptr, prm.rel_tolerance, prm.abs_tolerance,
+                                        prm.max_iterations, prm.gmres_restart, false, &its, &r0,
+                                        &rf, &conv))
+        return s;
+      rep.iterations = its;
+      rep.initial_residual = r0;
+      rep.final_residual = rf;
+      if (!conv)
+        status = fail(MPDG_ERR_NOT_CONVERGED, "iterative solver did not converge");
+    }
+  else if (!mixed)
     {
       if (mpdg_status s = pcg<double>(h, x, b, h->r.ptr, h->p.ptr, h->q.ptr, h->inv_diag.ptr,
                                       prm.rel_tolerance, prm.abs_tolerance, prm.max_iterations,
@@ -615,11 +786,26 @@ mpdg_status mpdg_heat_solve(mpdg_heat *h, double *x, const double *b,
           // does not oversolve past the outer tolerance
           const double inner_rel = std::max(prm.inner_rel_tolerance, 0.5 * tol / res);
           MPDG_HIP(launch_fill_f32(h->x32.ptr, 0.0f, h->N, h->stream));
-          if (mpdg_status s = pcg<float>(h, h->x32.ptr, h->r.ptr, h->r32.ptr, h->p32.ptr,
-                                         h->q32.ptr, h->inv32.ptr, inner_rel, 0.0,
-                                         prm.max_iterations - inner_total, check, true))
-            return s;
-          inner_total += h->host_scal->iteration;
+          if (use_gmres)
+            {
+              int its = 0;
+              double r0 = 0.0, rf = 0.0;
+              bool conv = false;
+              if (mpdg_status s = gmres<float>(h, h->x32.ptr, h->r.ptr, h->r32.ptr, h->p32.ptr,
+                                               h->q32.ptr, h->V32.ptr, h->inv32.ptr, inner_rel, 0.0,
+                                               prm.max_iterations - inner_total, prm.gmres_restart,
+                                               true, &its, &r0, &rf, &conv))
+                return s;
+              inner_total += its;
+            }
+          else
+            {
+              if (mpdg_status s = pcg<float>(h, h->x32.ptr, h->r.ptr, h->r32.ptr, h->p32.ptr,
+                                             h->q32.ptr, h->inv32.ptr, inner_rel, 0.0,
+                                             prm.max_iterations - inner_total, check, true))
+                return s;
+              inner_total += h->host_scal->iteration;
+            }
           MPDG_HIP(launch_add_f32(x, h->x32.ptr, h->N, h->stream));
         }
       rep.iterations = inner_total;

@@ -1165,6 +1165,43 @@ namespace mpdg
     }
   } // namespace
 
+  namespace
+  {
+    template <int P>
+    double ctable_defect_p(const Tables1D &t)
+    {
+      constexpr ct::Tab<P + 1> c = ct::make_tab<P>();
+      double d = 0.0;
+      for (int i = 0; i <= P; ++i)
+        {
+          d = fmax(d, fabs(c.gll[i] - t.gll[i]));
+          d = fmax(d, fabs(c.x[i] - t.xq[i]));
+          d = fmax(d, fabs(c.w[i] - t.wq[i]));
+          for (int j = 0; j <= P; ++j)
+            {
+              d = fmax(d, fabs(c.S[i][j] - t.S[i][j]));
+              d = fmax(d, fabs(c.C[i][j] - t.C[i][j]) / (1.0 + fabs(t.C[i][j])));
+              d = fmax(d, fabs(c.D[i][j] - t.D[i][j]) / (1.0 + fabs(t.D[i][j])));
+            }
+        }
+      return d;
+    }
+  } // namespace
+
+  double ctable_defect(int degree, const Tables1D &t)
+  {
+    switch (degree)
+      {
+        case 1: return ctable_defect_p<1>(t);
+        case 2: return ctable_defect_p<2>(t);
+        case 3: return ctable_defect_p<3>(t);
+        case 4: return ctable_defect_p<4>(t);
+        case 5: return ctable_defect_p<5>(t);
+        case 6: return ctable_defect_p<6>(t);
+        default: return 1.0;
+      }
+  }
+
   int cells_per_block(int degree, bool fp32) { return cells_per_block_for(degree, fp32); }
 
   int threads_per_block(int degree, bool fp32)

@@ -82,6 +82,8 @@ namespace mpdg
     return degree <= 2 ? 32 : (degree <= 4 ? 16 : (fp32 ? 16 : 8));
   }
   int cells_per_block(int degree, bool fp32);
+  // max difference between the compile-time and the long-double host tables
+  double ctable_defect(int degree, const Tables1D &t);
 
   // Persistent launch shape of the vmult: per colour the number of work items
   // (segment x row x z-chunk) and of workgroups looping over them.
@@ -143,4 +145,22 @@ namespace mpdg
   hipError_t launch_fill_f32(float *x, float v, int64_t n, hipStream_t s);
   hipError_t launch_dot(const double *a, const double *b, int64_t n, double *partials,
                         double *out, unsigned *counter, hipStream_t s);
+
+  // GMRES building blocks (classical Gram-Schmidt over a basis of at most
+  // kMaxBasis vectors stored with leading dimension ld).
+  constexpr int kMaxBasis = 32;
+  template <typename Number>
+  hipError_t launch_multi_dot(const Number *V, int64_t ld, int k, const Number *w, int64_t n,
+                              double *partials, double *out, unsigned *counter, hipStream_t s);
+  template <typename Number>
+  hipError_t launch_multi_axpy(Number *w, const Number *V, int64_t ld, int k, const double *h,
+                               int64_t n, double *partials, double *nrm2, unsigned *counter,
+                               hipStream_t s);
+  template <typename Number>
+  hipError_t launch_scale(Number *v, const Number *w, const double *nrm2, int64_t n, hipStream_t s);
+  template <typename Number>
+  hipError_t launch_diag_scale(Number *z, const Number *v, const Number *d, int64_t n, hipStream_t s);
+  template <typename Number, typename XNumber>
+  hipError_t launch_gmres_update(XNumber *x, const Number *V, int64_t ld, int k, const double *y,
+                                 const Number *d, int64_t n, hipStream_t s);
 } // namespace mpdg

@@ -266,6 +266,129 @@ namespace mpdg
         x[i] = v;
     }
 
+    // h[c] = V_c . w for c < k (V_c = V + c*ld); deterministic, result in out[c].
+    template <typename Number>
+    __global__ void __launch_bounds__(T)
+      multi_dot_kernel(const Number *V, int64_t ld, int k, const Number *w, int64_t n,
+                       double *partials, double *out, unsigned *counter)
+    {
+      __shared__ double sh[kMaxBasis][T / 64];
+      __shared__ bool last;
+      double acc[kMaxBasis];
+#pragma unroll
+      for (int c = 0; c < kMaxBasis; ++c)
+        acc[c] = 0.0;
+      for (int64_t i = blockIdx.x * (int64_t)T + threadIdx.x; i < n; i += (int64_t)gridDim.x * T)
+        {
+          const double wi = (double)w[i];
+#pragma unroll
+          for (int c = 0; c < kMaxBasis; ++c)
+            if (c < k)
+              acc[c] += (double)V[c * ld + i] * wi;
+        }
+      const int wv = threadIdx.x >> 6;
+      for (int c = 0; c < k; ++c)
+        {
+          const double v = wave_sum(acc[c]);
+          if ((threadIdx.x & 63) == 0)
+            sh[c][wv] = v;
+        }
+      __syncthreads();
+      if (threadIdx.x < k)
+        {
+          double s = 0.0;
+          for (int i = 0; i < T / 64; ++i)
+            s += sh[threadIdx.x][i];
+          partials[threadIdx.x * kVecBlocks + blockIdx.x] = s;
+        }
+      __syncthreads();
+      if (threadIdx.x == 0)
+        {
+          __threadfence();
+          last = atomicAdd(counter, 1u) == gridDim.x - 1;
+        }
+      __syncthreads();
+      if (!last)
+        return;
+      __threadfence();
+      // one thread per basis vector adds its partials in block order
+      if (threadIdx.x < k)
+        {
+          double s = 0.0;
+          for (int b = 0; b < (int)gridDim.x; ++b)
+            s += __hip_atomic_load(partials + threadIdx.x * kVecBlocks + b, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+          out[threadIdx.x] = s;
+        }
+      if (threadIdx.x == 0)
+        *counter = 0u;
+    }
+
+    // w -= sum_c h[c] V_c; with nrm2 != nullptr also ||w||^2 -> *nrm2.
+    template <typename Number>
+    __global__ void __launch_bounds__(T)
+      multi_axpy_kernel(Number *w, const Number *V, int64_t ld, int k, const double *h, int64_t n,
+                        double *partials, double *nrm2, unsigned *counter)
+    {
+      __shared__ double hs[kMaxBasis];
+      if (threadIdx.x < k)
+        hs[threadIdx.x] = h[threadIdx.x];
+      __syncthreads();
+      double v[1] = {0.0};
+      for (int64_t i = blockIdx.x * (int64_t)T + threadIdx.x; i < n; i += (int64_t)gridDim.x * T)
+        {
+          double wi = (double)w[i];
+          for (int c = 0; c < k; ++c)
+            wi -= hs[c] * (double)V[c * ld + i];
+          const Number r = (Number)wi;
+          w[i] = r;
+          v[0] += (double)r * (double)r;
+        }
+      if (nrm2 == nullptr)
+        return;
+      double tot[1];
+      if (finish_reduction<1>(v, partials, counter, tot) && threadIdx.x == 0)
+        *nrm2 = tot[0];
+    }
+
+    // v = w / sqrt(nrm2)
+    template <typename Number>
+    __global__ void __launch_bounds__(T)
+      scale_kernel(Number *v, const Number *w, const double *nrm2, int64_t n)
+    {
+      const double inv = 1.0 / sqrt(*nrm2);
+      for (int64_t i = blockIdx.x * (int64_t)T + threadIdx.x; i < n; i += (int64_t)gridDim.x * T)
+        v[i] = (Number)((double)w[i] * inv);
+    }
+
+    // z = D .* v (right Jacobi preconditioner)
+    template <typename Number>
+    __global__ void __launch_bounds__(T)
+      diag_scale_kernel(Number *z, const Number *v, const Number *d, int64_t n)
+    {
+      for (int64_t i = blockIdx.x * (int64_t)T + threadIdx.x; i < n; i += (int64_t)gridDim.x * T)
+        z[i] = v[i] * d[i];
+    }
+
+    // x += D .* (sum_c y[c] V_c)
+    template <typename Number, typename XNumber>
+    __global__ void __launch_bounds__(T)
+      gmres_update_kernel(XNumber *x, const Number *V, int64_t ld, int k, const double *y,
+                          const Number *d, int64_t n)
+    {
+      __shared__ double ys[kMaxBasis];
+      if (threadIdx.x < k)
+        ys[threadIdx.x] = y[threadIdx.x];
+      __syncthreads();
+      for (int64_t i = blockIdx.x * (int64_t)T + threadIdx.x; i < n; i += (int64_t)gridDim.x * T)
+        {
+          double s = 0.0;
+          for (int c = 0; c < k; ++c)
+            s += ys[c] * (double)V[c * ld + i];
+          x[i] = (XNumber)((double)x[i] + (double)d[i] * s);
+        }
+    }
+
     inline int grid_for(int64_t n)
     {
       const int64_t g = (n + T - 1) / T;
@@ -352,6 +475,69 @@ namespace mpdg
     fill_f32_kernel<<<grid_for(n), T, 0, s>>>(x, v, n);
     return hipGetLastError();
   }
+
+  template <typename Number>
+  hipError_t launch_multi_dot(const Number *V, int64_t ld, int k, const Number *w, int64_t n,
+                              double *partials, double *out, unsigned *counter, hipStream_t s)
+  {
+    if (k < 1 || k > kMaxBasis)
+      return hipErrorInvalidValue;
+    multi_dot_kernel<Number><<<kVecBlocks, T, 0, s>>>(V, ld, k, w, n, partials, out, counter);
+    return hipGetLastError();
+  }
+
+  template <typename Number>
+  hipError_t launch_multi_axpy(Number *w, const Number *V, int64_t ld, int k, const double *h,
+                               int64_t n, double *partials, double *nrm2, unsigned *counter,
+                               hipStream_t s)
+  {
+    if (k < 1 || k > kMaxBasis)
+      return hipErrorInvalidValue;
+    multi_axpy_kernel<Number><<<kVecBlocks, T, 0, s>>>(w, V, ld, k, h, n, partials, nrm2, counter);
+    return hipGetLastError();
+  }
+
+  template <typename Number>
+  hipError_t launch_scale(Number *v, const Number *w, const double *nrm2, int64_t n, hipStream_t s)
+  {
+    scale_kernel<Number><<<grid_for(n), T, 0, s>>>(v, w, nrm2, n);
+    return hipGetLastError();
+  }
+
+  template <typename Number>
+  hipError_t launch_diag_scale(Number *z, const Number *v, const Number *d, int64_t n, hipStream_t s)
+  {
+    diag_scale_kernel<Number><<<grid_for(n), T, 0, s>>>(z, v, d, n);
+    return hipGetLastError();
+  }
+
+  template <typename Number, typename XNumber>
+  hipError_t launch_gmres_update(XNumber *x, const Number *V, int64_t ld, int k, const double *y,
+                                 const Number *d, int64_t n, hipStream_t s)
+  {
+    if (k < 1 || k > kMaxBasis)
+      return hipErrorInvalidValue;
+    gmres_update_kernel<Number, XNumber><<<grid_for(n), T, 0, s>>>(x, V, ld, k, y, d, n);
+    return hipGetLastError();
+  }
+
+#define MPDG_GMRES_INST(NUM)                                                                     \
+  template hipError_t launch_multi_dot<NUM>(const NUM *, int64_t, int, const NUM *, int64_t,     \
+                                            double *, double *, unsigned *, hipStream_t);        \
+  template hipError_t launch_multi_axpy<NUM>(NUM *, const NUM *, int64_t, int, const double *,   \
+                                             int64_t, double *, double *, unsigned *,            \
+                                             hipStream_t);                                       \
+  template hipError_t launch_scale<NUM>(NUM *, const NUM *, const double *, int64_t, hipStream_t); \
+  template hipError_t launch_diag_scale<NUM>(NUM *, const NUM *, const NUM *, int64_t, hipStream_t);
+  MPDG_GMRES_INST(double)
+  MPDG_GMRES_INST(float)
+#undef MPDG_GMRES_INST
+  template hipError_t launch_gmres_update<double, double>(double *, const double *, int64_t, int,
+                                                          const double *, const double *, int64_t,
+                                                          hipStream_t);
+  template hipError_t launch_gmres_update<float, float>(float *, const float *, int64_t, int,
+                                                        const double *, const float *, int64_t,
+                                                        hipStream_t);
 
   template hipError_t launch_cg_init<double>(const double *, const double *, double *, double *,
                                              const double *, int64_t, double, double, int,

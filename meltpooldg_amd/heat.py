@@ -68,6 +68,7 @@ class LinearSolverData:
     mixed_precision: bool = False
     inner_rel_tolerance: float = 1e-2
     check_interval: int = 0
+    gmres_restart: int = 30
 
 
 def _ptr(t):
@@ -235,6 +236,7 @@ class LinearSolver:
         prm.mixed_precision = 1 if data.mixed_precision else 0
         prm.inner_rel_tolerance = float(data.inner_rel_tolerance)
         prm.check_interval = int(data.check_interval)
+        prm.gmres_restart = int(getattr(data, "gmres_restart", 30))
         op._check_vec(solution, "float64")
         op._check_vec(rhs, "float64")
         rep = L.SolverReport()
