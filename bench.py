@@ -91,6 +91,8 @@ def main():
     ap.add_argument("--no-solve", action="store_true", help="skip the CG time-to-solution legs")
     ap.add_argument("--mode", choices=["vmult", "copy"], default="vmult",
                     help="copy: time a plain device copy of one DoF vector (HBM counter calibration)")
+    ap.add_argument("--config", type=int, default=1,
+                    help="1: the headline line (default); 0, 2, 3, 4: the other workloads")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU design is "
@@ -105,6 +107,9 @@ def main():
     torch.cuda.set_device(dev)
     if args.mode == "copy":
         return copy_calibration(args, torch, dev)
+    if args.config != 1:
+        import bench_configs
+        return bench_configs.run(args.config, args)
     op = HeatDiffuseOperator(N_CELLS, DEGREE, conductivity_amplitude=K_AMP)
     op.reset_time_increment(DT)
     cnt = algorithmic_counts(N_CELLS, DEGREE)

@@ -1,0 +1,182 @@
+"""Workload runners for the heat-operator configurations other than the
+headline one (bench.py --config N). Each prints one JSON line with the GPU
+measurements, the error against the oracle and, where bounded, the CPU
+oracle's time on this host.
+
+config 0: 16^3 Q2, gas (rho_cp = k = 1), dt = 1e-3, Dirichlet all faces: one
+          vmult and CG + Jacobi to 1e-10 relative residual, FP64.
+config 2: 96^3 Q3, two-phase sphere (radius 0.3, centre 0.5^3) from a nodal
+          smoothed heaviside of width 2 cells, k 1:100 and rho*cp 1:10 (equal
+          cp, liquid inside): FP32 cell loop in FP64 defect-correction CG to
+          1e-8 against the FP64 CG solution.
+config 3: batched solves (see bench_batch.py).
+config 4: 64^3 Q5 convection-diffusion, vortex velocity, dt = 5e-4,
+          k = 1e-2: GMRES(30) + Jacobi, FP64 and FP32-mixed.
+"""
+import json
+import os
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+
+
+def _t(a, torch, dtype=None):
+    return torch.tensor(a, dtype=dtype or torch.float64, device="cuda")
+
+
+def _oracle_vmult_check(pb, op, torch, threads=16):
+    """One full-size vmult against the oracle (relative max-norm error) and the
+    oracle's own time for it."""
+    from oracle import cpu
+    cpu.set_threads(threads)
+    u = pb.random(0)
+    t0 = time.perf_counter()
+    ref = pb.vmult(u)
+    t_cpu = time.perf_counter() - t0
+    src = _t(u, torch)
+    dst = torch.empty_like(src)
+    op.vmult(dst, src)
+    torch.cuda.synchronize()
+    err = float(np.abs(dst.cpu().numpy() - ref).max() / np.abs(ref).max())
+    return err, t_cpu
+
+
+def _vmult_time(op, torch, dtype, n_rep=20):
+    N = op.n_dofs()
+    src = torch.rand(N, dtype=dtype, device="cuda")
+    dst = torch.empty_like(src)
+    op.vmult(dst, src)
+    ms = op.time_vmult(dst, src, n_rep)
+    return ms
+
+
+def _solve(op, b, torch, **kw):
+    from meltpooldg_amd import LinearSolver, LinearSolverData
+    x = torch.zeros_like(b)
+    data = LinearSolverData(preconditioner_type="Diagonal", **kw)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    LinearSolver.solve(op, x, b, data)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    rep = dict(LinearSolver.last_report)
+    rep["wall_s"] = t
+    return x, rep
+
+
+def config0(args, torch):
+    from meltpooldg_amd import HeatDiffuseOperator
+    from oracle import cpu
+    n, p = 16, 2
+    op = HeatDiffuseOperator(n, p)
+    op.reset_time_increment(1e-3)
+    pb = cpu.Problem(n, p, dt=1e-3)
+    err, t_cpu_vmult = _oracle_vmult_check(pb, op, torch)
+    b = pb.random(1)
+    t0 = time.perf_counter()
+    ref = pb.solve(b, rel_tol=1e-10, inv_diag=pb.inverse_diagonal())
+    t_cpu_solve = time.perf_counter() - t0
+    x, rep = _solve(op, _t(b, torch), torch, solver_type="CG", rel_tolerance=1e-10)
+    xerr = float(np.abs(x.cpu().numpy() - ref["x"]).max() / np.abs(ref["x"]).max())
+    ms = _vmult_time(op, torch, torch.float64)
+    return {"config": 0, "workload": "16^3 Q2 gas, CG+Jacobi to 1e-10, FP64", "n_dofs": op.n_dofs(),
+            "vmult_rel_err_vs_oracle": err, "gpu_vmult_ms": ms, "cpu_vmult_ms": t_cpu_vmult * 1e3,
+            "gpu_cg_iterations": rep["iterations"], "oracle_cg_iterations": ref["iterations"],
+            "gpu_cg_s": rep["wall_s"], "cpu_cg_s": t_cpu_solve, "solution_rel_err": xerr}
+
+
+def sphere_heaviside(n, p, radius=0.3, centre=0.5, width_cells=2.0):
+    """Nodal smoothed heaviside of 0.3 - |x - c| with eps = width/2 cells
+    (CharacteristicFunctions::smoothed_heaviside,
+    include/meltpooldg/utilities/characteristic_functions.hpp:17-28)."""
+    n1 = n * p + 1
+    x = np.linspace(0.0, 1.0, n1)
+    # GLL support points inside each cell
+    from oracle import tables as T
+    gll = T.gll(p)
+    coords = np.concatenate([(c + gll[:-1]) / n for c in range(n)] + [[1.0]])
+    eps = 0.5 * width_cells / n
+    Z, Y, X = np.meshgrid(coords, coords, coords, indexing="ij")
+    d = radius - np.sqrt((X - centre) ** 2 + (Y - centre) ** 2 + (Z - centre) ** 2)
+    H = np.where(d > eps, 1.0, np.where(d <= -eps, 0.0,
+                 (d + eps) / (2 * eps) + np.sin(np.pi * d / eps) / (2 * np.pi)))
+    return H.reshape(-1), x
+
+
+def config2(args, torch):
+    from meltpooldg_amd import HeatDiffuseOperator, MaterialData, PhaseData
+    from oracle import cpu
+    n, p = 96, 3
+    H, _ = sphere_heaviside(n, p)
+    mat = MaterialData(type="gas_liquid", gas=PhaseData(1.0, 1.0, 1.0),
+                       liquid=PhaseData(10.0, 1.0, 100.0))
+    results = {"config": 2, "workload": "96^3 Q3 two-phase sphere, k 1:100, rho*cp 1:10, CG+Jacobi to 1e-8"}
+    op = HeatDiffuseOperator(n, p, material=mat)
+    op.reset_time_increment(1e-3)
+    op.set_heaviside(_t(H, torch))
+    op32 = HeatDiffuseOperator(n, p, material=mat, precision="fp32")
+    op32.reset_time_increment(1e-3)
+    op32.set_heaviside(_t(H, torch))
+    pb = cpu.Problem(n, p, dt=1e-3, material={"type": "gas_liquid", "gas_rho": 1.0, "gas_cp": 1.0,
+                                              "gas_k": 1.0, "liq_rho": 10.0, "liq_cp": 1.0,
+                                              "liq_k": 100.0}, heaviside=H)
+    err, t_cpu_vmult = _oracle_vmult_check(pb, op, torch)
+    b = _t(pb.random(1), torch)
+    x64, rep64 = _solve(op, b, torch, solver_type="CG", rel_tolerance=1e-8)
+    xmx, repmx = _solve(op, b, torch, solver_type="CG", rel_tolerance=1e-8, mixed_precision=True)
+    # reference: FP64 solve to 1e-12 (the discrete solution the 1e-8 solves approximate)
+    xref, repref = _solve(op, b, torch, solver_type="CG", rel_tolerance=1e-12)
+    e = lambda x: float((x - xref).abs().max() / xref.abs().max())
+    results.update({"n_dofs": op.n_dofs(), "vmult_rel_err_vs_oracle": err,
+                    "cpu_vmult_ms": t_cpu_vmult * 1e3,
+                    "gpu_vmult_ms_fp64": _vmult_time(op, torch, torch.float64),
+                    "gpu_vmult_ms_fp32": _vmult_time(op32, torch, torch.float32),
+                    "fp64": {"s": rep64["wall_s"], "iterations": rep64["iterations"],
+                             "err_vs_fp64_1e-12": e(x64)},
+                    "fp32_mixed": {"s": repmx["wall_s"], "inner_iterations": repmx["iterations"],
+                                   "outer_iterations": repmx["outer_iterations"],
+                                   "err_vs_fp64_1e-12": e(xmx)},
+                    "speedup_mixed_vs_fp64": rep64["wall_s"] / repmx["wall_s"]})
+    return results
+
+
+def config4(args, torch):
+    from meltpooldg_amd import HeatDiffuseOperator, MaterialData, PhaseData
+    from oracle import cpu
+    n, p = 64, 5
+    mat = MaterialData(gas=PhaseData(1.0, 1.0, 1e-2))
+    op = HeatDiffuseOperator(n, p, material=mat, velocity="vortex", velocity_w0=0.2)
+    op.reset_time_increment(5e-4)
+    op32 = HeatDiffuseOperator(n, p, material=mat, velocity="vortex", velocity_w0=0.2, precision="fp32")
+    op32.reset_time_increment(5e-4)
+    pb = cpu.Problem(n, p, dt=5e-4, material={"type": "gas", "gas_rho": 1.0, "gas_cp": 1.0,
+                                              "gas_k": 1e-2}, velocity=True, w0=0.2)
+    err, t_cpu_vmult = _oracle_vmult_check(pb, op, torch)
+    b = _t(pb.random(1), torch)
+    x64, rep64 = _solve(op, b, torch, solver_type="GMRES", rel_tolerance=1e-8)
+    xmx, repmx = _solve(op, b, torch, solver_type="GMRES", rel_tolerance=1e-8, mixed_precision=True)
+    xref, _ = _solve(op, b, torch, solver_type="GMRES", rel_tolerance=1e-12)
+    e = lambda x: float((x - xref).abs().max() / xref.abs().max())
+    return {"config": 4, "workload": "64^3 Q5 convection-diffusion (vortex), GMRES(30)+Jacobi to 1e-8",
+            "n_dofs": op.n_dofs(), "vmult_rel_err_vs_oracle": err, "cpu_vmult_ms": t_cpu_vmult * 1e3,
+            "gpu_vmult_ms_fp64": _vmult_time(op, torch, torch.float64),
+            "gpu_vmult_ms_fp32": _vmult_time(op32, torch, torch.float32),
+            "fp64": {"s": rep64["wall_s"], "iterations": rep64["iterations"], "err_vs_fp64_1e-12": e(x64)},
+            "fp32_mixed": {"s": repmx["wall_s"], "inner_iterations": repmx["iterations"],
+                           "outer_iterations": repmx["outer_iterations"], "err_vs_fp64_1e-12": e(xmx)},
+            "speedup_mixed_vs_fp64": rep64["wall_s"] / repmx["wall_s"]}
+
+
+def run(config, args):
+    import torch
+    fn = {0: config0, 2: config2, 4: config4}
+    if config == 3:
+        import bench_batch
+        out = bench_batch.run(args, torch)
+    elif config in fn:
+        out = fn[config](args, torch)
+    else:
+        raise SystemExit(f"unknown config {config}")
+    print(json.dumps(out))

@@ -1,26 +1,24 @@
-// CDNA4 cell kernels of the matrix-free heat operator.
+// CDNA4 cell kernels of the matrix-free heat operator: the vmult and the
+// closed-form Jacobi diagonal.
 //
-// One workgroup = one row segment of CPB consecutive cells along x at a fixed
-// (cy, cz). Thread (l, s) owns cell l and "line" s = s0 + N*s1 of that cell.
-// All 1D contractions are done by a single thread over a whole line held in
-// registers (even-odd split), and the three directions are reached by moving
-// the cell tensor through LDS: x-lines (gather), y-lines, z-lines, and back.
-// The global gather/scatter uses x-lines with the cell index fastest across
-// lanes, so a wavefront touches contiguous runs of the lattice rows.
+// vmult_kernel (wavefront-segment form, see its comment) sweeps x-lines,
+// y-lines and z-lines of each cell through a wavefront's own LDS region with
+// the 1D contractions done per lane in registers (even-odd split, compile-time
+// tables). Shared faces are summed deterministically: x-faces inside a
+// workgroup, z-faces inside a work item's z-chunk, and everything else by eight
+// ordered colour passes in which a lattice node is stored by the first pass
+// that touches it and read-modify-written by the later ones.
 //
-// Scatter determinism: cells of a segment add their shared x-faces inside
-// the workgroup (own value + left neighbour's value), segments are coloured by
-// (sx%2, cy%2, cz%2) and the eight colour passes run in order on the stream.
-// A lattice node is stored by the first pass that touches it (even parity in
-// every direction where it is shared) and read-modify-written by the later
-// ones, so dst never needs clearing and every node's sum has a fixed order.
+// diag_kernel is one workgroup per row segment (thread = cell x line) with
+// block barriers between the line orientations; it runs once per solve.
 //
 // Reference behaviour reproduced (source/heat/heat_diffuse_operator.cpp):
 //   vmult / tangent_cell_loop :179-241 (gather with homogeneous constraints
 //   resolved to 0, distribute_local_to_global dropping constrained rows),
 //   tangent_local_cell_operation :955-1086 (evaluate values|gradients,
 //   val = dt^-1 rho_cp u, val_grad = k grad u, + rho_cp grad u . v,
-//   integrate values|gradients), material lookup :1133-1210.
+//   integrate values|gradients), material lookup :1133-1210,
+//   compute_inverse_diagonal_from_matrixfree :271-284.
 #include "ctables.hpp"
 #include "kernels.hpp"
 

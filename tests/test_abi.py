@@ -84,3 +84,18 @@ def test_product_never_imports_oracle():
             if f.endswith((".py", ".cpp", ".hip", ".hpp", ".h")):
                 text = open(os.path.join(dirpath, f)).read()
                 assert "oracle" not in text.lower(), os.path.join(dirpath, f)
+
+
+def test_cpp_adapter_compiles_and_links(lib, tmp_path):
+    """The header-only C++ adapter builds with g++ against the C ABI library and
+    turns status codes into exceptions (error paths need no GPU)."""
+    import subprocess
+    exe = tmp_path / "adapter_check"
+    libdir = os.path.join(ROOT, "meltpooldg_amd")
+    r = subprocess.run(["g++", "-std=c++17", "-O1", "-I", os.path.join(ROOT, "include"),
+                        os.path.join(ROOT, "tests", "adapter_check.cpp"), "-L", libdir,
+                        "-lmpdg_heat", f"-Wl,-rpath,{libdir}", "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0 and "adapter ok" in r.stdout, (r.returncode, r.stdout, r.stderr)
