@@ -146,6 +146,31 @@ void mpdg_solver_params_default(mpdg_solver_params *p);
 /* Thread-local description of the last error on this thread. */
 const char *mpdg_last_error(void);
 
+/* ---- batches of independent solves ---------------------------------------
+ * No single counterpart in the reference (one HeatDiffuseOperation per
+ * simulation); each member is the operator of mpdg_heat_create and each solve
+ * follows mpdg_heat_solve (CG + Jacobi, optional FP32 defect correction). All
+ * members are solved together: one batched vmult launch per degree and colour,
+ * segmented vector kernels, per-solve convergence flags, and the iteration
+ * chunk captured as one hipGraph and replayed. Members must share the material
+ * kind, conductivity field and velocity field; meshes and degrees may differ. */
+typedef struct mpdg_heat_batch mpdg_heat_batch;
+typedef struct {
+  int status;               /* 0 converged, 1 not converged */
+  int iterations;           /* PCG iterations (mixed: total inner iterations) */
+  int outer_iterations;     /* mixed precision: defect-correction steps */
+  double initial_residual;
+  double final_residual;    /* FP64 ||b - A x|| at exit (mixed), PCG residual otherwise */
+  double seconds;           /* time from the start of the batch solve to this member's convergence */
+} mpdg_batch_report;
+mpdg_status mpdg_heat_batch_create(const mpdg_heat_desc *descs, int count, mpdg_heat_batch **out);
+mpdg_status mpdg_heat_batch_destroy(mpdg_heat_batch *b);
+mpdg_status mpdg_heat_batch_set_dt(mpdg_heat_batch *b, double dt);
+int64_t mpdg_heat_batch_n_dofs(const mpdg_heat_batch *b, int member);
+/* x[i], rhs[i]: device pointers of member i; reports: count entries (may be NULL). */
+mpdg_status mpdg_heat_batch_solve(mpdg_heat_batch *b, double *const *x, const double *const *rhs,
+                                  const mpdg_solver_params *params, mpdg_batch_report *reports);
+
 /* Launch tuning of the cell kernels (no counterpart in the reference, whose
  * cell loop is tuned by deal.II's MatrixFree::AdditionalData). Used by the
  * autotuner; results do not depend on it beyond summation order. */

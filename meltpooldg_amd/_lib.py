@@ -70,6 +70,17 @@ class SolverReport(ctypes.Structure):
     ]
 
 
+class BatchReport(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_int),
+        ("iterations", ctypes.c_int),
+        ("outer_iterations", ctypes.c_int),
+        ("initial_residual", ctypes.c_double),
+        ("final_residual", ctypes.c_double),
+        ("seconds", ctypes.c_double),
+    ]
+
+
 class Tuning(ctypes.Structure):
     _fields_ = [("z_layers", ctypes.c_int), ("blocks_per_cu", ctypes.c_int)]
 
@@ -91,6 +102,13 @@ SIGNATURES = {
                                        ctypes.POINTER(SolverReport)]),
     "mpdg_solver_params_default": (None, [ctypes.POINTER(SolverParams)]),
     "mpdg_last_error": (ctypes.c_char_p, []),
+    "mpdg_heat_batch_create": (ctypes.c_int, [ctypes.POINTER(HeatDesc), ctypes.c_int,
+                                              ctypes.POINTER(_VP)]),
+    "mpdg_heat_batch_destroy": (ctypes.c_int, [_VP]),
+    "mpdg_heat_batch_set_dt": (ctypes.c_int, [_VP, ctypes.c_double]),
+    "mpdg_heat_batch_n_dofs": (ctypes.c_int64, [_VP, ctypes.c_int]),
+    "mpdg_heat_batch_solve": (ctypes.c_int, [_VP, ctypes.POINTER(_VP), ctypes.POINTER(_VP),
+                                             ctypes.POINTER(SolverParams), ctypes.POINTER(BatchReport)]),
     "mpdg_heat_get_tuning": (ctypes.c_int, [_VP, ctypes.POINTER(Tuning)]),
     "mpdg_heat_set_tuning": (ctypes.c_int, [_VP, ctypes.POINTER(Tuning)]),
     "mpdg_heat_time_vmult": (ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int, ctypes.c_int,

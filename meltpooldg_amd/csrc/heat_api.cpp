@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -865,6 +866,578 @@ mpdg_status mpdg_heat_time_vmult(mpdg_heat *h, double *dst, const double *src, i
   MPDG_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
   *ms_per_vmult = ms / repeats;
   return MPDG_OK;
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batches of independent solves
+// ---------------------------------------------------------------------------
+namespace
+{
+  template <typename Number>
+  struct BatchGroup
+  {
+    int degree = 0;
+    int nops = 0;
+    std::vector<int> members;          // solve index of each operator
+    std::vector<int> block_off;        // 8 x (nops + 1), host
+    BatchVmultPlan plan{};
+    int pbase = 0;                     // first vmult partial slot of the group
+    DevBuf<BatchOperator<Number>> ops; // src/dst: p -> q
+    DevBuf<BatchOperator<double>> opsx;// FP64 x -> q (defect-correction residual)
+    DevBuf<int> dev_block_off;
+    DevBuf<int> dev_members;
+  };
+} // namespace
+
+struct mpdg_heat_batch
+{
+  int count = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false; // graphs cannot be captured on the legacy NULL stream
+  std::vector<mpdg_heat *> members;
+  std::vector<int64_t> off;  // member offsets in the concatenated vectors
+  int64_t total = 0;
+  int nblocks = 0;           // workgroups of the segmented vector kernels
+  int coef = 0;
+  bool vel = false;
+  DevBuf<BatchSeg> seg;
+  DevBuf<double> X, B, R, P, Q, D, partials, rel, res;
+  DevBuf<float> R32, P32, Q32, X32, D32;
+  DevBuf<int> active;
+  DevBuf<SolverScalars> scal;
+  SolverScalars *host_scal = nullptr;
+  double *host_res = nullptr;
+  std::vector<BatchGroup<double>> g64;
+  std::vector<BatchGroup<float>> g32;
+  int vmult_slots64 = 0, vmult_slots32 = 0;
+  hipGraphExec_t graph64 = nullptr, graph32 = nullptr;
+  int graph_steps64 = 0, graph_steps32 = 0;
+
+  ~mpdg_heat_batch()
+  {
+    if (own_stream && stream)
+      (void)hipStreamSynchronize(stream);
+    if (graph64)
+      (void)hipGraphExecDestroy(graph64);
+    if (graph32)
+      (void)hipGraphExecDestroy(graph32);
+    for (auto *m : members)
+      delete m;
+    for (auto *b : {&X, &B, &R, &P, &Q, &D, &partials, &rel, &res})
+      b->release();
+    for (auto *b : {&R32, &P32, &Q32, &X32, &D32})
+      b->release();
+    active.release();
+    scal.release();
+    seg.release();
+    for (auto &g : g64)
+      {
+        g.ops.release();
+        g.opsx.release();
+        g.dev_block_off.release();
+        g.dev_members.release();
+      }
+    for (auto &g : g32)
+      {
+        g.ops.release();
+        g.opsx.release();
+        g.dev_block_off.release();
+        g.dev_members.release();
+      }
+    if (host_scal)
+      (void)hipHostFree(host_scal);
+    if (host_res)
+      (void)hipHostFree(host_res);
+    if (own_stream && stream)
+      (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace
+{
+  template <typename Number>
+  mpdg_status build_groups(mpdg_heat_batch *B, std::vector<BatchGroup<Number>> &groups,
+                           int &slots, Number *src, Number *dst)
+  {
+    const bool fp32 = sizeof(Number) == 4;
+    groups.clear();
+    std::vector<int> degrees;
+    for (auto *m : B->members)
+      if (std::find(degrees.begin(), degrees.end(), m->desc.degree) == degrees.end())
+        degrees.push_back(m->desc.degree);
+    std::sort(degrees.begin(), degrees.end());
+    slots = 0;
+    for (int deg : degrees)
+      {
+        BatchGroup<Number> g;
+        g.degree = deg;
+        std::vector<int> items;
+        std::vector<BatchOperator<Number>> ops;
+        std::vector<BatchOperator<double>> opsx;
+        const int cpb = vmult_cells_per_block(deg, fp32);
+        const int cpb64 = vmult_cells_per_block(deg, false);
+        for (int s = 0; s < B->count; ++s)
+          {
+            mpdg_heat *m = B->members[s];
+            if (m->desc.degree != deg)
+              continue;
+            const CellParams<Number> &prm = *reinterpret_cast<const CellParams<Number> *>(
+              fp32 ? (const void *)&m->prm32 : (const void *)&m->prm64);
+            const VmultGrid vg = vmult_grid(prm.nx, prm.ny, prm.nz, cpb, prm.zlayers, 1 << 30);
+            BatchOperator<Number> op{};
+            op.prm = prm;
+            op.src = src + B->off[s];
+            op.dst = dst + B->off[s];
+            op.heav = fp32 ? (const Number *)(const void *)m->heav32.ptr :
+                             (const Number *)(const void *)m->heav64.ptr;
+            op.solve = s;
+            for (int c = 0; c < 8; ++c)
+              {
+                op.items[c] = vg.items[c];
+                items.push_back(vg.items[c]);
+              }
+            ops.push_back(op);
+            BatchOperator<double> ox{};
+            ox.prm = m->prm64;
+            ox.src = B->X.ptr + B->off[s];
+            ox.dst = B->Q.ptr + B->off[s];
+            ox.heav = m->heav64.ptr;
+            ox.solve = s;
+            const VmultGrid vg64 = vmult_grid(m->prm64.nx, m->prm64.ny, m->prm64.nz, cpb64,
+                                              m->prm64.zlayers, 1 << 30);
+            for (int c = 0; c < 8; ++c)
+              ox.items[c] = vg64.items[c];
+            opsx.push_back(ox);
+            g.members.push_back(s);
+          }
+        g.nops = (int)ops.size();
+        g.block_off.assign(8 * (g.nops + 1), 0);
+        const int max_blocks = vmult_resident_blocks(deg, fp32, B->coef, B->vel) * device_cu_count();
+        g.plan = batch_vmult_plan(items.data(), g.nops, max_blocks, g.block_off.data());
+        g.pbase = slots;
+        slots += g.plan.total_slots;
+        MPDG_HIP(g.ops.alloc(g.nops));
+        MPDG_HIP(g.opsx.alloc(g.nops));
+        MPDG_HIP(g.dev_block_off.alloc(g.block_off.size()));
+        MPDG_HIP(g.dev_members.alloc(g.nops));
+        MPDG_HIP(hipMemcpy(g.ops.ptr, ops.data(), ops.size() * sizeof(ops[0]), hipMemcpyHostToDevice));
+        MPDG_HIP(hipMemcpy(g.opsx.ptr, opsx.data(), opsx.size() * sizeof(opsx[0]), hipMemcpyHostToDevice));
+        MPDG_HIP(hipMemcpy(g.dev_block_off.ptr, g.block_off.data(), g.block_off.size() * sizeof(int),
+                           hipMemcpyHostToDevice));
+        MPDG_HIP(hipMemcpy(g.dev_members.ptr, g.members.data(), g.members.size() * sizeof(int),
+                           hipMemcpyHostToDevice));
+        groups.push_back(std::move(g));
+      }
+    return MPDG_OK;
+  }
+
+  // q = A p for every member whose PCG is still running, then alpha per member.
+  template <typename Number>
+  mpdg_status batch_vmult_pq(mpdg_heat_batch *B, std::vector<BatchGroup<Number>> &groups)
+  {
+    for (auto &g : groups)
+      MPDG_HIP(launch_vmult_batch<Number>(g.degree, B->coef, B->vel, g.ops.ptr, g.dev_block_off.ptr,
+                                          g.nops, g.plan, B->partials.ptr + g.pbase, B->scal.ptr,
+                                          B->stream));
+    for (auto &g : groups)
+      MPDG_HIP(launch_batch_pap_finalize(g.dev_block_off.ptr, g.dev_members.ptr, g.nops, g.plan,
+                                         B->partials.ptr + g.pbase, B->scal.ptr, B->stream));
+    return MPDG_OK;
+  }
+
+  template <typename Number>
+  mpdg_status batch_pcg_steps(mpdg_heat_batch *B, std::vector<BatchGroup<Number>> &groups,
+                              int steps, int vslots, Number *x, Number *r, Number *p, Number *q,
+                              const Number *d)
+  {
+    double *vpart = B->partials.ptr + vslots;
+    for (int k = 0; k < steps; ++k)
+      {
+        if (mpdg_status st = batch_vmult_pq<Number>(B, groups))
+          return st;
+        MPDG_HIP(launch_batch_cg_update<Number>(x, r, p, q, d, B->seg.ptr, B->count, B->nblocks,
+                                                vpart, B->scal.ptr, B->stream));
+        MPDG_HIP(launch_batch_cg_direction<Number>(p, r, d, B->seg.ptr, B->count, B->nblocks,
+                                                   B->scal.ptr, B->stream));
+      }
+    return MPDG_OK;
+  }
+
+  // Captures `steps` PCG iterations of the whole batch into one executable graph.
+  template <typename Number>
+  mpdg_status batch_graph(mpdg_heat_batch *B, std::vector<BatchGroup<Number>> &groups, int steps,
+                          int vslots, Number *x, Number *r, Number *p, Number *q, const Number *d,
+                          hipGraphExec_t *exec)
+  {
+    hipGraph_t graph = nullptr;
+    MPDG_HIP(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
+    mpdg_status st = batch_pcg_steps<Number>(B, groups, steps, vslots, x, r, p, q, d);
+    const hipError_t e = hipStreamEndCapture(B->stream, &graph);
+    if (st)
+      {
+        if (graph)
+          (void)hipGraphDestroy(graph);
+        return st;
+      }
+    if (e != hipSuccess)
+      return fail(MPDG_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    const hipError_t e2 = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e2 != hipSuccess)
+      return fail(MPDG_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e2));
+    return MPDG_OK;
+  }
+
+  mpdg_status read_batch_scalars(mpdg_heat_batch *B)
+  {
+    MPDG_HIP(hipMemcpyAsync(B->host_scal, B->scal.ptr, B->count * sizeof(SolverScalars),
+                            hipMemcpyDeviceToHost, B->stream));
+    MPDG_HIP(hipStreamSynchronize(B->stream));
+    return MPDG_OK;
+  }
+} // namespace
+
+extern "C" {
+
+mpdg_status mpdg_heat_batch_create(const mpdg_heat_desc *descs, int count, mpdg_heat_batch **out)
+{
+  if (!descs || !out || count < 1)
+    return fail(MPDG_ERR_ARGUMENT, "batch needs a descriptor array and count >= 1");
+  *out = nullptr;
+  for (int s = 1; s < count; ++s)
+    if (descs[s].material != descs[0].material ||
+        descs[s].conductivity_field != descs[0].conductivity_field ||
+        descs[s].velocity != descs[0].velocity || descs[s].stream != descs[0].stream)
+      return fail(MPDG_ERR_UNSUPPORTED,
+                  "batch members must share material kind, conductivity and velocity fields and stream");
+  if (descs[0].material == MPDG_MATERIAL_GAS_LIQUID)
+    return fail(MPDG_ERR_UNSUPPORTED, "batches of gas_liquid operators are not built");
+  mpdg_heat_batch *B = new (std::nothrow) mpdg_heat_batch();
+  if (!B)
+    return fail(MPDG_ERR_ARGUMENT, "out of host memory");
+  B->count = count;
+  B->stream = (hipStream_t)descs[0].stream;
+  if (!B->stream)
+    {
+      if (hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) != hipSuccess)
+        {
+          delete B;
+          return fail(MPDG_ERR_HIP, "could not create the batch stream");
+        }
+      B->own_stream = true;
+    }
+  for (int s = 0; s < count; ++s)
+    {
+      mpdg_heat *h = nullptr;
+      mpdg_heat_desc d = descs[s];
+      d.stream = B->stream;
+      if (mpdg_status st = mpdg_heat_create(&d, &h))
+        {
+          delete B;
+          return st;
+        }
+      B->members.push_back(h);
+    }
+  B->coef = coef_mode(B->members[0]);
+  B->vel = descs[0].velocity != 0;
+  std::vector<BatchSeg> seg(count);
+  int64_t off = 0;
+  int blk = 0;
+  for (int s = 0; s < count; ++s)
+    {
+      const int64_t n = B->members[s]->N;
+      B->off.push_back(off);
+      seg[s].off = off;
+      seg[s].n = n;
+      seg[s].blk0 = blk;
+      seg[s].nblk = (int)std::max<int64_t>(1, std::min<int64_t>(256, (n + 4 * kBatchThreads - 1) /
+                                                                        (4 * kBatchThreads)));
+      blk += seg[s].nblk;
+      off += n;
+    }
+  B->total = off;
+  B->nblocks = blk;
+  hipError_t e = B->seg.alloc(count);
+  for (auto *buf : {&B->X, &B->B, &B->R, &B->P, &B->Q, &B->D})
+    if (e == hipSuccess)
+      e = buf->alloc(B->total);
+  for (auto *buf : {&B->R32, &B->P32, &B->Q32, &B->X32, &B->D32})
+    if (e == hipSuccess)
+      e = buf->alloc(B->total);
+  if (e == hipSuccess)
+    e = B->rel.alloc(count);
+  if (e == hipSuccess)
+    e = B->res.alloc(2 * count);
+  if (e == hipSuccess)
+    e = B->active.alloc(count);
+  if (e == hipSuccess)
+    e = B->scal.alloc(count);
+  if (e == hipSuccess)
+    e = hipMemcpy(B->seg.ptr, seg.data(), count * sizeof(BatchSeg), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    {
+      std::vector<SolverScalars> zero(count);
+      std::memset((void *)zero.data(), 0, count * sizeof(SolverScalars));
+      e = hipMemcpy(B->scal.ptr, zero.data(), count * sizeof(SolverScalars), hipMemcpyHostToDevice);
+    }
+  if (e == hipSuccess)
+    {
+      std::vector<double> zero(2 * count, 0.0);
+      e = hipMemcpy(B->res.ptr, zero.data(), 2 * count * sizeof(double), hipMemcpyHostToDevice);
+    }
+  if (e == hipSuccess)
+    e = hipHostMalloc((void **)&B->host_scal, count * sizeof(SolverScalars), hipHostMallocDefault);
+  if (e == hipSuccess)
+    e = hipHostMalloc((void **)&B->host_res, 2 * count * sizeof(double), hipHostMallocDefault);
+  if (e != hipSuccess)
+    {
+      delete B;
+      return fail(MPDG_ERR_HIP, std::string("batch set-up: ") + hipGetErrorString(e));
+    }
+  *out = B;
+  return MPDG_OK;
+}
+
+mpdg_status mpdg_heat_batch_destroy(mpdg_heat_batch *b)
+{
+  if (!b)
+    return MPDG_OK;
+  if (b->stream)
+    (void)hipStreamSynchronize(b->stream);
+  else
+    (void)hipDeviceSynchronize();
+  delete b;
+  return MPDG_OK;
+}
+
+int64_t mpdg_heat_batch_n_dofs(const mpdg_heat_batch *b, int member)
+{
+  if (!b || member < 0 || member >= b->count)
+    return -1;
+  return b->members[member]->N;
+}
+
+mpdg_status mpdg_heat_batch_set_dt(mpdg_heat_batch *b, double dt)
+{
+  if (!b)
+    return fail(MPDG_ERR_ARGUMENT, "null batch");
+  for (auto *m : b->members)
+    if (mpdg_status st = mpdg_heat_set_dt(m, dt))
+      return st;
+  // operator descriptors carry dt-dependent constants: rebuild them and the graphs
+  if (b->graph64)
+    (void)hipGraphExecDestroy(b->graph64);
+  if (b->graph32)
+    (void)hipGraphExecDestroy(b->graph32);
+  b->graph64 = b->graph32 = nullptr;
+  if (mpdg_status st = build_groups<double>(b, b->g64, b->vmult_slots64, b->P.ptr, b->Q.ptr))
+    return st;
+  if (mpdg_status st = build_groups<float>(b, b->g32, b->vmult_slots32, b->P32.ptr, b->Q32.ptr))
+    return st;
+  const size_t need = (size_t)std::max(b->vmult_slots64, b->vmult_slots32) + 2 * (size_t)b->nblocks;
+  MPDG_HIP(b->partials.alloc(need));
+  return MPDG_OK;
+}
+
+mpdg_status mpdg_heat_batch_solve(mpdg_heat_batch *b, double *const *x, const double *const *rhs,
+                                  const mpdg_solver_params *params, mpdg_batch_report *reports)
+{
+  if (!b || !x || !rhs)
+    return fail(MPDG_ERR_ARGUMENT, "null argument");
+  if (b->g64.empty())
+    return fail(MPDG_ERR_ZERO_TIME_INCREMENT, "call mpdg_heat_batch_set_dt first");
+  mpdg_solver_params prm;
+  if (params)
+    prm = *params;
+  else
+    mpdg_solver_params_default(&prm);
+  if (prm.solver_type != MPDG_SOLVER_CG)
+    return fail(MPDG_ERR_UNSUPPORTED, "batched solves use CG");
+  const int S = b->count;
+  const bool mixed = prm.mixed_precision != 0;
+  const int steps = prm.check_interval > 0 ? prm.check_interval : 8;
+  if (b->own_stream)
+    MPDG_HIP(hipDeviceSynchronize()); // inputs were produced on other streams
+  const auto t0 = std::chrono::steady_clock::now();
+  auto now = [&]() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  };
+  std::vector<mpdg_batch_report> rep(S);
+  for (auto &r : rep)
+    {
+      std::memset(&r, 0, sizeof(r));
+      r.status = 1;
+      r.seconds = -1.0;
+    }
+
+  // Jacobi diagonals (JacobiPreconditioner::update per member) and inputs
+  for (int s = 0; s < S; ++s)
+    {
+      mpdg_heat *m = b->members[s];
+      if (mpdg_status st = diagonal_inverse(m, b->D.ptr + b->off[s]))
+        return st;
+      MPDG_HIP(hipMemcpyAsync(b->B.ptr + b->off[s], rhs[s], m->N * sizeof(double),
+                              hipMemcpyDeviceToDevice, b->stream));
+      MPDG_HIP(hipMemcpyAsync(b->X.ptr + b->off[s], x[s], m->N * sizeof(double),
+                              hipMemcpyDeviceToDevice, b->stream));
+    }
+  std::vector<double> relv(S, prm.rel_tolerance);
+  MPDG_HIP(hipMemcpyAsync(b->rel.ptr, relv.data(), S * sizeof(double), hipMemcpyHostToDevice, b->stream));
+
+  auto outer_residual = [&]() -> mpdg_status {
+    // R = B - A64 X for every member, ||R|| per member
+    for (auto &g : b->g64)
+      MPDG_HIP(launch_vmult_batch<double>(g.degree, b->coef, b->vel, g.opsx.ptr, g.dev_block_off.ptr,
+                                          g.nops, g.plan, nullptr, nullptr, b->stream));
+    MPDG_HIP(launch_batch_residual(b->R.ptr, b->B.ptr, b->Q.ptr, b->seg.ptr, S, b->nblocks,
+                                   b->partials.ptr, b->res.ptr, b->stream));
+    MPDG_HIP(hipMemcpyAsync(b->host_res, b->res.ptr, S * sizeof(double), hipMemcpyDeviceToHost,
+                            b->stream));
+    MPDG_HIP(hipStreamSynchronize(b->stream));
+    return MPDG_OK;
+  };
+  // opsx was built for the FP64 group layout; its block offsets equal g64's
+  // (same degree and precision), so the same dev_block_off serves both.
+
+  if (!mixed)
+    {
+      if (!b->graph64 || b->graph_steps64 != steps)
+        {
+          if (b->graph64)
+            (void)hipGraphExecDestroy(b->graph64);
+          b->graph64 = nullptr;
+          if (mpdg_status st = batch_graph<double>(b, b->g64, steps, b->vmult_slots64, b->X.ptr,
+                                                   b->R.ptr, b->P.ptr, b->Q.ptr, b->D.ptr, &b->graph64))
+            return st;
+          b->graph_steps64 = steps;
+        }
+      // r = b - A x0 for all members, then the PCG start
+      for (auto &g : b->g64)
+        MPDG_HIP(launch_vmult_batch<double>(g.degree, b->coef, b->vel, g.opsx.ptr, g.dev_block_off.ptr,
+                                            g.nops, g.plan, nullptr, nullptr, b->stream));
+      MPDG_HIP(launch_batch_cg_init<double>(b->B.ptr, b->Q.ptr, b->R.ptr, b->P.ptr, b->D.ptr,
+                                            b->seg.ptr, S, b->nblocks,
+                                            b->partials.ptr + b->vmult_slots64, b->scal.ptr,
+                                            b->rel.ptr, prm.abs_tolerance, prm.max_iterations,
+                                            nullptr, b->stream));
+      for (;;)
+        {
+          if (mpdg_status st = read_batch_scalars(b))
+            return st;
+          bool all = true;
+          const double t = now();
+          for (int s = 0; s < S; ++s)
+            {
+              const SolverScalars &sc = b->host_scal[s];
+              if (sc.done && rep[s].seconds < 0)
+                {
+                  rep[s].seconds = t;
+                  rep[s].status = sc.status == 0 ? 0 : 1;
+                  rep[s].iterations = sc.iteration;
+                  rep[s].initial_residual = sc.res0;
+                  rep[s].final_residual = sc.res;
+                }
+              all = all && sc.done;
+            }
+          if (all)
+            break;
+          MPDG_HIP(hipGraphLaunch(b->graph64, b->stream));
+        }
+    }
+  else
+    {
+      if (!b->graph32 || b->graph_steps32 != steps)
+        {
+          if (b->graph32)
+            (void)hipGraphExecDestroy(b->graph32);
+          b->graph32 = nullptr;
+          if (mpdg_status st = batch_graph<float>(b, b->g32, steps, b->vmult_slots32, b->X32.ptr,
+                                                  b->R32.ptr, b->P32.ptr, b->Q32.ptr, b->D32.ptr,
+                                                  &b->graph32))
+            return st;
+          b->graph_steps32 = steps;
+        }
+      MPDG_HIP(launch_to_f32(b->D32.ptr, b->D.ptr, b->total, b->stream));
+      std::vector<double> tol(S, 0.0);
+      std::vector<int> act(S, 1), inner_total(S, 0), outer(S, 0);
+      for (int round = 0;; ++round)
+        {
+          if (mpdg_status st = outer_residual())
+            return st;
+          const double t = now();
+          bool any = false;
+          for (int s = 0; s < S; ++s)
+            {
+              const double r = b->host_res[s];
+              if (round == 0)
+                {
+                  tol[s] = std::max(prm.abs_tolerance, prm.rel_tolerance * r);
+                  rep[s].initial_residual = r;
+                }
+              rep[s].final_residual = r;
+              if (act[s] && (r <= tol[s] || !std::isfinite(r) || inner_total[s] >= prm.max_iterations))
+                {
+                  act[s] = 0;
+                  rep[s].seconds = t;
+                  rep[s].status = r <= tol[s] ? 0 : 1;
+                }
+              if (act[s])
+                {
+                  relv[s] = std::max(prm.inner_rel_tolerance, 0.5 * tol[s] / r);
+                  any = true;
+                  ++outer[s];
+                }
+            }
+          if (!any)
+            break;
+          MPDG_HIP(hipMemcpyAsync(b->rel.ptr, relv.data(), S * sizeof(double), hipMemcpyHostToDevice,
+                                  b->stream));
+          MPDG_HIP(hipMemcpyAsync(b->active.ptr, act.data(), S * sizeof(int), hipMemcpyHostToDevice,
+                                  b->stream));
+          MPDG_HIP(launch_fill_f32(b->X32.ptr, 0.0f, b->total, b->stream));
+          MPDG_HIP(launch_batch_cg_init<float>(b->R.ptr, nullptr, b->R32.ptr, b->P32.ptr, b->D32.ptr,
+                                               b->seg.ptr, S, b->nblocks,
+                                               b->partials.ptr + b->vmult_slots32, b->scal.ptr,
+                                               b->rel.ptr, 0.0, prm.max_iterations, b->active.ptr,
+                                               b->stream));
+          for (;;)
+            {
+              if (mpdg_status st = read_batch_scalars(b))
+                return st;
+              bool all = true;
+              for (int s = 0; s < S; ++s)
+                all = all && b->host_scal[s].done;
+              if (all)
+                break;
+              MPDG_HIP(hipGraphLaunch(b->graph32, b->stream));
+            }
+          for (int s = 0; s < S; ++s)
+            if (act[s])
+              inner_total[s] += b->host_scal[s].iteration;
+          MPDG_HIP(launch_add_f32(b->X.ptr, b->X32.ptr, b->total, b->stream));
+        }
+      for (int s = 0; s < S; ++s)
+        {
+          rep[s].iterations = inner_total[s];
+          rep[s].outer_iterations = outer[s];
+        }
+    }
+  for (int s = 0; s < S; ++s)
+    MPDG_HIP(hipMemcpyAsync(x[s], b->X.ptr + b->off[s], b->members[s]->N * sizeof(double),
+                            hipMemcpyDeviceToDevice, b->stream));
+  MPDG_HIP(hipStreamSynchronize(b->stream));
+  bool ok = true;
+  for (int s = 0; s < S; ++s)
+    ok = ok && rep[s].status == 0;
+  const double t_end = now();
+  for (int s = 0; s < S; ++s)
+    if (rep[s].seconds < 0)
+      rep[s].seconds = t_end;
+  if (reports)
+    std::copy(rep.begin(), rep.end(), reports);
+  return ok ? MPDG_OK : fail(MPDG_ERR_NOT_CONVERGED, "at least one batch member did not converge");
 }
 
 } // extern "C"

@@ -386,13 +386,16 @@ namespace mpdg
     // the chunk's outer z-faces take part in the colouring). The next layer's
     // x-lines and the current layer's read-modify-write values are loaded into
     // registers early so their latency overlaps the sum-factorisation work.
-    template <int P, typename Number, int COEF, bool VEL, int CW, int R, int WPB, int MINW>
-    __global__ void __launch_bounds__(64 * WPB, MINW)
-      vmult_kernel(const Number *__restrict__ src, Number *__restrict__ dst,
-                   const Number *__restrict__ heav, const VmultTables<Number> tab,
-                   const CellParams<Number> prm, const int colour, const int layers,
-                   const int items, double *__restrict__ partials, const int partial_offset,
-                   const int partial_total, const int finalize, SolverScalars *__restrict__ scal)
+    // Body of the vmult: this workgroup processes items item_first,
+    // item_first + item_stride, ... (< items) of one colour of one operator and
+    // returns this thread's share of p . Ap (sum over its cells of u . y).
+    template <int P, typename Number, int COEF, bool VEL, int CW, int R, int WPB>
+    __device__ __forceinline__ double vmult_body(const Number *__restrict__ src,
+                                                 Number *__restrict__ dst,
+                                                 const Number *__restrict__ heav,
+                                                 const CellParams<Number> &prm, const int colour,
+                                                 const int layers, const int items,
+                                                 const int item_first, const int item_stride)
     {
       constexpr int N = P + 1;
       constexpr int NL = N * N;
@@ -407,8 +410,6 @@ namespace mpdg
       constexpr ct::EO<N> TC = ct::VTab<P>::C;
       constexpr ct::EO<N> TCt = ct::VTab<P>::Ct;
       constexpr ct::Tab<N> T1 = ct::VTab<P>::t;
-      if (scal != nullptr && scal->done)
-        return;
 
       __shared__ Number A[CPB * STR];
       __shared__ Number B[CPB * STR];
@@ -416,7 +417,6 @@ namespace mpdg
       __shared__ Number RB[(H2 && VEL) ? CPB * STR : 1];
       __shared__ Number FZ[2][CPB * NL]; // z-face carry [cell][b][a], by layer parity
       __shared__ Number XE[2][CPB * NL]; // right x-face of each cell line [cell][line]
-      __shared__ double red[WPB];
 
       const uint32_t vbytes = (uint32_t)prm.n1x * prm.n1y * prm.n1z * sizeof(Number);
       const __amdgpu_buffer_rsrc_t rsrc = make_rsrc(src, vbytes);
@@ -441,7 +441,7 @@ namespace mpdg
           cb[r] = w * CW + l;
         }
 
-      int item = blockIdx.x;
+      int item = item_first;
       int j = 0;
       Item it = decode_item<Number, CPB>(prm, colour, layers, item);
       Number u[R][N], hl[R][N];
@@ -462,7 +462,7 @@ namespace mpdg
           Item it_n = it;
           if (j_n >= it.nl)
             {
-              item_n = item + gridDim.x;
+              item_n = item + item_stride;
               j_n = 0;
               if (item_n < items)
                 it_n = decode_item<Number, CPB>(prm, colour, layers, item_n);
@@ -787,9 +787,26 @@ namespace mpdg
           wave_sync();
         }
 
+      return acc;
+    }
+
+    template <int P, typename Number, int COEF, bool VEL, int CW, int R, int WPB, int MINW>
+    __global__ void __launch_bounds__(64 * WPB, MINW)
+      vmult_kernel(const Number *__restrict__ src, Number *__restrict__ dst,
+                   const Number *__restrict__ heav, const CellParams<Number> prm,
+                   const int colour, const int layers, const int items,
+                   double *__restrict__ partials, const int partial_offset,
+                   const int partial_total, const int finalize, SolverScalars *__restrict__ scal)
+    {
+      constexpr int THREADS = 64 * WPB;
+      if (scal != nullptr && scal->done)
+        return;
+      const double acc = vmult_body<P, Number, COEF, VEL, CW, R, WPB>(
+        src, dst, heav, prm, colour, layers, items, blockIdx.x, gridDim.x);
       if (partials != nullptr)
         {
           // p . Ap: sum over cells of u_cell . y_cell (constrained entries are 0)
+          __shared__ double red[WPB];
           const double bs = block_sum<THREADS>(acc, red);
           __shared__ bool is_last;
           if (threadIdx.x == 0)
@@ -816,6 +833,50 @@ namespace mpdg
                   scal->counter = 0u;
                 }
             }
+        }
+    }
+
+    // Batched vmult: the workgroups of one launch are split among the
+    // operators of a batch (one colour, one degree); workgroup b serves
+    // operator s with block_off[s] <= b < block_off[s+1] and loops over that
+    // operator's items of this colour. Each workgroup writes its p . Ap share
+    // to partials[b] (per-operator sums are formed by a separate kernel).
+    template <int P, typename Number, int COEF, bool VEL, int CW, int R, int WPB, int MINW>
+    __global__ void __launch_bounds__(64 * WPB, MINW)
+      vmult_batch_kernel(const BatchOperator<Number> *__restrict__ ops, const int *__restrict__ block_off,
+                         const int nops, const int colour, double *__restrict__ partials,
+                         const SolverScalars *__restrict__ scal)
+    {
+      constexpr int THREADS = 64 * WPB;
+      __shared__ int s_sh;
+      if (threadIdx.x == 0)
+        {
+          int lo = 0, hi = nops; // last s with block_off[s] <= blockIdx.x
+          while (hi - lo > 1)
+            {
+              const int mid = (lo + hi) / 2;
+              if (block_off[mid] <= (int)blockIdx.x)
+                lo = mid;
+              else
+                hi = mid;
+            }
+          s_sh = lo;
+        }
+      __syncthreads();
+      const int s = s_sh;
+      const BatchOperator<Number> &op = ops[s];
+      double acc = 0.0;
+      const bool skip = scal != nullptr && scal[op.solve].done;
+      if (!skip)
+        acc = vmult_body<P, Number, COEF, VEL, CW, R, WPB>(
+          op.src, op.dst, op.heav, op.prm, colour, op.prm.zlayers, op.items[colour],
+          blockIdx.x - block_off[s], block_off[s + 1] - block_off[s]);
+      if (partials != nullptr)
+        {
+          __shared__ double red[WPB];
+          const double bs = block_sum<THREADS>(acc, red);
+          if (threadIdx.x == 0)
+            partials[blockIdx.x] = bs;
         }
     }
 
@@ -1103,7 +1164,7 @@ namespace mpdg
         {
           if (vg.grid[c] == 0)
             continue;
-          kernel<<<vg.grid[c], THREADS, 0, stream>>>(src, dst, heav, tab, prm, c, prm.zlayers,
+          kernel<<<vg.grid[c], THREADS, 0, stream>>>(src, dst, heav, prm, c, prm.zlayers,
                                                      vg.items[c], partials, vg.offset[c], vg.total,
                                                      c == vg.last ? 1 : 0, scal);
         }
@@ -1281,6 +1342,175 @@ namespace mpdg
         default: return hipErrorInvalidValue;
       }
   }
+
+  namespace
+  {
+    template <typename Number, int P, int COEF, bool VEL>
+    hipError_t vmult_batch_passes(const BatchOperator<Number> *ops, const int *block_off, int nops,
+                                  const BatchVmultPlan &plan, double *partials,
+                                  const SolverScalars *scal, hipStream_t stream)
+    {
+      using Shape = VShape<P, sizeof(Number) == 4>;
+      constexpr int THREADS = 64 * Shape::WPB;
+      auto kernel = vmult_batch_kernel<P, Number, COEF, VEL, Shape::CW, Shape::R, Shape::WPB, Shape::MINW>;
+      for (int c = 0; c < 8; ++c)
+        {
+          if (plan.grid[c] == 0)
+            continue;
+          kernel<<<plan.grid[c], THREADS, 0, stream>>>(ops, block_off + c * (nops + 1), nops, c,
+                                                       partials ? partials + plan.pofs[c] : nullptr,
+                                                       scal);
+        }
+      return hipGetLastError();
+    }
+
+    template <typename Number, int P>
+    hipError_t vmult_batch_deg(int coef, bool vel, const BatchOperator<Number> *ops,
+                               const int *block_off, int nops, const BatchVmultPlan &plan,
+                               double *partials, const SolverScalars *scal, hipStream_t s)
+    {
+      switch (coef * 2 + (vel ? 1 : 0))
+        {
+          case 0: return vmult_batch_passes<Number, P, 0, false>(ops, block_off, nops, plan, partials, scal, s);
+          case 1: return vmult_batch_passes<Number, P, 0, true>(ops, block_off, nops, plan, partials, scal, s);
+          case 2: return vmult_batch_passes<Number, P, 1, false>(ops, block_off, nops, plan, partials, scal, s);
+          case 3: return vmult_batch_passes<Number, P, 1, true>(ops, block_off, nops, plan, partials, scal, s);
+          case 4: return vmult_batch_passes<Number, P, 2, false>(ops, block_off, nops, plan, partials, scal, s);
+          case 5: return vmult_batch_passes<Number, P, 2, true>(ops, block_off, nops, plan, partials, scal, s);
+          default: return hipErrorInvalidValue;
+        }
+    }
+
+    template <typename Number, int P, int COEF, bool VEL>
+    int resident_of()
+    {
+      using Shape = VShape<P, sizeof(Number) == 4>;
+      auto kernel = vmult_batch_kernel<P, Number, COEF, VEL, Shape::CW, Shape::R, Shape::WPB, Shape::MINW>;
+      int nb = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, 64 * Shape::WPB, 0) != hipSuccess || nb < 1)
+        nb = 1;
+      return nb < kMaxResidentPerCu ? nb : kMaxResidentPerCu;
+    }
+
+    __global__ void batch_pap_finalize_kernel(const int *block_off, const int *ops_solve, int nops,
+                                              BatchVmultPlan plan, const double *partials,
+                                              SolverScalars *scal)
+    {
+      // one workgroup per operator, one thread: fixed order over colours and blocks
+      const int o = blockIdx.x;
+      if (threadIdx.x != 0)
+        return;
+      SolverScalars &sc = scal[ops_solve[o]];
+      if (sc.done)
+        return;
+      double pap = 0.0;
+      for (int c = 0; c < 8; ++c)
+        {
+          const int *bo = block_off + c * (nops + 1);
+          for (int b = bo[o]; b < bo[o + 1]; ++b)
+            pap += partials[plan.pofs[c] + b];
+        }
+      sc.pap = pap;
+      sc.alpha = sc.rz / pap;
+    }
+  } // namespace
+
+  BatchVmultPlan batch_vmult_plan(const int *items, int nops, int max_blocks, int *block_off)
+  {
+    BatchVmultPlan plan{};
+    int slot = 0;
+    for (int c = 0; c < 8; ++c)
+      {
+        long total = 0;
+        for (int o = 0; o < nops; ++o)
+          total += items[o * 8 + c];
+        // items per workgroup so that the colour's launch stays within max_blocks
+        const long per = total > max_blocks ? (total + max_blocks - 1) / max_blocks : 1;
+        int *bo = block_off + c * (nops + 1);
+        bo[0] = 0;
+        for (int o = 0; o < nops; ++o)
+          {
+            const int it = items[o * 8 + c];
+            const int g = it == 0 ? 0 : (int)((it + per - 1) / per);
+            bo[o + 1] = bo[o] + g;
+          }
+        plan.grid[c] = bo[nops];
+        plan.pofs[c] = slot;
+        slot += plan.grid[c];
+      }
+    plan.total_slots = slot;
+    return plan;
+  }
+
+  int vmult_cells_per_block(int degree, bool fp32)
+  {
+#define MPDG_CPB(P)                                                                               \
+  case P:                                                                                         \
+    return fp32 ? VShape<P, true>::CW * VShape<P, true>::WPB : VShape<P, false>::CW * VShape<P, false>::WPB;
+    switch (degree)
+      {
+        MPDG_CPB(1)
+        MPDG_CPB(2)
+        MPDG_CPB(3)
+        MPDG_CPB(4)
+        MPDG_CPB(5)
+        MPDG_CPB(6)
+        default: return 1;
+      }
+#undef MPDG_CPB
+  }
+
+  int vmult_resident_blocks(int degree, bool fp32, int coef, bool vel)
+  {
+    (void)coef;
+    (void)vel;
+#define MPDG_RES(P)                                                                               \
+  case P:                                                                                         \
+    return fp32 ? resident_of<float, P, 0, false>() : resident_of<double, P, 0, false>();
+    switch (degree)
+      {
+        MPDG_RES(1)
+        MPDG_RES(2)
+        MPDG_RES(3)
+        MPDG_RES(4)
+        MPDG_RES(5)
+        MPDG_RES(6)
+        default: return 1;
+      }
+#undef MPDG_RES
+  }
+
+  template <typename Number>
+  hipError_t launch_vmult_batch(int degree, int coef, bool vel, const BatchOperator<Number> *ops,
+                                const int *block_off, int nops, const BatchVmultPlan &plan,
+                                double *partials, const SolverScalars *scal, hipStream_t stream)
+  {
+    switch (degree)
+      {
+        case 1: return vmult_batch_deg<Number, 1>(coef, vel, ops, block_off, nops, plan, partials, scal, stream);
+        case 2: return vmult_batch_deg<Number, 2>(coef, vel, ops, block_off, nops, plan, partials, scal, stream);
+        case 3: return vmult_batch_deg<Number, 3>(coef, vel, ops, block_off, nops, plan, partials, scal, stream);
+        case 4: return vmult_batch_deg<Number, 4>(coef, vel, ops, block_off, nops, plan, partials, scal, stream);
+        case 5: return vmult_batch_deg<Number, 5>(coef, vel, ops, block_off, nops, plan, partials, scal, stream);
+        case 6: return vmult_batch_deg<Number, 6>(coef, vel, ops, block_off, nops, plan, partials, scal, stream);
+        default: return hipErrorInvalidValue;
+      }
+  }
+
+  hipError_t launch_batch_pap_finalize(const int *block_off, const int *ops_solve, int nops,
+                                       const BatchVmultPlan &plan, const double *partials,
+                                       SolverScalars *scal, hipStream_t stream)
+  {
+    batch_pap_finalize_kernel<<<nops, 64, 0, stream>>>(block_off, ops_solve, nops, plan, partials, scal);
+    return hipGetLastError();
+  }
+
+  template hipError_t launch_vmult_batch<double>(int, int, bool, const BatchOperator<double> *,
+                                                 const int *, int, const BatchVmultPlan &, double *,
+                                                 const SolverScalars *, hipStream_t);
+  template hipError_t launch_vmult_batch<float>(int, int, bool, const BatchOperator<float> *,
+                                                const int *, int, const BatchVmultPlan &, double *,
+                                                const SolverScalars *, hipStream_t);
 
   template <typename Number>
   hipError_t launch_diagonal(int degree, int coef, bool vel, const CellParams<Number> &prm,

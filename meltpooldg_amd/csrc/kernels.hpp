@@ -116,6 +116,42 @@ namespace mpdg
                              const DiagTables<Number> &tab, Number *diag,
                              const Number *heaviside, hipStream_t stream);
 
+  // One operator of a batch: its parameters, the vectors its vmult reads and
+  // writes, its work items per colour and the solve it belongs to.
+  template <typename Number>
+  struct BatchOperator
+  {
+    CellParams<Number> prm;
+    const Number *src;
+    Number *dst;
+    const Number *heav;
+    int items[8];
+    int solve;
+  };
+
+  // Launch plan of a batched vmult over the operators of one degree.
+  struct BatchVmultPlan
+  {
+    int grid[8];      // workgroups per colour
+    int pofs[8];      // first partial slot of each colour
+    int total_slots;  // partial slots over all colours
+  };
+
+  // Builds block_off (8 x (nops+1), host) and the plan from per-operator items.
+  BatchVmultPlan batch_vmult_plan(const int *items /* nops x 8 */, int nops, int max_blocks,
+                                  int *block_off);
+  int vmult_resident_blocks(int degree, bool fp32, int coef, bool vel);
+  int vmult_cells_per_block(int degree, bool fp32);
+
+  template <typename Number>
+  hipError_t launch_vmult_batch(int degree, int coef, bool vel, const BatchOperator<Number> *ops,
+                                const int *block_off, int nops, const BatchVmultPlan &plan,
+                                double *partials, const SolverScalars *scal, hipStream_t stream);
+  // alpha_s = rz_s / sum of solve s's vmult partials (one block per operator).
+  hipError_t launch_batch_pap_finalize(const int *block_off, const int *ops_solve, int nops,
+                                       const BatchVmultPlan &plan, const double *partials,
+                                       SolverScalars *scal, hipStream_t stream);
+
   // Vector kernels (solver_kernels.hip).
   constexpr int kVecBlocks = 1024;
   constexpr int kVecThreads = 256;
@@ -145,6 +181,32 @@ namespace mpdg
   hipError_t launch_fill_f32(float *x, float v, int64_t n, hipStream_t s);
   hipError_t launch_dot(const double *a, const double *b, int64_t n, double *partials,
                         double *out, unsigned *counter, hipStream_t s);
+
+  // Batched (segmented) vector kernels, batch_kernels.hip. Solve s owns entries
+  // [off, off+n) of the concatenated vectors and workgroups [blk0, blk0+nblk).
+  struct BatchSeg
+  {
+    int64_t off, n;
+    int blk0, nblk;
+  };
+  constexpr int kBatchThreads = 256;
+  template <typename Number>
+  hipError_t launch_batch_cg_init(const double *b, const Number *ax, Number *r, Number *p,
+                                  const Number *inv, const BatchSeg *seg, int nsolve, int nblocks,
+                                  double *partials, SolverScalars *scal, const double *rel_tol,
+                                  double abs_tol, int max_it, const int *active, hipStream_t s);
+  template <typename Number>
+  hipError_t launch_batch_cg_update(Number *x, Number *r, const Number *p, const Number *q,
+                                    const Number *inv, const BatchSeg *seg, int nsolve,
+                                    int nblocks, double *partials, SolverScalars *scal,
+                                    hipStream_t s);
+  template <typename Number>
+  hipError_t launch_batch_cg_direction(Number *p, const Number *r, const Number *inv,
+                                       const BatchSeg *seg, int nsolve, int nblocks,
+                                       const SolverScalars *scal, hipStream_t s);
+  hipError_t launch_batch_residual(double *r, const double *b, const double *ax,
+                                   const BatchSeg *seg, int nsolve, int nblocks, double *partials,
+                                   double *res, hipStream_t s);
 
   // GMRES building blocks (classical Gram-Schmidt over a basis of at most
   // kMaxBasis vectors stored with leading dimension ld).

@@ -80,6 +80,32 @@ def _stream_handle():
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
+def _make_desc(n_cells, degree, material, box, precision, conductivity_amplitude, velocity,
+               velocity_w0, dirichlet_faces, stream):
+    n = (n_cells,) * 3 if isinstance(n_cells, int) else tuple(n_cells)
+    mat = material or MaterialData()
+    d = L.HeatDesc()
+    d.n_cells[:] = [int(v) for v in n]
+    d.degree = int(degree)
+    d.box_lo[:] = [float(v) for v in box[0]]
+    d.box_hi[:] = [float(v) for v in box[1]]
+    d.precision = L.MPDG_FP32 if precision == "fp32" else L.MPDG_FP64
+    d.material = 0 if mat.type == "gas" else 1
+    for dst, src in ((d.gas, mat.gas), (d.liquid, mat.liquid)):
+        dst.density = src.density
+        dst.specific_heat_capacity = src.specific_heat_capacity
+        dst.thermal_conductivity = src.thermal_conductivity
+    d.two_phase_sharp = 1 if mat.two_phase_fluid_properties_transition_type == "sharp" else 0
+    d.volume_specific_heat = int(mat.use_volume_specific_thermal_capacity_for_phase_interpolation)
+    d.conductivity_field = 1 if conductivity_amplitude else 0
+    d.conductivity_amplitude = float(conductivity_amplitude)
+    d.velocity = {None: 0, "none": 0, "vortex": 1}[velocity]
+    d.velocity_w0 = float(velocity_w0)
+    d.dirichlet_faces = int(dirichlet_faces)
+    d.stream = stream
+    return d
+
+
 class HeatDiffuseOperator:
     """Matrix-free heat tangent J = rho_cp/dt M + K_k + rho_cp C_u on a box mesh.
 
@@ -94,25 +120,8 @@ class HeatDiffuseOperator:
         self._lib = L.load()
         n = (n_cells,) * 3 if isinstance(n_cells, int) else tuple(n_cells)
         mat = material or MaterialData()
-        d = L.HeatDesc()
-        d.n_cells[:] = [int(v) for v in n]
-        d.degree = int(degree)
-        d.box_lo[:] = [float(v) for v in box[0]]
-        d.box_hi[:] = [float(v) for v in box[1]]
-        d.precision = L.MPDG_FP32 if precision == "fp32" else L.MPDG_FP64
-        d.material = 0 if mat.type == "gas" else 1
-        for dst, src in ((d.gas, mat.gas), (d.liquid, mat.liquid)):
-            dst.density = src.density
-            dst.specific_heat_capacity = src.specific_heat_capacity
-            dst.thermal_conductivity = src.thermal_conductivity
-        d.two_phase_sharp = 1 if mat.two_phase_fluid_properties_transition_type == "sharp" else 0
-        d.volume_specific_heat = int(mat.use_volume_specific_thermal_capacity_for_phase_interpolation)
-        d.conductivity_field = 1 if conductivity_amplitude else 0
-        d.conductivity_amplitude = float(conductivity_amplitude)
-        d.velocity = {None: 0, "none": 0, "vortex": 1}[velocity]
-        d.velocity_w0 = float(velocity_w0)
-        d.dirichlet_faces = int(dirichlet_faces)
-        d.stream = stream if stream is not None else _stream_handle()
+        d = _make_desc(n_cells, degree, mat, box, precision, conductivity_amplitude, velocity,
+                       velocity_w0, dirichlet_faces, stream if stream is not None else _stream_handle())
         self.desc = d
         h = ctypes.c_void_p()
         L.check(self._lib.mpdg_heat_create(ctypes.byref(d), ctypes.byref(h)))
@@ -251,3 +260,64 @@ class LinearSolver:
             raise NoConvergence(rep.iterations, rep.final_residual, identifier)
         L.check(st)
         return rep.iterations
+
+
+class HeatBatch:
+    """Independent heat solves of differing mesh size and degree solved together
+    (batched vmult per degree and colour, segmented vector kernels, per-solve
+    convergence, hipGraph-captured iteration chunks). ``operators`` are
+    HeatDiffuseOperator-style parameter dicts: n_cells, degree and optionally
+    material (MaterialData), conductivity_amplitude, velocity, dirichlet_faces."""
+
+    def __init__(self, members, stream=None):
+        self._lib = L.load()
+        n = len(members)
+        descs = (L.HeatDesc * n)()
+        for i, m in enumerate(members):
+            d = _make_desc(m.get("n_cells"), m.get("degree"), m.get("material"),
+                           m.get("box", ((0.0, 0.0, 0.0), (1.0, 1.0, 1.0))),
+                           m.get("precision", "fp64"), m.get("conductivity_amplitude", 0.0),
+                           m.get("velocity"), m.get("velocity_w0", 0.2),
+                           m.get("dirichlet_faces", 0x3F), stream if stream is not None else 0)
+            descs[i] = d
+        h = ctypes.c_void_p()
+        L.check(self._lib.mpdg_heat_batch_create(descs, n, ctypes.byref(h)))
+        self._h = h
+        self.count = n
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._lib.mpdg_heat_batch_destroy(h)
+            self._h = None
+
+    def reset_time_increment(self, dt):
+        L.check(self._lib.mpdg_heat_batch_set_dt(self._h, float(dt)))
+
+    def n_dofs(self, i):
+        return int(self._lib.mpdg_heat_batch_n_dofs(self._h, i))
+
+    def solve(self, xs, bs, data):
+        """Solves member i for xs[i] (initial guess on entry) with rhs bs[i];
+        returns the per-member reports (iterations, residuals, seconds to convergence)."""
+        n = self.count
+        xp = (ctypes.c_void_p * n)(*[x.data_ptr() for x in xs])
+        bp = (ctypes.c_void_p * n)(*[b.data_ptr() for b in bs])
+        prm = L.SolverParams()
+        self._lib.mpdg_solver_params_default(ctypes.byref(prm))
+        prm.solver_type = 0
+        prm.max_iterations = int(data.max_iterations)
+        prm.rel_tolerance = float(data.rel_tolerance)
+        prm.abs_tolerance = float(data.abs_tolerance)
+        prm.mixed_precision = 1 if data.mixed_precision else 0
+        prm.inner_rel_tolerance = float(data.inner_rel_tolerance)
+        prm.check_interval = int(data.check_interval)
+        reps = (L.BatchReport * n)()
+        st = self._lib.mpdg_heat_batch_solve(self._h, xp, bp, ctypes.byref(prm), reps)
+        out = [{"status": r.status, "iterations": r.iterations, "outer_iterations": r.outer_iterations,
+                "initial_residual": r.initial_residual, "final_residual": r.final_residual,
+                "seconds": r.seconds} for r in reps]
+        if st == L.MPDG_ERR_NOT_CONVERGED:
+            raise NoConvergence(max(r["iterations"] for r in out), max(r["final_residual"] for r in out))
+        L.check(st)
+        return out
