@@ -339,17 +339,20 @@ namespace mpdg
                                                  Number (&u)[P + 1])
     {
       constexpr int N = P + 1;
+      constexpr uint32_t SZ = sizeof(Number);
       const int Yi = it.cy * P + s0, Zi = cz * P + s1;
       const int X0 = (it.cx0 + cb) * P;
       const bool take = ok & (cb < it.ncell) & !dirichlet_node(prm, Yi, prm.n1y, 1) &
                         !dirichlet_node(prm, Zi, prm.n1z, 2);
-      const uint32_t base = (uint32_t)(X0 + prm.n1x * (Yi + prm.n1y * Zi)) * sizeof(Number);
+      // one masked offset per line; only the end nodes can sit on an x face
+      const uint32_t lb = take ? (uint32_t)(X0 + prm.n1x * (Yi + prm.n1y * Zi)) * SZ : kOutOfRange;
+      const bool x0d = (prm.faces & 1u) & (X0 == 0);
+      const bool x1d = ((prm.faces >> 1) & 1u) & (X0 + P == prm.n1x - 1);
+      u[0] = buf_load(src, x0d ? kOutOfRange : lb, Number());
 #pragma unroll
-      for (int a = 0; a < N; ++a)
-        {
-          const bool t = take & !dirichlet_node(prm, X0 + a, prm.n1x, 0);
-          u[a] = buf_load(src, t ? base + a * (uint32_t)sizeof(Number) : kOutOfRange, Number());
-        }
+      for (int a = 1; a < N - 1; ++a)
+        u[a] = buf_load(src, lb + a * SZ, Number());
+      u[N - 1] = buf_load(src, x1d ? kOutOfRange : lb + (N - 1) * SZ, Number());
     }
 
     template <int P, typename Number>
@@ -366,6 +369,92 @@ namespace mpdg
 #pragma unroll
       for (int a = 0; a < N; ++a)
         u[a] = buf_load(v, take ? base + a * (uint32_t)sizeof(Number) : kOutOfRange, Number());
+    }
+
+    // Per-lane constants of one line slot: offset of the line's first node from
+    // the layer's base node, and where the line sits in its cell.
+    enum LaneBit : unsigned
+    {
+      kS0Lo = 1u,  // s0 == 0   (y face of the cell, or qx == 0)
+      kS0Hi = 2u,  // s0 == p
+      kS1Lo = 4u,  // s1 == 0   (z face)
+      kS1Hi = 8u,  // s1 == p
+      kCb0 = 16u   // first cell of the segment
+    };
+
+    struct LaneConst
+    {
+      int loff;      // cb*p + n1x*(s0 + n1y*s1)
+      unsigned bits; // LaneBit flags
+    };
+
+    // Uniform facts of one (work item, cell layer): masks over LaneBit.
+    struct LayerU
+    {
+      int base;           // lattice index of the layer's first node (cx0*p, cy*p, cz*p)
+      unsigned rowc;      // lines on a Dirichlet y/z face
+      unsigned notfirst;  // lines shared in y/z with an earlier colour pass
+      unsigned defer;     // top-face lines carried to the next layer
+      bool x0d, x1d;      // segment's first / last cell touches a Dirichlet x face
+      bool x0nf, xPnf;    // segment's first / last x end shared with an earlier pass
+    };
+
+    template <typename Number, int P>
+    __device__ __forceinline__ LayerU layer_uniforms(const CellParams<Number> &prm, const Item &it,
+                                                     int j)
+    {
+      const int cz = it.cz0 + j;
+      const unsigned f = prm.faces;
+      LayerU u;
+      u.base = it.cx0 * P + prm.n1x * (it.cy * P + prm.n1y * (cz * P));
+      u.rowc = (((f >> 2) & 1u) && it.cy == 0 ? kS0Lo : 0u) |
+               (((f >> 3) & 1u) && it.cy == prm.ny - 1 ? kS0Hi : 0u) |
+               (((f >> 4) & 1u) && cz == 0 ? kS1Lo : 0u) |
+               (((f >> 5) & 1u) && cz == prm.nz - 1 ? kS1Hi : 0u);
+      const bool cy_odd = it.cy & 1, kz_odd = it.kz & 1;
+      u.notfirst = (cy_odd && it.cy > 0 ? kS0Lo : 0u) | (cy_odd && it.cy < prm.ny - 1 ? kS0Hi : 0u) |
+                   (kz_odd && j == 0 && it.cz0 > 0 ? kS1Lo : 0u) |
+                   (kz_odd && j == it.nl - 1 && it.cz0 + it.nl < prm.nz ? kS1Hi : 0u);
+      u.defer = j < it.nl - 1 ? kS1Hi : 0u;
+      u.x0d = (f & 1u) && it.cx0 == 0;
+      u.x1d = ((f >> 1) & 1u) && it.cx0 + it.ncell == prm.nx;
+      const bool sx_odd = it.sx & 1;
+      u.x0nf = sx_odd && it.cx0 > 0;
+      u.xPnf = sx_odd && it.cx0 + it.ncell < prm.nx;
+      return u;
+    }
+
+    // Gather of one x-line with homogeneous constraints read as 0
+    // (FEEvaluation::read_dof_values); only the end nodes can sit on an x face.
+    template <int P, typename Number>
+    __device__ __forceinline__ void gather_line(__amdgpu_buffer_rsrc_t src, const LaneConst &lc,
+                                                const LayerU &U, int cb, int ncell, bool ok,
+                                                Number (&u)[P + 1])
+    {
+      constexpr int N = P + 1;
+      constexpr uint32_t SZ = sizeof(Number);
+      const bool take = ok & (cb < ncell) & ((lc.bits & U.rowc) == 0u);
+      const uint32_t lb = take ? (uint32_t)(U.base + lc.loff) * SZ : kOutOfRange;
+      const bool x0 = U.x0d & ((lc.bits & kCb0) != 0u);
+      const bool x1 = U.x1d & (cb == ncell - 1);
+      u[0] = buf_load(src, x0 ? kOutOfRange : lb, Number());
+#pragma unroll
+      for (int a = 1; a < N - 1; ++a)
+        u[a] = buf_load(src, lb + a * SZ, Number());
+      u[N - 1] = buf_load(src, x1 ? kOutOfRange : lb + (N - 1) * SZ, Number());
+    }
+
+    template <int P, typename Number>
+    __device__ __forceinline__ void gather_line_plain(__amdgpu_buffer_rsrc_t src,
+                                                      const LaneConst &lc, const LayerU &U, int cb,
+                                                      int ncell, bool ok, Number (&u)[P + 1])
+    {
+      constexpr int N = P + 1;
+      constexpr uint32_t SZ = sizeof(Number);
+      const uint32_t lb = (ok & (cb < ncell)) ? (uint32_t)(U.base + lc.loff) * SZ : kOutOfRange;
+#pragma unroll
+      for (int a = 0; a < N; ++a)
+        u[a] = buf_load(src, lb + a * SZ, Number());
     }
 
     // Cell-loop vmult, wavefront-segment form.
@@ -440,17 +529,27 @@ namespace mpdg
           s1v[r] = sv[r] / N;
           cb[r] = w * CW + l;
         }
+      LaneConst lc[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        {
+          lc[r].loff = cb[r] * P + prm.n1x * (s0v[r] + prm.n1y * s1v[r]);
+          lc[r].bits = (s0v[r] == 0 ? kS0Lo : 0u) | (s0v[r] == P ? kS0Hi : 0u) |
+                       (s1v[r] == 0 ? kS1Lo : 0u) | (s1v[r] == P ? kS1Hi : 0u) |
+                       (cb[r] == 0 ? kCb0 : 0u);
+        }
 
       int item = item_first;
       int j = 0;
       Item it = decode_item<Number, CPB>(prm, colour, layers, item);
+      LayerU LU = layer_uniforms<Number, P>(prm, it, 0);
       Number u[R][N], hl[R][N];
 #pragma unroll
       for (int r = 0; r < R; ++r)
         {
-          gather_xline<P>(prm, rsrc, it, it.cz0, cb[r], s0v[r], s1v[r], true, u[r]);
+          gather_line<P>(rsrc, lc[r], LU, cb[r], it.ncell, true, u[r]);
           if constexpr (H2)
-            gather_plain<P>(prm, rheav, it, it.cz0, cb[r], s0v[r], s1v[r], true, hl[r]);
+            gather_line_plain<P>(rheav, lc[r], LU, cb[r], it.ncell, true, hl[r]);
         }
       double acc = 0.0;
       int step = 0; // layers done by this workgroup; selects the x-exchange buffer
@@ -489,51 +588,56 @@ namespace mpdg
             }
 
           // prefetch of the next layer's x-lines
+          const LayerU LUn = layer_uniforms<Number, P>(prm, it_n, j_n);
           Number un[R][N], hn[R][N];
 #pragma unroll
           for (int r = 0; r < R; ++r)
             {
-              gather_xline<P>(prm, rsrc, it_n, it_n.cz0 + j_n, cb[r], s0v[r], s1v[r],
-                              item_n < items, un[r]);
+              gather_line<P>(rsrc, lc[r], LUn, cb[r], it_n.ncell, item_n < items, un[r]);
               if constexpr (H2)
-                gather_plain<P>(prm, rheav, it_n, it_n.cz0 + j_n, cb[r], s0v[r], s1v[r],
-                                item_n < items, hn[r]);
+                gather_line_plain<P>(rheav, lc[r], LUn, cb[r], it_n.ncell, item_n < items, hn[r]);
             }
 
-          // scatter plan of each output x-line (b = s0, c = s1): byte offset of
-          // each node written (out of range when not written here) and early
-          // load of the nodes it read-modify-writes (0 where stored first)
-          uint32_t woff[R][N];
+          // scatter plan of each output x-line (b = s0, c = s1). Interior nodes
+          // (0 < a < p) share the line's mode; only the end nodes can be on an x
+          // face or shared with the neighbouring segment. Offsets are out of
+          // range where nothing is written / read; nodes stored first get 0
+          // from the early read-modify-write load.
+          uint32_t wl[R], w0o[R], wPo[R];
           Number dv[R][N];
-          bool zero[R][N];
+          bool zl[R], z0[R], zP[R];
 #pragma unroll
           for (int r = 0; r < R; ++r)
             {
-              const int b = s0v[r], c = s1v[r];
-              const int Yo = it.cy * P + b, Zo = cz * P + c;
-              const int X0 = (it.cx0 + cb[r]) * P;
-              const uint32_t obase = (uint32_t)(X0 + prm.n1x * (Yo + prm.n1y * Zo)) * sizeof(Number);
-              const bool ok = lv[r] & (cb[r] < it.ncell);
+              constexpr uint32_t SZ = sizeof(Number);
+              const uint32_t obase = (uint32_t)(LU.base + lc[r].loff) * SZ;
+              const unsigned bits = lc[r].bits;
               const bool last_cell = cb[r] == it.ncell - 1;
-              const bool defer = (c == P) & (j < it.nl - 1); // top face carried to the next layer
-              const bool ysh = ((b == 0) & (it.cy > 0)) | ((b == P) & (it.cy < prm.ny - 1));
-              const bool zsh = ((c == 0) & (j == 0) & (it.cz0 > 0)) |
-                               ((c == P) & (j == it.nl - 1) & (it.cz0 + it.nl < prm.nz));
-              const bool yz_first = !(ysh & (it.cy & 1)) & !(zsh & (it.kz & 1));
-              const bool rowc = dirichlet_node(prm, Yo, prm.n1y, 1) | dirichlet_node(prm, Zo, prm.n1z, 2);
+              const bool lw = lv[r] & (cb[r] < it.ncell) & ((bits & LU.defer) == 0u);
+              const bool rowc = (bits & LU.rowc) != 0u;
+              const bool yz_first = (bits & LU.notfirst) == 0u;
+              // interior nodes
+              wl[r] = (lw & (!rowc | yz_first)) ? obase : kOutOfRange;
+              zl[r] = rowc;
+              const uint32_t li = (lw & !yz_first & !rowc) ? obase : kOutOfRange;
 #pragma unroll
-              for (int a = 0; a < N; ++a)
-                {
-                  const bool xsh = ((a == 0) & (cb[r] == 0) & (it.cx0 > 0)) |
-                                   ((a == P) & last_cell & (it.cx0 + it.ncell < prm.nx));
-                  const bool first = yz_first & !(xsh & (it.sx & 1));
-                  const bool con = rowc | dirichlet_node(prm, X0 + a, prm.n1x, 0);
-                  const bool writes = ok & !defer & ((a < N - 1) | last_cell) & (!con | first);
-                  const uint32_t off = obase + a * (uint32_t)sizeof(Number);
-                  woff[r][a] = writes ? off : kOutOfRange;
-                  zero[r][a] = con;
-                  dv[r][a] = buf_load(rdst, (writes & !first & !con) ? off : kOutOfRange, Number());
-                }
+              for (int a = 1; a < N - 1; ++a)
+                dv[r][a] = buf_load(rdst, li + a * SZ, Number());
+              // node a = 0
+              const bool cb0 = (bits & kCb0) != 0u;
+              const bool first0 = yz_first & !(cb0 & LU.x0nf);
+              const bool con0 = rowc | (cb0 & LU.x0d);
+              const bool wr0 = lw & (!con0 | first0);
+              w0o[r] = wr0 ? obase : kOutOfRange;
+              z0[r] = con0;
+              dv[r][0] = buf_load(rdst, (wr0 & !first0 & !con0) ? obase : kOutOfRange, Number());
+              // node a = p (written by the segment's last cell only)
+              const bool firstP = yz_first & !(last_cell & LU.xPnf);
+              const bool conP = rowc | (last_cell & LU.x1d);
+              const bool wrP = lw & last_cell & (!conP | firstP);
+              wPo[r] = wrP ? obase + P * SZ : kOutOfRange;
+              zP[r] = conP;
+              dv[r][N - 1] = buf_load(rdst, (wrP & !firstP & !conP) ? obase + P * SZ : kOutOfRange, Number());
             }
           wave_sync();
 
@@ -766,9 +870,11 @@ namespace mpdg
 #pragma unroll
                 for (int a = 0; a < N; ++a)
                   FZ[j & 1][(cb[r] * N + b) * N + a] = Y[r][a];
+              buf_store(rdst, w0o[r], z0[r] ? Number(0) : dv[r][0] + Y[r][0]);
 #pragma unroll
-              for (int a = 0; a < N; ++a)
-                buf_store(rdst, woff[r][a], zero[r][a] ? Number(0) : dv[r][a] + Y[r][a]);
+              for (int a = 1; a < N - 1; ++a)
+                buf_store(rdst, wl[r] + a * (uint32_t)sizeof(Number), zl[r] ? Number(0) : dv[r][a] + Y[r][a]);
+              buf_store(rdst, wPo[r], zP[r] ? Number(0) : dv[r][N - 1] + Y[r][N - 1]);
             }
 
 #pragma unroll
@@ -783,6 +889,7 @@ namespace mpdg
           item = item_n;
           j = j_n;
           it = it_n;
+          LU = LUn;
           ++step;
           wave_sync();
         }
