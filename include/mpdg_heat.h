@@ -171,15 +171,31 @@ int64_t mpdg_heat_batch_n_dofs(const mpdg_heat_batch *b, int member);
 mpdg_status mpdg_heat_batch_solve(mpdg_heat_batch *b, double *const *x, const double *const *rhs,
                                   const mpdg_solver_params *params, mpdg_batch_report *reports);
 
-/* Launch tuning of the cell kernels (no counterpart in the reference, whose
- * cell loop is tuned by deal.II's MatrixFree::AdditionalData). Used by the
- * autotuner; results do not depend on it beyond summation order. */
+/* Launch tuning of the vmult kernels, one set per precision (no counterpart
+ * in the reference, whose cell loop is tuned by deal.II's
+ * MatrixFree::AdditionalData). Results do not depend on it beyond the order
+ * of floating-point summation. */
+typedef enum {
+  MPDG_VMULT_LINE = 0, /* lanes own the 1D lines of a few cells, LDS between directions */
+  MPDG_VMULT_CELL = 1  /* each lane owns a whole cell in registers (degree <= 2) */
+} mpdg_vmult_variant;
+
 typedef struct {
   int z_layers;       /* cell layers swept per vmult work item (>= 1) */
-  int blocks_per_cu;  /* persistent workgroups per CU, 0 = occupancy limit */
+  int blocks_per_cu;  /* persistent workgroups per CU, 0 = occupancy limit (<= 16) */
+  int variant;        /* mpdg_vmult_variant; MPDG_VMULT_CELL needs degree <= 2 */
+  int tile_log2;      /* cell variant: log2 of the tile's cells along x (3..6); a
+                         wavefront's tile is 2^tile_log2 x 2^(6 - tile_log2) cells */
 } mpdg_tuning;
-mpdg_status mpdg_heat_get_tuning(const mpdg_heat *h, mpdg_tuning *t);
-mpdg_status mpdg_heat_set_tuning(mpdg_heat *h, const mpdg_tuning *t);
+mpdg_status mpdg_heat_get_tuning(const mpdg_heat *h, int precision, mpdg_tuning *t);
+mpdg_status mpdg_heat_set_tuning(mpdg_heat *h, int precision, const mpdg_tuning *t);
+
+/* Autotuner: times the vmult of this operator in the given precision over the
+ * candidate variants, tiles, z-chunk lengths and workgroups per CU (each
+ * candidate `repeats` times after one warm-up), installs the fastest and
+ * returns it in *best (may be NULL). Uses the handle's solver workspace as
+ * scratch vectors; no counterpart in the reference. */
+mpdg_status mpdg_heat_autotune(mpdg_heat *h, int precision, int repeats, mpdg_tuning *best);
 
 /* Kernel timing hook for measurement: average device time of the last
  * n_launches vmult launches (ms), recorded with HIP events on the handle's stream. */

@@ -81,8 +81,13 @@ class BatchReport(ctypes.Structure):
     ]
 
 
+MPDG_VMULT_LINE = 0
+MPDG_VMULT_CELL = 1
+
+
 class Tuning(ctypes.Structure):
-    _fields_ = [("z_layers", ctypes.c_int), ("blocks_per_cu", ctypes.c_int)]
+    _fields_ = [("z_layers", ctypes.c_int), ("blocks_per_cu", ctypes.c_int),
+                ("variant", ctypes.c_int), ("tile_log2", ctypes.c_int)]
 
 
 # name -> (restype, argtypes); exactly the entry points include/mpdg_heat.h declares
@@ -109,8 +114,9 @@ SIGNATURES = {
     "mpdg_heat_batch_n_dofs": (ctypes.c_int64, [_VP, ctypes.c_int]),
     "mpdg_heat_batch_solve": (ctypes.c_int, [_VP, ctypes.POINTER(_VP), ctypes.POINTER(_VP),
                                              ctypes.POINTER(SolverParams), ctypes.POINTER(BatchReport)]),
-    "mpdg_heat_get_tuning": (ctypes.c_int, [_VP, ctypes.POINTER(Tuning)]),
-    "mpdg_heat_set_tuning": (ctypes.c_int, [_VP, ctypes.POINTER(Tuning)]),
+    "mpdg_heat_get_tuning": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.POINTER(Tuning)]),
+    "mpdg_heat_set_tuning": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.POINTER(Tuning)]),
+    "mpdg_heat_autotune": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Tuning)]),
     "mpdg_heat_time_vmult": (ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int, ctypes.c_int,
                                             ctypes.POINTER(ctypes.c_double)]),
 }

@@ -191,6 +191,43 @@ namespace mpdg
       return r;
     }
 
+    // Dense N x N matrix applied as y_i = sum_j a[i][j] x_j.
+    template <int N>
+    struct Mat
+    {
+      double a[N][N]{};
+    };
+
+    // Dense 1D matrices of the cell-form vmult, with the quadrature weights
+    // folded in so that the q-point work needs no weights:
+    //   kind 0: S[q][i]                (interpolate nodal -> Gauss)
+    //   kind 1: C[q][r]                (collocation derivative on Gauss points)
+    //   kind 2: w_r C[r][q] / w_q      (W^-1 C^T W: the flux back to values)
+    //   kind 3: S[q][i] w_q, as [i][q] (S^T W: integrate)
+    template <int P>
+    constexpr Mat<P + 1> cell_matrix(const Tab<P + 1> &t, int kind)
+    {
+      constexpr int N = P + 1;
+      Mat<N> m{};
+      for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j)
+          m.a[i][j] = kind == 0 ? t.S[i][j]
+                    : kind == 1 ? t.C[i][j]
+                    : kind == 2 ? t.w[j] * t.C[j][i] / t.w[i]
+                                : t.S[j][i] * t.w[j];
+      return m;
+    }
+
+    template <int P>
+    struct CTab
+    {
+      static constexpr Tab<P + 1> t = make_tab<P>();
+      static constexpr Mat<P + 1> S = cell_matrix<P>(t, 0);
+      static constexpr Mat<P + 1> C = cell_matrix<P>(t, 1);
+      static constexpr Mat<P + 1> CW = cell_matrix<P>(t, 2);
+      static constexpr Mat<P + 1> SW = cell_matrix<P>(t, 3);
+    };
+
     // All even-odd tables of the vmult kernel for degree P.
     template <int P>
     struct VTab

@@ -1,0 +1,210 @@
+// Device helpers shared by the cell kernels: even-odd 1D contractions with
+// compile-time tables, raw buffer loads/stores with out-of-range masking, the
+// two-phase material blend, and deterministic block reductions.
+#pragma once
+
+#include "ctables.hpp"
+#include "kernels.hpp"
+
+namespace mpdg
+{
+  // Non-returning atomic add of a double through a buffer resource (the LLVM
+  // intrinsic behind buffer_atomic_add_f64; out-of-range offsets are dropped).
+  __device__ double buffer_atomic_fadd_f64(double v, __amdgpu_buffer_rsrc_t r, int off, int soff,
+                                           int aux) __asm("llvm.amdgcn.raw.ptr.buffer.atomic.fadd.f64");
+
+  namespace
+  {
+    // y = M x with a compile-time even-odd table (entries folded into the code).
+    template <int N, int SIGN, typename Number>
+    __device__ __forceinline__ void eo_apply(const ct::EO<N> &M, const Number (&x)[N], Number (&y)[N])
+    {
+      constexpr int H = N / 2;
+      Number E[H], O[H];
+#pragma unroll
+      for (int j = 0; j < H; ++j)
+        {
+          E[j] = x[j] + x[N - 1 - j];
+          O[j] = x[j] - x[N - 1 - j];
+        }
+#pragma unroll
+      for (int i = 0; i < H; ++i)
+        {
+          Number a = Number(M.e[i][0]) * E[0];
+          Number b = Number(M.o[i][0]) * O[0];
+#pragma unroll
+          for (int j = 1; j < H; ++j)
+            {
+              a += Number(M.e[i][j]) * E[j];
+              b += Number(M.o[i][j]) * O[j];
+            }
+          if constexpr (N % 2 == 1)
+            a += Number(M.mc[i]) * x[H];
+          y[i] = a + b;
+          y[N - 1 - i] = SIGN > 0 ? a - b : b - a;
+        }
+      if constexpr (N % 2 == 1)
+        {
+          Number m = Number(M.mm) * x[H];
+#pragma unroll
+          for (int j = 0; j < H; ++j)
+            m += Number(M.mr[j]) * (SIGN > 0 ? E[j] : O[j]);
+          y[H] = m;
+        }
+    }
+
+    // Raw buffer access: an offset at or past the buffer's byte size reads 0
+    // and drops the store, so masked lanes need no branch.
+    constexpr uint32_t kOutOfRange = 0xFFFFFF00u;
+
+    __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *p, uint32_t bytes)
+    {
+      return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes,
+                                               0x00020000);
+    }
+
+    typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+
+    __device__ __forceinline__ double buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off, double)
+    {
+      return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+    }
+    __device__ __forceinline__ float buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off, float)
+    {
+      return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+    }
+    __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t off, double v)
+    {
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, off, 0, 0);
+    }
+    __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t off, float v)
+    {
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, 0);
+    }
+
+    // dst[off] += v as one device-scope atomic (no return value).
+    __device__ __forceinline__ void buf_add(__amdgpu_buffer_rsrc_t r, uint32_t off, double v)
+    {
+      (void)buffer_atomic_fadd_f64(v, r, (int)off, 0, 0);
+    }
+    __device__ __forceinline__ void buf_add(__amdgpu_buffer_rsrc_t r, uint32_t off, float v)
+    {
+      (void)__builtin_amdgcn_raw_ptr_buffer_atomic_fadd_f32(v, r, (int)off, 0, 0);
+    }
+
+    // on_dirichlet without short-circuit branches
+    template <typename Number>
+    __device__ __forceinline__ bool dirichlet_node(const CellParams<Number> &prm, int i, int n1, int d)
+    {
+      const unsigned lo = (prm.faces >> (2 * d)) & 1u;
+      const unsigned hi = (prm.faces >> (2 * d + 1)) & 1u;
+      return (lo & (unsigned)(i == 0)) | (hi & (unsigned)(i == n1 - 1));
+    }
+
+    // Material::compute_parameters for gas_liquid with the blend of
+    // LevelSet::Tools::interpolate (material.templates.hpp:515-524,
+    // level_set_tools.hpp:40-43); rho*cp as in heat_diffuse_operator.cpp:1167-1210.
+    template <typename Number>
+    __device__ __forceinline__ void two_phase(const CellParams<Number> &prm, Number H, Number &rc,
+                                              Number &k)
+    {
+      const Number w = prm.sharp ? (H > Number(0.5) ? Number(1) : Number(0)) : H;
+      const Number one_w = Number(1) - w;
+      k = one_w * prm.g_k + w * prm.l_k;
+      if (prm.volume_specific)
+        rc = one_w * prm.g_rhocp + w * prm.l_rhocp;
+      else
+        rc = (one_w * prm.g_rho + w * prm.l_rho) * (one_w * prm.g_cp + w * prm.l_cp);
+    }
+
+    // Deterministic block sum (fixed butterfly + fixed-order wave sum).
+    template <int THREADS>
+    __device__ __forceinline__ double block_sum(double v, double *sh)
+    {
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1)
+        v += __shfl_xor(v, off, 64);
+      const int wave = threadIdx.x >> 6;
+      if ((threadIdx.x & 63) == 0)
+        sh[wave] = v;
+      __syncthreads();
+      double s = 0.0;
+      if (threadIdx.x == 0)
+        {
+          for (int w = 0; w < THREADS / 64; ++w)
+            s += sh[w];
+        }
+      return s; // valid in thread 0
+    }
+
+    // Sum of partials[0..count) in a fixed order by one block.
+    template <int THREADS>
+    __device__ __forceinline__ double ordered_sum(const double *partials, int count, double *sh)
+    {
+      double v = 0.0;
+      for (int i = threadIdx.x; i < count; i += THREADS)
+        v += __hip_atomic_load(partials + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __syncthreads();
+      return block_sum<THREADS>(v, sh);
+    }
+
+    // p . Ap of one colour pass of a vmult: each workgroup stores its sum in
+    // partials[offset + block]; in the last pass (finalize), the workgroup that
+    // finishes last adds all `total` slots in index order and sets scal->pap
+    // and scal->alpha = rz / pap. Every thread of the workgroup must call it.
+    template <int THREADS>
+    __device__ __forceinline__ void pass_partials(double acc, double *partials, int offset,
+                                                  int total, int finalize, SolverScalars *scal)
+    {
+      __shared__ double red[THREADS / 64];
+      __shared__ bool is_last;
+      const double bs = block_sum<THREADS>(acc, red);
+      if (threadIdx.x == 0)
+        {
+          partials[offset + blockIdx.x] = bs;
+          bool lst = false;
+          if (finalize)
+            {
+              __threadfence();
+              const unsigned prev = atomicAdd(&scal->counter, 1u);
+              lst = prev == gridDim.x - 1;
+            }
+          is_last = lst;
+        }
+      __syncthreads();
+      if (is_last)
+        {
+          __threadfence();
+          const double pap = ordered_sum<THREADS>(partials, total, red);
+          if (threadIdx.x == 0)
+            {
+              scal->pap = pap;
+              scal->alpha = scal->rz / pap;
+              scal->counter = 0u;
+            }
+        }
+    }
+
+    // Operator served by this workgroup in a batched launch: the last s with
+    // block_off[s] <= blockIdx.x (block_off holds nops + 1 ascending entries).
+    __device__ __forceinline__ int find_block_owner(const int *block_off, int nops)
+    {
+      __shared__ int s_sh;
+      if (threadIdx.x == 0)
+        {
+          int lo = 0, hi = nops;
+          while (hi - lo > 1)
+            {
+              const int mid = (lo + hi) / 2;
+              if (block_off[mid] <= (int)blockIdx.x)
+                lo = mid;
+              else
+                hi = mid;
+            }
+          s_sh = lo;
+        }
+      __syncthreads();
+      return s_sh;
+    }
+  } // namespace
+} // namespace mpdg

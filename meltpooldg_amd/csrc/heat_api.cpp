@@ -75,8 +75,7 @@ struct mpdg_heat
   double h[3]{};
   double dt = 0.0;
   bool dt_set = false;
-  int zlayers = 4;       // cell layers per vmult work item (z-chunk)
-  int blocks_per_cu = 0; // persistent workgroups per CU, 0 = occupancy limit
+  mpdg_tuning tune[2]{}; // vmult launch tuning per precision (MPDG_FP64, MPDG_FP32)
 
   CellParams<double> prm64{};
   CellParams<float> prm32{};
@@ -121,6 +120,35 @@ struct mpdg_heat
 
 namespace
 {
+  // Default launch tuning per degree and precision: the fastest setting of a
+  // sweep over z-chunk length, workgroups per CU and variant on an MI355X at
+  // the sizes of the benchmark workloads (mpdg_heat_autotune refines it for a
+  // given operator).
+  mpdg_tuning default_tuning(int degree, bool fp32)
+  {
+    mpdg_tuning t{};
+    t.variant = MPDG_VMULT_LINE;
+    t.tile_log2 = 4;
+    switch (degree)
+      {
+        case 1: t.z_layers = fp32 ? 4 : 2; t.blocks_per_cu = fp32 ? 4 : 0; break;
+        case 2: t.z_layers = fp32 ? 4 : 2; t.blocks_per_cu = fp32 ? 8 : 4; break;
+        case 3: t.z_layers = 4; t.blocks_per_cu = 4; break;
+        case 4: t.z_layers = fp32 ? 2 : 4; t.blocks_per_cu = 4; break;
+        case 5: t.z_layers = fp32 ? 2 : 4; t.blocks_per_cu = fp32 ? 4 : 0; break;
+        default: t.z_layers = 2; t.blocks_per_cu = 4; break;
+      }
+    return t;
+  }
+
+  bool tuning_valid(const mpdg_tuning &t, int degree)
+  {
+    return t.z_layers >= 1 && t.blocks_per_cu >= 0 && t.blocks_per_cu <= kMaxResidentPerCu &&
+           (t.variant == MPDG_VMULT_LINE ||
+            (t.variant == MPDG_VMULT_CELL && degree <= kCellMaxDegree && t.tile_log2 >= 3 &&
+             t.tile_log2 <= 6));
+  }
+
   template <typename Number>
   void fill_params(mpdg_heat *H, CellParams<Number> &prm, const Number *tab)
   {
@@ -160,8 +188,12 @@ namespace
     prm.l_rhocp = (Number)l_rhocp;
     prm.sharp = d.two_phase_sharp;
     prm.volume_specific = d.volume_specific_heat;
-    prm.zlayers = std::max(1, std::min(H->zlayers, d.n_cells[2]));
-    prm.blocks_per_cu = H->blocks_per_cu;
+    const mpdg_tuning &tu = H->tune[sizeof(Number) == 4 ? MPDG_FP32 : MPDG_FP64];
+    prm.zlayers = std::max(1, std::min(tu.z_layers, d.n_cells[2]));
+    prm.blocks_per_cu = tu.blocks_per_cu;
+    prm.variant = (tu.variant == MPDG_VMULT_CELL && d.degree <= kCellMaxDegree) ? kVariantCell :
+                                                                                   kVariantLine;
+    prm.tile_log2 = tu.tile_log2;
     // per-direction tables: [sin2pi dir 0 | dir 1 | dir 2 | sinpi2 dir 0 | dir 1 | dir 2]
     size_t off = 0;
     for (int k = 0; k < 3; ++k)
@@ -521,6 +553,8 @@ mpdg_status mpdg_heat_create(const mpdg_heat_desc *desc, mpdg_heat **out)
   if (!H)
     return fail(MPDG_ERR_ARGUMENT, "out of host memory");
   H->desc = d;
+  H->tune[MPDG_FP64] = default_tuning(d.degree, false);
+  H->tune[MPDG_FP32] = default_tuning(d.degree, true);
   H->stream = (hipStream_t)d.stream;
   H->N = N;
   H->N_free = Nf;
@@ -820,27 +854,99 @@ mpdg_status mpdg_heat_solve(mpdg_heat *h, double *x, const double *b,
   return status;
 }
 
-mpdg_status mpdg_heat_get_tuning(const mpdg_heat *h, mpdg_tuning *t)
+mpdg_status mpdg_heat_get_tuning(const mpdg_heat *h, int precision, mpdg_tuning *t)
 {
   if (!h || !t)
     return fail(MPDG_ERR_ARGUMENT, "null argument");
-  t->z_layers = h->zlayers;
-  t->blocks_per_cu = h->blocks_per_cu;
+  if (precision != MPDG_FP64 && precision != MPDG_FP32)
+    return fail(MPDG_ERR_ARGUMENT, "precision must be MPDG_FP64 or MPDG_FP32");
+  *t = h->tune[precision];
   return MPDG_OK;
 }
 
-mpdg_status mpdg_heat_set_tuning(mpdg_heat *h, const mpdg_tuning *t)
+mpdg_status mpdg_heat_set_tuning(mpdg_heat *h, int precision, const mpdg_tuning *t)
 {
   if (!h || !t)
     return fail(MPDG_ERR_ARGUMENT, "null argument");
-  if (t->z_layers < 1 || t->blocks_per_cu < 0 || t->blocks_per_cu > kMaxResidentPerCu)
-    return fail(MPDG_ERR_ARGUMENT, "z_layers must be >= 1 and blocks_per_cu in 0..16");
+  if (precision != MPDG_FP64 && precision != MPDG_FP32)
+    return fail(MPDG_ERR_ARGUMENT, "precision must be MPDG_FP64 or MPDG_FP32");
+  if (!tuning_valid(*t, h->desc.degree))
+    return fail(MPDG_ERR_ARGUMENT,
+                "tuning needs z_layers >= 1, blocks_per_cu in 0..16 and, for the cell variant, "
+                "degree <= 2 and tile_log2 in 3..6");
   if (h->stream)
     MPDG_HIP(hipStreamSynchronize(h->stream));
-  h->zlayers = t->z_layers;
-  h->blocks_per_cu = t->blocks_per_cu;
+  h->tune[precision] = *t;
   fill_params<double>(h, h->prm64, h->sin_tab64.ptr);
   fill_params<float>(h, h->prm32, h->sin_tab32.ptr);
+  return MPDG_OK;
+}
+
+mpdg_status mpdg_heat_autotune(mpdg_heat *h, int precision, int repeats, mpdg_tuning *best)
+{
+  if (mpdg_status s = check_ready(h))
+    return s;
+  if (precision != MPDG_FP64 && precision != MPDG_FP32)
+    return fail(MPDG_ERR_ARGUMENT, "precision must be MPDG_FP64 or MPDG_FP32");
+  if (repeats < 1)
+    return fail(MPDG_ERR_ARGUMENT, "repeats must be >= 1");
+  const bool fp32 = precision == MPDG_FP32;
+  if (mpdg_status s = ensure_workspace(h, fp32))
+    return s;
+  // scratch vectors: src = a fill of ones, dst = the PCG's q
+  void *src = fp32 ? (void *)h->p32.ptr : (void *)h->p.ptr;
+  void *dst = fp32 ? (void *)h->q32.ptr : (void *)h->q.ptr;
+  if (fp32)
+    MPDG_HIP(launch_fill_f32((float *)src, 1.0f, h->N, h->stream));
+  else
+    MPDG_HIP(launch_fill((double *)src, 1.0, h->N, h->stream));
+
+  std::vector<mpdg_tuning> cand;
+  const int nz = h->desc.n_cells[2];
+  for (int zl : {1, 2, 4, 8})
+    {
+      if (zl > 1 && zl >= 2 * nz)
+        continue;
+      for (int bpc : {0, 2, 4, 8})
+        {
+          mpdg_tuning t{zl, bpc, MPDG_VMULT_LINE, 4};
+          cand.push_back(t);
+          if (h->desc.degree <= kCellMaxDegree)
+            for (int tl : {3, 4, 5, 6})
+              cand.push_back(mpdg_tuning{zl, bpc, MPDG_VMULT_CELL, tl});
+        }
+    }
+  const mpdg_tuning saved = h->tune[precision];
+  double best_ms = 0.0;
+  mpdg_tuning pick = saved;
+  for (const mpdg_tuning &t : cand)
+    {
+      h->tune[precision] = t;
+      fill_params<double>(h, h->prm64, h->sin_tab64.ptr);
+      fill_params<float>(h, h->prm32, h->sin_tab32.ptr);
+      double ms = 0.0;
+      mpdg_status st = fp32 ? vmult32(h, (float *)dst, (const float *)src, nullptr, nullptr) :
+                              vmult64(h, (double *)dst, (const double *)src, nullptr, nullptr);
+      if (st == MPDG_OK)
+        st = mpdg_heat_time_vmult(h, (double *)dst, (const double *)src, repeats, precision, &ms);
+      if (st != MPDG_OK)
+        {
+          h->tune[precision] = saved;
+          fill_params<double>(h, h->prm64, h->sin_tab64.ptr);
+          fill_params<float>(h, h->prm32, h->sin_tab32.ptr);
+          return st;
+        }
+      if (best_ms == 0.0 || ms < best_ms)
+        {
+          best_ms = ms;
+          pick = t;
+        }
+    }
+  h->tune[precision] = pick;
+  fill_params<double>(h, h->prm64, h->sin_tab64.ptr);
+  fill_params<float>(h, h->prm32, h->sin_tab32.ptr);
+  if (best)
+    *best = pick;
   return MPDG_OK;
 }
 
@@ -880,6 +986,8 @@ namespace
   {
     int degree = 0;
     int nops = 0;
+    int variant = kVariantLine;        // kernel form of ops (precision Number)
+    int variant64 = kVariantLine;      // kernel form of opsx (FP64)
     std::vector<int> members;          // solve index of each operator
     std::vector<int> block_off;        // 8 x (nops + 1), host
     BatchVmultPlan plan{};
@@ -973,19 +1081,32 @@ namespace
       {
         BatchGroup<Number> g;
         g.degree = deg;
-        std::vector<int> items;
+        std::vector<int> items; // per operator and colour, in workgroup units of the plan
         std::vector<BatchOperator<Number>> ops;
         std::vector<BatchOperator<double>> opsx;
-        const int cpb = vmult_cells_per_block(deg, fp32);
-        const int cpb64 = vmult_cells_per_block(deg, false);
+        int variant = -1, tile = 4, variant64 = -1, tile64 = 4;
         for (int s = 0; s < B->count; ++s)
           {
             mpdg_heat *m = B->members[s];
             if (m->desc.degree != deg)
               continue;
-            const CellParams<Number> &prm = *reinterpret_cast<const CellParams<Number> *>(
+            CellParams<Number> prm = *reinterpret_cast<const CellParams<Number> *>(
               fp32 ? (const void *)&m->prm32 : (const void *)&m->prm64);
-            const VmultGrid vg = vmult_grid(prm.nx, prm.ny, prm.nz, cpb, prm.zlayers, 1 << 30);
+            CellParams<double> prm64 = m->prm64;
+            if (variant < 0)
+              {
+                variant = prm.variant;
+                tile = prm.tile_log2;
+                variant64 = prm64.variant;
+                tile64 = prm64.tile_log2;
+              }
+            // one kernel form per group and precision
+            prm.variant = variant;
+            prm.tile_log2 = tile;
+            prm64.variant = variant64;
+            prm64.tile_log2 = tile64;
+            const VmultGrid vg = vmult_items<Number>(deg, prm);
+            const int per_block = variant == kVariantCell ? cell_items_per_block() : 1;
             BatchOperator<Number> op{};
             op.prm = prm;
             op.src = src + B->off[s];
@@ -996,25 +1117,27 @@ namespace
             for (int c = 0; c < 8; ++c)
               {
                 op.items[c] = vg.items[c];
-                items.push_back(vg.items[c]);
+                items.push_back((vg.items[c] + per_block - 1) / per_block);
               }
             ops.push_back(op);
             BatchOperator<double> ox{};
-            ox.prm = m->prm64;
+            ox.prm = prm64;
             ox.src = B->X.ptr + B->off[s];
             ox.dst = B->Q.ptr + B->off[s];
             ox.heav = m->heav64.ptr;
             ox.solve = s;
-            const VmultGrid vg64 = vmult_grid(m->prm64.nx, m->prm64.ny, m->prm64.nz, cpb64,
-                                              m->prm64.zlayers, 1 << 30);
+            const VmultGrid vg64 = vmult_items<double>(deg, prm64);
             for (int c = 0; c < 8; ++c)
               ox.items[c] = vg64.items[c];
             opsx.push_back(ox);
             g.members.push_back(s);
           }
+        g.variant = variant;
+        g.variant64 = variant64;
         g.nops = (int)ops.size();
         g.block_off.assign(8 * (g.nops + 1), 0);
-        const int max_blocks = vmult_resident_blocks(deg, fp32, B->coef, B->vel) * device_cu_count();
+        const int max_blocks =
+          vmult_resident_blocks(deg, fp32, B->coef, B->vel, variant) * device_cu_count();
         g.plan = batch_vmult_plan(items.data(), g.nops, max_blocks, g.block_off.data());
         g.pbase = slots;
         slots += g.plan.total_slots;
@@ -1038,7 +1161,8 @@ namespace
   mpdg_status batch_vmult_pq(mpdg_heat_batch *B, std::vector<BatchGroup<Number>> &groups)
   {
     for (auto &g : groups)
-      MPDG_HIP(launch_vmult_batch<Number>(g.degree, B->coef, B->vel, g.ops.ptr, g.dev_block_off.ptr,
+      MPDG_HIP(launch_vmult_batch<Number>(g.degree, B->coef, B->vel, g.variant, g.ops.ptr,
+                                          g.dev_block_off.ptr,
                                           g.nops, g.plan, B->partials.ptr + g.pbase, B->scal.ptr,
                                           B->stream));
     for (auto &g : groups)
@@ -1289,7 +1413,8 @@ mpdg_status mpdg_heat_batch_solve(mpdg_heat_batch *b, double *const *x, const do
   auto outer_residual = [&]() -> mpdg_status {
     // R = B - A64 X for every member, ||R|| per member
     for (auto &g : b->g64)
-      MPDG_HIP(launch_vmult_batch<double>(g.degree, b->coef, b->vel, g.opsx.ptr, g.dev_block_off.ptr,
+      MPDG_HIP(launch_vmult_batch<double>(g.degree, b->coef, b->vel, g.variant64, g.opsx.ptr,
+                                          g.dev_block_off.ptr,
                                           g.nops, g.plan, nullptr, nullptr, b->stream));
     MPDG_HIP(launch_batch_residual(b->R.ptr, b->B.ptr, b->Q.ptr, b->seg.ptr, S, b->nblocks,
                                    b->partials.ptr, b->res.ptr, b->stream));
@@ -1315,7 +1440,8 @@ mpdg_status mpdg_heat_batch_solve(mpdg_heat_batch *b, double *const *x, const do
         }
       // r = b - A x0 for all members, then the PCG start
       for (auto &g : b->g64)
-        MPDG_HIP(launch_vmult_batch<double>(g.degree, b->coef, b->vel, g.opsx.ptr, g.dev_block_off.ptr,
+        MPDG_HIP(launch_vmult_batch<double>(g.degree, b->coef, b->vel, g.variant64, g.opsx.ptr,
+                                          g.dev_block_off.ptr,
                                             g.nops, g.plan, nullptr, nullptr, b->stream));
       MPDG_HIP(launch_batch_cg_init<double>(b->B.ptr, b->Q.ptr, b->R.ptr, b->P.ptr, b->D.ptr,
                                             b->seg.ptr, S, b->nblocks,

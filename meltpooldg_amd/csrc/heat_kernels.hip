@@ -20,6 +20,7 @@
 //   integrate values|gradients), material lookup :1133-1210,
 //   compute_inverse_diagonal_from_matrixfree :271-284.
 #include "ctables.hpp"
+#include "device_util.hpp"
 #include "kernels.hpp"
 
 namespace mpdg
@@ -88,43 +89,6 @@ namespace mpdg
         }
     }
 
-    // y = M x with a compile-time even-odd table (entries folded into the code).
-    template <int N, int SIGN, typename Number>
-    __device__ __forceinline__ void eo_apply(const ct::EO<N> &M, const Number (&x)[N], Number (&y)[N])
-    {
-      constexpr int H = N / 2;
-      Number E[H], O[H];
-#pragma unroll
-      for (int j = 0; j < H; ++j)
-        {
-          E[j] = x[j] + x[N - 1 - j];
-          O[j] = x[j] - x[N - 1 - j];
-        }
-#pragma unroll
-      for (int i = 0; i < H; ++i)
-        {
-          Number a = Number(M.e[i][0]) * E[0];
-          Number b = Number(M.o[i][0]) * O[0];
-#pragma unroll
-          for (int j = 1; j < H; ++j)
-            {
-              a += Number(M.e[i][j]) * E[j];
-              b += Number(M.o[i][j]) * O[j];
-            }
-          if constexpr (N % 2 == 1)
-            a += Number(M.mc[i]) * x[H];
-          y[i] = a + b;
-          y[N - 1 - i] = SIGN > 0 ? a - b : b - a;
-        }
-      if constexpr (N % 2 == 1)
-        {
-          Number m = Number(M.mm) * x[H];
-#pragma unroll
-          for (int j = 0; j < H; ++j)
-            m += Number(M.mr[j]) * (SIGN > 0 ? E[j] : O[j]);
-          y[H] = m;
-        }
-    }
 
     template <typename Number>
     __device__ __forceinline__ bool on_dirichlet(const CellParams<Number> &prm, int i, int d)
@@ -132,91 +96,6 @@ namespace mpdg
       const int n1 = d == 0 ? prm.n1x : (d == 1 ? prm.n1y : prm.n1z);
       return (((prm.faces >> (2 * d)) & 1u) && i == 0) ||
              (((prm.faces >> (2 * d + 1)) & 1u) && i == n1 - 1);
-    }
-
-    // Raw buffer access: an offset at or past the buffer's byte size reads 0
-    // and drops the store, so masked lanes need no branch.
-    constexpr uint32_t kOutOfRange = 0xFFFFFF00u;
-
-    __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *p, uint32_t bytes)
-    {
-      return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes,
-                                               0x00020000);
-    }
-
-    typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
-
-    __device__ __forceinline__ double buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off, double)
-    {
-      return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
-    }
-    __device__ __forceinline__ float buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off, float)
-    {
-      return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
-    }
-    __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t off, double v)
-    {
-      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, off, 0, 0);
-    }
-    __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t off, float v)
-    {
-      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, 0);
-    }
-
-    // on_dirichlet without short-circuit branches
-    template <typename Number>
-    __device__ __forceinline__ bool dirichlet_node(const CellParams<Number> &prm, int i, int n1, int d)
-    {
-      const unsigned lo = (prm.faces >> (2 * d)) & 1u;
-      const unsigned hi = (prm.faces >> (2 * d + 1)) & 1u;
-      return (lo & (unsigned)(i == 0)) | (hi & (unsigned)(i == n1 - 1));
-    }
-
-    // Material::compute_parameters for gas_liquid with the blend of
-    // LevelSet::Tools::interpolate (material.templates.hpp:515-524,
-    // level_set_tools.hpp:40-43); rho*cp as in heat_diffuse_operator.cpp:1167-1210.
-    template <typename Number>
-    __device__ __forceinline__ void two_phase(const CellParams<Number> &prm, Number H, Number &rc,
-                                              Number &k)
-    {
-      const Number w = prm.sharp ? (H > Number(0.5) ? Number(1) : Number(0)) : H;
-      const Number one_w = Number(1) - w;
-      k = one_w * prm.g_k + w * prm.l_k;
-      if (prm.volume_specific)
-        rc = one_w * prm.g_rhocp + w * prm.l_rhocp;
-      else
-        rc = (one_w * prm.g_rho + w * prm.l_rho) * (one_w * prm.g_cp + w * prm.l_cp);
-    }
-
-    // Deterministic block sum (fixed butterfly + fixed-order wave sum).
-    template <int THREADS>
-    __device__ __forceinline__ double block_sum(double v, double *sh)
-    {
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1)
-        v += __shfl_xor(v, off, 64);
-      const int wave = threadIdx.x >> 6;
-      if ((threadIdx.x & 63) == 0)
-        sh[wave] = v;
-      __syncthreads();
-      double s = 0.0;
-      if (threadIdx.x == 0)
-        {
-          for (int w = 0; w < THREADS / 64; ++w)
-            s += sh[w];
-        }
-      return s; // valid in thread 0
-    }
-
-    // Sum of partials[0..count) in a fixed order by one block.
-    template <int THREADS>
-    __device__ __forceinline__ double ordered_sum(const double *partials, int count, double *sh)
-    {
-      double v = 0.0;
-      for (int i = threadIdx.x; i < count; i += THREADS)
-        v += __hip_atomic_load(partials + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __syncthreads();
-      return block_sum<THREADS>(v, sh);
     }
 
     struct Segment
@@ -910,37 +789,9 @@ namespace mpdg
         return;
       const double acc = vmult_body<P, Number, COEF, VEL, CW, R, WPB>(
         src, dst, heav, prm, colour, layers, items, blockIdx.x, gridDim.x);
+      // p . Ap: sum over cells of u_cell . y_cell (constrained entries are 0)
       if (partials != nullptr)
-        {
-          // p . Ap: sum over cells of u_cell . y_cell (constrained entries are 0)
-          __shared__ double red[WPB];
-          const double bs = block_sum<THREADS>(acc, red);
-          __shared__ bool is_last;
-          if (threadIdx.x == 0)
-            {
-              partials[partial_offset + blockIdx.x] = bs;
-              bool lst = false;
-              if (finalize)
-                {
-                  __threadfence();
-                  const unsigned prev = atomicAdd(&scal->counter, 1u);
-                  lst = prev == gridDim.x - 1;
-                }
-              is_last = lst;
-            }
-          __syncthreads();
-          if (is_last)
-            {
-              __threadfence();
-              const double pap = ordered_sum<THREADS>(partials, partial_total, red);
-              if (threadIdx.x == 0)
-                {
-                  scal->pap = pap;
-                  scal->alpha = scal->rz / pap;
-                  scal->counter = 0u;
-                }
-            }
-        }
+        pass_partials<THREADS>(acc, partials, partial_offset, partial_total, finalize, scal);
     }
 
     // Batched vmult: the workgroups of one launch are split among the
@@ -955,22 +806,7 @@ namespace mpdg
                          const SolverScalars *__restrict__ scal)
     {
       constexpr int THREADS = 64 * WPB;
-      __shared__ int s_sh;
-      if (threadIdx.x == 0)
-        {
-          int lo = 0, hi = nops; // last s with block_off[s] <= blockIdx.x
-          while (hi - lo > 1)
-            {
-              const int mid = (lo + hi) / 2;
-              if (block_off[mid] <= (int)blockIdx.x)
-                lo = mid;
-              else
-                hi = mid;
-            }
-          s_sh = lo;
-        }
-      __syncthreads();
-      const int s = s_sh;
+      const int s = find_block_owner(block_off, nops);
       const BatchOperator<Number> &op = ops[s];
       double acc = 0.0;
       const bool skip = scal != nullptr && scal[op.solve].done;
@@ -1438,6 +1274,9 @@ namespace mpdg
                           const Number *heaviside, double *partials, SolverScalars *scal,
                           hipStream_t stream)
   {
+    if (prm.variant == kVariantCell && degree <= kCellMaxDegree)
+      return launch_vmult_cell<Number>(degree, coef, vel, prm, src, dst, heaviside, partials, scal,
+                                       stream);
     switch (degree)
       {
         case 1: return vmult_deg<Number, 1>(coef, vel, prm, tab, src, dst, heaviside, partials, scal, stream);
@@ -1567,10 +1406,23 @@ namespace mpdg
 #undef MPDG_CPB
   }
 
-  int vmult_resident_blocks(int degree, bool fp32, int coef, bool vel)
+  template <typename Number>
+  VmultGrid vmult_items(int degree, const CellParams<Number> &prm)
+  {
+    if (prm.variant == kVariantCell && degree <= kCellMaxDegree)
+      return cell_grid(prm.nx, prm.ny, prm.nz, prm.tile_log2, prm.zlayers, 1 << 30);
+    return vmult_grid(prm.nx, prm.ny, prm.nz, vmult_cells_per_block(degree, sizeof(Number) == 4),
+                      prm.zlayers, 1 << 30);
+  }
+  template VmultGrid vmult_items<double>(int, const CellParams<double> &);
+  template VmultGrid vmult_items<float>(int, const CellParams<float> &);
+
+  int vmult_resident_blocks(int degree, bool fp32, int coef, bool vel, int variant)
   {
     (void)coef;
     (void)vel;
+    if (variant == kVariantCell && degree <= kCellMaxDegree)
+      return cell_resident_blocks(degree, fp32);
 #define MPDG_RES(P)                                                                               \
   case P:                                                                                         \
     return fp32 ? resident_of<float, P, 0, false>() : resident_of<double, P, 0, false>();
@@ -1588,10 +1440,14 @@ namespace mpdg
   }
 
   template <typename Number>
-  hipError_t launch_vmult_batch(int degree, int coef, bool vel, const BatchOperator<Number> *ops,
-                                const int *block_off, int nops, const BatchVmultPlan &plan,
-                                double *partials, const SolverScalars *scal, hipStream_t stream)
+  hipError_t launch_vmult_batch(int degree, int coef, bool vel, int variant,
+                                const BatchOperator<Number> *ops, const int *block_off, int nops,
+                                const BatchVmultPlan &plan, double *partials,
+                                const SolverScalars *scal, hipStream_t stream)
   {
+    if (variant == kVariantCell && degree <= kCellMaxDegree)
+      return launch_vmult_cell_batch<Number>(degree, coef, vel, ops, block_off, nops, plan,
+                                             partials, scal, stream);
     switch (degree)
       {
         case 1: return vmult_batch_deg<Number, 1>(coef, vel, ops, block_off, nops, plan, partials, scal, stream);
@@ -1612,10 +1468,10 @@ namespace mpdg
     return hipGetLastError();
   }
 
-  template hipError_t launch_vmult_batch<double>(int, int, bool, const BatchOperator<double> *,
+  template hipError_t launch_vmult_batch<double>(int, int, bool, int, const BatchOperator<double> *,
                                                  const int *, int, const BatchVmultPlan &, double *,
                                                  const SolverScalars *, hipStream_t);
-  template hipError_t launch_vmult_batch<float>(int, int, bool, const BatchOperator<float> *,
+  template hipError_t launch_vmult_batch<float>(int, int, bool, int, const BatchOperator<float> *,
                                                 const int *, int, const BatchVmultPlan &, double *,
                                                 const SolverScalars *, hipStream_t);
 

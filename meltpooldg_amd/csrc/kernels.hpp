@@ -28,6 +28,8 @@ namespace mpdg
     int sharp, volume_specific;
     int zlayers;              // cell layers per z-chunk of a vmult work item
     int blocks_per_cu;        // persistent workgroups per CU (0: occupancy limit)
+    int variant;              // vmult form: kVariantLine or kVariantCell
+    int tile_log2;            // cell form: log2 of the tile's cells along x (3..6)
     const Number *sin2pi[3]; // sin(2 pi x) at the quadrature coordinates of each direction
     const Number *sinpi2[3]; // sin^2(pi x) at the same coordinates
   };
@@ -65,6 +67,17 @@ namespace mpdg
     kCoefSine = 1,
     kCoefHeaviside = 2
   };
+
+  // Form of the vmult kernel. Line: the lanes of a wavefront own the 1D lines
+  // of a few cells and move between line directions through LDS (any degree).
+  // Cell: each lane owns one whole cell in registers, a wavefront a tile of
+  // 64 cells; shared faces are summed with cross-lane shuffles (degree <= 2).
+  enum VmultVariant
+  {
+    kVariantLine = 0,
+    kVariantCell = 1
+  };
+  constexpr int kCellMaxDegree = 2;
 
   // Launch shape of one colour pass.
   struct ColourGrid
@@ -111,6 +124,19 @@ namespace mpdg
                           const Number *heaviside, double *partials, SolverScalars *scal,
                           hipStream_t stream);
 
+  // Cell form of the vmult (cell_kernels.hip), same contract as launch_vmult,
+  // degree <= kCellMaxDegree; launch_vmult forwards to it when
+  // prm.variant == kVariantCell.
+  template <typename Number>
+  hipError_t launch_vmult_cell(int degree, int coef, bool vel, const CellParams<Number> &prm,
+                               const Number *src, Number *dst, const Number *heaviside,
+                               double *partials, SolverScalars *scal, hipStream_t stream);
+  // Per colour: tiles of 2^tile_log2 x 2^(6-tile_log2) cells times z-chunks,
+  // four tiles in flight per workgroup.
+  VmultGrid cell_grid(int nx, int ny, int nz, int tile_log2, int layers, int max_blocks);
+  int cell_items_per_block();
+  int cell_resident_blocks(int degree, bool fp32);
+
   template <typename Number>
   hipError_t launch_diagonal(int degree, int coef, bool vel, const CellParams<Number> &prm,
                              const DiagTables<Number> &tab, Number *diag,
@@ -140,13 +166,22 @@ namespace mpdg
   // Builds block_off (8 x (nops+1), host) and the plan from per-operator items.
   BatchVmultPlan batch_vmult_plan(const int *items /* nops x 8 */, int nops, int max_blocks,
                                   int *block_off);
-  int vmult_resident_blocks(int degree, bool fp32, int coef, bool vel);
+  int vmult_resident_blocks(int degree, bool fp32, int coef, bool vel, int variant);
   int vmult_cells_per_block(int degree, bool fp32);
+  // Work items per colour of one operator's vmult in its kernel form.
+  template <typename Number>
+  VmultGrid vmult_items(int degree, const CellParams<Number> &prm);
 
   template <typename Number>
-  hipError_t launch_vmult_batch(int degree, int coef, bool vel, const BatchOperator<Number> *ops,
+  hipError_t launch_vmult_batch(int degree, int coef, bool vel, int variant,
+                                const BatchOperator<Number> *ops,
                                 const int *block_off, int nops, const BatchVmultPlan &plan,
                                 double *partials, const SolverScalars *scal, hipStream_t stream);
+  template <typename Number>
+  hipError_t launch_vmult_cell_batch(int degree, int coef, bool vel,
+                                     const BatchOperator<Number> *ops, const int *block_off,
+                                     int nops, const BatchVmultPlan &plan, double *partials,
+                                     const SolverScalars *scal, hipStream_t stream);
   // alpha_s = rz_s / sum of solve s's vmult partials (one block per operator).
   hipError_t launch_batch_pap_finalize(const int *block_off, const int *ops_solve, int nops,
                                        const BatchVmultPlan &plan, const double *partials,

@@ -129,6 +129,7 @@ class HeatDiffuseOperator:
         self.n_cells = n
         self.degree = degree
         self.material = mat
+        self._precision = "fp32" if precision == "fp32" else "fp64"
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -171,17 +172,44 @@ class HeatDiffuseOperator:
         self._check_vec(diag, "float64")
         L.check(self._lib.mpdg_heat_inverse_diagonal(self._h, _ptr(diag)))
 
-    def get_tuning(self):
-        t = L.Tuning()
-        L.check(self._lib.mpdg_heat_get_tuning(self._h, ctypes.byref(t)))
-        return {"z_layers": t.z_layers, "blocks_per_cu": t.blocks_per_cu}
+    _VARIANTS = {"line": L.MPDG_VMULT_LINE, "cell": L.MPDG_VMULT_CELL}
 
-    def set_tuning(self, z_layers=None, blocks_per_cu=None):
-        cur = self.get_tuning()
+    def _prec_code(self, precision):
+        precision = self._precision if precision is None else precision
+        if precision not in ("fp64", "fp32"):
+            raise ValueError("precision must be 'fp64' or 'fp32'")
+        return L.MPDG_FP32 if precision == "fp32" else L.MPDG_FP64
+
+    @classmethod
+    def _tuning_dict(cls, t):
+        names = {v: k for k, v in cls._VARIANTS.items()}
+        return {"z_layers": t.z_layers, "blocks_per_cu": t.blocks_per_cu,
+                "variant": names.get(t.variant, t.variant), "tile_log2": t.tile_log2}
+
+    def get_tuning(self, precision=None):
+        """Launch tuning of the vmult kernels for one precision (engine extension)."""
+        t = L.Tuning()
+        L.check(self._lib.mpdg_heat_get_tuning(self._h, self._prec_code(precision), ctypes.byref(t)))
+        return self._tuning_dict(t)
+
+    def set_tuning(self, z_layers=None, blocks_per_cu=None, variant=None, tile_log2=None,
+                   precision=None):
+        cur = self.get_tuning(precision)
         t = L.Tuning()
         t.z_layers = int(cur["z_layers"] if z_layers is None else z_layers)
         t.blocks_per_cu = int(cur["blocks_per_cu"] if blocks_per_cu is None else blocks_per_cu)
-        L.check(self._lib.mpdg_heat_set_tuning(self._h, ctypes.byref(t)))
+        v = cur["variant"] if variant is None else variant
+        t.variant = self._VARIANTS[v] if isinstance(v, str) else int(v)
+        t.tile_log2 = int(cur["tile_log2"] if tile_log2 is None else tile_log2)
+        L.check(self._lib.mpdg_heat_set_tuning(self._h, self._prec_code(precision), ctypes.byref(t)))
+
+    def autotune(self, precision=None, repeats=5):
+        """Times the candidate kernel forms and launch shapes of this operator's
+        vmult, installs the fastest and returns it (engine extension)."""
+        t = L.Tuning()
+        L.check(self._lib.mpdg_heat_autotune(self._h, self._prec_code(precision), int(repeats),
+                                             ctypes.byref(t)))
+        return self._tuning_dict(t)
 
     def time_vmult(self, dst, src, repeats=10):
         ms = ctypes.c_double()
