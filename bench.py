@@ -43,22 +43,53 @@ def algorithmic_counts(n_cells, degree, bytes_per=8):
             "flops": cells * (12 * nq**4 + 36 * nq**3)}
 
 
-def cpu_baseline(sample_vmults=3, threads=None):
+def cpu_baseline(threads=None, budget_s=10.0, min_samples=20):
     """Times the oracle's cell-wise sum-factorised FP64 vmult on the host cores
-    (the reference cannot be built here: it needs deal.II + p4est + Trilinos + MPI)."""
+    (the reference cannot be built here: it needs deal.II + p4est + Trilinos +
+    MPI): median of at least `min_samples` vmults after 3 warm-ups, sampling
+    for about `budget_s` seconds; also one core, 3 vmults."""
     from oracle import cpu
     threads = threads or min(16, os.cpu_count() or 1)
-    cpu.set_threads(threads)
     pb = cpu.Problem(N_CELLS, DEGREE, dt=DT, k_amplitude=K_AMP)
     u = pb.random(0)
-    pb.vmult(u)  # warm-up
-    t0 = time.perf_counter()
-    for _ in range(sample_vmults):
-        pb.vmult(u)
-    t = (time.perf_counter() - t0) / sample_vmults
+
+    def sample(nthreads, warm, least, budget):
+        cpu.set_threads(nthreads)
+        for _ in range(warm):
+            pb.vmult(u)
+        ts = []
+        t_start = time.perf_counter()
+        while len(ts) < least or time.perf_counter() - t_start < budget:
+            t0 = time.perf_counter()
+            pb.vmult(u)
+            ts.append(time.perf_counter() - t0)
+            if len(ts) >= 10 * least:
+                break
+        return float(np.median(ts)), len(ts)
+
+    t, k = sample(threads, 3, min_samples, budget_s)
+    t1, k1 = sample(1, 0, 3, 0.0)
+    cpu.set_threads(threads)
     return {"value": round(pb.N / t / 1e9, 4), "unit": "GDoF/s", "cores": threads, "kind": "port",
-            "sample": f"{sample_vmults} FP64 vmults of the 64^3 Q4 operator (C++ restatement of the "
-                      f"reference cell loop, OpenMP {threads} threads), {t*1e3:.1f} ms each"}
+            "sample": f"median of {k} FP64 vmults of the 64^3 Q4 operator after 3 warm-ups "
+                      f"(C++ restatement of the reference cell loop, not deal.II; OpenMP {threads} "
+                      f"threads), {t*1e3:.1f} ms each; 1 core: {t1*1e3:.0f} ms per vmult "
+                      f"({pb.N / t1 / 1e9:.4f} GDoF/s, {k1} vmults)",
+            "one_core_value": round(pb.N / t1 / 1e9, 5)}
+
+
+def measured_traffic(kernel_name):
+    """HBM bytes per launch of the vmult kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary, if it was taken on this kernel."""
+    path = os.path.join(ROOT, "profiles", "vmult_q4_64_fp64_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("kernel") != kernel_name:
+        return None
+    return t.get("bytes_per_launch")
 
 
 def copy_calibration(args, torch, dev):
@@ -89,6 +120,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-solve", action="store_true", help="skip the CG time-to-solution legs")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default launch tuning instead of autotuning the operator")
+    ap.add_argument("--tuning", default=None,
+                    help="fixed FP64 tuning z_layers,blocks_per_cu[,variant,tile_log2] instead of "
+                         "autotuning (profiling runs)")
     ap.add_argument("--mode", choices=["vmult", "copy"], default="vmult",
                     help="copy: time a plain device copy of one DoF vector (HBM counter calibration)")
     ap.add_argument("--config", type=int, default=1,
@@ -112,12 +148,23 @@ def main():
         return bench_configs.run(args.config, args)
     op = HeatDiffuseOperator(N_CELLS, DEGREE, conductivity_amplitude=K_AMP)
     op.reset_time_increment(DT)
+    if args.tuning:
+        f = args.tuning.split(",")
+        op.set_tuning(z_layers=int(f[0]), blocks_per_cu=int(f[1]),
+                      variant=f[2] if len(f) > 2 else None,
+                      tile_log2=int(f[3]) if len(f) > 3 else None, precision="fp64")
+        tuning = op.get_tuning("fp64")
+    elif args.no_tune:
+        tuning = op.get_tuning("fp64")
+    else:
+        tuning = op.autotune("fp64", repeats=3)
     cnt = algorithmic_counts(N_CELLS, DEGREE)
     N = op.n_dofs()
     assert N == cnt["N"]
     src = torch.tensor(random_vector(N_CELLS, DEGREE, 0), dtype=torch.float64, device=dev)
     dst = torch.empty_like(src)
 
+    # the library launches on the handle's stream: torch's current stream here
     stream = torch.cuda.current_stream()
     for _ in range(args.warmup):
         op.vmult(dst, src)
@@ -135,28 +182,36 @@ def main():
     ms_step = wall * 1e3 / args.steps
     gdofs = N / (ms_step * 1e-3) / 1e9
 
-    # vmult = 8 launches of the cell kernel (one per colour); per-launch bytes
-    # and flops are 1/8 of the vmult's, per-launch time 1/8 of the vmult's
+    # one vmult = 8 launches of the cell kernel (one per colour pass): per-launch
+    # algorithmic flops and bytes are 1/8 of the vmult's, the launch time 1/8 of
+    # the event time per vmult
     t_k = ms_event * 1e-3
     ach_flops = cnt["flops"] / t_k / 1e12
     ach_bytes = cnt["bytes"] / t_k / 1e9
+    kernel = "vmult_kernel<4, double, 1, false, 2, 1, 4, 4>"  # line form; the cell form is p <= 2
     roofline = {"bound": "mfma", "achieved": round(ach_flops, 3), "peak": FP64_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(ach_flops / FP64_PEAK_TFLOPS, 4), "traffic": None,
-                "kernel": "vmult_kernel<4,double,sine> x 8 colour passes",
+                "unit": "TFLOP/s", "frac": round(ach_flops / FP64_PEAK_TFLOPS, 4),
+                "traffic": measured_traffic(kernel),
+                "kernel": kernel + " x 8 colour passes",
                 "launch_ms": round(ms_event / 8, 5),
+                "algorithmic_bytes_per_launch": cnt["bytes"] // 8,
+                "algorithmic_flops_per_launch": cnt["flops"] // 8,
                 "hbm_achieved_GBs": round(ach_bytes, 1),
                 "hbm_frac": round(ach_bytes / HBM_PEAK_GBS, 4),
-                "algorithmic_bytes_per_vmult": cnt["bytes"],
-                "algorithmic_flops_per_vmult": cnt["flops"]}
+                "tuning": tuning}
 
     solve = None
     if not args.no_solve:
         b = torch.tensor(random_vector(N_CELLS, DEGREE, 1), dtype=torch.float64, device=dev)
         solve = {}
+        if not (args.no_tune or args.tuning):
+            solve["fp32_tuning"] = op.autotune("fp32", repeats=3)
         for name, mixed in (("fp64", False), ("fp32_mixed", True)):
-            x = torch.zeros_like(b)
             data = LinearSolverData(solver_type="CG", preconditioner_type="Diagonal",
                                     rel_tolerance=1e-8, mixed_precision=mixed)
+            x = torch.zeros_like(b)
+            LinearSolver.solve(op, x, b, data)  # warm-up: workspace allocation
+            x.zero_()
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             LinearSolver.solve(op, x, b, data)

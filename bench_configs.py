@@ -53,9 +53,12 @@ def _vmult_time(op, torch, dtype, n_rep=20):
 
 
 def _solve(op, b, torch, **kw):
+    """Solve from x = 0 after one untimed warm-up solve (workspace allocation)."""
     from meltpooldg_amd import LinearSolver, LinearSolverData
     x = torch.zeros_like(b)
     data = LinearSolverData(preconditioner_type="Diagonal", **kw)
+    LinearSolver.solve(op, x, b, data)
+    x.zero_()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     LinearSolver.solve(op, x, b, data)
@@ -119,6 +122,8 @@ def config2(args, torch):
     op32 = HeatDiffuseOperator(n, p, material=mat, precision="fp32")
     op32.reset_time_increment(1e-3)
     op32.set_heaviside(_t(H, torch))
+    results["tuning"] = {"fp64": op.autotune("fp64", repeats=3), "fp32": op.autotune("fp32", repeats=3),
+                         "fp32_operator": op32.autotune("fp32", repeats=3)}
     pb = cpu.Problem(n, p, dt=1e-3, material={"type": "gas_liquid", "gas_rho": 1.0, "gas_cp": 1.0,
                                               "gas_k": 1.0, "liq_rho": 10.0, "liq_cp": 1.0,
                                               "liq_k": 100.0}, heaviside=H)
@@ -151,6 +156,8 @@ def config4(args, torch):
     op.reset_time_increment(5e-4)
     op32 = HeatDiffuseOperator(n, p, material=mat, velocity="vortex", velocity_w0=0.2, precision="fp32")
     op32.reset_time_increment(5e-4)
+    tuning = {"fp64": op.autotune("fp64", repeats=3), "fp32": op.autotune("fp32", repeats=3),
+              "fp32_operator": op32.autotune("fp32", repeats=3)}
     pb = cpu.Problem(n, p, dt=5e-4, material={"type": "gas", "gas_rho": 1.0, "gas_cp": 1.0,
                                               "gas_k": 1e-2}, velocity=True, w0=0.2)
     err, t_cpu_vmult = _oracle_vmult_check(pb, op, torch)
@@ -160,7 +167,7 @@ def config4(args, torch):
     xref, _ = _solve(op, b, torch, solver_type="GMRES", rel_tolerance=1e-12)
     e = lambda x: float((x - xref).abs().max() / xref.abs().max())
     return {"config": 4, "workload": "64^3 Q5 convection-diffusion (vortex), GMRES(30)+Jacobi to 1e-8",
-            "n_dofs": op.n_dofs(), "vmult_rel_err_vs_oracle": err, "cpu_vmult_ms": t_cpu_vmult * 1e3,
+            "n_dofs": op.n_dofs(), "tuning": tuning, "vmult_rel_err_vs_oracle": err, "cpu_vmult_ms": t_cpu_vmult * 1e3,
             "gpu_vmult_ms_fp64": _vmult_time(op, torch, torch.float64),
             "gpu_vmult_ms_fp32": _vmult_time(op32, torch, torch.float32),
             "fp64": {"s": rep64["wall_s"], "iterations": rep64["iterations"], "err_vs_fp64_1e-12": e(x64)},

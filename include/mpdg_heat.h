@@ -176,14 +176,16 @@ mpdg_status mpdg_heat_batch_solve(mpdg_heat_batch *b, double *const *x, const do
  * MatrixFree::AdditionalData). Results do not depend on it beyond the order
  * of floating-point summation. */
 typedef enum {
-  MPDG_VMULT_LINE = 0, /* lanes own the 1D lines of a few cells, LDS between directions */
-  MPDG_VMULT_CELL = 1  /* each lane owns a whole cell in registers (degree <= 2) */
+  MPDG_VMULT_LINE = 0,     /* lanes own the 1D lines of a few cells, LDS between directions */
+  MPDG_VMULT_CELL = 1,     /* each lane owns a whole cell in registers (degree <= 2) */
+  MPDG_VMULT_LINE_MFMA = 2 /* line form with the 1D contractions on FP32 matrix cores
+                              (MPDG_FP32 tuning only, degree >= 3) */
 } mpdg_vmult_variant;
 
 typedef struct {
   int z_layers;       /* cell layers swept per vmult work item (>= 1) */
   int blocks_per_cu;  /* persistent workgroups per CU, 0 = occupancy limit (<= 16) */
-  int variant;        /* mpdg_vmult_variant; MPDG_VMULT_CELL needs degree <= 2 */
+  int variant;        /* mpdg_vmult_variant (see the limits above) */
   int tile_log2;      /* cell variant: log2 of the tile's cells along x (3..6); a
                          wavefront's tile is 2^tile_log2 x 2^(6 - tile_log2) cells */
 } mpdg_tuning;

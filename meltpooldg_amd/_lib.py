@@ -83,6 +83,7 @@ class BatchReport(ctypes.Structure):
 
 MPDG_VMULT_LINE = 0
 MPDG_VMULT_CELL = 1
+MPDG_VMULT_LINE_MFMA = 2
 
 
 class Tuning(ctypes.Structure):

@@ -204,6 +204,7 @@ namespace mpdg
     //   kind 1: C[q][r]                (collocation derivative on Gauss points)
     //   kind 2: w_r C[r][q] / w_q      (W^-1 C^T W: the flux back to values)
     //   kind 3: S[q][i] w_q, as [i][q] (S^T W: integrate)
+    //   kind 4: S^T, kind 5: C^T       (unweighted transposes)
     template <int P>
     constexpr Mat<P + 1> cell_matrix(const Tab<P + 1> &t, int kind)
     {
@@ -214,7 +215,9 @@ namespace mpdg
           m.a[i][j] = kind == 0 ? t.S[i][j]
                     : kind == 1 ? t.C[i][j]
                     : kind == 2 ? t.w[j] * t.C[j][i] / t.w[i]
-                                : t.S[j][i] * t.w[j];
+                    : kind == 3 ? t.S[j][i] * t.w[j]
+                    : kind == 4 ? t.S[j][i]
+                                : t.C[j][i];
       return m;
     }
 
@@ -226,6 +229,8 @@ namespace mpdg
       static constexpr Mat<P + 1> C = cell_matrix<P>(t, 1);
       static constexpr Mat<P + 1> CW = cell_matrix<P>(t, 2);
       static constexpr Mat<P + 1> SW = cell_matrix<P>(t, 3);
+      static constexpr Mat<P + 1> St = cell_matrix<P>(t, 4);
+      static constexpr Mat<P + 1> Ct = cell_matrix<P>(t, 5);
     };
 
     // All even-odd tables of the vmult kernel for degree P.

@@ -124,29 +124,46 @@ namespace
   // sweep over z-chunk length, workgroups per CU and variant on an MI355X at
   // the sizes of the benchmark workloads (mpdg_heat_autotune refines it for a
   // given operator).
-  mpdg_tuning default_tuning(int degree, bool fp32)
+  mpdg_tuning default_tuning(int degree, bool fp32, int64_t cells)
   {
     mpdg_tuning t{};
     t.variant = MPDG_VMULT_LINE;
     t.tile_log2 = 4;
+    // the cell form needs about a million cells to fill the chip in each of
+    // its eight colour passes (64 cells per wavefront)
+    if (degree == 1 && cells >= (int64_t(1) << 20))
+      {
+        t.variant = MPDG_VMULT_CELL;
+        t.tile_log2 = 5;
+        t.z_layers = 2;
+        t.blocks_per_cu = 0;
+        return t;
+      }
     switch (degree)
       {
         case 1: t.z_layers = fp32 ? 4 : 2; t.blocks_per_cu = fp32 ? 4 : 0; break;
         case 2: t.z_layers = fp32 ? 4 : 2; t.blocks_per_cu = fp32 ? 8 : 4; break;
-        case 3: t.z_layers = 4; t.blocks_per_cu = 4; break;
-        case 4: t.z_layers = fp32 ? 2 : 4; t.blocks_per_cu = 4; break;
-        case 5: t.z_layers = fp32 ? 2 : 4; t.blocks_per_cu = fp32 ? 4 : 0; break;
+        case 3: t.z_layers = fp32 ? 2 : 4; t.blocks_per_cu = fp32 ? 8 : 4; break;
+        case 4: t.z_layers = fp32 ? 2 : 4; t.blocks_per_cu = fp32 ? 8 : 4; break;
+        case 5:
+          // FP32: the matrix-core form (256-VGPR budget, no spills) is the faster one
+          t.z_layers = fp32 ? 2 : 4;
+          t.blocks_per_cu = 0;
+          t.variant = fp32 ? MPDG_VMULT_LINE_MFMA : MPDG_VMULT_LINE;
+          break;
         default: t.z_layers = 2; t.blocks_per_cu = 4; break;
       }
     return t;
   }
 
-  bool tuning_valid(const mpdg_tuning &t, int degree)
+  bool tuning_valid(const mpdg_tuning &t, int degree, int precision)
   {
     return t.z_layers >= 1 && t.blocks_per_cu >= 0 && t.blocks_per_cu <= kMaxResidentPerCu &&
            (t.variant == MPDG_VMULT_LINE ||
             (t.variant == MPDG_VMULT_CELL && degree <= kCellMaxDegree && t.tile_log2 >= 3 &&
-             t.tile_log2 <= 6));
+             t.tile_log2 <= 6) ||
+            (t.variant == MPDG_VMULT_LINE_MFMA && degree >= kMfmaMinDegree &&
+             precision == MPDG_FP32));
   }
 
   template <typename Number>
@@ -192,7 +209,8 @@ namespace
     prm.zlayers = std::max(1, std::min(tu.z_layers, d.n_cells[2]));
     prm.blocks_per_cu = tu.blocks_per_cu;
     prm.variant = (tu.variant == MPDG_VMULT_CELL && d.degree <= kCellMaxDegree) ? kVariantCell :
-                                                                                   kVariantLine;
+                  (tu.variant == MPDG_VMULT_LINE_MFMA && d.degree >= kMfmaMinDegree &&
+                   sizeof(Number) == 4) ? kVariantLineMfma : kVariantLine;
     prm.tile_log2 = tu.tile_log2;
     // per-direction tables: [sin2pi dir 0 | dir 1 | dir 2 | sinpi2 dir 0 | dir 1 | dir 2]
     size_t off = 0;
@@ -553,8 +571,9 @@ mpdg_status mpdg_heat_create(const mpdg_heat_desc *desc, mpdg_heat **out)
   if (!H)
     return fail(MPDG_ERR_ARGUMENT, "out of host memory");
   H->desc = d;
-  H->tune[MPDG_FP64] = default_tuning(d.degree, false);
-  H->tune[MPDG_FP32] = default_tuning(d.degree, true);
+  const int64_t cells = (int64_t)d.n_cells[0] * d.n_cells[1] * d.n_cells[2];
+  H->tune[MPDG_FP64] = default_tuning(d.degree, false, cells);
+  H->tune[MPDG_FP32] = default_tuning(d.degree, true, cells);
   H->stream = (hipStream_t)d.stream;
   H->N = N;
   H->N_free = Nf;
@@ -870,10 +889,10 @@ mpdg_status mpdg_heat_set_tuning(mpdg_heat *h, int precision, const mpdg_tuning 
     return fail(MPDG_ERR_ARGUMENT, "null argument");
   if (precision != MPDG_FP64 && precision != MPDG_FP32)
     return fail(MPDG_ERR_ARGUMENT, "precision must be MPDG_FP64 or MPDG_FP32");
-  if (!tuning_valid(*t, h->desc.degree))
+  if (!tuning_valid(*t, h->desc.degree, precision))
     return fail(MPDG_ERR_ARGUMENT,
-                "tuning needs z_layers >= 1, blocks_per_cu in 0..16 and, for the cell variant, "
-                "degree <= 2 and tile_log2 in 3..6");
+                "tuning needs z_layers >= 1, blocks_per_cu in 0..16; the cell variant degree <= 2 "
+                "and tile_log2 in 3..6; the MFMA variant degree >= 3 and MPDG_FP32");
   if (h->stream)
     MPDG_HIP(hipStreamSynchronize(h->stream));
   h->tune[precision] = *t;
@@ -911,6 +930,8 @@ mpdg_status mpdg_heat_autotune(mpdg_heat *h, int precision, int repeats, mpdg_tu
         {
           mpdg_tuning t{zl, bpc, MPDG_VMULT_LINE, 4};
           cand.push_back(t);
+          if (fp32 && h->desc.degree >= kMfmaMinDegree)
+            cand.push_back(mpdg_tuning{zl, bpc, MPDG_VMULT_LINE_MFMA, 4});
           if (h->desc.degree <= kCellMaxDegree)
             for (int tl : {3, 4, 5, 6})
               cand.push_back(mpdg_tuning{zl, bpc, MPDG_VMULT_CELL, tl});

@@ -182,6 +182,105 @@ namespace mpdg
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 
+    // The four 1D matrices of the line contractions.
+    enum LineMatrix
+    {
+      kLmS = 0,  // interpolate to Gauss points
+      kLmC = 1,  // collocation derivative on Gauss points
+      kLmCt = 2, // its transpose (flux back to values)
+      kLmSt = 3  // S^T (integrate)
+    };
+
+    template <int P, int M>
+    constexpr ct::Mat<P + 1> line_mat()
+    {
+      return M == kLmS ? ct::CTab<P>::S :
+             M == kLmC ? ct::CTab<P>::C :
+             M == kLmCt ? ct::CTab<P>::Ct : ct::CTab<P>::St;
+    }
+
+    template <int P, int M>
+    constexpr ct::EO<P + 1> line_eo()
+    {
+      return M == kLmS ? ct::VTab<P>::S :
+             M == kLmC ? ct::VTab<P>::C :
+             M == kLmCt ? ct::VTab<P>::Ct : ct::VTab<P>::St;
+    }
+
+    // MFMA form of a line contraction (FP32, 4 or more points): lane l owns
+    // line l. v_mfma_f32_4x4x1_16b treats lanes 4b..4b+3 as block b and forms
+    // D[i][j] += A[i] B[j] with A[i] from lane 4b+i and B[j] from lane 4b+j,
+    // leaving D[i][j] in register i of lane 4b+j. With A[i] = M[i][k] and
+    // B[j] = x_k of lane 4b+j, K = n steps give lane l the outputs y_0..y_3 of
+    // its own line; outputs 4.. (n > 4) are left to the vector ALU. col[m][k]
+    // holds M_m[lane % 4][k], the A operand of step k of matrix m.
+    template <int N, bool MF>
+    struct LineCols
+    {
+      float col[4][MF ? N : 1];
+    };
+
+    template <int P, int M>
+    __device__ __forceinline__ float line_col(int r, int k)
+    {
+      constexpr ct::Mat<P + 1> A = line_mat<P, M>();
+      return (float)(r == 0 ? A.a[0][k] : r == 1 ? A.a[1][k] : r == 2 ? A.a[2][k] : A.a[3][k]);
+    }
+
+    template <int P, bool MF>
+    __device__ __forceinline__ LineCols<P + 1, MF> line_cols(int lane)
+    {
+      LineCols<P + 1, MF> c{};
+      if constexpr (MF)
+        {
+          const int r = lane & 3;
+#pragma unroll
+          for (int k = 0; k < P + 1; ++k)
+            {
+              c.col[kLmS][k] = line_col<P, kLmS>(r, k);
+              c.col[kLmC][k] = line_col<P, kLmC>(r, k);
+              c.col[kLmCt][k] = line_col<P, kLmCt>(r, k);
+              c.col[kLmSt][k] = line_col<P, kLmSt>(r, k);
+            }
+        }
+      return c;
+    }
+
+    // y = M x along one line: MFMA form (MF) or even-odd vector-ALU form.
+    template <int P, int M, bool MF, typename Number>
+    __device__ __forceinline__ void line_apply(const LineCols<P + 1, MF> &lc,
+                                               const Number (&x)[P + 1], Number (&y)[P + 1])
+    {
+      constexpr int N = P + 1;
+      if constexpr (MF)
+        {
+          static_assert(sizeof(Number) == 4 && N >= 4, "MFMA line form: FP32, 4+ points");
+          typedef float v4f __attribute__((ext_vector_type(4)));
+          v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+          for (int k = 0; k < N; ++k)
+            acc = __builtin_amdgcn_mfma_f32_4x4x1f32(lc.col[M][k], x[k], acc, 0, 0, 0);
+          constexpr ct::Mat<N> A = line_mat<P, M>();
+#pragma unroll
+          for (int q = 4; q < N; ++q)
+            {
+              Number s = Number(A.a[q][0]) * x[0];
+#pragma unroll
+              for (int k = 1; k < N; ++k)
+                s += Number(A.a[q][k]) * x[k];
+              y[q] = s;
+            }
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            y[q] = acc[q];
+        }
+      else
+        {
+          constexpr ct::EO<N> T = line_eo<P, M>();
+          eo_apply<N, (M == kLmS || M == kLmSt) ? 1 : -1>(T, x, y);
+        }
+    }
+
     // Work item of the vmult: x segment sx (CPB cells), row cy, z chunk kz of
     // up to `layers` cell layers.
     struct Item
@@ -357,7 +456,7 @@ namespace mpdg
     // Body of the vmult: this workgroup processes items item_first,
     // item_first + item_stride, ... (< items) of one colour of one operator and
     // returns this thread's share of p . Ap (sum over its cells of u . y).
-    template <int P, typename Number, int COEF, bool VEL, int CW, int R, int WPB>
+    template <int P, typename Number, int COEF, bool VEL, int CW, int R, int WPB, bool MF>
     __device__ __forceinline__ double vmult_body(const Number *__restrict__ src,
                                                  Number *__restrict__ dst,
                                                  const Number *__restrict__ heav,
@@ -373,10 +472,6 @@ namespace mpdg
       constexpr int THREADS = 64 * WPB;
       constexpr bool H2 = COEF == kCoefHeaviside;
       static_assert(LT <= 64 * R && LT > 64 * (R - 1), "lines must fill the last lane slot");
-      constexpr ct::EO<N> TS = ct::VTab<P>::S;
-      constexpr ct::EO<N> TSt = ct::VTab<P>::St;
-      constexpr ct::EO<N> TC = ct::VTab<P>::C;
-      constexpr ct::EO<N> TCt = ct::VTab<P>::Ct;
       constexpr ct::Tab<N> T1 = ct::VTab<P>::t;
 
       __shared__ Number A[CPB * STR];
@@ -392,6 +487,7 @@ namespace mpdg
       const __amdgpu_buffer_rsrc_t rheav = make_rsrc(H2 ? heav : src, vbytes);
       const int w = threadIdx.x >> 6;
       const int lane = threadIdx.x & 63;
+      const LineCols<N, MF> LC = line_cols<P, MF>(lane);
       int cb[R], s0v[R], s1v[R], sv[R];
       bool lv[R];
 #pragma unroll
@@ -452,14 +548,14 @@ namespace mpdg
             {
               Number *Al = A + cb[r] * STR;
               Number t1[N];
-              eo_apply<N, 1>(TS, u[r], t1);
+              line_apply<P, kLmS, MF>(LC, u[r], t1);
 #pragma unroll
               for (int q = 0; q < N; ++q)
                 Al[(s1v[r] * N + s0v[r]) * N + q] = t1[q];
               if constexpr (H2)
                 {
                   Number *KBl = KB + cb[r] * STR;
-                  eo_apply<N, 1>(TS, hl[r], t1);
+                  line_apply<P, kLmS, MF>(LC, hl[r], t1);
 #pragma unroll
                   for (int q = 0; q < N; ++q)
                     KBl[(s1v[r] * N + s0v[r]) * N + q] = t1[q];
@@ -529,7 +625,7 @@ namespace mpdg
 #pragma unroll
               for (int b = 0; b < N; ++b)
                 v[b] = Al[(s1v[r] * N + b) * N + s0v[r]];
-              eo_apply<N, 1>(TS, v, o);
+              line_apply<P, kLmS, MF>(LC, v, o);
 #pragma unroll
               for (int b = 0; b < N; ++b)
                 Al[(s1v[r] * N + b) * N + s0v[r]] = o[b];
@@ -539,7 +635,7 @@ namespace mpdg
 #pragma unroll
                   for (int b = 0; b < N; ++b)
                     v[b] = KBl[(s1v[r] * N + b) * N + s0v[r]];
-                  eo_apply<N, 1>(TS, v, o);
+                  line_apply<P, kLmS, MF>(LC, v, o);
 #pragma unroll
                   for (int b = 0; b < N; ++b)
                     KBl[(s1v[r] * N + b) * N + s0v[r]] = o[b];
@@ -557,8 +653,8 @@ namespace mpdg
 #pragma unroll
               for (int c = 0; c < N; ++c)
                 v[c] = Al[(c * N + s1v[r]) * N + s0v[r]];
-              eo_apply<N, 1>(TS, v, U[r]);
-              eo_apply<N, -1>(TC, U[r], Gz[r]);
+              line_apply<P, kLmS, MF>(LC, v, U[r]);
+              line_apply<P, kLmC, MF>(LC, U[r], Gz[r]);
 #pragma unroll
               for (int q = 0; q < N; ++q)
                 Al[(q * N + s1v[r]) * N + s0v[r]] = U[r][q];
@@ -570,7 +666,7 @@ namespace mpdg
 #pragma unroll
                   for (int c = 0; c < N; ++c)
                     v[c] = KBl[(c * N + s1v[r]) * N + s0v[r]];
-                  eo_apply<N, 1>(TS, v, Hq);
+                  line_apply<P, kLmS, MF>(LC, v, Hq);
 #pragma unroll
                   for (int q = 0; q < N; ++q)
                     {
@@ -599,7 +695,7 @@ namespace mpdg
 #pragma unroll
               for (int q = 0; q < N; ++q)
                 v[q] = Al[(s1 * N + s0) * N + q];
-              eo_apply<N, -1>(TC, v, gx);
+              line_apply<P, kLmC, MF>(LC, v, gx);
               const Number wyz = (Number)T1.w[s0] * (Number)T1.w[s1];
 #pragma unroll
               for (int q = 0; q < N; ++q)
@@ -614,7 +710,7 @@ namespace mpdg
                       cv[q] = rc * w3 * prm.vf[0] * vx * gx[q];
                     }
                 }
-              eo_apply<N, -1>(TCt, f, o);
+              line_apply<P, kLmCt, MF>(LC, f, o);
 #pragma unroll
               for (int q = 0; q < N; ++q)
                 Bl[(s1 * N + s0) * N + q] = VEL ? o[q] + cv[q] : o[q];
@@ -635,7 +731,7 @@ namespace mpdg
 #pragma unroll
               for (int q = 0; q < N; ++q)
                 v[q] = Al[(s1 * N + q) * N + s0];
-              eo_apply<N, -1>(TC, v, gy);
+              line_apply<P, kLmC, MF>(LC, v, gy);
               const Number wxz = (Number)T1.w[s0] * (Number)T1.w[s1];
 #pragma unroll
               for (int q = 0; q < N; ++q)
@@ -650,7 +746,7 @@ namespace mpdg
                       cv[q] = rc * w3 * prm.vf[1] * vy * gy[q];
                     }
                 }
-              eo_apply<N, -1>(TCt, f, o);
+              line_apply<P, kLmCt, MF>(LC, f, o);
 #pragma unroll
               for (int q = 0; q < N; ++q)
                 {
@@ -678,7 +774,7 @@ namespace mpdg
                   const Number k = conductivity<Number, COEF, N>(prm, KBl, cx, it.cy, cz, s0, s1, q);
                   f[q] = k * w3 * prm.kf[2] * Gz[r][q];
                 }
-              eo_apply<N, -1>(TCt, f, o);
+              line_apply<P, kLmCt, MF>(LC, f, o);
 #pragma unroll
               for (int q = 0; q < N; ++q)
                 {
@@ -692,7 +788,7 @@ namespace mpdg
                     }
                   V[q] = val;
                 }
-              eo_apply<N, 1>(TSt, V, y1);
+              line_apply<P, kLmSt, MF>(LC, V, y1);
 #pragma unroll
               for (int c = 0; c < N; ++c)
                 Al[(c * N + s1) * N + s0] = y1[c];
@@ -708,7 +804,7 @@ namespace mpdg
 #pragma unroll
               for (int q = 0; q < N; ++q)
                 v[q] = Al[(s1v[r] * N + q) * N + s0v[r]];
-              eo_apply<N, 1>(TSt, v, o);
+              line_apply<P, kLmSt, MF>(LC, v, o);
 #pragma unroll
               for (int b = 0; b < N; ++b)
                 Al[(s1v[r] * N + b) * N + s0v[r]] = o[b];
@@ -725,7 +821,7 @@ namespace mpdg
 #pragma unroll
               for (int q = 0; q < N; ++q)
                 v[q] = Al[(s1v[r] * N + s0v[r]) * N + q];
-              eo_apply<N, 1>(TSt, v, Y[r]);
+              line_apply<P, kLmSt, MF>(LC, v, Y[r]);
               if (lv[r])
 #pragma unroll
                 for (int a = 0; a < N; ++a)
@@ -776,7 +872,7 @@ namespace mpdg
       return acc;
     }
 
-    template <int P, typename Number, int COEF, bool VEL, int CW, int R, int WPB, int MINW>
+    template <int P, typename Number, int COEF, bool VEL, int CW, int R, int WPB, int MINW, bool MF>
     __global__ void __launch_bounds__(64 * WPB, MINW)
       vmult_kernel(const Number *__restrict__ src, Number *__restrict__ dst,
                    const Number *__restrict__ heav, const CellParams<Number> prm,
@@ -787,7 +883,7 @@ namespace mpdg
       constexpr int THREADS = 64 * WPB;
       if (scal != nullptr && scal->done)
         return;
-      const double acc = vmult_body<P, Number, COEF, VEL, CW, R, WPB>(
+      const double acc = vmult_body<P, Number, COEF, VEL, CW, R, WPB, MF>(
         src, dst, heav, prm, colour, layers, items, blockIdx.x, gridDim.x);
       // p . Ap: sum over cells of u_cell . y_cell (constrained entries are 0)
       if (partials != nullptr)
@@ -799,7 +895,7 @@ namespace mpdg
     // operator s with block_off[s] <= b < block_off[s+1] and loops over that
     // operator's items of this colour. Each workgroup writes its p . Ap share
     // to partials[b] (per-operator sums are formed by a separate kernel).
-    template <int P, typename Number, int COEF, bool VEL, int CW, int R, int WPB, int MINW>
+    template <int P, typename Number, int COEF, bool VEL, int CW, int R, int WPB, int MINW, bool MF>
     __global__ void __launch_bounds__(64 * WPB, MINW)
       vmult_batch_kernel(const BatchOperator<Number> *__restrict__ ops, const int *__restrict__ block_off,
                          const int nops, const int colour, double *__restrict__ partials,
@@ -811,7 +907,7 @@ namespace mpdg
       double acc = 0.0;
       const bool skip = scal != nullptr && scal[op.solve].done;
       if (!skip)
-        acc = vmult_body<P, Number, COEF, VEL, CW, R, WPB>(
+        acc = vmult_body<P, Number, COEF, VEL, CW, R, WPB, MF>(
           op.src, op.dst, op.heav, op.prm, colour, op.prm.zlayers, op.items[colour],
           blockIdx.x - block_off[s], block_off[s + 1] - block_off[s]);
       if (partials != nullptr)
@@ -1072,26 +1168,29 @@ namespace mpdg
 
     // Wavefront shape of the vmult per degree: CW cells per wavefront, R line
     // slots per lane (CW*(P+1)^2 lines fill the last slot), 4 wavefronts per
-    // workgroup.
-    template <int P, bool FP32>
+    // workgroup. The MFMA form holds the matrix columns in 4(P+1) more VGPRs,
+    // so it takes fewer cells per wavefront at degree 4 and 6 and a 256-VGPR
+    // budget at degree 5.
+    template <int P, bool FP32, bool MF = false>
     struct VShape
     {
-      static constexpr int CW = P == 1 ? 16 : P == 2 ? 7 : P == 3 ? 4 : P == 4 ? (FP32 ? 5 : 2) : P == 5 ? (FP32 ? 3 : 1) : 1;
+      static constexpr int CW = MF ? (P == 4 ? 2 : P == 5 ? 3 : P == 6 ? 1 : 4)
+                                   : P == 1 ? 16 : P == 2 ? 7 : P == 3 ? 4 : P == 4 ? (FP32 ? 5 : 2) : P == 5 ? (FP32 ? 3 : 1) : 1;
       static constexpr int R = (CW * (P + 1) * (P + 1) + 63) / 64;
       static constexpr int WPB = 4;
       // register budget: 128 VGPRs (4 waves per SIMD), 256 for FP64 from degree 5
-      static constexpr int MINW = (P >= 5 && !FP32) ? 2 : kMinWavesPerSimd;
+      static constexpr int MINW = ((P >= 5 && !FP32) || (MF && P == 5)) ? 2 : kMinWavesPerSimd;
     };
 
-    template <typename Number, int P, int COEF, bool VEL>
+    template <typename Number, int P, int COEF, bool VEL, bool MF>
     hipError_t vmult_passes(const CellParams<Number> &prm, const VmultTables<Number> &tab,
                             const Number *src, Number *dst, const Number *heav, double *partials,
                             SolverScalars *scal, hipStream_t stream)
     {
-      using Shape = VShape<P, sizeof(Number) == 4>;
+      using Shape = VShape<P, sizeof(Number) == 4, MF>;
       constexpr int THREADS = 64 * Shape::WPB;
       constexpr int CPB = Shape::CW * Shape::WPB;
-      auto kernel = vmult_kernel<P, Number, COEF, VEL, Shape::CW, Shape::R, Shape::WPB, Shape::MINW>;
+      auto kernel = vmult_kernel<P, Number, COEF, VEL, Shape::CW, Shape::R, Shape::WPB, Shape::MINW, MF>;
       static int resident = 0; // workgroups per CU, from the occupancy calculator
       if (resident == 0)
         {
@@ -1132,21 +1231,39 @@ namespace mpdg
       return hipGetLastError();
     }
 
+    // MFMA line form exists for FP32 from degree 3 (4+ points per line)
+    template <typename Number, int P>
+    constexpr bool has_mfma_form()
+    {
+      return sizeof(Number) == 4 && P >= 3;
+    }
+
+    template <typename Number, int P, bool MF>
+    hipError_t vmult_deg_mf(int coef, bool vel, const CellParams<Number> &prm,
+                            const VmultTables<Number> &tab, const Number *src, Number *dst,
+                            const Number *heav, double *partials, SolverScalars *scal, hipStream_t s)
+    {
+      switch (coef * 2 + (vel ? 1 : 0))
+        {
+          case 0: return vmult_passes<Number, P, 0, false, MF>(prm, tab, src, dst, heav, partials, scal, s);
+          case 1: return vmult_passes<Number, P, 0, true, MF>(prm, tab, src, dst, heav, partials, scal, s);
+          case 2: return vmult_passes<Number, P, 1, false, MF>(prm, tab, src, dst, heav, partials, scal, s);
+          case 3: return vmult_passes<Number, P, 1, true, MF>(prm, tab, src, dst, heav, partials, scal, s);
+          case 4: return vmult_passes<Number, P, 2, false, MF>(prm, tab, src, dst, heav, partials, scal, s);
+          case 5: return vmult_passes<Number, P, 2, true, MF>(prm, tab, src, dst, heav, partials, scal, s);
+          default: return hipErrorInvalidValue;
+        }
+    }
+
     template <typename Number, int P>
     hipError_t vmult_deg(int coef, bool vel, const CellParams<Number> &prm,
                          const VmultTables<Number> &tab, const Number *src, Number *dst,
                          const Number *heav, double *partials, SolverScalars *scal, hipStream_t s)
     {
-      switch (coef * 2 + (vel ? 1 : 0))
-        {
-          case 0: return vmult_passes<Number, P, 0, false>(prm, tab, src, dst, heav, partials, scal, s);
-          case 1: return vmult_passes<Number, P, 0, true>(prm, tab, src, dst, heav, partials, scal, s);
-          case 2: return vmult_passes<Number, P, 1, false>(prm, tab, src, dst, heav, partials, scal, s);
-          case 3: return vmult_passes<Number, P, 1, true>(prm, tab, src, dst, heav, partials, scal, s);
-          case 4: return vmult_passes<Number, P, 2, false>(prm, tab, src, dst, heav, partials, scal, s);
-          case 5: return vmult_passes<Number, P, 2, true>(prm, tab, src, dst, heav, partials, scal, s);
-          default: return hipErrorInvalidValue;
-        }
+      if constexpr (has_mfma_form<Number, P>())
+        if (prm.variant == kVariantLineMfma)
+          return vmult_deg_mf<Number, P, true>(coef, vel, prm, tab, src, dst, heav, partials, scal, s);
+      return vmult_deg_mf<Number, P, false>(coef, vel, prm, tab, src, dst, heav, partials, scal, s);
     }
 
     template <typename Number, int P>
@@ -1291,14 +1408,14 @@ namespace mpdg
 
   namespace
   {
-    template <typename Number, int P, int COEF, bool VEL>
+    template <typename Number, int P, int COEF, bool VEL, bool MF>
     hipError_t vmult_batch_passes(const BatchOperator<Number> *ops, const int *block_off, int nops,
                                   const BatchVmultPlan &plan, double *partials,
                                   const SolverScalars *scal, hipStream_t stream)
     {
-      using Shape = VShape<P, sizeof(Number) == 4>;
+      using Shape = VShape<P, sizeof(Number) == 4, MF>;
       constexpr int THREADS = 64 * Shape::WPB;
-      auto kernel = vmult_batch_kernel<P, Number, COEF, VEL, Shape::CW, Shape::R, Shape::WPB, Shape::MINW>;
+      auto kernel = vmult_batch_kernel<P, Number, COEF, VEL, Shape::CW, Shape::R, Shape::WPB, Shape::MINW, MF>;
       for (int c = 0; c < 8; ++c)
         {
           if (plan.grid[c] == 0)
@@ -1310,28 +1427,39 @@ namespace mpdg
       return hipGetLastError();
     }
 
-    template <typename Number, int P>
-    hipError_t vmult_batch_deg(int coef, bool vel, const BatchOperator<Number> *ops,
-                               const int *block_off, int nops, const BatchVmultPlan &plan,
-                               double *partials, const SolverScalars *scal, hipStream_t s)
+    template <typename Number, int P, bool MF>
+    hipError_t vmult_batch_deg_mf(int coef, bool vel, const BatchOperator<Number> *ops,
+                                  const int *block_off, int nops, const BatchVmultPlan &plan,
+                                  double *partials, const SolverScalars *scal, hipStream_t s)
     {
       switch (coef * 2 + (vel ? 1 : 0))
         {
-          case 0: return vmult_batch_passes<Number, P, 0, false>(ops, block_off, nops, plan, partials, scal, s);
-          case 1: return vmult_batch_passes<Number, P, 0, true>(ops, block_off, nops, plan, partials, scal, s);
-          case 2: return vmult_batch_passes<Number, P, 1, false>(ops, block_off, nops, plan, partials, scal, s);
-          case 3: return vmult_batch_passes<Number, P, 1, true>(ops, block_off, nops, plan, partials, scal, s);
-          case 4: return vmult_batch_passes<Number, P, 2, false>(ops, block_off, nops, plan, partials, scal, s);
-          case 5: return vmult_batch_passes<Number, P, 2, true>(ops, block_off, nops, plan, partials, scal, s);
+          case 0: return vmult_batch_passes<Number, P, 0, false, MF>(ops, block_off, nops, plan, partials, scal, s);
+          case 1: return vmult_batch_passes<Number, P, 0, true, MF>(ops, block_off, nops, plan, partials, scal, s);
+          case 2: return vmult_batch_passes<Number, P, 1, false, MF>(ops, block_off, nops, plan, partials, scal, s);
+          case 3: return vmult_batch_passes<Number, P, 1, true, MF>(ops, block_off, nops, plan, partials, scal, s);
+          case 4: return vmult_batch_passes<Number, P, 2, false, MF>(ops, block_off, nops, plan, partials, scal, s);
+          case 5: return vmult_batch_passes<Number, P, 2, true, MF>(ops, block_off, nops, plan, partials, scal, s);
           default: return hipErrorInvalidValue;
         }
     }
 
-    template <typename Number, int P, int COEF, bool VEL>
+    template <typename Number, int P>
+    hipError_t vmult_batch_deg(int coef, bool vel, int variant, const BatchOperator<Number> *ops,
+                               const int *block_off, int nops, const BatchVmultPlan &plan,
+                               double *partials, const SolverScalars *scal, hipStream_t s)
+    {
+      if constexpr (has_mfma_form<Number, P>())
+        if (variant == kVariantLineMfma)
+          return vmult_batch_deg_mf<Number, P, true>(coef, vel, ops, block_off, nops, plan, partials, scal, s);
+      return vmult_batch_deg_mf<Number, P, false>(coef, vel, ops, block_off, nops, plan, partials, scal, s);
+    }
+
+    template <typename Number, int P, int COEF, bool VEL, bool MF>
     int resident_of()
     {
-      using Shape = VShape<P, sizeof(Number) == 4>;
-      auto kernel = vmult_batch_kernel<P, Number, COEF, VEL, Shape::CW, Shape::R, Shape::WPB, Shape::MINW>;
+      using Shape = VShape<P, sizeof(Number) == 4, MF>;
+      auto kernel = vmult_batch_kernel<P, Number, COEF, VEL, Shape::CW, Shape::R, Shape::WPB, Shape::MINW, MF>;
       int nb = 0;
       if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, 64 * Shape::WPB, 0) != hipSuccess || nb < 1)
         nb = 1;
@@ -1388,11 +1516,13 @@ namespace mpdg
     return plan;
   }
 
-  int vmult_cells_per_block(int degree, bool fp32)
+  int vmult_cells_per_block(int degree, bool fp32, bool mf)
   {
 #define MPDG_CPB(P)                                                                               \
   case P:                                                                                         \
-    return fp32 ? VShape<P, true>::CW * VShape<P, true>::WPB : VShape<P, false>::CW * VShape<P, false>::WPB;
+    return mf ? VShape<P, true, true>::CW * VShape<P, true, true>::WPB                            \
+              : fp32 ? VShape<P, true>::CW * VShape<P, true>::WPB                                 \
+                     : VShape<P, false>::CW * VShape<P, false>::WPB;
     switch (degree)
       {
         MPDG_CPB(1)
@@ -1411,7 +1541,8 @@ namespace mpdg
   {
     if (prm.variant == kVariantCell && degree <= kCellMaxDegree)
       return cell_grid(prm.nx, prm.ny, prm.nz, prm.tile_log2, prm.zlayers, 1 << 30);
-    return vmult_grid(prm.nx, prm.ny, prm.nz, vmult_cells_per_block(degree, sizeof(Number) == 4),
+    const bool mf = prm.variant == kVariantLineMfma && sizeof(Number) == 4 && degree >= kMfmaMinDegree;
+    return vmult_grid(prm.nx, prm.ny, prm.nz, vmult_cells_per_block(degree, sizeof(Number) == 4, mf),
                       prm.zlayers, 1 << 30);
   }
   template VmultGrid vmult_items<double>(int, const CellParams<double> &);
@@ -1425,7 +1556,9 @@ namespace mpdg
       return cell_resident_blocks(degree, fp32);
 #define MPDG_RES(P)                                                                               \
   case P:                                                                                         \
-    return fp32 ? resident_of<float, P, 0, false>() : resident_of<double, P, 0, false>();
+    return mf && P >= kMfmaMinDegree ? resident_of<float, P, 0, false, P >= kMfmaMinDegree>()     \
+           : fp32 ? resident_of<float, P, 0, false, false>() : resident_of<double, P, 0, false, false>();
+    const bool mf = fp32 && variant == kVariantLineMfma;
     switch (degree)
       {
         MPDG_RES(1)
@@ -1450,12 +1583,12 @@ namespace mpdg
                                              partials, scal, stream);
     switch (degree)
       {
-        case 1: return vmult_batch_deg<Number, 1>(coef, vel, ops, block_off, nops, plan, partials, scal, stream);
-        case 2: return vmult_batch_deg<Number, 2>(coef, vel, ops, block_off, nops, plan, partials, scal, stream);
-        case 3: return vmult_batch_deg<Number, 3>(coef, vel, ops, block_off, nops, plan, partials, scal, stream);
-        case 4: return vmult_batch_deg<Number, 4>(coef, vel, ops, block_off, nops, plan, partials, scal, stream);
-        case 5: return vmult_batch_deg<Number, 5>(coef, vel, ops, block_off, nops, plan, partials, scal, stream);
-        case 6: return vmult_batch_deg<Number, 6>(coef, vel, ops, block_off, nops, plan, partials, scal, stream);
+        case 1: return vmult_batch_deg<Number, 1>(coef, vel, variant, ops, block_off, nops, plan, partials, scal, stream);
+        case 2: return vmult_batch_deg<Number, 2>(coef, vel, variant, ops, block_off, nops, plan, partials, scal, stream);
+        case 3: return vmult_batch_deg<Number, 3>(coef, vel, variant, ops, block_off, nops, plan, partials, scal, stream);
+        case 4: return vmult_batch_deg<Number, 4>(coef, vel, variant, ops, block_off, nops, plan, partials, scal, stream);
+        case 5: return vmult_batch_deg<Number, 5>(coef, vel, variant, ops, block_off, nops, plan, partials, scal, stream);
+        case 6: return vmult_batch_deg<Number, 6>(coef, vel, variant, ops, block_off, nops, plan, partials, scal, stream);
         default: return hipErrorInvalidValue;
       }
   }

@@ -75,8 +75,10 @@ namespace mpdg
   enum VmultVariant
   {
     kVariantLine = 0,
-    kVariantCell = 1
+    kVariantCell = 1,
+    kVariantLineMfma = 2 // line form with the 1D contractions on FP32 MFMA (degree >= 3)
   };
+  constexpr int kMfmaMinDegree = 3;
   constexpr int kCellMaxDegree = 2;
 
   // Launch shape of one colour pass.
@@ -167,7 +169,7 @@ namespace mpdg
   BatchVmultPlan batch_vmult_plan(const int *items /* nops x 8 */, int nops, int max_blocks,
                                   int *block_off);
   int vmult_resident_blocks(int degree, bool fp32, int coef, bool vel, int variant);
-  int vmult_cells_per_block(int degree, bool fp32);
+  int vmult_cells_per_block(int degree, bool fp32, bool mfma);
   // Work items per colour of one operator's vmult in its kernel form.
   template <typename Number>
   VmultGrid vmult_items(int degree, const CellParams<Number> &prm);

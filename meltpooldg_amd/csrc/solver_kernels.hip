@@ -210,6 +210,124 @@ namespace mpdg
         p[i] = r[i] * inv[i] + beta * p[i];
     }
 
+    // 16-byte vector of Number (2 doubles or 4 floats) for the wide forms of
+    // the update kernels; pointers are 16-byte aligned (checked at launch).
+    template <typename Number>
+    struct Wide;
+    template <>
+    struct Wide<double>
+    {
+      typedef double type __attribute__((ext_vector_type(2)));
+      static constexpr int W = 2;
+    };
+    template <>
+    struct Wide<float>
+    {
+      typedef float type __attribute__((ext_vector_type(4)));
+      static constexpr int W = 4;
+    };
+
+    // cg_update_kernel over 16-byte vectors: thread t of the fixed grid owns
+    // vectors t, t + grid, ... and (for the tail n % W) entry n - n % W + t.
+    template <typename Number>
+    __global__ void __launch_bounds__(T)
+      cg_update_wide_kernel(Number *x, Number *r, const Number *p, const Number *q,
+                            const Number *inv, int64_t n, double *partials, SolverScalars *scal,
+                            double *history, int history_len)
+    {
+      if (scal->done)
+        return;
+      using V = typename Wide<Number>::type;
+      constexpr int W = Wide<Number>::W;
+      const Number a = (Number)scal->alpha;
+      const int64_t nv = n / W;
+      V *X = reinterpret_cast<V *>(x);
+      V *Rv = reinterpret_cast<V *>(r);
+      const V *Pv = reinterpret_cast<const V *>(p);
+      const V *Qv = reinterpret_cast<const V *>(q);
+      const V *Iv = reinterpret_cast<const V *>(inv);
+      double v[2] = {0.0, 0.0};
+      const int64_t t0 = blockIdx.x * (int64_t)T + threadIdx.x;
+      const int64_t stride = (int64_t)gridDim.x * T;
+      for (int64_t j = t0; j < nv; j += stride)
+        {
+          const V pv = Pv[j], qv = Qv[j], iv = Iv[j];
+          V xv = X[j], rv = Rv[j];
+          xv = xv + a * pv;
+          rv = rv - a * qv;
+          X[j] = xv;
+          Rv[j] = rv;
+#pragma unroll
+          for (int k = 0; k < W; ++k)
+            {
+              v[0] += (double)rv[k] * (double)rv[k];
+              v[1] += (double)rv[k] * (double)(rv[k] * iv[k]);
+            }
+        }
+      const int64_t i = nv * W + t0;
+      if (i < n)
+        {
+          x[i] = x[i] + a * p[i];
+          const Number ri = r[i] - a * q[i];
+          r[i] = ri;
+          v[0] += (double)ri * (double)ri;
+          v[1] += (double)ri * (double)(ri * inv[i]);
+        }
+      double tot[2];
+      if (finish_reduction<2>(v, partials, &scal->counter, tot) && threadIdx.x == 0)
+        {
+          const int it = scal->iteration + 1;
+          const double res = sqrt(fabs(tot[0]));
+          scal->iteration = it;
+          scal->rr = tot[0];
+          scal->res = res;
+          if (history && it < history_len)
+            history[it] = res;
+          scal->beta = tot[1] / scal->rz;
+          scal->rz_old = scal->rz;
+          scal->rz = tot[1];
+          if (res <= scal->tol)
+            {
+              scal->done = 1;
+              scal->status = 0;
+            }
+          else if (it >= scal->max_it || isnan(res))
+            {
+              scal->done = 1;
+              scal->status = 1;
+            }
+        }
+    }
+
+    template <typename Number>
+    __global__ void __launch_bounds__(T)
+      cg_direction_wide_kernel(Number *p, const Number *r, const Number *inv, int64_t n,
+                               const SolverScalars *scal)
+    {
+      if (scal->done)
+        return;
+      using V = typename Wide<Number>::type;
+      constexpr int W = Wide<Number>::W;
+      const Number beta = (Number)scal->beta;
+      const int64_t nv = n / W;
+      V *Pv = reinterpret_cast<V *>(p);
+      const V *Rv = reinterpret_cast<const V *>(r);
+      const V *Iv = reinterpret_cast<const V *>(inv);
+      const int64_t t0 = blockIdx.x * (int64_t)T + threadIdx.x;
+      for (int64_t j = t0; j < nv; j += (int64_t)gridDim.x * T)
+        Pv[j] = Rv[j] * Iv[j] + beta * Pv[j];
+      const int64_t i = nv * W + t0;
+      if (i < n)
+        p[i] = r[i] * inv[i] + beta * p[i];
+    }
+
+    bool aligned16(const void *a, const void *b = nullptr, const void *c = nullptr,
+                   const void *d = nullptr, const void *e = nullptr)
+    {
+      auto ok = [](const void *v) { return v == nullptr || (reinterpret_cast<uintptr_t>(v) & 15u) == 0; };
+      return ok(a) && ok(b) && ok(c) && ok(d) && ok(e);
+    }
+
     // r = b - ax (FP64), rr -> scal->rr / res.
     __global__ void __launch_bounds__(T)
       residual_kernel(double *r, const double *b, const double *ax, int64_t n, double *partials,
@@ -425,8 +543,12 @@ namespace mpdg
                               const Number *inv_diag, int64_t n, double *partials,
                               SolverScalars *scal, double *history, int history_len, hipStream_t s)
   {
-    cg_update_kernel<Number><<<kVecBlocks, T, 0, s>>>(x, r, p, q, inv_diag, n, partials, scal,
-                                                      history, history_len);
+    if (aligned16(x, r, p, q, inv_diag))
+      cg_update_wide_kernel<Number><<<kVecBlocks, T, 0, s>>>(x, r, p, q, inv_diag, n, partials,
+                                                             scal, history, history_len);
+    else
+      cg_update_kernel<Number><<<kVecBlocks, T, 0, s>>>(x, r, p, q, inv_diag, n, partials, scal,
+                                                        history, history_len);
     return hipGetLastError();
   }
 
@@ -434,7 +556,13 @@ namespace mpdg
   hipError_t launch_cg_direction(Number *p, const Number *r, const Number *inv_diag, int64_t n,
                                  const SolverScalars *scal, hipStream_t s)
   {
-    cg_direction_kernel<Number><<<grid_for(n), T, 0, s>>>(p, r, inv_diag, n, scal);
+    if (aligned16(p, r, inv_diag))
+      {
+        const int64_t nv = (n + Wide<Number>::W - 1) / Wide<Number>::W;
+        cg_direction_wide_kernel<Number><<<grid_for(nv), T, 0, s>>>(p, r, inv_diag, n, scal);
+      }
+    else
+      cg_direction_kernel<Number><<<grid_for(n), T, 0, s>>>(p, r, inv_diag, n, scal);
     return hipGetLastError();
   }
 

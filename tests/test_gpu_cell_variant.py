@@ -145,9 +145,9 @@ def test_autotune_installs_a_valid_tuning(p):
         best = op.autotune(precision=prec, repeats=2)
         print("autotune", p, prec, best, flush=True)
         assert best == op.get_tuning(prec)
-        assert best["variant"] in ("line", "cell")
-        if p > 2:
-            assert best["variant"] == "line"
+        allowed = {"line"} | ({"cell"} if p <= 2 else set()) | \
+            ({"mfma"} if p >= 3 and prec == "fp32" else set())
+        assert best["variant"] in allowed
     assert rel_err(_vmult(op, u), pb.vmult(u)) < 1e-12
 
 
