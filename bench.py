@@ -113,6 +113,42 @@ def copy_calibration(args, torch, dev):
                       "GBs": 16 * N / (ms * 1e-3) / 1e9}))
 
 
+def time_solves(op, b, torch, LinearSolver, LinearSolverData):
+    """CG + Jacobi to 1e-8 in FP64 and FP32-mixed, each after one untimed warm-up
+    solve (workspace allocation)."""
+    solve = {}
+    for name, mixed in (("fp64", False), ("fp32_mixed", True)):
+        data = LinearSolverData(solver_type="CG", preconditioner_type="Diagonal",
+                                rel_tolerance=1e-8, mixed_precision=mixed)
+        x = torch.zeros_like(b)
+        LinearSolver.solve(op, x, b, data)
+        x.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        LinearSolver.solve(op, x, b, data)
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        rep = LinearSolver.last_report
+        solve[name] = {"seconds": round(t, 4), "iterations": rep["iterations"],
+                       "outer_iterations": rep["outer_iterations"],
+                       "rel_residual": rep["final_residual"] / rep["initial_residual"]}
+        if mixed:
+            solve[name]["speedup_vs_fp64"] = round(solve["fp64"]["seconds"] / t, 3)
+    return solve
+
+
+def solve_legs(args, torch, dev):
+    from meltpooldg_amd import HeatDiffuseOperator, LinearSolver, LinearSolverData
+    from oracle.rng import random_vector
+    op = HeatDiffuseOperator(N_CELLS, DEGREE, conductivity_amplitude=K_AMP)
+    op.reset_time_increment(DT)
+    b = torch.tensor(random_vector(N_CELLS, DEGREE, 1), dtype=torch.float64, device=dev)
+    print(json.dumps({"mode": "solve", "tuning": {"fp64": op.get_tuning("fp64"),
+                                                  "fp32": op.get_tuning("fp32")},
+                      "solve_time_to_1e-8": time_solves(op, b, torch, LinearSolver,
+                                                        LinearSolverData)}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,13 +156,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-solve", action="store_true", help="skip the CG time-to-solution legs")
-    ap.add_argument("--no-tune", action="store_true",
-                    help="keep the default launch tuning instead of autotuning the operator")
+    ap.add_argument("--tune", action="store_true",
+                    help="autotune the operator (FP64 and FP32) before timing instead of using the "
+                         "per-degree default tuning (itself the autotuner's pick on this workload)")
     ap.add_argument("--tuning", default=None,
-                    help="fixed FP64 tuning z_layers,blocks_per_cu[,variant,tile_log2] instead of "
-                         "autotuning (profiling runs)")
-    ap.add_argument("--mode", choices=["vmult", "copy"], default="vmult",
-                    help="copy: time a plain device copy of one DoF vector (HBM counter calibration)")
+                    help="fixed FP64 tuning z_layers,blocks_per_cu[,variant,tile_log2]")
+    ap.add_argument("--mode", choices=["vmult", "copy", "solve"], default="vmult",
+                    help="copy: time a plain device copy of one DoF vector (HBM counter calibration); "
+                         "solve: only the FP64 and FP32-mixed CG solves (solver profiling)")
     ap.add_argument("--config", type=int, default=1,
                     help="1: the headline line (default); 0, 2, 3, 4: the other workloads")
     args = ap.parse_args()
@@ -143,6 +180,8 @@ def main():
     torch.cuda.set_device(dev)
     if args.mode == "copy":
         return copy_calibration(args, torch, dev)
+    if args.mode == "solve":
+        return solve_legs(args, torch, dev)
     if args.config != 1:
         import bench_configs
         return bench_configs.run(args.config, args)
@@ -154,10 +193,10 @@ def main():
                       variant=f[2] if len(f) > 2 else None,
                       tile_log2=int(f[3]) if len(f) > 3 else None, precision="fp64")
         tuning = op.get_tuning("fp64")
-    elif args.no_tune:
-        tuning = op.get_tuning("fp64")
-    else:
+    elif args.tune:
         tuning = op.autotune("fp64", repeats=3)
+    else:
+        tuning = op.get_tuning("fp64")
     cnt = algorithmic_counts(N_CELLS, DEGREE)
     N = op.n_dofs()
     assert N == cnt["N"]
@@ -203,26 +242,9 @@ def main():
     solve = None
     if not args.no_solve:
         b = torch.tensor(random_vector(N_CELLS, DEGREE, 1), dtype=torch.float64, device=dev)
-        solve = {}
-        if not (args.no_tune or args.tuning):
-            solve["fp32_tuning"] = op.autotune("fp32", repeats=3)
-        for name, mixed in (("fp64", False), ("fp32_mixed", True)):
-            data = LinearSolverData(solver_type="CG", preconditioner_type="Diagonal",
-                                    rel_tolerance=1e-8, mixed_precision=mixed)
-            x = torch.zeros_like(b)
-            LinearSolver.solve(op, x, b, data)  # warm-up: workspace allocation
-            x.zero_()
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            LinearSolver.solve(op, x, b, data)
-            torch.cuda.synchronize()
-            t = time.perf_counter() - t0
-            rep = LinearSolver.last_report
-            solve[name] = {"seconds": round(t, 4), "iterations": rep["iterations"],
-                           "outer_iterations": rep["outer_iterations"],
-                           "rel_residual": rep["final_residual"] / rep["initial_residual"]}
-            if mixed:
-                solve[name]["speedup_vs_fp64"] = round(solve["fp64"]["seconds"] / t, 3)
+        fp32_tuning = op.autotune("fp32", repeats=3) if args.tune else op.get_tuning("fp32")
+        solve = time_solves(op, b, torch, LinearSolver, LinearSolverData)
+        solve["fp32_tuning"] = fp32_tuning
 
     cpu = None if args.no_cpu else cpu_baseline()
 
