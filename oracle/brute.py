@@ -52,6 +52,15 @@ def assemble(n, p, dt, mat, heaviside=None, k_amplitude=0.0, velocity=False, w0=
     """Return (A_full, mask): dense lattice matrix without constraints and the
     constrained-DoF mask. The constrained operator is A_full restricted to free
     rows/columns, zero elsewhere."""
+    A, _, _, mask = assemble_parts(n, p, dt, mat, heaviside, k_amplitude, velocity, w0, lo, hi,
+                                   dirichlet_faces)
+    return A, mask
+
+
+def assemble_parts(n, p, dt, mat, heaviside=None, k_amplitude=0.0, velocity=False, w0=0.2,
+                   lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), dirichlet_faces=0x3F):
+    """(A_full, M_rc, M_1, mask): the full tangent, its mass part rho_cp/dt M and
+    the unit mass matrix M_1 (integral of phi_i phi_j), all without constraints."""
     n = (n, n, n) if np.isscalar(n) else tuple(n)
     n1 = lattice_shape(n, p)
     N = n1[0] * n1[1] * n1[2]
@@ -65,6 +74,8 @@ def assemble(n, p, dt, mat, heaviside=None, k_amplitude=0.0, velocity=False, w0=
     Gz = np.einsum("zc,yb,xa->zyxcba", D, S, S).reshape(nq**3, nq**3) / h[2]
     W = np.einsum("z,y,x->zyx", wq, wq, wq).reshape(-1) * h[0] * h[1] * h[2]
     A = np.zeros((N, N))
+    Mrc = np.zeros((N, N))
+    M1 = np.zeros((N, N))
     loc = np.arange(nq)
     for cz in range(n[2]):
         for cy in range(n[1]):
@@ -83,7 +94,10 @@ def assemble(n, p, dt, mat, heaviside=None, k_amplitude=0.0, velocity=False, w0=
                 k = k * np.ones(nq**3)
                 if k_amplitude:
                     k = k + k_amplitude * np.sin(2 * np.pi * Xq) * np.sin(2 * np.pi * Yq) * np.sin(2 * np.pi * Zq)
-                Aloc = Phi.T @ ((W * rc / dt)[:, None] * Phi)
+                Mloc = Phi.T @ ((W * rc / dt)[:, None] * Phi)
+                Mrc[np.ix_(dofs, dofs)] += Mloc
+                M1[np.ix_(dofs, dofs)] += Phi.T @ (W[:, None] * Phi)
+                Aloc = Mloc.copy()
                 for G in (Gx, Gy, Gz):
                     Aloc += G.T @ ((W * k)[:, None] * G)
                 if velocity:
@@ -94,7 +108,7 @@ def assemble(n, p, dt, mat, heaviside=None, k_amplitude=0.0, velocity=False, w0=
                     Aloc += Phi.T @ ((W * rc)[:, None] * adv)
                 A[np.ix_(dofs, dofs)] += Aloc
     mask = constrained_mask(n, p, dirichlet_faces)
-    return A, mask
+    return A, Mrc, M1, mask
 
 
 def constrained(A, mask):
@@ -106,6 +120,16 @@ def constrained(A, mask):
 
 def vmult(A, mask, src):
     return constrained(A, mask) @ src
+
+
+def create_rhs(A, Mrc, M1, mask, T, T_old, f=None, dst=None):
+    """dst + (-(A_full T - M_rc T_old - M_1 f)) on the free rows; T, T_old, f are
+    used with their constrained entries (read plain), constrained rows of dst
+    are left as they are."""
+    r = -(A @ T - Mrc @ T_old - (M1 @ f if f is not None else 0.0))
+    out = np.zeros(A.shape[0]) if dst is None else np.array(dst, dtype=float)
+    out[~mask] += r[~mask]
+    return out
 
 
 def diagonal(A, mask):

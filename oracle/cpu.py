@@ -55,6 +55,7 @@ def lib():
         L.oracle_random_vector.argtypes = [P, ctypes.c_uint64, dp]
         L.oracle_material.argtypes = [ctypes.c_int] * 4 + [ctypes.c_double] * 7 + [dp] * 4
         L.oracle_vmult.argtypes = [P, dp, dp]
+        L.oracle_create_rhs.argtypes = [P, dp, dp, dp, dp]
         L.oracle_diagonal.argtypes = [P, dp]
         L.oracle_inverse_diagonal.argtypes = [P, dp]
         L.oracle_solve.argtypes = [P, ctypes.c_int, ctypes.c_int, dp, dp, dp, ctypes.c_double,
@@ -148,6 +149,16 @@ class Problem:
         dst = np.zeros(self.N)
         assert lib().oracle_vmult(ctypes.byref(self.s), _dp(src), _dp(dst)) == 0
         return dst
+
+    def create_rhs(self, temperature, temperature_old, heat_source=None, dst=None):
+        """dst + the residual right-hand side (create_rhs adds into dst)."""
+        tn = np.ascontiguousarray(temperature, dtype=np.float64)
+        to = np.ascontiguousarray(temperature_old, dtype=np.float64)
+        f = None if heat_source is None else np.ascontiguousarray(heat_source, dtype=np.float64)
+        out = np.zeros(self.N) if dst is None else np.array(dst, dtype=np.float64)
+        assert lib().oracle_create_rhs(ctypes.byref(self.s), _dp(tn), _dp(to),
+                                       _dp(f) if f is not None else None, _dp(out)) == 0
+        return out
 
     def diagonal(self):
         d = np.zeros(self.N)

@@ -415,6 +415,111 @@ namespace
     });
   }
 
+  // rhs_cell_loop restated (source/heat/heat_diffuse_operator.cpp:393-529) for
+  // temperature-independent materials without evaporation: T_new, T_old and
+  // the heat source read plain (constrained values included), at q
+  //   val      = dt^-1 rho_cp (T_new - T_old) - f  (+ rho_cp grad T_new . u)
+  //   val_grad = k grad T_new
+  // submitted with the factor -1 and integrated (values | gradients).
+  void local_rhs(const oracle_problem *pb, const Tables &t, const Mesh &m, const CellGeom &g,
+                 const double *tn, const double *to, const double *fl, const double *Hloc,
+                 double *y)
+  {
+    const int n = t.n;
+    const int n3 = n * n * n;
+    double tS[MAXN * MAXN * MAXN], tD[MAXN * MAXN * MAXN];
+    double tSS[MAXN * MAXN * MAXN], tSD[MAXN * MAXN * MAXN], tDS[MAXN * MAXN * MAXN];
+    double U[MAXN * MAXN * MAXN], Gx[MAXN * MAXN * MAXN], Gy[MAXN * MAXN * MAXN],
+      Gz[MAXN * MAXN * MAXN], Hq[MAXN * MAXN * MAXN], Uo[MAXN * MAXN * MAXN],
+      Fq[MAXN * MAXN * MAXN];
+
+    // T_new_eval.evaluate(values | gradients)
+    apply_dir<false>(t.S, n, 0, tn, tS);
+    apply_dir<false>(t.D, n, 0, tn, tD);
+    apply_dir<false>(t.S, n, 1, tS, tSS);
+    apply_dir<false>(t.D, n, 1, tS, tSD);
+    apply_dir<false>(t.S, n, 1, tD, tDS);
+    apply_dir<false>(t.S, n, 2, tSS, U);
+    apply_dir<false>(t.D, n, 2, tSS, Gz);
+    apply_dir<false>(t.S, n, 2, tSD, Gy);
+    apply_dir<false>(t.S, n, 2, tDS, Gx);
+    // T_old_eval / heat_source_eval.evaluate(values)
+    apply_dir<false>(t.S, n, 0, to, tS);
+    apply_dir<false>(t.S, n, 1, tS, tSS);
+    apply_dir<false>(t.S, n, 2, tSS, Uo);
+    apply_dir<false>(t.S, n, 0, fl, tS);
+    apply_dir<false>(t.S, n, 1, tS, tSS);
+    apply_dir<false>(t.S, n, 2, tSS, Fq);
+    if (pb->material == 1)
+      {
+        apply_dir<false>(t.S, n, 0, Hloc, tS);
+        apply_dir<false>(t.S, n, 1, tS, tSS);
+        apply_dir<false>(t.S, n, 2, tSS, Hq);
+      }
+
+    const double hx = m.h[0], hy = m.h[1], hz = m.h[2];
+    const double jac_det = hx * hy * hz;
+    const double dt_inv = 1. / pb->dt;
+    const double two_pi = 2.0 * (double)PI_L;
+    const double pi = (double)PI_L;
+
+    double V[MAXN * MAXN * MAXN], Fx[MAXN * MAXN * MAXN], Fy[MAXN * MAXN * MAXN],
+      Fz[MAXN * MAXN * MAXN];
+    for (int qz = 0; qz < n; ++qz)
+      for (int qy = 0; qy < n; ++qy)
+        for (int qx = 0; qx < n; ++qx)
+          {
+            const int q = (qz * n + qy) * n + qx;
+            const double xq = m.lo[0] + (g.c[0] + t.X[qx]) * hx;
+            const double yq = m.lo[1] + (g.c[1] + t.X[qy]) * hy;
+            const double zq = m.lo[2] + (g.c[2] + t.X[qz]) * hz;
+            const double JxW = t.W[qx] * t.W[qy] * t.W[qz] * jac_det;
+
+            double rho_cp, k;
+            material_at(pb, pb->material == 1 ? Hq[q] : 0.0, rho_cp, k);
+            if (pb->conductivity_field == 1)
+              k += pb->k_amplitude * std::sin(two_pi * xq) * std::sin(two_pi * yq) *
+                   std::sin(two_pi * zq);
+
+            const double gx = Gx[q] / hx, gy = Gy[q] / hy, gz = Gz[q] / hz;
+            double val = dt_inv * rho_cp * (U[q] - Uo[q]) - Fq[q];
+            if (pb->velocity == 1)
+              {
+                const double sx = std::sin(pi * xq), sy = std::sin(pi * yq);
+                const double vx = -sx * sx * std::sin(two_pi * yq);
+                const double vy = std::sin(two_pi * xq) * sy * sy;
+                const double vz = pb->w0;
+                val += rho_cp * (gx * vx + gy * vy + gz * vz);
+              }
+            // submit_value(-1 val), submit_gradient(-1 val_grad)
+            V[q] = -val * JxW;
+            Fx[q] = -k * gx * JxW / hx;
+            Fy[q] = -k * gy * JxW / hy;
+            Fz[q] = -k * gz * JxW / hz;
+          }
+
+    double acc[MAXN * MAXN * MAXN];
+    double a1[MAXN * MAXN * MAXN], a2[MAXN * MAXN * MAXN];
+    apply_dir<true>(t.S, n, 2, V, a1);
+    apply_dir<true>(t.S, n, 1, a1, a2);
+    apply_dir<true>(t.S, n, 0, a2, acc);
+    apply_dir<true>(t.S, n, 2, Fx, a1);
+    apply_dir<true>(t.S, n, 1, a1, a2);
+    apply_dir<true>(t.D, n, 0, a2, a1);
+    for (int i = 0; i < n3; ++i)
+      acc[i] += a1[i];
+    apply_dir<true>(t.S, n, 2, Fy, a1);
+    apply_dir<true>(t.D, n, 1, a1, a2);
+    apply_dir<true>(t.S, n, 0, a2, a1);
+    for (int i = 0; i < n3; ++i)
+      acc[i] += a1[i];
+    apply_dir<true>(t.D, n, 2, Fz, a1);
+    apply_dir<true>(t.S, n, 1, a1, a2);
+    apply_dir<true>(t.S, n, 0, a2, a1);
+    for (int i = 0; i < n3; ++i)
+      y[i] = acc[i] + a1[i];
+  }
+
   double dot(const std::vector<double> &a, const std::vector<double> &b)
   {
     double s = 0.0;
@@ -550,6 +655,47 @@ int oracle_vmult(const oracle_problem *pb, const double *src, double *dst)
   const Tables t = make_tables(pb->degree);
   const Mesh m = make_mesh(pb);
   vmult_impl(pb, t, m, src, dst);
+  return 0;
+}
+
+// create_rhs restated (heat_diffuse_operator.cpp:783-806): MatrixFree::loop over
+// the cells with zero_dst = false, so dst accumulates; integrate_scatter through
+// the Dirichlet-constrained DoF handler drops constrained rows. No Robin faces
+// and no cut-cell evaporation term. heat_source may be NULL (f = 0).
+int oracle_create_rhs(const oracle_problem *pb, const double *temperature,
+                      const double *temperature_old, const double *heat_source, double *dst)
+{
+  if (!valid(pb) || !temperature || !temperature_old || !dst)
+    return -1;
+  const Tables t = make_tables(pb->degree);
+  const Mesh m = make_mesh(pb);
+  const int n = t.n;
+  for_cells_coloured(m, [&](const CellGeom &g) {
+    double tn[MAXN * MAXN * MAXN], to[MAXN * MAXN * MAXN], fl[MAXN * MAXN * MAXN],
+      Hl[MAXN * MAXN * MAXN], y[MAXN * MAXN * MAXN];
+    for (int c = 0; c < n; ++c)
+      for (int b = 0; b < n; ++b)
+        for (int a = 0; a < n; ++a)
+          {
+            const int l = (c * n + b) * n + a;
+            const int64_t I = node_index(m, g, a, b, c);
+            // read_dof_values_plain: constrained entries as stored
+            tn[l] = temperature[I];
+            to[l] = temperature_old[I];
+            fl[l] = heat_source ? heat_source[I] : 0.0;
+            Hl[l] = pb->material == 1 ? pb->heaviside[I] : 0.0;
+          }
+    local_rhs(pb, t, m, g, tn, to, fl, Hl, y);
+    for (int c = 0; c < n; ++c)
+      for (int b = 0; b < n; ++b)
+        for (int a = 0; a < n; ++a)
+          {
+            const bool con = is_constrained(pb, m, (int64_t)g.c[0] * m.p + a,
+                                            (int64_t)g.c[1] * m.p + b, (int64_t)g.c[2] * m.p + c);
+            if (!con)
+              dst[node_index(m, g, a, b, c)] += y[(c * n + b) * n + a];
+          }
+  });
   return 0;
 }
 

@@ -69,6 +69,9 @@ int oracle_material(int material, int sharp, int volume_specific, int consistent
                     double liq_k, double heaviside, double *rho_cp, double *k, double *rho,
                     double *cp);
 int oracle_vmult(const oracle_problem *pb, const double *src, double *dst);
+/* dst += residual right-hand side (create_rhs); heat_source may be NULL */
+int oracle_create_rhs(const oracle_problem *pb, const double *temperature,
+                      const double *temperature_old, const double *heat_source, double *dst);
 int oracle_diagonal(const oracle_problem *pb, double *diag);
 int oracle_inverse_diagonal(const oracle_problem *pb, double *inv_diag);
 /* solver: 0 = CG, 1 = GMRES(restart); x holds the initial guess on entry */
