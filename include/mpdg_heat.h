@@ -130,6 +130,17 @@ mpdg_status mpdg_heat_vmult(mpdg_heat *h, double *dst, const double *src);
 mpdg_status mpdg_heat_vmult_f32(mpdg_heat *h, float *dst, const float *src);
 /* compute_inverse_diagonal_from_matrixfree (heat_diffuse_operator.cpp:271-284), FP64. */
 mpdg_status mpdg_heat_inverse_diagonal(mpdg_heat *h, double *inv_diag);
+/* HeatDiffuseMultiPhaseOperator::create_rhs(dst, src = temperature_old)
+ * (heat_diffuse_operator.cpp:783-806, rhs_cell_loop :393-529), FP64:
+ *   dst += -( rho_cp/dt M (T - T_old) - M_1 f + K_k T + rho_cp C_u T )
+ * on the free rows; constrained rows of dst are left unchanged. T, T_old and
+ * the heat source f are read plain (constrained entries as stored), as the
+ * reference's read_dof_values_plain does. The reference operator holds T and
+ * f by reference from its constructor (heat_diffuse_operator.hpp:186-188);
+ * here they are arguments. heat_source may be NULL (f = 0). No Robin faces and
+ * no evaporative cooling term. */
+mpdg_status mpdg_heat_create_rhs(mpdg_heat *h, double *dst, const double *temperature,
+                                 const double *temperature_old, const double *heat_source);
 /* JacobiPreconditioner::vmult (preconditioner_jacobi.hpp:51-56): dst = inv_diag .* src. */
 mpdg_status mpdg_heat_jacobi_vmult(mpdg_heat *h, double *dst, const double *src,
                                    const double *inv_diag);

@@ -83,6 +83,20 @@ namespace mpdg
       check_size(diag);
       check(mpdg_heat_inverse_diagonal(h_, diag.data));
     }
+    // HeatDiffuseMultiPhaseOperator::create_rhs(dst, src = temperature_old): adds
+    // the residual right-hand side to dst; temperature and heat_source are the
+    // vectors the reference operator holds from its constructor.
+    void create_rhs(DeviceVector &dst, const DeviceVector &src, const DeviceVector &temperature,
+                    const DeviceVector *heat_source = nullptr) const
+    {
+      check_size(dst);
+      check_size(src);
+      check_size(temperature);
+      if (heat_source)
+        check_size(*heat_source);
+      check(mpdg_heat_create_rhs(h_, dst.data, temperature.data, src.data,
+                                 heat_source ? heat_source->data : nullptr));
+    }
 
     mpdg_heat *handle() const { return h_; }
 

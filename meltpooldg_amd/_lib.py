@@ -103,6 +103,7 @@ SIGNATURES = {
     "mpdg_heat_vmult": (ctypes.c_int, [_VP, _VP, _VP]),
     "mpdg_heat_vmult_f32": (ctypes.c_int, [_VP, _VP, _VP]),
     "mpdg_heat_inverse_diagonal": (ctypes.c_int, [_VP, _VP]),
+    "mpdg_heat_create_rhs": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP]),
     "mpdg_heat_jacobi_vmult": (ctypes.c_int, [_VP, _VP, _VP, _VP]),
     "mpdg_heat_solve": (ctypes.c_int, [_VP, _VP, _VP, ctypes.POINTER(SolverParams),
                                        ctypes.POINTER(SolverReport)]),

@@ -185,6 +185,62 @@ namespace mpdg
         }
     }
 
+
+    // Per-cell LDS stride of an N^3 tensor: odd, so lanes working on
+    // consecutive cells hit distinct banks.
+    template <int N>
+    constexpr int lds_stride()
+    {
+      return (N * N * N) % 2 == 0 ? N * N * N + 1 : N * N * N;
+    }
+
+    template <typename Number>
+    __device__ __forceinline__ bool on_dirichlet(const CellParams<Number> &prm, int i, int d)
+    {
+      const int n1 = d == 0 ? prm.n1x : (d == 1 ? prm.n1y : prm.n1z);
+      return (((prm.faces >> (2 * d)) & 1u) && i == 0) ||
+             (((prm.faces >> (2 * d + 1)) & 1u) && i == n1 - 1);
+    }
+
+    struct Segment
+    {
+      int sx, cy, cz, cx0, ncell;
+    };
+
+    template <typename Number, int CPB>
+    __device__ __forceinline__ Segment segment_of(const CellParams<Number> &prm, int colour)
+    {
+      const int nsx = (prm.nx + CPB - 1) / CPB;
+      const int ox = colour & 1, oy = (colour >> 1) & 1, oz = (colour >> 2) & 1;
+      const int msx = (nsx - ox + 1) / 2;
+      const int my = (prm.ny - oy + 1) / 2;
+      int b = blockIdx.x;
+      Segment s;
+      s.sx = ox + 2 * (b % msx);
+      b /= msx;
+      s.cy = oy + 2 * (b % my);
+      s.cz = oz + 2 * (b / my);
+      s.cx0 = s.sx * CPB;
+      s.ncell = min(CPB, prm.nx - s.cx0);
+      return s;
+    }
+
+    // Conductivity at quadrature point (qx, qy, qz) of cell (cx, cy, cz):
+    // constant, separable sine, or the two-phase value staged in KBl.
+    template <typename Number, int COEF, int N>
+    __device__ __forceinline__ Number conductivity(const CellParams<Number> &prm,
+                                                   const Number *KBl, int cx, int cy, int cz,
+                                                   int qx, int qy, int qz)
+    {
+      if constexpr (COEF == kCoefConstant)
+        return prm.k0;
+      else if constexpr (COEF == kCoefSine)
+        return prm.k0 + prm.kamp * prm.sin2pi[0][cx * N + qx] * prm.sin2pi[1][cy * N + qy] *
+                          prm.sin2pi[2][cz * N + qz];
+      else
+        return KBl[(qz * N + qy) * N + qx];
+    }
+
     // Operator served by this workgroup in a batched launch: the last s with
     // block_off[s] <= blockIdx.x (block_off holds nops + 1 ascending entries).
     __device__ __forceinline__ int find_block_owner(const int *block_off, int nops)

@@ -184,6 +184,7 @@ namespace
     const double dt_inv = H->dt_set ? 1.0 / H->dt : 0.0; // OperatorBase::reset_time_increment
     prm.mass = (Number)(dt_inv * g_rhocp * vol);
     prm.dtinv_vol = (Number)(dt_inv * vol);
+    prm.vol = (Number)vol;
     for (int k = 0; k < 3; ++k)
       {
         prm.kf[k] = (Number)(vol / (H->h[k] * H->h[k]));
@@ -727,6 +728,18 @@ mpdg_status mpdg_heat_inverse_diagonal(mpdg_heat *h, double *inv_diag)
   if (!inv_diag)
     return fail(MPDG_ERR_ARGUMENT, "null vector");
   return diagonal_inverse(h, inv_diag);
+}
+
+mpdg_status mpdg_heat_create_rhs(mpdg_heat *h, double *dst, const double *temperature,
+                                 const double *temperature_old, const double *heat_source)
+{
+  if (mpdg_status s = check_ready(h))
+    return s;
+  if (!dst || !temperature || !temperature_old)
+    return fail(MPDG_ERR_ARGUMENT, "null vector");
+  MPDG_HIP(launch_create_rhs(h->desc.degree, coef_mode(h), h->desc.velocity != 0, h->prm64, dst,
+                             temperature, temperature_old, heat_source, h->heav64.ptr, h->stream));
+  return MPDG_OK;
 }
 
 mpdg_status mpdg_heat_jacobi_vmult(mpdg_heat *h, double *dst, const double *src,

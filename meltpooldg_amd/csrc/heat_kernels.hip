@@ -27,13 +27,6 @@ namespace mpdg
 {
   namespace
   {
-    template <int N>
-    constexpr int lds_stride()
-    {
-      // odd per-cell stride keeps lanes of consecutive cells on distinct banks
-      return (N * N * N) % 2 == 0 ? N * N * N + 1 : N * N * N;
-    }
-
     template <int P, bool FP32>
     constexpr int cpb()
     {
@@ -90,37 +83,6 @@ namespace mpdg
     }
 
 
-    template <typename Number>
-    __device__ __forceinline__ bool on_dirichlet(const CellParams<Number> &prm, int i, int d)
-    {
-      const int n1 = d == 0 ? prm.n1x : (d == 1 ? prm.n1y : prm.n1z);
-      return (((prm.faces >> (2 * d)) & 1u) && i == 0) ||
-             (((prm.faces >> (2 * d + 1)) & 1u) && i == n1 - 1);
-    }
-
-    struct Segment
-    {
-      int sx, cy, cz, cx0, ncell;
-    };
-
-    template <typename Number, int CPB>
-    __device__ __forceinline__ Segment segment_of(const CellParams<Number> &prm, int colour)
-    {
-      const int nsx = (prm.nx + CPB - 1) / CPB;
-      const int ox = colour & 1, oy = (colour >> 1) & 1, oz = (colour >> 2) & 1;
-      const int msx = (nsx - ox + 1) / 2;
-      const int my = (prm.ny - oy + 1) / 2;
-      int b = blockIdx.x;
-      Segment s;
-      s.sx = ox + 2 * (b % msx);
-      b /= msx;
-      s.cy = oy + 2 * (b % my);
-      s.cz = oz + 2 * (b / my);
-      s.cx0 = s.sx * CPB;
-      s.ncell = min(CPB, prm.nx - s.cx0);
-      return s;
-    }
-
     // Coloured first-touch scatter of one cell x-line (b = s0, c = s1) after the
     // in-segment x-face exchange (see the file header).
     template <int P, typename Number, int CPB>
@@ -157,20 +119,6 @@ namespace mpdg
           else
             *ptr = *ptr + Y[a];
         }
-    }
-
-    template <typename Number, int COEF, int N>
-    __device__ __forceinline__ Number conductivity(const CellParams<Number> &prm,
-                                                   const Number *KBl, int cx, int cy, int cz,
-                                                   int qx, int qy, int qz)
-    {
-      if constexpr (COEF == kCoefConstant)
-        return prm.k0;
-      else if constexpr (COEF == kCoefSine)
-        return prm.k0 + prm.kamp * prm.sin2pi[0][cx * N + qx] * prm.sin2pi[1][cy * N + qy] *
-                          prm.sin2pi[2][cz * N + qz];
-      else
-        return KBl[(qz * N + qy) * N + qx];
     }
 
     // Orders this wavefront's LDS accesses across lanes: LDS instructions of one

@@ -18,6 +18,7 @@ namespace mpdg
     unsigned faces;       // Dirichlet face mask (bit 2d: lo, 2d+1: hi)
     Number mass;          // rho_cp / dt * |J| for a constant material
     Number dtinv_vol;     // |J| / dt (two-phase: multiplied by rho_cp(H) at q)
+    Number vol;           // |J| (cell volume, affine Cartesian cells)
     Number kf[3];         // |J| / h_d^2 (Laplace metric, affine Cartesian cells)
     Number vf[3];         // |J| / h_d (convection metric)
     Number k0;            // material conductivity (constant material)
@@ -138,6 +139,14 @@ namespace mpdg
   VmultGrid cell_grid(int nx, int ny, int nz, int tile_log2, int layers, int max_blocks);
   int cell_items_per_block();
   int cell_resident_blocks(int degree, bool fp32);
+
+  // Residual right-hand side (rhs_kernels.hip): dst += -(rho_cp/dt M (T - T_old)
+  // - M_1 f + K T + rho_cp C_u T) on the free rows, T, T_old, f read plain
+  // (constrained values included); heat_source may be null.
+  hipError_t launch_create_rhs(int degree, int coef, bool vel, const CellParams<double> &prm,
+                               double *dst, const double *temperature,
+                               const double *temperature_old, const double *heat_source,
+                               const double *heaviside, hipStream_t stream);
 
   template <typename Number>
   hipError_t launch_diagonal(int degree, int coef, bool vel, const CellParams<Number> &prm,

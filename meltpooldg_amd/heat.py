@@ -186,6 +186,17 @@ class HeatDiffuseOperator:
         return {"z_layers": t.z_layers, "blocks_per_cu": t.blocks_per_cu,
                 "variant": names.get(t.variant, t.variant), "tile_log2": t.tile_log2}
 
+    def create_rhs(self, dst, src, temperature, heat_source=None):
+        """HeatDiffuseMultiPhaseOperator::create_rhs(dst, src = temperature_old)
+        (source/heat/heat_diffuse_operator.cpp:783-806): adds the residual
+        right-hand side to ``dst`` on the free rows. The reference operator holds
+        ``temperature`` and ``heat_source`` from its constructor; here they are
+        arguments (``heat_source`` None: no source). FP64 vectors."""
+        for v in (dst, src, temperature) + ((heat_source,) if heat_source is not None else ()):
+            self._check_vec(v, "float64")
+        L.check(self._lib.mpdg_heat_create_rhs(self._h, _ptr(dst), _ptr(temperature), _ptr(src),
+                                               _ptr(heat_source) if heat_source is not None else None))
+
     def get_tuning(self, precision=None):
         """Launch tuning of the vmult kernels for one precision (engine extension)."""
         t = L.Tuning()
