@@ -56,6 +56,7 @@ def lib():
         L.oracle_material.argtypes = [ctypes.c_int] * 4 + [ctypes.c_double] * 7 + [dp] * 4
         L.oracle_vmult.argtypes = [P, dp, dp]
         L.oracle_create_rhs.argtypes = [P, dp, dp, dp, dp]
+        L.oracle_l2_norm.argtypes = [P, dp, dp]
         L.oracle_diagonal.argtypes = [P, dp]
         L.oracle_inverse_diagonal.argtypes = [P, dp]
         L.oracle_solve.argtypes = [P, ctypes.c_int, ctypes.c_int, dp, dp, dp, ctypes.c_double,
@@ -159,6 +160,13 @@ class Problem:
         assert lib().oracle_create_rhs(ctypes.byref(self.s), _dp(tn), _dp(to),
                                        _dp(f) if f is not None else None, _dp(out)) == 0
         return out
+
+    def l2_norm(self, v):
+        """L2 norm of the FE field v with the operator's Gauss quadrature."""
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        out = np.zeros(1)
+        assert lib().oracle_l2_norm(ctypes.byref(self.s), _dp(v), _dp(out)) == 0
+        return float(out[0])
 
     def diagonal(self):
         d = np.zeros(self.N)

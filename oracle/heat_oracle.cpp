@@ -699,6 +699,53 @@ int oracle_create_rhs(const oracle_problem *pb, const double *temperature,
   return 0;
 }
 
+// VectorTools::compute_norm (L2) of the FE field restated
+// (source/utilities/vector_tools.cpp:170-200 -> compute_global_error_norm against
+// the zero function): sqrt(sum over cells and Gauss points of JxW u(x_q)^2),
+// values read plain.
+int oracle_l2_norm(const oracle_problem *pb, const double *u, double *norm)
+{
+  if (!valid(pb) || !u || !norm)
+    return -1;
+  const Tables t = make_tables(pb->degree);
+  const Mesh m = make_mesh(pb);
+  const int n = t.n;
+  const double jac_det = m.h[0] * m.h[1] * m.h[2];
+  const int64_t cells = (int64_t)m.n[0] * m.n[1] * m.n[2];
+  std::vector<double> per_cell(cells, 0.0);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < cells; ++i)
+    {
+      CellGeom g;
+      g.c[0] = (int)(i % m.n[0]);
+      g.c[1] = (int)((i / m.n[0]) % m.n[1]);
+      g.c[2] = (int)(i / ((int64_t)m.n[0] * m.n[1]));
+      double ul[MAXN * MAXN * MAXN], a1[MAXN * MAXN * MAXN], a2[MAXN * MAXN * MAXN],
+        uq[MAXN * MAXN * MAXN];
+      for (int c = 0; c < n; ++c)
+        for (int b = 0; b < n; ++b)
+          for (int a = 0; a < n; ++a)
+            ul[(c * n + b) * n + a] = u[node_index(m, g, a, b, c)];
+      apply_dir<false>(t.S, n, 0, ul, a1);
+      apply_dir<false>(t.S, n, 1, a1, a2);
+      apply_dir<false>(t.S, n, 2, a2, uq);
+      double s = 0.0;
+      for (int qz = 0; qz < n; ++qz)
+        for (int qy = 0; qy < n; ++qy)
+          for (int qx = 0; qx < n; ++qx)
+            {
+              const double v = uq[(qz * n + qy) * n + qx];
+              s += t.W[qx] * t.W[qy] * t.W[qz] * jac_det * v * v;
+            }
+      per_cell[i] = s;
+    }
+  double total = 0.0;
+  for (double v : per_cell)
+    total += v;
+  *norm = std::sqrt(total);
+  return 0;
+}
+
 // MatrixFreeTools::compute_diagonal restated: apply the cell operator to each
 // local unit vector and keep the matching entry (heat_diffuse_operator.cpp:332-358).
 int oracle_diagonal(const oracle_problem *pb, double *diag)

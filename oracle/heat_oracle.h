@@ -72,6 +72,8 @@ int oracle_vmult(const oracle_problem *pb, const double *src, double *dst);
 /* dst += residual right-hand side (create_rhs); heat_source may be NULL */
 int oracle_create_rhs(const oracle_problem *pb, const double *temperature,
                       const double *temperature_old, const double *heat_source, double *dst);
+/* L2 norm of the FE field v (Gauss quadrature of degree + 1 points) */
+int oracle_l2_norm(const oracle_problem *pb, const double *v, double *norm);
 int oracle_diagonal(const oracle_problem *pb, double *diag);
 int oracle_inverse_diagonal(const oracle_problem *pb, double *inv_diag);
 /* solver: 0 = CG, 1 = GMRES(restart); x holds the initial guess on entry */
