@@ -182,6 +182,62 @@ int64_t mpdg_heat_batch_n_dofs(const mpdg_heat_batch *b, int member);
 mpdg_status mpdg_heat_batch_solve(mpdg_heat_batch *b, double *const *x, const double *const *rhs,
                                   const mpdg_solver_params *params, mpdg_batch_report *reports);
 
+/* ---------------------------------------------------------------------------
+ * Heat time step: Newton on the residual (create_rhs) with the tangent solve.
+ * ------------------------------------------------------------------------- */
+
+/* NonlinearSolverData (include/meltpooldg/linear_algebra/nonlinear_solver_data.hpp:10-20). */
+typedef struct {
+  int max_nonlinear_iterations;          /* 10 */
+  double field_correction_tolerance;     /* 1e-10 */
+  double residual_tolerance;             /* 1e-9 */
+  int max_nonlinear_iterations_alt;      /* 0: further increments with the _alt tolerances */
+  double field_correction_tolerance_alt; /* 1e-9 */
+  double residual_tolerance_alt;         /* 1e-8 */
+  int use_relative_tolerances;           /* 0: absolute l2 norms of rhs and increment */
+} mpdg_newton_params;
+void mpdg_newton_params_default(mpdg_newton_params *p);
+
+typedef struct {
+  int iterations;          /* increments solved before convergence was detected */
+  int linear_iterations;   /* Krylov iterations summed over the increments */
+  double initial_residual; /* ||rhs|| of the first increment */
+  double residual;         /* last ||rhs|| (divided by the first when relative) */
+  double update;           /* last ||dT|| (divided by ||T|| when relative) */
+  double seconds;
+} mpdg_newton_report;
+
+/* HeatDiffuseOperation::solve (heat_diffuse_operation.cpp:261-298) with
+ * NewtonRaphsonSolver::solve / is_converged / solve_increment
+ * (newton_raphson_solver.cpp:22-80, 118-175) and the heat operation's callbacks
+ * (heat_diffuse_operation.cpp:82-116). On entry `temperature` holds the start
+ * value (the predictor, Dirichlet values set); on return the time step's
+ * solution. Per increment: rhs = create_rhs(T, T_old, f); J dT = rhs solved by
+ * mpdg_heat_solve with `linear` from dT = 0; converged when ||rhs|| <
+ * residual_tolerance and ||dT|| < field_correction_tolerance (that last dT is
+ * not applied), otherwise T += dT (dT is zero on constrained DoFs). Returns
+ * MPDG_ERR_NOT_CONVERGED where the reference throws ExcNewtonDidNotConverge.
+ * newton / linear / report may be NULL (defaults; the heat defaults of the
+ * reference's linear solver are GMRES + Diagonal, heat_data.cpp:11-12). */
+mpdg_status mpdg_heat_newton_solve(mpdg_heat *h, double *temperature,
+                                   const double *temperature_old, const double *heat_source,
+                                   const mpdg_newton_params *newton,
+                                   const mpdg_solver_params *linear, mpdg_newton_report *report);
+
+/* compute_linear_predictor (linear_algebra/predictor_linear.cpp:76-96):
+ * predictor = (1 + dt/dt_old) old - (dt/dt_old) old_old, or old if |dt_old| < 1e-12. */
+mpdg_status mpdg_heat_linear_predictor(mpdg_heat *h, double *predictor, const double *old_vec,
+                                       const double *old_old_vec, double dt, double dt_old);
+
+/* AffineConstraints::distribute for the operator's homogeneous Dirichlet
+ * constraints (the predictor step of heat_diffuse_operation.cpp:248-249): sets
+ * the constrained entries of v to 0. */
+mpdg_status mpdg_heat_distribute_constraints(mpdg_heat *h, double *v);
+
+/* VectorTools::compute_norm, L2 (utilities/vector_tools.cpp:170-200): L2 norm of
+ * the FE field v with the operator's Gauss quadrature, entries read plain. */
+mpdg_status mpdg_heat_l2_norm(mpdg_heat *h, const double *v, double *norm);
+
 /* Launch tuning of the vmult kernels, one set per precision (no counterpart
  * in the reference, whose cell loop is tuned by deal.II's
  * MatrixFree::AdditionalData). Results do not depend on it beyond the order

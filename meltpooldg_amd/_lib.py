@@ -81,6 +81,29 @@ class BatchReport(ctypes.Structure):
     ]
 
 
+class NewtonParams(ctypes.Structure):
+    _fields_ = [
+        ("max_nonlinear_iterations", ctypes.c_int),
+        ("field_correction_tolerance", ctypes.c_double),
+        ("residual_tolerance", ctypes.c_double),
+        ("max_nonlinear_iterations_alt", ctypes.c_int),
+        ("field_correction_tolerance_alt", ctypes.c_double),
+        ("residual_tolerance_alt", ctypes.c_double),
+        ("use_relative_tolerances", ctypes.c_int),
+    ]
+
+
+class NewtonReport(ctypes.Structure):
+    _fields_ = [
+        ("iterations", ctypes.c_int),
+        ("linear_iterations", ctypes.c_int),
+        ("initial_residual", ctypes.c_double),
+        ("residual", ctypes.c_double),
+        ("update", ctypes.c_double),
+        ("seconds", ctypes.c_double),
+    ]
+
+
 MPDG_VMULT_LINE = 0
 MPDG_VMULT_CELL = 1
 MPDG_VMULT_LINE_MFMA = 2
@@ -104,6 +127,14 @@ SIGNATURES = {
     "mpdg_heat_vmult_f32": (ctypes.c_int, [_VP, _VP, _VP]),
     "mpdg_heat_inverse_diagonal": (ctypes.c_int, [_VP, _VP]),
     "mpdg_heat_create_rhs": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP]),
+    "mpdg_newton_params_default": (None, [ctypes.POINTER(NewtonParams)]),
+    "mpdg_heat_newton_solve": (ctypes.c_int, [_VP, _VP, _VP, _VP, ctypes.POINTER(NewtonParams),
+                                              ctypes.POINTER(SolverParams),
+                                              ctypes.POINTER(NewtonReport)]),
+    "mpdg_heat_linear_predictor": (ctypes.c_int, [_VP, _VP, _VP, _VP, ctypes.c_double,
+                                                  ctypes.c_double]),
+    "mpdg_heat_l2_norm": (ctypes.c_int, [_VP, _VP, ctypes.POINTER(ctypes.c_double)]),
+    "mpdg_heat_distribute_constraints": (ctypes.c_int, [_VP, _VP]),
     "mpdg_heat_jacobi_vmult": (ctypes.c_int, [_VP, _VP, _VP, _VP]),
     "mpdg_heat_solve": (ctypes.c_int, [_VP, _VP, _VP, ctypes.POINTER(SolverParams),
                                        ctypes.POINTER(SolverReport)]),
@@ -134,6 +165,14 @@ def load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is not built; run `python -m meltpooldg_amd.build` "
                           "(the HIP library is the only compute path)")
+    # One HIP runtime per process: torch ships its own libamdhip64.so.7 and refers
+    # to it as libamdhip64.so, so if this library loaded /opt/rocm's copy first,
+    # torch would load a second runtime (and the first one would see no device).
+    # Loading torch first makes this library bind to the runtime already present.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

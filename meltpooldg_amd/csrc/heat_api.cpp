@@ -99,6 +99,11 @@ struct mpdg_heat
   SolverScalars *host_scal = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
+  // Newton time step: residual, increment, reduction scratch and results
+  DevBuf<double> nrhs, ndu, npart, nres;
+  DevBuf<unsigned> nticket;
+  double *host_nres = nullptr;
+
   ~mpdg_heat()
   {
     for (auto *b : {&sin_tab64, &sin_tab32_src, &heav64, &r, &p, &q, &inv_diag, &partials, &history,
@@ -108,6 +113,11 @@ struct mpdg_heat
       b->release();
     if (host_gm)
       (void)hipHostFree(host_gm);
+    for (auto *b : {&nrhs, &ndu, &npart, &nres})
+      b->release();
+    nticket.release();
+    if (host_nres)
+      (void)hipHostFree(host_nres);
     scal.release();
     if (host_scal)
       (void)hipHostFree(host_scal);
@@ -305,6 +315,37 @@ namespace
     MPDG_HIP(hipStreamSynchronize(H->stream));
     return MPDG_OK;
   }
+  // Scratch of the time-step routines: two N-vectors, reduction partials, a
+  // zeroed ticket counter and a pinned result slot.
+  mpdg_status ensure_newton(mpdg_heat *H)
+  {
+    MPDG_HIP(H->nrhs.alloc(H->N));
+    MPDG_HIP(H->ndu.alloc(H->N));
+    const int nb = l2_norm_blocks(H->desc.degree, H->desc.n_cells[0], H->desc.n_cells[1],
+                                  H->desc.n_cells[2]);
+    MPDG_HIP(H->npart.alloc((size_t)std::max(nb, kVecBlocks)));
+    MPDG_HIP(H->nres.alloc(4));
+    if (!H->nticket.ptr)
+      {
+        MPDG_HIP(H->nticket.alloc(1));
+        MPDG_HIP(hipMemset(H->nticket.ptr, 0, sizeof(unsigned)));
+      }
+    if (!H->host_nres)
+      MPDG_HIP(hipHostMalloc((void **)&H->host_nres, 4 * sizeof(double), hipHostMallocDefault));
+    return MPDG_OK;
+  }
+
+  // Euclidean norm of a device vector (deterministic reduction), synchronous.
+  mpdg_status device_norm(mpdg_heat *H, const double *v, double &out)
+  {
+    MPDG_HIP(launch_dot(v, v, H->N, H->npart.ptr, H->nres.ptr, H->nticket.ptr, H->stream));
+    MPDG_HIP(hipMemcpyAsync(H->host_nres, H->nres.ptr, sizeof(double), hipMemcpyDeviceToHost,
+                            H->stream));
+    MPDG_HIP(hipStreamSynchronize(H->stream));
+    out = std::sqrt(H->host_nres[0]);
+    return MPDG_OK;
+  }
+
 
   mpdg_status vmult64(mpdg_heat *H, double *dst, const double *src, double *partials,
                       SolverScalars *scal)
@@ -740,6 +781,148 @@ mpdg_status mpdg_heat_create_rhs(mpdg_heat *h, double *dst, const double *temper
   MPDG_HIP(launch_create_rhs(h->desc.degree, coef_mode(h), h->desc.velocity != 0, h->prm64, dst,
                              temperature, temperature_old, heat_source, h->heav64.ptr, h->stream));
   return MPDG_OK;
+}
+
+void mpdg_newton_params_default(mpdg_newton_params *p)
+{
+  if (!p)
+    return;
+  p->max_nonlinear_iterations = 10;
+  p->field_correction_tolerance = 1e-10;
+  p->residual_tolerance = 1e-9;
+  p->max_nonlinear_iterations_alt = 0;
+  p->field_correction_tolerance_alt = 1e-9;
+  p->residual_tolerance_alt = 1e-8;
+  p->use_relative_tolerances = 0;
+}
+
+mpdg_status mpdg_heat_distribute_constraints(mpdg_heat *h, double *v)
+{
+  if (!h || !v)
+    return fail(MPDG_ERR_ARGUMENT, "null argument");
+  MPDG_HIP(launch_zero_constrained(v, h->prm64.n1x, h->prm64.n1y, h->prm64.n1z,
+                                   h->desc.dirichlet_faces, h->stream));
+  return MPDG_OK;
+}
+
+mpdg_status mpdg_heat_l2_norm(mpdg_heat *h, const double *v, double *norm)
+{
+  if (!h || !v || !norm)
+    return fail(MPDG_ERR_ARGUMENT, "null argument");
+  if (mpdg_status s = ensure_newton(h))
+    return s;
+  MPDG_HIP(launch_l2_norm_sq(h->desc.degree, h->prm64, v, h->npart.ptr, h->nticket.ptr,
+                             h->nres.ptr, h->stream));
+  MPDG_HIP(hipMemcpyAsync(h->host_nres, h->nres.ptr, sizeof(double), hipMemcpyDeviceToHost,
+                          h->stream));
+  MPDG_HIP(hipStreamSynchronize(h->stream));
+  *norm = std::sqrt(h->host_nres[0]);
+  return MPDG_OK;
+}
+
+mpdg_status mpdg_heat_linear_predictor(mpdg_heat *h, double *predictor, const double *old_vec,
+                                       const double *old_old_vec, double dt, double dt_old)
+{
+  if (!h || !predictor || !old_vec || !old_old_vec)
+    return fail(MPDG_ERR_ARGUMENT, "null argument");
+  if (std::abs(dt_old) < 1e-12)
+    {
+      if (predictor != old_vec)
+        MPDG_HIP(hipMemcpyAsync(predictor, old_vec, h->N * sizeof(double),
+                                hipMemcpyDeviceToDevice, h->stream));
+      return MPDG_OK;
+    }
+  const double frac = dt / dt_old;
+  MPDG_HIP(launch_axpby(predictor, frac + 1.0, old_vec, -frac, old_old_vec, h->N, h->stream));
+  return MPDG_OK;
+}
+
+mpdg_status mpdg_heat_newton_solve(mpdg_heat *h, double *temperature,
+                                   const double *temperature_old, const double *heat_source,
+                                   const mpdg_newton_params *newton,
+                                   const mpdg_solver_params *linear, mpdg_newton_report *report)
+{
+  if (mpdg_status s = check_ready(h))
+    return s;
+  if (!temperature || !temperature_old)
+    return fail(MPDG_ERR_ARGUMENT, "null vector");
+  mpdg_newton_params np;
+  if (newton)
+    np = *newton;
+  else
+    mpdg_newton_params_default(&np);
+  mpdg_solver_params lp;
+  if (linear)
+    lp = *linear;
+  else
+    {
+      mpdg_solver_params_default(&lp);
+      lp.solver_type = MPDG_SOLVER_GMRES; // heat defaults (heat_data.cpp:11-12)
+      lp.preconditioner_type = MPDG_PRECONDITIONER_DIAGONAL;
+    }
+  if (np.max_nonlinear_iterations < 0 || np.max_nonlinear_iterations_alt < 0)
+    return fail(MPDG_ERR_ARGUMENT, "negative Newton iteration limit");
+  if (mpdg_status s = ensure_newton(h))
+    return s;
+  const auto t0 = std::chrono::steady_clock::now();
+  mpdg_newton_report rep{};
+  double res_tol = np.residual_tolerance, corr_tol = np.field_correction_tolerance;
+  const int max_total = std::max(np.max_nonlinear_iterations + np.max_nonlinear_iterations_alt, 1);
+  double r0 = 0.0;
+  for (int i = 0; i < max_total; ++i)
+    {
+      if (i == np.max_nonlinear_iterations)
+        {
+          res_tol = np.residual_tolerance_alt;
+          corr_tol = np.field_correction_tolerance_alt;
+        }
+      // solve_increment: rhs = 0, dT = 0, residual, Jacobian solve
+      MPDG_HIP(launch_fill(h->nrhs.ptr, 0.0, h->N, h->stream));
+      MPDG_HIP(launch_fill(h->ndu.ptr, 0.0, h->N, h->stream));
+      if (mpdg_status s = mpdg_heat_create_rhs(h, h->nrhs.ptr, temperature, temperature_old,
+                                               heat_source))
+        return s;
+      mpdg_solver_report lr{};
+      const mpdg_status ls = mpdg_heat_solve(h, h->ndu.ptr, h->nrhs.ptr, &lp, &lr);
+      rep.linear_iterations += lr.iterations;
+      if (ls != MPDG_OK)
+        return ls;
+      double res = 0.0, upd = 0.0;
+      if (mpdg_status s = device_norm(h, h->nrhs.ptr, res))
+        return s;
+      if (mpdg_status s = device_norm(h, h->ndu.ptr, upd))
+        return s;
+      if (i == 0)
+        rep.initial_residual = r0 = res;
+      if (np.use_relative_tolerances)
+        {
+          if (r0 > 0.0)
+            res /= r0;
+          double tn = 0.0;
+          if (mpdg_status s = device_norm(h, temperature, tn))
+            return s;
+          if (tn > 0.0)
+            upd /= tn;
+        }
+      rep.residual = res;
+      rep.update = upd;
+      rep.iterations = i;
+      if (res < res_tol && upd < corr_tol)
+        {
+          rep.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+          if (report)
+            *report = rep;
+          return MPDG_OK;
+        }
+      // solution += solution_update (homogeneous constraints: dT is 0 there)
+      MPDG_HIP(launch_axpby(temperature, 1.0, temperature, 1.0, h->ndu.ptr, h->N, h->stream));
+    }
+  MPDG_HIP(hipStreamSynchronize(h->stream));
+  rep.iterations = max_total;
+  rep.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (report)
+    *report = rep;
+  return fail(MPDG_ERR_NOT_CONVERGED, "Newton-Raphson solver did not converge");
 }
 
 mpdg_status mpdg_heat_jacobi_vmult(mpdg_heat *h, double *dst, const double *src,

@@ -148,6 +148,12 @@ namespace mpdg
                                const double *temperature_old, const double *heat_source,
                                const double *heaviside, hipStream_t stream);
 
+  // *out = sum over cells and Gauss points of JxW u^2 (FE L2 norm squared);
+  // partials needs l2_norm_blocks() slots, *ticket must start at 0 (reset on exit).
+  int l2_norm_blocks(int degree, int nx, int ny, int nz);
+  hipError_t launch_l2_norm_sq(int degree, const CellParams<double> &prm, const double *u,
+                               double *partials, unsigned *ticket, double *out, hipStream_t s);
+
   template <typename Number>
   hipError_t launch_diagonal(int degree, int coef, bool vel, const CellParams<Number> &prm,
                              const DiagTables<Number> &tab, Number *diag,
@@ -221,6 +227,11 @@ namespace mpdg
   // Mixed precision helpers.
   hipError_t launch_residual(double *r, const double *b, const double *ax, int64_t n,
                              double *partials, SolverScalars *scal, hipStream_t s);
+  hipError_t launch_zero_constrained(double *v, int n1x, int n1y, int n1z, unsigned faces,
+                                     hipStream_t s);
+  // dst = a x + b y (dst may alias x or y)
+  hipError_t launch_axpby(double *dst, double a, const double *x, double b, const double *y,
+                          int64_t n, hipStream_t s);
   hipError_t launch_to_f32(float *dst, const double *src, int64_t n, hipStream_t s);
   hipError_t launch_add_f32(double *x, const float *d, int64_t n, hipStream_t s);
   hipError_t launch_fill(double *x, double v, int64_t n, hipStream_t s);

@@ -298,6 +298,113 @@ namespace mpdg
         }
     }
 
+    // Squared L2 norm of the FE field u: one workgroup per row segment of CPB
+    // cells (all segments in one launch), values interpolated to the Gauss
+    // points through LDS, JxW u^2 summed per workgroup into partials[block];
+    // the workgroup that finishes last (ticket) adds the partials in block
+    // order into *out, so the result is reproducible.
+    template <int P, int CPB, int THREADS>
+    __global__ void __launch_bounds__(THREADS)
+      l2_norm_kernel(const double *__restrict__ u, const CellParams<double> prm,
+                     double *__restrict__ partials, unsigned *__restrict__ ticket,
+                     double *__restrict__ out)
+    {
+      using Number = double;
+      constexpr int N = P + 1;
+      constexpr int NL = N * N;
+      constexpr int STR = lds_stride<N>();
+      constexpr ct::EO<N> TS = ct::VTab<P>::S;
+      constexpr ct::Tab<N> T1 = ct::VTab<P>::t;
+      __shared__ Number A[CPB * STR];
+      __shared__ double red[THREADS / 64];
+      __shared__ bool is_last;
+
+      const int nsx = (prm.nx + CPB - 1) / CPB;
+      int b = blockIdx.x;
+      const int sx = b % nsx;
+      b /= nsx;
+      const int cy = b % prm.ny;
+      const int cz = b / prm.ny;
+      const int cx0 = sx * CPB;
+      const int ncell = min(CPB, prm.nx - cx0);
+      const int t = threadIdx.x;
+      const bool active = t < CPB * NL;
+      const int l = t % CPB;
+      const int s = t / CPB;
+      const int s0 = s % N, s1 = s / N;
+      const bool cell_ok = active && l < ncell;
+      Number *Al = A + l * STR;
+
+      if (active)
+        {
+          const int64_t base = (int64_t)(cx0 + l) * P +
+                               (int64_t)prm.n1x * (cy * P + s0 + (int64_t)prm.n1y * (cz * P + s1));
+          Number v[N], o[N];
+#pragma unroll
+          for (int a = 0; a < N; ++a)
+            v[a] = cell_ok ? u[base + a] : Number(0);
+          eo_apply<N, 1>(TS, v, o);
+#pragma unroll
+          for (int q = 0; q < N; ++q)
+            Al[(s1 * N + s0) * N + q] = o[q];
+        }
+      __syncthreads();
+      if (active)
+        {
+          Number v[N], o[N];
+#pragma unroll
+          for (int b2 = 0; b2 < N; ++b2)
+            v[b2] = Al[(s1 * N + b2) * N + s0];
+          eo_apply<N, 1>(TS, v, o);
+#pragma unroll
+          for (int b2 = 0; b2 < N; ++b2)
+            Al[(s1 * N + b2) * N + s0] = o[b2];
+        }
+      __syncthreads();
+      double acc = 0.0;
+      if (active)
+        {
+          Number v[N], o[N];
+#pragma unroll
+          for (int c = 0; c < N; ++c)
+            v[c] = Al[(c * N + s1) * N + s0];
+          eo_apply<N, 1>(TS, v, o);
+          const double wxy = T1.w[s0] * T1.w[s1] * (double)prm.vol;
+#pragma unroll
+          for (int q = 0; q < N; ++q)
+            acc += wxy * T1.w[q] * o[q] * o[q];
+        }
+      const double bs = block_sum<THREADS>(acc, red);
+      if (threadIdx.x == 0)
+        {
+          partials[blockIdx.x] = bs;
+          __threadfence();
+          is_last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+        }
+      __syncthreads();
+      if (is_last)
+        {
+          __threadfence();
+          const double total = ordered_sum<THREADS>(partials, gridDim.x, red);
+          if (threadIdx.x == 0)
+            {
+              *out = total;
+              *ticket = 0u;
+            }
+        }
+    }
+
+    template <int P>
+    hipError_t l2_norm_p(const CellParams<double> &prm, const double *u, double *partials,
+                         unsigned *ticket, double *out, hipStream_t s)
+    {
+      constexpr int CPB = cells_per_block_for(P, false);
+      constexpr int THREADS = ((CPB * (P + 1) * (P + 1) + 63) / 64) * 64;
+      const int blocks = ((prm.nx + CPB - 1) / CPB) * prm.ny * prm.nz;
+      l2_norm_kernel<P, CPB, THREADS><<<blocks, THREADS, 0, s>>>(u, prm, partials, ticket, out);
+      return hipGetLastError();
+    }
+
     template <int P, int COEF, bool VEL>
     hipError_t rhs_passes(const CellParams<double> &prm, double *dst, const double *tn,
                           const double *to, const double *f, const double *heav, hipStream_t s)
@@ -332,6 +439,27 @@ namespace mpdg
         }
     }
   } // namespace
+
+  int l2_norm_blocks(int degree, int nx, int ny, int nz)
+  {
+    const int cpb = cells_per_block_for(degree, false);
+    return ((nx + cpb - 1) / cpb) * ny * nz;
+  }
+
+  hipError_t launch_l2_norm_sq(int degree, const CellParams<double> &prm, const double *u,
+                               double *partials, unsigned *ticket, double *out, hipStream_t s)
+  {
+    switch (degree)
+      {
+        case 1: return l2_norm_p<1>(prm, u, partials, ticket, out, s);
+        case 2: return l2_norm_p<2>(prm, u, partials, ticket, out, s);
+        case 3: return l2_norm_p<3>(prm, u, partials, ticket, out, s);
+        case 4: return l2_norm_p<4>(prm, u, partials, ticket, out, s);
+        case 5: return l2_norm_p<5>(prm, u, partials, ticket, out, s);
+        case 6: return l2_norm_p<6>(prm, u, partials, ticket, out, s);
+        default: return hipErrorInvalidValue;
+      }
+  }
 
   hipError_t launch_create_rhs(int degree, int coef, bool vel, const CellParams<double> &prm,
                                double *dst, const double *temperature,

@@ -360,6 +360,31 @@ namespace mpdg
         *out = tot[0];
     }
 
+    __global__ void __launch_bounds__(T)
+      axpby_kernel(double *dst, double a, const double *x, double b, const double *y, int64_t n)
+    {
+      for (int64_t i = blockIdx.x * (int64_t)T + threadIdx.x; i < n; i += (int64_t)gridDim.x * T)
+        dst[i] = a * x[i] + b * y[i];
+    }
+
+    // v = 0 on the lattice nodes of the Dirichlet faces (bit 2d: x_d = lo, 2d+1: hi)
+    __global__ void __launch_bounds__(T)
+      zero_constrained_kernel(double *v, int n1x, int n1y, int n1z, unsigned faces)
+    {
+      const int64_t n = (int64_t)n1x * n1y * n1z;
+      for (int64_t i = blockIdx.x * (int64_t)T + threadIdx.x; i < n; i += (int64_t)gridDim.x * T)
+        {
+          const int ix = (int)(i % n1x);
+          const int iy = (int)((i / n1x) % n1y);
+          const int iz = (int)(i / ((int64_t)n1x * n1y));
+          const bool con = ((faces & 1u) && ix == 0) || ((faces & 2u) && ix == n1x - 1) ||
+                           ((faces & 4u) && iy == 0) || ((faces & 8u) && iy == n1y - 1) ||
+                           ((faces & 16u) && iz == 0) || ((faces & 32u) && iz == n1z - 1);
+          if (con)
+            v[i] = 0.0;
+        }
+    }
+
     __global__ void __launch_bounds__(T) to_f32_kernel(float *dst, const double *src, int64_t n)
     {
       for (int64_t i = blockIdx.x * (int64_t)T + threadIdx.x; i < n; i += (int64_t)gridDim.x * T)
@@ -577,6 +602,21 @@ namespace mpdg
                         unsigned *counter, hipStream_t s)
   {
     dot_kernel<<<kVecBlocks, T, 0, s>>>(a, b, n, partials, out, counter);
+    return hipGetLastError();
+  }
+
+  hipError_t launch_zero_constrained(double *v, int n1x, int n1y, int n1z, unsigned faces,
+                                     hipStream_t s)
+  {
+    zero_constrained_kernel<<<grid_for((int64_t)n1x * n1y * n1z), T, 0, s>>>(v, n1x, n1y, n1z,
+                                                                            faces);
+    return hipGetLastError();
+  }
+
+  hipError_t launch_axpby(double *dst, double a, const double *x, double b, const double *y,
+                          int64_t n, hipStream_t s)
+  {
+    axpby_kernel<<<grid_for(n), T, 0, s>>>(dst, a, x, b, y, n);
     return hipGetLastError();
   }
 

@@ -257,6 +257,23 @@ class JacobiPreconditioner:
         L.check(op._lib.mpdg_heat_jacobi_vmult(op._h, _ptr(dst), _ptr(src), _ptr(self.inverse_diag)))
 
 
+def _solver_params(lib, data, use_diag=None):
+    prm = L.SolverParams()
+    lib.mpdg_solver_params_default(ctypes.byref(prm))
+    prm.solver_type = {"CG": 0, "GMRES": 1}[data.solver_type]
+    if use_diag is None:
+        use_diag = data.preconditioner_type == "Diagonal"
+    prm.preconditioner_type = 3 if use_diag else 0
+    prm.max_iterations = int(data.max_iterations)
+    prm.rel_tolerance = float(data.rel_tolerance)
+    prm.abs_tolerance = float(data.abs_tolerance)
+    prm.mixed_precision = 1 if data.mixed_precision else 0
+    prm.inner_rel_tolerance = float(data.inner_rel_tolerance)
+    prm.check_interval = int(data.check_interval)
+    prm.gmres_restart = int(getattr(data, "gmres_restart", 30))
+    return prm
+
+
 class LinearSolver:
     """linear_solver.hpp:26-134."""
 
@@ -273,18 +290,8 @@ class LinearSolver:
         (source/heat/heat_diffuse_operation.cpp:275-276).
         """
         op = system_matrix
-        prm = L.SolverParams()
-        op._lib.mpdg_solver_params_default(ctypes.byref(prm))
-        prm.solver_type = {"CG": 0, "GMRES": 1}[data.solver_type]
         use_diag = isinstance(preconditioner, JacobiPreconditioner) or data.preconditioner_type == "Diagonal"
-        prm.preconditioner_type = 3 if use_diag else 0
-        prm.max_iterations = int(data.max_iterations)
-        prm.rel_tolerance = float(data.rel_tolerance)
-        prm.abs_tolerance = float(data.abs_tolerance)
-        prm.mixed_precision = 1 if data.mixed_precision else 0
-        prm.inner_rel_tolerance = float(data.inner_rel_tolerance)
-        prm.check_interval = int(data.check_interval)
-        prm.gmres_restart = int(getattr(data, "gmres_restart", 30))
+        prm = _solver_params(op._lib, data, use_diag)
         op._check_vec(solution, "float64")
         op._check_vec(rhs, "float64")
         rep = L.SolverReport()
@@ -299,6 +306,97 @@ class LinearSolver:
             raise NoConvergence(rep.iterations, rep.final_residual, identifier)
         L.check(st)
         return rep.iterations
+
+
+@dataclass
+class NonlinearSolverData:
+    """include/meltpooldg/linear_algebra/nonlinear_solver_data.hpp:10-20 (same defaults)."""
+    max_nonlinear_iterations: int = 10
+    field_correction_tolerance: float = 1e-10
+    residual_tolerance: float = 1e-9
+    max_nonlinear_iterations_alt: int = 0
+    field_correction_tolerance_alt: float = 1e-9
+    residual_tolerance_alt: float = 1e-8
+    use_relative_tolerances: bool = False
+
+
+class NewtonDidNotConverge(RuntimeError):
+    """ExcNewtonDidNotConverge (newton_raphson_solver.cpp:79) /
+    ExcHeatTransferNoConvergence (heat_diffuse_operation.cpp:282-285)."""
+
+
+class HeatDiffuseOperation:
+    """Time stepping of the heat equation on the device
+    (source/heat/heat_diffuse_operation.cpp):
+
+    * solution history T (current), T_old, T_old_old and the previous time
+      increment;
+    * ``init_time_advance(dt)``: reset_time_increment, the linear-extrapolation
+      predictor of the heat defaults (heat_data.cpp:13,
+      predictor_linear.cpp:76-96), homogeneous constraints distributed;
+    * ``solve()``: NewtonRaphsonSolver::solve on the residual create_rhs with
+      the tangent solve (GMRES + Diagonal by default, heat_data.cpp:11-12);
+    * ``l2_norm()``: the norm the reference prints after each Newton solve
+      (compute_norm of the current solution, heat_diffuse_operation.cpp:112-115).
+
+    ``temperature``: initial condition (FP64 device vector, copied)."""
+
+    def __init__(self, operator, temperature, heat_source=None, linear_solver=None,
+                 nlsolve=None):
+        import torch
+        self.op = operator
+        self.temperature = temperature.clone()
+        self.temperature_old = temperature.clone()
+        self.temperature_old_old = temperature.clone()
+        self.heat_source = heat_source
+        self.linear_solver = linear_solver or LinearSolverData(solver_type="GMRES",
+                                                               preconditioner_type="Diagonal")
+        self.nlsolve = nlsolve or NonlinearSolverData()
+        self.dt = 0.0
+        self.dt_old = 0.0
+        self.last_report = None
+        self._torch = torch
+
+    def init_time_advance(self, dt):
+        lib, h = self.op._lib, self.op._h
+        self.temperature_old_old.copy_(self.temperature_old)
+        self.temperature_old.copy_(self.temperature)
+        self.dt_old, self.dt = self.dt, float(dt)
+        self.op.reset_time_increment(self.dt)
+        L.check(lib.mpdg_heat_linear_predictor(h, _ptr(self.temperature), _ptr(self.temperature_old),
+                                               _ptr(self.temperature_old_old), self.dt, self.dt_old))
+        L.check(lib.mpdg_heat_distribute_constraints(h, _ptr(self.temperature)))
+
+    def solve(self):
+        lib, h = self.op._lib, self.op._h
+        n = self.nlsolve
+        np_ = L.NewtonParams(int(n.max_nonlinear_iterations), float(n.field_correction_tolerance),
+                             float(n.residual_tolerance), int(n.max_nonlinear_iterations_alt),
+                             float(n.field_correction_tolerance_alt),
+                             float(n.residual_tolerance_alt), 1 if n.use_relative_tolerances else 0)
+        lp = _solver_params(lib, self.linear_solver)
+        rep = L.NewtonReport()
+        f = self.heat_source
+        st = lib.mpdg_heat_newton_solve(h, _ptr(self.temperature), _ptr(self.temperature_old),
+                                        _ptr(f) if f is not None else None, ctypes.byref(np_),
+                                        ctypes.byref(lp), ctypes.byref(rep))
+        self.last_report = {"iterations": rep.iterations, "linear_iterations": rep.linear_iterations,
+                            "initial_residual": rep.initial_residual, "residual": rep.residual,
+                            "update": rep.update, "seconds": rep.seconds}
+        if st == L.MPDG_ERR_NOT_CONVERGED:
+            raise NewtonDidNotConverge(lib.mpdg_last_error().decode())
+        L.check(st)
+        return self.last_report
+
+    def advance(self, dt):
+        self.init_time_advance(dt)
+        return self.solve()
+
+    def l2_norm(self, v=None):
+        out = ctypes.c_double()
+        L.check(self.op._lib.mpdg_heat_l2_norm(self.op._h, _ptr(self.temperature if v is None else v),
+                                               ctypes.byref(out)))
+        return out.value
 
 
 class HeatBatch:

@@ -140,6 +140,10 @@ class Problem:
         lib().oracle_constrained_mask(ctypes.byref(self.s), m.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)))
         return m.astype(bool)
 
+    def set_dt(self, dt):
+        """OperatorBase::reset_time_increment."""
+        self.s.dt = float(dt)
+
     def random(self, seed):
         v = np.zeros(self.N)
         lib().oracle_random_vector(ctypes.byref(self.s), seed, _dp(v))
