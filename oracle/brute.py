@@ -162,3 +162,60 @@ def pcg_dense(Ac, b, inv_diag, rel_tol, abs_tol=1e-20, max_it=10000):
         p = z + (rz_new / rz) * p
         rz = rz_new
     return x, max_it
+
+
+STEFAN_BOLTZMANN = 5.670374419e-8
+
+
+def _face_cells(n, p, f):
+    """Yield (dofs, (e1, e2)) of every face cell on box face f = 2d + side; dofs
+    in the face-local order a1 + (p+1) a2 over the tangential directions e1 < e2."""
+    n1 = lattice_shape(n, p)
+    d, side = f >> 1, f & 1
+    e1, e2 = [e for e in range(3) if e != d]
+    loc = np.arange(p + 1)
+    for c2 in range(n[e2]):
+        for c1 in range(n[e1]):
+            I = [None, None, None]
+            I[d] = np.full((p + 1, p + 1), (n1[d] - 1) if side else 0)
+            I[e1] = np.broadcast_to(c1 * p + loc[None, :], (p + 1, p + 1))
+            I[e2] = np.broadcast_to(c2 * p + loc[:, None], (p + 1, p + 1))
+            yield (I[0] + n1[0] * (I[1] + n1[1] * I[2])).reshape(-1), (e1, e2)
+
+
+def robin_face_parts(n, p, robin, T_tangent, T_rhs, mask, lo=(0.0, 0.0, 0.0),
+                     hi=(1.0, 1.0, 1.0)):
+    """(A_face, r_face): the dense face tangent linearised at T_tangent (read with
+    the constrained entries as 0) and the face residual at T_rhs (read plain),
+    both without constraints, from explicit 2D face basis matrices on each face
+    cell (convection alpha (T - T_inf), radiation eps sigma (T^4 - T_inf^4))."""
+    n = (n, n, n) if np.isscalar(n) else tuple(n)
+    n1 = lattice_shape(n, p)
+    N = n1[0] * n1[1] * n1[2]
+    h = [(hi[d] - lo[d]) / n[d] for d in range(3)]
+    _, wq, S, _ = _basis_1d(p)
+    Phi = np.einsum("yb,xa->yxba", S, S).reshape((p + 1) ** 2, (p + 1) ** 2)
+    W = np.einsum("y,x->yx", wq, wq).reshape(-1)
+    Tt = np.where(mask, 0.0, T_tangent) if T_tangent is not None else np.zeros(N)
+    A = np.zeros((N, N))
+    r = np.zeros(N)
+    alpha = robin.get("convection_coefficient", 0.0)
+    es = robin.get("emissivity", 0.0) * STEFAN_BOLTZMANN
+    for f in range(6):
+        conv = (robin.get("convection_faces", 0) >> f) & 1
+        rad = (robin.get("radiation_faces", 0) >> f) & 1
+        if not (conv or rad):
+            continue
+        for dofs, (e1, e2) in _face_cells(n, p, f):
+            JxW = W * h[e1] * h[e2]
+            c = np.zeros_like(W)
+            val = np.zeros_like(W)
+            if conv:
+                c += alpha
+                val -= alpha * (Phi @ T_rhs[dofs] - robin["convection_t_inf"])
+            if rad:
+                c += 4.0 * es * (Phi @ Tt[dofs]) ** 3
+                val -= es * ((Phi @ T_rhs[dofs]) ** 4 - robin["radiation_t_inf"] ** 4)
+            A[np.ix_(dofs, dofs)] += Phi.T @ ((JxW * c)[:, None] * Phi)
+            r[dofs] += Phi.T @ (JxW * val)
+    return A, r

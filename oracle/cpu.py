@@ -33,7 +33,16 @@ class OracleProblem(ctypes.Structure):
         ("w0", ctypes.c_double),
         ("dirichlet_faces", ctypes.c_uint),
         ("heaviside", ctypes.POINTER(ctypes.c_double)),
+        ("convection_faces", ctypes.c_uint),
+        ("convection_coefficient", ctypes.c_double),
+        ("convection_t_inf", ctypes.c_double),
+        ("radiation_faces", ctypes.c_uint),
+        ("emissivity", ctypes.c_double),
+        ("radiation_t_inf", ctypes.c_double),
+        ("temperature", ctypes.POINTER(ctypes.c_double)),
     ]
+
+STEFAN_BOLTZMANN = 5.670374419e-8
 
 
 def build():
@@ -99,11 +108,14 @@ def material(heaviside, mat):
 
 
 class Problem:
-    """Python mirror of ``oracle_problem``; keeps the heaviside array alive."""
+    """Python mirror of ``oracle_problem``; keeps the heaviside and temperature
+    arrays alive. ``robin`` (optional dict): convection_faces,
+    convection_coefficient, convection_t_inf, radiation_faces, emissivity,
+    radiation_t_inf; faces are bit masks like ``dirichlet_faces``."""
 
     def __init__(self, n, degree, dt=1e-3, material=None, k_amplitude=0.0, velocity=False,
                  w0=0.2, heaviside=None, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0),
-                 dirichlet_faces=0x3F):
+                 dirichlet_faces=0x3F, robin=None, temperature=None):
         mat = material or {"type": "gas", "gas_rho": 1.0, "gas_cp": 1.0, "gas_k": 1.0}
         n = (n, n, n) if np.isscalar(n) else tuple(n)
         s = OracleProblem()
@@ -125,6 +137,14 @@ class Problem:
         s.velocity = 1 if velocity else 0
         s.w0 = w0
         s.dirichlet_faces = dirichlet_faces
+        rb = robin or {}
+        s.convection_faces = int(rb.get("convection_faces", 0))
+        s.convection_coefficient = float(rb.get("convection_coefficient", 0.0))
+        s.convection_t_inf = float(rb.get("convection_t_inf", 0.0))
+        s.radiation_faces = int(rb.get("radiation_faces", 0))
+        s.emissivity = float(rb.get("emissivity", 0.0))
+        s.radiation_t_inf = float(rb.get("radiation_t_inf", 0.0))
+        self._T = None
         self._h = None
         if heaviside is not None:
             self._h = np.ascontiguousarray(heaviside, dtype=np.float64)
@@ -134,6 +154,15 @@ class Problem:
         self.degree = degree
         self.N = int(lib().oracle_n_dofs(ctypes.byref(s)))
         assert self.N > 0, "invalid oracle problem"
+        if temperature is not None:
+            self.set_temperature(temperature)
+
+    def set_temperature(self, temperature):
+        """The temperature the tangent is linearised at (the operator's
+        reference to the current solution)."""
+        self._T = np.array(temperature, dtype=np.float64)
+        assert self._T.shape == (self.N,)
+        self.s.temperature = _dp(self._T)
 
     def mask(self):
         m = np.zeros(self.N, dtype=np.uint8)

@@ -382,6 +382,11 @@ namespace
       }
   }
 
+  void robin_vmult(const oracle_problem *pb, const Tables &t, const Mesh &m, const double *src,
+                   double *dst);
+
+  // HeatDiffuseMultiPhaseOperator::vmult (heat_diffuse_operator.cpp:179-197):
+  // the cell loop, then the boundary-face loop of the Robin faces.
   void vmult_impl(const oracle_problem *pb, const Tables &t, const Mesh &m, const double *src,
                   double *dst)
   {
@@ -413,6 +418,7 @@ namespace
                 dst[node_index(m, g, a, b, c)] += y[(c * n + b) * n + a];
             }
     });
+    robin_vmult(pb, t, m, src, dst);
   }
 
   // rhs_cell_loop restated (source/heat/heat_diffuse_operator.cpp:393-529) for
@@ -518,6 +524,171 @@ namespace
     apply_dir<true>(t.S, n, 0, a2, a1);
     for (int i = 0; i < n3; ++i)
       y[i] = acc[i] + a1[i];
+  }
+
+  // ---- Robin boundary faces ------------------------------------------------
+  // Box face f = 2d + side (deal.II's colorized boundary id of hyper_rectangle):
+  // normal direction d, tangential directions e1 < e2.
+  struct BoxFace
+  {
+    int f, d, side, e1, e2;
+  };
+
+  BoxFace box_face(int f)
+  {
+    BoxFace b;
+    b.f = f;
+    b.d = f >> 1;
+    b.side = f & 1;
+    b.e1 = b.d == 0 ? 1 : 0;
+    b.e2 = b.d == 2 ? 1 : 2;
+    return b;
+  }
+
+  unsigned robin_faces(const oracle_problem *pb)
+  {
+    return (pb->convection_faces | pb->radiation_faces) & 0x3Fu;
+  }
+
+  // Lattice coordinates of local node (a1, a2) of face cell (c1, c2) on face b.
+  inline void face_node(const Mesh &m, const BoxFace &b, int c1, int c2, int a1, int a2,
+                        int64_t I[3])
+  {
+    I[b.d] = b.side ? m.N1[b.d] - 1 : 0;
+    I[b.e1] = (int64_t)c1 * m.p + a1;
+    I[b.e2] = (int64_t)c2 * m.p + a2;
+  }
+
+  enum FaceMode
+  {
+    kFaceTangent = 0,
+    kFaceRhs = 1
+  };
+
+  // One face cell: face values of T (and of the increment u for the tangent) at
+  // the (p+1)^2 Gauss points, the quadrature-point physics, and the integration
+  // against the face shape functions (FEFaceEvaluation evaluate/integrate of
+  // values). Node order a1 + n a2.
+  //   tangent (heat_diffuse_operator.cpp:1101-1125):
+  //     val = alpha u + 4 eps sigma T^3 u
+  //   rhs (heat_diffuse_operator.cpp:748-776):
+  //     val = -eps sigma (T^4 - T_inf^4) - alpha (T - T_inf)
+  void face_local(const oracle_problem *pb, const Tables &t, const Mesh &m, const BoxFace &b,
+                  int mode, const double *Tl, const double *ul, double *y)
+  {
+    const int n = t.n;
+    const bool conv = (pb->convection_faces >> b.f) & 1u;
+    const bool rad = (pb->radiation_faces >> b.f) & 1u;
+    const double area = m.h[b.e1] * m.h[b.e2];
+    double V[MAXN][MAXN];
+    for (int q2 = 0; q2 < n; ++q2)
+      for (int q1 = 0; q1 < n; ++q1)
+        {
+          double T = 0.0, u = 0.0;
+          for (int a2 = 0; a2 < n; ++a2)
+            for (int a1 = 0; a1 < n; ++a1)
+              {
+                const double phi = t.S[q1][a1] * t.S[q2][a2];
+                T += phi * Tl[a2 * n + a1];
+                if (mode == kFaceTangent)
+                  u += phi * ul[a2 * n + a1];
+              }
+          double val = 0.0;
+          if (mode == kFaceTangent)
+            {
+              if (conv)
+                val += pb->convection_coefficient * u;
+              if (rad)
+                val += 4. * pb->emissivity * ORACLE_STEFAN_BOLTZMANN * (T * T * T) * u;
+            }
+          else
+            {
+              const double Ti = pb->radiation_t_inf;
+              if (rad)
+                val -= pb->emissivity * ORACLE_STEFAN_BOLTZMANN *
+                       ((T * T) * (T * T) - (Ti * Ti) * (Ti * Ti));
+              if (conv)
+                val -= pb->convection_coefficient * (T - pb->convection_t_inf);
+            }
+          V[q1][q2] = val * t.W[q1] * t.W[q2] * area;
+        }
+    for (int a2 = 0; a2 < n; ++a2)
+      for (int a1 = 0; a1 < n; ++a1)
+        {
+          double s = 0.0;
+          for (int q2 = 0; q2 < n; ++q2)
+            for (int q1 = 0; q1 < n; ++q1)
+              s += t.S[q1][a1] * t.S[q2][a2] * V[q1][q2];
+          y[a2 * n + a1] = s;
+        }
+  }
+
+  // The boundary-face part of MatrixFree::loop: every face cell of every Robin
+  // face, faces ascending, face cells lexicographic; body(b, c1, c2).
+  template <typename Body>
+  void for_robin_face_cells(const oracle_problem *pb, const Mesh &m, Body body)
+  {
+    const unsigned mask = robin_faces(pb);
+    for (int f = 0; f < 6; ++f)
+      {
+        if (!((mask >> f) & 1u))
+          continue;
+        const BoxFace b = box_face(f);
+        for (int c2 = 0; c2 < m.n[b.e2]; ++c2)
+          for (int c1 = 0; c1 < m.n[b.e1]; ++c1)
+            body(b, c1, c2);
+      }
+  }
+
+  // Gathers T (constraint-resolved: Dirichlet entries read 0, or plain) and u
+  // (constraint-resolved) on one face cell.
+  void face_gather(const oracle_problem *pb, const Mesh &m, const BoxFace &b, int c1, int c2,
+                   const double *T, bool T_plain, const double *u, double *Tl, double *ul)
+  {
+    const int n = m.p + 1;
+    for (int a2 = 0; a2 < n; ++a2)
+      for (int a1 = 0; a1 < n; ++a1)
+        {
+          int64_t I[3];
+          face_node(m, b, c1, c2, a1, a2, I);
+          const bool con = is_constrained(pb, m, I[0], I[1], I[2]);
+          const int64_t g = I[0] + m.N1[0] * (I[1] + m.N1[1] * I[2]);
+          Tl[a2 * n + a1] = T ? ((con && !T_plain) ? 0.0 : T[g]) : 0.0;
+          if (ul)
+            ul[a2 * n + a1] = con ? 0.0 : u[g];
+        }
+  }
+
+  // distribute_local_to_global of a face cell: constrained rows dropped.
+  void face_scatter(const oracle_problem *pb, const Mesh &m, const BoxFace &b, int c1, int c2,
+                    const double *y, double *dst)
+  {
+    const int n = m.p + 1;
+    for (int a2 = 0; a2 < n; ++a2)
+      for (int a1 = 0; a1 < n; ++a1)
+        {
+          int64_t I[3];
+          face_node(m, b, c1, c2, a1, a2, I);
+          if (!is_constrained(pb, m, I[0], I[1], I[2]))
+            dst[I[0] + m.N1[0] * (I[1] + m.N1[1] * I[2])] += y[a2 * n + a1];
+        }
+  }
+
+  // tangent_boundary_loop (heat_diffuse_operator.cpp:243-269): dst += A_face src.
+  void robin_vmult(const oracle_problem *pb, const Tables &t, const Mesh &m, const double *src,
+                   double *dst)
+  {
+    for_robin_face_cells(pb, m, [&](const BoxFace &b, int c1, int c2) {
+      double Tl[MAXN * MAXN], ul[MAXN * MAXN], y[MAXN * MAXN];
+      face_gather(pb, m, b, c1, c2, pb->temperature, false, src, Tl, ul);
+      face_local(pb, t, m, b, kFaceTangent, Tl, ul, y);
+      face_scatter(pb, m, b, c1, c2, y, dst);
+    });
+  }
+
+  bool robin_ready(const oracle_problem *pb)
+  {
+    return !(pb->radiation_faces & 0x3Fu) || pb->temperature;
   }
 
   double dot(const std::vector<double> &a, const std::vector<double> &b)
@@ -650,7 +821,7 @@ int oracle_material(int material, int sharp, int volume_specific, int consistent
 
 int oracle_vmult(const oracle_problem *pb, const double *src, double *dst)
 {
-  if (!valid(pb))
+  if (!valid(pb) || !robin_ready(pb))
     return -1;
   const Tables t = make_tables(pb->degree);
   const Mesh m = make_mesh(pb);
@@ -660,8 +831,9 @@ int oracle_vmult(const oracle_problem *pb, const double *src, double *dst)
 
 // create_rhs restated (heat_diffuse_operator.cpp:783-806): MatrixFree::loop over
 // the cells with zero_dst = false, so dst accumulates; integrate_scatter through
-// the Dirichlet-constrained DoF handler drops constrained rows. No Robin faces
-// and no cut-cell evaporation term. heat_source may be NULL (f = 0).
+// the Dirichlet-constrained DoF handler drops constrained rows; then the Robin
+// faces. No Neumann faces and no cut-cell evaporation term. heat_source may be
+// NULL (f = 0).
 int oracle_create_rhs(const oracle_problem *pb, const double *temperature,
                       const double *temperature_old, const double *heat_source, double *dst)
 {
@@ -695,6 +867,13 @@ int oracle_create_rhs(const oracle_problem *pb, const double *temperature,
             if (!con)
               dst[node_index(m, g, a, b, c)] += y[(c * n + b) * n + a];
           }
+  });
+  // rhs_boundary_loop (heat_diffuse_operator.cpp:719-781): T read plain
+  for_robin_face_cells(pb, m, [&](const BoxFace &b, int c1, int c2) {
+    double Tl[MAXN * MAXN], y[MAXN * MAXN];
+    face_gather(pb, m, b, c1, c2, temperature, true, nullptr, Tl, nullptr);
+    face_local(pb, t, m, b, kFaceRhs, Tl, nullptr, y);
+    face_scatter(pb, m, b, c1, c2, y, dst);
   });
   return 0;
 }
@@ -746,11 +925,12 @@ int oracle_l2_norm(const oracle_problem *pb, const double *u, double *norm)
   return 0;
 }
 
-// MatrixFreeTools::compute_diagonal restated: apply the cell operator to each
-// local unit vector and keep the matching entry (heat_diffuse_operator.cpp:332-358).
+// MatrixFreeTools::compute_diagonal restated: apply the cell operator (and the
+// Robin face operator) to each local unit vector and keep the matching entry
+// (heat_diffuse_operator.cpp:332-358).
 int oracle_diagonal(const oracle_problem *pb, double *diag)
 {
-  if (!valid(pb))
+  if (!valid(pb) || !robin_ready(pb))
     return -1;
   const Tables t = make_tables(pb->degree);
   const Mesh m = make_mesh(pb);
@@ -781,6 +961,21 @@ int oracle_diagonal(const oracle_problem *pb, double *diag)
               diag[node_index(m, g, a, b, c)] += d[(c * n + b) * n + a];
           }
   });
+  // boundary operation of compute_diagonal (heat_diffuse_operator.cpp:352-356):
+  // each face-local unit vector through tangent_local_boundary_operation
+  const int n2 = n * n;
+  for_robin_face_cells(pb, m, [&](const BoxFace &b, int c1, int c2) {
+    double Tl[MAXN * MAXN], e[MAXN * MAXN], y[MAXN * MAXN], dl[MAXN * MAXN];
+    face_gather(pb, m, b, c1, c2, pb->temperature, false, nullptr, Tl, nullptr);
+    for (int j = 0; j < n2; ++j)
+      {
+        std::fill(e, e + n2, 0.0);
+        e[j] = 1.0;
+        face_local(pb, t, m, b, kFaceTangent, Tl, e, y);
+        dl[j] = y[j];
+      }
+    face_scatter(pb, m, b, c1, c2, dl, diag);
+  });
   return 0;
 }
 
@@ -805,7 +1000,7 @@ int oracle_solve(const oracle_problem *pb, int solver, int restart, const double
                  int *iterations, double *initial_res, double *final_res, double *history,
                  int history_len)
 {
-  if (!valid(pb) || (solver == 1 && restart < 1))
+  if (!valid(pb) || !robin_ready(pb) || (solver == 1 && restart < 1))
     return -1;
   const Tables t = make_tables(pb->degree);
   const Mesh m = make_mesh(pb);

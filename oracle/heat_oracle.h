@@ -18,6 +18,11 @@
  *     (heat_diffuse_operator.cpp:271-284, 297-391);
  *   - homogeneous Dirichlet semantics on the marked box faces
  *     (include/meltpooldg/utilities/constraints.hpp:49-108);
+ *   - convective and radiative (Robin) boundary faces: tangent_boundary_loop /
+ *     tangent_local_boundary_operation (heat_diffuse_operator.cpp:243-269,
+ *     1088-1129), the boundary lambda of the diagonal (:352-356) and
+ *     rhs_boundary_loop (:719-781), with the Stefan-Boltzmann constant of
+ *     include/meltpooldg/utilities/physical_constants.hpp:6;
  *   - LinearSolver::solve with CG or right-preconditioned GMRES and the
  *     ReductionControl stopping rule (include/meltpooldg/linear_algebra/
  *     linear_solver.hpp:26-134) and JacobiPreconditioner::vmult
@@ -58,7 +63,18 @@ typedef struct {
   double w0;
   unsigned dirichlet_faces; /* bit f: face f = 2*d + side is Dirichlet */
   const double *heaviside;  /* lattice vector, used when material == 1 */
+  /* Robin faces (bit f as for dirichlet_faces): convection alpha (T - T_inf) and
+   * radiation emissivity sigma (T^4 - T_inf^4) */
+  unsigned convection_faces;
+  double convection_coefficient, convection_t_inf;
+  unsigned radiation_faces;
+  double emissivity, radiation_t_inf;
+  /* temperature the tangent is linearised at (lattice vector, read with the
+   * Dirichlet entries resolved to 0); needed when radiation_faces != 0 */
+  const double *temperature;
 } oracle_problem;
+
+#define ORACLE_STEFAN_BOLTZMANN 5.670374419e-8
 
 int oracle_tables(int p, double *gll, double *gauss_pts, double *gauss_w, double *S, double *D);
 int64_t oracle_n_dofs(const oracle_problem *pb);

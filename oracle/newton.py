@@ -11,6 +11,10 @@ infrastructure only).
   LinearSolver::solve with the heat operator and the Jacobi preconditioner
   from a zero initial update, the update is applied only when not converged,
   constraints of the update are homogeneous.
+* The operator reads the current solution as its linearisation point (the
+  radiation tangent, heat_diffuse_operator.cpp:1096-1099), and the Jacobi
+  preconditioner is updated once per time step, at the start value
+  (HeatDiffuseOperation::solve, source/heat/heat_diffuse_operation.cpp:251-253).
 * NonlinearSolverData defaults (include/meltpooldg/linear_algebra/
   nonlinear_solver_data.hpp:13-19).
 """
@@ -43,6 +47,7 @@ def newton_time_step(pb, T, T_old, heat_source=None, data=None, solver="GMRES", 
     """Returns (T_new, report): T is the start value (the predictor)."""
     data = data or NewtonData()
     T = np.array(T, dtype=float)
+    pb.set_temperature(T)
     inv_diag = pb.inverse_diagonal()
     res_tol, corr_tol = data.residual_tolerance, data.field_correction_tolerance
     max_total = data.max_nonlinear_iterations + data.max_nonlinear_iterations_alt
@@ -51,6 +56,7 @@ def newton_time_step(pb, T, T_old, heat_source=None, data=None, solver="GMRES", 
     for i in range(max(max_total, 1)):
         if i == data.max_nonlinear_iterations:
             res_tol, corr_tol = data.residual_tolerance_alt, data.field_correction_tolerance_alt
+        pb.set_temperature(T)
         rhs = pb.create_rhs(T, T_old, heat_source)
         sol = pb.solve(rhs, solver=solver, rel_tol=rel_tol, abs_tol=abs_tol, max_it=max_it,
                        inv_diag=inv_diag, restart=restart)
