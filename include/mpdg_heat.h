@@ -80,6 +80,18 @@ typedef struct {
   int velocity;                /* mpdg_velocity_field */
   double velocity_w0;
   unsigned dirichlet_faces;    /* bit 2d: face x_d = lo, bit 2d+1: x_d = hi; 0x3F = all */
+  /* Robin faces, bits as for dirichlet_faces (the colorized boundary ids of the
+   * box): convection alpha (T - T_inf) and radiation emissivity sigma
+   * (T^4 - T_inf^4), sigma = 5.670374419e-8 (HeatData::convection / ::radiation,
+   * source/heat/heat_data.cpp:55-73; tangent_local_boundary_operation and
+   * rhs_boundary_loop, source/heat/heat_diffuse_operator.cpp:719-781,
+   * 1088-1129). All zero: no Robin face. */
+  unsigned convection_faces;
+  double convection_coefficient;
+  double convection_temperature_infinity;
+  unsigned radiation_faces;
+  double emissivity;
+  double radiation_temperature_infinity;
   void *stream;                /* hipStream_t the handle launches on; NULL = default stream */
 } mpdg_heat_desc;
 
@@ -124,11 +136,23 @@ mpdg_status mpdg_heat_set_dt(mpdg_heat *h, double dt);
  * The engine copies it (and converts to the cell-loop precision). */
 mpdg_status mpdg_heat_set_heaviside(mpdg_heat *h, const double *heaviside);
 
-/* HeatDiffuseMultiPhaseOperator::vmult (heat_diffuse_operator.cpp:179-197), FP64 vectors. */
+/* The temperature the tangent is linearised at: the operator's reference to the
+ * current solution (constructor argument temperature_in,
+ * include/meltpooldg/heat/heat_diffuse_operator.hpp:128,186), which
+ * tangent_local_boundary_operation reads with the Dirichlet entries resolved
+ * to 0 (heat_diffuse_operator.cpp:1096-1099). Device pointer of n_dofs()
+ * doubles, kept (not copied) and read by every later vmult, inverse diagonal
+ * and solve; required when radiation_faces != 0, NULL clears it.
+ * mpdg_heat_newton_solve points it at its temperature argument. */
+mpdg_status mpdg_heat_set_temperature(mpdg_heat *h, const double *temperature);
+
+/* HeatDiffuseMultiPhaseOperator::vmult (heat_diffuse_operator.cpp:179-197), FP64 vectors:
+ * the cell loop, then the Robin faces (tangent_boundary_loop :243-269). */
 mpdg_status mpdg_heat_vmult(mpdg_heat *h, double *dst, const double *src);
 /* Same operator with FP32 vectors and FP32 cell loop (inner operator of the mixed solve). */
 mpdg_status mpdg_heat_vmult_f32(mpdg_heat *h, float *dst, const float *src);
-/* compute_inverse_diagonal_from_matrixfree (heat_diffuse_operator.cpp:271-284), FP64. */
+/* compute_inverse_diagonal_from_matrixfree (heat_diffuse_operator.cpp:271-284), FP64,
+ * with the Robin faces' part (the boundary operation, :352-356). */
 mpdg_status mpdg_heat_inverse_diagonal(mpdg_heat *h, double *inv_diag);
 /* HeatDiffuseMultiPhaseOperator::create_rhs(dst, src = temperature_old)
  * (heat_diffuse_operator.cpp:783-806, rhs_cell_loop :393-529), FP64:
@@ -137,8 +161,10 @@ mpdg_status mpdg_heat_inverse_diagonal(mpdg_heat *h, double *inv_diag);
  * the heat source f are read plain (constrained entries as stored), as the
  * reference's read_dof_values_plain does. The reference operator holds T and
  * f by reference from its constructor (heat_diffuse_operator.hpp:186-188);
- * here they are arguments. heat_source may be NULL (f = 0). No Robin faces and
- * no evaporative cooling term. */
+ * here they are arguments. heat_source may be NULL (f = 0). The Robin faces add
+ * -(alpha (T - T_inf) + eps sigma (T^4 - T_inf^4)) integrated over the faces
+ * (rhs_boundary_loop :719-781), T read plain. No Neumann faces and no
+ * evaporative cooling term. */
 mpdg_status mpdg_heat_create_rhs(mpdg_heat *h, double *dst, const double *temperature,
                                  const double *temperature_old, const double *heat_source);
 /* JacobiPreconditioner::vmult (preconditioner_jacobi.hpp:51-56): dst = inv_diag .* src. */
@@ -164,7 +190,8 @@ const char *mpdg_last_error(void);
  * members are solved together: one batched vmult launch per degree and colour,
  * segmented vector kernels, per-solve convergence flags, and the iteration
  * chunk captured as one hipGraph and replayed. Members must share the material
- * kind, conductivity field and velocity field; meshes and degrees may differ. */
+ * kind, conductivity field and velocity field; meshes and degrees may differ.
+ * Members with Robin faces are rejected (MPDG_ERR_UNSUPPORTED). */
 typedef struct mpdg_heat_batch mpdg_heat_batch;
 typedef struct {
   int status;               /* 0 converged, 1 not converged */
@@ -212,7 +239,10 @@ typedef struct {
  * (newton_raphson_solver.cpp:22-80, 118-175) and the heat operation's callbacks
  * (heat_diffuse_operation.cpp:82-116). On entry `temperature` holds the start
  * value (the predictor, Dirichlet values set); on return the time step's
- * solution. Per increment: rhs = create_rhs(T, T_old, f); J dT = rhs solved by
+ * solution. The operator's linearisation temperature (mpdg_heat_set_temperature)
+ * is `temperature` itself, and the Jacobi preconditioner is computed once, at
+ * the start value (preconditioner.update() before newton.solve,
+ * heat_diffuse_operation.cpp:275-276). Per increment: rhs = create_rhs(T, T_old, f); J dT = rhs solved by
  * mpdg_heat_solve with `linear` from dT = 0; converged when ||rhs|| <
  * residual_tolerance and ||dT|| < field_correction_tolerance (that last dT is
  * not applied), otherwise T += dT (dT is zero on constrained DoFs). Returns

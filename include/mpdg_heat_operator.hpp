@@ -68,6 +68,13 @@ namespace mpdg
     void reset_time_increment(double dt) { check(mpdg_heat_set_dt(h_, dt)); }
     // level_set_as_heaviside registration (gas_liquid material)
     void set_heaviside(const DeviceVector &H) { check(mpdg_heat_set_heaviside(h_, H.data)); }
+    // the operator's reference to the current solution (heat_diffuse_operator.hpp:128):
+    // the linearisation point of the radiation tangent; the pointer is kept
+    void set_temperature(const DeviceVector &T)
+    {
+      check_size(T);
+      check(mpdg_heat_set_temperature(h_, T.data));
+    }
     std::size_t n_dofs() const { return (std::size_t)mpdg_heat_n_dofs(h_); }
 
     // OperatorMatrixFree::vmult

@@ -4,10 +4,10 @@ The compute path is libmpdg_heat.so (HIP kernels for gfx950 behind the C ABI
 of include/mpdg_heat.h). This package is the host-side mirror of MeltPoolDG's
 heat operator / Jacobi / LinearSolver interface on top of it.
 """
-from .heat import (HeatBatch, HeatDiffuseOperation, HeatDiffuseOperator, JacobiPreconditioner, LinearSolver,
-                   LinearSolverData, MaterialData, NewtonDidNotConverge, NoConvergence, NonlinearSolverData,
-                   PhaseData, ZeroTimeIncrement)
+from .heat import (ConvectionBC, HeatBatch, HeatDiffuseOperation, HeatDiffuseOperator, JacobiPreconditioner,
+                   LinearSolver, LinearSolverData, MaterialData, NewtonDidNotConverge, NoConvergence,
+                   NonlinearSolverData, PhaseData, RadiationBC, ZeroTimeIncrement)
 
-__all__ = ["HeatBatch", "HeatDiffuseOperation", "HeatDiffuseOperator", "JacobiPreconditioner", "LinearSolver",
-           "LinearSolverData", "MaterialData", "NewtonDidNotConverge", "NoConvergence", "NonlinearSolverData",
-           "PhaseData", "ZeroTimeIncrement"]
+__all__ = ["ConvectionBC", "HeatBatch", "HeatDiffuseOperation", "HeatDiffuseOperator", "JacobiPreconditioner",
+           "LinearSolver", "LinearSolverData", "MaterialData", "NewtonDidNotConverge", "NoConvergence",
+           "NonlinearSolverData", "PhaseData", "RadiationBC", "ZeroTimeIncrement"]

@@ -42,6 +42,12 @@ class HeatDesc(ctypes.Structure):
         ("velocity", ctypes.c_int),
         ("velocity_w0", ctypes.c_double),
         ("dirichlet_faces", ctypes.c_uint),
+        ("convection_faces", ctypes.c_uint),
+        ("convection_coefficient", ctypes.c_double),
+        ("convection_temperature_infinity", ctypes.c_double),
+        ("radiation_faces", ctypes.c_uint),
+        ("emissivity", ctypes.c_double),
+        ("radiation_temperature_infinity", ctypes.c_double),
         ("stream", ctypes.c_void_p),
     ]
 
@@ -123,6 +129,7 @@ SIGNATURES = {
     "mpdg_heat_n_free_dofs": (ctypes.c_int64, [_VP]),
     "mpdg_heat_set_dt": (ctypes.c_int, [_VP, ctypes.c_double]),
     "mpdg_heat_set_heaviside": (ctypes.c_int, [_VP, _VP]),
+    "mpdg_heat_set_temperature": (ctypes.c_int, [_VP, _VP]),
     "mpdg_heat_vmult": (ctypes.c_int, [_VP, _VP, _VP]),
     "mpdg_heat_vmult_f32": (ctypes.c_int, [_VP, _VP, _VP]),
     "mpdg_heat_inverse_diagonal": (ctypes.c_int, [_VP, _VP]),

@@ -89,6 +89,11 @@ struct mpdg_heat
   DevBuf<float> heav32;
   bool has_heaviside = false;
 
+  // Robin faces and the temperature the tangent is linearised at (a caller's
+  // device vector, not a copy: the reference operator holds a reference)
+  RobinParams robin{};
+  const double *temperature = nullptr;
+
   // solver workspace
   DevBuf<double> r, p, q, inv_diag, partials, history;
   DevBuf<float> r32, p32, q32, x32, inv32;
@@ -176,6 +181,39 @@ namespace
              precision == MPDG_FP32));
   }
 
+  // Robin faces of a descriptor: faces ascending, each face's node lattice a
+  // contiguous thread range (tangential directions e1 < e2, e1 fastest).
+  RobinParams robin_params(const mpdg_heat_desc &d, const double *h)
+  {
+    constexpr double kStefanBoltzmann = 5.670374419e-8; // physical_constants.hpp:6
+    RobinParams rp{};
+    rp.conv = d.convection_faces & 0x3Fu;
+    rp.rad = d.radiation_faces & 0x3Fu;
+    rp.robin = rp.conv | rp.rad;
+    rp.dirichlet = d.dirichlet_faces & 0x3Fu;
+    for (int k = 0; k < 3; ++k)
+      {
+        rp.n[k] = d.n_cells[k];
+        rp.n1[k] = d.n_cells[k] * d.degree + 1;
+        rp.h[k] = h[k];
+      }
+    rp.alpha = d.convection_coefficient;
+    rp.t_inf_conv = d.convection_temperature_infinity;
+    rp.eps_sigma = d.emissivity * kStefanBoltzmann;
+    rp.t_inf_rad = d.radiation_temperature_infinity;
+    rp.node_off[0] = 0;
+    for (int f = 0; f < 6; ++f)
+      if ((rp.robin >> f) & 1u)
+        {
+          const int dd = f >> 1;
+          const int e1 = dd == 0 ? 1 : 0, e2 = dd == 2 ? 1 : 2;
+          rp.face[rp.nfaces] = f;
+          rp.node_off[rp.nfaces + 1] = rp.node_off[rp.nfaces] + (int64_t)rp.n1[e1] * rp.n1[e2];
+          ++rp.nfaces;
+        }
+    return rp;
+  }
+
   template <typename Number>
   void fill_params(mpdg_heat *H, CellParams<Number> &prm, const Number *tab)
   {
@@ -257,6 +295,15 @@ namespace
     return MPDG_OK;
   }
 
+  // The tangent of radiating faces depends on the registered temperature.
+  mpdg_status check_tangent(const mpdg_heat *H)
+  {
+    if (H->robin.rad && !H->temperature)
+      return fail(MPDG_ERR_ARGUMENT,
+                  "radiation faces need the linearisation temperature: mpdg_heat_set_temperature");
+    return MPDG_OK;
+  }
+
   mpdg_status ensure_workspace(mpdg_heat *H, bool mixed)
   {
     const ColourGrid cg = colour_grid(H->desc.n_cells[0], H->desc.n_cells[1], H->desc.n_cells[2],
@@ -265,7 +312,8 @@ namespace
                                         H->desc.n_cells[2], cells_per_block(H->desc.degree, true));
     const size_t vmult_slots = 8 * (size_t)kMaxResidentPerCu * device_cu_count();
     const size_t np = std::max<size_t>({(size_t)cg.total, (size_t)cg32.total, vmult_slots,
-                                        (size_t)kMaxBasis * kVecBlocks});
+                                        (size_t)kMaxBasis * kVecBlocks,
+                                        (size_t)robin_blocks(H->robin)});
     MPDG_HIP(H->partials.alloc(np));
     MPDG_HIP(H->r.alloc(H->N));
     MPDG_HIP(H->p.alloc(H->N));
@@ -347,26 +395,41 @@ namespace
   }
 
 
+  // The cell loop, then the Robin faces (MatrixFree::loop's boundary-face part).
+  // With scal, the cell loop leaves p.Ap of the cells in scal->pap and the face
+  // launch adds the faces' part and sets alpha.
   mpdg_status vmult64(mpdg_heat *H, double *dst, const double *src, double *partials,
                       SolverScalars *scal)
   {
+    if (mpdg_status s = check_tangent(H))
+      return s;
     MPDG_HIP(launch_vmult<double>(H->desc.degree, coef_mode(H), H->desc.velocity != 0, H->prm64,
                                   H->vt64, src, dst, H->heav64.ptr, partials, scal, H->stream));
+    MPDG_HIP(launch_robin<double>(H->desc.degree, kRobinTangent, H->robin, dst, src,
+                                  H->temperature, partials, scal, H->stream));
     return MPDG_OK;
   }
 
   mpdg_status vmult32(mpdg_heat *H, float *dst, const float *src, double *partials,
                       SolverScalars *scal)
   {
+    if (mpdg_status s = check_tangent(H))
+      return s;
     MPDG_HIP(launch_vmult<float>(H->desc.degree, coef_mode(H), H->desc.velocity != 0, H->prm32,
                                  H->vt32, src, dst, H->heav32.ptr, partials, scal, H->stream));
+    MPDG_HIP(launch_robin<float>(H->desc.degree, kRobinTangent, H->robin, dst, src,
+                                 H->temperature, partials, scal, H->stream));
     return MPDG_OK;
   }
 
   mpdg_status diagonal_inverse(mpdg_heat *H, double *inv)
   {
+    if (mpdg_status s = check_tangent(H))
+      return s;
     MPDG_HIP(launch_diagonal<double>(H->desc.degree, coef_mode(H), H->desc.velocity != 0,
                                      H->prm64, H->dt64, inv, H->heav64.ptr, H->stream));
+    MPDG_HIP(launch_robin<double>(H->desc.degree, kRobinDiagonal, H->robin, inv, nullptr,
+                                  H->temperature, nullptr, nullptr, H->stream));
     MPDG_HIP(launch_invert_diagonal(inv, H->N, H->stream));
     return MPDG_OK;
   }
@@ -608,6 +671,12 @@ mpdg_status mpdg_heat_create(const mpdg_heat_desc *desc, mpdg_heat **out)
     return fail(MPDG_ERR_UNSUPPORTED, "conductivity field not supported");
   if (d.material == MPDG_MATERIAL_GAS_LIQUID && d.conductivity_field != MPDG_CONDUCTIVITY_MATERIAL)
     return fail(MPDG_ERR_UNSUPPORTED, "separable conductivity applies to the gas material only");
+  if ((d.dirichlet_faces | d.convection_faces | d.radiation_faces) & ~0x3Fu)
+    return fail(MPDG_ERR_ARGUMENT, "face masks have 6 bits (faces 2d + side)");
+  if (!std::isfinite(d.convection_coefficient) || !std::isfinite(d.emissivity) ||
+      !std::isfinite(d.convection_temperature_infinity) ||
+      !std::isfinite(d.radiation_temperature_infinity))
+    return fail(MPDG_ERR_ARGUMENT, "Robin face parameters must be finite");
 
   mpdg_heat *H = new (std::nothrow) mpdg_heat();
   if (!H)
@@ -695,6 +764,7 @@ mpdg_status mpdg_heat_create(const mpdg_heat_desc *desc, mpdg_heat **out)
     }
   fill_params<double>(H, H->prm64, H->sin_tab64.ptr);
   fill_params<float>(H, H->prm32, H->sin_tab32.ptr);
+  H->robin = robin_params(d, H->h);
   *out = H;
   return MPDG_OK;
 }
@@ -740,6 +810,14 @@ mpdg_status mpdg_heat_set_heaviside(mpdg_heat *h, const double *heaviside)
   return MPDG_OK;
 }
 
+mpdg_status mpdg_heat_set_temperature(mpdg_heat *h, const double *temperature)
+{
+  if (!h)
+    return fail(MPDG_ERR_ARGUMENT, "null handle");
+  h->temperature = temperature;
+  return MPDG_OK;
+}
+
 mpdg_status mpdg_heat_vmult(mpdg_heat *h, double *dst, const double *src)
 {
   if (mpdg_status s = check_ready(h))
@@ -780,6 +858,9 @@ mpdg_status mpdg_heat_create_rhs(mpdg_heat *h, double *dst, const double *temper
     return fail(MPDG_ERR_ARGUMENT, "null vector");
   MPDG_HIP(launch_create_rhs(h->desc.degree, coef_mode(h), h->desc.velocity != 0, h->prm64, dst,
                              temperature, temperature_old, heat_source, h->heav64.ptr, h->stream));
+  // rhs_boundary_loop: the Robin faces with T read plain
+  MPDG_HIP(launch_robin<double>(h->desc.degree, kRobinRhs, h->robin, dst, nullptr, temperature,
+                                nullptr, nullptr, h->stream));
   return MPDG_OK;
 }
 
@@ -837,6 +918,13 @@ mpdg_status mpdg_heat_linear_predictor(mpdg_heat *h, double *predictor, const do
   return MPDG_OK;
 }
 
+// LinearSolver::solve; with update_preconditioner the Jacobi preconditioner is
+// updated first (JacobiPreconditioner::update), otherwise the handle's current
+// inverse diagonal is used as it is.
+static mpdg_status solve_impl(mpdg_heat *h, double *x, const double *b,
+                              const mpdg_solver_params *params, mpdg_solver_report *report,
+                              bool update_preconditioner);
+
 mpdg_status mpdg_heat_newton_solve(mpdg_heat *h, double *temperature,
                                    const double *temperature_old, const double *heat_source,
                                    const mpdg_newton_params *newton,
@@ -864,7 +952,20 @@ mpdg_status mpdg_heat_newton_solve(mpdg_heat *h, double *temperature,
     return fail(MPDG_ERR_ARGUMENT, "negative Newton iteration limit");
   if (mpdg_status s = ensure_newton(h))
     return s;
+  if (mpdg_status s = ensure_workspace(h, lp.mixed_precision != 0))
+    return s;
   const auto t0 = std::chrono::steady_clock::now();
+  // the operator reads the current solution (heat_diffuse_operator.hpp:128), and
+  // the preconditioner is updated once, at the start value
+  // (heat_diffuse_operation.cpp:275-276)
+  h->temperature = temperature;
+  if (lp.preconditioner_type == MPDG_PRECONDITIONER_DIAGONAL)
+    {
+      if (mpdg_status s = diagonal_inverse(h, h->inv_diag.ptr))
+        return s;
+    }
+  else
+    MPDG_HIP(launch_fill(h->inv_diag.ptr, 1.0, h->N, h->stream));
   mpdg_newton_report rep{};
   double res_tol = np.residual_tolerance, corr_tol = np.field_correction_tolerance;
   const int max_total = std::max(np.max_nonlinear_iterations + np.max_nonlinear_iterations_alt, 1);
@@ -883,7 +984,7 @@ mpdg_status mpdg_heat_newton_solve(mpdg_heat *h, double *temperature,
                                                heat_source))
         return s;
       mpdg_solver_report lr{};
-      const mpdg_status ls = mpdg_heat_solve(h, h->ndu.ptr, h->nrhs.ptr, &lp, &lr);
+      const mpdg_status ls = solve_impl(h, h->ndu.ptr, h->nrhs.ptr, &lp, &lr, false);
       rep.linear_iterations += lr.iterations;
       if (ls != MPDG_OK)
         return ls;
@@ -937,6 +1038,13 @@ mpdg_status mpdg_heat_jacobi_vmult(mpdg_heat *h, double *dst, const double *src,
 mpdg_status mpdg_heat_solve(mpdg_heat *h, double *x, const double *b,
                             const mpdg_solver_params *params, mpdg_solver_report *report)
 {
+  return solve_impl(h, x, b, params, report, true);
+}
+
+static mpdg_status solve_impl(mpdg_heat *h, double *x, const double *b,
+                              const mpdg_solver_params *params, mpdg_solver_report *report,
+                              bool update_preconditioner)
+{
   if (mpdg_status s = check_ready(h))
     return s;
   if (!x || !b)
@@ -964,7 +1072,9 @@ mpdg_status mpdg_heat_solve(mpdg_heat *h, double *x, const double *b,
   const auto t0 = std::chrono::steady_clock::now();
 
   // JacobiPreconditioner::update -> compute_inverse_diagonal_from_matrixfree
-  if (prm.preconditioner_type == MPDG_PRECONDITIONER_DIAGONAL)
+  if (!update_preconditioner)
+    ;
+  else if (prm.preconditioner_type == MPDG_PRECONDITIONER_DIAGONAL)
     {
       if (mpdg_status s = diagonal_inverse(h, h->inv_diag.ptr))
         return s;
@@ -1455,6 +1565,9 @@ mpdg_status mpdg_heat_batch_create(const mpdg_heat_desc *descs, int count, mpdg_
                   "batch members must share material kind, conductivity and velocity fields and stream");
   if (descs[0].material == MPDG_MATERIAL_GAS_LIQUID)
     return fail(MPDG_ERR_UNSUPPORTED, "batches of gas_liquid operators are not built");
+  for (int s = 0; s < count; ++s)
+    if ((descs[s].convection_faces | descs[s].radiation_faces) & 0x3Fu)
+      return fail(MPDG_ERR_UNSUPPORTED, "batches of operators with Robin faces are not built");
   mpdg_heat_batch *B = new (std::nothrow) mpdg_heat_batch();
   if (!B)
     return fail(MPDG_ERR_ARGUMENT, "out of host memory");

@@ -159,6 +159,39 @@ namespace mpdg
                              const DiagTables<Number> &tab, Number *diag,
                              const Number *heaviside, hipStream_t stream);
 
+  // Robin boundary faces of one operator (face_kernels.hip): convection
+  // alpha (T - T_inf) and radiation eps sigma (T^4 - T_inf^4) on box faces
+  // f = 2d + side. One thread per node of a Robin face; a node on several
+  // Robin faces belongs to the lowest of them, which sums all their
+  // contributions in ascending face order, so every row is written once.
+  struct RobinParams
+  {
+    int nfaces;           // Robin faces of the launch, ascending face id
+    int face[6];
+    int64_t node_off[7];  // prefix sums of the faces' node lattices (thread ranges)
+    unsigned robin;       // mask of the Robin faces
+    unsigned conv, rad;   // convection / radiation face masks
+    unsigned dirichlet;   // Dirichlet face mask
+    int n[3], n1[3];      // cells and lattice nodes per direction
+    double h[3];          // cell extents
+    double alpha, t_inf_conv, eps_sigma, t_inf_rad;
+  };
+  enum RobinMode
+  {
+    kRobinTangent = 0,  // dst += A_face src (src and T constraint-resolved)
+    kRobinDiagonal = 1, // dst += diag(A_face)
+    kRobinRhs = 2       // dst += -(alpha (T - T_inf) + eps sigma (T^4 - T_inf^4)) faces, T plain
+  };
+  constexpr int kRobinThreads = 256;
+  int robin_blocks(const RobinParams &rp);
+  // Tangent with scal != nullptr: also adds p.(A_face p) to the p.Ap that the
+  // cell vmult has just stored in scal->pap and resets scal->alpha; does
+  // nothing once scal->done is set. partials needs robin_blocks() slots.
+  template <typename Number>
+  hipError_t launch_robin(int degree, int mode, const RobinParams &rp, Number *dst,
+                          const Number *src, const double *temperature, double *partials,
+                          SolverScalars *scal, hipStream_t stream);
+
   // One operator of a batch: its parameters, the vectors its vmult reads and
   // writes, its work items per colour and the solve it belongs to.
   template <typename Number>

@@ -55,6 +55,30 @@ class MaterialData:
 
 
 @dataclass
+class RadiationBC:
+    """HeatData::RadiationBC (include/meltpooldg/heat/heat_data.hpp:46-50)."""
+    emissivity: float = 0.0
+    temperature_infinity: float = 0.0
+
+
+@dataclass
+class ConvectionBC:
+    """HeatData::ConvectionBC (include/meltpooldg/heat/heat_data.hpp:52-56)."""
+    convection_coefficient: float = 0.0
+    temperature_infinity: float = 0.0
+
+
+def _face_mask(boundary_ids):
+    """Colorized boundary ids of the box (2d: x_d = lo, 2d + 1: x_d = hi) as a bit mask."""
+    mask = 0
+    for b in boundary_ids or ():
+        if not 0 <= int(b) < 6:
+            raise ValueError(f"boundary id {b} is not a face of the box (0..5)")
+        mask |= 1 << int(b)
+    return mask
+
+
+@dataclass
 class LinearSolverData:
     """include/meltpooldg/linear_algebra/linear_solver_data.hpp:37-51 (same defaults)."""
     do_matrix_free: bool = True
@@ -81,7 +105,8 @@ def _stream_handle():
 
 
 def _make_desc(n_cells, degree, material, box, precision, conductivity_amplitude, velocity,
-               velocity_w0, dirichlet_faces, stream):
+               velocity_w0, dirichlet_faces, stream, convection=None, radiation=None,
+               convection_boundary_ids=(), radiation_boundary_ids=()):
     n = (n_cells,) * 3 if isinstance(n_cells, int) else tuple(n_cells)
     mat = material or MaterialData()
     d = L.HeatDesc()
@@ -102,6 +127,14 @@ def _make_desc(n_cells, degree, material, box, precision, conductivity_amplitude
     d.velocity = {None: 0, "none": 0, "vortex": 1}[velocity]
     d.velocity_w0 = float(velocity_w0)
     d.dirichlet_faces = int(dirichlet_faces)
+    conv = convection or ConvectionBC()
+    rad = radiation or RadiationBC()
+    d.convection_faces = _face_mask(convection_boundary_ids)
+    d.convection_coefficient = float(conv.convection_coefficient)
+    d.convection_temperature_infinity = float(conv.temperature_infinity)
+    d.radiation_faces = _face_mask(radiation_boundary_ids)
+    d.emissivity = float(rad.emissivity)
+    d.radiation_temperature_infinity = float(rad.temperature_infinity)
     d.stream = stream
     return d
 
@@ -112,16 +145,24 @@ class HeatDiffuseOperator:
     Parameters mirror the reference's set-up: FE_Q(degree) on Gauss-Lobatto
     points with QGauss(degree+1) on ``n_cells`` affine hex cells of the box,
     homogeneous Dirichlet on ``dirichlet_faces`` (bit 2d: x_d = lo, 2d+1: hi).
+    Robin faces as the reference attaches them: ``convection_boundary_ids`` /
+    ``radiation_boundary_ids`` (the bc_convection_indices / bc_radiation_indices
+    of heat_diffuse_operator.cpp:76-77, colorized ids 0..5 of the box) with the
+    parameters ``convection`` (ConvectionBC) and ``radiation`` (RadiationBC);
+    radiation makes the tangent depend on the temperature registered with
+    ``set_temperature``.
     """
 
     def __init__(self, n_cells, degree, material=None, box=((0.0, 0.0, 0.0), (1.0, 1.0, 1.0)),
                  precision="fp64", conductivity_amplitude=0.0, velocity=None, velocity_w0=0.2,
-                 dirichlet_faces=0x3F, stream=None):
+                 dirichlet_faces=0x3F, stream=None, convection=None, radiation=None,
+                 convection_boundary_ids=(), radiation_boundary_ids=()):
         self._lib = L.load()
         n = (n_cells,) * 3 if isinstance(n_cells, int) else tuple(n_cells)
         mat = material or MaterialData()
         d = _make_desc(n_cells, degree, mat, box, precision, conductivity_amplitude, velocity,
-                       velocity_w0, dirichlet_faces, stream if stream is not None else _stream_handle())
+                       velocity_w0, dirichlet_faces, stream if stream is not None else _stream_handle(),
+                       convection, radiation, convection_boundary_ids, radiation_boundary_ids)
         self.desc = d
         h = ctypes.c_void_p()
         L.check(self._lib.mpdg_heat_create(ctypes.byref(d), ctypes.byref(h)))
@@ -150,6 +191,19 @@ class HeatDiffuseOperator:
 
     def n_free_dofs(self):
         return int(self._lib.mpdg_heat_n_free_dofs(self._h))
+
+    def set_temperature(self, temperature):
+        """The temperature the tangent is linearised at: the operator's reference
+        to the current solution (heat_diffuse_operator.hpp:128). The engine keeps
+        the device pointer, so later changes of ``temperature`` are seen; None
+        clears it."""
+        if temperature is None:
+            L.check(self._lib.mpdg_heat_set_temperature(self._h, None))
+            self._temperature = None
+            return
+        self._check_vec(temperature, "float64")
+        L.check(self._lib.mpdg_heat_set_temperature(self._h, _ptr(temperature)))
+        self._temperature = temperature
 
     def set_heaviside(self, heaviside):
         """Register the level-set heaviside field (read plain, interpolated at q)."""
@@ -377,6 +431,7 @@ class HeatDiffuseOperation:
         lp = _solver_params(lib, self.linear_solver)
         rep = L.NewtonReport()
         f = self.heat_source
+        self.op._temperature = self.temperature  # the solve points the operator at it
         st = lib.mpdg_heat_newton_solve(h, _ptr(self.temperature), _ptr(self.temperature_old),
                                         _ptr(f) if f is not None else None, ctypes.byref(np_),
                                         ctypes.byref(lp), ctypes.byref(rep))

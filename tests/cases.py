@@ -32,14 +32,21 @@ def oracle_problem(case):
     from oracle import cpu
     return cpu.Problem(case["n"], case["degree"], dt=case["dt"], material=case["material"],
                        k_amplitude=case["k_amplitude"], velocity=case["velocity"],
-                       heaviside=case["heaviside"], dirichlet_faces=case["faces"])
+                       heaviside=case["heaviside"], dirichlet_faces=case["faces"],
+                       robin=case.get("robin"), lo=case.get("lo", (0.0, 0.0, 0.0)),
+                       hi=case.get("hi", (1.0, 1.0, 1.0)))
+
+
+def _ids(mask):
+    return [f for f in range(6) if (mask >> f) & 1]
 
 
 def engine_operator(case, precision="fp64"):
     """HeatDiffuseOperator with the same parameters (needs a GPU)."""
     import torch
-    from meltpooldg_amd import HeatDiffuseOperator, MaterialData, PhaseData
+    from meltpooldg_amd import ConvectionBC, HeatDiffuseOperator, MaterialData, PhaseData, RadiationBC
     m = case["material"]
+    rb = case.get("robin") or {}
     mat = MaterialData(
         type=m.get("type", "gas"),
         gas=PhaseData(m["gas_rho"], m["gas_cp"], m["gas_k"]),
@@ -49,7 +56,14 @@ def engine_operator(case, precision="fp64"):
     op = HeatDiffuseOperator(case["n"], case["degree"], material=mat, precision=precision,
                              conductivity_amplitude=case["k_amplitude"],
                              velocity="vortex" if case["velocity"] else None,
-                             dirichlet_faces=case["faces"])
+                             dirichlet_faces=case["faces"],
+                             box=(case.get("lo", (0.0, 0.0, 0.0)), case.get("hi", (1.0, 1.0, 1.0))),
+                             convection=ConvectionBC(rb.get("convection_coefficient", 0.0),
+                                                     rb.get("convection_t_inf", 0.0)),
+                             radiation=RadiationBC(rb.get("emissivity", 0.0),
+                                                   rb.get("radiation_t_inf", 0.0)),
+                             convection_boundary_ids=_ids(rb.get("convection_faces", 0)),
+                             radiation_boundary_ids=_ids(rb.get("radiation_faces", 0)))
     op.reset_time_increment(case["dt"])
     if case["heaviside"] is not None:
         op.set_heaviside(torch.tensor(case["heaviside"], dtype=torch.float64, device="cuda"))
@@ -57,12 +71,14 @@ def engine_operator(case, precision="fp64"):
 
 
 def make_case(n, degree, dt=1e-3, material=None, k_amplitude=0.0, velocity=False, faces=0x3F,
-              heaviside=None):
+              heaviside=None, robin=None, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0)):
+    """robin: the oracle's dict (convection_faces, convection_coefficient,
+    convection_t_inf, radiation_faces, emissivity, radiation_t_inf)."""
     n = (n,) * 3 if isinstance(n, int) else tuple(n)
     return {"name": f"n{n}_p{degree}", "n": n, "degree": degree, "dt": dt,
             "material": material or {"type": "gas", "gas_rho": 1.0, "gas_cp": 1.0, "gas_k": 1.0},
             "k_amplitude": k_amplitude, "velocity": velocity, "faces": faces,
-            "heaviside": heaviside}
+            "heaviside": heaviside, "robin": robin, "lo": lo, "hi": hi}
 
 
 def rel_err(a, b):

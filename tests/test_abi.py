@@ -54,6 +54,14 @@ def test_argument_errors_without_gpu(lib):
     d.box_hi[2] = 1.0
     d.material = 7
     assert lib.mpdg_heat_create(ctypes.byref(d), ctypes.byref(h)) == L.MPDG_ERR_UNSUPPORTED
+    d.material = 0
+    d.convection_faces = 0x40  # no seventh face
+    assert lib.mpdg_heat_create(ctypes.byref(d), ctypes.byref(h)) == L.MPDG_ERR_ARGUMENT
+    d.convection_faces = 0x02
+    d.emissivity = float("nan")
+    assert lib.mpdg_heat_create(ctypes.byref(d), ctypes.byref(h)) == L.MPDG_ERR_ARGUMENT
+    d.emissivity = 0.0
+    assert lib.mpdg_heat_set_temperature(None, None) == L.MPDG_ERR_ARGUMENT
     assert lib.mpdg_heat_set_dt(None, 1.0) == L.MPDG_ERR_ARGUMENT
     assert lib.mpdg_heat_vmult(None, None, None) == L.MPDG_ERR_ARGUMENT
     assert lib.mpdg_heat_n_dofs(None) == -1
@@ -68,6 +76,15 @@ def test_solver_defaults_follow_linear_solver_data(lib):
     assert p.rel_tolerance == 1e-12
     assert p.abs_tolerance == 1e-20
     assert p.solver_type == 0 and p.preconditioner_type == 3
+
+
+def test_robin_boundary_ids_to_face_masks():
+    from meltpooldg_amd.heat import _face_mask
+    assert _face_mask(()) == 0
+    assert _face_mask([1]) == 0b10
+    assert _face_mask((0, 5, 1)) == 0b100011
+    with pytest.raises(ValueError):
+        _face_mask([6])
 
 
 def test_python_mirror_defaults():
