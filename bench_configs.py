@@ -12,6 +12,15 @@ config 2: 96^3 Q3, two-phase sphere (radius 0.3, centre 0.5^3) from a nodal
 config 3: batched solves (see bench_batch.py).
 config 4: 64^3 Q5 convection-diffusion, vortex velocity, dt = 5e-4,
           k = 1e-2: GMRES(30) + Jacobi, FP64 and FP32-mixed.
+robin (bench.py --mode robin): the physics of the reference's
+          unidirectional_heat_transfer Robin cases (steel: rho 7850, cp 460,
+          k 55.563; dt = 0.01; T = 1000; convection alpha = 25 and radiation
+          emissivity 0.98 to T_inf = 300) on the unit cube with 64^3 Q4 cells,
+          the Robin face being the top face z = 1 and every other face
+          insulated: the vmult with and without the face launches, the
+          full-size vmult against the oracle, and one device Newton time step
+          (GMRES + Jacobi, the heat defaults; Newton residual / correction
+          tolerances 1e-7 / 1e-9 absolute).
 """
 import json
 import os
@@ -174,6 +183,56 @@ def config4(args, torch):
             "fp32_mixed": {"s": repmx["wall_s"], "inner_iterations": repmx["iterations"],
                            "outer_iterations": repmx["outer_iterations"], "err_vs_fp64_1e-12": e(xmx)},
             "speedup_mixed_vs_fp64": rep64["wall_s"] / repmx["wall_s"]}
+
+
+def robin(args, torch):
+    from meltpooldg_amd import (ConvectionBC, HeatDiffuseOperation, HeatDiffuseOperator,
+                                LinearSolverData, MaterialData, NonlinearSolverData, PhaseData,
+                                RadiationBC)
+    from oracle import cpu
+    n, p, dt = 64, 4, 1e-2
+    steel = MaterialData(gas=PhaseData(7850.0, 460.0, 55.563))
+    conv, rad = ConvectionBC(25.0, 300.0), RadiationBC(0.98, 300.0)
+    top = 5
+    op = HeatDiffuseOperator(n, p, material=steel, dirichlet_faces=0, convection=conv,
+                             radiation=rad, convection_boundary_ids=[top],
+                             radiation_boundary_ids=[top])
+    op.reset_time_increment(dt)
+    plain = HeatDiffuseOperator(n, p, material=steel, dirichlet_faces=0)
+    plain.reset_time_increment(dt)
+    N = op.n_dofs()
+    T = torch.full((N,), 1000.0, dtype=torch.float64, device="cuda")
+    op.set_temperature(T)
+    ms_faces = _vmult_time(op, torch, torch.float64, 50)
+    ms_plain = _vmult_time(plain, torch, torch.float64, 50)
+    pb = cpu.Problem(n, p, dt=dt, material={"type": "gas", "gas_rho": 7850.0, "gas_cp": 460.0,
+                                            "gas_k": 55.563}, dirichlet_faces=0,
+                     robin=dict(convection_faces=1 << top, convection_coefficient=25.0,
+                                convection_t_inf=300.0, radiation_faces=1 << top,
+                                emissivity=0.98, radiation_t_inf=300.0),
+                     temperature=np.full(N, 1000.0))
+    err, t_cpu_vmult = _oracle_vmult_check(pb, op, torch)
+    # absolute Newton tolerances sized for 17 M DoFs: the residual's round-off
+    # floor (entries ~ rho_cp/dt |cell| T, summed over the lattice) is ~1e-9,
+    # the NonlinearSolverData default
+    nl = NonlinearSolverData(residual_tolerance=1e-7, field_correction_tolerance=1e-9)
+    heat = HeatDiffuseOperation(op, T, nlsolve=nl, linear_solver=LinearSolverData(
+        solver_type="GMRES", preconditioner_type="Diagonal", rel_tolerance=1e-12))
+    heat.advance(dt)  # warm-up step (workspace allocation)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rep = heat.advance(dt)
+    torch.cuda.synchronize()
+    t_step = time.perf_counter() - t0
+    face_nodes = (n * p + 1) ** 2
+    return {"mode": "robin", "workload": "64^3 Q4 steel (7850, 460, 55.563), dt 0.01, top face "
+                                         "convection 25 / radiation 0.98 (T_inf 300), T=1000, "
+                                         "insulated elsewhere",
+            "n_dofs": N, "face_nodes": face_nodes,
+            "gpu_vmult_ms_with_faces": ms_faces, "gpu_vmult_ms_without_faces": ms_plain,
+            "face_share_of_vmult": (ms_faces - ms_plain) / ms_faces,
+            "vmult_rel_err_vs_oracle": err, "cpu_vmult_ms": t_cpu_vmult * 1e3,
+            "newton_step_s": t_step, "newton": rep}
 
 
 def run(config, args):

@@ -93,6 +93,7 @@ struct mpdg_heat
   // device vector, not a copy: the reference operator holds a reference)
   RobinParams robin{};
   const double *temperature = nullptr;
+  DevBuf<double> robin_rows; // one row of (p+1)^2 face integrals per Robin face cell
 
   // solver workspace
   DevBuf<double> r, p, q, inv_diag, partials, history;
@@ -112,7 +113,7 @@ struct mpdg_heat
   ~mpdg_heat()
   {
     for (auto *b : {&sin_tab64, &sin_tab32_src, &heav64, &r, &p, &q, &inv_diag, &partials, &history,
-                    &V64, &gm})
+                    &V64, &gm, &robin_rows})
       b->release();
     for (auto *b : {&sin_tab32, &heav32, &r32, &p32, &q32, &x32, &inv32, &V32})
       b->release();
@@ -202,6 +203,7 @@ namespace
     rp.eps_sigma = d.emissivity * kStefanBoltzmann;
     rp.t_inf_rad = d.radiation_temperature_infinity;
     rp.node_off[0] = 0;
+    rp.cell_off[0] = 0;
     for (int f = 0; f < 6; ++f)
       if ((rp.robin >> f) & 1u)
         {
@@ -209,6 +211,7 @@ namespace
           const int e1 = dd == 0 ? 1 : 0, e2 = dd == 2 ? 1 : 2;
           rp.face[rp.nfaces] = f;
           rp.node_off[rp.nfaces + 1] = rp.node_off[rp.nfaces] + (int64_t)rp.n1[e1] * rp.n1[e2];
+          rp.cell_off[rp.nfaces + 1] = rp.cell_off[rp.nfaces] + (int64_t)rp.n[e1] * rp.n[e2];
           ++rp.nfaces;
         }
     return rp;
@@ -395,31 +398,38 @@ namespace
   }
 
 
-  // The cell loop, then the Robin faces (MatrixFree::loop's boundary-face part).
-  // With scal, the cell loop leaves p.Ap of the cells in scal->pap and the face
-  // launch adds the faces' part and sets alpha.
-  mpdg_status vmult64(mpdg_heat *H, double *dst, const double *src, double *partials,
-                      SolverScalars *scal)
+  // The cell loop, then the Robin faces (MatrixFree::loop's boundary-face part)
+  // on the same stream. With scal, the cell loop leaves p.Ap of the cells in
+  // scal->pap and the face launches add the faces' part and set alpha. (Running
+  // the face-cell integrals on a second stream beside the cell loop measured
+  // slower: the fork and join cost more than the overlap saves.)
+  template <typename Number>
+  mpdg_status vmult_any(mpdg_heat *H, Number *dst, const Number *src, double *partials,
+                        SolverScalars *scal)
   {
     if (mpdg_status s = check_tangent(H))
       return s;
-    MPDG_HIP(launch_vmult<double>(H->desc.degree, coef_mode(H), H->desc.velocity != 0, H->prm64,
-                                  H->vt64, src, dst, H->heav64.ptr, partials, scal, H->stream));
-    MPDG_HIP(launch_robin<double>(H->desc.degree, kRobinTangent, H->robin, dst, src,
-                                  H->temperature, partials, scal, H->stream));
+    if constexpr (sizeof(Number) == 8)
+      MPDG_HIP(launch_vmult<double>(H->desc.degree, coef_mode(H), H->desc.velocity != 0, H->prm64,
+                                    H->vt64, src, dst, H->heav64.ptr, partials, scal, H->stream));
+    else
+      MPDG_HIP(launch_vmult<float>(H->desc.degree, coef_mode(H), H->desc.velocity != 0, H->prm32,
+                                   H->vt32, src, dst, H->heav32.ptr, partials, scal, H->stream));
+    MPDG_HIP(launch_robin<Number>(H->desc.degree, kRobinTangent, H->robin, dst, src,
+                                  H->temperature, H->robin_rows.ptr, partials, scal, H->stream));
     return MPDG_OK;
+  }
+
+  mpdg_status vmult64(mpdg_heat *H, double *dst, const double *src, double *partials,
+                      SolverScalars *scal)
+  {
+    return vmult_any<double>(H, dst, src, partials, scal);
   }
 
   mpdg_status vmult32(mpdg_heat *H, float *dst, const float *src, double *partials,
                       SolverScalars *scal)
   {
-    if (mpdg_status s = check_tangent(H))
-      return s;
-    MPDG_HIP(launch_vmult<float>(H->desc.degree, coef_mode(H), H->desc.velocity != 0, H->prm32,
-                                 H->vt32, src, dst, H->heav32.ptr, partials, scal, H->stream));
-    MPDG_HIP(launch_robin<float>(H->desc.degree, kRobinTangent, H->robin, dst, src,
-                                 H->temperature, partials, scal, H->stream));
-    return MPDG_OK;
+    return vmult_any<float>(H, dst, src, partials, scal);
   }
 
   mpdg_status diagonal_inverse(mpdg_heat *H, double *inv)
@@ -429,7 +439,7 @@ namespace
     MPDG_HIP(launch_diagonal<double>(H->desc.degree, coef_mode(H), H->desc.velocity != 0,
                                      H->prm64, H->dt64, inv, H->heav64.ptr, H->stream));
     MPDG_HIP(launch_robin<double>(H->desc.degree, kRobinDiagonal, H->robin, inv, nullptr,
-                                  H->temperature, nullptr, nullptr, H->stream));
+                                  H->temperature, H->robin_rows.ptr, nullptr, nullptr, H->stream));
     MPDG_HIP(launch_invert_diagonal(inv, H->N, H->stream));
     return MPDG_OK;
   }
@@ -765,6 +775,15 @@ mpdg_status mpdg_heat_create(const mpdg_heat_desc *desc, mpdg_heat **out)
   fill_params<double>(H, H->prm64, H->sin_tab64.ptr);
   fill_params<float>(H, H->prm32, H->sin_tab32.ptr);
   H->robin = robin_params(d, H->h);
+  if (H->robin.nfaces > 0)
+    {
+      const hipError_t er = H->robin_rows.alloc((size_t)robin_scratch(H->robin, d.degree));
+      if (er != hipSuccess)
+        {
+          delete H;
+          return fail(MPDG_ERR_HIP, std::string("Robin face scratch: ") + hipGetErrorString(er));
+        }
+    }
   *out = H;
   return MPDG_OK;
 }
@@ -860,7 +879,7 @@ mpdg_status mpdg_heat_create_rhs(mpdg_heat *h, double *dst, const double *temper
                              temperature, temperature_old, heat_source, h->heav64.ptr, h->stream));
   // rhs_boundary_loop: the Robin faces with T read plain
   MPDG_HIP(launch_robin<double>(h->desc.degree, kRobinRhs, h->robin, dst, nullptr, temperature,
-                                nullptr, nullptr, h->stream));
+                                h->robin_rows.ptr, nullptr, nullptr, h->stream));
   return MPDG_OK;
 }
 

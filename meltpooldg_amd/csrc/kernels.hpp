@@ -161,14 +161,17 @@ namespace mpdg
 
   // Robin boundary faces of one operator (face_kernels.hip): convection
   // alpha (T - T_inf) and radiation eps sigma (T^4 - T_inf^4) on box faces
-  // f = 2d + side. One thread per node of a Robin face; a node on several
-  // Robin faces belongs to the lowest of them, which sums all their
-  // contributions in ascending face order, so every row is written once.
+  // f = 2d + side. A wavefront per face cell integrates it into a scratch
+  // row of (p+1)^2 values; then one thread per node of a Robin face adds the
+  // rows of the face cells around it. A node on several Robin faces belongs to
+  // the lowest of them, which sums all their contributions in ascending face
+  // order, so every row is written once, in a fixed order.
   struct RobinParams
   {
     int nfaces;           // Robin faces of the launch, ascending face id
     int face[6];
     int64_t node_off[7];  // prefix sums of the faces' node lattices (thread ranges)
+    int64_t cell_off[7];  // prefix sums of the faces' face cells (scratch rows)
     unsigned robin;       // mask of the Robin faces
     unsigned conv, rad;   // convection / radiation face masks
     unsigned dirichlet;   // Dirichlet face mask
@@ -183,14 +186,18 @@ namespace mpdg
     kRobinRhs = 2       // dst += -(alpha (T - T_inf) + eps sigma (T^4 - T_inf^4)) faces, T plain
   };
   constexpr int kRobinThreads = 256;
+  // workgroups of the node launch (partial-dot slots it needs)
   int robin_blocks(const RobinParams &rp);
+  // doubles of the face-cell scratch rows
+  int64_t robin_scratch(const RobinParams &rp, int degree);
   // Tangent with scal != nullptr: also adds p.(A_face p) to the p.Ap that the
   // cell vmult has just stored in scal->pap and resets scal->alpha; does
-  // nothing once scal->done is set. partials needs robin_blocks() slots.
+  // nothing once scal->done is set. partials needs robin_blocks() slots,
+  // scratch robin_scratch() doubles.
   template <typename Number>
   hipError_t launch_robin(int degree, int mode, const RobinParams &rp, Number *dst,
-                          const Number *src, const double *temperature, double *partials,
-                          SolverScalars *scal, hipStream_t stream);
+                          const Number *src, const double *temperature, double *scratch,
+                          double *partials, SolverScalars *scal, hipStream_t stream);
 
   // One operator of a batch: its parameters, the vectors its vmult reads and
   // writes, its work items per colour and the solve it belongs to.
