@@ -57,7 +57,9 @@ typedef enum {
 /* Prescribed advection velocity evaluated at the quadrature points. */
 typedef enum {
   MPDG_VELOCITY_NONE = 0,
-  MPDG_VELOCITY_VORTEX = 1 /* (-sin^2(pi x) sin(2 pi y), sin(2 pi x) sin^2(pi y), w0) */
+  MPDG_VELOCITY_VORTEX = 1,  /* (-sin^2(pi x) sin(2 pi y), sin(2 pi x) sin^2(pi y), w0) */
+  MPDG_VELOCITY_CONSTANT = 2 /* velocity_value, e.g. a nodally interpolated constant
+                                prescribed_velocity field (exact for any FE_Q degree) */
 } mpdg_velocity_field;
 
 typedef struct {
@@ -78,7 +80,8 @@ typedef struct {
   int conductivity_field;      /* mpdg_conductivity_field */
   double conductivity_amplitude;
   int velocity;                /* mpdg_velocity_field */
-  double velocity_w0;
+  double velocity_w0;          /* vortex: z velocity */
+  double velocity_value[3];    /* constant: the velocity vector */
   unsigned dirichlet_faces;    /* bit 2d: face x_d = lo, bit 2d+1: x_d = hi; 0x3F = all */
   /* Robin faces, bits as for dirichlet_faces (the colorized boundary ids of the
    * box): convection alpha (T - T_inf) and radiation emissivity sigma

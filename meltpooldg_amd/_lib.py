@@ -41,6 +41,7 @@ class HeatDesc(ctypes.Structure):
         ("conductivity_amplitude", ctypes.c_double),
         ("velocity", ctypes.c_int),
         ("velocity_w0", ctypes.c_double),
+        ("velocity_value", ctypes.c_double * 3),
         ("dirichlet_faces", ctypes.c_uint),
         ("convection_faces", ctypes.c_uint),
         ("convection_coefficient", ctypes.c_double),

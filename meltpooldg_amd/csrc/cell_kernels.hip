@@ -185,8 +185,11 @@ namespace mpdg
           const int cz0 = kz * layers;
           const int nl = min(layers, prm.nz - cz0);
 
-          Number s2x[N], s2y[N], p2x[N], p2y[N];
-          if constexpr (SINE || VEL)
+          // separable coefficient factors of this column of cells: sine
+          // conductivity, and the in-plane velocity u_x = ux_x(x) ux_y(y),
+          // u_y = uy_x(x) uy_y(y)
+          Number s2x[N], s2y[N], ux_x[N], ux_y[N], uy_x[N], uy_y[N];
+          if constexpr (SINE)
 #pragma unroll
             for (int q = 0; q < N; ++q)
               {
@@ -197,8 +200,10 @@ namespace mpdg
 #pragma unroll
             for (int q = 0; q < N; ++q)
               {
-                p2x[q] = prm.sinpi2[0][cxc * N + q];
-                p2y[q] = prm.sinpi2[1][cyc * N + q];
+                ux_x[q] = prm.vel_x[0][cxc * N + q];
+                ux_y[q] = prm.vel_y[0][cyc * N + q];
+                uy_x[q] = prm.vel_x[1][cxc * N + q];
+                uy_y[q] = prm.vel_y[1][cyc * N + q];
               }
 
           Number carry[N][N]; // upper z-face of the previous layer, after the x/y exchange
@@ -353,7 +358,7 @@ namespace mpdg
                         Number val = o[qz][qy][qx] + K[1] * t[qy];
                         if constexpr (VEL)
                           val += (H2 ? rcq[qy] : prm.rho_cp0) * prm.vf[1] *
-                                 (s2x[qx] * p2y[qy]) * g[qy];
+                                 (uy_x[qx] * uy_y[qy]) * g[qy];
                         o[qz][qy][qx] = val;
                       }
                   }
@@ -399,7 +404,7 @@ namespace mpdg
                         Number val = o[qz][qy][qx] + K[0] * t[qx];
                         if constexpr (VEL)
                           val += (H2 ? rcq[qx] : prm.rho_cp0) * prm.vf[0] *
-                                 (-p2x[qx] * s2y[qy]) * g[qx];
+                                 (ux_x[qx] * ux_y[qy]) * g[qx];
                         o[qz][qy][qx] = val;
                         cellsum += Number(T1.w[qx] * T1.w[qy] * T1.w[qz]) * (x[qx] * val);
                       }

@@ -217,6 +217,60 @@ namespace
     return rp;
   }
 
+  // Analytic coefficient tables at the Gauss coordinates of each cell row:
+  // sin(2 pi x_d) per direction d (separable conductivity), then the separable
+  // in-plane velocity, u_x = X_x(x) Y_x(y) and u_y = X_y(x) Y_y(y), as
+  // [X_x | Y_x | X_y | Y_y]:
+  //   vortex   u = (-sin^2(pi x) sin(2 pi y), sin(2 pi x) sin^2(pi y), w0);
+  //   constant u = (v0, v1, v2).
+  std::vector<double> coefficient_tables(const mpdg_heat_desc &d, const Tables1D &t,
+                                         const double *h)
+  {
+    const int n = t.n;
+    auto coords = [&](int k, auto f) {
+      std::vector<double> out;
+      for (int c = 0; c < d.n_cells[k]; ++c)
+        for (int q = 0; q < n; ++q)
+          out.push_back(f(d.box_lo[k] + (c + t.xq[q]) * h[k]));
+      return out;
+    };
+    const double two_pi = 2.0 * M_PI;
+    auto sin2pi = [&](double x) { return std::sin(two_pi * x); };
+    auto sinpi2 = [&](double x) {
+      const double s = std::sin(M_PI * x);
+      return s * s;
+    };
+    std::vector<double> tab;
+    for (int k = 0; k < 3; ++k)
+      {
+        const std::vector<double> v = coords(k, sin2pi);
+        tab.insert(tab.end(), v.begin(), v.end());
+      }
+    std::vector<double> X[2], Y[2];
+    if (d.velocity == MPDG_VELOCITY_VORTEX)
+      {
+        X[0] = coords(0, [&](double x) { return -sinpi2(x); });
+        Y[0] = coords(1, sin2pi);
+        X[1] = coords(0, sin2pi);
+        Y[1] = coords(1, sinpi2);
+      }
+    else
+      {
+        const bool cst = d.velocity == MPDG_VELOCITY_CONSTANT;
+        const double v0 = cst ? d.velocity_value[0] : 0.0, v1 = cst ? d.velocity_value[1] : 0.0;
+        X[0] = coords(0, [&](double) { return v0; });
+        Y[0] = coords(1, [&](double) { return 1.0; });
+        X[1] = coords(0, [&](double) { return 1.0; });
+        Y[1] = coords(1, [&](double) { return v1; });
+      }
+    for (int c = 0; c < 2; ++c)
+      {
+        tab.insert(tab.end(), X[c].begin(), X[c].end());
+        tab.insert(tab.end(), Y[c].begin(), Y[c].end());
+      }
+    return tab;
+  }
+
   template <typename Number>
   void fill_params(mpdg_heat *H, CellParams<Number> &prm, const Number *tab)
   {
@@ -246,7 +300,7 @@ namespace
                           d.conductivity_amplitude :
                           0.0);
     prm.rho_cp0 = (Number)g_rhocp;
-    prm.w0 = (Number)d.velocity_w0;
+    prm.w0 = (Number)(d.velocity == MPDG_VELOCITY_CONSTANT ? d.velocity_value[2] : d.velocity_w0);
     prm.g_rho = (Number)d.gas.density;
     prm.g_cp = (Number)d.gas.specific_heat_capacity;
     prm.g_k = (Number)d.gas.thermal_conductivity;
@@ -264,17 +318,20 @@ namespace
                   (tu.variant == MPDG_VMULT_LINE_MFMA && d.degree >= kMfmaMinDegree &&
                    sizeof(Number) == 4) ? kVariantLineMfma : kVariantLine;
     prm.tile_log2 = tu.tile_log2;
-    // per-direction tables: [sin2pi dir 0 | dir 1 | dir 2 | sinpi2 dir 0 | dir 1 | dir 2]
+    // per-direction tables (see coefficient_tables): [sin2pi dir 0 | dir 1 | dir 2 |
+    // u_x x-factor | u_x y-factor | u_y x-factor | u_y y-factor]
     size_t off = 0;
     for (int k = 0; k < 3; ++k)
       {
         prm.sin2pi[k] = tab + off;
         off += (size_t)d.n_cells[k] * n;
       }
-    for (int k = 0; k < 3; ++k)
+    for (int c = 0; c < 2; ++c)
       {
-        prm.sinpi2[k] = tab + off;
-        off += (size_t)d.n_cells[k] * n;
+        prm.vel_x[c] = tab + off;
+        off += (size_t)d.n_cells[0] * n;
+        prm.vel_y[c] = tab + off;
+        off += (size_t)d.n_cells[1] * n;
       }
   }
 
@@ -674,8 +731,13 @@ mpdg_status mpdg_heat_create(const mpdg_heat_desc *desc, mpdg_heat **out)
     return fail(MPDG_ERR_ARGUMENT, "unknown precision");
   if (d.material != MPDG_MATERIAL_GAS && d.material != MPDG_MATERIAL_GAS_LIQUID)
     return fail(MPDG_ERR_UNSUPPORTED, "material type not supported");
-  if (d.velocity != MPDG_VELOCITY_NONE && d.velocity != MPDG_VELOCITY_VORTEX)
+  if (d.velocity != MPDG_VELOCITY_NONE && d.velocity != MPDG_VELOCITY_VORTEX &&
+      d.velocity != MPDG_VELOCITY_CONSTANT)
     return fail(MPDG_ERR_UNSUPPORTED, "velocity field not supported");
+  if (d.velocity == MPDG_VELOCITY_CONSTANT &&
+      !(std::isfinite(d.velocity_value[0]) && std::isfinite(d.velocity_value[1]) &&
+        std::isfinite(d.velocity_value[2])))
+    return fail(MPDG_ERR_ARGUMENT, "constant velocity must be finite");
   if (d.conductivity_field != MPDG_CONDUCTIVITY_MATERIAL &&
       d.conductivity_field != MPDG_CONDUCTIVITY_SEPARABLE_SINE)
     return fail(MPDG_ERR_UNSUPPORTED, "conductivity field not supported");
@@ -738,23 +800,7 @@ mpdg_status mpdg_heat_create(const mpdg_heat_desc *desc, mpdg_heat **out)
       return fail(MPDG_ERR_ARGUMENT, "1D tables failed their symmetry / compile-time consistency check");
     }
 
-  // analytic coefficient tables at the quadrature coordinates
-  std::vector<double> tab;
-  const double two_pi = 2.0 * M_PI;
-  for (int pass = 0; pass < 2; ++pass)
-    for (int k = 0; k < 3; ++k)
-      for (int c = 0; c < d.n_cells[k]; ++c)
-        for (int qq = 0; qq < n; ++qq)
-          {
-            const double x = d.box_lo[k] + (c + t.xq[qq]) * H->h[k];
-            if (pass == 0)
-              tab.push_back(std::sin(two_pi * x));
-            else
-              {
-                const double s = std::sin(M_PI * x);
-                tab.push_back(s * s);
-              }
-          }
+  const std::vector<double> tab = coefficient_tables(d, t, H->h);
   std::vector<float> tabf(tab.begin(), tab.end());
   hipError_t e = H->sin_tab64.alloc(tab.size());
   if (e == hipSuccess)

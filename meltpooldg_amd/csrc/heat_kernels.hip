@@ -654,7 +654,7 @@ namespace mpdg
                   if constexpr (VEL)
                     {
                       const Number rc = H2 ? RBl[(s1 * N + s0) * N + q] : prm.rho_cp0;
-                      const Number vx = -prm.sinpi2[0][cx * N + q] * prm.sin2pi[1][it.cy * N + s0];
+                      const Number vx = prm.vel_x[0][cx * N + q] * prm.vel_y[0][it.cy * N + s0];
                       cv[q] = rc * w3 * prm.vf[0] * vx * gx[q];
                     }
                 }
@@ -690,7 +690,7 @@ namespace mpdg
                   if constexpr (VEL)
                     {
                       const Number rc = H2 ? RBl[(s1 * N + q) * N + s0] : prm.rho_cp0;
-                      const Number vy = prm.sin2pi[0][cx * N + s0] * prm.sinpi2[1][it.cy * N + q];
+                      const Number vy = prm.vel_x[1][cx * N + s0] * prm.vel_y[1][it.cy * N + q];
                       cv[q] = rc * w3 * prm.vf[1] * vy * gy[q];
                     }
                 }
@@ -970,8 +970,8 @@ namespace mpdg
               Fkz[q] = k * w3 * prm.kf[2];
               if constexpr (VEL)
                 {
-                  const Number vx = -prm.sinpi2[0][cx * N + q] * prm.sin2pi[1][g.cy * N + s0];
-                  const Number vy = prm.sin2pi[0][cx * N + q] * prm.sinpi2[1][g.cy * N + s0];
+                  const Number vx = prm.vel_x[0][cx * N + q] * prm.vel_y[0][g.cy * N + s0];
+                  const Number vy = prm.vel_x[1][cx * N + q] * prm.vel_y[1][g.cy * N + s0];
                   Fvx[q] = rc * w3 * prm.vf[0] * vx;
                   Fvy[q] = rc * w3 * prm.vf[1] * vy;
                   Fvz[q] = rc * w3 * prm.vf[2] * prm.w0;

@@ -24,7 +24,7 @@ namespace mpdg
     Number k0;            // material conductivity (constant material)
     Number kamp;          // separable-sine conductivity amplitude
     Number rho_cp0;       // constant rho*cp (convection term)
-    Number w0;            // vortex axial velocity
+    Number w0;            // z velocity (constant over the box)
     Number g_rho, g_cp, g_k, l_rho, l_cp, l_k, g_rhocp, l_rhocp;
     int sharp, volume_specific;
     int zlayers;              // cell layers per z-chunk of a vmult work item
@@ -32,7 +32,10 @@ namespace mpdg
     int variant;              // vmult form: kVariantLine or kVariantCell
     int tile_log2;            // cell form: log2 of the tile's cells along x (3..6)
     const Number *sin2pi[3]; // sin(2 pi x) at the quadrature coordinates of each direction
-    const Number *sinpi2[3]; // sin^2(pi x) at the same coordinates
+    // in-plane velocity at q, separable: u_x = vel_x[0](x) vel_y[0](y), u_y = vel_x[1](x) vel_y[1](y)
+    // (tables over the x resp. y quadrature coordinates); u_z = w0
+    const Number *vel_x[2];
+    const Number *vel_y[2];
   };
 
   // 1D tables of the vmult kernel in even-odd form (device copy by value).

@@ -176,7 +176,7 @@ namespace mpdg
               if constexpr (VEL)
                 {
                   const Number rc = H2 ? RBl[(s1 * N + s0) * N + q] : prm.rho_cp0;
-                  const Number vx = -prm.sinpi2[0][cx * N + q] * prm.sin2pi[1][g.cy * N + s0];
+                  const Number vx = prm.vel_x[0][cx * N + q] * prm.vel_y[0][g.cy * N + s0];
                   cv[q] = -rc * w3 * prm.vf[0] * vx * gx[q];
                 }
             }
@@ -205,7 +205,7 @@ namespace mpdg
               if constexpr (VEL)
                 {
                   const Number rc = H2 ? RBl[(s1 * N + q) * N + s0] : prm.rho_cp0;
-                  const Number vy = prm.sin2pi[0][cx * N + s0] * prm.sinpi2[1][g.cy * N + q];
+                  const Number vy = prm.vel_x[1][cx * N + s0] * prm.vel_y[1][g.cy * N + q];
                   cv[q] = -rc * w3 * prm.vf[1] * vy * gy[q];
                 }
             }

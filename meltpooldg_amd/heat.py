@@ -106,7 +106,7 @@ def _stream_handle():
 
 def _make_desc(n_cells, degree, material, box, precision, conductivity_amplitude, velocity,
                velocity_w0, dirichlet_faces, stream, convection=None, radiation=None,
-               convection_boundary_ids=(), radiation_boundary_ids=()):
+               convection_boundary_ids=(), radiation_boundary_ids=(), velocity_value=None):
     n = (n_cells,) * 3 if isinstance(n_cells, int) else tuple(n_cells)
     mat = material or MaterialData()
     d = L.HeatDesc()
@@ -124,8 +124,11 @@ def _make_desc(n_cells, degree, material, box, precision, conductivity_amplitude
     d.volume_specific_heat = int(mat.use_volume_specific_thermal_capacity_for_phase_interpolation)
     d.conductivity_field = 1 if conductivity_amplitude else 0
     d.conductivity_amplitude = float(conductivity_amplitude)
-    d.velocity = {None: 0, "none": 0, "vortex": 1}[velocity]
+    d.velocity = {None: 0, "none": 0, "vortex": 1, "constant": 2}[velocity]
     d.velocity_w0 = float(velocity_w0)
+    if velocity == "constant":
+        v = tuple(float(c) for c in velocity_value)
+        d.velocity_value[:] = list(v) + [0.0] * (3 - len(v))
     d.dirichlet_faces = int(dirichlet_faces)
     conv = convection or ConvectionBC()
     rad = radiation or RadiationBC()
@@ -150,19 +153,22 @@ class HeatDiffuseOperator:
     of heat_diffuse_operator.cpp:76-77, colorized ids 0..5 of the box) with the
     parameters ``convection`` (ConvectionBC) and ``radiation`` (RadiationBC);
     radiation makes the tangent depend on the temperature registered with
-    ``set_temperature``.
+    ``set_temperature``. ``velocity``: None, "vortex" (analytic at q, z component
+    ``velocity_w0``) or "constant" (``velocity_value``, a prescribed_velocity
+    field constant in space).
     """
 
     def __init__(self, n_cells, degree, material=None, box=((0.0, 0.0, 0.0), (1.0, 1.0, 1.0)),
                  precision="fp64", conductivity_amplitude=0.0, velocity=None, velocity_w0=0.2,
                  dirichlet_faces=0x3F, stream=None, convection=None, radiation=None,
-                 convection_boundary_ids=(), radiation_boundary_ids=()):
+                 convection_boundary_ids=(), radiation_boundary_ids=(), velocity_value=None):
         self._lib = L.load()
         n = (n_cells,) * 3 if isinstance(n_cells, int) else tuple(n_cells)
         mat = material or MaterialData()
         d = _make_desc(n_cells, degree, mat, box, precision, conductivity_amplitude, velocity,
                        velocity_w0, dirichlet_faces, stream if stream is not None else _stream_handle(),
-                       convection, radiation, convection_boundary_ids, radiation_boundary_ids)
+                       convection, radiation, convection_boundary_ids, radiation_boundary_ids,
+                       velocity_value)
         self.desc = d
         h = ctypes.c_void_p()
         L.check(self._lib.mpdg_heat_create(ctypes.byref(d), ctypes.byref(h)))

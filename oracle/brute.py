@@ -48,19 +48,21 @@ def material(H, mat):
 
 
 def assemble(n, p, dt, mat, heaviside=None, k_amplitude=0.0, velocity=False, w0=0.2,
-             lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), dirichlet_faces=0x3F):
+             lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), dirichlet_faces=0x3F, velocity_value=None):
     """Return (A_full, mask): dense lattice matrix without constraints and the
     constrained-DoF mask. The constrained operator is A_full restricted to free
     rows/columns, zero elsewhere."""
     A, _, _, mask = assemble_parts(n, p, dt, mat, heaviside, k_amplitude, velocity, w0, lo, hi,
-                                   dirichlet_faces)
+                                   dirichlet_faces, velocity_value)
     return A, mask
 
 
 def assemble_parts(n, p, dt, mat, heaviside=None, k_amplitude=0.0, velocity=False, w0=0.2,
-                   lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), dirichlet_faces=0x3F):
+                   lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), dirichlet_faces=0x3F,
+                   velocity_value=None):
     """(A_full, M_rc, M_1, mask): the full tangent, its mass part rho_cp/dt M and
-    the unit mass matrix M_1 (integral of phi_i phi_j), all without constraints."""
+    the unit mass matrix M_1 (integral of phi_i phi_j), all without constraints.
+    velocity: False, True / "vortex", or "constant" with velocity_value."""
     n = (n, n, n) if np.isscalar(n) else tuple(n)
     n1 = lattice_shape(n, p)
     N = n1[0] * n1[1] * n1[2]
@@ -100,10 +102,13 @@ def assemble_parts(n, p, dt, mat, heaviside=None, k_amplitude=0.0, velocity=Fals
                 Aloc = Mloc.copy()
                 for G in (Gx, Gy, Gz):
                     Aloc += G.T @ ((W * k)[:, None] * G)
-                if velocity:
+                if velocity == "constant":
+                    vx, vy, vz = (np.full_like(Xq, c) for c in velocity_value)
+                elif velocity:
                     vx = -np.sin(np.pi * Xq) ** 2 * np.sin(2 * np.pi * Yq)
                     vy = np.sin(2 * np.pi * Xq) * np.sin(np.pi * Yq) ** 2
                     vz = w0 * np.ones_like(Xq)
+                if velocity:
                     adv = vx[:, None] * Gx + vy[:, None] * Gy + vz[:, None] * Gz
                     Aloc += Phi.T @ ((W * rc)[:, None] * adv)
                 A[np.ix_(dofs, dofs)] += Aloc

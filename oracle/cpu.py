@@ -40,6 +40,7 @@ class OracleProblem(ctypes.Structure):
         ("emissivity", ctypes.c_double),
         ("radiation_t_inf", ctypes.c_double),
         ("temperature", ctypes.POINTER(ctypes.c_double)),
+        ("velocity_value", ctypes.c_double * 3),
     ]
 
 STEFAN_BOLTZMANN = 5.670374419e-8
@@ -115,7 +116,7 @@ class Problem:
 
     def __init__(self, n, degree, dt=1e-3, material=None, k_amplitude=0.0, velocity=False,
                  w0=0.2, heaviside=None, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0),
-                 dirichlet_faces=0x3F, robin=None, temperature=None):
+                 dirichlet_faces=0x3F, robin=None, temperature=None, velocity_value=None):
         mat = material or {"type": "gas", "gas_rho": 1.0, "gas_cp": 1.0, "gas_k": 1.0}
         n = (n, n, n) if np.isscalar(n) else tuple(n)
         s = OracleProblem()
@@ -134,8 +135,11 @@ class Problem:
         s.consistent_with_evaporation = int(mat.get("consistent_with_evaporation", 0))
         s.conductivity_field = 1 if k_amplitude else 0
         s.k_amplitude = k_amplitude
-        s.velocity = 1 if velocity else 0
+        # velocity: False/None, True or "vortex", "constant" (velocity_value)
+        s.velocity = 2 if velocity == "constant" else (1 if velocity else 0)
         s.w0 = w0
+        if velocity == "constant":
+            s.velocity_value[:] = [float(c) for c in velocity_value]
         s.dirichlet_faces = dirichlet_faces
         rb = robin or {}
         s.convection_faces = int(rb.get("convection_faces", 0))

@@ -232,6 +232,25 @@ namespace
       }
   }
 
+  // Prescribed velocity at a quadrature point: 1 = the vortex field (analytic at
+  // q), 2 = a constant field (what VectorTools::interpolate of a constant
+  // prescribed_velocity function gives at any point).
+  inline void velocity_at(const oracle_problem *pb, double xq, double yq, double v[3])
+  {
+    if (pb->velocity == 2)
+      {
+        v[0] = pb->velocity_value[0];
+        v[1] = pb->velocity_value[1];
+        v[2] = pb->velocity_value[2];
+        return;
+      }
+    const double pi = (double)PI_L, two_pi = 2.0 * (double)PI_L;
+    const double sx = std::sin(pi * xq), sy = std::sin(pi * yq);
+    v[0] = -sx * sx * std::sin(two_pi * yq);
+    v[1] = std::sin(two_pi * xq) * sy * sy;
+    v[2] = pb->w0;
+  }
+
   // 1D contraction along one tensor direction of a local n^3 array
   // (index (c*n+b)*n+a, a = x fastest). out = M in along `dir`, or M^T.
   template <bool transpose>
@@ -292,7 +311,6 @@ namespace
     const double jac_det = hx * hy * hz;
     const double dt_inv = 1. / pb->dt;
     const double two_pi = 2.0 * (double)PI_L;
-    const double pi = (double)PI_L;
 
     double V[MAXN * MAXN * MAXN], Fx[MAXN * MAXN * MAXN], Fy[MAXN * MAXN * MAXN],
       Fz[MAXN * MAXN * MAXN];
@@ -314,13 +332,11 @@ namespace
 
             const double gx = Gx[q] / hx, gy = Gy[q] / hy, gz = Gz[q] / hz;
             double val = dt_inv * rho_cp * U[q];
-            if (pb->velocity == 1)
+            if (pb->velocity != 0)
               {
-                const double sx = std::sin(pi * xq), sy = std::sin(pi * yq);
-                const double vx = -sx * sx * std::sin(two_pi * yq);
-                const double vy = std::sin(two_pi * xq) * sy * sy;
-                const double vz = pb->w0;
-                val += rho_cp * (gx * vx + gy * vy + gz * vz);
+                double v[3];
+                velocity_at(pb, xq, yq, v);
+                val += rho_cp * (gx * v[0] + gy * v[1] + gz * v[2]);
               }
             V[q] = val * JxW;
             Fx[q] = k * gx * JxW / hx;
@@ -467,7 +483,6 @@ namespace
     const double jac_det = hx * hy * hz;
     const double dt_inv = 1. / pb->dt;
     const double two_pi = 2.0 * (double)PI_L;
-    const double pi = (double)PI_L;
 
     double V[MAXN * MAXN * MAXN], Fx[MAXN * MAXN * MAXN], Fy[MAXN * MAXN * MAXN],
       Fz[MAXN * MAXN * MAXN];
@@ -489,13 +504,11 @@ namespace
 
             const double gx = Gx[q] / hx, gy = Gy[q] / hy, gz = Gz[q] / hz;
             double val = dt_inv * rho_cp * (U[q] - Uo[q]) - Fq[q];
-            if (pb->velocity == 1)
+            if (pb->velocity != 0)
               {
-                const double sx = std::sin(pi * xq), sy = std::sin(pi * yq);
-                const double vx = -sx * sx * std::sin(two_pi * yq);
-                const double vy = std::sin(two_pi * xq) * sy * sy;
-                const double vz = pb->w0;
-                val += rho_cp * (gx * vx + gy * vy + gz * vz);
+                double v[3];
+                velocity_at(pb, xq, yq, v);
+                val += rho_cp * (gx * v[0] + gy * v[1] + gz * v[2]);
               }
             // submit_value(-1 val), submit_gradient(-1 val_grad)
             V[q] = -val * JxW;

@@ -44,7 +44,8 @@ enum { ORACLE_MAX_DEGREE = 7 };
 
 /* material: 0 = gas (constant), 1 = gas_liquid (nodal heaviside) */
 /* conductivity_field: 0 = from material, 1 = material k + amp*s(x)s(y)s(z), s = sin(2 pi t) */
-/* velocity: 0 = none, 1 = vortex (-sin^2(pi x) sin(2 pi y), sin(2 pi x) sin^2(pi y), w0) */
+/* velocity: 0 = none, 1 = vortex (-sin^2(pi x) sin(2 pi y), sin(2 pi x) sin^2(pi y), w0),
+ *           2 = constant velocity_value */
 typedef struct {
   int n[3];
   int degree;
@@ -72,6 +73,7 @@ typedef struct {
   /* temperature the tangent is linearised at (lattice vector, read with the
    * Dirichlet entries resolved to 0); needed when radiation_faces != 0 */
   const double *temperature;
+  double velocity_value[3]; /* velocity == 2 */
 } oracle_problem;
 
 #define ORACLE_STEFAN_BOLTZMANN 5.670374419e-8

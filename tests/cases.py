@@ -34,7 +34,7 @@ def oracle_problem(case):
                        k_amplitude=case["k_amplitude"], velocity=case["velocity"],
                        heaviside=case["heaviside"], dirichlet_faces=case["faces"],
                        robin=case.get("robin"), lo=case.get("lo", (0.0, 0.0, 0.0)),
-                       hi=case.get("hi", (1.0, 1.0, 1.0)))
+                       hi=case.get("hi", (1.0, 1.0, 1.0)), velocity_value=case.get("velocity_value"))
 
 
 def _ids(mask):
@@ -55,7 +55,9 @@ def engine_operator(case, precision="fp64"):
         use_volume_specific_thermal_capacity_for_phase_interpolation=bool(m.get("volume_specific")))
     op = HeatDiffuseOperator(case["n"], case["degree"], material=mat, precision=precision,
                              conductivity_amplitude=case["k_amplitude"],
-                             velocity="vortex" if case["velocity"] else None,
+                             velocity=("constant" if case["velocity"] == "constant" else
+                                       "vortex" if case["velocity"] else None),
+                             velocity_value=case.get("velocity_value"),
                              dirichlet_faces=case["faces"],
                              box=(case.get("lo", (0.0, 0.0, 0.0)), case.get("hi", (1.0, 1.0, 1.0))),
                              convection=ConvectionBC(rb.get("convection_coefficient", 0.0),
@@ -71,14 +73,17 @@ def engine_operator(case, precision="fp64"):
 
 
 def make_case(n, degree, dt=1e-3, material=None, k_amplitude=0.0, velocity=False, faces=0x3F,
-              heaviside=None, robin=None, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0)):
-    """robin: the oracle's dict (convection_faces, convection_coefficient,
-    convection_t_inf, radiation_faces, emissivity, radiation_t_inf)."""
+              heaviside=None, robin=None, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0),
+              velocity_value=None):
+    """velocity: False, True (vortex) or "constant" (velocity_value); robin: the
+    oracle's dict (convection_faces, convection_coefficient, convection_t_inf,
+    radiation_faces, emissivity, radiation_t_inf)."""
     n = (n,) * 3 if isinstance(n, int) else tuple(n)
     return {"name": f"n{n}_p{degree}", "n": n, "degree": degree, "dt": dt,
             "material": material or {"type": "gas", "gas_rho": 1.0, "gas_cp": 1.0, "gas_k": 1.0},
             "k_amplitude": k_amplitude, "velocity": velocity, "faces": faces,
-            "heaviside": heaviside, "robin": robin, "lo": lo, "hi": hi}
+            "heaviside": heaviside, "robin": robin, "lo": lo, "hi": hi,
+            "velocity_value": velocity_value}
 
 
 def rel_err(a, b):
