@@ -378,6 +378,7 @@ class NonlinearSolverData:
     field_correction_tolerance_alt: float = 1e-9
     residual_tolerance_alt: float = 1e-8
     use_relative_tolerances: bool = False
+    verbosity_level: int = -1
 
 
 class NewtonDidNotConverge(RuntimeError):
@@ -391,9 +392,13 @@ class HeatDiffuseOperation:
 
     * solution history T (current), T_old, T_old_old and the previous time
       increment;
-    * ``init_time_advance(dt)``: reset_time_increment, the linear-extrapolation
-      predictor of the heat defaults (heat_data.cpp:13,
-      predictor_linear.cpp:76-96), homogeneous constraints distributed;
+    * ``init_time_advance(dt, dt_old=None)``: reset_time_increment, the
+      predictor (Predictor::vmult, include/meltpooldg/linear_algebra/
+      predictor.hpp:118-160: "none" keeps the current solution, "zero",
+      "linear_extrapolation", the heat default, heat_data.cpp:13,
+      predictor_linear.cpp:76-96), homogeneous constraints distributed.
+      ``dt_old`` is the time iterator's old increment (default: the previous
+      call's dt, 0 on the first call);
     * ``solve()``: NewtonRaphsonSolver::solve on the residual create_rhs with
       the tangent solve (GMRES + Diagonal by default, heat_data.cpp:11-12);
     * ``l2_norm()``: the norm the reference prints after each Newton solve
@@ -402,7 +407,11 @@ class HeatDiffuseOperation:
     ``temperature``: initial condition (FP64 device vector, copied)."""
 
     def __init__(self, operator, temperature, heat_source=None, linear_solver=None,
-                 nlsolve=None):
+                 nlsolve=None, predictor="linear_extrapolation"):
+        if predictor not in ("none", "zero", "linear_extrapolation"):
+            raise ValueError(f"predictor type {predictor!r} is not built "
+                             "(none, zero, linear_extrapolation)")
+        self.predictor = predictor
         import torch
         self.op = operator
         self.temperature = temperature.clone()
@@ -417,14 +426,19 @@ class HeatDiffuseOperation:
         self.last_report = None
         self._torch = torch
 
-    def init_time_advance(self, dt):
+    def init_time_advance(self, dt, dt_old=None):
         lib, h = self.op._lib, self.op._h
         self.temperature_old_old.copy_(self.temperature_old)
         self.temperature_old.copy_(self.temperature)
-        self.dt_old, self.dt = self.dt, float(dt)
+        self.dt_old, self.dt = (self.dt if dt_old is None else float(dt_old)), float(dt)
         self.op.reset_time_increment(self.dt)
-        L.check(lib.mpdg_heat_linear_predictor(h, _ptr(self.temperature), _ptr(self.temperature_old),
-                                               _ptr(self.temperature_old_old), self.dt, self.dt_old))
+        if self.predictor == "linear_extrapolation":
+            L.check(lib.mpdg_heat_linear_predictor(h, _ptr(self.temperature),
+                                                   _ptr(self.temperature_old),
+                                                   _ptr(self.temperature_old_old), self.dt,
+                                                   self.dt_old))
+        elif self.predictor == "zero":
+            self.temperature.zero_()
         L.check(lib.mpdg_heat_distribute_constraints(h, _ptr(self.temperature)))
 
     def solve(self):
@@ -449,8 +463,8 @@ class HeatDiffuseOperation:
         L.check(st)
         return self.last_report
 
-    def advance(self, dt):
-        self.init_time_advance(dt)
+    def advance(self, dt, dt_old=None):
+        self.init_time_advance(dt, dt_old)
         return self.solve()
 
     def l2_norm(self, v=None):
