@@ -318,6 +318,15 @@ namespace mpdg
   hipError_t launch_multi_axpy(Number *w, const Number *V, int64_t ld, int k, const double *h,
                                int64_t n, double *partials, double *nrm2, unsigned *counter,
                                hipStream_t s);
+  // w -= sum_c h[c] V_c and out[c] = V_c . w (the updated w) in one sweep
+  template <typename Number>
+  hipError_t launch_multi_axpy_dot(Number *w, const Number *V, int64_t ld, int k, const double *h,
+                                   int64_t n, double *partials, double *out, unsigned *counter,
+                                   hipStream_t s);
+  // v = w / sqrt(nrm2) and z = d .* v in one pass
+  template <typename Number>
+  hipError_t launch_scale_precondition(Number *v, Number *z, const Number *w, const Number *d,
+                                       const double *nrm2, int64_t n, hipStream_t s);
   template <typename Number>
   hipError_t launch_scale(Number *v, const Number *w, const double *nrm2, int64_t n, hipStream_t s);
   template <typename Number>

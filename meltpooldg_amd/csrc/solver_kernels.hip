@@ -494,6 +494,303 @@ namespace mpdg
         *nrm2 = tot[0];
     }
 
+    // w -= sum_c h1[c] V_c, then h2[c] = V_c . w (the classical Gram-Schmidt pass
+    // and the dot products of the re-orthogonalisation pass in one sweep; the dots
+    // use w rounded to Number, as a separate multi_dot after the axpy would).
+    template <typename Number>
+    __global__ void __launch_bounds__(T)
+      multi_axpy_dot_kernel(Number *w, const Number *V, int64_t ld, int k, const double *h,
+                            int64_t n, double *partials, double *out, unsigned *counter)
+    {
+      __shared__ double hs[kMaxBasis];
+      __shared__ double sh[kMaxBasis][T / 64];
+      __shared__ bool last;
+      if (threadIdx.x < k)
+        hs[threadIdx.x] = h[threadIdx.x];
+      __syncthreads();
+      double acc[kMaxBasis];
+#pragma unroll
+      for (int c = 0; c < kMaxBasis; ++c)
+        acc[c] = 0.0;
+      for (int64_t i = blockIdx.x * (int64_t)T + threadIdx.x; i < n; i += (int64_t)gridDim.x * T)
+        {
+          double wi = (double)w[i];
+          for (int c = 0; c < k; ++c)
+            wi -= hs[c] * (double)V[c * ld + i];
+          const Number r = (Number)wi;
+          w[i] = r;
+#pragma unroll
+          for (int c = 0; c < kMaxBasis; ++c)
+            if (c < k)
+              acc[c] += (double)V[c * ld + i] * (double)r;
+        }
+      const int wv = threadIdx.x >> 6;
+      for (int c = 0; c < k; ++c)
+        {
+          const double v = wave_sum(acc[c]);
+          if ((threadIdx.x & 63) == 0)
+            sh[c][wv] = v;
+        }
+      __syncthreads();
+      if (threadIdx.x < k)
+        {
+          double s = 0.0;
+          for (int i = 0; i < T / 64; ++i)
+            s += sh[threadIdx.x][i];
+          partials[threadIdx.x * kVecBlocks + blockIdx.x] = s;
+        }
+      __syncthreads();
+      if (threadIdx.x == 0)
+        {
+          __threadfence();
+          last = atomicAdd(counter, 1u) == gridDim.x - 1;
+        }
+      __syncthreads();
+      if (!last)
+        return;
+      __threadfence();
+      if (threadIdx.x < k)
+        {
+          double s = 0.0;
+          for (int b = 0; b < (int)gridDim.x; ++b)
+            s += __hip_atomic_load(partials + threadIdx.x * kVecBlocks + b, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+          out[threadIdx.x] = s;
+        }
+      if (threadIdx.x == 0)
+        *counter = 0u;
+    }
+
+    // ---- 16-byte sweeps over the basis -------------------------------------
+    // The basis vectors start 16-byte aligned (ld a multiple of the vector
+    // width), so a lane moves W entries of every vector per step. MODE:
+    //   0  h[c] = V_c . w
+    //   1  w -= sum_c h[c] V_c, then out[c] = V_c . w   (w rounded to Number)
+    //   2  w -= sum_c h[c] V_c, then out[0] = ||w||^2
+    // Lane t of the fixed grid owns vector chunks t, t + grid, ... and (for the
+    // tail n % W) entry n - n % W + t; partials are summed in block order.
+    enum SweepMode
+    {
+      kSweepDot = 0,
+      kSweepAxpyDot = 1,
+      kSweepAxpyNorm = 2
+    };
+
+    template <typename Number, int MODE>
+    __global__ void __launch_bounds__(T)
+      basis_sweep_kernel(Number *w, const Number *V, int64_t ld, int k, const double *h,
+                         int64_t n, double *partials, double *out, unsigned *counter)
+    {
+      using Vec = typename Wide<Number>::type;
+      constexpr int W = Wide<Number>::W;
+      constexpr int NACC = MODE == kSweepAxpyNorm ? 1 : kMaxBasis;
+      constexpr int G = 8; // basis vectors loaded together
+      __shared__ double hs[kMaxBasis];
+      __shared__ double sh[NACC][T / 64];
+      __shared__ bool last;
+      // coefficients past k are 0, so the clamped loads below add nothing
+      if (MODE != kSweepDot && threadIdx.x < kMaxBasis)
+        hs[threadIdx.x] = (int)threadIdx.x < k ? h[threadIdx.x] : 0.0;
+      __syncthreads();
+      double acc[NACC];
+#pragma unroll
+      for (int c = 0; c < NACC; ++c)
+        acc[c] = 0.0;
+      const int64_t nv = n / W;
+      const int64_t ldv = ld / W;
+      const Vec *Vv = reinterpret_cast<const Vec *>(V);
+      Vec *Wv = reinterpret_cast<Vec *>(w);
+      const int64_t t0 = blockIdx.x * (int64_t)T + threadIdx.x;
+      const int64_t stride = (int64_t)gridDim.x * T;
+      // V_c for c = g*G .. g*G+G-1, indices past k clamped to k-1 (loads stay in
+      // bounds and are issued together; their contributions are discarded)
+      auto load_group = [&](int g, int64_t j, Vec (&vc)[G]) {
+#pragma unroll
+        for (int u = 0; u < G; ++u)
+          {
+            const int c = min(g * G + u, k - 1);
+            vc[u] = Vv[c * ldv + j];
+          }
+      };
+      for (int64_t j = t0; j < nv; j += stride)
+        {
+          const Vec wv = Wv[j];
+          double wd[W];
+#pragma unroll
+          for (int u = 0; u < W; ++u)
+            wd[u] = (double)wv[u];
+          if constexpr (MODE == kSweepDot)
+            {
+#pragma unroll
+              for (int g = 0; g < kMaxBasis / G; ++g)
+                if (g * G < k)
+                  {
+                    Vec vc[G];
+                    load_group(g, j, vc);
+#pragma unroll
+                    for (int u = 0; u < G; ++u)
+                      {
+                        double d = 0.0;
+#pragma unroll
+                        for (int e = 0; e < W; ++e)
+                          d += (double)vc[u][e] * wd[e];
+                        acc[g * G + u] = g * G + u < k ? acc[g * G + u] + d : acc[g * G + u];
+                      }
+                  }
+            }
+          else
+            {
+#pragma unroll
+              for (int g = 0; g < kMaxBasis / G; ++g)
+                if (g * G < k)
+                  {
+                    Vec vc[G];
+                    load_group(g, j, vc);
+#pragma unroll
+                    for (int u = 0; u < G; ++u)
+#pragma unroll
+                      for (int e = 0; e < W; ++e)
+                        wd[e] -= hs[g * G + u] * (double)vc[u][e];
+                  }
+              Vec r;
+#pragma unroll
+              for (int e = 0; e < W; ++e)
+                r[e] = (Number)wd[e];
+              Wv[j] = r;
+              if constexpr (MODE == kSweepAxpyNorm)
+                {
+#pragma unroll
+                  for (int e = 0; e < W; ++e)
+                    acc[0] += (double)r[e] * (double)r[e];
+                }
+              else
+                {
+#pragma unroll
+                  for (int g = 0; g < kMaxBasis / G; ++g)
+                    if (g * G < k)
+                      {
+                        Vec vc[G];
+                        load_group(g, j, vc);
+#pragma unroll
+                        for (int u = 0; u < G; ++u)
+                          {
+                            double d = 0.0;
+#pragma unroll
+                            for (int e = 0; e < W; ++e)
+                              d += (double)vc[u][e] * (double)r[e];
+                            acc[g * G + u] = g * G + u < k ? acc[g * G + u] + d : acc[g * G + u];
+                          }
+                      }
+                }
+            }
+        }
+      const int64_t i = nv * W + t0;
+      if (i < n)
+        {
+          double wi = (double)w[i];
+          if constexpr (MODE == kSweepDot)
+            {
+#pragma unroll
+              for (int c = 0; c < kMaxBasis; ++c)
+                if (c < k)
+                  acc[c] += (double)V[c * ld + i] * wi;
+            }
+          else
+            {
+              for (int c = 0; c < k; ++c)
+                wi -= hs[c] * (double)V[c * ld + i];
+              const Number r = (Number)wi;
+              w[i] = r;
+              if constexpr (MODE == kSweepAxpyNorm)
+                acc[0] += (double)r * (double)r;
+              else
+                {
+#pragma unroll
+                  for (int c = 0; c < kMaxBasis; ++c)
+                    if (c < k)
+                      acc[c] += (double)V[c * ld + i] * (double)r;
+                }
+            }
+        }
+      const int nacc = MODE == kSweepAxpyNorm ? 1 : k;
+      const int wv = threadIdx.x >> 6;
+      // compile-time indices keep acc in registers
+#pragma unroll
+      for (int c = 0; c < NACC; ++c)
+        if (c < nacc)
+          {
+            const double v = wave_sum(acc[c]);
+            if ((threadIdx.x & 63) == 0)
+              sh[c][wv] = v;
+          }
+      __syncthreads();
+      if (threadIdx.x < nacc)
+        {
+          double s = 0.0;
+          for (int q = 0; q < T / 64; ++q)
+            s += sh[threadIdx.x][q];
+          partials[threadIdx.x * kVecBlocks + blockIdx.x] = s;
+        }
+      __syncthreads();
+      if (threadIdx.x == 0)
+        {
+          __threadfence();
+          last = atomicAdd(counter, 1u) == gridDim.x - 1;
+        }
+      __syncthreads();
+      if (!last)
+        return;
+      __threadfence();
+      if (threadIdx.x < nacc)
+        {
+          double s = 0.0;
+          for (int b = 0; b < (int)gridDim.x; ++b)
+            s += __hip_atomic_load(partials + threadIdx.x * kVecBlocks + b, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+          out[threadIdx.x] = s;
+        }
+      if (threadIdx.x == 0)
+        *counter = 0u;
+    }
+
+    // z = D .* (w / sqrt(nrm2)) and v = w / sqrt(nrm2) in one pass (the new
+    // basis vector and its right-preconditioned image), 16-byte accesses.
+    template <typename Number>
+    __global__ void __launch_bounds__(T)
+      scale_precondition_kernel(Number *v, Number *z, const Number *w, const Number *d,
+                                const double *nrm2, int64_t n)
+    {
+      using Vec = typename Wide<Number>::type;
+      constexpr int W = Wide<Number>::W;
+      const double inv = 1.0 / sqrt(*nrm2);
+      const int64_t nv = n / W;
+      const Vec *Wv = reinterpret_cast<const Vec *>(w);
+      const Vec *Dv = reinterpret_cast<const Vec *>(d);
+      Vec *Vv = reinterpret_cast<Vec *>(v);
+      Vec *Zv = reinterpret_cast<Vec *>(z);
+      const int64_t t0 = blockIdx.x * (int64_t)T + threadIdx.x;
+      for (int64_t j = t0; j < nv; j += (int64_t)gridDim.x * T)
+        {
+          const Vec wv = Wv[j], dv = Dv[j];
+          Vec a, b;
+#pragma unroll
+          for (int u = 0; u < W; ++u)
+            {
+              a[u] = (Number)((double)wv[u] * inv);
+              b[u] = a[u] * dv[u];
+            }
+          Vv[j] = a;
+          Zv[j] = b;
+        }
+      const int64_t i = nv * W + t0;
+      if (i < n)
+        {
+          const Number a = (Number)((double)w[i] * inv);
+          v[i] = a;
+          z[i] = a * d[i];
+        }
+    }
+
     // v = w / sqrt(nrm2)
     template <typename Number>
     __global__ void __launch_bounds__(T)
@@ -530,6 +827,13 @@ namespace mpdg
             s += ys[c] * (double)V[c * ld + i];
           x[i] = (XNumber)((double)x[i] + (double)d[i] * s);
         }
+    }
+
+    // 16-byte sweeps need every basis vector and w 16-byte aligned
+    template <typename Number>
+    bool wide_basis(const Number *V, int64_t ld, const Number *w)
+    {
+      return aligned16(V, w) && ld % Wide<Number>::W == 0;
     }
 
     inline int grid_for(int64_t n)
@@ -650,7 +954,11 @@ namespace mpdg
   {
     if (k < 1 || k > kMaxBasis)
       return hipErrorInvalidValue;
-    multi_dot_kernel<Number><<<kVecBlocks, T, 0, s>>>(V, ld, k, w, n, partials, out, counter);
+    if (wide_basis(V, ld, w))
+      basis_sweep_kernel<Number, kSweepDot><<<kVecBlocks, T, 0, s>>>(
+        const_cast<Number *>(w), V, ld, k, nullptr, n, partials, out, counter);
+    else
+      multi_dot_kernel<Number><<<kVecBlocks, T, 0, s>>>(V, ld, k, w, n, partials, out, counter);
     return hipGetLastError();
   }
 
@@ -661,7 +969,45 @@ namespace mpdg
   {
     if (k < 1 || k > kMaxBasis)
       return hipErrorInvalidValue;
-    multi_axpy_kernel<Number><<<kVecBlocks, T, 0, s>>>(w, V, ld, k, h, n, partials, nrm2, counter);
+    if (nrm2 != nullptr && wide_basis(V, ld, w))
+      basis_sweep_kernel<Number, kSweepAxpyNorm><<<kVecBlocks, T, 0, s>>>(w, V, ld, k, h, n,
+                                                                         partials, nrm2, counter);
+    else
+      multi_axpy_kernel<Number><<<kVecBlocks, T, 0, s>>>(w, V, ld, k, h, n, partials, nrm2,
+                                                         counter);
+    return hipGetLastError();
+  }
+
+  template <typename Number>
+  hipError_t launch_multi_axpy_dot(Number *w, const Number *V, int64_t ld, int k, const double *h,
+                                   int64_t n, double *partials, double *out, unsigned *counter,
+                                   hipStream_t s)
+  {
+    if (k < 1 || k > kMaxBasis)
+      return hipErrorInvalidValue;
+    if (wide_basis(V, ld, w))
+      basis_sweep_kernel<Number, kSweepAxpyDot><<<kVecBlocks, T, 0, s>>>(w, V, ld, k, h, n, partials,
+                                                                        out, counter);
+    else
+      multi_axpy_dot_kernel<Number><<<kVecBlocks, T, 0, s>>>(w, V, ld, k, h, n, partials, out,
+                                                             counter);
+    return hipGetLastError();
+  }
+
+  template <typename Number>
+  hipError_t launch_scale_precondition(Number *v, Number *z, const Number *w, const Number *d,
+                                       const double *nrm2, int64_t n, hipStream_t s)
+  {
+    if (aligned16(v, z, w, d))
+      {
+        const int64_t nv = (n + Wide<Number>::W - 1) / Wide<Number>::W;
+        scale_precondition_kernel<Number><<<grid_for(nv), T, 0, s>>>(v, z, w, d, nrm2, n);
+      }
+    else
+      {
+        scale_kernel<Number><<<grid_for(n), T, 0, s>>>(v, w, nrm2, n);
+        diag_scale_kernel<Number><<<grid_for(n), T, 0, s>>>(z, v, d, n);
+      }
     return hipGetLastError();
   }
 
@@ -695,6 +1041,11 @@ namespace mpdg
   template hipError_t launch_multi_axpy<NUM>(NUM *, const NUM *, int64_t, int, const double *,   \
                                              int64_t, double *, double *, unsigned *,            \
                                              hipStream_t);                                       \
+  template hipError_t launch_multi_axpy_dot<NUM>(NUM *, const NUM *, int64_t, int, const double *,\
+                                                 int64_t, double *, double *, unsigned *,        \
+                                                 hipStream_t);                                   \
+  template hipError_t launch_scale_precondition<NUM>(NUM *, NUM *, const NUM *, const NUM *,      \
+                                                     const double *, int64_t, hipStream_t);      \
   template hipError_t launch_scale<NUM>(NUM *, const NUM *, const double *, int64_t, hipStream_t); \
   template hipError_t launch_diag_scale<NUM>(NUM *, const NUM *, const NUM *, int64_t, hipStream_t);
   MPDG_GMRES_INST(double)
