@@ -44,8 +44,11 @@ typedef enum { MPDG_FP64 = 0, MPDG_FP32 = 1 } mpdg_precision;
 
 /* Material model (include/meltpooldg/core/material.hpp:46-55). */
 typedef enum {
-  MPDG_MATERIAL_GAS = 0,        /* constant rho, cp, k */
-  MPDG_MATERIAL_GAS_LIQUID = 1  /* blend by a nodal heaviside field, mpdg_heat_set_heaviside */
+  MPDG_MATERIAL_GAS = 0,          /* constant rho, cp, k */
+  MPDG_MATERIAL_GAS_LIQUID = 1,   /* blend by a nodal heaviside field, mpdg_heat_set_heaviside */
+  MPDG_MATERIAL_LIQUID_SOLID = 2, /* liquid/solid by the temperature's solid fraction (FP64) */
+  MPDG_MATERIAL_GAS_LIQUID_SOLID = 3 /* gas / (liquid-solid) by the heaviside, then the
+                                        temperature (FP64) */
 } mpdg_material_type;
 
 /* Extra conductivity field on top of the material's conductivity. */
@@ -77,6 +80,14 @@ typedef struct {
   mpdg_phase gas, liquid;      /* MaterialData gas / liquid */
   int two_phase_sharp;         /* TwoPhaseFluidPropertiesTransitionType::sharp */
   int volume_specific_heat;    /* use_volume_specific_thermal_capacity_for_phase_interpolation */
+  /* liquid_solid / gas_liquid_solid (MaterialData, include/meltpooldg/core/
+   * material_data.hpp): the solid phase and the solid fraction's law, mushy zone
+   * clamp((liquidus - T) / (liquidus - solidus), 0, 1) or, when
+   * solid_liquid_sharp (SolidLiquidPropertiesTransitionType::sharp), 1 below
+   * the solidus temperature. No apparent capacity (latent heat of fusion 0). */
+  mpdg_phase solid;
+  double solidus_temperature, liquidus_temperature;
+  int solid_liquid_sharp;
   int conductivity_field;      /* mpdg_conductivity_field */
   double conductivity_amplitude;
   int velocity;                /* mpdg_velocity_field */
@@ -148,6 +159,15 @@ mpdg_status mpdg_heat_set_heaviside(mpdg_heat *h, const double *heaviside);
  * and solve; required when radiation_faces != 0, NULL clears it.
  * mpdg_heat_newton_solve points it at its temperature argument. */
 mpdg_status mpdg_heat_set_temperature(mpdg_heat *h, const double *temperature);
+/* The old temperature of a temperature-dependent material's tangent: the
+ * operator's reference to temperature_old (heat_diffuse_operator.hpp:129,187),
+ * which tangent_local_cell_operation reads plain with do_solidification
+ * (heat_diffuse_operator.cpp:1003-1014). The linearisation temperature
+ * (mpdg_heat_set_temperature) is then also read plain by the cells. Device
+ * pointer of n_dofs() doubles, kept (not copied); required by vmult, inverse
+ * diagonal and solve for MPDG_MATERIAL_LIQUID_SOLID / _GAS_LIQUID_SOLID, NULL
+ * clears it. mpdg_heat_newton_solve points it at its temperature_old argument. */
+mpdg_status mpdg_heat_set_temperature_old(mpdg_heat *h, const double *temperature_old);
 
 /* HeatDiffuseMultiPhaseOperator::vmult (heat_diffuse_operator.cpp:179-197), FP64 vectors:
  * the cell loop, then the Robin faces (tangent_boundary_loop :243-269). */
@@ -166,8 +186,9 @@ mpdg_status mpdg_heat_inverse_diagonal(mpdg_heat *h, double *inv_diag);
  * f by reference from its constructor (heat_diffuse_operator.hpp:186-188);
  * here they are arguments. heat_source may be NULL (f = 0). The Robin faces add
  * -(alpha (T - T_inf) + eps sigma (T^4 - T_inf^4)) integrated over the faces
- * (rhs_boundary_loop :719-781), T read plain. No Neumann faces and no
- * evaporative cooling term. */
+ * (rhs_boundary_loop :719-781), T read plain. A temperature-dependent
+ * material is evaluated at T(q) (get_material_parameters :1133-1165). No
+ * Neumann faces and no evaporative cooling term. */
 mpdg_status mpdg_heat_create_rhs(mpdg_heat *h, double *dst, const double *temperature,
                                  const double *temperature_old, const double *heat_source);
 /* JacobiPreconditioner::vmult (preconditioner_jacobi.hpp:51-56): dst = inv_diag .* src. */

@@ -37,6 +37,10 @@ class HeatDesc(ctypes.Structure):
         ("liquid", Phase),
         ("two_phase_sharp", ctypes.c_int),
         ("volume_specific_heat", ctypes.c_int),
+        ("solid", Phase),
+        ("solidus_temperature", ctypes.c_double),
+        ("liquidus_temperature", ctypes.c_double),
+        ("solid_liquid_sharp", ctypes.c_int),
         ("conductivity_field", ctypes.c_int),
         ("conductivity_amplitude", ctypes.c_double),
         ("velocity", ctypes.c_int),
@@ -131,6 +135,7 @@ SIGNATURES = {
     "mpdg_heat_set_dt": (ctypes.c_int, [_VP, ctypes.c_double]),
     "mpdg_heat_set_heaviside": (ctypes.c_int, [_VP, _VP]),
     "mpdg_heat_set_temperature": (ctypes.c_int, [_VP, _VP]),
+    "mpdg_heat_set_temperature_old": (ctypes.c_int, [_VP, _VP]),
     "mpdg_heat_vmult": (ctypes.c_int, [_VP, _VP, _VP]),
     "mpdg_heat_vmult_f32": (ctypes.c_int, [_VP, _VP, _VP]),
     "mpdg_heat_inverse_diagonal": (ctypes.c_int, [_VP, _VP]),

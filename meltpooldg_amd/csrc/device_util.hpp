@@ -53,6 +53,46 @@ namespace mpdg
         }
     }
 
+    // y = M x with M in even-odd form held at run time (kernel argument), SIGN
+    // the symmetry sign of M.
+    template <int N, int SIGN, typename Number>
+    __device__ __forceinline__ void eo_apply(const EvenOdd<Number> &M, const Number (&x)[N],
+                                             Number (&y)[N])
+    {
+      constexpr int H = N / 2;
+      Number E[H], O[H];
+#pragma unroll
+      for (int j = 0; j < H; ++j)
+        {
+          E[j] = x[j] + x[N - 1 - j];
+          O[j] = x[j] - x[N - 1 - j];
+        }
+#pragma unroll
+      for (int i = 0; i < H; ++i)
+        {
+          Number a = M.e[i][0] * E[0];
+          Number b = M.o[i][0] * O[0];
+#pragma unroll
+          for (int j = 1; j < H; ++j)
+            {
+              a += M.e[i][j] * E[j];
+              b += M.o[i][j] * O[j];
+            }
+          if constexpr (N % 2 == 1)
+            a += M.mc[i] * x[H];
+          y[i] = a + b;
+          y[N - 1 - i] = SIGN > 0 ? a - b : b - a;
+        }
+      if constexpr (N % 2 == 1)
+        {
+          Number m = M.mm * x[H];
+#pragma unroll
+          for (int j = 0; j < H; ++j)
+            m += M.mr[j] * (SIGN > 0 ? E[j] : O[j]);
+          y[H] = m;
+        }
+    }
+
     // Raw buffer access: an offset at or past the buffer's byte size reads 0
     // and drops the store, so masked lanes need no branch.
     constexpr uint32_t kOutOfRange = 0xFFFFFF00u;
@@ -115,6 +155,67 @@ namespace mpdg
         rc = one_w * prm.g_rhocp + w * prm.l_rhocp;
       else
         rc = (one_w * prm.g_rho + w * prm.l_rho) * (one_w * prm.g_cp + w * prm.l_cp);
+    }
+
+    // Temperature-dependent Material::compute_parameters_internal for
+    // liquid_solid (prm.tmat == 2) and gas_liquid_solid (3) with the
+    // temperature derivatives (material.templates.hpp:296-495, 540-756): solid
+    // fraction fs = clamp((liquidus - T) / (liquidus - solidus), 0, 1), or
+    // T < solidus for the sharp law; each property blends liquid and solid by
+    // the cubic smoothstep of fs (exact phase values and zero derivative for a
+    // pure phase), then gas_liquid_solid blends the gas value with weight 1 - w
+    // (w = H, or H > 0.5 when sharp) and scales the derivative by w; rho_cp and
+    // its derivative as get_material_parameters_and_derivatives
+    // (heat_diffuse_operator.cpp:1167-1210).
+    template <typename Number>
+    __device__ __forceinline__ void solid_liquid_material(const CellParams<Number> &prm, Number H,
+                                                          Number T, Number &rc, Number &k,
+                                                          Number &drc, Number &dk)
+    {
+      Number fs;
+      if (prm.sl_sharp)
+        fs = T < prm.solidus ? Number(1) : Number(0);
+      else
+        {
+          fs = (prm.liquidus - T) * prm.inv_mushy;
+          fs = fs > Number(1) ? Number(1) : (fs < Number(0) ? Number(0) : fs);
+        }
+      const bool solid = fs == Number(1);
+      const bool pure = solid || fs == Number(0);
+      const Number sm = fs * fs * (Number(3) - Number(2) * fs);
+      const Number dsm = pure ? Number(0) : -prm.inv_mushy * Number(6) * fs * (Number(1) - fs);
+      auto blend = [&](Number l, Number s, Number &v, Number &dv) {
+        v = solid ? s : l + (s - l) * sm;
+        dv = (s - l) * dsm;
+      };
+      Number cp, dcp, rho, drho, vc, dvc;
+      blend(prm.l_k, prm.s_k, k, dk);
+      blend(prm.l_cp, prm.s_cp, cp, dcp);
+      blend(prm.l_rho, prm.s_rho, rho, drho);
+      blend(prm.l_rhocp, prm.s_rhocp, vc, dvc);
+      if (prm.tmat == 3)
+        {
+          const Number w = prm.sharp ? (H > Number(0.5) ? Number(1) : Number(0)) : H;
+          const Number one_w = Number(1) - w;
+          k = one_w * prm.g_k + w * k;
+          dk *= w;
+          cp = one_w * prm.g_cp + w * cp;
+          dcp *= w;
+          rho = one_w * prm.g_rho + w * rho;
+          drho *= w;
+          vc = one_w * prm.g_rhocp + w * vc;
+          dvc *= w;
+        }
+      if (prm.volume_specific)
+        {
+          rc = vc;
+          drc = dvc;
+        }
+      else
+        {
+          rc = rho * cp;
+          drc = dcp * rho + drho * cp;
+        }
     }
 
     // Deterministic block sum (fixed butterfly + fixed-order wave sum).
@@ -239,6 +340,46 @@ namespace mpdg
                           prm.sin2pi[2][cz * N + qz];
       else
         return KBl[(qz * N + qy) * N + qx];
+    }
+
+    // Coloured first-touch scatter of one cell x-line (b = s0, c = s1) after the
+    // in-segment x-face exchange: the first cell in colour order to reach a node
+    // stores its value (0 on a constrained node), later cells add; the x-line
+    // end a = p is written by the segment's last cell only.
+    template <int P, typename Number, int CPB>
+    __device__ __forceinline__ void scatter_xline(const CellParams<Number> &prm, const Segment &g,
+                                                  int l, int s0, int s1, const Number (&Y)[P + 1],
+                                                  Number *__restrict__ dst)
+    {
+      constexpr int N = P + 1;
+      const int Yi = g.cy * P + s0, Zi = g.cz * P + s1;
+      const bool ysh = (s0 == 0 && g.cy > 0) || (s0 == P && g.cy < prm.ny - 1);
+      const bool zsh = (s1 == 0 && g.cz > 0) || (s1 == P && g.cz < prm.nz - 1);
+      const bool yz_first = !(ysh && (g.cy & 1)) && !(zsh && (g.cz & 1));
+      const bool rowc = on_dirichlet(prm, Yi, 1) || on_dirichlet(prm, Zi, 2);
+      const int X0 = (g.cx0 + l) * P;
+      const int64_t base = X0 + (int64_t)prm.n1x * (Yi + (int64_t)prm.n1y * Zi);
+      const bool last = l == g.ncell - 1;
+#pragma unroll
+      for (int a = 0; a < N; ++a)
+        {
+          if (a == N - 1 && !last)
+            continue;
+          const bool xsh = (a == 0 && l == 0 && g.cx0 > 0) ||
+                           (a == P && last && g.cx0 + g.ncell < prm.nx);
+          const bool first = yz_first && !(xsh && (g.sx & 1));
+          const bool con = rowc || on_dirichlet(prm, X0 + a, 0);
+          Number *ptr = dst + base + a;
+          if (con)
+            {
+              if (first)
+                *ptr = Number(0);
+            }
+          else if (first)
+            *ptr = Y[a];
+          else
+            *ptr = *ptr + Y[a];
+        }
     }
 
     // Operator served by this workgroup in a batched launch: the last s with

@@ -93,6 +93,9 @@ struct mpdg_heat
   // device vector, not a copy: the reference operator holds a reference)
   RobinParams robin{};
   const double *temperature = nullptr;
+  // old temperature the tangent of a temperature-dependent material reads
+  // (the operator's reference to temperature_old), also a caller's vector
+  const double *temperature_old = nullptr;
   DevBuf<double> robin_rows; // one row of (p+1)^2 face integrals per Robin face cell
 
   // solver workspace
@@ -311,6 +314,20 @@ namespace
     prm.l_rhocp = (Number)l_rhocp;
     prm.sharp = d.two_phase_sharp;
     prm.volume_specific = d.volume_specific_heat;
+    prm.tmat = (d.material == MPDG_MATERIAL_LIQUID_SOLID || d.material == MPDG_MATERIAL_GAS_LIQUID_SOLID) ?
+                 d.material :
+                 0;
+    prm.sl_sharp = d.solid_liquid_sharp;
+    prm.s_rho = (Number)d.solid.density;
+    prm.s_cp = (Number)d.solid.specific_heat_capacity;
+    prm.s_k = (Number)d.solid.thermal_conductivity;
+    prm.s_rhocp = (Number)(d.solid.density * d.solid.specific_heat_capacity);
+    prm.solidus = (Number)d.solidus_temperature;
+    prm.liquidus = (Number)d.liquidus_temperature;
+    // Material::inv_mushy_interval (material.hpp:648-654); unused by the sharp law
+    prm.inv_mushy = (Number)(d.liquidus_temperature > d.solidus_temperature ?
+                               1.0 / (d.liquidus_temperature - d.solidus_temperature) :
+                               0.0);
     const mpdg_tuning &tu = H->tune[sizeof(Number) == 4 ? MPDG_FP32 : MPDG_FP64];
     prm.zlayers = std::max(1, std::min(tu.z_layers, d.n_cells[2]));
     prm.blocks_per_cu = tu.blocks_per_cu;
@@ -335,8 +352,20 @@ namespace
       }
   }
 
+  bool temperature_dependent(const mpdg_heat_desc &d)
+  {
+    return d.material == MPDG_MATERIAL_LIQUID_SOLID || d.material == MPDG_MATERIAL_GAS_LIQUID_SOLID;
+  }
+
+  bool uses_heaviside(const mpdg_heat_desc &d)
+  {
+    return d.material == MPDG_MATERIAL_GAS_LIQUID || d.material == MPDG_MATERIAL_GAS_LIQUID_SOLID;
+  }
+
   int coef_mode(const mpdg_heat *H)
   {
+    if (temperature_dependent(H->desc))
+      return kCoefTemperature;
     if (H->desc.material == MPDG_MATERIAL_GAS_LIQUID)
       return kCoefHeaviside;
     return H->desc.conductivity_field == MPDG_CONDUCTIVITY_SEPARABLE_SINE ? kCoefSine :
@@ -350,8 +379,8 @@ namespace
     if (!H->dt_set)
       return fail(MPDG_ERR_ZERO_TIME_INCREMENT,
                   "time increment not set: call mpdg_heat_set_dt (reset_time_increment) first");
-    if (H->desc.material == MPDG_MATERIAL_GAS_LIQUID && !H->has_heaviside)
-      return fail(MPDG_ERR_ARGUMENT, "gas_liquid material needs mpdg_heat_set_heaviside");
+    if (uses_heaviside(H->desc) && !H->has_heaviside)
+      return fail(MPDG_ERR_ARGUMENT, "gas_liquid(_solid) material needs mpdg_heat_set_heaviside");
     return MPDG_OK;
   }
 
@@ -361,6 +390,10 @@ namespace
     if (H->robin.rad && !H->temperature)
       return fail(MPDG_ERR_ARGUMENT,
                   "radiation faces need the linearisation temperature: mpdg_heat_set_temperature");
+    if (temperature_dependent(H->desc) && (!H->temperature || !H->temperature_old))
+      return fail(MPDG_ERR_ARGUMENT,
+                  "a temperature-dependent material needs the linearisation temperature and the "
+                  "old temperature: mpdg_heat_set_temperature / mpdg_heat_set_temperature_old");
     return MPDG_OK;
   }
 
@@ -373,7 +406,9 @@ namespace
     const size_t vmult_slots = 8 * (size_t)kMaxResidentPerCu * device_cu_count();
     const size_t np = std::max<size_t>({(size_t)cg.total, (size_t)cg32.total, vmult_slots,
                                         (size_t)kMaxBasis * kVecBlocks,
-                                        (size_t)robin_blocks(H->robin)});
+                                        (size_t)robin_blocks(H->robin),
+                                        (size_t)tmat_blocks(H->desc.degree, H->desc.n_cells[0],
+                                                            H->desc.n_cells[1], H->desc.n_cells[2])});
     MPDG_HIP(H->partials.alloc(np));
     MPDG_HIP(H->r.alloc(H->N));
     MPDG_HIP(H->p.alloc(H->N));
@@ -470,7 +505,18 @@ namespace
   {
     if (mpdg_status s = check_tangent(H))
       return s;
-    if constexpr (sizeof(Number) == 8)
+    if (temperature_dependent(H->desc))
+      {
+        // the solidification tangent (FP64 only; creation rejects FP32 handles)
+        if constexpr (sizeof(Number) == 8)
+          MPDG_HIP(launch_tmat_vmult(H->desc.degree, H->desc.velocity != 0, H->prm64, src, dst,
+                                     H->temperature, H->temperature_old, H->heav64.ptr, partials,
+                                     scal, H->stream));
+        else
+          return fail(MPDG_ERR_UNSUPPORTED,
+                      "temperature-dependent materials have an FP64 cell loop only");
+      }
+    else if constexpr (sizeof(Number) == 8)
       MPDG_HIP(launch_vmult<double>(H->desc.degree, coef_mode(H), H->desc.velocity != 0, H->prm64,
                                     H->vt64, src, dst, H->heav64.ptr, partials, scal, H->stream));
     else
@@ -497,8 +543,12 @@ namespace
   {
     if (mpdg_status s = check_tangent(H))
       return s;
-    MPDG_HIP(launch_diagonal<double>(H->desc.degree, coef_mode(H), H->desc.velocity != 0,
-                                     H->prm64, H->dt64, inv, H->heav64.ptr, H->stream));
+    if (temperature_dependent(H->desc))
+      MPDG_HIP(launch_tmat_diagonal(H->desc.degree, H->desc.velocity != 0, H->prm64, H->dt64, inv,
+                                    H->temperature, H->temperature_old, H->heav64.ptr, H->stream));
+    else
+      MPDG_HIP(launch_diagonal<double>(H->desc.degree, coef_mode(H), H->desc.velocity != 0,
+                                       H->prm64, H->dt64, inv, H->heav64.ptr, H->stream));
     MPDG_HIP(launch_robin<double>(H->desc.degree, kRobinDiagonal, H->robin, inv, nullptr,
                                   H->temperature, H->robin_rows.ptr, nullptr, nullptr, H->stream));
     MPDG_HIP(launch_invert_diagonal(inv, H->N, H->stream));
@@ -736,8 +786,23 @@ mpdg_status mpdg_heat_create(const mpdg_heat_desc *desc, mpdg_heat **out)
     return fail(MPDG_ERR_ARGUMENT, "mesh too large");
   if (d.precision != MPDG_FP64 && d.precision != MPDG_FP32)
     return fail(MPDG_ERR_ARGUMENT, "unknown precision");
-  if (d.material != MPDG_MATERIAL_GAS && d.material != MPDG_MATERIAL_GAS_LIQUID)
+  if (d.material != MPDG_MATERIAL_GAS && d.material != MPDG_MATERIAL_GAS_LIQUID &&
+      d.material != MPDG_MATERIAL_LIQUID_SOLID && d.material != MPDG_MATERIAL_GAS_LIQUID_SOLID)
     return fail(MPDG_ERR_UNSUPPORTED, "material type not supported");
+  if (temperature_dependent(d))
+    {
+      if (d.precision != MPDG_FP64)
+        return fail(MPDG_ERR_UNSUPPORTED,
+                    "temperature-dependent materials have an FP64 cell loop only");
+      if (d.conductivity_field != MPDG_CONDUCTIVITY_MATERIAL)
+        return fail(MPDG_ERR_UNSUPPORTED,
+                    "separable conductivity applies to the gas material only");
+      if (!std::isfinite(d.solidus_temperature) || !std::isfinite(d.liquidus_temperature) ||
+          (d.solid_liquid_sharp ? !(d.liquidus_temperature >= d.solidus_temperature) :
+                                  !(d.liquidus_temperature > d.solidus_temperature)))
+        return fail(MPDG_ERR_ARGUMENT, "the mushy zone needs liquidus > solidus temperature "
+                                       "(>= for the sharp solid-liquid transition)");
+    }
   if (d.velocity != MPDG_VELOCITY_NONE && d.velocity != MPDG_VELOCITY_VORTEX &&
       d.velocity != MPDG_VELOCITY_CONSTANT)
     return fail(MPDG_ERR_UNSUPPORTED, "velocity field not supported");
@@ -748,7 +813,7 @@ mpdg_status mpdg_heat_create(const mpdg_heat_desc *desc, mpdg_heat **out)
   if (d.conductivity_field != MPDG_CONDUCTIVITY_MATERIAL &&
       d.conductivity_field != MPDG_CONDUCTIVITY_SEPARABLE_SINE)
     return fail(MPDG_ERR_UNSUPPORTED, "conductivity field not supported");
-  if (d.material == MPDG_MATERIAL_GAS_LIQUID && d.conductivity_field != MPDG_CONDUCTIVITY_MATERIAL)
+  if (uses_heaviside(d) && d.conductivity_field != MPDG_CONDUCTIVITY_MATERIAL)
     return fail(MPDG_ERR_UNSUPPORTED, "separable conductivity applies to the gas material only");
   if ((d.dirichlet_faces | d.convection_faces | d.radiation_faces) & ~0x3Fu)
     return fail(MPDG_ERR_ARGUMENT, "face masks have 6 bits (faces 2d + side)");
@@ -887,6 +952,14 @@ mpdg_status mpdg_heat_set_temperature(mpdg_heat *h, const double *temperature)
   if (!h)
     return fail(MPDG_ERR_ARGUMENT, "null handle");
   h->temperature = temperature;
+  return MPDG_OK;
+}
+
+mpdg_status mpdg_heat_set_temperature_old(mpdg_heat *h, const double *temperature_old)
+{
+  if (!h)
+    return fail(MPDG_ERR_ARGUMENT, "null handle");
+  h->temperature_old = temperature_old;
   return MPDG_OK;
 }
 
@@ -1031,6 +1104,7 @@ mpdg_status mpdg_heat_newton_solve(mpdg_heat *h, double *temperature,
   // the preconditioner is updated once, at the start value
   // (heat_diffuse_operation.cpp:275-276)
   h->temperature = temperature;
+  h->temperature_old = temperature_old;
   if (lp.preconditioner_type == MPDG_PRECONDITIONER_DIAGONAL)
     {
       if (mpdg_status s = diagonal_inverse(h, h->inv_diag.ptr))
@@ -1134,6 +1208,9 @@ static mpdg_status solve_impl(mpdg_heat *h, double *x, const double *b,
   if (prm.max_iterations < 1 || !(prm.rel_tolerance >= 0) || !(prm.abs_tolerance >= 0))
     return fail(MPDG_ERR_ARGUMENT, "invalid solver parameters");
   const bool mixed = prm.mixed_precision != 0;
+  if (mixed && temperature_dependent(h->desc))
+    return fail(MPDG_ERR_UNSUPPORTED,
+                "temperature-dependent materials have an FP64 cell loop only (no mixed precision)");
   if (mpdg_status s = ensure_workspace(h, mixed))
     return s;
   if (use_gmres)
@@ -1287,6 +1364,9 @@ mpdg_status mpdg_heat_autotune(mpdg_heat *h, int precision, int repeats, mpdg_tu
     return fail(MPDG_ERR_ARGUMENT, "precision must be MPDG_FP64 or MPDG_FP32");
   if (repeats < 1)
     return fail(MPDG_ERR_ARGUMENT, "repeats must be >= 1");
+  if (temperature_dependent(h->desc))
+    return fail(MPDG_ERR_UNSUPPORTED,
+                "the tangent of a temperature-dependent material has one launch form (no tuning)");
   const bool fp32 = precision == MPDG_FP32;
   if (mpdg_status s = ensure_workspace(h, fp32))
     return s;
@@ -1635,8 +1715,8 @@ mpdg_status mpdg_heat_batch_create(const mpdg_heat_desc *descs, int count, mpdg_
         descs[s].velocity != descs[0].velocity || descs[s].stream != descs[0].stream)
       return fail(MPDG_ERR_UNSUPPORTED,
                   "batch members must share material kind, conductivity and velocity fields and stream");
-  if (descs[0].material == MPDG_MATERIAL_GAS_LIQUID)
-    return fail(MPDG_ERR_UNSUPPORTED, "batches of gas_liquid operators are not built");
+  if (descs[0].material != MPDG_MATERIAL_GAS)
+    return fail(MPDG_ERR_UNSUPPORTED, "batches are built for the gas material only");
   for (int s = 0; s < count; ++s)
     if ((descs[s].convection_faces | descs[s].radiation_faces) & 0x3Fu)
       return fail(MPDG_ERR_UNSUPPORTED, "batches of operators with Robin faces are not built");

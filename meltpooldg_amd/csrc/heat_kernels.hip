@@ -43,84 +43,6 @@ namespace mpdg
     // (HIP's second __launch_bounds__ argument): 4 caps the kernel at 128 VGPRs.
     constexpr int kMinWavesPerSimd = 4;
 
-    // y = M x with M in even-odd form, SIGN the symmetry sign of M.
-    template <int N, int SIGN, typename Number>
-    __device__ __forceinline__ void eo_apply(const EvenOdd<Number> &M, const Number (&x)[N],
-                                             Number (&y)[N])
-    {
-      constexpr int H = N / 2;
-      Number E[H], O[H];
-#pragma unroll
-      for (int j = 0; j < H; ++j)
-        {
-          E[j] = x[j] + x[N - 1 - j];
-          O[j] = x[j] - x[N - 1 - j];
-        }
-#pragma unroll
-      for (int i = 0; i < H; ++i)
-        {
-          Number a = M.e[i][0] * E[0];
-          Number b = M.o[i][0] * O[0];
-#pragma unroll
-          for (int j = 1; j < H; ++j)
-            {
-              a += M.e[i][j] * E[j];
-              b += M.o[i][j] * O[j];
-            }
-          if constexpr (N % 2 == 1)
-            a += M.mc[i] * x[H];
-          y[i] = a + b;
-          y[N - 1 - i] = SIGN > 0 ? a - b : b - a;
-        }
-      if constexpr (N % 2 == 1)
-        {
-          Number m = M.mm * x[H];
-#pragma unroll
-          for (int j = 0; j < H; ++j)
-            m += M.mr[j] * (SIGN > 0 ? E[j] : O[j]);
-          y[H] = m;
-        }
-    }
-
-
-    // Coloured first-touch scatter of one cell x-line (b = s0, c = s1) after the
-    // in-segment x-face exchange (see the file header).
-    template <int P, typename Number, int CPB>
-    __device__ __forceinline__ void scatter_xline(const CellParams<Number> &prm, const Segment &g,
-                                                  int l, int s0, int s1, const Number (&Y)[P + 1],
-                                                  Number *__restrict__ dst)
-    {
-      constexpr int N = P + 1;
-      const int Yi = g.cy * P + s0, Zi = g.cz * P + s1;
-      const bool ysh = (s0 == 0 && g.cy > 0) || (s0 == P && g.cy < prm.ny - 1);
-      const bool zsh = (s1 == 0 && g.cz > 0) || (s1 == P && g.cz < prm.nz - 1);
-      const bool yz_first = !(ysh && (g.cy & 1)) && !(zsh && (g.cz & 1));
-      const bool rowc = on_dirichlet(prm, Yi, 1) || on_dirichlet(prm, Zi, 2);
-      const int X0 = (g.cx0 + l) * P;
-      const int64_t base = X0 + (int64_t)prm.n1x * (Yi + (int64_t)prm.n1y * Zi);
-      const bool last = l == g.ncell - 1;
-#pragma unroll
-      for (int a = 0; a < N; ++a)
-        {
-          if (a == N - 1 && !last)
-            continue;
-          const bool xsh = (a == 0 && l == 0 && g.cx0 > 0) ||
-                           (a == P && last && g.cx0 + g.ncell < prm.nx);
-          const bool first = yz_first && !(xsh && (g.sx & 1));
-          const bool con = rowc || on_dirichlet(prm, X0 + a, 0);
-          Number *ptr = dst + base + a;
-          if (con)
-            {
-              if (first)
-                *ptr = Number(0);
-            }
-          else if (first)
-            *ptr = Y[a];
-          else
-            *ptr = *ptr + Y[a];
-        }
-    }
-
     // Orders this wavefront's LDS accesses across lanes: LDS instructions of one
     // wavefront execute in issue order, so a compiler barrier is enough.
     __device__ __forceinline__ void wave_sync()

@@ -27,6 +27,12 @@ namespace mpdg
     Number w0;            // z velocity (constant over the box)
     Number g_rho, g_cp, g_k, l_rho, l_cp, l_k, g_rhocp, l_rhocp;
     int sharp, volume_specific;
+    // temperature-dependent material: tmat = 2 liquid_solid, 3 gas_liquid_solid
+    // (0 otherwise); solid phase, solid-fraction law (mushy zone or sharp at the
+    // solidus) with inv_mushy = 1 / (liquidus - solidus)
+    int tmat, sl_sharp;
+    Number s_rho, s_cp, s_k, s_rhocp;
+    Number solidus, liquidus, inv_mushy;
     int zlayers;              // cell layers per z-chunk of a vmult work item
     int blocks_per_cu;        // persistent workgroups per CU (0: occupancy limit)
     int variant;              // vmult form: kVariantLine or kVariantCell
@@ -69,7 +75,8 @@ namespace mpdg
   {
     kCoefConstant = 0,
     kCoefSine = 1,
-    kCoefHeaviside = 2
+    kCoefHeaviside = 2,
+    kCoefTemperature = 3 // material evaluated at T(q) (and H(q)): liquid_solid, gas_liquid_solid
   };
 
   // Form of the vmult kernel. Line: the lanes of a wavefront own the 1D lines
@@ -154,6 +161,23 @@ namespace mpdg
   // *out = sum over cells and Gauss points of JxW u^2 (FE L2 norm squared);
   // partials needs l2_norm_blocks() slots, *ticket must start at 0 (reset on exit).
   int l2_norm_blocks(int degree, int nx, int ny, int nz);
+  // Tangent of a temperature-dependent material (material_kernels.hip), FP64:
+  // dst = J src over 8 colour passes (dst is zeroed first), the material and
+  // its temperature derivatives evaluated at T(q) (T and T_old read plain,
+  // heaviside when prm.tmat == 3). With scal != nullptr it reduces p.Ap into
+  // scal->pap and sets scal->alpha = rz / pap like launch_vmult.
+  hipError_t launch_tmat_vmult(int degree, bool vel, const CellParams<double> &prm,
+                               const double *src, double *dst, const double *temperature,
+                               const double *temperature_old, const double *heaviside,
+                               double *partials, SolverScalars *scal, hipStream_t stream);
+  // Diagonal of the same tangent (not inverted), dst overwritten.
+  hipError_t launch_tmat_diagonal(int degree, bool vel, const CellParams<double> &prm,
+                                  const DiagTables<double> &tab, double *dst,
+                                  const double *temperature, const double *temperature_old,
+                                  const double *heaviside, hipStream_t stream);
+  // Workgroup partial slots of launch_tmat_vmult.
+  int tmat_blocks(int degree, int nx, int ny, int nz);
+
   hipError_t launch_l2_norm_sq(int degree, const CellParams<double> &prm, const double *u,
                                double *partials, unsigned *ticket, double *out, hipStream_t s);
 

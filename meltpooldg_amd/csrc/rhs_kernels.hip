@@ -3,8 +3,9 @@
 // dst += -( rho_cp/dt M (T_new - T_old) - M_1 f + K_k T_new + rho_cp C_u T_new )
 // on the free rows, with T_new, T_old and the heat source f read plain
 // (constrained values included), as source/heat/heat_diffuse_operator.cpp
-// create_rhs :783-806 / rhs_cell_loop :393-529 do for a temperature-independent
-// material without evaporation: at q
+// create_rhs :783-806 / rhs_cell_loop :393-529 do without evaporation, the
+// material evaluated at the heaviside and, for liquid_solid / gas_liquid_solid,
+// at T_new(q) (get_material_parameters :1133-1165): at q
 //   val      = dt^-1 rho_cp (T_new - T_old) - f  (+ rho_cp grad T_new . u),
 //   val_grad = k grad T_new,
 // submitted with the factor -1 and integrated against values and gradients.
@@ -31,7 +32,8 @@ namespace mpdg
       constexpr int N = P + 1;
       constexpr int NL = N * N;
       constexpr int STR = lds_stride<N>();
-      constexpr bool H2 = COEF == kCoefHeaviside;
+      // QC: rho_cp and k vary per quadrature point (staged in KB / RB)
+      constexpr bool QC = COEF == kCoefHeaviside || COEF == kCoefTemperature;
       constexpr ct::EO<N> TS = ct::VTab<P>::S;
       constexpr ct::EO<N> TSt = ct::VTab<P>::St;
       constexpr ct::EO<N> TC = ct::VTab<P>::C;
@@ -41,11 +43,13 @@ namespace mpdg
       __shared__ Number A[CPB * STR];  // T_new (nodal -> q)
       __shared__ Number Bd[CPB * STR]; // T_new - T_old; later the q-space accumulator
       __shared__ Number Cf[CPB * STR]; // heat source
-      __shared__ Number KB[H2 ? CPB * STR : 1];
-      __shared__ Number RB[H2 ? CPB * STR : 1];
+      __shared__ Number KB[QC ? CPB * STR : 1];
+      __shared__ Number RB[QC ? CPB * STR : 1];
       __shared__ Number XE[CPB * NL];
 
       const Segment g = segment_of<Number, CPB>(prm, colour);
+      // the heaviside is read for gas_liquid and gas_liquid_solid
+      const bool use_h = COEF == kCoefHeaviside || (COEF == kCoefTemperature && prm.tmat == 3);
       const int t = threadIdx.x;
       const bool active = t < CPB * NL;
       const int l = t % CPB;
@@ -56,8 +60,8 @@ namespace mpdg
       Number *Al = A + l * STR;
       Number *Bl = Bd + l * STR;
       Number *Cl = Cf + l * STR;
-      Number *KBl = KB + (H2 ? l * STR : 0);
-      Number *RBl = RB + (H2 ? l * STR : 0);
+      Number *KBl = KB + (QC ? l * STR : 0);
+      Number *RBl = RB + (QC ? l * STR : 0);
 
       // ---- x-lines (y = s0, z = s1): plain gathers, interpolate in x ----------
       Number dq[N], fq[N];
@@ -72,7 +76,7 @@ namespace mpdg
               tn[a] = cell_ok ? tnew[base + a] : Number(0);
               dl[a] = cell_ok ? tn[a] - told[base + a] : Number(0);
               fl[a] = (cell_ok && fsrc) ? fsrc[base + a] : Number(0);
-              hl[a] = (H2 && cell_ok) ? heav[base + a] : Number(0);
+              hl[a] = (QC && use_h && cell_ok) ? heav[base + a] : Number(0);
             }
           const int row = (s1 * N + s0) * N;
           eo_apply<N, 1>(TS, tn, o);
@@ -87,13 +91,14 @@ namespace mpdg
 #pragma unroll
           for (int q = 0; q < N; ++q)
             Cl[row + q] = o[q];
-          if constexpr (H2)
-            {
-              eo_apply<N, 1>(TS, hl, o);
+          if constexpr (QC)
+            if (use_h)
+              {
+                eo_apply<N, 1>(TS, hl, o);
 #pragma unroll
-              for (int q = 0; q < N; ++q)
-                KBl[row + q] = o[q];
-            }
+                for (int q = 0; q < N; ++q)
+                  KBl[row + q] = o[q];
+              }
         }
       __syncthreads();
 
@@ -113,8 +118,9 @@ namespace mpdg
           sweep(Al);
           sweep(Bl);
           sweep(Cl);
-          if constexpr (H2)
-            sweep(KBl);
+          if constexpr (QC)
+            if (use_h)
+              sweep(KBl);
         }
       __syncthreads();
 
@@ -139,18 +145,33 @@ namespace mpdg
 #pragma unroll
           for (int q = 0; q < N; ++q)
             Al[(q * N + s1) * N + s0] = U[q];
-          if constexpr (H2)
+          if constexpr (QC)
             {
               Number Hq[N];
+              if (use_h)
+                {
 #pragma unroll
-              for (int c = 0; c < N; ++c)
-                v[c] = KBl[(c * N + s1) * N + s0];
-              eo_apply<N, 1>(TS, v, Hq);
+                  for (int c = 0; c < N; ++c)
+                    v[c] = KBl[(c * N + s1) * N + s0];
+                  eo_apply<N, 1>(TS, v, Hq);
+                }
+              else
+                {
+#pragma unroll
+                  for (int q = 0; q < N; ++q)
+                    Hq[q] = Number(0);
+                }
 #pragma unroll
               for (int q = 0; q < N; ++q)
                 {
                   Number rc, k;
-                  two_phase(prm, Hq[q], rc, k);
+                  if constexpr (COEF == kCoefTemperature)
+                    {
+                      Number drc, dk;
+                      solid_liquid_material(prm, Hq[q], U[q], rc, k, drc, dk);
+                    }
+                  else
+                    two_phase(prm, Hq[q], rc, k);
                   KBl[(q * N + s1) * N + s0] = k;
                   RBl[(q * N + s1) * N + s0] = rc;
                 }
@@ -175,7 +196,7 @@ namespace mpdg
               f[q] = -k * w3 * prm.kf[0] * gx[q];
               if constexpr (VEL)
                 {
-                  const Number rc = H2 ? RBl[(s1 * N + s0) * N + q] : prm.rho_cp0;
+                  const Number rc = QC ? RBl[(s1 * N + s0) * N + q] : prm.rho_cp0;
                   const Number vx = prm.vel_x[0][cx * N + q] * prm.vel_y[0][g.cy * N + s0];
                   cv[q] = -rc * w3 * prm.vf[0] * vx * gx[q];
                 }
@@ -204,7 +225,7 @@ namespace mpdg
               f[q] = -k * w3 * prm.kf[1] * gy[q];
               if constexpr (VEL)
                 {
-                  const Number rc = H2 ? RBl[(s1 * N + q) * N + s0] : prm.rho_cp0;
+                  const Number rc = QC ? RBl[(s1 * N + q) * N + s0] : prm.rho_cp0;
                   const Number vy = prm.vel_x[1][cx * N + s0] * prm.vel_y[1][g.cy * N + q];
                   cv[q] = -rc * w3 * prm.vf[1] * vy * gy[q];
                 }
@@ -236,8 +257,8 @@ namespace mpdg
           for (int q = 0; q < N; ++q)
             {
               const Number w3 = Number(T1.w[q]) * wxy;
-              const Number rc = H2 ? RBl[(q * N + s1) * N + s0] : prm.rho_cp0;
-              const Number mass = H2 ? rc * prm.dtinv_vol : prm.mass;
+              const Number rc = QC ? RBl[(q * N + s1) * N + s0] : prm.rho_cp0;
+              const Number mass = QC ? rc * prm.dtinv_vol : prm.mass;
               Number val = -(mass * w3 * dq[q] - prm.vol * w3 * fq[q]);
               if constexpr (VEL)
                 val -= rc * w3 * prm.vf[2] * prm.w0 * Gz[q];
@@ -435,6 +456,8 @@ namespace mpdg
           case 3: return rhs_passes<P, 1, true>(prm, dst, tn, to, f, heav, s);
           case 4: return rhs_passes<P, 2, false>(prm, dst, tn, to, f, heav, s);
           case 5: return rhs_passes<P, 2, true>(prm, dst, tn, to, f, heav, s);
+          case 6: return rhs_passes<P, 3, false>(prm, dst, tn, to, f, heav, s);
+          case 7: return rhs_passes<P, 3, true>(prm, dst, tn, to, f, heav, s);
           default: return hipErrorInvalidValue;
         }
     }

@@ -44,14 +44,27 @@ class PhaseData:
     thermal_conductivity: float = 1.0
 
 
+MATERIAL_TYPES = {"gas": 0, "gas_liquid": 1, "liquid_solid": 2, "gas_liquid_solid": 3}
+
+
 @dataclass
 class MaterialData:
-    """Material model of include/meltpooldg/core/material.hpp:46-55 ('gas' or 'gas_liquid')."""
+    """Material model of include/meltpooldg/core/material.hpp:46-55 ('gas',
+    'gas_liquid', 'liquid_solid' or 'gas_liquid_solid') with the MaterialData
+    fields it reads (include/meltpooldg/core/material_data.hpp). The
+    temperature-dependent types take the solid fraction from the solidus and
+    liquidus temperatures ('mushy_zone' or 'sharp'); the apparent capacity
+    (latent heat of fusion > 0) is not built."""
     type: str = "gas"
     gas: PhaseData = field(default_factory=PhaseData)
     liquid: PhaseData = field(default_factory=PhaseData)
+    solid: PhaseData = field(default_factory=PhaseData)
     two_phase_fluid_properties_transition_type: str = "smooth"
     use_volume_specific_thermal_capacity_for_phase_interpolation: bool = False
+    solid_liquid_properties_transition_type: str = "mushy_zone"
+    solidus_temperature: float = 0.0
+    liquidus_temperature: float = 0.0
+    latent_heat_of_fusion: float = 0.0
 
 
 @dataclass
@@ -115,13 +128,22 @@ def _make_desc(n_cells, degree, material, box, precision, conductivity_amplitude
     d.box_lo[:] = [float(v) for v in box[0]]
     d.box_hi[:] = [float(v) for v in box[1]]
     d.precision = L.MPDG_FP32 if precision == "fp32" else L.MPDG_FP64
-    d.material = 0 if mat.type == "gas" else 1
-    for dst, src in ((d.gas, mat.gas), (d.liquid, mat.liquid)):
+    if mat.type not in MATERIAL_TYPES:
+        raise ValueError(f"material type {mat.type!r} is not built ({sorted(MATERIAL_TYPES)})")
+    if mat.latent_heat_of_fusion > 0.0:
+        raise ValueError("latent heat of fusion > 0 (apparent capacity) is not built")
+    if mat.solid_liquid_properties_transition_type not in ("mushy_zone", "sharp"):
+        raise ValueError("solid liquid properties transition type must be mushy_zone or sharp")
+    d.material = MATERIAL_TYPES[mat.type]
+    for dst, src in ((d.gas, mat.gas), (d.liquid, mat.liquid), (d.solid, mat.solid)):
         dst.density = src.density
         dst.specific_heat_capacity = src.specific_heat_capacity
         dst.thermal_conductivity = src.thermal_conductivity
     d.two_phase_sharp = 1 if mat.two_phase_fluid_properties_transition_type == "sharp" else 0
     d.volume_specific_heat = int(mat.use_volume_specific_thermal_capacity_for_phase_interpolation)
+    d.solidus_temperature = float(mat.solidus_temperature)
+    d.liquidus_temperature = float(mat.liquidus_temperature)
+    d.solid_liquid_sharp = int(mat.solid_liquid_properties_transition_type == "sharp")
     d.conductivity_field = 1 if conductivity_amplitude else 0
     d.conductivity_amplitude = float(conductivity_amplitude)
     d.velocity = {None: 0, "none": 0, "vortex": 1, "constant": 2}[velocity]
@@ -210,6 +232,18 @@ class HeatDiffuseOperator:
         self._check_vec(temperature, "float64")
         L.check(self._lib.mpdg_heat_set_temperature(self._h, _ptr(temperature)))
         self._temperature = temperature
+
+    def set_temperature_old(self, temperature_old):
+        """The old temperature a temperature-dependent material's tangent reads
+        (the operator's reference to temperature_old, heat_diffuse_operator.hpp:129).
+        Kept by pointer like set_temperature; None clears it."""
+        if temperature_old is None:
+            L.check(self._lib.mpdg_heat_set_temperature_old(self._h, None))
+            self._temperature_old = None
+            return
+        self._check_vec(temperature_old, "float64")
+        L.check(self._lib.mpdg_heat_set_temperature_old(self._h, _ptr(temperature_old)))
+        self._temperature_old = temperature_old
 
     def set_heaviside(self, heaviside):
         """Register the level-set heaviside field (read plain, interpolated at q)."""
@@ -451,7 +485,9 @@ class HeatDiffuseOperation:
         lp = _solver_params(lib, self.linear_solver)
         rep = L.NewtonReport()
         f = self.heat_source
-        self.op._temperature = self.temperature  # the solve points the operator at it
+        # the solve points the operator at the solution and the old solution
+        self.op._temperature = self.temperature
+        self.op._temperature_old = self.temperature_old
         st = lib.mpdg_heat_newton_solve(h, _ptr(self.temperature), _ptr(self.temperature_old),
                                         _ptr(f) if f is not None else None, ctypes.byref(np_),
                                         ctypes.byref(lp), ctypes.byref(rep))

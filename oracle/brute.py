@@ -224,3 +224,96 @@ def robin_face_parts(n, p, robin, T_tangent, T_rhs, mask, lo=(0.0, 0.0, 0.0),
             A[np.ix_(dofs, dofs)] += Phi.T @ ((JxW * c)[:, None] * Phi)
             r[dofs] += Phi.T @ (JxW * val)
     return A, r
+
+
+def material_t(H, T, mat):
+    """(rho_cp, k, d rho_cp/dT, dk/dT) at quadrature points for the
+    temperature-dependent materials liquid_solid and gas_liquid_solid (own numpy
+    statement of material.templates.hpp:296-495, 540-756): solid fraction
+    fs = clip((T_l - T) / (T_l - T_s), 0, 1) (mushy zone) or T < T_s (sharp);
+    liquid/solid blend l + (s - l) fs^2 (3 - 2 fs), derivative
+    -(s - l) 6 fs (1 - fs) / (T_l - T_s), both exact phase values (derivative 0)
+    at fs in {0, 1}; gas_liquid_solid blends the gas value with weight 1 - w
+    and scales the derivative by w (w = H, or H > 0.5 when sharp)."""
+    T = np.asarray(T, dtype=float)
+    span = mat["liquidus"] - mat["solidus"]
+    if mat.get("solid_liquid_sharp"):
+        fs = (T < mat["solidus"]).astype(float)
+    else:
+        fs = np.clip((mat["liquidus"] - T) / span, 0.0, 1.0)
+    inner = (fs > 0.0) & (fs < 1.0)
+    smooth = fs * fs * (3.0 - 2.0 * fs)
+    dsmooth = np.where(inner, -6.0 * fs * (1.0 - fs) / (span if span else 1.0), 0.0)
+
+    def blend(liq, sol):
+        return liq + (sol - liq) * smooth, (sol - liq) * dsmooth
+
+    k, dk = blend(mat["liq_k"], mat["sol_k"])
+    cp, dcp = blend(mat["liq_cp"], mat["sol_cp"])
+    rho, drho = blend(mat["liq_rho"], mat["sol_rho"])
+    vc, dvc = blend(mat["liq_rho"] * mat["liq_cp"], mat["sol_rho"] * mat["sol_cp"])
+    if mat["type"] == "gas_liquid_solid":
+        H = np.asarray(H, dtype=float)
+        w = (H > 0.5).astype(float) if mat.get("sharp") else H
+
+        def gas(g, v, dv):
+            return (1.0 - w) * g + w * v, w * dv
+
+        k, dk = gas(mat["gas_k"], k, dk)
+        cp, dcp = gas(mat["gas_cp"], cp, dcp)
+        rho, drho = gas(mat["gas_rho"], rho, drho)
+        vc, dvc = gas(mat["gas_rho"] * mat["gas_cp"], vc, dvc)
+    if mat.get("volume_specific"):
+        return vc, k, dvc, dk
+    return rho * cp, k, dcp * rho + drho * cp, dk
+
+
+def solidification_parts(n, p, dt, mat, T, T_old, heaviside=None, velocity=None,
+                         velocity_value=(0.0, 0.0, 0.0), lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0),
+                         heat_source=None):
+    """(J_full, r_full): the dense tangent linearised at T (T and T_old read
+    plain) and the residual right-hand side at T, both without constraints,
+    for a temperature-dependent material, by explicit 3D quadrature:
+      r_i = -sum_q JxW [(rc(T)/dt (T - T_old) - f + rc(T) u.grad T) phi_i
+                        + k(T) grad T . grad phi_i]
+      J_ij = sum_q JxW [(rc/dt + drc/dt (T - T_old) + drc u.grad T) phi_j phi_i
+                        + rc (u.grad phi_j) phi_i + (k grad phi_j + dk grad T phi_j) . grad phi_i]
+    velocity: None or "constant" (velocity_value)."""
+    n = (n, n, n) if np.isscalar(n) else tuple(n)
+    n1 = lattice_shape(n, p)
+    N = n1[0] * n1[1] * n1[2]
+    xq, wq, S, D = _basis_1d(p)
+    nq = p + 1
+    h = [(hi[d] - lo[d]) / n[d] for d in range(3)]
+    Phi = np.einsum("zc,yb,xa->zyxcba", S, S, S).reshape(nq**3, nq**3)
+    Gs = [np.einsum("zc,yb,xa->zyxcba", S, S, D).reshape(nq**3, nq**3) / h[0],
+          np.einsum("zc,yb,xa->zyxcba", S, D, S).reshape(nq**3, nq**3) / h[1],
+          np.einsum("zc,yb,xa->zyxcba", D, S, S).reshape(nq**3, nq**3) / h[2]]
+    W = np.einsum("z,y,x->zyx", wq, wq, wq).reshape(-1) * h[0] * h[1] * h[2]
+    J = np.zeros((N, N))
+    r = np.zeros(N)
+    u = np.asarray(velocity_value, dtype=float) if velocity == "constant" else np.zeros(3)
+    loc = np.arange(nq)
+    for cz in range(n[2]):
+        for cy in range(n[1]):
+            for cx in range(n[0]):
+                ix, iy, iz = cx * p + loc, cy * p + loc, cz * p + loc
+                dofs = (ix[None, None, :] + n1[0] * (iy[None, :, None] + n1[1] * iz[:, None, None])).reshape(-1)
+                Tq = Phi @ T[dofs]
+                Toq = Phi @ T_old[dofs]
+                fq = Phi @ heat_source[dofs] if heat_source is not None else np.zeros(nq**3)
+                gT = [G @ T[dofs] for G in Gs]
+                Hq = Phi @ heaviside[dofs] if heaviside is not None else np.zeros(nq**3)
+                rc, k, drc, dk = material_t(Hq, Tq, mat)
+                ugT = sum(u[d] * gT[d] for d in range(3))
+                val = rc / dt * (Tq - Toq) - fq + rc * ugT
+                r[dofs] -= Phi.T @ (W * val) + sum(Gs[d].T @ (W * k * gT[d]) for d in range(3))
+                mass = rc / dt + drc / dt * (Tq - Toq) + drc * ugT
+                A = Phi.T @ ((W * mass)[:, None] * Phi)
+                adv = sum(u[d] * Gs[d] for d in range(3))
+                A += Phi.T @ ((W * rc)[:, None] * adv)
+                for d in range(3):
+                    A += Gs[d].T @ ((W * k)[:, None] * Gs[d])
+                    A += Gs[d].T @ ((W * dk * gT[d])[:, None] * Phi)
+                J[np.ix_(dofs, dofs)] += A
+    return J, r

@@ -41,7 +41,16 @@ class OracleProblem(ctypes.Structure):
         ("radiation_t_inf", ctypes.c_double),
         ("temperature", ctypes.POINTER(ctypes.c_double)),
         ("velocity_value", ctypes.c_double * 3),
+        ("sol_rho", ctypes.c_double),
+        ("sol_cp", ctypes.c_double),
+        ("sol_k", ctypes.c_double),
+        ("solidus", ctypes.c_double),
+        ("liquidus", ctypes.c_double),
+        ("solid_liquid_sharp", ctypes.c_int),
+        ("temperature_old", ctypes.POINTER(ctypes.c_double)),
     ]
+
+MATERIAL_TYPES = {"gas": 0, "gas_liquid": 1, "liquid_solid": 2, "gas_liquid_solid": 3}
 
 STEFAN_BOLTZMANN = 5.670374419e-8
 
@@ -64,6 +73,7 @@ def lib():
         L.oracle_constrained_mask.argtypes = [P, ctypes.POINTER(ctypes.c_ubyte)]
         L.oracle_random_vector.argtypes = [P, ctypes.c_uint64, dp]
         L.oracle_material.argtypes = [ctypes.c_int] * 4 + [ctypes.c_double] * 7 + [dp] * 4
+        L.oracle_material_state.argtypes = [P, ctypes.c_double, ctypes.c_double, dp]
         L.oracle_vmult.argtypes = [P, dp, dp]
         L.oracle_create_rhs.argtypes = [P, dp, dp, dp, dp]
         L.oracle_l2_norm.argtypes = [P, dp, dp]
@@ -108,6 +118,34 @@ def material(heaviside, mat):
     return {"rho_cp": rho_cp, "k": k, "rho": rho, "cp": cp}
 
 
+def _fill_material(s, mat):
+    s.material = MATERIAL_TYPES[mat.get("type", "gas")]
+    s.gas_rho, s.gas_cp, s.gas_k = mat.get("gas_rho", 0.0), mat.get("gas_cp", 0.0), mat.get("gas_k", 0.0)
+    s.liq_rho = mat.get("liq_rho", 0.0)
+    s.liq_cp = mat.get("liq_cp", 0.0)
+    s.liq_k = mat.get("liq_k", 0.0)
+    s.sol_rho = mat.get("sol_rho", 0.0)
+    s.sol_cp = mat.get("sol_cp", 0.0)
+    s.sol_k = mat.get("sol_k", 0.0)
+    s.solidus = mat.get("solidus", 0.0)
+    s.liquidus = mat.get("liquidus", 0.0)
+    s.solid_liquid_sharp = int(mat.get("solid_liquid_sharp", 0))
+    s.sharp = int(mat.get("sharp", 0))
+    s.volume_specific = int(mat.get("volume_specific", 0))
+    s.consistent_with_evaporation = int(mat.get("consistent_with_evaporation", 0))
+
+
+def material_state(heaviside, temperature, mat):
+    """Material state at (H, T): rho_cp, k, their temperature derivatives and
+    rho, cp, d_rho, d_cp."""
+    s = OracleProblem()
+    _fill_material(s, mat)
+    out = np.zeros(8)
+    assert lib().oracle_material_state(ctypes.byref(s), heaviside, temperature, _dp(out)) == 0
+    return dict(zip(("rho_cp", "k", "d_rho_cp", "d_k", "rho", "cp", "d_rho", "d_cp"),
+                    (float(v) for v in out)))
+
+
 class Problem:
     """Python mirror of ``oracle_problem``; keeps the heaviside and temperature
     arrays alive. ``robin`` (optional dict): convection_faces,
@@ -116,7 +154,8 @@ class Problem:
 
     def __init__(self, n, degree, dt=1e-3, material=None, k_amplitude=0.0, velocity=False,
                  w0=0.2, heaviside=None, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0),
-                 dirichlet_faces=0x3F, robin=None, temperature=None, velocity_value=None):
+                 dirichlet_faces=0x3F, robin=None, temperature=None, velocity_value=None,
+                 temperature_old=None):
         mat = material or {"type": "gas", "gas_rho": 1.0, "gas_cp": 1.0, "gas_k": 1.0}
         n = (n, n, n) if np.isscalar(n) else tuple(n)
         s = OracleProblem()
@@ -125,14 +164,7 @@ class Problem:
         s.lo[:] = list(lo)
         s.hi[:] = list(hi)
         s.dt = dt
-        s.material = 0 if mat.get("type", "gas") == "gas" else 1
-        s.gas_rho, s.gas_cp, s.gas_k = mat["gas_rho"], mat["gas_cp"], mat["gas_k"]
-        s.liq_rho = mat.get("liq_rho", 0.0)
-        s.liq_cp = mat.get("liq_cp", 0.0)
-        s.liq_k = mat.get("liq_k", 0.0)
-        s.sharp = int(mat.get("sharp", 0))
-        s.volume_specific = int(mat.get("volume_specific", 0))
-        s.consistent_with_evaporation = int(mat.get("consistent_with_evaporation", 0))
+        _fill_material(s, mat)
         s.conductivity_field = 1 if k_amplitude else 0
         s.k_amplitude = k_amplitude
         # velocity: False/None, True or "vortex", "constant" (velocity_value)
@@ -149,6 +181,7 @@ class Problem:
         s.emissivity = float(rb.get("emissivity", 0.0))
         s.radiation_t_inf = float(rb.get("radiation_t_inf", 0.0))
         self._T = None
+        self._To = None
         self._h = None
         if heaviside is not None:
             self._h = np.ascontiguousarray(heaviside, dtype=np.float64)
@@ -160,6 +193,8 @@ class Problem:
         assert self.N > 0, "invalid oracle problem"
         if temperature is not None:
             self.set_temperature(temperature)
+        if temperature_old is not None:
+            self.set_temperature_old(temperature_old)
 
     def set_temperature(self, temperature):
         """The temperature the tangent is linearised at (the operator's
@@ -167,6 +202,13 @@ class Problem:
         self._T = np.array(temperature, dtype=np.float64)
         assert self._T.shape == (self.N,)
         self.s.temperature = _dp(self._T)
+
+    def set_temperature_old(self, temperature_old):
+        """The old temperature the tangent of a temperature-dependent material
+        reads (the operator's reference to temperature_old)."""
+        self._To = np.array(temperature_old, dtype=np.float64)
+        assert self._To.shape == (self.N,)
+        self.s.temperature_old = _dp(self._To)
 
     def mask(self):
         m = np.zeros(self.N, dtype=np.uint8)

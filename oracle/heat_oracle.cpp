@@ -182,9 +182,35 @@ namespace
         return false;
     if (!(pb->dt != 0.0))
       return false;
-    if (pb->material == 1 && !pb->heaviside)
+    if (pb->material < 0 || pb->material > 3)
       return false;
+    if ((pb->material == 1 || pb->material == 3) && !pb->heaviside)
+      return false;
+    if (pb->material >= 2)
+      {
+        if (pb->consistent_with_evaporation)
+          return false;
+        if (pb->solid_liquid_sharp ? !(pb->liquidus >= pb->solidus) :
+                                     !(pb->liquidus > pb->solidus))
+          return false;
+      }
     return true;
+  }
+
+  inline bool uses_heaviside(const oracle_problem *pb)
+  {
+    return pb->material == 1 || pb->material == 3;
+  }
+
+  inline bool temperature_dependent(const oracle_problem *pb)
+  {
+    return pb->material >= 2;
+  }
+
+  // the tangent of a temperature-dependent material reads T and T_old
+  bool tangent_ready(const oracle_problem *pb)
+  {
+    return !temperature_dependent(pb) || (pb->temperature && pb->temperature_old);
   }
 
   inline bool is_constrained(const oracle_problem *pb, const Mesh &m, int64_t ix, int64_t iy,
@@ -232,6 +258,95 @@ namespace
       }
   }
 
+  // Material state at one quadrature point, with the temperature derivatives
+  // the tangent needs.
+  struct MatState
+  {
+    double rho_cp, k, d_rho_cp, d_k, rho, cp, d_rho, d_cp;
+  };
+
+  // compute_temperature_dependent_solid_fraction (material.templates.hpp:740-756):
+  // mushy zone: (liquidus - T) / (liquidus - solidus) limited to [0, 1]; sharp:
+  // 1 below the solidus temperature, else 0.
+  inline double solid_fraction(const oracle_problem *pb, double T, double inv_mushy)
+  {
+    if (pb->solid_liquid_sharp)
+      return T < pb->solidus ? 1.0 : 0.0;
+    const double f = (pb->liquidus - T) * inv_mushy;
+    return f > 1.0 ? 1.0 : (f < 0.0 ? 0.0 : f);
+  }
+
+  // Liquid/solid blend of one property by the cubic smoothstep of the solid
+  // fraction fs, and its temperature derivative (compute_solid_liquid_phases_property
+  // and compute_temperature_derivative_of_solid_liquid_phases_property,
+  // material.templates.hpp:540-600; LevelSet::Tools::interpolate_cubic(_derivative),
+  // level_set_tools.hpp:66-90): pure phases take the phase value and derivative 0.
+  inline void solid_liquid(double fs, double inv_mushy, double liquid, double solid, double &v,
+                           double &dv)
+  {
+    if (fs == 0.0 || fs == 1.0)
+      {
+        v = fs == 0.0 ? liquid : solid;
+        dv = 0.0;
+        return;
+      }
+    v = liquid + (solid - liquid) * (fs * fs * (3.0 - 2.0 * fs));
+    dv = -inv_mushy * (solid - liquid) * (6.0 * fs * (1.0 - fs));
+  }
+
+  // Material::compute_parameters_internal for all four diffuse material types
+  // (material.templates.hpp:223-495) and the rho_cp combination of
+  // get_material_parameters_and_derivatives (heat_diffuse_operator.cpp:1167-1210):
+  // rho_cp = rho cp with d(rho cp) = d_cp rho + d_rho cp, or the blended
+  // volume-specific capacity and its derivative.
+  MatState material_state(const oracle_problem *pb, double H, double T)
+  {
+    MatState s;
+    if (!temperature_dependent(pb))
+      {
+        material_at(pb, H, s.rho_cp, s.k);
+        const double w = pb->sharp ? (H > 0.5 ? 1.0 : 0.0) : H;
+        s.rho = pb->material == 0 ? pb->gas_rho :
+                (pb->consistent_with_evaporation ? 1. / ((1. - H) / pb->gas_rho + H / pb->liq_rho) :
+                                                   (1. - w) * pb->gas_rho + w * pb->liq_rho);
+        s.cp = pb->material == 0 ? pb->gas_cp : (1. - w) * pb->gas_cp + w * pb->liq_cp;
+        s.d_rho_cp = s.d_k = s.d_rho = s.d_cp = 0.0;
+        return s;
+      }
+    const double inv_mushy = 1.0 / (pb->liquidus - pb->solidus);
+    const double fs = solid_fraction(pb, T, inv_mushy);
+    double vc, dvc;
+    solid_liquid(fs, inv_mushy, pb->liq_k, pb->sol_k, s.k, s.d_k);
+    solid_liquid(fs, inv_mushy, pb->liq_cp, pb->sol_cp, s.cp, s.d_cp);
+    solid_liquid(fs, inv_mushy, pb->liq_rho, pb->sol_rho, s.rho, s.d_rho);
+    solid_liquid(fs, inv_mushy, pb->liq_rho * pb->liq_cp, pb->sol_rho * pb->sol_cp, vc, dvc);
+    if (pb->material == 3)
+      {
+        // gas / (liquid-solid) blend by the heaviside; the derivatives carry the
+        // blend weight (material.templates.hpp:378-495, 629-709)
+        const double w = pb->sharp ? (H > 0.5 ? 1.0 : 0.0) : H;
+        s.k = (1. - w) * pb->gas_k + w * s.k;
+        s.d_k *= w;
+        s.cp = (1. - w) * pb->gas_cp + w * s.cp;
+        s.d_cp *= w;
+        s.rho = (1. - w) * pb->gas_rho + w * s.rho;
+        s.d_rho *= w;
+        vc = (1. - w) * (pb->gas_rho * pb->gas_cp) + w * vc;
+        dvc *= w;
+      }
+    if (pb->volume_specific)
+      {
+        s.rho_cp = vc;
+        s.d_rho_cp = dvc;
+      }
+    else
+      {
+        s.rho_cp = s.rho * s.cp;
+        s.d_rho_cp = s.d_cp * s.rho + s.d_rho * s.cp;
+      }
+    return s;
+  }
+
   // Prescribed velocity at a quadrature point: 1 = the vortex field (analytic at
   // q), 2 = a constant field (what VectorTools::interpolate of a constant
   // prescribed_velocity function gives at any point).
@@ -276,10 +391,41 @@ namespace
     int c[3];
   };
 
+  // Coefficient fields of one cell, read plain (read_dof_values_plain): the
+  // heaviside (materials 1, 3) and, for a temperature-dependent material, the
+  // linearisation temperature and the old temperature.
+  struct CellFields
+  {
+    double H[MAXN * MAXN * MAXN], T[MAXN * MAXN * MAXN], To[MAXN * MAXN * MAXN];
+  };
+
+  // values (and gradients, when G is given) of a nodal cell field at the Gauss points
+  void evaluate_field(const Tables &t, const double *u, double *U, double (*G)[MAXN * MAXN * MAXN])
+  {
+    const int n = t.n;
+    double tS[MAXN * MAXN * MAXN], tD[MAXN * MAXN * MAXN];
+    double tSS[MAXN * MAXN * MAXN], tSD[MAXN * MAXN * MAXN], tDS[MAXN * MAXN * MAXN];
+    apply_dir<false>(t.S, n, 0, u, tS);
+    apply_dir<false>(t.S, n, 1, tS, tSS);
+    apply_dir<false>(t.S, n, 2, tSS, U);
+    if (G)
+      {
+        apply_dir<false>(t.D, n, 0, u, tD);
+        apply_dir<false>(t.D, n, 1, tS, tSD);
+        apply_dir<false>(t.S, n, 1, tD, tDS);
+        apply_dir<false>(t.D, n, 2, tSS, G[2]);
+        apply_dir<false>(t.S, n, 2, tSD, G[1]);
+        apply_dir<false>(t.S, n, 2, tDS, G[0]);
+      }
+  }
+
   // tangent_local_cell_operation for one cell (heat_diffuse_operator.cpp:955-1086):
   // evaluate values|gradients, quadrature-point physics, integrate values|gradients.
+  // With a temperature-dependent material the linearisation adds, at q,
+  //   val      += dt^-1 d(rho_cp)/dT (T - T_old) u  (+ d(rho_cp)/dT (grad T . v) u)
+  //   val_grad += dk/dT grad T u                                     (:1044-1056)
   void local_apply(const oracle_problem *pb, const Tables &t, const Mesh &m, const CellGeom &g,
-                   const double *u, const double *Hloc, double *y)
+                   const double *u, const CellFields &fl, double *y)
   {
     const int n = t.n;
     const int n3 = n * n * n;
@@ -299,12 +445,14 @@ namespace
     apply_dir<false>(t.S, n, 2, tSD, Gy);
     apply_dir<false>(t.S, n, 2, tDS, Gx);
 
-    if (pb->material == 1)
+    const bool td = temperature_dependent(pb);
+    if (uses_heaviside(pb))
+      evaluate_field(t, fl.H, Hq, nullptr); // heaviside_eval.evaluate(values)
+    double TU[MAXN * MAXN * MAXN], TG[3][MAXN * MAXN * MAXN], ToU[MAXN * MAXN * MAXN];
+    if (td)
       {
-        // heaviside_eval.read_dof_values_plain + evaluate(values)
-        apply_dir<false>(t.S, n, 0, Hloc, tS);
-        apply_dir<false>(t.S, n, 1, tS, tSS);
-        apply_dir<false>(t.S, n, 2, tSS, Hq);
+        evaluate_field(t, fl.T, TU, TG);      // T_new_eval.evaluate(values | gradients)
+        evaluate_field(t, fl.To, ToU, nullptr); // T_old_eval.evaluate(values)
       }
 
     const double hx = m.h[0], hy = m.h[1], hz = m.h[2];
@@ -324,24 +472,37 @@ namespace
             const double zq = m.lo[2] + (g.c[2] + t.X[qz]) * hz;
             const double JxW = t.W[qx] * t.W[qy] * t.W[qz] * jac_det;
 
-            double rho_cp, k;
-            material_at(pb, pb->material == 1 ? Hq[q] : 0.0, rho_cp, k);
+            const MatState ms = material_state(pb, uses_heaviside(pb) ? Hq[q] : 0.0,
+                                               td ? TU[q] : 0.0);
+            const double rho_cp = ms.rho_cp;
+            double k = ms.k;
             if (pb->conductivity_field == 1)
               k += pb->k_amplitude * std::sin(two_pi * xq) * std::sin(two_pi * yq) *
                    std::sin(two_pi * zq);
 
             const double gx = Gx[q] / hx, gy = Gy[q] / hy, gz = Gz[q] / hz;
             double val = dt_inv * rho_cp * U[q];
+            double fx = k * gx, fy = k * gy, fz = k * gz;
+            double v[3] = {0.0, 0.0, 0.0};
             if (pb->velocity != 0)
               {
-                double v[3];
                 velocity_at(pb, xq, yq, v);
                 val += rho_cp * (gx * v[0] + gy * v[1] + gz * v[2]);
               }
+            if (td)
+              {
+                const double tgx = TG[0][q] / hx, tgy = TG[1][q] / hy, tgz = TG[2][q] / hz;
+                val += dt_inv * ms.d_rho_cp * (TU[q] - ToU[q]) * U[q];
+                fx += ms.d_k * tgx * U[q];
+                fy += ms.d_k * tgy * U[q];
+                fz += ms.d_k * tgz * U[q];
+                if (pb->velocity != 0)
+                  val += ms.d_rho_cp * (tgx * v[0] + tgy * v[1] + tgz * v[2]) * U[q];
+              }
             V[q] = val * JxW;
-            Fx[q] = k * gx * JxW / hx;
-            Fy[q] = k * gy * JxW / hy;
-            Fz[q] = k * gz * JxW / hz;
+            Fx[q] = fx * JxW / hx;
+            Fy[q] = fy * JxW / hy;
+            Fz[q] = fz * JxW / hz;
           }
 
     // integrate(values | gradients)
@@ -377,6 +538,24 @@ namespace
     return ix + m.N1[0] * (iy + m.N1[1] * iz);
   }
 
+  // read_dof_values_plain of the heaviside and, for a temperature-dependent
+  // material, of T and T_old
+  void gather_fields(const oracle_problem *pb, const Mesh &m, const CellGeom &g, const double *T,
+                     const double *To, CellFields &fl)
+  {
+    const int n = m.p + 1;
+    for (int c = 0; c < n; ++c)
+      for (int b = 0; b < n; ++b)
+        for (int a = 0; a < n; ++a)
+          {
+            const int l = (c * n + b) * n + a;
+            const int64_t I = node_index(m, g, a, b, c);
+            fl.H[l] = uses_heaviside(pb) ? pb->heaviside[I] : 0.0;
+            fl.T[l] = temperature_dependent(pb) ? T[I] : 0.0;
+            fl.To[l] = temperature_dependent(pb) ? To[I] : 0.0;
+          }
+  }
+
   template <typename Body>
   void for_cells_coloured(const Mesh &m, Body body)
   {
@@ -409,7 +588,9 @@ namespace
     const int n = t.n;
     std::fill(dst, dst + m.N, 0.0);
     for_cells_coloured(m, [&](const CellGeom &g) {
-      double u[MAXN * MAXN * MAXN], Hl[MAXN * MAXN * MAXN], y[MAXN * MAXN * MAXN];
+      double u[MAXN * MAXN * MAXN], y[MAXN * MAXN * MAXN];
+      CellFields fl;
+      gather_fields(pb, m, g, pb->temperature, pb->temperature_old, fl);
       for (int c = 0; c < n; ++c)
         for (int b = 0; b < n; ++b)
           for (int a = 0; a < n; ++a)
@@ -420,9 +601,8 @@ namespace
                                               (int64_t)g.c[1] * m.p + b, (int64_t)g.c[2] * m.p + c);
               // read_dof_values: homogeneous constraints resolve to 0
               u[l] = con ? 0.0 : src[I];
-              Hl[l] = pb->material == 1 ? pb->heaviside[I] : 0.0;
             }
-      local_apply(pb, t, m, g, u, Hl, y);
+      local_apply(pb, t, m, g, u, fl, y);
       for (int c = 0; c < n; ++c)
         for (int b = 0; b < n; ++b)
           for (int a = 0; a < n; ++a)
@@ -437,8 +617,8 @@ namespace
     robin_vmult(pb, t, m, src, dst);
   }
 
-  // rhs_cell_loop restated (source/heat/heat_diffuse_operator.cpp:393-529) for
-  // temperature-independent materials without evaporation: T_new, T_old and
+  // rhs_cell_loop restated (source/heat/heat_diffuse_operator.cpp:393-529)
+  // without evaporation, the material evaluated at T_new(q): T_new, T_old and
   // the heat source read plain (constrained values included), at q
   //   val      = dt^-1 rho_cp (T_new - T_old) - f  (+ rho_cp grad T_new . u)
   //   val_grad = k grad T_new
@@ -472,7 +652,7 @@ namespace
     apply_dir<false>(t.S, n, 0, fl, tS);
     apply_dir<false>(t.S, n, 1, tS, tSS);
     apply_dir<false>(t.S, n, 2, tSS, Fq);
-    if (pb->material == 1)
+    if (uses_heaviside(pb))
       {
         apply_dir<false>(t.S, n, 0, Hloc, tS);
         apply_dir<false>(t.S, n, 1, tS, tSS);
@@ -496,8 +676,10 @@ namespace
             const double zq = m.lo[2] + (g.c[2] + t.X[qz]) * hz;
             const double JxW = t.W[qx] * t.W[qy] * t.W[qz] * jac_det;
 
-            double rho_cp, k;
-            material_at(pb, pb->material == 1 ? Hq[q] : 0.0, rho_cp, k);
+            // get_material_parameters (heat_diffuse_operator.cpp:1133-1165) at T_new(q)
+            const MatState ms = material_state(pb, uses_heaviside(pb) ? Hq[q] : 0.0, U[q]);
+            const double rho_cp = ms.rho_cp;
+            double k = ms.k;
             if (pb->conductivity_field == 1)
               k += pb->k_amplitude * std::sin(two_pi * xq) * std::sin(two_pi * yq) *
                    std::sin(two_pi * zq);
@@ -832,9 +1014,19 @@ int oracle_material(int material, int sharp, int volume_specific, int consistent
   return 0;
 }
 
+int oracle_material_state(const oracle_problem *pb, double H, double T, double *out)
+{
+  if (!pb || !out || pb->material < 0 || pb->material > 3)
+    return -1;
+  const MatState s = material_state(pb, H, T);
+  const double v[8] = {s.rho_cp, s.k, s.d_rho_cp, s.d_k, s.rho, s.cp, s.d_rho, s.d_cp};
+  std::memcpy(out, v, sizeof(v));
+  return 0;
+}
+
 int oracle_vmult(const oracle_problem *pb, const double *src, double *dst)
 {
-  if (!valid(pb) || !robin_ready(pb))
+  if (!valid(pb) || !robin_ready(pb) || !tangent_ready(pb))
     return -1;
   const Tables t = make_tables(pb->degree);
   const Mesh m = make_mesh(pb);
@@ -868,7 +1060,7 @@ int oracle_create_rhs(const oracle_problem *pb, const double *temperature,
             tn[l] = temperature[I];
             to[l] = temperature_old[I];
             fl[l] = heat_source ? heat_source[I] : 0.0;
-            Hl[l] = pb->material == 1 ? pb->heaviside[I] : 0.0;
+            Hl[l] = uses_heaviside(pb) ? pb->heaviside[I] : 0.0;
           }
     local_rhs(pb, t, m, g, tn, to, fl, Hl, y);
     for (int c = 0; c < n; ++c)
@@ -943,7 +1135,7 @@ int oracle_l2_norm(const oracle_problem *pb, const double *u, double *norm)
 // (heat_diffuse_operator.cpp:332-358).
 int oracle_diagonal(const oracle_problem *pb, double *diag)
 {
-  if (!valid(pb) || !robin_ready(pb))
+  if (!valid(pb) || !robin_ready(pb) || !tangent_ready(pb))
     return -1;
   const Tables t = make_tables(pb->degree);
   const Mesh m = make_mesh(pb);
@@ -951,17 +1143,15 @@ int oracle_diagonal(const oracle_problem *pb, double *diag)
   const int n3 = n * n * n;
   std::fill(diag, diag + m.N, 0.0);
   for_cells_coloured(m, [&](const CellGeom &g) {
-    double u[MAXN * MAXN * MAXN], Hl[MAXN * MAXN * MAXN], y[MAXN * MAXN * MAXN];
+    double u[MAXN * MAXN * MAXN], y[MAXN * MAXN * MAXN];
     double d[MAXN * MAXN * MAXN];
-    for (int c = 0; c < n; ++c)
-      for (int b = 0; b < n; ++b)
-        for (int a = 0; a < n; ++a)
-          Hl[(c * n + b) * n + a] = pb->material == 1 ? pb->heaviside[node_index(m, g, a, b, c)] : 0.0;
+    CellFields fl;
+    gather_fields(pb, m, g, pb->temperature, pb->temperature_old, fl);
     for (int j = 0; j < n3; ++j)
       {
         std::fill(u, u + n3, 0.0);
         u[j] = 1.0;
-        local_apply(pb, t, m, g, u, Hl, y);
+        local_apply(pb, t, m, g, u, fl, y);
         d[j] = y[j];
       }
     for (int c = 0; c < n; ++c)
@@ -1013,7 +1203,7 @@ int oracle_solve(const oracle_problem *pb, int solver, int restart, const double
                  int *iterations, double *initial_res, double *final_res, double *history,
                  int history_len)
 {
-  if (!valid(pb) || !robin_ready(pb) || (solver == 1 && restart < 1))
+  if (!valid(pb) || !robin_ready(pb) || !tangent_ready(pb) || (solver == 1 && restart < 1))
     return -1;
   const Tables t = make_tables(pb->degree);
   const Mesh m = make_mesh(pb);

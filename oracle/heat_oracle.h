@@ -11,8 +11,12 @@
  *     source/core/finite_element_data.cpp:61-65);
  *   - HeatDiffuseMultiPhaseOperator::vmult / tangent_cell_loop /
  *     tangent_local_cell_operation (source/heat/heat_diffuse_operator.cpp:
- *     179-241, 955-1086), with the gas and gas_liquid Material models
- *     (include/meltpooldg/core/material.templates.hpp:104-136,223-295,510-539);
+ *     179-241, 955-1086), with the gas, gas_liquid, liquid_solid and
+ *     gas_liquid_solid Material models and their temperature derivatives
+ *     (include/meltpooldg/core/material.templates.hpp:104-136,223-495,510-756;
+ *     get_material_parameters(_and_derivatives), heat_diffuse_operator.cpp:
+ *     1133-1210), including the solidification terms of the tangent
+ *     (:1003-1014, 1044-1056) and of the residual;
  *   - compute_inverse_diagonal_from_matrixfree, column-by-column as
  *     MatrixFreeTools::compute_diagonal does, plus the 1e-10 guard
  *     (heat_diffuse_operator.cpp:271-284, 297-391);
@@ -42,7 +46,9 @@ extern "C" {
 
 enum { ORACLE_MAX_DEGREE = 7 };
 
-/* material: 0 = gas (constant), 1 = gas_liquid (nodal heaviside) */
+/* material: 0 = gas (constant), 1 = gas_liquid (nodal heaviside),
+ *           2 = liquid_solid (solid fraction from the temperature),
+ *           3 = gas_liquid_solid (heaviside and temperature) */
 /* conductivity_field: 0 = from material, 1 = material k + amp*s(x)s(y)s(z), s = sin(2 pi t) */
 /* velocity: 0 = none, 1 = vortex (-sin^2(pi x) sin(2 pi y), sin(2 pi x) sin^2(pi y), w0),
  *           2 = constant velocity_value */
@@ -74,6 +80,15 @@ typedef struct {
    * Dirichlet entries resolved to 0); needed when radiation_faces != 0 */
   const double *temperature;
   double velocity_value[3]; /* velocity == 2 */
+  /* materials 2, 3: the solid phase and the solid fraction's temperature law
+   * (mushy zone: clamp((liquidus - T) / (liquidus - solidus), 0, 1); sharp:
+   * T < solidus). No apparent capacity (latent heat of fusion 0). */
+  double sol_rho, sol_cp, sol_k;
+  double solidus, liquidus;
+  int solid_liquid_sharp;
+  /* materials 2, 3: the tangent's cells read `temperature` and this old
+   * temperature plain (read_dof_values_plain, heat_diffuse_operator.cpp:1004-1014) */
+  const double *temperature_old;
 } oracle_problem;
 
 #define ORACLE_STEFAN_BOLTZMANN 5.670374419e-8
@@ -86,6 +101,10 @@ int oracle_material(int material, int sharp, int volume_specific, int consistent
                     double gas_rho, double gas_cp, double gas_k, double liq_rho, double liq_cp,
                     double liq_k, double heaviside, double *rho_cp, double *k, double *rho,
                     double *cp);
+/* Material state at heaviside H and temperature T: out = {rho_cp, k, d rho_cp/dT,
+ * dk/dT, rho, cp, d rho/dT, d cp/dT} (MaterialParameterValues and the
+ * rho_cp combination of get_material_parameters_and_derivatives) */
+int oracle_material_state(const oracle_problem *pb, double H, double T, double *out);
 int oracle_vmult(const oracle_problem *pb, const double *src, double *dst);
 /* dst += residual right-hand side (create_rhs); heat_source may be NULL */
 int oracle_create_rhs(const oracle_problem *pb, const double *temperature,

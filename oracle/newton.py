@@ -12,7 +12,8 @@ infrastructure only).
   from a zero initial update, the update is applied only when not converged,
   constraints of the update are homogeneous.
 * The operator reads the current solution as its linearisation point (the
-  radiation tangent, heat_diffuse_operator.cpp:1096-1099), and the Jacobi
+  radiation tangent, heat_diffuse_operator.cpp:1096-1099; the solidification
+  terms, :1003-1014, which also read the old temperature), and the Jacobi
   preconditioner is updated once per time step, at the start value
   (HeatDiffuseOperation::solve, source/heat/heat_diffuse_operation.cpp:251-253).
 * NonlinearSolverData defaults (include/meltpooldg/linear_algebra/
@@ -48,6 +49,7 @@ def newton_time_step(pb, T, T_old, heat_source=None, data=None, solver="GMRES", 
     data = data or NewtonData()
     T = np.array(T, dtype=float)
     pb.set_temperature(T)
+    pb.set_temperature_old(T_old)
     inv_diag = pb.inverse_diagonal()
     res_tol, corr_tol = data.residual_tolerance, data.field_correction_tolerance
     max_total = data.max_nonlinear_iterations + data.max_nonlinear_iterations_alt

@@ -51,8 +51,12 @@ def engine_operator(case, precision="fp64"):
         type=m.get("type", "gas"),
         gas=PhaseData(m["gas_rho"], m["gas_cp"], m["gas_k"]),
         liquid=PhaseData(m.get("liq_rho", 1.0), m.get("liq_cp", 1.0), m.get("liq_k", 1.0)),
+        solid=PhaseData(m.get("sol_rho", 1.0), m.get("sol_cp", 1.0), m.get("sol_k", 1.0)),
         two_phase_fluid_properties_transition_type="sharp" if m.get("sharp") else "smooth",
-        use_volume_specific_thermal_capacity_for_phase_interpolation=bool(m.get("volume_specific")))
+        use_volume_specific_thermal_capacity_for_phase_interpolation=bool(m.get("volume_specific")),
+        solid_liquid_properties_transition_type=("sharp" if m.get("solid_liquid_sharp") else
+                                                 "mushy_zone"),
+        solidus_temperature=m.get("solidus", 0.0), liquidus_temperature=m.get("liquidus", 0.0))
     op = HeatDiffuseOperator(case["n"], case["degree"], material=mat, precision=precision,
                              conductivity_amplitude=case["k_amplitude"],
                              velocity=("constant" if case["velocity"] == "constant" else

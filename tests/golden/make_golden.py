@@ -7,7 +7,10 @@
 * material_known_answers.json: the values printed by the reference's own
   unit test unit_tests/material.cc (unit_tests/material.output), transcribed
   as data: MaterialData gas k=10 cp=0 rho=1000, liquid k=40 cp=1000 rho=8000,
-  evaluated at heaviside 0.2 (double) and 0.6 (VectorizedArray).
+  solid k=30 cp=900 rho=7000, mushy zone between solidus 0 and liquidus 100,
+  smooth two-phase blend, evaluated at heaviside 0.2 / temperature 20
+  (double) and heaviside 0.6 / temperature 60 (VectorizedArray), with the
+  temperature derivatives of the liquid_solid and gas_liquid_solid rows.
 """
 import json
 import os
@@ -55,10 +58,17 @@ def main():
                  mask=mask)
         print(name, src.size)
 
+
+def write_known_answers():
     known = {
         "material_data": {
             "gas": {"thermal_conductivity": 10.0, "specific_heat_capacity": 0.0, "density": 1000.0},
             "liquid": {"thermal_conductivity": 40.0, "specific_heat_capacity": 1000.0, "density": 8000.0},
+            "solid": {"thermal_conductivity": 30.0, "specific_heat_capacity": 900.0, "density": 7000.0},
+            "solid_liquid_properties_transition_type": "mushy_zone",
+            "two_phase_fluid_properties_transition_type": "smooth",
+            "solidus_temperature": 0.0,
+            "liquidus_temperature": 100.0,
         },
         "source": "unit_tests/material.cc + unit_tests/material.output",
         "rows": [
@@ -70,6 +80,14 @@ def main():
              "rho": 1212.12},
             {"type": "gas_liquid_consistent_with_evaporation", "heaviside": 0.6, "k": 28, "cp": 600,
              "rho": 2105.26},
+            {"type": "liquid_solid", "heaviside": 0.0, "temperature": 20.0, "k": 31.04, "cp": 910.4,
+             "rho": 7104, "d_k": 0.096, "d_cp": 0.96, "d_rho": 9.6},
+            {"type": "liquid_solid", "heaviside": 0.0, "temperature": 60.0, "k": 36.48, "cp": 964.8,
+             "rho": 7648, "d_k": 0.144, "d_cp": 1.44, "d_rho": 14.4},
+            {"type": "gas_liquid_solid", "heaviside": 0.2, "temperature": 20.0, "k": 14.208,
+             "cp": 182.08, "rho": 2220.8, "d_k": 0.0192, "d_cp": 0.192, "d_rho": 1.92},
+            {"type": "gas_liquid_solid", "heaviside": 0.6, "temperature": 60.0, "k": 25.888,
+             "cp": 578.88, "rho": 4988.8, "d_k": 0.0864, "d_cp": 0.864, "d_rho": 8.64},
         ],
         "print_precision_digits": 6,
     }
@@ -79,3 +97,4 @@ def main():
 
 if __name__ == "__main__":
     main()
+    write_known_answers()
