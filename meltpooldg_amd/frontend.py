@@ -12,18 +12,26 @@
   (source/core/material_data.cpp), the application-specific and the
   case-specific parameters. An unknown key in a modelled section is an error,
   as in the reference's ParameterHandler; settings that ask for physics this
-  engine does not run (cut operator, solidification, evaporation, laser, AMR,
-  time-dependent boundary conditions, least-squares predictor, non-Jacobi
-  preconditioners) raise ``UnsupportedFeature`` naming the key.
+  engine does not run (cut operator, evaporation, laser, AMR, the apparent
+  capacity of a latent heat of fusion, time-dependent boundary conditions,
+  least-squares predictor) raise ``UnsupportedFeature`` naming the key. The
+  ILU preconditioner, which needs an assembled sparse matrix, runs as the
+  Jacobi (Diagonal) preconditioner: the Newton solution does not depend on the
+  preconditioner beyond the linear tolerance, only the Krylov iteration counts
+  do, and ``run`` says so in its log.
 * ``setup(parameters)`` builds the simulation case (the reference's
   SimulationCase subclasses): ``unidirectional_heat_transfer``
   (applications/mp-heat-transfer/cases/unidirectional_heat_transfer/
   unidirectional_heat_transfer.cpp) in 1D and 2D, extruded to the engine's 3D
-  box (below).
+  box (below), with the material type of determine_material_type
+  (source/core/material.cpp:7-39): gas, gas_liquid, liquid_solid or
+  gas_liquid_solid.
 * ``run(parameters)`` is HeatTransferApplication::run
   (applications/mp-heat-transfer/heat_transfer_application.cpp:44-135) on the
   device: the time iterator (source/time_integration/time_iterator.cpp:26-67),
-  the prescribed fields, and HeatDiffuseOperation::solve per step, printing
+  the prescribed fields (a time-dependent heaviside re-interpolated at each
+  step's new time, heat_transfer_application.cpp:67-70, 470-480), and
+  HeatDiffuseOperation::solve per step, printing
   the time increment and "Newton Raphson solver converged: ||solution|| = ..."
   lines as the reference does.
 
@@ -264,17 +272,27 @@ def check_supported(prm):
                                  "operator is built")
     if h.enable_time_dependent_bc:
         raise UnsupportedFeature("heat/enable time dependent bc: not built")
-    if prm.application_specific.do_solidification or prm.case_specific.get("do solidification"):
-        raise UnsupportedFeature("do solidification: temperature-dependent (solidification) "
-                                 "materials are not built")
+    if prm.application_specific.do_solidification:
+        m = prm.material
+        if m.latent_heat_of_fusion > 0.0:
+            raise UnsupportedFeature("material/latent heat of fusion > 0: the apparent capacity "
+                                     "is not built")
+        if m.solid_liquid_properties_transition_type not in ("mushy_zone", "sharp"):
+            raise UnsupportedFeature("material/solid liquid properties transition type = "
+                                     f"{m.solid_liquid_properties_transition_type!r}")
+        if (m.solid_liquid_properties_transition_type == "mushy_zone" and
+                not m.liquidus_temperature > m.solidus_temperature):
+            raise UnsupportedFeature("material: the mushy zone needs liquidus temperature > "
+                                     "solidus temperature")
     if h.predictor.type not in ("none", "zero", "linear_extrapolation"):
         raise UnsupportedFeature(f"heat/predictor/type = {h.predictor.type!r}: not built")
     ls = h.linear_solver
     if ls.solver_type not in ("CG", "GMRES"):
         raise UnsupportedFeature(f"heat/linear solver/solver type = {ls.solver_type!r}")
-    if ls.preconditioner_type not in ("Identity", "Diagonal"):
+    if ls.preconditioner_type not in ("Identity", "Diagonal", "ILU"):
         raise UnsupportedFeature(f"heat/linear solver/preconditioner type = "
-                                 f"{ls.preconditioner_type!r}: only Identity and Diagonal are built")
+                                 f"{ls.preconditioner_type!r}: Identity and Diagonal are built "
+                                 "(ILU runs as Diagonal)")
     if not ls.do_matrix_free:
         raise UnsupportedFeature("heat/linear solver/do matrix free = false: the engine is "
                                  "matrix-free")
@@ -330,6 +348,7 @@ class CaseSetup:
     dirichlet_faces: int
     temperature0: object       # lattice vector
     coords: tuple              # lattice coordinates per direction
+    heaviside_at: object = None  # t -> lattice vector, for a time-dependent heaviside
 
     def norm_scale(self):
         """Factor from the 3D L2 norm to the case's own dimension."""
@@ -350,9 +369,14 @@ def _lattice_coords(n_cells, degree, box):
 def _unidirectional_heat_transfer(prm):
     """SimulationUnidirectionalHeatTransfer (unidirectional_heat_transfer.cpp):
     the box [0, 0.1]^dim, refined globally; Robin conditions on the right face
-    (dim 1: the lower one); T0 = 1000 with a Robin face, else T0 = x; the
+    (dim 1: the lower one); T0 linear from 1960 to 1980 in x with
+    solidification, else T0 = 1000 with a Robin face, else T0 = x; the
     prescribed velocity (-v, 0) if v != 0; for two phases the heaviside of the
-    horizontal level set 0.05 - y with eps = 0.01."""
+    horizontal level set 0.05 - y, or with solidification of the covected
+    vertical one 0.1 * 2/3 - x - v t (time-dependent), both with eps = 0.01.
+    The material type follows determine_material_type
+    (source/core/material.cpp:7-39) from the heaviside's presence and the
+    application's do solidification."""
     from .heat import MaterialData, PhaseData
     cs = {"do solidification": False, "do two phase": False, "velocity": 0.0}
     for k, v in prm.case_specific.items():
@@ -383,7 +407,10 @@ def _unidirectional_heat_transfer(prm):
     conv_faces = robin_face if conv.convection_coefficient > 0.0 else 0
     coords = _lattice_coords(n_cells, degree, box)
     Z, Y, X = np.meshgrid(coords[2], coords[1], coords[0], indexing="ij")
-    if rad.emissivity > 0.0 or conv.convection_coefficient > 0.0:
+    if cs["do solidification"]:
+        # LinearTemp(1960, 1980)
+        T0 = (1960.0 + (1980.0 - 1960.0) * (X - x_min) / (x_max - x_min)).reshape(-1)
+    elif rad.emissivity > 0.0 or conv.convection_coefficient > 0.0:
         T0 = np.full(X.size, 1000.0)
     else:
         # LinearTemp(0, 0.1): left + (right - left) (x - x_min) / (x_max - x_min)
@@ -392,22 +419,38 @@ def _unidirectional_heat_transfer(prm):
     velocity = (-v, 0.0, 0.0) if v != 0.0 else None
     two_phase = bool(cs["do two phase"])
     heaviside = None
+    heaviside_at = None
     if two_phase:
-        if dim != 2:
-            raise UnsupportedFeature("do two phase in 1D: the horizontal level set needs y")
-        heaviside = smoothed_heaviside(x_max / 2 - Y, 0.01).reshape(-1)
+        if cs["do solidification"]:
+            # CovectedVerticalLevelSetHeaviside(v): level 0.1 * 2/3, moving with v t
+            Xf = X.reshape(-1)
+
+            def heaviside_at(t):
+                return smoothed_heaviside(x_max * 2.0 / 3.0 - Xf - v * t, 0.01)
+
+            heaviside = heaviside_at(prm.time_stepping.start_time)
+        else:
+            if dim != 2:
+                raise UnsupportedFeature("do two phase in 1D: the horizontal level set needs y")
+            heaviside = smoothed_heaviside(x_max / 2 - Y, 0.01).reshape(-1)
+    solid = bool(prm.application_specific.do_solidification)
     m = prm.material
     phase = lambda ph: PhaseData(ph.density, ph.specific_heat_capacity, ph.thermal_conductivity)
+    mtype = {(False, False): "gas", (True, False): "gas_liquid", (False, True): "liquid_solid",
+             (True, True): "gas_liquid_solid"}[(two_phase, solid)]
     material = MaterialData(
-        type="gas_liquid" if two_phase else "gas", gas=phase(m.gas), liquid=phase(m.liquid),
+        type=mtype, gas=phase(m.gas), liquid=phase(m.liquid), solid=phase(m.solid),
         two_phase_fluid_properties_transition_type=m.two_phase_fluid_properties_transition_type,
         use_volume_specific_thermal_capacity_for_phase_interpolation=(
-            prm.heat.diffuse.use_volume_specific_thermal_capacity_for_phase_interpolation))
+            prm.heat.diffuse.use_volume_specific_thermal_capacity_for_phase_interpolation),
+        solid_liquid_properties_transition_type=m.solid_liquid_properties_transition_type,
+        solidus_temperature=m.solidus_temperature, liquidus_temperature=m.liquidus_temperature,
+        latent_heat_of_fusion=m.latent_heat_of_fusion)
     return CaseSetup(dimension=dim, n_cells=n_cells, degree=degree, box=box, extrusion=depth,
                      material=material, two_phase=two_phase, heaviside=heaviside,
                      velocity=velocity, convection=conv, radiation=rad,
                      convection_faces=conv_faces, radiation_faces=rad_faces, dirichlet_faces=0,
-                     temperature0=T0, coords=coords)
+                     temperature0=T0, coords=coords, heaviside_at=heaviside_at)
 
 
 CASES = {"unidirectional_heat_transfer": _unidirectional_heat_transfer}
@@ -478,8 +521,13 @@ def run(prm, log=print, max_steps=None):
     if case.heaviside is not None:
         op.set_heaviside(torch.tensor(case.heaviside, dtype=torch.float64, device="cuda"))
     T0 = torch.tensor(case.temperature0, dtype=torch.float64, device="cuda")
-    heat = HeatDiffuseOperation(op, T0, linear_solver=prm.heat.linear_solver,
-                                nlsolve=prm.heat.nlsolve, predictor=prm.heat.predictor.type)
+    ls = prm.heat.linear_solver
+    if ls.preconditioner_type == "ILU":
+        from dataclasses import replace
+        log("note: heat/linear solver/preconditioner type ILU runs as Diagonal (Jacobi)")
+        ls = replace(ls, preconditioner_type="Diagonal")
+    heat = HeatDiffuseOperation(op, T0, linear_solver=ls, nlsolve=prm.heat.nlsolve,
+                                predictor=prm.heat.predictor.type)
     it = TimeIterator(prm.time_stepping)
     out = []
     while not it.is_finished():
@@ -489,6 +537,9 @@ def run(prm, log=print, max_steps=None):
         dt = it.compute_next_time_increment()
         log(f"| Time increment # {it.n_steps:<6d} t = {t0:.4e} to {it.current_time:.4e} "
             f"( dt = {dt:.4e} )")
+        if case.heaviside_at is not None:
+            op.set_heaviside(torch.tensor(case.heaviside_at(it.current_time), dtype=torch.float64,
+                                          device="cuda"))
         rep = heat.advance(dt, it.old_increment)
         norm = heat.l2_norm() * case.norm_scale()
         log(f"| Newton Raphson solver converged: ||solution|| = {norm:.5e}")

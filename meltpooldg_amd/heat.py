@@ -356,6 +356,9 @@ def _solver_params(lib, data, use_diag=None):
     lib.mpdg_solver_params_default(ctypes.byref(prm))
     prm.solver_type = {"CG": 0, "GMRES": 1}[data.solver_type]
     if use_diag is None:
+        if data.preconditioner_type not in ("Identity", "Diagonal"):
+            raise ValueError(f"preconditioner type {data.preconditioner_type!r} is not built "
+                             "(Identity, Diagonal)")
         use_diag = data.preconditioner_type == "Diagonal"
     prm.preconditioner_type = 3 if use_diag else 0
     prm.max_iterations = int(data.max_iterations)

@@ -203,6 +203,12 @@ class Problem:
         assert self._T.shape == (self.N,)
         self.s.temperature = _dp(self._T)
 
+    def set_heaviside(self, heaviside):
+        """Replace the heaviside field (a time-dependent prescribed heaviside)."""
+        self._h = np.ascontiguousarray(heaviside, dtype=np.float64)
+        assert self._h.shape == (self.N,)
+        self.s.heaviside = _dp(self._h)
+
     def set_temperature_old(self, temperature_old):
         """The old temperature the tangent of a temperature-dependent material
         reads (the operator's reference to temperature_old)."""

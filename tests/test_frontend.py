@@ -3,9 +3,11 @@
 * Every regression input of the reference's unidirectional_heat_transfer case
   (tests/golden/unidirectional_cases.json, written by
   tests/golden/make_unidirectional_cases.py from the reference's own test JSONs
-  and outputs) parses; the seven that need only the built physics set up, the
-  ten that need the cut operator, solidification, evaporation or AMR are
-  rejected with UnsupportedFeature naming the key.
+  and outputs) parses; the nine that need only the built physics set up
+  (two of them with solidification, the covected time-dependent heaviside
+  and the ILU preconditioner, which runs as Diagonal), the eight that need
+  the cut operator, evaporation or AMR are rejected with UnsupportedFeature
+  naming the key.
 * The parameter structures' defaults equal the ones
   applications/mp-heat-transfer/parameters.schema.json lists (kept in the same
   fixture), except where the reference's own constructors override them
@@ -38,7 +40,9 @@ SUPPORTED = ["unidirectional_heat_transfer", "unidirectional_heat_transfer_with_
              "unidirectional_heat_transfer_with_radiation_and_convection",
              "unidirectional_heat_transfer_with_two_phase",
              "unidirectional_heat_transfer_with_two_phase_rho_times_cp",
-             "unidirectional_heat_transfer_with_velocity"]
+             "unidirectional_heat_transfer_with_velocity",
+             "unidirectional_heat_transfer_with_two_phase_and_solidification",
+             "unidirectional_heat_transfer_with_velocity_and_solidification"]
 REJECTED = {
     "unidirectional_heat_transfer_one_phase_cut": "operator type",
     "unidirectional_heat_transfer_one_phase_cut_evaporative_heat_loss": "evaporation",
@@ -48,8 +52,6 @@ REJECTED = {
     "unidirectional_heat_transfer_two_phase_cut_solidification": "operator type",
     "unidirectional_heat_transfer_two_phase_cut_with_velocity": "operator type",
     "unidirectional_heat_transfer_with_radiation_and_convection_amr": "do amr",
-    "unidirectional_heat_transfer_with_two_phase_and_solidification": "solidification",
-    "unidirectional_heat_transfer_with_velocity_and_solidification": "solidification",
 }
 
 
@@ -170,6 +172,10 @@ def oracle_problem(cs, dt):
     mat = {"type": m.type, "gas_rho": m.gas.density, "gas_cp": m.gas.specific_heat_capacity,
            "gas_k": m.gas.thermal_conductivity, "liq_rho": m.liquid.density,
            "liq_cp": m.liquid.specific_heat_capacity, "liq_k": m.liquid.thermal_conductivity,
+           "sol_rho": m.solid.density, "sol_cp": m.solid.specific_heat_capacity,
+           "sol_k": m.solid.thermal_conductivity, "solidus": m.solidus_temperature,
+           "liquidus": m.liquidus_temperature,
+           "solid_liquid_sharp": int(m.solid_liquid_properties_transition_type == "sharp"),
            "sharp": int(m.two_phase_fluid_properties_transition_type == "sharp"),
            "volume_specific": int(m.use_volume_specific_thermal_capacity_for_phase_interpolation)}
     robin = dict(convection_faces=cs.convection_faces,
@@ -202,6 +208,8 @@ def test_oracle_reproduces_reference_norms_through_the_front_end(name):
     while not it.is_finished():
         dt = it.compute_next_time_increment()
         pb.set_dt(dt)
+        if cs.heaviside_at is not None:
+            pb.set_heaviside(cs.heaviside_at(it.current_time))
         T_old_old, T_old = T_old, T
         pred = linear_predictor(T_old, T_old_old, dt, it.old_increment)
         T, rep = newton_time_step(pb, pred, T_old, data=data, solver=ls.solver_type,
@@ -213,3 +221,36 @@ def test_oracle_reproduces_reference_norms_through_the_front_end(name):
     assert len(norms) == len(ref)
     for step, (v, r) in enumerate(zip(norms, ref)):
         assert matches_printed(v, r), (name, step + 1, v, r)
+
+
+def test_solidification_case_set_up():
+    """do solidification: T0 = LinearTemp(1960, 1980), the material type of
+    determine_material_type, and for two phases the covected vertical level
+    set 0.1 * 2/3 - x - v t, re-evaluated at each step's time (its zero
+    level, where the smoothed heaviside is 1/2, moves by -v t in x)."""
+    prm = F.parse_parameters(CASES["unidirectional_heat_transfer_with_two_phase_and_solidification"]["input"])
+    cs = F.setup(prm)
+    assert cs.material.type == "gas_liquid_solid" and cs.material.solid.thermal_conductivity == 17.0
+    assert (cs.material.solidus_temperature, cs.material.liquidus_temperature) == (1966.0, 1974.0)
+    X = np.broadcast_to(cs.coords[0][None, None, :], (len(cs.coords[2]), len(cs.coords[1]),
+                                                      len(cs.coords[0]))).reshape(-1)
+    assert np.allclose(cs.temperature0, 1960.0 + 200.0 * X, rtol=0, atol=1e-12)
+    v = 1e-4
+    for t in (0.0, 1.0, 5.0):
+        H = cs.heaviside_at(t)
+        x0 = 0.1 * 2.0 / 3.0 - v * t
+        # the heaviside is 1 left of the moving level, 0 right of it, 1/2 on it
+        assert np.all(H[X < x0 - 0.01] == 1.0) and np.all(H[X >= x0 + 0.01] == 0.0)
+        mid = F.smoothed_heaviside(np.array([0.0]), 0.01)[0]
+        assert mid == pytest.approx(0.5, abs=1e-15)
+    assert not np.array_equal(cs.heaviside_at(0.0), cs.heaviside_at(5.0))
+    assert np.array_equal(cs.heaviside, cs.heaviside_at(prm.time_stepping.start_time))
+    prm = F.parse_parameters(CASES["unidirectional_heat_transfer_with_velocity_and_solidification"]["input"])
+    cs = F.setup(prm)
+    assert cs.material.type == "liquid_solid" and cs.heaviside is None and cs.heaviside_at is None
+    assert cs.velocity == (-0.005, 0.0, 0.0)
+    base = CASES["unidirectional_heat_transfer_with_velocity_and_solidification"]["input"]
+    bad = json.loads(json.dumps(base))
+    bad["material"]["latent heat of fusion"] = 2.5e5
+    with pytest.raises(F.UnsupportedFeature, match="latent heat"):
+        F.parse_parameters(bad)

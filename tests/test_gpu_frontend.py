@@ -35,6 +35,8 @@ def oracle_norms(prm, cs):
     while not it.is_finished():
         dt = it.compute_next_time_increment()
         pb.set_dt(dt)
+        if cs.heaviside_at is not None:
+            pb.set_heaviside(cs.heaviside_at(it.current_time))
         T_old_old, T_old = T_old, T
         T, rep = newton_time_step(pb, linear_predictor(T_old, T_old_old, dt, it.old_increment),
                                   T_old, data=data, solver=ls.solver_type,
@@ -62,6 +64,8 @@ def test_reference_inputs_on_the_device(name):
         # orders may stop one increment apart (the norms above agree to 1e-11)
         assert abs(r["newton"]["iterations"] - o_it) <= 1
     assert sum("Newton Raphson solver converged: ||solution|| = " in s for s in lines) == len(ref)
+    ilu = prm.heat.linear_solver.preconditioner_type == "ILU"
+    assert any("ILU runs as Diagonal" in s for s in lines) == ilu
 
 
 def test_command_line_entry_point(tmp_path, capsys):
