@@ -235,6 +235,76 @@ def robin(args, torch):
             "newton_step_s": t_step, "newton": rep}
 
 
+def solidification(args, torch):
+    """Tangent of a temperature-dependent (gas_liquid_solid) material at full size:
+    64^3 Q4 steel with a gas phase, temperature crossing the mushy zone, a
+    heaviside of a plane level set and a constant velocity."""
+    from meltpooldg_amd import (HeatDiffuseOperation, HeatDiffuseOperator, LinearSolverData,
+                                MaterialData, NonlinearSolverData, PhaseData)
+    from oracle import cpu
+    n, p, dt = 64, 4, 1e-2
+    gas, liq, sol = (3925.0, 230.0, 27.7815), (7850.0, 460.0, 55.563), (7850.0, 700.0, 17.0)
+    mat = MaterialData(type="gas_liquid_solid", gas=PhaseData(*gas), liquid=PhaseData(*liq),
+                       solid=PhaseData(*sol), two_phase_fluid_properties_transition_type="sharp",
+                       solidus_temperature=1966.0, liquidus_temperature=1974.0)
+    vel = (-0.005, 0.0, 0.0)
+    op = HeatDiffuseOperator(n, p, material=mat, dirichlet_faces=0, velocity="constant",
+                             velocity_value=vel)
+    op.reset_time_increment(dt)
+    plain = HeatDiffuseOperator(n, p, material=MaterialData(type="gas_liquid", gas=PhaseData(*gas),
+                                                            liquid=PhaseData(*liq)),
+                                dirichlet_faces=0, velocity="constant", velocity_value=vel)
+    plain.reset_time_increment(dt)
+    N = op.n_dofs()
+    n1 = n * p + 1
+    x = np.linspace(0.0, 1.0, n1)
+    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+    T = (1960.0 + 20.0 * X + 2.0 * np.sin(2 * np.pi * Y) * np.sin(2 * np.pi * Z)).reshape(-1)
+    To = T - 0.1
+    H = np.clip(0.5 + (0.7 - Z.reshape(-1)) / 0.04, 0.0, 1.0)
+    Tg, Tog, Hg = _t(T, torch), _t(To, torch), _t(H, torch)
+    op.set_heaviside(Hg)
+    plain.set_heaviside(Hg)
+    op.set_temperature(Tg)
+    op.set_temperature_old(Tog)
+    ms_tmat = _vmult_time(op, torch, torch.float64, 30)
+    ms_plain = _vmult_time(plain, torch, torch.float64, 30)
+    inv = torch.empty_like(Tg)
+    op.compute_inverse_diagonal_from_matrixfree(inv)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(10):
+        op.compute_inverse_diagonal_from_matrixfree(inv)
+    torch.cuda.synchronize()
+    ms_diag = (time.perf_counter() - t0) / 10 * 1e3
+    mdict = {"type": "gas_liquid_solid", "gas_rho": gas[0], "gas_cp": gas[1], "gas_k": gas[2],
+             "liq_rho": liq[0], "liq_cp": liq[1], "liq_k": liq[2], "sol_rho": sol[0],
+             "sol_cp": sol[1], "sol_k": sol[2], "solidus": 1966.0, "liquidus": 1974.0, "sharp": 1}
+    pb = cpu.Problem(n, p, dt=dt, material=mdict, heaviside=H, dirichlet_faces=0,
+                     velocity="constant", velocity_value=vel, temperature=T, temperature_old=To)
+    err, t_cpu_vmult = _oracle_vmult_check(pb, op, torch)
+    # a time step from T (old = T): absolute tolerances sized for 17 M DoFs
+    nl = NonlinearSolverData(residual_tolerance=1e-6, field_correction_tolerance=1e-8)
+    heat = HeatDiffuseOperation(op, Tg, nlsolve=nl, linear_solver=LinearSolverData(
+        solver_type="GMRES", preconditioner_type="Diagonal", rel_tolerance=1e-10))
+    heat.advance(dt)  # warm-up step (workspace allocation)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rep = heat.advance(dt)
+    torch.cuda.synchronize()
+    t_step = time.perf_counter() - t0
+    bytes_alg = 5 * 8 * N  # src, dst, T, T_old, H
+    return {"mode": "solidification",
+            "workload": "64^3 Q4 gas_liquid_solid steel (solidus 1966, liquidus 1974), T "
+                        "crossing the mushy zone, plane heaviside, velocity (-0.005, 0, 0), dt 0.01",
+            "n_dofs": N, "gpu_vmult_ms": ms_tmat, "gpu_vmult_gdofs": N / ms_tmat / 1e6,
+            "algorithmic_bytes_per_vmult": bytes_alg,
+            "algorithmic_gbs": bytes_alg / ms_tmat / 1e6,
+            "gpu_vmult_ms_gas_liquid_same_fields": ms_plain, "gpu_diagonal_ms": ms_diag,
+            "vmult_rel_err_vs_oracle": err, "cpu_vmult_ms": t_cpu_vmult * 1e3,
+            "newton_step_s": t_step, "newton": rep}
+
+
 def run(config, args):
     import torch
     fn = {0: config0, 2: config2, 4: config4}

@@ -75,6 +75,13 @@ namespace mpdg
       check_size(T);
       check(mpdg_heat_set_temperature(h_, T.data));
     }
+    // the operator's reference to the old solution (heat_diffuse_operator.hpp:129),
+    // read by the tangent of a temperature-dependent material; the pointer is kept
+    void set_temperature_old(const DeviceVector &T_old)
+    {
+      check_size(T_old);
+      check(mpdg_heat_set_temperature_old(h_, T_old.data));
+    }
     std::size_t n_dofs() const { return (std::size_t)mpdg_heat_n_dofs(h_); }
 
     // OperatorMatrixFree::vmult
