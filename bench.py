@@ -161,12 +161,14 @@ def main():
                          "per-degree default tuning (itself the autotuner's pick on this workload)")
     ap.add_argument("--tuning", default=None,
                     help="fixed FP64 tuning z_layers,blocks_per_cu[,variant,tile_log2]")
-    ap.add_argument("--mode", choices=["vmult", "copy", "solve", "robin", "solidification"],
+    ap.add_argument("--mode", choices=["vmult", "copy", "solve", "robin", "solidification",
+                                       "timestep"],
                     default="vmult",
                     help="copy: time a plain device copy of one DoF vector (HBM counter calibration); "
                          "solve: only the FP64 and FP32-mixed CG solves (solver profiling); "
                          "robin: the headline operator with a convecting and radiating face; "
-                         "solidification: the tangent of a temperature-dependent material")
+                         "solidification: the tangent of a temperature-dependent material; "
+                         "timestep: the reference's regression inputs, graphs vs eager")
     ap.add_argument("--config", type=int, default=1,
                     help="1: the headline line (default); 0, 2, 3, 4: the other workloads")
     args = ap.parse_args()
@@ -191,6 +193,9 @@ def main():
     if args.mode == "solidification":
         import bench_configs
         return print(json.dumps(bench_configs.solidification(args, torch)))
+    if args.mode == "timestep":
+        import bench_configs
+        return print(json.dumps(bench_configs.timestep(args, torch)))
     if args.config != 1:
         import bench_configs
         return bench_configs.run(args.config, args)

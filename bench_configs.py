@@ -305,6 +305,40 @@ def solidification(args, torch):
             "newton_step_s": t_step, "newton": rep}
 
 
+def timestep(args, torch):
+    """The reference's own unidirectional_heat_transfer regression inputs that the
+    engine runs (tests/golden/unidirectional_cases.json) through the JSON front
+    end, each whole run timed with the Krylov restart cycles replayed as
+    hipGraphs and with eager launches (after one untimed warm-up run)."""
+    from meltpooldg_amd import frontend as F
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden",
+                        "unidirectional_cases.json")
+    cases = json.load(open(path))["cases"]
+    out = []
+    for name, c in sorted(cases.items()):
+        try:
+            prm = F.parse_parameters(c["input"])
+            F.setup(prm)
+        except (F.UnsupportedFeature, F.UnknownParameter):
+            continue
+        row = {"input": name}
+        for graphs in (True, False):
+            F.run(prm, log=lambda s: None, graphs=graphs)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            rec = F.run(prm, log=lambda s: None, graphs=graphs)
+            torch.cuda.synchronize()
+            key = "graphs" if graphs else "eager"
+            row[key + "_s"] = time.perf_counter() - t0
+            row["steps"] = len(rec)
+            row["linear_iterations"] = sum(r["newton"]["linear_iterations"] for r in rec)
+        row["speedup"] = row["eager_s"] / row["graphs_s"]
+        out.append(row)
+    return {"mode": "timestep", "workload": "the reference's unidirectional_heat_transfer "
+                                            "regression inputs (2D / 1D Q1, extruded)",
+            "runs": out}
+
+
 def run(config, args):
     import torch
     fn = {0: config0, 2: config2, 4: config4}

@@ -292,6 +292,14 @@ mpdg_status mpdg_heat_distribute_constraints(mpdg_heat *h, double *v);
  * the FE field v with the operator's Gauss quadrature, entries read plain. */
 mpdg_status mpdg_heat_l2_norm(mpdg_heat *h, const double *v, double *norm);
 
+/* Krylov iteration chunks as hipGraphs (no counterpart in the reference, whose
+ * solvers run on the host): with enable != 0 (the default) the PCG solves
+ * replay one captured graph per chunk of check_interval iterations after the
+ * first chunk, on a capture stream of the handle's own that is ordered after
+ * and before the handle's stream by events. Results are bitwise those of the
+ * eager launches; 0 launches every kernel eagerly. */
+mpdg_status mpdg_heat_set_graphs(mpdg_heat *h, int enable);
+
 /* Launch tuning of the vmult kernels, one set per precision (no counterpart
  * in the reference, whose cell loop is tuned by deal.II's
  * MatrixFree::AdditionalData). Results do not depend on it beyond the order

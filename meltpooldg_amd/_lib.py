@@ -136,6 +136,7 @@ SIGNATURES = {
     "mpdg_heat_set_heaviside": (ctypes.c_int, [_VP, _VP]),
     "mpdg_heat_set_temperature": (ctypes.c_int, [_VP, _VP]),
     "mpdg_heat_set_temperature_old": (ctypes.c_int, [_VP, _VP]),
+    "mpdg_heat_set_graphs": (ctypes.c_int, [_VP, ctypes.c_int]),
     "mpdg_heat_vmult": (ctypes.c_int, [_VP, _VP, _VP]),
     "mpdg_heat_vmult_f32": (ctypes.c_int, [_VP, _VP, _VP]),
     "mpdg_heat_inverse_diagonal": (ctypes.c_int, [_VP, _VP]),

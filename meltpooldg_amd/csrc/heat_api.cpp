@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 #include <new>
 #include <string>
@@ -103,10 +104,30 @@ struct mpdg_heat
   DevBuf<float> r32, p32, q32, x32, inv32;
   DevBuf<double> V64, gm;  // GMRES basis (FP64) and small coefficient block
   DevBuf<float> V32;       // GMRES basis of the FP32 inner solves
-  double *host_gm = nullptr;
   DevBuf<SolverScalars> scal;
   SolverScalars *host_scal = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+  // Krylov iteration chunks captured as hipGraphs: a capture stream of the
+  // handle's own (the caller's stream may be the legacy NULL stream, which
+  // cannot be captured), fork / join events, one cached graph per precision.
+  // state_gen changes with every operator-parameter update (fill_params), so a
+  // graph never replays stale kernel arguments.
+  struct ChunkGraph
+  {
+    hipGraphExec_t exec = nullptr;
+    uint64_t gen = 0;
+    int steps = 0;
+    const void *key[16]{};
+  };
+  bool use_graphs = true;
+  uint64_t state_gen = 1;
+  hipStream_t gstream = nullptr;
+  hipEvent_t gfork = nullptr, gjoin = nullptr;
+  ChunkGraph pcg_graph[2];
+  std::vector<ChunkGraph> gmres_graphs[2]; // per precision: the cycle's step chunks, then its end
+  DevBuf<GmresState> gst;            // device GMRES state
+  GmresState *host_gst = nullptr;    // pinned copy of its scalar head
 
   // Newton time step: residual, increment, reduction scratch and results
   DevBuf<double> nrhs, ndu, npart, nres;
@@ -120,8 +141,6 @@ struct mpdg_heat
       b->release();
     for (auto *b : {&sin_tab32, &heav32, &r32, &p32, &q32, &x32, &inv32, &V32})
       b->release();
-    if (host_gm)
-      (void)hipHostFree(host_gm);
     for (auto *b : {&nrhs, &ndu, &npart, &nres})
       b->release();
     nticket.release();
@@ -134,6 +153,22 @@ struct mpdg_heat
       (void)hipEventDestroy(ev0);
     if (ev1)
       (void)hipEventDestroy(ev1);
+    for (auto &g : pcg_graph)
+      if (g.exec)
+        (void)hipGraphExecDestroy(g.exec);
+    for (auto &v : gmres_graphs)
+      for (auto &g : v)
+        if (g.exec)
+          (void)hipGraphExecDestroy(g.exec);
+    gst.release();
+    if (host_gst)
+      (void)hipHostFree(host_gst);
+    if (gfork)
+      (void)hipEventDestroy(gfork);
+    if (gjoin)
+      (void)hipEventDestroy(gjoin);
+    if (gstream)
+      (void)hipStreamDestroy(gstream);
   }
 };
 
@@ -277,6 +312,7 @@ namespace
   template <typename Number>
   void fill_params(mpdg_heat *H, CellParams<Number> &prm, const Number *tab)
   {
+    ++H->state_gen; // kernel arguments change: captured graphs are stale
     const mpdg_heat_desc &d = H->desc;
     const int n = H->t1d.n;
     prm.nx = d.n_cells[0];
@@ -434,9 +470,8 @@ namespace
     return MPDG_OK;
   }
 
-  // GMRES coefficient block layout (doubles): h1[32] | h2[32] | nrm2 | y[32]
-  constexpr int kGmH1 = 0, kGmH2 = kMaxBasis, kGmNrm = 2 * kMaxBasis, kGmY = 2 * kMaxBasis + 8,
-                kGmSize = 3 * kMaxBasis + 8;
+  // GMRES coefficient block layout (doubles): h1[32] | h2[32] | nrm2
+  constexpr int kGmH1 = 0, kGmH2 = kMaxBasis, kGmNrm = 2 * kMaxBasis, kGmSize = 2 * kMaxBasis + 8;
 
   // Leading dimension of the GMRES basis: N rounded up to 4 entries, so every
   // basis vector starts 16-byte aligned (FP64 and FP32) for the wide sweeps.
@@ -450,8 +485,15 @@ namespace
     else
       MPDG_HIP(H->V64.alloc(nb));
     MPDG_HIP(H->gm.alloc(kGmSize));
-    if (!H->host_gm)
-      MPDG_HIP(hipHostMalloc((void **)&H->host_gm, kGmSize * sizeof(double), hipHostMallocDefault));
+    if (!H->gst.ptr)
+      {
+        // the vmult's p.Ap ticket in gst->sc.counter must start at 0
+        MPDG_HIP(H->gst.alloc(1));
+        const GmresState zero{};
+        MPDG_HIP(hipMemcpy(H->gst.ptr, &zero, sizeof(zero), hipMemcpyHostToDevice));
+      }
+    if (!H->host_gst)
+      MPDG_HIP(hipHostMalloc((void **)&H->host_gst, sizeof(GmresState), hipHostMallocDefault));
     return MPDG_OK;
   }
 
@@ -555,6 +597,72 @@ namespace
     return MPDG_OK;
   }
 
+  // Runs one Krylov chunk (`launch` issues its kernels on H->stream). With
+  // graphs enabled, a chunk is launched eagerly the first time it is seen with
+  // a set of vectors, chunk length and operator state, captured into a graph
+  // the second time, and replayed from then on; any change of those starts
+  // over. H->stream is then the handle's capture stream (solve_impl moves the
+  // whole solve onto it).
+  template <typename Launch>
+  mpdg_status run_chunk(mpdg_heat *H, mpdg_heat::ChunkGraph &g, int steps,
+                        const void *const *key, int nkey, Launch launch)
+  {
+    if (!H->use_graphs || H->stream == nullptr)
+      return launch();
+    const void *full[16]{};
+    const void *extra[] = {H->temperature, H->temperature_old, H->heav64.ptr, H->heav32.ptr,
+                           H->robin_rows.ptr};
+    int nk = 0;
+    for (int i = 0; i < nkey && nk < 16; ++i)
+      full[nk++] = key[i];
+    for (const void *e : extra)
+      if (nk < 16)
+        full[nk++] = e;
+    const bool same = g.gen == H->state_gen && g.steps == steps &&
+                      std::memcmp(g.key, full, sizeof(full)) == 0;
+    if (!same)
+      {
+        if (g.exec)
+          {
+            MPDG_HIP(hipStreamSynchronize(H->stream));
+            (void)hipGraphExecDestroy(g.exec);
+            g.exec = nullptr;
+          }
+        g.gen = H->state_gen;
+        g.steps = steps;
+        std::memcpy(g.key, full, sizeof(full));
+        return launch(); // first use: eager
+      }
+    if (!g.exec)
+      {
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamBeginCapture(H->stream, hipStreamCaptureModeThreadLocal);
+        mpdg_status st = MPDG_OK;
+        if (e == hipSuccess)
+          {
+            st = launch();
+            e = hipStreamEndCapture(H->stream, &graph);
+          }
+        if (st)
+          {
+            if (graph)
+              (void)hipGraphDestroy(graph);
+            return st;
+          }
+        if (e != hipSuccess)
+          return fail(MPDG_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+        e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess)
+          {
+            g.exec = nullptr;
+            return fail(MPDG_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+          }
+      }
+    MPDG_HIP(hipGraphLaunch(g.exec, H->stream));
+    return MPDG_OK;
+  }
+
   // Device PCG on precision Number; x and the workspace vectors in Number,
   // b in double. Returns through scal (iteration, status, res).
   template <typename Number>
@@ -581,37 +689,51 @@ namespace
         MPDG_HIP(launch_cg_init<Number>(b, q, r, p, inv, H->N, rel_tol, abs_tol, max_it, part, sc,
                                         H->stream));
       }
-    for (int done_it = 0;;)
+    auto chunk = [&]() -> mpdg_status {
+      for (int k = 0; k < check; ++k)
+        {
+          mpdg_status st;
+          if constexpr (sizeof(Number) == 8)
+            st = vmult64(H, q, p, part, sc);
+          else
+            st = vmult32(H, q, p, part, sc);
+          if (st)
+            return st;
+          MPDG_HIP(launch_cg_update<Number>(x, r, p, q, inv, H->N, part, sc, H->history.ptr,
+                                            (int)H->history.n, H->stream));
+          MPDG_HIP(launch_cg_direction<Number>(p, r, inv, H->N, sc, H->stream));
+        }
+      return MPDG_OK;
+    };
+    const void *key[] = {x, r, p, q, inv, part, sc, H->history.ptr,
+                         (const void *)(uintptr_t)H->history.n};
+    for (;;)
       {
-        for (int k = 0; k < check; ++k)
-          {
-            mpdg_status st;
-            if constexpr (sizeof(Number) == 8)
-              st = vmult64(H, q, p, part, sc);
-            else
-              st = vmult32(H, q, p, part, sc);
-            if (st)
-              return st;
-            MPDG_HIP(launch_cg_update<Number>(x, r, p, q, inv, H->N, part, sc, H->history.ptr,
-                                              (int)H->history.n, H->stream));
-            MPDG_HIP(launch_cg_direction<Number>(p, r, inv, H->N, sc, H->stream));
-          }
-        done_it += check;
+        if (mpdg_status st = run_chunk(H, H->pcg_graph[sizeof(Number) == 4], check, key,
+                                       sizeof(key) / sizeof(key[0]), chunk))
+          return st;
         if (mpdg_status st = read_scalars(H))
           return st;
         if (H->host_scal->done)
           break;
-        (void)done_it;
       }
     return MPDG_OK;
   }
 
   // Right-preconditioned GMRES(restart) with Jacobi, classical Gram-Schmidt
-  // plus one re-orthogonalisation pass, Givens rotations on the host; the
-  // inner check uses the least-squares residual estimate, each restart the
-  // true residual (deal.II SolverGMRES with right preconditioning,
-  // linear_solver.hpp:112-119). Number is the precision of the basis and of
-  // the operator; b is FP64.
+  // plus one re-orthogonalisation pass; the inner check uses the least-squares
+  // residual estimate, each restart the true residual (deal.II SolverGMRES with
+  // right preconditioning, linear_solver.hpp:112-119). Number is the precision
+  // of the basis and of the operator; b is FP64.
+  //
+  // The Hessenberg column, Givens rotations and back substitution run in
+  // one-thread kernels on the device (GmresState). A restart cycle is cut into
+  // chunks of `chunk` Arnoldi steps; the host reads the cycle's stop flag after
+  // each chunk, and every kernel of a step does nothing once the cycle has
+  // stopped (the flag is the state's sc.done), so a chunk is a fixed launch
+  // sequence. The cycle ends with the update x += D V y over its basis size and
+  // the true residual for the next cycle. Chunks and the end replay as captured
+  // graphs (run_chunk).
   template <typename Number>
   mpdg_status gmres(mpdg_heat *H, Number *x, const double *b, Number *r, Number *z, Number *w,
                     Number *V, const Number *inv, double rel_tol, double abs_tol, int max_it,
@@ -624,106 +746,112 @@ namespace
     SolverScalars *sc = H->scal.ptr;
     unsigned *counter = &sc->counter;
     double *gm = H->gm.ptr;
-    auto op = [&](Number *dst, const Number *src) -> mpdg_status {
+    GmresState *st = H->gst.ptr;
+    const int chunk = std::min(restart, H->N < (1 << 20) ? 5 : 10);
+    const int nchunks = (restart + chunk - 1) / chunk;
+    std::vector<mpdg_heat::ChunkGraph> &graphs = H->gmres_graphs[sizeof(Number) == 4];
+    if ((int)graphs.size() < nchunks + 1)
+      graphs.resize(nchunks + 1);
+    auto op = [&](Number *dst, const Number *src, SolverScalars *stop) -> mpdg_status {
       if constexpr (sizeof(Number) == 8)
-        return vmult64(H, dst, src, nullptr, nullptr);
+        return vmult64(H, dst, src, stop ? part : nullptr, stop);
       else
-        return vmult32(H, dst, src, nullptr, nullptr);
+        return vmult32(H, dst, src, stop ? part : nullptr, stop);
     };
+    // r = b - A x and ||r|| into sc->res (z receives D r, unused)
     auto residual = [&](bool zero_x) -> mpdg_status {
       if (!zero_x)
-        if (mpdg_status st = op(w, x))
-          return st;
-      // r = b - A x and ||r|| (z receives D r, unused)
+        if (mpdg_status e = op(w, x, nullptr))
+          return e;
       MPDG_HIP(launch_cg_init<Number>(b, zero_x ? nullptr : w, r, z, inv, n, 0.0, 0.0, 1, part, sc,
                                       H->stream));
+      return MPDG_OK;
+    };
+    auto steps = [&](int j0, int j1) -> mpdg_status {
+      const int *stop = &st->sc.done;
+      for (int j = j0; j < j1; ++j)
+        {
+          if (mpdg_status e = op(w, z, &st->sc))
+            return e;
+          // classical Gram-Schmidt with one re-orthogonalisation pass, in three
+          // sweeps over the basis: h1 = V^T w; w -= V h1 with h2 = V^T w; w -= V h2
+          // with ||w||^2
+          MPDG_HIP(launch_multi_dot<Number>(V, ld, j + 1, w, n, part, gm + kGmH1, counter,
+                                            H->stream, stop));
+          MPDG_HIP(launch_multi_axpy_dot<Number>(w, V, ld, j + 1, gm + kGmH1, n, part, gm + kGmH2,
+                                                 counter, H->stream, stop));
+          MPDG_HIP(launch_multi_axpy<Number>(w, V, ld, j + 1, gm + kGmH2, n, part, gm + kGmNrm,
+                                             counter, H->stream, stop));
+          MPDG_HIP(launch_gmres_givens(st, gm + kGmH1, gm + kGmH2, gm + kGmNrm, j, restart,
+                                       H->stream));
+          // v_{j+1} = w / ||w|| and the next step's z = D v_{j+1}
+          MPDG_HIP(launch_scale_precondition<Number>(V + (j + 1) * ld, z, w, inv, gm + kGmNrm, n,
+                                                     H->stream, stop));
+        }
+      return MPDG_OK;
+    };
+    auto finish = [&]() -> mpdg_status {
+      // x += D V y, y from the triangular system, over the cycle's basis
+      MPDG_HIP(launch_gmres_solve(st, H->stream));
+      MPDG_HIP((launch_gmres_update<Number, Number>(x, V, ld, restart, st->y, inv, n, H->stream,
+                                                    &st->k)));
+      return residual(false);
+    };
+    auto read_state = [&]() -> mpdg_status {
+      MPDG_HIP(hipMemcpyAsync(H->host_gst, st, offsetof(GmresState, cs), hipMemcpyDeviceToHost,
+                              H->stream));
       return read_scalars(H);
     };
-    if (mpdg_status st = residual(x_is_zero))
-      return st;
-    double res = H->host_scal->res;
-    const double res0 = res;
+    if (mpdg_status e = residual(x_is_zero))
+      return e;
+    if (mpdg_status e = read_scalars(H))
+      return e;
+    const double res0 = H->host_scal->res;
     const double tol = std::max(abs_tol, rel_tol * res0);
+    double res = res0;
     int it = 0;
     bool conv = res <= tol;
-    std::vector<double> Hm((size_t)(restart + 1) * restart), cs(restart), sn(restart),
-      g(restart + 1), y(restart);
+    if (!conv && max_it > 0)
+      MPDG_HIP(launch_gmres_init(st, tol, max_it, H->stream));
+    const void *key[] = {x, b, r, z, w, V, inv, part, sc, st, gm};
+    const int nkey = sizeof(key) / sizeof(key[0]);
     while (!conv && it < max_it)
       {
-        // v_0 = r / ||r|| and z = D v_0
+        // v_0 = r / ||r||, z = D v_0, g = (||r||, 0, ...)
+        MPDG_HIP(launch_gmres_start(st, sc, restart, H->stream));
         MPDG_HIP(launch_scale_precondition<Number>(V, z, r, inv, &sc->rr, n, H->stream));
-        std::fill(g.begin(), g.end(), 0.0);
-        g[0] = res;
-        int j = 0;
-        for (; j < restart; ++j)
+        for (int c = 0; c < nchunks; ++c)
           {
-            ++it;
-            if (mpdg_status st = op(w, z))
-              return st;
-            // classical Gram-Schmidt with one re-orthogonalisation pass, in three
-            // sweeps over the basis: h1 = V^T w; w -= V h1 with h2 = V^T w; w -= V h2
-            // with ||w||^2
-            MPDG_HIP(launch_multi_dot<Number>(V, ld, j + 1, w, n, part, gm + kGmH1, counter, H->stream));
-            MPDG_HIP(launch_multi_axpy_dot<Number>(w, V, ld, j + 1, gm + kGmH1, n, part, gm + kGmH2,
-                                                   counter, H->stream));
-            MPDG_HIP(launch_multi_axpy<Number>(w, V, ld, j + 1, gm + kGmH2, n, part, gm + kGmNrm,
-                                               counter, H->stream));
-            MPDG_HIP(hipMemcpyAsync(H->host_gm, gm, (kGmNrm + 1) * sizeof(double),
-                                    hipMemcpyDeviceToHost, H->stream));
-            MPDG_HIP(hipStreamSynchronize(H->stream));
-            std::vector<double> col(j + 2);
-            for (int l = 0; l <= j; ++l)
-              col[l] = H->host_gm[kGmH1 + l] + H->host_gm[kGmH2 + l];
-            const double hn = std::sqrt(H->host_gm[kGmNrm]);
-            col[j + 1] = hn;
-            // v_{j+1} = w / ||w|| and the next step's z = D v_{j+1}
-            if (hn > 0.0 && j + 1 <= restart)
-              MPDG_HIP(launch_scale_precondition<Number>(V + (j + 1) * ld, z, w, inv, gm + kGmNrm, n,
-                                                         H->stream));
-            for (int l = 0; l < j; ++l)
+            const int j0 = c * chunk, j1 = std::min(restart, j0 + chunk);
+            if (mpdg_status e = run_chunk(H, graphs[c], restart * 64 + j0, key, nkey,
+                                          [&]() { return steps(j0, j1); }))
+              return e;
+            if (c + 1 < nchunks)
               {
-                const double a = cs[l] * col[l] + sn[l] * col[l + 1];
-                const double bb = -sn[l] * col[l] + cs[l] * col[l + 1];
-                col[l] = a;
-                col[l + 1] = bb;
-              }
-            const double den = std::hypot(col[j], col[j + 1]);
-            cs[j] = col[j] / den;
-            sn[j] = col[j + 1] / den;
-            col[j] = den;
-            col[j + 1] = 0.0;
-            g[j + 1] = -sn[j] * g[j];
-            g[j] = cs[j] * g[j];
-            for (int l = 0; l <= j; ++l)
-              Hm[(size_t)l * restart + j] = col[l];
-            res = std::fabs(g[j + 1]);
-            if (res <= tol || it >= max_it || std::isnan(res) || hn == 0.0)
-              {
-                ++j;
-                break;
+                if (mpdg_status e = read_state())
+                  return e;
+                if (H->host_gst->sc.done)
+                  break;
               }
           }
-        // x += D V y, y from the triangular system
-        for (int l = j - 1; l >= 0; --l)
-          {
-            double sum = g[l];
-            for (int c = l + 1; c < j; ++c)
-              sum -= Hm[(size_t)l * restart + c] * y[c];
-            y[l] = sum / Hm[(size_t)l * restart + l];
-          }
-        std::copy(y.begin(), y.begin() + j, H->host_gm + kGmY);
-        MPDG_HIP(hipMemcpyAsync(gm + kGmY, H->host_gm + kGmY, j * sizeof(double),
-                                hipMemcpyHostToDevice, H->stream));
-        MPDG_HIP((launch_gmres_update<Number, Number>(x, V, ld, j, gm + kGmY, inv, n, H->stream)));
-        if (res <= tol)
+        if (mpdg_status e = run_chunk(H, graphs[nchunks], restart * 64 + kMaxBasis, key, nkey,
+                                      finish))
+          return e;
+        if (mpdg_status e = read_state())
+          return e;
+        const GmresState &hs = *H->host_gst;
+        it = hs.it;
+        if (hs.inner_conv)
           {
             conv = true;
+            res = hs.res;
             break;
           }
-        if (it >= max_it || std::isnan(res))
-          break;
-        if (mpdg_status st = residual(false))
-          return st;
+        if (it >= max_it || std::isnan(hs.res))
+          {
+            res = hs.res;
+            break;
+          }
         res = H->host_scal->res;
         conv = res <= tol;
       }
@@ -927,6 +1055,8 @@ mpdg_status mpdg_heat_set_dt(mpdg_heat *h, double dt)
     return fail(MPDG_ERR_ARGUMENT, "null handle");
   if (dt == 0.0 || !std::isfinite(dt))
     return fail(MPDG_ERR_ZERO_TIME_INCREMENT, "time increment must be finite and nonzero");
+  if (h->dt_set && dt == h->dt)
+    return MPDG_OK; // nothing changes (captured solver graphs stay valid)
   h->dt = dt;
   h->dt_set = true;
   fill_params<double>(h, h->prm64, h->sin_tab64.ptr);
@@ -952,6 +1082,14 @@ mpdg_status mpdg_heat_set_temperature(mpdg_heat *h, const double *temperature)
   if (!h)
     return fail(MPDG_ERR_ARGUMENT, "null handle");
   h->temperature = temperature;
+  return MPDG_OK;
+}
+
+mpdg_status mpdg_heat_set_graphs(mpdg_heat *h, int enable)
+{
+  if (!h)
+    return fail(MPDG_ERR_ARGUMENT, "null handle");
+  h->use_graphs = enable != 0;
   return MPDG_OK;
 }
 
@@ -1069,6 +1207,10 @@ mpdg_status mpdg_heat_linear_predictor(mpdg_heat *h, double *predictor, const do
 static mpdg_status solve_impl(mpdg_heat *h, double *x, const double *b,
                               const mpdg_solver_params *params, mpdg_solver_report *report,
                               bool update_preconditioner);
+static mpdg_status solve_on_stream(mpdg_heat *h, double *x, const double *b,
+                                   const mpdg_solver_params &prm, mpdg_solver_report *report,
+                                   bool update_preconditioner, bool mixed, bool use_gmres,
+                                   int check);
 
 mpdg_status mpdg_heat_newton_solve(mpdg_heat *h, double *temperature,
                                    const double *temperature_old, const double *heat_source,
@@ -1217,6 +1359,39 @@ static mpdg_status solve_impl(mpdg_heat *h, double *x, const double *b,
     if (mpdg_status s = ensure_gmres(h, prm.gmres_restart, mixed))
       return s;
   const int check = prm.check_interval > 0 ? prm.check_interval : auto_check(h);
+  // With graphs the solve runs on the handle's capture stream (the caller's
+  // stream may be the legacy NULL stream, which cannot be captured): one fork
+  // event orders it after the caller's work, one join event orders the
+  // caller's later work after it.
+  hipStream_t user = h->stream;
+  if (h->use_graphs)
+    {
+      if (!h->gstream)
+        {
+          MPDG_HIP(hipStreamCreateWithFlags(&h->gstream, hipStreamNonBlocking));
+          MPDG_HIP(hipEventCreateWithFlags(&h->gfork, hipEventDisableTiming));
+          MPDG_HIP(hipEventCreateWithFlags(&h->gjoin, hipEventDisableTiming));
+        }
+      MPDG_HIP(hipEventRecord(h->gfork, user));
+      MPDG_HIP(hipStreamWaitEvent(h->gstream, h->gfork, 0));
+      h->stream = h->gstream;
+    }
+  const mpdg_status st = solve_on_stream(h, x, b, prm, report, update_preconditioner, mixed,
+                                         use_gmres, check);
+  if (h->stream != user)
+    {
+      h->stream = user;
+      MPDG_HIP(hipEventRecord(h->gjoin, h->gstream));
+      MPDG_HIP(hipStreamWaitEvent(user, h->gjoin, 0));
+    }
+  return st;
+}
+
+static mpdg_status solve_on_stream(mpdg_heat *h, double *x, const double *b,
+                                   const mpdg_solver_params &prm, mpdg_solver_report *report,
+                                   bool update_preconditioner, bool mixed, bool use_gmres,
+                                   int check)
+{
   mpdg_solver_report rep{};
   const auto t0 = std::chrono::steady_clock::now();
 

@@ -333,29 +333,64 @@ namespace mpdg
                                    double *res, hipStream_t s);
 
   // GMRES building blocks (classical Gram-Schmidt over a basis of at most
-  // kMaxBasis vectors stored with leading dimension ld).
+  // kMaxBasis vectors stored with leading dimension ld). A non-null `stop`
+  // makes a launch do nothing when *stop != 0 (a restart cycle that has
+  // stopped), uniformly over the grid: the flag is not written while it runs.
   constexpr int kMaxBasis = 32;
   template <typename Number>
   hipError_t launch_multi_dot(const Number *V, int64_t ld, int k, const Number *w, int64_t n,
-                              double *partials, double *out, unsigned *counter, hipStream_t s);
+                              double *partials, double *out, unsigned *counter, hipStream_t s,
+                              const int *stop = nullptr);
   template <typename Number>
   hipError_t launch_multi_axpy(Number *w, const Number *V, int64_t ld, int k, const double *h,
                                int64_t n, double *partials, double *nrm2, unsigned *counter,
-                               hipStream_t s);
+                               hipStream_t s, const int *stop = nullptr);
   // w -= sum_c h[c] V_c and out[c] = V_c . w (the updated w) in one sweep
   template <typename Number>
   hipError_t launch_multi_axpy_dot(Number *w, const Number *V, int64_t ld, int k, const double *h,
                                    int64_t n, double *partials, double *out, unsigned *counter,
-                                   hipStream_t s);
+                                   hipStream_t s, const int *stop = nullptr);
   // v = w / sqrt(nrm2) and z = d .* v in one pass
   template <typename Number>
   hipError_t launch_scale_precondition(Number *v, Number *z, const Number *w, const Number *d,
-                                       const double *nrm2, int64_t n, hipStream_t s);
+                                       const double *nrm2, int64_t n, hipStream_t s,
+                                       const int *stop = nullptr);
   template <typename Number>
   hipError_t launch_scale(Number *v, const Number *w, const double *nrm2, int64_t n, hipStream_t s);
   template <typename Number>
   hipError_t launch_diag_scale(Number *z, const Number *v, const Number *d, int64_t n, hipStream_t s);
+  // x += D V y over the first k basis vectors; a non-null kdev supplies k from
+  // device memory (the cycle's basis size)
   template <typename Number, typename XNumber>
   hipError_t launch_gmres_update(XNumber *x, const Number *V, int64_t ld, int k, const double *y,
-                                 const Number *d, int64_t n, hipStream_t s);
+                                 const Number *d, int64_t n, hipStream_t s,
+                                 const int *kdev = nullptr);
+
+  // Device state of right-preconditioned GMRES(m): the Hessenberg matrix, the
+  // Givens rotations and the least-squares right-hand side live in device
+  // memory, so a whole restart cycle runs without the host. sc.done is the
+  // cycle's stop flag (the vmult and basis kernels honour it).
+  struct GmresState
+  {
+    SolverScalars sc;
+    double tol;        // max(abs_tol, rel_tol ||r_0||)
+    double res;        // |g_{j+1}| after the last step
+    int it, max_it;    // iterations so far, limit
+    int k;             // basis vectors of the cycle's update
+    int inner_conv;    // the cycle stopped on the residual estimate
+    double cs[kMaxBasis], sn[kMaxBasis], g[kMaxBasis + 1], y[kMaxBasis];
+    double hm[kMaxBasis * kMaxBasis]; // Hessenberg column j of row l at l * kMaxBasis + j
+  };
+  // it = 0, tol, max_it (once per solve)
+  hipError_t launch_gmres_init(GmresState *st, double tol, int max_it, hipStream_t s);
+  // a cycle starts from the residual norm rs->res: g = (res, 0, ...), stop cleared
+  hipError_t launch_gmres_start(GmresState *st, const SolverScalars *rs, int restart,
+                                hipStream_t s);
+  // step j: Hessenberg column from the two Gram-Schmidt passes (h1 + h2) and
+  // ||w|| = sqrt(*nrm2), the earlier rotations, a new rotation, the residual
+  // estimate |g_{j+1}| and the stop test (converged, limit, breakdown, NaN)
+  hipError_t launch_gmres_givens(GmresState *st, const double *h1, const double *h2,
+                                 const double *nrm2, int j, int restart, hipStream_t s);
+  // y from the k x k triangular system (back substitution)
+  hipError_t launch_gmres_solve(GmresState *st, hipStream_t s);
 } // namespace mpdg

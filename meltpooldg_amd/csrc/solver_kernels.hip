@@ -2,10 +2,12 @@
 // updates with deterministic FP64 reductions, diagonal inversion, and the
 // precision conversions of the defect-correction loop.
 //
-// Reductions: each kernel runs a fixed grid (kVecBlocks x kVecThreads) with a
-// grid-stride loop, so every partial sum covers a fixed index set; the last
-// block to finish (atomic ticket) adds the partials in index order. Results
-// are therefore bitwise reproducible run to run.
+// Reductions: each kernel runs a grid fixed by the vector length (the CG
+// kernels kVecBlocks x kVecThreads, the GMRES basis sweeps at most that, fewer
+// blocks for short vectors) with a grid-stride loop, so every partial sum
+// covers a fixed index set; the last block to finish (atomic ticket) adds the
+// partials in a fixed order. Results are therefore bitwise reproducible run to
+// run.
 //
 // Reference semantics restated: deal.II SolverCG under ReductionControl
 // (include/meltpooldg/linear_algebra/linear_solver.hpp:36-38,106-111),
@@ -27,6 +29,26 @@ namespace mpdg
       for (int off = 32; off > 0; off >>= 1)
         v += __shfl_xor(v, off, 64);
       return v;
+    }
+
+    // Final pass of a multi-column reduction, in the last block: column c
+    // (partials[c * kVecBlocks + b], b < gridDim.x) is summed by wave c mod 4,
+    // its lanes striding over the blocks, then a fixed butterfly over the lanes,
+    // so the order is fixed and the latency chain is gridDim.x / 64 loads.
+    __device__ __forceinline__ void ordered_column_sums(const double *partials, int ncol,
+                                                        double *out)
+    {
+      const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+      for (int c = wv; c < ncol; c += T / 64)
+        {
+          double s = 0.0;
+          for (int b = lane; b < (int)gridDim.x; b += 64)
+            s += __hip_atomic_load(partials + c * kVecBlocks + b, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+          s = wave_sum(s);
+          if (lane == 0)
+            out[c] = s;
+        }
     }
 
     // Sums NV values per thread over the block; result valid in thread 0.
@@ -413,8 +435,10 @@ namespace mpdg
     template <typename Number>
     __global__ void __launch_bounds__(T)
       multi_dot_kernel(const Number *V, int64_t ld, int k, const Number *w, int64_t n,
-                       double *partials, double *out, unsigned *counter)
+                       double *partials, double *out, unsigned *counter, const int *stop)
     {
+      if (stop != nullptr && *stop)
+        return;
       __shared__ double sh[kMaxBasis][T / 64];
       __shared__ bool last;
       double acc[kMaxBasis];
@@ -454,15 +478,7 @@ namespace mpdg
       if (!last)
         return;
       __threadfence();
-      // one thread per basis vector adds its partials in block order
-      if (threadIdx.x < k)
-        {
-          double s = 0.0;
-          for (int b = 0; b < (int)gridDim.x; ++b)
-            s += __hip_atomic_load(partials + threadIdx.x * kVecBlocks + b, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-          out[threadIdx.x] = s;
-        }
+      ordered_column_sums(partials, k, out);
       if (threadIdx.x == 0)
         *counter = 0u;
     }
@@ -471,8 +487,10 @@ namespace mpdg
     template <typename Number>
     __global__ void __launch_bounds__(T)
       multi_axpy_kernel(Number *w, const Number *V, int64_t ld, int k, const double *h, int64_t n,
-                        double *partials, double *nrm2, unsigned *counter)
+                        double *partials, double *nrm2, unsigned *counter, const int *stop)
     {
+      if (stop != nullptr && *stop)
+        return;
       __shared__ double hs[kMaxBasis];
       if (threadIdx.x < k)
         hs[threadIdx.x] = h[threadIdx.x];
@@ -500,8 +518,11 @@ namespace mpdg
     template <typename Number>
     __global__ void __launch_bounds__(T)
       multi_axpy_dot_kernel(Number *w, const Number *V, int64_t ld, int k, const double *h,
-                            int64_t n, double *partials, double *out, unsigned *counter)
+                            int64_t n, double *partials, double *out, unsigned *counter,
+                            const int *stop)
     {
+      if (stop != nullptr && *stop)
+        return;
       __shared__ double hs[kMaxBasis];
       __shared__ double sh[kMaxBasis][T / 64];
       __shared__ bool last;
@@ -549,14 +570,7 @@ namespace mpdg
       if (!last)
         return;
       __threadfence();
-      if (threadIdx.x < k)
-        {
-          double s = 0.0;
-          for (int b = 0; b < (int)gridDim.x; ++b)
-            s += __hip_atomic_load(partials + threadIdx.x * kVecBlocks + b, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-          out[threadIdx.x] = s;
-        }
+      ordered_column_sums(partials, k, out);
       if (threadIdx.x == 0)
         *counter = 0u;
     }
@@ -579,8 +593,11 @@ namespace mpdg
     template <typename Number, int MODE>
     __global__ void __launch_bounds__(T)
       basis_sweep_kernel(Number *w, const Number *V, int64_t ld, int k, const double *h,
-                         int64_t n, double *partials, double *out, unsigned *counter)
+                         int64_t n, double *partials, double *out, unsigned *counter,
+                         const int *stop)
     {
+      if (stop != nullptr && *stop)
+        return;
       using Vec = typename Wide<Number>::type;
       constexpr int W = Wide<Number>::W;
       constexpr int NACC = MODE == kSweepAxpyNorm ? 1 : kMaxBasis;
@@ -741,14 +758,7 @@ namespace mpdg
       if (!last)
         return;
       __threadfence();
-      if (threadIdx.x < nacc)
-        {
-          double s = 0.0;
-          for (int b = 0; b < (int)gridDim.x; ++b)
-            s += __hip_atomic_load(partials + threadIdx.x * kVecBlocks + b, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-          out[threadIdx.x] = s;
-        }
+      ordered_column_sums(partials, nacc, out);
       if (threadIdx.x == 0)
         *counter = 0u;
     }
@@ -758,8 +768,10 @@ namespace mpdg
     template <typename Number>
     __global__ void __launch_bounds__(T)
       scale_precondition_kernel(Number *v, Number *z, const Number *w, const Number *d,
-                                const double *nrm2, int64_t n)
+                                const double *nrm2, int64_t n, const int *stop)
     {
+      if (stop != nullptr && *stop)
+        return;
       using Vec = typename Wide<Number>::type;
       constexpr int W = Wide<Number>::W;
       const double inv = 1.0 / sqrt(*nrm2);
@@ -794,8 +806,10 @@ namespace mpdg
     // v = w / sqrt(nrm2)
     template <typename Number>
     __global__ void __launch_bounds__(T)
-      scale_kernel(Number *v, const Number *w, const double *nrm2, int64_t n)
+      scale_kernel(Number *v, const Number *w, const double *nrm2, int64_t n, const int *stop)
     {
+      if (stop != nullptr && *stop)
+        return;
       const double inv = 1.0 / sqrt(*nrm2);
       for (int64_t i = blockIdx.x * (int64_t)T + threadIdx.x; i < n; i += (int64_t)gridDim.x * T)
         v[i] = (Number)((double)w[i] * inv);
@@ -804,8 +818,10 @@ namespace mpdg
     // z = D .* v (right Jacobi preconditioner)
     template <typename Number>
     __global__ void __launch_bounds__(T)
-      diag_scale_kernel(Number *z, const Number *v, const Number *d, int64_t n)
+      diag_scale_kernel(Number *z, const Number *v, const Number *d, int64_t n, const int *stop)
     {
+      if (stop != nullptr && *stop)
+        return;
       for (int64_t i = blockIdx.x * (int64_t)T + threadIdx.x; i < n; i += (int64_t)gridDim.x * T)
         z[i] = v[i] * d[i];
     }
@@ -814,8 +830,10 @@ namespace mpdg
     template <typename Number, typename XNumber>
     __global__ void __launch_bounds__(T)
       gmres_update_kernel(XNumber *x, const Number *V, int64_t ld, int k, const double *y,
-                          const Number *d, int64_t n)
+                          const Number *d, int64_t n, const int *kdev)
     {
+      if (kdev != nullptr)
+        k = min(max(*kdev, 0), kMaxBasis);
       __shared__ double ys[kMaxBasis];
       if (threadIdx.x < k)
         ys[threadIdx.x] = y[threadIdx.x];
@@ -840,6 +858,13 @@ namespace mpdg
     {
       const int64_t g = (n + T - 1) / T;
       return (int)(g < kVecBlocks ? (g < 1 ? 1 : g) : kVecBlocks);
+    }
+
+    // grid of the 16-byte basis sweeps (W entries per lane and step)
+    template <typename Number>
+    int sweep_grid(int64_t n)
+    {
+      return grid_for((n + Wide<Number>::W - 1) / Wide<Number>::W);
     }
   } // namespace
 
@@ -950,63 +975,66 @@ namespace mpdg
 
   template <typename Number>
   hipError_t launch_multi_dot(const Number *V, int64_t ld, int k, const Number *w, int64_t n,
-                              double *partials, double *out, unsigned *counter, hipStream_t s)
+                              double *partials, double *out, unsigned *counter, hipStream_t s,
+                              const int *stop)
   {
     if (k < 1 || k > kMaxBasis)
       return hipErrorInvalidValue;
     if (wide_basis(V, ld, w))
-      basis_sweep_kernel<Number, kSweepDot><<<kVecBlocks, T, 0, s>>>(
-        const_cast<Number *>(w), V, ld, k, nullptr, n, partials, out, counter);
+      basis_sweep_kernel<Number, kSweepDot><<<sweep_grid<Number>(n), T, 0, s>>>(
+        const_cast<Number *>(w), V, ld, k, nullptr, n, partials, out, counter, stop);
     else
-      multi_dot_kernel<Number><<<kVecBlocks, T, 0, s>>>(V, ld, k, w, n, partials, out, counter);
+      multi_dot_kernel<Number><<<grid_for(n), T, 0, s>>>(V, ld, k, w, n, partials, out, counter,
+                                                        stop);
     return hipGetLastError();
   }
 
   template <typename Number>
   hipError_t launch_multi_axpy(Number *w, const Number *V, int64_t ld, int k, const double *h,
                                int64_t n, double *partials, double *nrm2, unsigned *counter,
-                               hipStream_t s)
+                               hipStream_t s, const int *stop)
   {
     if (k < 1 || k > kMaxBasis)
       return hipErrorInvalidValue;
     if (nrm2 != nullptr && wide_basis(V, ld, w))
-      basis_sweep_kernel<Number, kSweepAxpyNorm><<<kVecBlocks, T, 0, s>>>(w, V, ld, k, h, n,
-                                                                         partials, nrm2, counter);
+      basis_sweep_kernel<Number, kSweepAxpyNorm><<<sweep_grid<Number>(n), T, 0, s>>>(
+        w, V, ld, k, h, n, partials, nrm2, counter, stop);
     else
-      multi_axpy_kernel<Number><<<kVecBlocks, T, 0, s>>>(w, V, ld, k, h, n, partials, nrm2,
-                                                         counter);
+      multi_axpy_kernel<Number><<<grid_for(n), T, 0, s>>>(w, V, ld, k, h, n, partials, nrm2,
+                                                         counter, stop);
     return hipGetLastError();
   }
 
   template <typename Number>
   hipError_t launch_multi_axpy_dot(Number *w, const Number *V, int64_t ld, int k, const double *h,
                                    int64_t n, double *partials, double *out, unsigned *counter,
-                                   hipStream_t s)
+                                   hipStream_t s, const int *stop)
   {
     if (k < 1 || k > kMaxBasis)
       return hipErrorInvalidValue;
     if (wide_basis(V, ld, w))
-      basis_sweep_kernel<Number, kSweepAxpyDot><<<kVecBlocks, T, 0, s>>>(w, V, ld, k, h, n, partials,
-                                                                        out, counter);
+      basis_sweep_kernel<Number, kSweepAxpyDot><<<sweep_grid<Number>(n), T, 0, s>>>(
+        w, V, ld, k, h, n, partials, out, counter, stop);
     else
-      multi_axpy_dot_kernel<Number><<<kVecBlocks, T, 0, s>>>(w, V, ld, k, h, n, partials, out,
-                                                             counter);
+      multi_axpy_dot_kernel<Number><<<grid_for(n), T, 0, s>>>(w, V, ld, k, h, n, partials, out,
+                                                             counter, stop);
     return hipGetLastError();
   }
 
   template <typename Number>
   hipError_t launch_scale_precondition(Number *v, Number *z, const Number *w, const Number *d,
-                                       const double *nrm2, int64_t n, hipStream_t s)
+                                       const double *nrm2, int64_t n, hipStream_t s,
+                                       const int *stop)
   {
     if (aligned16(v, z, w, d))
       {
         const int64_t nv = (n + Wide<Number>::W - 1) / Wide<Number>::W;
-        scale_precondition_kernel<Number><<<grid_for(nv), T, 0, s>>>(v, z, w, d, nrm2, n);
+        scale_precondition_kernel<Number><<<grid_for(nv), T, 0, s>>>(v, z, w, d, nrm2, n, stop);
       }
     else
       {
-        scale_kernel<Number><<<grid_for(n), T, 0, s>>>(v, w, nrm2, n);
-        diag_scale_kernel<Number><<<grid_for(n), T, 0, s>>>(z, v, d, n);
+        scale_kernel<Number><<<grid_for(n), T, 0, s>>>(v, w, nrm2, n, stop);
+        diag_scale_kernel<Number><<<grid_for(n), T, 0, s>>>(z, v, d, n, stop);
       }
     return hipGetLastError();
   }
@@ -1014,38 +1042,144 @@ namespace mpdg
   template <typename Number>
   hipError_t launch_scale(Number *v, const Number *w, const double *nrm2, int64_t n, hipStream_t s)
   {
-    scale_kernel<Number><<<grid_for(n), T, 0, s>>>(v, w, nrm2, n);
+    scale_kernel<Number><<<grid_for(n), T, 0, s>>>(v, w, nrm2, n, nullptr);
     return hipGetLastError();
   }
 
   template <typename Number>
   hipError_t launch_diag_scale(Number *z, const Number *v, const Number *d, int64_t n, hipStream_t s)
   {
-    diag_scale_kernel<Number><<<grid_for(n), T, 0, s>>>(z, v, d, n);
+    diag_scale_kernel<Number><<<grid_for(n), T, 0, s>>>(z, v, d, n, nullptr);
     return hipGetLastError();
   }
 
   template <typename Number, typename XNumber>
   hipError_t launch_gmres_update(XNumber *x, const Number *V, int64_t ld, int k, const double *y,
-                                 const Number *d, int64_t n, hipStream_t s)
+                                 const Number *d, int64_t n, hipStream_t s, const int *kdev)
   {
     if (k < 1 || k > kMaxBasis)
       return hipErrorInvalidValue;
-    gmres_update_kernel<Number, XNumber><<<grid_for(n), T, 0, s>>>(x, V, ld, k, y, d, n);
+    gmres_update_kernel<Number, XNumber><<<grid_for(n), T, 0, s>>>(x, V, ld, k, y, d, n, kdev);
+    return hipGetLastError();
+  }
+
+  // ---- GMRES restart-cycle control on the device (one thread each) ----------
+  namespace
+  {
+    __global__ void gmres_init_kernel(GmresState *st, double tol, int max_it)
+    {
+      st->tol = tol;
+      st->max_it = max_it;
+      st->it = 0;
+      st->res = 0.0;
+      st->sc.done = 0;
+    }
+
+    __global__ void gmres_start_kernel(GmresState *st, const SolverScalars *rs, int restart)
+    {
+      st->sc.done = 0;
+      st->inner_conv = 0;
+      st->k = restart;
+      st->g[0] = rs->res;
+      for (int l = 1; l <= kMaxBasis; ++l)
+        st->g[l] = 0.0;
+    }
+
+    // The arithmetic of deal.II's Givens update restated: rotate the new
+    // column by the earlier rotations, annihilate its subdiagonal entry, rotate g.
+    __global__ void gmres_givens_kernel(GmresState *st, const double *h1, const double *h2,
+                                        const double *nrm2, int j, int restart)
+    {
+      if (st->sc.done)
+        return;
+      st->it += 1;
+      double col[kMaxBasis + 1];
+      for (int l = 0; l <= j; ++l)
+        col[l] = h1[l] + h2[l];
+      const double hn = sqrt(*nrm2);
+      col[j + 1] = hn;
+      for (int l = 0; l < j; ++l)
+        {
+          const double a = st->cs[l] * col[l] + st->sn[l] * col[l + 1];
+          const double b = -st->sn[l] * col[l] + st->cs[l] * col[l + 1];
+          col[l] = a;
+          col[l + 1] = b;
+        }
+      const double den = hypot(col[j], col[j + 1]);
+      st->cs[j] = col[j] / den;
+      st->sn[j] = col[j + 1] / den;
+      col[j] = den;
+      st->g[j + 1] = -st->sn[j] * st->g[j];
+      st->g[j] = st->cs[j] * st->g[j];
+      for (int l = 0; l <= j; ++l)
+        st->hm[l * kMaxBasis + j] = col[l];
+      const double res = fabs(st->g[j + 1]);
+      st->res = res;
+      if (res <= st->tol || st->it >= st->max_it || isnan(res) || hn == 0.0)
+        {
+          st->sc.done = 1;
+          st->k = j + 1;
+          st->inner_conv = res <= st->tol ? 1 : 0;
+        }
+      else if (j + 1 == restart)
+        st->k = restart;
+    }
+
+    __global__ void gmres_solve_kernel(GmresState *st)
+    {
+      const int k = st->k;
+      for (int l = k - 1; l >= 0; --l)
+        {
+          double sum = st->g[l];
+          for (int c = l + 1; c < k; ++c)
+            sum -= st->hm[l * kMaxBasis + c] * st->y[c];
+          st->y[l] = sum / st->hm[l * kMaxBasis + l];
+        }
+    }
+  } // namespace
+
+  hipError_t launch_gmres_init(GmresState *st, double tol, int max_it, hipStream_t s)
+  {
+    gmres_init_kernel<<<1, 1, 0, s>>>(st, tol, max_it);
+    return hipGetLastError();
+  }
+
+  hipError_t launch_gmres_start(GmresState *st, const SolverScalars *rs, int restart, hipStream_t s)
+  {
+    if (restart < 1 || restart > kMaxBasis - 1)
+      return hipErrorInvalidValue;
+    gmres_start_kernel<<<1, 1, 0, s>>>(st, rs, restart);
+    return hipGetLastError();
+  }
+
+  hipError_t launch_gmres_givens(GmresState *st, const double *h1, const double *h2,
+                                 const double *nrm2, int j, int restart, hipStream_t s)
+  {
+    if (j < 0 || j >= restart || restart > kMaxBasis - 1)
+      return hipErrorInvalidValue;
+    gmres_givens_kernel<<<1, 1, 0, s>>>(st, h1, h2, nrm2, j, restart);
+    return hipGetLastError();
+  }
+
+  hipError_t launch_gmres_solve(GmresState *st, hipStream_t s)
+  {
+    gmres_solve_kernel<<<1, 1, 0, s>>>(st);
     return hipGetLastError();
   }
 
 #define MPDG_GMRES_INST(NUM)                                                                     \
   template hipError_t launch_multi_dot<NUM>(const NUM *, int64_t, int, const NUM *, int64_t,     \
-                                            double *, double *, unsigned *, hipStream_t);        \
+                                            double *, double *, unsigned *, hipStream_t,         \
+                                            const int *);                                        \
   template hipError_t launch_multi_axpy<NUM>(NUM *, const NUM *, int64_t, int, const double *,   \
                                              int64_t, double *, double *, unsigned *,            \
-                                             hipStream_t);                                       \
+                                             hipStream_t, const int *);                          \
   template hipError_t launch_multi_axpy_dot<NUM>(NUM *, const NUM *, int64_t, int, const double *,\
                                                  int64_t, double *, double *, unsigned *,        \
-                                                 hipStream_t);                                   \
+                                                 hipStream_t, const int *);                      \
   template hipError_t launch_scale_precondition<NUM>(NUM *, NUM *, const NUM *, const NUM *,      \
-                                                     const double *, int64_t, hipStream_t);      \
+                                                     const double *, int64_t, hipStream_t,       \
+                                                     const int *);                               \
   template hipError_t launch_scale<NUM>(NUM *, const NUM *, const double *, int64_t, hipStream_t); \
   template hipError_t launch_diag_scale<NUM>(NUM *, const NUM *, const NUM *, int64_t, hipStream_t);
   MPDG_GMRES_INST(double)
@@ -1053,10 +1187,10 @@ namespace mpdg
 #undef MPDG_GMRES_INST
   template hipError_t launch_gmres_update<double, double>(double *, const double *, int64_t, int,
                                                           const double *, const double *, int64_t,
-                                                          hipStream_t);
+                                                          hipStream_t, const int *);
   template hipError_t launch_gmres_update<float, float>(float *, const float *, int64_t, int,
                                                         const double *, const float *, int64_t,
-                                                        hipStream_t);
+                                                        hipStream_t, const int *);
 
   template hipError_t launch_cg_init<double>(const double *, const double *, double *, double *,
                                              const double *, int64_t, double, double, int,

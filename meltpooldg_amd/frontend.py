@@ -508,16 +508,18 @@ def make_operator(case, stream=None):
     return op
 
 
-def run(prm, log=print, max_steps=None):
+def run(prm, log=print, max_steps=None, graphs=True):
     """HeatTransferApplication::run on the device. Returns one record per time
     step: step, time, dt, the L2 norm of the solution (in the case's own
-    dimension) and the Newton report."""
+    dimension) and the Newton report. ``graphs``: replay the Krylov restart
+    cycles / iteration chunks as captured hipGraphs (False: eager launches)."""
     import torch
     from .heat import HeatDiffuseOperation
     if not isinstance(prm, Parameters):
         prm = parse_parameters(prm)
     case = setup(prm)
     op = make_operator(case)
+    op.set_graphs(graphs)
     if case.heaviside is not None:
         op.set_heaviside(torch.tensor(case.heaviside, dtype=torch.float64, device="cuda"))
     T0 = torch.tensor(case.temperature0, dtype=torch.float64, device="cuda")

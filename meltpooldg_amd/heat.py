@@ -245,6 +245,11 @@ class HeatDiffuseOperator:
         L.check(self._lib.mpdg_heat_set_temperature_old(self._h, _ptr(temperature_old)))
         self._temperature_old = temperature_old
 
+    def set_graphs(self, enable=True):
+        """Replay the Krylov iteration chunks of this operator's solves as
+        captured hipGraphs (default) or launch every kernel eagerly."""
+        L.check(self._lib.mpdg_heat_set_graphs(self._h, 1 if enable else 0))
+
     def set_heaviside(self, heaviside):
         """Register the level-set heaviside field (read plain, interpolated at q)."""
         self._check_vec(heaviside, "float64")
