@@ -268,7 +268,7 @@ namespace mpdg
         }
       if constexpr (DOT)
         {
-          if (scal == nullptr)
+          if (scal == nullptr || partials == nullptr)
             return;
           // p.(A_face p): block sums, then the last workgroup adds them in
           // index order to the cell loop's p.Ap and resets alpha

@@ -537,8 +537,9 @@ namespace
 
 
   // The cell loop, then the Robin faces (MatrixFree::loop's boundary-face part)
-  // on the same stream. With scal, the cell loop leaves p.Ap of the cells in
-  // scal->pap and the face launches add the faces' part and set alpha. (Running
+  // on the same stream. With partials and scal, the cell loop leaves p.Ap of the
+  // cells in scal->pap and the face launches add the faces' part and set alpha;
+  // scal alone only stops the launches once scal->done is set. (Running
   // the face-cell integrals on a second stream beside the cell loop measured
   // slower: the fork and join cost more than the overlap saves.)
   template <typename Number>
@@ -752,11 +753,12 @@ namespace
     std::vector<mpdg_heat::ChunkGraph> &graphs = H->gmres_graphs[sizeof(Number) == 4];
     if ((int)graphs.size() < nchunks + 1)
       graphs.resize(nchunks + 1);
+    // with `stop` the vmult does nothing once the cycle has stopped (no p.Ap)
     auto op = [&](Number *dst, const Number *src, SolverScalars *stop) -> mpdg_status {
       if constexpr (sizeof(Number) == 8)
-        return vmult64(H, dst, src, stop ? part : nullptr, stop);
+        return vmult64(H, dst, src, nullptr, stop);
       else
-        return vmult32(H, dst, src, stop ? part : nullptr, stop);
+        return vmult32(H, dst, src, nullptr, stop);
     };
     // r = b - A x and ||r|| into sc->res (z receives D r, unused)
     auto residual = [&](bool zero_x) -> mpdg_status {

@@ -128,9 +128,9 @@ namespace mpdg
   int threads_per_block(int degree, bool fp32);
   ColourGrid colour_grid(int nx, int ny, int nz, int cpb);
 
-  // Cell-loop vmult: dst = A src over all 8 colour passes. With scal != nullptr
-  // it also reduces p.Ap into scal->pap and sets scal->alpha = rz / pap, and it
-  // does nothing once scal->done is set.
+  // Cell-loop vmult: dst = A src over all 8 colour passes. With partials (and
+  // scal) it also reduces p.Ap into scal->pap and sets scal->alpha = rz / pap;
+  // with scal it does nothing once scal->done is set.
   template <typename Number>
   hipError_t launch_vmult(int degree, int coef, bool vel, const CellParams<Number> &prm,
                           const VmultTables<Number> &tab, const Number *src, Number *dst,
@@ -162,10 +162,10 @@ namespace mpdg
   // partials needs l2_norm_blocks() slots, *ticket must start at 0 (reset on exit).
   int l2_norm_blocks(int degree, int nx, int ny, int nz);
   // Tangent of a temperature-dependent material (material_kernels.hip), FP64:
-  // dst = J src over 8 colour passes (dst is zeroed first), the material and
-  // its temperature derivatives evaluated at T(q) (T and T_old read plain,
-  // heaviside when prm.tmat == 3). With scal != nullptr it reduces p.Ap into
-  // scal->pap and sets scal->alpha = rz / pap like launch_vmult.
+  // dst = J src over 8 colour passes (coloured first-touch scatter), the
+  // material and its temperature derivatives evaluated at T(q) (T and T_old
+  // read plain, heaviside when prm.tmat == 3). Same p.Ap / stop contract as
+  // launch_vmult.
   hipError_t launch_tmat_vmult(int degree, bool vel, const CellParams<double> &prm,
                                const double *src, double *dst, const double *temperature,
                                const double *temperature_old, const double *heaviside,

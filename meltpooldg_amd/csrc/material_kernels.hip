@@ -209,7 +209,9 @@ namespace mpdg
       __syncthreads();
     }
 
-    // dst = J src, one colour pass; partial p.Ap of the pass when scal is given.
+    // dst = J src, one colour pass; partial p.Ap of the pass when partials is
+    // given (then scal receives p.Ap); a given scal alone only stops the launch
+    // once scal->done is set.
     template <int P, bool VEL, int CPB, int THREADS>
     __global__ void __launch_bounds__(THREADS)
       tmat_vmult_kernel(const double *__restrict__ src, double *__restrict__ dst,
@@ -421,7 +423,7 @@ namespace mpdg
             if (a < N - 1 || last)
               acc += ul[a] * Y[a];
         }
-      if (scal != nullptr)
+      if (partials != nullptr)
         pass_partials<THREADS>(acc, partials, offset, total, finalize, scal);
     }
 
