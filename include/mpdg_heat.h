@@ -293,11 +293,13 @@ mpdg_status mpdg_heat_distribute_constraints(mpdg_heat *h, double *v);
 mpdg_status mpdg_heat_l2_norm(mpdg_heat *h, const double *v, double *norm);
 
 /* Krylov iteration chunks as hipGraphs (no counterpart in the reference, whose
- * solvers run on the host): with enable != 0 (the default) the PCG solves
- * replay one captured graph per chunk of check_interval iterations after the
- * first chunk, on a capture stream of the handle's own that is ordered after
- * and before the handle's stream by events. Results are bitwise those of the
- * eager launches; 0 launches every kernel eagerly. */
+ * solvers run on the host): with enable != 0 (the default) a solve runs on a
+ * capture stream of the handle's own, ordered after and before the handle's
+ * stream by events, and replays captured graphs of its chunks (PCG:
+ * check_interval iterations; GMRES: a few Arnoldi steps of a restart cycle,
+ * and the cycle's end), each captured the second time it is seen with the
+ * same vectors and operator state. Results are bitwise those of the eager
+ * launches; 0 launches every kernel eagerly on the handle's stream. */
 mpdg_status mpdg_heat_set_graphs(mpdg_heat *h, int enable);
 
 /* Launch tuning of the vmult kernels, one set per precision (no counterpart
