@@ -51,6 +51,13 @@ typedef enum {
                                         temperature (FP64) */
 } mpdg_material_type;
 
+/* ApparentCapacityType (include/meltpooldg/core/material_data.hpp:43-62). */
+typedef enum {
+  MPDG_APPARENT_CAPACITY_POLY4_BELL = 0, /* L (2/dT) 15/16 (1 - xi^2)^2, xi = 1 - 2 fs */
+  MPDG_APPARENT_CAPACITY_CONSTANT = 1,   /* L / dT */
+  MPDG_APPARENT_CAPACITY_QLQ = 2         /* quadratic-linear-quadratic, peak 2 L / dT */
+} mpdg_apparent_capacity;
+
 /* Extra conductivity field on top of the material's conductivity. */
 typedef enum {
   MPDG_CONDUCTIVITY_MATERIAL = 0,
@@ -84,10 +91,15 @@ typedef struct {
    * material_data.hpp): the solid phase and the solid fraction's law, mushy zone
    * clamp((liquidus - T) / (liquidus - solidus), 0, 1) or, when
    * solid_liquid_sharp (SolidLiquidPropertiesTransitionType::sharp), 1 below
-   * the solidus temperature. No apparent capacity (latent heat of fusion 0). */
+   * the solidus temperature. */
   mpdg_phase solid;
   double solidus_temperature, liquidus_temperature;
   int solid_liquid_sharp;
+  /* latent heat of fusion > 0: its apparent capacity (Heat::ApparentCapacity,
+   * include/meltpooldg/heat/apparent_capacity.hpp:59) is added to c_p in the
+   * mushy zone, of type mpdg_apparent_capacity; not with volume_specific_heat */
+  double latent_heat_of_fusion;
+  int apparent_capacity_type;
   int conductivity_field;      /* mpdg_conductivity_field */
   double conductivity_amplitude;
   int velocity;                /* mpdg_velocity_field */

@@ -41,6 +41,8 @@ class HeatDesc(ctypes.Structure):
         ("solidus_temperature", ctypes.c_double),
         ("liquidus_temperature", ctypes.c_double),
         ("solid_liquid_sharp", ctypes.c_int),
+        ("latent_heat_of_fusion", ctypes.c_double),
+        ("apparent_capacity_type", ctypes.c_int),
         ("conductivity_field", ctypes.c_int),
         ("conductivity_amplitude", ctypes.c_double),
         ("velocity", ctypes.c_int),

@@ -163,10 +163,69 @@ namespace mpdg
     // fraction fs = clamp((liquidus - T) / (liquidus - solidus), 0, 1), or
     // T < solidus for the sharp law; each property blends liquid and solid by
     // the cubic smoothstep of fs (exact phase values and zero derivative for a
-    // pure phase), then gas_liquid_solid blends the gas value with weight 1 - w
+    // pure phase), c_p gains the apparent capacity of a latent heat of fusion
+    // (material.templates.hpp:563-617), then gas_liquid_solid blends the gas
+    // value with weight 1 - w
     // (w = H, or H > 0.5 when sharp) and scales the derivative by w; rho_cp and
     // its derivative as get_material_parameters_and_derivatives
     // (heat_diffuse_operator.cpp:1167-1210).
+    // C1 ramp from (x1, y1) to (x2, y2), zero slope at both ends over a width
+    // delta, linear in between (slope (y2 - y1) / (x2 - x1 - delta)).
+    template <typename Number>
+    __device__ __forceinline__ void qlq_ramp(Number x, Number x1, Number x2, Number y1, Number y2,
+                                             Number delta, Number &v, Number &dv)
+    {
+      const Number s = (y2 - y1) / (x2 - x1 - delta);
+      if (x < x1 + delta)
+        {
+          const Number d = x - x1;
+          v = y1 + Number(0.5) * s * d * d / delta;
+          dv = s * d / delta;
+        }
+      else if (x > x2 - delta)
+        {
+          const Number d = x2 - x;
+          v = y2 - Number(0.5) * s * d * d / delta;
+          dv = s * d / delta;
+        }
+      else
+        {
+          v = y1 + s * (x - x1 - Number(0.5) * delta);
+          dv = s;
+        }
+    }
+
+    // Heat::ApparentCapacity (source/heat/apparent_capacity.cpp:157-311): the
+    // latent heat L as an extra c_p inside the mushy zone (0 < fs < 1) and its
+    // temperature derivative: poly4_bell L (2/dT) 15/16 (1 - xi^2)^2 with
+    // xi = 1 - 2 fs; constant L / dT; qlq C1 ramps 0 -> 2 L / dT -> 0 over
+    // (T_s, T_m, T_l) with transition width dT / 6.
+    template <typename Number>
+    __device__ __forceinline__ void apparent_capacity(const CellParams<Number> &prm, Number fs,
+                                                      Number &c, Number &dc)
+    {
+      c = dc = Number(0);
+      if (!(prm.latent > Number(0)) || fs <= Number(0) || fs >= Number(1))
+        return;
+      const Number Ts = prm.solidus, Tl = prm.liquidus, dT = Tl - Ts, L = prm.latent;
+      if (prm.ac_type == 1)
+        c = L / dT;
+      else if (prm.ac_type == 0)
+        {
+          const Number r = Number(2) / dT, xi = Number(1) - Number(2) * fs, q = Number(1) - xi * xi;
+          c = L * r * Number(15.0 / 16.0) * q * q;
+          dc = L * r * r * Number(15.0 / 4.0) * xi * (xi * xi - Number(1));
+        }
+      else
+        {
+          const Number T = Tl - fs * dT, Tm = Number(0.5) * (Ts + Tl), peak = Number(2) * L / dT;
+          if (T <= Tm)
+            qlq_ramp(T, Ts, Tm, Number(0), peak, dT / Number(6), c, dc);
+          else
+            qlq_ramp(T, Tm, Tl, peak, Number(0), dT / Number(6), c, dc);
+        }
+    }
+
     template <typename Number>
     __device__ __forceinline__ void solid_liquid_material(const CellParams<Number> &prm, Number H,
                                                           Number T, Number &rc, Number &k,
@@ -191,6 +250,10 @@ namespace mpdg
       Number cp, dcp, rho, drho, vc, dvc;
       blend(prm.l_k, prm.s_k, k, dk);
       blend(prm.l_cp, prm.s_cp, cp, dcp);
+      Number ac, dac;
+      apparent_capacity(prm, fs, ac, dac);
+      cp += ac;
+      dcp += dac;
       blend(prm.l_rho, prm.s_rho, rho, drho);
       blend(prm.l_rhocp, prm.s_rhocp, vc, dvc);
       if (prm.tmat == 3)

@@ -364,6 +364,8 @@ namespace
     prm.inv_mushy = (Number)(d.liquidus_temperature > d.solidus_temperature ?
                                1.0 / (d.liquidus_temperature - d.solidus_temperature) :
                                0.0);
+    prm.ac_type = d.apparent_capacity_type;
+    prm.latent = (Number)(prm.tmat != 0 ? d.latent_heat_of_fusion : 0.0);
     const mpdg_tuning &tu = H->tune[sizeof(Number) == 4 ? MPDG_FP32 : MPDG_FP64];
     prm.zlayers = std::max(1, std::min(tu.z_layers, d.n_cells[2]));
     prm.blocks_per_cu = tu.blocks_per_cu;
@@ -932,6 +934,22 @@ mpdg_status mpdg_heat_create(const mpdg_heat_desc *desc, mpdg_heat **out)
                                   !(d.liquidus_temperature > d.solidus_temperature)))
         return fail(MPDG_ERR_ARGUMENT, "the mushy zone needs liquidus > solidus temperature "
                                        "(>= for the sharp solid-liquid transition)");
+      if (d.latent_heat_of_fusion != 0.0)
+        {
+          // Heat::ApparentCapacity's and Material's checks (apparent_capacity.cpp:165-168,
+          // material.templates.hpp:276-281)
+          if (!(d.latent_heat_of_fusion > 0.0) || !std::isfinite(d.latent_heat_of_fusion) ||
+              d.apparent_capacity_type < MPDG_APPARENT_CAPACITY_POLY4_BELL ||
+              d.apparent_capacity_type > MPDG_APPARENT_CAPACITY_QLQ)
+            return fail(MPDG_ERR_ARGUMENT, "latent heat of fusion must be finite and >= 0 with an "
+                                           "apparent capacity type of mpdg_apparent_capacity");
+          if (!(d.liquidus_temperature > d.solidus_temperature))
+            return fail(MPDG_ERR_ARGUMENT, "The liquidus temperature must be greater than the "
+                                           "solidus temperature for the apparent capacity");
+          if (d.volume_specific_heat)
+            return fail(MPDG_ERR_UNSUPPORTED, "the apparent capacity is not implemented for the "
+                                              "volume specific heat capacity");
+        }
     }
   if (d.velocity != MPDG_VELOCITY_NONE && d.velocity != MPDG_VELOCITY_VORTEX &&
       d.velocity != MPDG_VELOCITY_CONSTANT)

@@ -33,6 +33,10 @@ namespace mpdg
     int tmat, sl_sharp;
     Number s_rho, s_cp, s_k, s_rhocp;
     Number solidus, liquidus, inv_mushy;
+    // apparent capacity of the latent heat of fusion (latent = 0: none):
+    // ac_type 0 poly4_bell, 1 constant, 2 qlq
+    int ac_type;
+    Number latent;
     int zlayers;              // cell layers per z-chunk of a vmult work item
     int blocks_per_cu;        // persistent workgroups per CU (0: occupancy limit)
     int variant;              // vmult form: kVariantLine or kVariantCell

@@ -12,9 +12,9 @@
   (source/core/material_data.cpp), the application-specific and the
   case-specific parameters. An unknown key in a modelled section is an error,
   as in the reference's ParameterHandler; settings that ask for physics this
-  engine does not run (cut operator, evaporation, laser, AMR, the apparent
-  capacity of a latent heat of fusion, time-dependent boundary conditions,
-  least-squares predictor) raise ``UnsupportedFeature`` naming the key. The
+  engine does not run (cut operator, evaporation, laser, AMR, time-dependent
+  boundary conditions, least-squares predictor) raise ``UnsupportedFeature``
+  naming the key. The
   ILU preconditioner, which needs an assembled sparse matrix, runs as the
   Jacobi (Diagonal) preconditioner: the Newton solution does not depend on the
   preconditioner beyond the linear tolerance, only the Krylov iteration counts
@@ -275,8 +275,13 @@ def check_supported(prm):
     if prm.application_specific.do_solidification:
         m = prm.material
         if m.latent_heat_of_fusion > 0.0:
-            raise UnsupportedFeature("material/latent heat of fusion > 0: the apparent capacity "
-                                     "is not built")
+            if m.apparent_capacity_type not in ("poly4_bell", "constant", "qlq"):
+                raise UnsupportedFeature(f"material/apparent capacity type = "
+                                         f"{m.apparent_capacity_type!r}")
+            if prm.heat.diffuse.use_volume_specific_thermal_capacity_for_phase_interpolation:
+                raise UnsupportedFeature("material/latent heat of fusion with the volume-specific "
+                                         "heat capacity: the reference's apparent capacity is not "
+                                         "implemented for it")
         if m.solid_liquid_properties_transition_type not in ("mushy_zone", "sharp"):
             raise UnsupportedFeature("material/solid liquid properties transition type = "
                                      f"{m.solid_liquid_properties_transition_type!r}")
@@ -445,7 +450,8 @@ def _unidirectional_heat_transfer(prm):
             prm.heat.diffuse.use_volume_specific_thermal_capacity_for_phase_interpolation),
         solid_liquid_properties_transition_type=m.solid_liquid_properties_transition_type,
         solidus_temperature=m.solidus_temperature, liquidus_temperature=m.liquidus_temperature,
-        latent_heat_of_fusion=m.latent_heat_of_fusion)
+        latent_heat_of_fusion=m.latent_heat_of_fusion,
+        apparent_capacity_type=m.apparent_capacity_type)
     return CaseSetup(dimension=dim, n_cells=n_cells, degree=degree, box=box, extrusion=depth,
                      material=material, two_phase=two_phase, heaviside=heaviside,
                      velocity=velocity, convection=conv, radiation=rad,

@@ -45,6 +45,7 @@ class PhaseData:
 
 
 MATERIAL_TYPES = {"gas": 0, "gas_liquid": 1, "liquid_solid": 2, "gas_liquid_solid": 3}
+APPARENT_CAPACITY_TYPES = {"poly4_bell": 0, "constant": 1, "qlq": 2}
 
 
 @dataclass
@@ -53,8 +54,9 @@ class MaterialData:
     'gas_liquid', 'liquid_solid' or 'gas_liquid_solid') with the MaterialData
     fields it reads (include/meltpooldg/core/material_data.hpp). The
     temperature-dependent types take the solid fraction from the solidus and
-    liquidus temperatures ('mushy_zone' or 'sharp'); the apparent capacity
-    (latent heat of fusion > 0) is not built."""
+    liquidus temperatures ('mushy_zone' or 'sharp') and, with a latent heat of
+    fusion > 0, add its apparent capacity ('poly4_bell', 'constant' or 'qlq',
+    include/meltpooldg/heat/apparent_capacity.hpp) to c_p."""
     type: str = "gas"
     gas: PhaseData = field(default_factory=PhaseData)
     liquid: PhaseData = field(default_factory=PhaseData)
@@ -65,6 +67,7 @@ class MaterialData:
     solidus_temperature: float = 0.0
     liquidus_temperature: float = 0.0
     latent_heat_of_fusion: float = 0.0
+    apparent_capacity_type: str = "qlq"
 
 
 @dataclass
@@ -130,8 +133,9 @@ def _make_desc(n_cells, degree, material, box, precision, conductivity_amplitude
     d.precision = L.MPDG_FP32 if precision == "fp32" else L.MPDG_FP64
     if mat.type not in MATERIAL_TYPES:
         raise ValueError(f"material type {mat.type!r} is not built ({sorted(MATERIAL_TYPES)})")
-    if mat.latent_heat_of_fusion > 0.0:
-        raise ValueError("latent heat of fusion > 0 (apparent capacity) is not built")
+    if mat.apparent_capacity_type not in APPARENT_CAPACITY_TYPES:
+        raise ValueError(f"apparent capacity type {mat.apparent_capacity_type!r} is not one of "
+                         f"{sorted(APPARENT_CAPACITY_TYPES)}")
     if mat.solid_liquid_properties_transition_type not in ("mushy_zone", "sharp"):
         raise ValueError("solid liquid properties transition type must be mushy_zone or sharp")
     d.material = MATERIAL_TYPES[mat.type]
@@ -144,6 +148,8 @@ def _make_desc(n_cells, degree, material, box, precision, conductivity_amplitude
     d.solidus_temperature = float(mat.solidus_temperature)
     d.liquidus_temperature = float(mat.liquidus_temperature)
     d.solid_liquid_sharp = int(mat.solid_liquid_properties_transition_type == "sharp")
+    d.latent_heat_of_fusion = float(mat.latent_heat_of_fusion)
+    d.apparent_capacity_type = APPARENT_CAPACITY_TYPES[mat.apparent_capacity_type]
     d.conductivity_field = 1 if conductivity_amplitude else 0
     d.conductivity_amplitude = float(conductivity_amplitude)
     d.velocity = {None: 0, "none": 0, "vortex": 1, "constant": 2}[velocity]

@@ -226,6 +226,51 @@ def robin_face_parts(n, p, robin, T_tangent, T_rhs, mask, lo=(0.0, 0.0, 0.0),
     return A, r
 
 
+def apparent_capacity_t(T, mat):
+    """(c, dc/dT) of the apparent capacity at temperatures T (own numpy statement
+    of source/heat/apparent_capacity.cpp): the latent heat L over the mushy zone
+    (T_s, T_l), zero outside it, as a function of the temperature directly:
+    poly4_bell  L (2/dT) 15/16 (1 - xi^2)^2 with xi = (2 T - T_s - T_l) / dT,
+    constant    L / dT,
+    qlq         two zero-slope quadratic-linear-quadratic ramps, 0 -> 2 L / dT
+                on (T_s, T_m) and back on (T_m, T_l), transition width dT / 6,
+                built here from the ramp's integral form."""
+    T = np.asarray(T, dtype=float)
+    L = float(mat.get("latent_heat", 0.0))
+    zero = np.zeros_like(T)
+    if L <= 0.0:
+        return zero, zero
+    Ts, Tl = mat["solidus"], mat["liquidus"]
+    dT = Tl - Ts
+    inside = (T > Ts) & (T < Tl)
+    kind = mat.get("apparent_capacity", "qlq")
+    if kind == "constant":
+        return np.where(inside, L / dT, 0.0), zero
+    if kind == "poly4_bell":
+        xi = (2.0 * T - Ts - Tl) / dT
+        c = L * (2.0 / dT) * 15.0 / 16.0 * (1.0 - xi ** 2) ** 2
+        dc = L * (2.0 / dT) ** 2 * 15.0 / 4.0 * (xi ** 3 - xi)
+        return np.where(inside, c, 0.0), np.where(inside, dc, 0.0)
+    # qlq: slope profile of a ramp rising by `rise` over (a, b): linear up over
+    # width d, flat, linear down over d; the ramp is its integral
+    Tm, peak, d = 0.5 * (Ts + Tl), 2.0 * L / dT, dT / 6.0
+
+    def ramp(x, a, b, y0, rise):
+        s = rise / (b - a - d)
+        u = np.clip(x - a, 0.0, d)                 # rising part of the slope
+        w = np.clip(x - (b - d), 0.0, d)           # falling part
+        mid = np.clip(x - a - d, 0.0, b - a - 2 * d)
+        val = y0 + s * (u * u / (2 * d) + mid + (w - w * w / (2 * d)))
+        slope = np.where(x < a + d, s * (x - a) / d, np.where(x > b - d, s * (b - x) / d, s))
+        return val, slope
+
+    up_v, up_s = ramp(T, Ts, Tm, 0.0, peak)
+    dn_v, dn_s = ramp(T, Tm, Tl, peak, -peak)
+    c = np.where(T <= Tm, up_v, dn_v)
+    dc = np.where(T <= Tm, up_s, dn_s)
+    return np.where(inside, c, 0.0), np.where(inside, dc, 0.0)
+
+
 def material_t(H, T, mat):
     """(rho_cp, k, d rho_cp/dT, dk/dT) at quadrature points for the
     temperature-dependent materials liquid_solid and gas_liquid_solid (own numpy
@@ -250,6 +295,8 @@ def material_t(H, T, mat):
 
     k, dk = blend(mat["liq_k"], mat["sol_k"])
     cp, dcp = blend(mat["liq_cp"], mat["sol_cp"])
+    ac, dac = apparent_capacity_t(T, mat)
+    cp, dcp = cp + ac, dcp + dac
     rho, drho = blend(mat["liq_rho"], mat["sol_rho"])
     vc, dvc = blend(mat["liq_rho"] * mat["liq_cp"], mat["sol_rho"] * mat["sol_cp"])
     if mat["type"] == "gas_liquid_solid":

@@ -47,8 +47,12 @@ class OracleProblem(ctypes.Structure):
         ("solidus", ctypes.c_double),
         ("liquidus", ctypes.c_double),
         ("solid_liquid_sharp", ctypes.c_int),
+        ("latent_heat", ctypes.c_double),
+        ("apparent_capacity", ctypes.c_int),
         ("temperature_old", ctypes.POINTER(ctypes.c_double)),
     ]
+
+APPARENT_CAPACITY_TYPES = {"poly4_bell": 0, "constant": 1, "qlq": 2}
 
 MATERIAL_TYPES = {"gas": 0, "gas_liquid": 1, "liquid_solid": 2, "gas_liquid_solid": 3}
 
@@ -130,6 +134,8 @@ def _fill_material(s, mat):
     s.solidus = mat.get("solidus", 0.0)
     s.liquidus = mat.get("liquidus", 0.0)
     s.solid_liquid_sharp = int(mat.get("solid_liquid_sharp", 0))
+    s.latent_heat = float(mat.get("latent_heat", 0.0))
+    s.apparent_capacity = APPARENT_CAPACITY_TYPES[mat.get("apparent_capacity", "qlq")]
     s.sharp = int(mat.get("sharp", 0))
     s.volume_specific = int(mat.get("volume_specific", 0))
     s.consistent_with_evaporation = int(mat.get("consistent_with_evaporation", 0))

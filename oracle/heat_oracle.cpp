@@ -193,6 +193,12 @@ namespace
         if (pb->solid_liquid_sharp ? !(pb->liquidus >= pb->solidus) :
                                      !(pb->liquidus > pb->solidus))
           return false;
+        // the reference rejects the apparent capacity with volume-specific
+        // interpolation and needs liquidus > solidus for it
+        if (pb->latent_heat > 0.0 &&
+            (pb->volume_specific || !(pb->liquidus > pb->solidus) || pb->apparent_capacity < 0 ||
+             pb->apparent_capacity > 2))
+          return false;
       }
     return true;
   }
@@ -276,6 +282,64 @@ namespace
     return f > 1.0 ? 1.0 : (f < 0.0 ? 0.0 : f);
   }
 
+  // C1 ramp from (x1, y1) to (x2, y2): quadratic with zero slope at either end
+  // over a width delta, linear in between with the slope s that makes the
+  // pieces meet, s = (y2 - y1) / (x2 - x1 - delta) (the quadratic-linear-
+  // quadratic profile of source/heat/apparent_capacity.cpp:42-155).
+  inline void qlq_ramp(double x, double x1, double x2, double y1, double y2, double delta,
+                       double &v, double &dv)
+  {
+    const double s = (y2 - y1) / (x2 - x1 - delta);
+    if (x < x1 + delta)
+      {
+        const double d = x - x1;
+        v = y1 + 0.5 * s * d * d / delta;
+        dv = s * d / delta;
+      }
+    else if (x > x2 - delta)
+      {
+        const double d = x2 - x;
+        v = y2 - 0.5 * s * d * d / delta;
+        dv = s * d / delta;
+      }
+    else
+      {
+        v = y1 + s * (x - x1 - 0.5 * delta);
+        dv = s;
+      }
+  }
+
+  // Heat::ApparentCapacity (source/heat/apparent_capacity.cpp:157-311): the
+  // latent heat L spread over the mushy zone as an extra c_p, zero outside
+  // 0 < fs < 1, and its temperature derivative:
+  //   poly4_bell  L (2/dT) b(xi), b(x) = 15/16 (1 - x^2)^2, xi = 1 - 2 fs;
+  //   constant    L / dT;
+  //   qlq         a C1 ramp from 0 at the solidus to 2 L / dT at the mid
+  //               temperature and back to 0 at the liquidus (width dT / 6).
+  inline void apparent_capacity(const oracle_problem *pb, double fs, double &c, double &dc)
+  {
+    c = dc = 0.0;
+    if (!(pb->latent_heat > 0.0) || fs <= 0.0 || fs >= 1.0)
+      return;
+    const double Ts = pb->solidus, Tl = pb->liquidus, dT = Tl - Ts, L = pb->latent_heat;
+    if (pb->apparent_capacity == 1)
+      c = L / dT;
+    else if (pb->apparent_capacity == 0)
+      {
+        const double r = 2.0 / dT, xi = 1.0 - 2.0 * fs, q = 1.0 - xi * xi;
+        c = L * r * (15.0 / 16.0) * q * q;
+        dc = L * r * r * (15.0 / 4.0) * xi * (xi * xi - 1.0);
+      }
+    else
+      {
+        const double T = Tl - fs * dT, Tm = 0.5 * (Ts + Tl), peak = 2.0 * L / dT;
+        if (T <= Tm)
+          qlq_ramp(T, Ts, Tm, 0.0, peak, dT / 6.0, c, dc);
+        else
+          qlq_ramp(T, Tm, Tl, peak, 0.0, dT / 6.0, c, dc);
+      }
+  }
+
   // Liquid/solid blend of one property by the cubic smoothstep of the solid
   // fraction fs, and its temperature derivative (compute_solid_liquid_phases_property
   // and compute_temperature_derivative_of_solid_liquid_phases_property,
@@ -318,6 +382,12 @@ namespace
     double vc, dvc;
     solid_liquid(fs, inv_mushy, pb->liq_k, pb->sol_k, s.k, s.d_k);
     solid_liquid(fs, inv_mushy, pb->liq_cp, pb->sol_cp, s.cp, s.d_cp);
+    // compute_solid_liquid_phases_specific_heat_capacity (material.templates.hpp:
+    // 563-577, 600-617): c_p and its derivative carry the apparent capacity
+    double ac, dac;
+    apparent_capacity(pb, fs, ac, dac);
+    s.cp += ac;
+    s.d_cp += dac;
     solid_liquid(fs, inv_mushy, pb->liq_rho, pb->sol_rho, s.rho, s.d_rho);
     solid_liquid(fs, inv_mushy, pb->liq_rho * pb->liq_cp, pb->sol_rho * pb->sol_cp, vc, dvc);
     if (pb->material == 3)

@@ -82,10 +82,14 @@ typedef struct {
   double velocity_value[3]; /* velocity == 2 */
   /* materials 2, 3: the solid phase and the solid fraction's temperature law
    * (mushy zone: clamp((liquidus - T) / (liquidus - solidus), 0, 1); sharp:
-   * T < solidus). No apparent capacity (latent heat of fusion 0). */
+   * T < solidus), and the apparent capacity of a latent heat of fusion. */
   double sol_rho, sol_cp, sol_k;
   double solidus, liquidus;
   int solid_liquid_sharp;
+  /* apparent capacity of the latent heat of fusion (0: none) added to c_p in
+   * the mushy zone: 0 = poly4_bell, 1 = constant, 2 = qlq (ApparentCapacityType) */
+  double latent_heat;
+  int apparent_capacity;
   /* materials 2, 3: the tangent's cells read `temperature` and this old
    * temperature plain (read_dof_values_plain, heat_diffuse_operator.cpp:1004-1014) */
   const double *temperature_old;

@@ -56,7 +56,9 @@ def engine_operator(case, precision="fp64"):
         use_volume_specific_thermal_capacity_for_phase_interpolation=bool(m.get("volume_specific")),
         solid_liquid_properties_transition_type=("sharp" if m.get("solid_liquid_sharp") else
                                                  "mushy_zone"),
-        solidus_temperature=m.get("solidus", 0.0), liquidus_temperature=m.get("liquidus", 0.0))
+        solidus_temperature=m.get("solidus", 0.0), liquidus_temperature=m.get("liquidus", 0.0),
+        latent_heat_of_fusion=m.get("latent_heat", 0.0),
+        apparent_capacity_type=m.get("apparent_capacity", "qlq"))
     op = HeatDiffuseOperator(case["n"], case["degree"], material=mat, precision=precision,
                              conductivity_amplitude=case["k_amplitude"],
                              velocity=("constant" if case["velocity"] == "constant" else

@@ -250,7 +250,15 @@ def test_solidification_case_set_up():
     assert cs.material.type == "liquid_solid" and cs.heaviside is None and cs.heaviside_at is None
     assert cs.velocity == (-0.005, 0.0, 0.0)
     base = CASES["unidirectional_heat_transfer_with_velocity_and_solidification"]["input"]
-    bad = json.loads(json.dumps(base))
-    bad["material"]["latent heat of fusion"] = 2.5e5
+    # a latent heat of fusion adds its apparent capacity (default type qlq)
+    latent = json.loads(json.dumps(base))
+    latent["material"]["latent heat of fusion"] = 2.5e5
+    latent["material"]["apparent capacity type"] = "poly4_bell"
+    cs = F.setup(F.parse_parameters(latent))
+    assert cs.material.latent_heat_of_fusion == 2.5e5
+    assert cs.material.apparent_capacity_type == "poly4_bell"
+    # ... but, as in the reference, not with the volume-specific heat capacity
+    bad = json.loads(json.dumps(latent))
+    bad["heat"]["diffuse"] = {"use volume-specific thermal capacity for phase interpolation": True}
     with pytest.raises(F.UnsupportedFeature, match="latent heat"):
         F.parse_parameters(bad)

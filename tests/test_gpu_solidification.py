@@ -12,7 +12,9 @@ oracle's iteration counts and solutions within 1e-8; the Newton time step
 the oracle's increments and state within 1e-9. The temperature fields cross
 the whole mushy zone, so pure liquid, pure solid and mushy quadrature points
 all occur; the steel-like phases (k 55.6 liquid / 17 solid, cp 460 / 700)
-and dt make the derivative terms a visible part of the tangent.
+and dt make the derivative terms a visible part of the tangent. The latent
+heat of fusion's apparent capacity (poly4_bell, constant, qlq) is checked the
+same way on every degree.
 """
 import numpy as np
 import pytest
@@ -172,3 +174,39 @@ def test_solidification_newton_time_step_matches_oracle():
     out = opn.advance(0.01)
     assert out["iterations"] == rep["iterations"], (out, rep)
     assert rel_err(_np(opn.temperature), T_ref) < 1e-9
+
+
+@pytest.mark.parametrize("p", [1, 2, 3, 4, 5, 6])
+def test_latent_heat_operator_vs_oracle(p):
+    import torch
+    layouts = [((4, 3, 3), "liquid_solid", 0, 0, 0, (-0.02, 0.0, 0.01), 0b000001, "qlq"),
+               ((3, 4, 3), "gas_liquid_solid", 1, 0, 0, None, 0b100000, "poly4_bell"),
+               ((3, 3, 4), "gas_liquid_solid", 0, 0, 0, (0.01, 0.01, 0.0), 0, "constant")]
+    for i, (n, mat, sharp, vs, sls, vel, faces, kind) in enumerate(layouts):
+        print("latent", p, n, mat, kind, flush=True)
+        T, To, H = _fields(None, n, p, 70 + i)
+        case = _case(p, (n, mat, sharp, vs, sls, vel, faces), H)
+        case["material"] = dict(case["material"], latent_heat=2.7e5, apparent_capacity=kind)
+        pb = oracle_problem(case)
+        pb.set_temperature(T)
+        pb.set_temperature_old(To)
+        op = engine_operator(case)
+        Tg, Tog = _t(T), _t(To)
+        op.set_temperature(Tg)
+        op.set_temperature_old(Tog)
+        src = np.random.default_rng(17 + i).uniform(-1.0, 1.0, pb.N)
+        dst = torch.empty_like(Tg)
+        op.vmult(dst, _t(src))
+        assert rel_err(_np(dst), pb.vmult(src)) < 1e-12, (p, kind)
+        inv = torch.empty_like(Tg)
+        op.compute_inverse_diagonal_from_matrixfree(inv)
+        # the apparent capacity's derivative term dt^-1 d(rho cp)/dT (T - T_old)
+        # nearly cancels the mass term on some rows, so their small diagonal
+        # entries carry the rounding of the large terms: compare the diagonal
+        # itself (free rows), scaled by its largest entry
+        free = ~pb.mask()
+        d_ref = pb.diagonal()[free]
+        assert rel_err(1.0 / _np(inv)[free], d_ref) < 1e-12, (p, kind)
+        rhs = _t(np.zeros(pb.N))
+        op.create_rhs(rhs, Tog, Tg)
+        assert rel_err(_np(rhs), pb.create_rhs(T, To)) < 1e-11, (p, kind)

@@ -19,7 +19,15 @@ Independent checks:
   constant velocity and a heaviside, with partial Dirichlet faces;
 * the tangent is the derivative of minus the residual (central differences),
   which fixes every derivative term's sign, factor and argument;
-* Newton converges quadratically with this tangent.
+* Newton converges quadratically with this tangent;
+* the apparent capacity of a latent heat of fusion (Heat::ApparentCapacity,
+  source/heat/apparent_capacity.cpp; poly4_bell, constant, qlq) against the
+  properties the reference's own unit test asserts
+  (unit_tests/heat/apparent_capacity.cpp: L = 338e3, solidus 271, liquidus
+  275): it vanishes with its derivative at the solidus and liquidus, its
+  midpoint-rule integral over the mushy zone (1000 steps) is L to 1e-8, and
+  its temperature derivative matches central differences; the C++ oracle and
+  the independent numpy statement in oracle/brute.py agree.
 """
 import json
 import os
@@ -184,3 +192,78 @@ def test_newton_converges_quadratically_through_the_mushy_zone():
     assert rep["converged"] and rep["iterations"] <= 5
     # the converged state is a root of the residual
     assert np.abs(pb.create_rhs(Tn, T_old)).max() < 1e-9
+
+
+AC_MATERIAL = {"type": "liquid_solid", "liq_rho": 1.0, "liq_cp": 0.0, "liq_k": 1.0,
+               "sol_rho": 1.0, "sol_cp": 0.0, "sol_k": 1.0, "solidus": 271.0,
+               "liquidus": 275.0, "latent_heat": 338e3}
+
+
+@pytest.mark.parametrize("kind", ["poly4_bell", "constant", "qlq"])
+def test_apparent_capacity_follows_the_reference_unit_test(kind):
+    m = dict(AC_MATERIAL, apparent_capacity=kind)
+    # with c_p = 0 in both phases the material's c_p is the apparent capacity
+    ac = lambda T: cpu.material_state(0.0, T, m)["cp"]
+    dac = lambda T: cpu.material_state(0.0, T, m)["d_cp"]
+    Ts, Tl, L = m["solidus"], m["liquidus"], m["latent_heat"]
+    if kind != "constant":
+        for T in (Ts, Tl):
+            assert abs(ac(T)) <= 1e-6 and abs(dac(T)) <= 1e-6
+    n = 1000
+    h = (Tl - Ts) / n
+    integral = sum(ac(Ts + (i + 0.5) * h) for i in range(n)) * h
+    assert abs(integral - L) <= 1e-8 * L, integral
+    eps = 1e-5
+    T = Ts + 0.1
+    for _ in range(3):
+        fd = (ac(T + eps) - ac(T - eps)) / (2.0 * eps)
+        assert abs(dac(T) - fd) <= 1e-5 * max(1.0, abs(fd)), (kind, T)
+        T += (Tl - Ts) / 5.0
+    # outside the mushy zone nothing is added
+    assert ac(Ts - 0.5) == 0.0 and ac(Tl + 0.5) == 0.0
+    # the independent numpy statement
+    grid = np.linspace(270.0, 276.0, 61)
+    c, dc = brute.apparent_capacity_t(grid, m)
+    ref = np.array([ac(t) for t in grid])
+    dref = np.array([dac(t) for t in grid])
+    assert np.abs(c - ref).max() <= 1e-12 * max(np.abs(ref).max(), 1.0)
+    assert np.abs(dc - dref).max() <= 1e-12 * max(np.abs(dref).max(), 1.0)
+
+
+LATENT_CASES = [
+    ((3, 2, 1), 1, "liquid_solid", 0, (-0.3, 0.2, 0.0), 0, "qlq"),
+    ((2, 2, 2), 2, "gas_liquid_solid", 1, None, 0b000011, "poly4_bell"),
+    ((2, 1, 2), 3, "gas_liquid_solid", 0, (0.1, 0.0, -0.2), 0, "constant"),
+]
+
+
+@pytest.mark.parametrize("n,p,mat,sharp,vel,faces,kind", LATENT_CASES)
+def test_latent_heat_tangent_and_residual(n, p, mat, sharp, vel, faces, kind):
+    T, T_old, H = _fields(n, p, 23 + p)
+    pb, m = _problem(n, p, mat, sharp, 0, vel, faces, T, T_old, H)
+    m.update(latent_heat=2.7e5, apparent_capacity=kind)
+    pb = cpu.Problem(n, p, dt=0.05, material=m, heaviside=H if mat == "gas_liquid_solid" else None,
+                     lo=(0.0, 0.0, 0.0), hi=(0.1, 0.1, 0.1), dirichlet_faces=faces,
+                     velocity="constant" if vel else None, velocity_value=vel,
+                     temperature=T, temperature_old=T_old)
+    mask = constrained_mask(n, p, faces)
+    J, r = brute.solidification_parts(n, p, 0.05, m, T, T_old,
+                                      heaviside=H if mat == "gas_liquid_solid" else None,
+                                      velocity="constant" if vel else None,
+                                      velocity_value=vel or (0.0, 0.0, 0.0), hi=(0.1, 0.1, 0.1))
+    src = np.random.default_rng(4).uniform(-1.0, 1.0, pb.N)
+    ref = brute.constrained(J, mask) @ src
+    assert np.abs(pb.vmult(src) - ref).max() <= 1e-12 * np.abs(ref).max()
+    d_ref = np.where(mask, 0.0, np.diag(J))
+    assert np.abs(pb.diagonal() - d_ref).max() <= 1e-12 * np.abs(d_ref).max()
+    r_ref = np.where(mask, 0.0, r)
+    assert np.abs(pb.create_rhs(T, T_old) - r_ref).max() <= 1e-11 * np.abs(r_ref).max()
+    # derivative of minus the residual, inside the mushy zone away from the
+    # qlq ramp's joints (central differences with a small step)
+    Tm = np.clip(T, 1967.2, 1972.8)
+    pb.set_temperature(Tm)
+    delta = np.where(pb.mask(), 0.0, np.random.default_rng(8).uniform(-1.0, 1.0, pb.N))
+    e = 1e-6
+    fd = -(pb.create_rhs(Tm + e * delta, T_old) - pb.create_rhs(Tm - e * delta, T_old)) / (2 * e)
+    Jd = pb.vmult(delta)
+    assert np.abs(fd - Jd).max() <= 1e-4 * np.abs(Jd).max()
