@@ -155,6 +155,23 @@ mpdg_status mpdg_heat_destroy(mpdg_heat *h);
 int64_t mpdg_heat_n_dofs(const mpdg_heat *h);
 int64_t mpdg_heat_n_free_dofs(const mpdg_heat *h);
 
+/* Numbering. The engine's vectors use the lexicographic lattice of the box's
+ * support points, I = ix + N1x (iy + N1y iz). A caller whose DoFs are numbered
+ * otherwise (deal.II's DoFHandler, e.g. from DoFTools::map_dofs_to_support_points)
+ * maps them once: mpdg_lattice_index_of_points gives, for each of the n_dofs()
+ * support points (x, y, z triples, host memory), its lattice index (host
+ * output); the points must be the box's support points, each once, or
+ * MPDG_ERR_ARGUMENT is returned. Host computation, no GPU needed. Then
+ * mpdg_heat_to_lattice / mpdg_heat_from_lattice move device vectors between
+ * the caller's numbering and the lattice with that index (device memory,
+ * n_dofs() entries, on the handle's stream). */
+mpdg_status mpdg_lattice_index_of_points(const mpdg_heat_desc *desc, const double *points,
+                                         int64_t count, uint32_t *lattice_index);
+mpdg_status mpdg_heat_to_lattice(mpdg_heat *h, double *lattice, const double *dofs,
+                                 const uint32_t *lattice_index);
+mpdg_status mpdg_heat_from_lattice(mpdg_heat *h, double *dofs, const double *lattice,
+                                   const uint32_t *lattice_index);
+
 /* OperatorBase::reset_time_increment (operator_base.hpp:41-46); dt = 0 is rejected. */
 mpdg_status mpdg_heat_set_dt(mpdg_heat *h, double dt);
 /* level_set_as_heaviside registration (heat_diffuse_operator.hpp:137-143): device

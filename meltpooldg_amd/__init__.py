@@ -6,8 +6,10 @@ heat operator / Jacobi / LinearSolver interface on top of it.
 """
 from .heat import (ConvectionBC, HeatBatch, HeatDiffuseOperation, HeatDiffuseOperator, JacobiPreconditioner,
                    LinearSolver, LinearSolverData, MaterialData, NewtonDidNotConverge, NoConvergence,
-                   NonlinearSolverData, PhaseData, RadiationBC, ZeroTimeIncrement)
+                   NonlinearSolverData, PhaseData, RadiationBC, ZeroTimeIncrement,
+                   lattice_index_of_points)
 
 __all__ = ["ConvectionBC", "HeatBatch", "HeatDiffuseOperation", "HeatDiffuseOperator", "JacobiPreconditioner",
            "LinearSolver", "LinearSolverData", "MaterialData", "NewtonDidNotConverge", "NoConvergence",
-           "NonlinearSolverData", "PhaseData", "RadiationBC", "ZeroTimeIncrement"]
+           "NonlinearSolverData", "PhaseData", "RadiationBC", "ZeroTimeIncrement",
+           "lattice_index_of_points"]

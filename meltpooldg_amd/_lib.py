@@ -139,6 +139,9 @@ SIGNATURES = {
     "mpdg_heat_set_temperature": (ctypes.c_int, [_VP, _VP]),
     "mpdg_heat_set_temperature_old": (ctypes.c_int, [_VP, _VP]),
     "mpdg_heat_set_graphs": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "mpdg_lattice_index_of_points": (ctypes.c_int, [_VP, _VP, ctypes.c_int64, _VP]),
+    "mpdg_heat_to_lattice": (ctypes.c_int, [_VP, _VP, _VP, _VP]),
+    "mpdg_heat_from_lattice": (ctypes.c_int, [_VP, _VP, _VP, _VP]),
     "mpdg_heat_vmult": (ctypes.c_int, [_VP, _VP, _VP]),
     "mpdg_heat_vmult_f32": (ctypes.c_int, [_VP, _VP, _VP]),
     "mpdg_heat_inverse_diagonal": (ctypes.c_int, [_VP, _VP]),

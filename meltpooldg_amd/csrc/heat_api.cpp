@@ -1105,6 +1105,87 @@ mpdg_status mpdg_heat_set_temperature(mpdg_heat *h, const double *temperature)
   return MPDG_OK;
 }
 
+mpdg_status mpdg_lattice_index_of_points(const mpdg_heat_desc *desc, const double *points,
+                                         int64_t count, uint32_t *lattice_index)
+{
+  if (!desc || !points || !lattice_index)
+    return fail(MPDG_ERR_ARGUMENT, "null argument");
+  const mpdg_heat_desc &d = *desc;
+  if (d.degree < 1 || d.degree > kMaxDegree)
+    return fail(MPDG_ERR_UNSUPPORTED, "degree must be in 1.." + std::to_string(kMaxDegree));
+  const Tables1D t = make_tables(d.degree);
+  // support-point coordinates of each lattice line, ascending
+  std::vector<double> coords[3];
+  double tol[3];
+  int64_t N = 1;
+  for (int k = 0; k < 3; ++k)
+    {
+      if (d.n_cells[k] < 1 || !(d.box_hi[k] > d.box_lo[k]))
+        return fail(MPDG_ERR_ARGUMENT, "invalid box");
+      const double h = (d.box_hi[k] - d.box_lo[k]) / d.n_cells[k];
+      for (int c = 0; c < d.n_cells[k]; ++c)
+        for (int a = 0; a < d.degree; ++a)
+          coords[k].push_back(d.box_lo[k] + (c + t.gll[a]) * h);
+      coords[k].push_back(d.box_hi[k]);
+      // a point matches the support point within a quarter of the smallest gap
+      double gap = h;
+      for (size_t i = 1; i < coords[k].size(); ++i)
+        gap = std::min(gap, coords[k][i] - coords[k][i - 1]);
+      tol[k] = 0.25 * gap;
+      N *= (int64_t)coords[k].size();
+    }
+  if (count != N)
+    return fail(MPDG_ERR_ARGUMENT, "count must be the number of support points n_dofs()");
+  if (N > ((int64_t)1 << 32))
+    return fail(MPDG_ERR_ARGUMENT, "lattice too large for 32-bit indices");
+  std::vector<unsigned char> seen((size_t)N, 0);
+  for (int64_t i = 0; i < count; ++i)
+    {
+      int64_t idx[3];
+      for (int k = 0; k < 3; ++k)
+        {
+          const double x = points[3 * i + k];
+          const std::vector<double> &c = coords[k];
+          const auto it = std::lower_bound(c.begin(), c.end(), x);
+          int64_t j = it - c.begin();
+          if (j == (int64_t)c.size() || (j > 0 && x - c[j - 1] < c[j] - x))
+            --j;
+          if (!(std::fabs(c[j] - x) <= tol[k]))
+            return fail(MPDG_ERR_ARGUMENT, "point " + std::to_string(i) +
+                                             " is not a support point of the box");
+          idx[k] = j;
+        }
+      const int64_t I = idx[0] + (int64_t)coords[0].size() * (idx[1] + (int64_t)coords[1].size() * idx[2]);
+      if (seen[I])
+        return fail(MPDG_ERR_ARGUMENT, "support point " + std::to_string(I) + " given twice");
+      seen[I] = 1;
+      lattice_index[i] = (uint32_t)I;
+    }
+  return MPDG_OK;
+}
+
+mpdg_status mpdg_heat_to_lattice(mpdg_heat *h, double *lattice, const double *dofs,
+                                 const uint32_t *lattice_index)
+{
+  if (!h || !lattice || !dofs || !lattice_index)
+    return fail(MPDG_ERR_ARGUMENT, "null argument");
+  if ((const void *)lattice == (const void *)dofs)
+    return fail(MPDG_ERR_ARGUMENT, "the permutation cannot run in place");
+  MPDG_HIP(launch_permute(lattice, dofs, lattice_index, h->N, true, h->stream));
+  return MPDG_OK;
+}
+
+mpdg_status mpdg_heat_from_lattice(mpdg_heat *h, double *dofs, const double *lattice,
+                                   const uint32_t *lattice_index)
+{
+  if (!h || !lattice || !dofs || !lattice_index)
+    return fail(MPDG_ERR_ARGUMENT, "null argument");
+  if ((const void *)lattice == (const void *)dofs)
+    return fail(MPDG_ERR_ARGUMENT, "the permutation cannot run in place");
+  MPDG_HIP(launch_permute(dofs, lattice, lattice_index, h->N, false, h->stream));
+  return MPDG_OK;
+}
+
 mpdg_status mpdg_heat_set_graphs(mpdg_heat *h, int enable)
 {
   if (!h)

@@ -306,6 +306,9 @@ namespace mpdg
   hipError_t launch_to_f32(float *dst, const double *src, int64_t n, hipStream_t s);
   hipError_t launch_add_f32(double *x, const float *d, int64_t n, hipStream_t s);
   hipError_t launch_fill(double *x, double v, int64_t n, hipStream_t s);
+  // lattice[index[i]] = dofs[i] (to_lattice) or dofs[i] = lattice[index[i]]
+  hipError_t launch_permute(double *dst, const double *src, const uint32_t *index, int64_t n,
+                            bool to_lattice, hipStream_t s);
   hipError_t launch_fill_f32(float *x, float v, int64_t n, hipStream_t s);
   hipError_t launch_dot(const double *a, const double *b, int64_t n, double *partials,
                         double *out, unsigned *counter, hipStream_t s);

@@ -967,6 +967,32 @@ namespace mpdg
     return hipGetLastError();
   }
 
+  namespace
+  {
+    __global__ void __launch_bounds__(T)
+      permute_kernel(double *dst, const double *src, const uint32_t *index, int64_t n,
+                     bool to_lattice)
+    {
+      for (int64_t i = blockIdx.x * (int64_t)T + threadIdx.x; i < n; i += (int64_t)gridDim.x * T)
+        {
+          const uint32_t j = index[i];
+          if ((int64_t)j >= n) // not a lattice index: left out rather than read or written
+            continue;
+          if (to_lattice)
+            dst[j] = src[i];
+          else
+            dst[i] = src[j];
+        }
+    }
+  } // namespace
+
+  hipError_t launch_permute(double *dst, const double *src, const uint32_t *index, int64_t n,
+                            bool to_lattice, hipStream_t s)
+  {
+    permute_kernel<<<grid_for(n), T, 0, s>>>(dst, src, index, n, to_lattice);
+    return hipGetLastError();
+  }
+
   hipError_t launch_fill_f32(float *x, float v, int64_t n, hipStream_t s)
   {
     fill_f32_kernel<<<grid_for(n), T, 0, s>>>(x, v, n);

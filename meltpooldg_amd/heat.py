@@ -170,6 +170,21 @@ def _make_desc(n_cells, degree, material, box, precision, conductivity_amplitude
     return d
 
 
+def lattice_index_of_points(n_cells, degree, points, box=((0.0, 0.0, 0.0), (1.0, 1.0, 1.0))):
+    """Lattice index of each support point (numpy array of shape (n_dofs, 3),
+    e.g. DoFTools::map_dofs_to_support_points of the reference's DoFHandler) of
+    FE_Q(degree) on the box mesh; numpy uint32 array. Raises on a point that is
+    not a support point or a support point given twice. Host computation."""
+    import numpy as np
+    d = _make_desc(n_cells, degree, None, box, "fp64", 0.0, None, 0.2, 0, None)
+    pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros(pts.shape[0], dtype=np.uint32)
+    lib = L.load()
+    L.check(lib.mpdg_lattice_index_of_points(ctypes.byref(d), ctypes.c_void_p(pts.ctypes.data),
+                                             pts.shape[0], ctypes.c_void_p(out.ctypes.data)))
+    return out
+
+
 class HeatDiffuseOperator:
     """Matrix-free heat tangent J = rho_cp/dt M + K_k + rho_cp C_u on a box mesh.
 
@@ -250,6 +265,28 @@ class HeatDiffuseOperator:
         self._check_vec(temperature_old, "float64")
         L.check(self._lib.mpdg_heat_set_temperature_old(self._h, _ptr(temperature_old)))
         self._temperature_old = temperature_old
+
+    def to_lattice(self, lattice, dofs, lattice_index):
+        """lattice[lattice_index[i]] = dofs[i] (device tensors; lattice_index an int32
+        / uint32 device tensor from lattice_index_of_points)."""
+        self._check_vec(lattice, "float64")
+        self._check_vec(dofs, "float64")
+        self._check_index(lattice_index)
+        L.check(self._lib.mpdg_heat_to_lattice(self._h, _ptr(lattice), _ptr(dofs), _ptr(lattice_index)))
+
+    def from_lattice(self, dofs, lattice, lattice_index):
+        """dofs[i] = lattice[lattice_index[i]]."""
+        self._check_vec(lattice, "float64")
+        self._check_vec(dofs, "float64")
+        self._check_index(lattice_index)
+        L.check(self._lib.mpdg_heat_from_lattice(self._h, _ptr(dofs), _ptr(lattice), _ptr(lattice_index)))
+
+    def _check_index(self, index):
+        import torch
+        if not (index.is_cuda and index.dtype in (torch.int32, torch.uint32)
+                and index.is_contiguous() and index.numel() == self.n_dofs()):
+            raise ValueError("lattice_index must be a contiguous int32/uint32 device tensor "
+                             "of n_dofs() entries")
 
     def set_graphs(self, enable=True):
         """Replay the Krylov iteration chunks of this operator's solves as

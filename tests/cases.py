@@ -96,3 +96,16 @@ def rel_err(a, b):
     a = np.asarray(a)
     b = np.asarray(b)
     return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def support_points(n, p, lo, hi):
+    """All support points of FE_Q(p) on the box, lattice order (x fastest)."""
+    from numpy.polynomial import legendre as Lg
+    inner = np.sort(Lg.Legendre.basis(p).deriv().roots().real) if p > 1 else np.array([])
+    gll = np.concatenate([[0.0], 0.5 * (inner + 1.0)])
+    axes = []
+    for d in range(3):
+        h = (hi[d] - lo[d]) / n[d]
+        axes.append(np.concatenate([lo[d] + (c + gll) * h for c in range(n[d])] + [[hi[d]]]))
+    Z, Y, X = np.meshgrid(axes[2], axes[1], axes[0], indexing="ij")
+    return np.stack([X.reshape(-1), Y.reshape(-1), Z.reshape(-1)], axis=1)

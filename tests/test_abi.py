@@ -5,6 +5,7 @@ import ctypes
 import os
 import re
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -116,3 +117,27 @@ def test_cpp_adapter_compiles_and_links(lib, tmp_path):
     assert r.returncode == 0, r.stderr
     r = subprocess.run([str(exe)], capture_output=True, text=True)
     assert r.returncode == 0 and "adapter ok" in r.stdout, (r.returncode, r.stdout, r.stderr)
+
+
+def test_lattice_index_of_points_inverts_any_numbering():
+    """mpdg_lattice_index_of_points (host, no GPU): a shuffled numbering of the
+    box's support points (as a DoFHandler would give) maps back to the lattice;
+    a point off the lattice or a repeated one is an error."""
+    from meltpooldg_amd import heat
+    from tests.cases import support_points
+    n, p, lo, hi = (3, 2, 4), 3, (0.0, -1.0, 0.5), (1.5, 1.0, 2.0)
+    pts = support_points(n, p, lo, hi)
+    order = np.random.default_rng(1).permutation(len(pts))
+    shuffled = pts[order] + np.random.default_rng(2).uniform(-1e-12, 1e-12, pts.shape)
+    idx = heat.lattice_index_of_points(n, p, shuffled, box=(lo, hi))
+    assert idx.dtype == np.uint32 and np.array_equal(idx, order.astype(np.uint32))
+    bad = shuffled.copy()
+    bad[5, 2] += 0.03  # z nodes are >= 0.10 apart here; a match allows a quarter of that
+    with pytest.raises(RuntimeError, match="not a support point"):
+        heat.lattice_index_of_points(n, p, bad, box=(lo, hi))
+    dup = shuffled.copy()
+    dup[7] = dup[8]
+    with pytest.raises(RuntimeError, match="given twice"):
+        heat.lattice_index_of_points(n, p, dup, box=(lo, hi))
+    with pytest.raises(RuntimeError, match="count"):
+        heat.lattice_index_of_points(n, p, shuffled[:-1], box=(lo, hi))
