@@ -160,7 +160,7 @@ def main():
                     help="autotune the operator (FP64 and FP32) before timing instead of using the "
                          "per-degree default tuning (itself the autotuner's pick on this workload)")
     ap.add_argument("--tuning", default=None,
-                    help="fixed FP64 tuning z_layers,blocks_per_cu[,variant,tile_log2]")
+                    help="fixed FP64 tuning z_layers,blocks_per_cu[,variant,tile_log2,schedule]")
     ap.add_argument("--mode", choices=["vmult", "copy", "solve", "robin", "solidification",
                                        "timestep"],
                     default="vmult",
@@ -205,7 +205,8 @@ def main():
         f = args.tuning.split(",")
         op.set_tuning(z_layers=int(f[0]), blocks_per_cu=int(f[1]),
                       variant=f[2] if len(f) > 2 else None,
-                      tile_log2=int(f[3]) if len(f) > 3 else None, precision="fp64")
+                      tile_log2=int(f[3]) if len(f) > 3 else None,
+                      schedule=f[4] if len(f) > 4 else None, precision="fp64")
         tuning = op.get_tuning("fp64")
     elif args.tune:
         tuning = op.autotune("fp64", repeats=3)

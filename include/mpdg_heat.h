@@ -342,12 +342,24 @@ typedef enum {
                               (MPDG_FP32 tuning only, degree >= 3) */
 } mpdg_vmult_variant;
 
+/* How the line forms order their colour passes. Either way every lattice node
+ * receives its cell contributions in the same colour order, so both give
+ * bitwise the same dst. */
+typedef enum {
+  MPDG_SCHEDULE_PASSES = 0,  /* one launch per colour pass (8 launches per vmult) */
+  MPDG_SCHEDULE_ORDERED = 1  /* one launch per vmult: work items are taken in colour
+                                order and each waits only for the neighbouring items
+                                of earlier colours (line forms; the cell form always
+                                runs colour passes) */
+} mpdg_vmult_schedule;
+
 typedef struct {
   int z_layers;       /* cell layers swept per vmult work item (>= 1) */
   int blocks_per_cu;  /* persistent workgroups per CU, 0 = occupancy limit (<= 16) */
   int variant;        /* mpdg_vmult_variant (see the limits above) */
   int tile_log2;      /* cell variant: log2 of the tile's cells along x (3..6); a
                          wavefront's tile is 2^tile_log2 x 2^(6 - tile_log2) cells */
+  int schedule;       /* mpdg_vmult_schedule */
 } mpdg_tuning;
 mpdg_status mpdg_heat_get_tuning(const mpdg_heat *h, int precision, mpdg_tuning *t);
 mpdg_status mpdg_heat_set_tuning(mpdg_heat *h, int precision, const mpdg_tuning *t);

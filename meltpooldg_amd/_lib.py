@@ -124,7 +124,8 @@ MPDG_VMULT_LINE_MFMA = 2
 
 class Tuning(ctypes.Structure):
     _fields_ = [("z_layers", ctypes.c_int), ("blocks_per_cu", ctypes.c_int),
-                ("variant", ctypes.c_int), ("tile_log2", ctypes.c_int)]
+                ("variant", ctypes.c_int), ("tile_log2", ctypes.c_int),
+                ("schedule", ctypes.c_int)]
 
 
 # name -> (restype, argtypes); exactly the entry points include/mpdg_heat.h declares

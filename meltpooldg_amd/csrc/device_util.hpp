@@ -122,6 +122,25 @@ namespace mpdg
       __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, 0);
     }
 
+    // Agent-coherent forms (cache policy sc1): a store that reaches the point
+    // of coherence of all XCDs before it completes, a load served from there.
+    __device__ __forceinline__ double buf_load_coherent(__amdgpu_buffer_rsrc_t r, uint32_t off, double)
+    {
+      return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 16));
+    }
+    __device__ __forceinline__ float buf_load_coherent(__amdgpu_buffer_rsrc_t r, uint32_t off, float)
+    {
+      return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 16));
+    }
+    __device__ __forceinline__ void buf_store_coherent(__amdgpu_buffer_rsrc_t r, uint32_t off, double v)
+    {
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, off, 0, 16);
+    }
+    __device__ __forceinline__ void buf_store_coherent(__amdgpu_buffer_rsrc_t r, uint32_t off, float v)
+    {
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, 16);
+    }
+
     // dst[off] += v as one device-scope atomic (no return value).
     __device__ __forceinline__ void buf_add(__amdgpu_buffer_rsrc_t r, uint32_t off, double v)
     {

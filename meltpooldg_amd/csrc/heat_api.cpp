@@ -80,6 +80,9 @@ struct mpdg_heat
 
   CellParams<double> prm64{};
   CellParams<float> prm32{};
+  // ordered vmult schedule: device state and per-item flags per precision
+  DevBuf<OrderState> order[2];
+  DevBuf<unsigned> order_flags[2];
   VmultTables<double> vt64{};
   VmultTables<float> vt32{};
   DiagTables<double> dt64{};
@@ -144,6 +147,11 @@ struct mpdg_heat
     for (auto *b : {&nrhs, &ndu, &npart, &nres})
       b->release();
     nticket.release();
+    for (int k = 0; k < 2; ++k)
+      {
+        order[k].release();
+        order_flags[k].release();
+      }
     if (host_nres)
       (void)hipHostFree(host_nres);
     scal.release();
@@ -213,6 +221,7 @@ namespace
   bool tuning_valid(const mpdg_tuning &t, int degree, int precision)
   {
     return t.z_layers >= 1 && t.blocks_per_cu >= 0 && t.blocks_per_cu <= kMaxResidentPerCu &&
+           (t.schedule == MPDG_SCHEDULE_PASSES || t.schedule == MPDG_SCHEDULE_ORDERED) &&
            (t.variant == MPDG_VMULT_LINE ||
             (t.variant == MPDG_VMULT_CELL && degree <= kCellMaxDegree && t.tile_log2 >= 3 &&
              t.tile_log2 <= 6) ||
@@ -373,6 +382,11 @@ namespace
                   (tu.variant == MPDG_VMULT_LINE_MFMA && d.degree >= kMfmaMinDegree &&
                    sizeof(Number) == 4) ? kVariantLineMfma : kVariantLine;
     prm.tile_log2 = tu.tile_log2;
+    const int pi = sizeof(Number) == 4 ? MPDG_FP32 : MPDG_FP64;
+    const bool ordered = tu.schedule == MPDG_SCHEDULE_ORDERED && prm.variant != kVariantCell &&
+                         H->order[pi].ptr != nullptr;
+    prm.order = ordered ? H->order[pi].ptr : nullptr;
+    prm.order_flags = ordered ? H->order_flags[pi].ptr : nullptr;
     // per-direction tables (see coefficient_tables): [sin2pi dir 0 | dir 1 | dir 2 |
     // u_x x-factor | u_x y-factor | u_y x-factor | u_y y-factor]
     size_t off = 0;
@@ -442,7 +456,9 @@ namespace
     const ColourGrid cg32 = colour_grid(H->desc.n_cells[0], H->desc.n_cells[1],
                                         H->desc.n_cells[2], cells_per_block(H->desc.degree, true));
     const size_t vmult_slots = 8 * (size_t)kMaxResidentPerCu * device_cu_count();
-    const size_t np = std::max<size_t>({(size_t)cg.total, (size_t)cg32.total, vmult_slots,
+    // the ordered vmult schedule keeps one p.Ap share per work item (at most one per cell)
+    const size_t cells = (size_t)H->desc.n_cells[0] * H->desc.n_cells[1] * H->desc.n_cells[2];
+    const size_t np = std::max<size_t>({(size_t)cg.total, (size_t)cg32.total, vmult_slots, cells,
                                         (size_t)kMaxBasis * kVecBlocks,
                                         (size_t)robin_blocks(H->robin),
                                         (size_t)tmat_blocks(H->desc.degree, H->desc.n_cells[0],
@@ -1033,6 +1049,18 @@ mpdg_status mpdg_heat_create(const mpdg_heat_desc *desc, mpdg_heat **out)
     e = hipEventCreate(&H->ev0);
   if (e == hipSuccess)
     e = hipEventCreate(&H->ev1);
+  // ordered vmult schedule: counters and epoch at 0, one flag per possible work
+  // item (an item holds at least one cell)
+  for (int k = 0; k < 2 && e == hipSuccess; ++k)
+    {
+      e = H->order[k].alloc(1);
+      if (e == hipSuccess)
+        e = H->order_flags[k].alloc((size_t)cells);
+      if (e == hipSuccess)
+        e = hipMemset(H->order[k].ptr, 0, sizeof(OrderState));
+      if (e == hipSuccess)
+        e = hipMemset(H->order_flags[k].ptr, 0, (size_t)cells * sizeof(unsigned));
+    }
   if (e != hipSuccess)
     {
       delete H;
@@ -1662,13 +1690,15 @@ mpdg_status mpdg_heat_autotune(mpdg_heat *h, int precision, int repeats, mpdg_tu
         continue;
       for (int bpc : {0, 2, 4, 8})
         {
-          mpdg_tuning t{zl, bpc, MPDG_VMULT_LINE, 4};
-          cand.push_back(t);
-          if (fp32 && h->desc.degree >= kMfmaMinDegree)
-            cand.push_back(mpdg_tuning{zl, bpc, MPDG_VMULT_LINE_MFMA, 4});
+          for (int sch : {MPDG_SCHEDULE_PASSES, MPDG_SCHEDULE_ORDERED})
+            {
+              cand.push_back(mpdg_tuning{zl, bpc, MPDG_VMULT_LINE, 4, sch});
+              if (fp32 && h->desc.degree >= kMfmaMinDegree)
+                cand.push_back(mpdg_tuning{zl, bpc, MPDG_VMULT_LINE_MFMA, 4, sch});
+            }
           if (h->desc.degree <= kCellMaxDegree)
             for (int tl : {3, 4, 5, 6})
-              cand.push_back(mpdg_tuning{zl, bpc, MPDG_VMULT_CELL, tl});
+              cand.push_back(mpdg_tuning{zl, bpc, MPDG_VMULT_CELL, tl, MPDG_SCHEDULE_PASSES});
         }
     }
   const mpdg_tuning saved = h->tune[precision];

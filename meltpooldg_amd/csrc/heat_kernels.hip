@@ -178,6 +178,127 @@ namespace mpdg
       return it;
     }
 
+    // Ordered schedule: ticket t of a launch is item t - off[c] of colour c,
+    // off[c] <= t < off[c + 1] (colours in pass order).
+    struct OrderGrid
+    {
+      int off[9];
+    };
+
+    // og.off[c] for a run-time c, as a select chain over the (scalar) kernel
+    // arguments: a dynamic index would copy the array to private memory
+    __device__ __forceinline__ int order_off(const OrderGrid &og, int c)
+    {
+      int r = 0;
+#pragma unroll
+      for (int k = 0; k < 9; ++k)
+        r = k == c ? og.off[k] : r;
+      return r;
+    }
+
+    template <typename Number, int CPB>
+    __device__ __forceinline__ Item decode_ticket(const CellParams<Number> &prm,
+                                                  const OrderGrid &og, int layers, int t)
+    {
+      int c = 0;
+#pragma unroll
+      for (int k = 1; k < 8; ++k)
+        c += t >= og.off[k] ? 1 : 0;
+      return decode_item<Number, CPB>(prm, c, layers, t - order_off(og, c));
+    }
+
+    // Ordered schedule, cross-workgroup visibility of dst: the read-modify-write
+    // loads and the stores of dst use the agent-coherent cache policy, and a
+    // workgroup publishes an item only after its stores have completed
+    // (vmcnt(0)), so neighbours on other XCDs read the sums without whole-cache
+    // write-back / invalidate fences (which cost more than the whole vmult).
+    constexpr bool kOrderCoherentDst = true;
+
+    // vmcnt(0): this wavefront's vector memory operations have completed
+    __device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+    template <bool C, typename Number>
+    __device__ __forceinline__ Number dst_load(__amdgpu_buffer_rsrc_t r, uint32_t off)
+    {
+      if constexpr (C)
+        return buf_load_coherent(r, off, Number());
+      else
+        return buf_load(r, off, Number());
+    }
+    template <bool C, typename Number>
+    __device__ __forceinline__ void dst_store(__amdgpu_buffer_rsrc_t r, uint32_t off, Number v)
+    {
+      if constexpr (C)
+        buf_store_coherent(r, off, v);
+      else
+        buf_store(r, off, v);
+    }
+
+    // Next ticket of this workgroup. Q = min(gridDim, kOrderQueues) queues, one
+    // per XCD (workgroups are dispatched to XCDs round-robin): workgroup b takes
+    // the tickets q, q + Q, q + 2Q, ... of queue q = b mod Q in increasing order,
+    // each queue counter in its own cache line so the XCDs do not contend on one
+    // atomic. Workgroups 0..Q-1 are dispatched first and leave only after their
+    // queue is exhausted, so every queue is served while it has tickets.
+    __device__ __forceinline__ int order_grab(OrderState *os)
+    {
+      const int Q = min((int)gridDim.x, kOrderQueues);
+      const int q = (int)blockIdx.x % Q;
+      const unsigned k = atomicAdd(&os->ticket[q][0], 1u);
+      return (int)k * Q + q;
+    }
+
+    // Bound of one wait for a neighbouring item (2^20 polls of a few hundred
+    // cycles each: about a second, far above any item's run time, so only a
+    // defect can reach it; the wait then gives up and sets order->error).
+    constexpr unsigned kOrderSpinLimit = 1u << 20;
+
+    // Ordered schedule: before item `it` reads back (read-modify-write) any node
+    // it shares with its neighbours, every neighbouring item of an earlier
+    // colour must have stored. Items touch nodes of their 26 neighbours in the
+    // (segment, row, z-chunk) grid; a neighbour's colour is its parity, and its
+    // ticket inverts decode_item. Lanes 0..26 of wave 0 poll one neighbour
+    // each; the workgroup then synchronises (and, without coherent dst
+    // accesses, acquires at agent scope) so the neighbours' stores are visible.
+    // Every awaited ticket is lower than this one, and the lowest unfinished
+    // ticket never waits: its queue serves tickets in order, so it is held or
+    // next to be taken by a running workgroup. Hence the waits end.
+    template <typename Number, int CPB>
+    __device__ __forceinline__ void order_wait(const CellParams<Number> &prm, const OrderGrid &og,
+                                               const Item &it, int layers, unsigned epoch)
+    {
+      const int k = threadIdx.x;
+      if (k < 27 && k != 13)
+        {
+          const int nsx = (prm.nx + CPB - 1) / CPB;
+          const int nkz = (prm.nz + layers - 1) / layers;
+          const int x = it.sx + k % 3 - 1, y = it.cy + (k / 3) % 3 - 1, z = it.kz + k / 9 - 1;
+          const int c = (it.sx & 1) | ((it.cy & 1) << 1) | ((it.kz & 1) << 2);
+          const int cn = (x & 1) | ((y & 1) << 1) | ((z & 1) << 2);
+          if (x >= 0 && x < nsx && y >= 0 && y < prm.ny && z >= 0 && z < nkz && cn < c)
+            {
+              const int msx = (nsx - (cn & 1) + 1) / 2;
+              const int my = (prm.ny - ((cn >> 1) & 1) + 1) / 2;
+              const int t = order_off(og, cn) + (x >> 1) + msx * ((y >> 1) + my * (z >> 1));
+              unsigned spins = 0;
+              while (__hip_atomic_load(prm.order_flags + t, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT) != epoch)
+                {
+                  __builtin_amdgcn_s_sleep(2);
+                  if (++spins > kOrderSpinLimit)
+                    {
+                      __hip_atomic_fetch_or(&prm.order->error, 1u, __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_AGENT);
+                      break;
+                    }
+                }
+            }
+        }
+      __syncthreads();
+      if constexpr (!kOrderCoherentDst)
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+
     // Gather of one x-line (y = s0, z = s1 of cell layer cz, block cell cb) with
     // homogeneous constraints read as 0 (FEEvaluation::read_dof_values).
     template <int P, typename Number>
@@ -326,13 +447,23 @@ namespace mpdg
     // Body of the vmult: this workgroup processes items item_first,
     // item_first + item_stride, ... (< items) of one colour of one operator and
     // returns this thread's share of p . Ap (sum over its cells of u . y).
-    template <int P, typename Number, int COEF, bool VEL, int CW, int R, int WPB, bool MF>
+    //
+    // Ordered schedule (ORD): one launch covers all colours. `item` is then the
+    // launch's ticket (items of colour 0 first), taken from prm.order->ticket
+    // one item ahead; each item waits for its earlier-colour neighbours before
+    // its read-modify-write loads and, after its stores, publishes its flag
+    // (and, with ipart, its p . Ap share in ipart[ticket]).
+    template <int P, typename Number, int COEF, bool VEL, int CW, int R, int WPB, bool MF,
+              bool ORD = false>
     __device__ __forceinline__ double vmult_body(const Number *__restrict__ src,
                                                  Number *__restrict__ dst,
                                                  const Number *__restrict__ heav,
                                                  const CellParams<Number> &prm, const int colour,
                                                  const int layers, const int items,
-                                                 const int item_first, const int item_stride)
+                                                 const int item_first, const int item_stride,
+                                                 const OrderGrid og = OrderGrid{},
+                                                 double *__restrict__ ipart = nullptr,
+                                                 unsigned epoch = 0)
     {
       constexpr int N = P + 1;
       constexpr int NL = N * N;
@@ -341,6 +472,7 @@ namespace mpdg
       constexpr int CPB = CW * WPB;
       constexpr int THREADS = 64 * WPB;
       constexpr bool H2 = COEF == kCoefHeaviside;
+      constexpr bool CD = ORD && kOrderCoherentDst; // agent-coherent dst accesses
       static_assert(LT <= 64 * R && LT > 64 * (R - 1), "lines must fill the last lane slot");
       constexpr ct::Tab<N> T1 = ct::VTab<P>::t;
 
@@ -384,17 +516,34 @@ namespace mpdg
                        (cb[r] == 0 ? kCb0 : 0u);
         }
 
+      __shared__ int s_ticket[1];
+      __shared__ double s_red[ORD ? WPB : 1];
       int item = item_first;
+      if constexpr (ORD)
+        {
+          if (threadIdx.x == 0)
+            s_ticket[0] = order_grab(prm.order);
+          __syncthreads();
+          // uniform (scalar) from here on, like the passes' item index
+          item = __builtin_amdgcn_readfirstlane(s_ticket[0]);
+        }
       int j = 0;
-      Item it = decode_item<Number, CPB>(prm, colour, layers, item);
+      Item it{};
+      if constexpr (ORD)
+        {
+          if (item < items)
+            it = decode_ticket<Number, CPB>(prm, og, layers, item);
+        }
+      else
+        it = decode_item<Number, CPB>(prm, colour, layers, item);
       LayerU LU = layer_uniforms<Number, P>(prm, it, 0);
       Number u[R][N], hl[R][N];
 #pragma unroll
       for (int r = 0; r < R; ++r)
         {
-          gather_line<P>(rsrc, lc[r], LU, cb[r], it.ncell, true, u[r]);
+          gather_line<P>(rsrc, lc[r], LU, cb[r], it.ncell, item < items, u[r]);
           if constexpr (H2)
-            gather_line_plain<P>(rheav, lc[r], LU, cb[r], it.ncell, true, hl[r]);
+            gather_line_plain<P>(rheav, lc[r], LU, cb[r], it.ncell, item < items, hl[r]);
         }
       double acc = 0.0;
       int step = 0; // layers done by this workgroup; selects the x-exchange buffer
@@ -406,10 +555,22 @@ namespace mpdg
           Item it_n = it;
           if (j_n >= it.nl)
             {
-              item_n = item + item_stride;
               j_n = 0;
-              if (item_n < items)
-                it_n = decode_item<Number, CPB>(prm, colour, layers, item_n);
+              if constexpr (ORD)
+                {
+                  if (threadIdx.x == 0)
+                    s_ticket[0] = order_grab(prm.order);
+                  __syncthreads();
+                  item_n = __builtin_amdgcn_readfirstlane(s_ticket[0]);
+                  if (item_n < items)
+                    it_n = decode_ticket<Number, CPB>(prm, og, layers, item_n);
+                }
+              else
+                {
+                  item_n = item + item_stride;
+                  if (item_n < items)
+                    it_n = decode_item<Number, CPB>(prm, colour, layers, item_n);
+                }
             }
 
           // ---- x-lines (y = s0, z = s1): interpolate in x ------------------------
@@ -443,6 +604,10 @@ namespace mpdg
                 gather_line_plain<P>(rheav, lc[r], LUn, cb[r], it_n.ncell, item_n < items, hn[r]);
             }
 
+          if constexpr (ORD)
+            if (j == 0)
+              order_wait<Number, CPB>(prm, og, it, layers, epoch);
+
           // scatter plan of each output x-line (b = s0, c = s1). Interior nodes
           // (0 < a < p) share the line's mode; only the end nodes can be on an x
           // face or shared with the neighbouring segment. Offsets are out of
@@ -467,7 +632,7 @@ namespace mpdg
               const uint32_t li = (lw & !yz_first & !rowc) ? obase : kOutOfRange;
 #pragma unroll
               for (int a = 1; a < N - 1; ++a)
-                dv[r][a] = buf_load(rdst, li + a * SZ, Number());
+                dv[r][a] = dst_load<CD, Number>(rdst, li + a * SZ);
               // node a = 0
               const bool cb0 = (bits & kCb0) != 0u;
               const bool first0 = yz_first & !(cb0 & LU.x0nf);
@@ -475,14 +640,14 @@ namespace mpdg
               const bool wr0 = lw & (!con0 | first0);
               w0o[r] = wr0 ? obase : kOutOfRange;
               z0[r] = con0;
-              dv[r][0] = buf_load(rdst, (wr0 & !first0 & !con0) ? obase : kOutOfRange, Number());
+              dv[r][0] = dst_load<CD, Number>(rdst, (wr0 & !first0 & !con0) ? obase : kOutOfRange);
               // node a = p (written by the segment's last cell only)
               const bool firstP = yz_first & !(last_cell & LU.xPnf);
               const bool conP = rowc | (last_cell & LU.x1d);
               const bool wrP = lw & last_cell & (!conP | firstP);
               wPo[r] = wrP ? obase + P * SZ : kOutOfRange;
               zP[r] = conP;
-              dv[r][N - 1] = buf_load(rdst, (wrP & !firstP & !conP) ? obase + P * SZ : kOutOfRange, Number());
+              dv[r][N - 1] = dst_load<CD, Number>(rdst, (wrP & !firstP & !conP) ? obase + P * SZ : kOutOfRange);
             }
           wave_sync();
 
@@ -715,12 +880,36 @@ namespace mpdg
 #pragma unroll
                 for (int a = 0; a < N; ++a)
                   FZ[j & 1][(cb[r] * N + b) * N + a] = Y[r][a];
-              buf_store(rdst, w0o[r], z0[r] ? Number(0) : dv[r][0] + Y[r][0]);
+              dst_store<CD>(rdst, w0o[r], z0[r] ? Number(0) : dv[r][0] + Y[r][0]);
 #pragma unroll
               for (int a = 1; a < N - 1; ++a)
-                buf_store(rdst, wl[r] + a * (uint32_t)sizeof(Number), zl[r] ? Number(0) : dv[r][a] + Y[r][a]);
-              buf_store(rdst, wPo[r], zP[r] ? Number(0) : dv[r][N - 1] + Y[r][N - 1]);
+                dst_store<CD>(rdst, wl[r] + a * (uint32_t)sizeof(Number), zl[r] ? Number(0) : dv[r][a] + Y[r][a]);
+              dst_store<CD>(rdst, wPo[r], zP[r] ? Number(0) : dv[r][N - 1] + Y[r][N - 1]);
             }
+
+          if constexpr (ORD)
+            if (j == it.nl - 1)
+              {
+                // the item's stores reach agent scope before its flag does
+                if constexpr (kOrderCoherentDst)
+                  wait_vmem();
+                else
+                  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                double bs = 0.0;
+                if (ipart != nullptr)
+                  bs = block_sum<THREADS>(acc, s_red);
+                else
+                  __syncthreads();
+                acc = 0.0;
+                if (threadIdx.x == 0)
+                  {
+                    if (ipart != nullptr)
+                      __hip_atomic_store(ipart + item, bs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    // every wave of the workgroup released its stores before the barrier
+                    __hip_atomic_store(prm.order_flags + item, epoch, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                  }
+              }
 
 #pragma unroll
           for (int r = 0; r < R; ++r)
@@ -758,6 +947,60 @@ namespace mpdg
       // p . Ap: sum over cells of u_cell . y_cell (constrained entries are 0)
       if (partials != nullptr)
         pass_partials<THREADS>(acc, partials, partial_offset, partial_total, finalize, scal);
+    }
+
+    // Ordered schedule: the whole vmult in one launch (see vmult_body). The
+    // workgroup that leaves last sums the per-item p . Ap shares in ticket order
+    // (with partials and scal: scal->pap, scal->alpha = rz / pap, as the last
+    // colour pass does), then resets the ticket and exit counters and advances
+    // the epoch for the next launch on the stream.
+    template <int P, typename Number, int COEF, bool VEL, int CW, int R, int WPB, int MINW, bool MF>
+    __global__ void __launch_bounds__(64 * WPB, MINW)
+      vmult_ordered_kernel(const Number *__restrict__ src, Number *__restrict__ dst,
+                           const Number *__restrict__ heav, const CellParams<Number> prm,
+                           const int layers, const OrderGrid og, double *__restrict__ partials,
+                           SolverScalars *__restrict__ scal)
+    {
+      constexpr int THREADS = 64 * WPB;
+      if (scal != nullptr && scal->done)
+        return;
+      __shared__ unsigned s_epoch;
+      __shared__ bool s_last;
+      __shared__ double red[WPB];
+      if (threadIdx.x == 0)
+        s_epoch = __hip_atomic_load(&prm.order->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u;
+      __syncthreads();
+      const unsigned epoch = __builtin_amdgcn_readfirstlane(s_epoch);
+      const int total = og.off[8];
+      double *ipart = (partials != nullptr && scal != nullptr) ? partials : nullptr;
+      vmult_body<P, Number, COEF, VEL, CW, R, WPB, MF, true>(src, dst, heav, prm, 0, layers, total,
+                                                             0, 0, og, ipart, epoch);
+      // the per-item shares are agent-coherent stores: completing them orders
+      // them before the exit count
+      wait_vmem();
+      __syncthreads();
+      if (threadIdx.x == 0)
+        s_last = atomicAdd(&prm.order->done, 1u) == gridDim.x - 1;
+      __syncthreads();
+      if (!s_last)
+        return;
+      if (ipart != nullptr)
+        {
+          const double pap = ordered_sum<THREADS>(ipart, total, red);
+          if (threadIdx.x == 0)
+            {
+              scal->pap = pap;
+              scal->alpha = scal->rz / pap;
+            }
+        }
+      if (threadIdx.x < kOrderQueues)
+        __hip_atomic_store(&prm.order->ticket[threadIdx.x][0], 0u, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+      if (threadIdx.x == 0)
+        {
+          __hip_atomic_store(&prm.order->done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_store(&prm.order->epoch, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
     }
 
     // Batched vmult: the workgroups of one launch are split among the
@@ -1072,6 +1315,27 @@ namespace mpdg
       const int per_cu = prm.blocks_per_cu > 0 ? prm.blocks_per_cu : resident;
       const VmultGrid vg = vmult_grid(prm.nx, prm.ny, prm.nz, CPB, prm.zlayers,
                                       per_cu * device_cu_count());
+      if (prm.order != nullptr)
+        {
+          auto okernel =
+            vmult_ordered_kernel<P, Number, COEF, VEL, Shape::CW, Shape::R, Shape::WPB, Shape::MINW, MF>;
+          static int oresident = 0;
+          if (oresident == 0)
+            {
+              int nb = 0;
+              if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, okernel, THREADS, 0) != hipSuccess ||
+                  nb < 1)
+                nb = 1;
+              oresident = nb < kMaxResidentPerCu ? nb : kMaxResidentPerCu;
+            }
+          OrderGrid og{};
+          for (int c = 0; c < 8; ++c)
+            og.off[c + 1] = og.off[c] + vg.items[c];
+          const int opc = prm.blocks_per_cu > 0 ? prm.blocks_per_cu : oresident;
+          const int grid = std::max(1, std::min(og.off[8], opc * device_cu_count()));
+          okernel<<<grid, THREADS, 0, stream>>>(src, dst, heav, prm, prm.zlayers, og, partials, scal);
+          return hipGetLastError();
+        }
       for (int c = 0; c < 8; ++c)
         {
           if (vg.grid[c] == 0)

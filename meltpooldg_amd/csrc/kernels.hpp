@@ -41,11 +41,29 @@ namespace mpdg
     int blocks_per_cu;        // persistent workgroups per CU (0: occupancy limit)
     int variant;              // vmult form: kVariantLine or kVariantCell
     int tile_log2;            // cell form: log2 of the tile's cells along x (3..6)
+    // line forms, ordered schedule (order != nullptr): one launch per vmult,
+    // work items taken in colour order from order->ticket, each waiting for the
+    // completion flags (order_flags, one per item, >= nx*ny*nz entries) of its
+    // neighbouring items of earlier colours
+    struct OrderState *order;
+    unsigned *order_flags;
     const Number *sin2pi[3]; // sin(2 pi x) at the quadrature coordinates of each direction
     // in-plane velocity at q, separable: u_x = vel_x[0](x) vel_y[0](y), u_y = vel_x[1](x) vel_y[1](y)
     // (tables over the x resp. y quadrature coordinates); u_z = w0
     const Number *vel_x[2];
     const Number *vel_y[2];
+  };
+
+  // Device state of the ordered vmult schedule, one per operator precision
+  // (zero-initialised once). ticket: next work item of each queue; done: workgroups that have
+  // left; epoch: number of finished launches (an item's flag holds the epoch of
+  // the launch that completed it, so flags never need clearing); error: set if
+  // a bounded wait for a neighbour ran out (results then unreliable).
+  constexpr int kOrderQueues = 8; // one ticket queue per XCD
+  struct OrderState
+  {
+    unsigned ticket[kOrderQueues][32]; // queue q's counter in its own 128-byte line
+    unsigned done, epoch, error;
   };
 
   // 1D tables of the vmult kernel in even-odd form (device copy by value).

@@ -326,7 +326,8 @@ class HeatDiffuseOperator:
     def _tuning_dict(cls, t):
         names = {v: k for k, v in cls._VARIANTS.items()}
         return {"z_layers": t.z_layers, "blocks_per_cu": t.blocks_per_cu,
-                "variant": names.get(t.variant, t.variant), "tile_log2": t.tile_log2}
+                "variant": names.get(t.variant, t.variant), "tile_log2": t.tile_log2,
+                "schedule": "ordered" if t.schedule == 1 else "passes"}
 
     def create_rhs(self, dst, src, temperature, heat_source=None):
         """HeatDiffuseMultiPhaseOperator::create_rhs(dst, src = temperature_old)
@@ -346,7 +347,7 @@ class HeatDiffuseOperator:
         return self._tuning_dict(t)
 
     def set_tuning(self, z_layers=None, blocks_per_cu=None, variant=None, tile_log2=None,
-                   precision=None):
+                   precision=None, schedule=None):
         cur = self.get_tuning(precision)
         t = L.Tuning()
         t.z_layers = int(cur["z_layers"] if z_layers is None else z_layers)
@@ -354,6 +355,10 @@ class HeatDiffuseOperator:
         v = cur["variant"] if variant is None else variant
         t.variant = self._VARIANTS[v] if isinstance(v, str) else int(v)
         t.tile_log2 = int(cur["tile_log2"] if tile_log2 is None else tile_log2)
+        sc = cur["schedule"] if schedule is None else schedule
+        if sc not in ("passes", "ordered"):
+            raise ValueError("schedule must be 'passes' or 'ordered'")
+        t.schedule = 1 if sc == "ordered" else 0
         L.check(self._lib.mpdg_heat_set_tuning(self._h, self._prec_code(precision), ctypes.byref(t)))
 
     def autotune(self, precision=None, repeats=5):
