@@ -12,6 +12,7 @@
 // ReductionControl, include/meltpooldg/linear_algebra/linear_solver.hpp:36-38,
 // 106-111, Jacobi dst = src .* inverse_diag, preconditioner_jacobi.hpp:51-56).
 #include "kernels.hpp"
+#include "reduce.hpp"
 
 namespace mpdg
 {
@@ -67,23 +68,18 @@ namespace mpdg
               double s = 0.0;
               for (int i = 0; i < T / 64; ++i)
                 s += sh[k][i];
-              partials[k * nblocks + blockIdx.x] = s;
+              store_partial(partials + k * nblocks + blockIdx.x, s);
             }
-          __threadfence();
+          wait_vmem();
           last = atomicAdd(counter, 1u) == (unsigned)sg.nblk - 1;
         }
       __syncthreads();
       if (!last || threadIdx.x != 0)
         return false;
-      __threadfence();
 #pragma unroll
       for (int k = 0; k < NV; ++k)
         {
-          double s = 0.0;
-          for (int b = sg.blk0; b < sg.blk0 + sg.nblk; ++b)
-            s += __hip_atomic_load(partials + k * nblocks + b, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-          tot[k] = s;
+          tot[k] = strided_sum(partials + k * nblocks, sg.blk0, sg.blk0 + sg.nblk, 1);
         }
       *counter = 0u;
       return true;

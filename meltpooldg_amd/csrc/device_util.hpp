@@ -5,6 +5,7 @@
 
 #include "ctables.hpp"
 #include "kernels.hpp"
+#include "reduce.hpp"
 
 namespace mpdg
 {
@@ -324,9 +325,7 @@ namespace mpdg
     template <int THREADS>
     __device__ __forceinline__ double ordered_sum(const double *partials, int count, double *sh)
     {
-      double v = 0.0;
-      for (int i = threadIdx.x; i < count; i += THREADS)
-        v += __hip_atomic_load(partials + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const double v = strided_sum(partials, threadIdx.x, count, THREADS);
       __syncthreads();
       return block_sum<THREADS>(v, sh);
     }
@@ -344,11 +343,11 @@ namespace mpdg
       const double bs = block_sum<THREADS>(acc, red);
       if (threadIdx.x == 0)
         {
-          partials[offset + blockIdx.x] = bs;
+          store_partial(partials + offset + blockIdx.x, bs);
           bool lst = false;
           if (finalize)
             {
-              __threadfence();
+              wait_vmem();
               const unsigned prev = atomicAdd(&scal->counter, 1u);
               lst = prev == gridDim.x - 1;
             }
@@ -357,7 +356,6 @@ namespace mpdg
       __syncthreads();
       if (is_last)
         {
-          __threadfence();
           const double pap = ordered_sum<THREADS>(partials, total, red);
           if (threadIdx.x == 0)
             {

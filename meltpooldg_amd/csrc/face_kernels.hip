@@ -277,15 +277,14 @@ namespace mpdg
           const double bs = block_sum<kRobinThreads>(pdot, red);
           if (threadIdx.x == 0)
             {
-              partials[blockIdx.x] = bs;
-              __threadfence();
+              store_partial(partials + blockIdx.x, bs);
+              wait_vmem();
               const unsigned prev = atomicAdd(&scal->counter, 1u);
               is_last = prev == gridDim.x - 1;
             }
           __syncthreads();
           if (is_last)
             {
-              __threadfence();
               const double s = ordered_sum<kRobinThreads>(partials, gridDim.x, red);
               if (threadIdx.x == 0)
                 {

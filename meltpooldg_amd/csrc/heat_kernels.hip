@@ -214,9 +214,6 @@ namespace mpdg
     // write-back / invalidate fences (which cost more than the whole vmult).
     constexpr bool kOrderCoherentDst = true;
 
-    // vmcnt(0): this wavefront's vector memory operations have completed
-    __device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
-
     template <bool C, typename Number>
     __device__ __forceinline__ Number dst_load(__amdgpu_buffer_rsrc_t r, uint32_t off)
     {

@@ -398,14 +398,13 @@ namespace mpdg
       const double bs = block_sum<THREADS>(acc, red);
       if (threadIdx.x == 0)
         {
-          partials[blockIdx.x] = bs;
-          __threadfence();
+          store_partial(partials + blockIdx.x, bs);
+          wait_vmem();
           is_last = atomicAdd(ticket, 1u) == gridDim.x - 1;
         }
       __syncthreads();
       if (is_last)
         {
-          __threadfence();
           const double total = ordered_sum<THREADS>(partials, gridDim.x, red);
           if (threadIdx.x == 0)
             {

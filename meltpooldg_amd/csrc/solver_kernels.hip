@@ -16,6 +16,7 @@
 // and the inversion guard of compute_inverse_diagonal_from_matrixfree
 // (source/heat/heat_diffuse_operator.cpp:282-283).
 #include "kernels.hpp"
+#include "reduce.hpp"
 
 namespace mpdg
 {
@@ -41,10 +42,7 @@ namespace mpdg
       const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
       for (int c = wv; c < ncol; c += T / 64)
         {
-          double s = 0.0;
-          for (int b = lane; b < (int)gridDim.x; b += 64)
-            s += __hip_atomic_load(partials + c * kVecBlocks + b, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
+          double s = strided_sum(partials + c * kVecBlocks, lane, (int)gridDim.x, 64);
           s = wave_sum(s);
           if (lane == 0)
             out[c] = s;
@@ -89,24 +87,18 @@ namespace mpdg
         {
 #pragma unroll
           for (int k = 0; k < NV; ++k)
-            partials[k * kVecBlocks + blockIdx.x] = v[k];
-          __threadfence();
+            store_partial(partials + k * kVecBlocks + blockIdx.x, v[k]);
+          wait_vmem();
           const unsigned prev = atomicAdd(counter, 1u);
           last = prev == gridDim.x - 1;
         }
       __syncthreads();
       if (!last)
         return false;
-      __threadfence();
       double s[NV];
 #pragma unroll
       for (int k = 0; k < NV; ++k)
-        {
-          s[k] = 0.0;
-          for (int i = threadIdx.x; i < (int)gridDim.x; i += T)
-            s[k] += __hip_atomic_load(partials + k * kVecBlocks + i, __ATOMIC_RELAXED,
-                                      __HIP_MEMORY_SCOPE_AGENT);
-        }
+        s[k] = strided_sum(partials + k * kVecBlocks, threadIdx.x, (int)gridDim.x, T);
       block_sums<NV>(s, sh);
       if (threadIdx.x == 0)
         {
@@ -466,18 +458,15 @@ namespace mpdg
           double s = 0.0;
           for (int i = 0; i < T / 64; ++i)
             s += sh[threadIdx.x][i];
-          partials[threadIdx.x * kVecBlocks + blockIdx.x] = s;
+          store_partial(partials + threadIdx.x * kVecBlocks + blockIdx.x, s);
+          wait_vmem();
         }
       __syncthreads();
       if (threadIdx.x == 0)
-        {
-          __threadfence();
-          last = atomicAdd(counter, 1u) == gridDim.x - 1;
-        }
+        last = atomicAdd(counter, 1u) == gridDim.x - 1;
       __syncthreads();
       if (!last)
         return;
-      __threadfence();
       ordered_column_sums(partials, k, out);
       if (threadIdx.x == 0)
         *counter = 0u;
@@ -558,18 +547,15 @@ namespace mpdg
           double s = 0.0;
           for (int i = 0; i < T / 64; ++i)
             s += sh[threadIdx.x][i];
-          partials[threadIdx.x * kVecBlocks + blockIdx.x] = s;
+          store_partial(partials + threadIdx.x * kVecBlocks + blockIdx.x, s);
+          wait_vmem();
         }
       __syncthreads();
       if (threadIdx.x == 0)
-        {
-          __threadfence();
-          last = atomicAdd(counter, 1u) == gridDim.x - 1;
-        }
+        last = atomicAdd(counter, 1u) == gridDim.x - 1;
       __syncthreads();
       if (!last)
         return;
-      __threadfence();
       ordered_column_sums(partials, k, out);
       if (threadIdx.x == 0)
         *counter = 0u;
@@ -746,18 +732,15 @@ namespace mpdg
           double s = 0.0;
           for (int q = 0; q < T / 64; ++q)
             s += sh[threadIdx.x][q];
-          partials[threadIdx.x * kVecBlocks + blockIdx.x] = s;
+          store_partial(partials + threadIdx.x * kVecBlocks + blockIdx.x, s);
+          wait_vmem();
         }
       __syncthreads();
       if (threadIdx.x == 0)
-        {
-          __threadfence();
-          last = atomicAdd(counter, 1u) == gridDim.x - 1;
-        }
+        last = atomicAdd(counter, 1u) == gridDim.x - 1;
       __syncthreads();
       if (!last)
         return;
-      __threadfence();
       ordered_column_sums(partials, nacc, out);
       if (threadIdx.x == 0)
         *counter = 0u;
