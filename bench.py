@@ -152,8 +152,10 @@ def solve_legs(args, torch, dev):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    # a vmult is 0.25 ms: 100 untimed ones (25 ms) bring the GPU to its
+    # steady clocks; with 5 the first timed vmults ran about 10 % slower
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-solve", action="store_true", help="skip the CG time-to-solution legs")
     ap.add_argument("--tune", action="store_true",
