@@ -206,7 +206,7 @@ namespace
         case 1: t.z_layers = fp32 ? 4 : 2; t.blocks_per_cu = fp32 ? 4 : 0; break;
         case 2: t.z_layers = fp32 ? 4 : 2; t.blocks_per_cu = fp32 ? 8 : 4; break;
         case 3: t.z_layers = fp32 ? 2 : 4; t.blocks_per_cu = fp32 ? 8 : 4; break;
-        case 4: t.z_layers = fp32 ? 2 : 4; t.blocks_per_cu = fp32 ? 8 : 4; break;
+        case 4: t.z_layers = fp32 ? 2 : 4; t.blocks_per_cu = fp32 ? 0 : 4; break;
         case 5:
           // FP32: the matrix-core form (256-VGPR budget, no spills) is the faster one
           t.z_layers = fp32 ? 2 : 4;

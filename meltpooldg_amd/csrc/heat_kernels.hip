@@ -1280,16 +1280,21 @@ namespace mpdg
     // slots per lane (CW*(P+1)^2 lines fill the last slot), 4 wavefronts per
     // workgroup. The MFMA form holds the matrix columns in 4(P+1) more VGPRs,
     // so it takes fewer cells per wavefront at degree 4 and 6 and a 256-VGPR
-    // budget at degree 5.
-    template <int P, bool FP32, bool MF = false>
+    // budget at degree 5. LEAN (colour-pass launches of an operator without a
+    // velocity or a nodal heaviside): the FP32 degree-4 FMA form takes 2 cells
+    // per wavefront under an 80-VGPR budget (6 waves per SIMD), which measured
+    // 106.7 against 96.7 GDoF/s for 5 cells at 4 waves (64^3, sine k); with a
+    // velocity or a heaviside that budget spills.
+    template <int P, bool FP32, bool MF = false, bool LEAN = false>
     struct VShape
     {
+      static constexpr bool L4 = LEAN && FP32 && !MF && P == 4;
       static constexpr int CW = MF ? (P == 4 ? 2 : P == 5 ? 3 : P == 6 ? 1 : 4)
-                                   : P == 1 ? 16 : P == 2 ? 7 : P == 3 ? 4 : P == 4 ? (FP32 ? 5 : 2) : P == 5 ? (FP32 ? 3 : 1) : 1;
+                                   : P == 1 ? 16 : P == 2 ? 7 : P == 3 ? 4 : P == 4 ? (FP32 ? (L4 ? 2 : 5) : 2) : P == 5 ? (FP32 ? 3 : 1) : 1;
       static constexpr int R = (CW * (P + 1) * (P + 1) + 63) / 64;
       static constexpr int WPB = 4;
       // register budget: 128 VGPRs (4 waves per SIMD), 256 for FP64 from degree 5
-      static constexpr int MINW = ((P >= 5 && !FP32) || (MF && P == 5)) ? 2 : kMinWavesPerSimd;
+      static constexpr int MINW = L4 ? 6 : ((P >= 5 && !FP32) || (MF && P == 5)) ? 2 : kMinWavesPerSimd;
     };
 
     template <typename Number, int P, int COEF, bool VEL, bool MF>
@@ -1297,7 +1302,7 @@ namespace mpdg
                             const Number *src, Number *dst, const Number *heav, double *partials,
                             SolverScalars *scal, hipStream_t stream)
     {
-      using Shape = VShape<P, sizeof(Number) == 4, MF>;
+      using Shape = VShape<P, sizeof(Number) == 4, MF, COEF != kCoefHeaviside && !VEL>;
       constexpr int THREADS = 64 * Shape::WPB;
       constexpr int CPB = Shape::CW * Shape::WPB;
       auto kernel = vmult_kernel<P, Number, COEF, VEL, Shape::CW, Shape::R, Shape::WPB, Shape::MINW, MF>;
@@ -1314,23 +1319,28 @@ namespace mpdg
                                       per_cu * device_cu_count());
       if (prm.order != nullptr)
         {
+          // the ordered kernel holds more state: the non-lean shape
+          using OShape = VShape<P, sizeof(Number) == 4, MF>;
+          constexpr int OTHREADS = 64 * OShape::WPB;
           auto okernel =
-            vmult_ordered_kernel<P, Number, COEF, VEL, Shape::CW, Shape::R, Shape::WPB, Shape::MINW, MF>;
+            vmult_ordered_kernel<P, Number, COEF, VEL, OShape::CW, OShape::R, OShape::WPB, OShape::MINW, MF>;
           static int oresident = 0;
           if (oresident == 0)
             {
               int nb = 0;
-              if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, okernel, THREADS, 0) != hipSuccess ||
+              if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, okernel, OTHREADS, 0) != hipSuccess ||
                   nb < 1)
                 nb = 1;
               oresident = nb < kMaxResidentPerCu ? nb : kMaxResidentPerCu;
             }
+          const int opc = prm.blocks_per_cu > 0 ? prm.blocks_per_cu : oresident;
+          const VmultGrid og8 = vmult_grid(prm.nx, prm.ny, prm.nz, OShape::CW * OShape::WPB,
+                                           prm.zlayers, opc * device_cu_count());
           OrderGrid og{};
           for (int c = 0; c < 8; ++c)
-            og.off[c + 1] = og.off[c] + vg.items[c];
-          const int opc = prm.blocks_per_cu > 0 ? prm.blocks_per_cu : oresident;
+            og.off[c + 1] = og.off[c] + og8.items[c];
           const int grid = std::max(1, std::min(og.off[8], opc * device_cu_count()));
-          okernel<<<grid, THREADS, 0, stream>>>(src, dst, heav, prm, prm.zlayers, og, partials, scal);
+          okernel<<<grid, OTHREADS, 0, stream>>>(src, dst, heav, prm, prm.zlayers, og, partials, scal);
           return hipGetLastError();
         }
       for (int c = 0; c < 8; ++c)
