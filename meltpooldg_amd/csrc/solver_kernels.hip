@@ -843,6 +843,14 @@ namespace mpdg
       return (int)(g < kVecBlocks ? (g < 1 ? 1 : g) : kVecBlocks);
     }
 
+    // grid of a pure streaming kernel (no block partials): one element per
+    // thread, so every load of the vector is in flight at once
+    inline int stream_grid(int64_t n)
+    {
+      const int64_t g = (n + T - 1) / T;
+      return (int)(g < 1 ? 1 : (g > (1 << 24) ? (1 << 24) : g));
+    }
+
     // grid of the 16-byte basis sweeps (W entries per lane and step)
     template <typename Number>
     int sweep_grid(int64_t n)
@@ -853,14 +861,14 @@ namespace mpdg
 
   hipError_t launch_invert_diagonal(double *d, int64_t n, hipStream_t s)
   {
-    invert_kernel<<<grid_for(n), T, 0, s>>>(d, n);
+    invert_kernel<<<stream_grid(n), T, 0, s>>>(d, n);
     return hipGetLastError();
   }
 
   hipError_t launch_jacobi(double *dst, const double *src, const double *inv, int64_t n,
                            hipStream_t s)
   {
-    jacobi_kernel<<<grid_for(n), T, 0, s>>>(dst, src, inv, n);
+    jacobi_kernel<<<stream_grid(n), T, 0, s>>>(dst, src, inv, n);
     return hipGetLastError();
   }
 
@@ -896,10 +904,10 @@ namespace mpdg
     if (aligned16(p, r, inv_diag))
       {
         const int64_t nv = (n + Wide<Number>::W - 1) / Wide<Number>::W;
-        cg_direction_wide_kernel<Number><<<grid_for(nv), T, 0, s>>>(p, r, inv_diag, n, scal);
+        cg_direction_wide_kernel<Number><<<stream_grid(nv), T, 0, s>>>(p, r, inv_diag, n, scal);
       }
     else
-      cg_direction_kernel<Number><<<grid_for(n), T, 0, s>>>(p, r, inv_diag, n, scal);
+      cg_direction_kernel<Number><<<stream_grid(n), T, 0, s>>>(p, r, inv_diag, n, scal);
     return hipGetLastError();
   }
 
@@ -920,7 +928,7 @@ namespace mpdg
   hipError_t launch_zero_constrained(double *v, int n1x, int n1y, int n1z, unsigned faces,
                                      hipStream_t s)
   {
-    zero_constrained_kernel<<<grid_for((int64_t)n1x * n1y * n1z), T, 0, s>>>(v, n1x, n1y, n1z,
+    zero_constrained_kernel<<<stream_grid((int64_t)n1x * n1y * n1z), T, 0, s>>>(v, n1x, n1y, n1z,
                                                                             faces);
     return hipGetLastError();
   }
@@ -928,25 +936,25 @@ namespace mpdg
   hipError_t launch_axpby(double *dst, double a, const double *x, double b, const double *y,
                           int64_t n, hipStream_t s)
   {
-    axpby_kernel<<<grid_for(n), T, 0, s>>>(dst, a, x, b, y, n);
+    axpby_kernel<<<stream_grid(n), T, 0, s>>>(dst, a, x, b, y, n);
     return hipGetLastError();
   }
 
   hipError_t launch_to_f32(float *dst, const double *src, int64_t n, hipStream_t s)
   {
-    to_f32_kernel<<<grid_for(n), T, 0, s>>>(dst, src, n);
+    to_f32_kernel<<<stream_grid(n), T, 0, s>>>(dst, src, n);
     return hipGetLastError();
   }
 
   hipError_t launch_add_f32(double *x, const float *d, int64_t n, hipStream_t s)
   {
-    add_f32_kernel<<<grid_for(n), T, 0, s>>>(x, d, n);
+    add_f32_kernel<<<stream_grid(n), T, 0, s>>>(x, d, n);
     return hipGetLastError();
   }
 
   hipError_t launch_fill(double *x, double v, int64_t n, hipStream_t s)
   {
-    fill_kernel<<<grid_for(n), T, 0, s>>>(x, v, n);
+    fill_kernel<<<stream_grid(n), T, 0, s>>>(x, v, n);
     return hipGetLastError();
   }
 
@@ -972,13 +980,13 @@ namespace mpdg
   hipError_t launch_permute(double *dst, const double *src, const uint32_t *index, int64_t n,
                             bool to_lattice, hipStream_t s)
   {
-    permute_kernel<<<grid_for(n), T, 0, s>>>(dst, src, index, n, to_lattice);
+    permute_kernel<<<stream_grid(n), T, 0, s>>>(dst, src, index, n, to_lattice);
     return hipGetLastError();
   }
 
   hipError_t launch_fill_f32(float *x, float v, int64_t n, hipStream_t s)
   {
-    fill_f32_kernel<<<grid_for(n), T, 0, s>>>(x, v, n);
+    fill_f32_kernel<<<stream_grid(n), T, 0, s>>>(x, v, n);
     return hipGetLastError();
   }
 
@@ -1038,12 +1046,12 @@ namespace mpdg
     if (aligned16(v, z, w, d))
       {
         const int64_t nv = (n + Wide<Number>::W - 1) / Wide<Number>::W;
-        scale_precondition_kernel<Number><<<grid_for(nv), T, 0, s>>>(v, z, w, d, nrm2, n, stop);
+        scale_precondition_kernel<Number><<<stream_grid(nv), T, 0, s>>>(v, z, w, d, nrm2, n, stop);
       }
     else
       {
-        scale_kernel<Number><<<grid_for(n), T, 0, s>>>(v, w, nrm2, n, stop);
-        diag_scale_kernel<Number><<<grid_for(n), T, 0, s>>>(z, v, d, n, stop);
+        scale_kernel<Number><<<stream_grid(n), T, 0, s>>>(v, w, nrm2, n, stop);
+        diag_scale_kernel<Number><<<stream_grid(n), T, 0, s>>>(z, v, d, n, stop);
       }
     return hipGetLastError();
   }
@@ -1051,14 +1059,14 @@ namespace mpdg
   template <typename Number>
   hipError_t launch_scale(Number *v, const Number *w, const double *nrm2, int64_t n, hipStream_t s)
   {
-    scale_kernel<Number><<<grid_for(n), T, 0, s>>>(v, w, nrm2, n, nullptr);
+    scale_kernel<Number><<<stream_grid(n), T, 0, s>>>(v, w, nrm2, n, nullptr);
     return hipGetLastError();
   }
 
   template <typename Number>
   hipError_t launch_diag_scale(Number *z, const Number *v, const Number *d, int64_t n, hipStream_t s)
   {
-    diag_scale_kernel<Number><<<grid_for(n), T, 0, s>>>(z, v, d, n, nullptr);
+    diag_scale_kernel<Number><<<stream_grid(n), T, 0, s>>>(z, v, d, n, nullptr);
     return hipGetLastError();
   }
 
@@ -1068,7 +1076,7 @@ namespace mpdg
   {
     if (k < 1 || k > kMaxBasis)
       return hipErrorInvalidValue;
-    gmres_update_kernel<Number, XNumber><<<grid_for(n), T, 0, s>>>(x, V, ld, k, y, d, n, kdev);
+    gmres_update_kernel<Number, XNumber><<<stream_grid(n), T, 0, s>>>(x, V, ld, k, y, d, n, kdev);
     return hipGetLastError();
   }
 
