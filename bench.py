@@ -27,6 +27,7 @@ N_CELLS = 64
 DEGREE = 4
 DT = 1e-3
 K_AMP = 0.5
+INNER_REL_TOLERANCE = 1e-4  # mixed CG: reduction asked of each inner FP32 solve
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
@@ -115,11 +116,15 @@ def copy_calibration(args, torch, dev):
 
 def time_solves(op, b, torch, LinearSolver, LinearSolverData):
     """CG + Jacobi to 1e-8 in FP64 and FP32-mixed, each after one untimed warm-up
-    solve (workspace allocation)."""
+    solve (workspace allocation). The mixed solve asks each inner FP32 CG for a
+    1e-4 reduction: two outer FP64 corrections reach 1e-8 (277 inner iterations,
+    0.091 s, against four corrections and 286 iterations, 0.095 s, at the 1e-2
+    default); the stopping test is the FP64 residual either way."""
     solve = {}
     for name, mixed in (("fp64", False), ("fp32_mixed", True)):
         data = LinearSolverData(solver_type="CG", preconditioner_type="Diagonal",
-                                rel_tolerance=1e-8, mixed_precision=mixed)
+                                rel_tolerance=1e-8, mixed_precision=mixed,
+                                inner_rel_tolerance=INNER_REL_TOLERANCE)
         x = torch.zeros_like(b)
         LinearSolver.solve(op, x, b, data)
         x.zero_()
@@ -134,6 +139,7 @@ def time_solves(op, b, torch, LinearSolver, LinearSolverData):
                        "rel_residual": rep["final_residual"] / rep["initial_residual"]}
         if mixed:
             solve[name]["speedup_vs_fp64"] = round(solve["fp64"]["seconds"] / t, 3)
+            solve[name]["inner_rel_tolerance"] = INNER_REL_TOLERANCE
     return solve
 
 
