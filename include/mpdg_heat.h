@@ -350,7 +350,9 @@ typedef enum {
   MPDG_SCHEDULE_ORDERED = 1  /* one launch per vmult: work items are taken in colour
                                 order and each waits only for the neighbouring items
                                 of earlier colours (line forms; the cell form always
-                                runs colour passes) */
+                                runs colour passes). Each wait is bounded: if one
+                                ever gives up, mpdg_heat_solve and mpdg_heat_time_vmult
+                                return MPDG_ERR_HIP */
 } mpdg_vmult_schedule;
 
 typedef struct {

@@ -449,6 +449,25 @@ namespace
     return MPDG_OK;
   }
 
+  // After a synchronising call: a bounded wait of the ordered vmult schedule
+  // that gave up (only a defect can cause it) leaves its error bit set, and the
+  // results since then are not to be trusted.
+  mpdg_status check_order_state(mpdg_heat *H)
+  {
+    for (int k = 0; k < 2; ++k)
+      {
+        const bool active = k == MPDG_FP64 ? H->prm64.order != nullptr : H->prm32.order != nullptr;
+        if (!active)
+          continue;
+        unsigned err = 0;
+        MPDG_HIP(hipMemcpy(&err, &H->order[k].ptr->error, sizeof(unsigned), hipMemcpyDeviceToHost));
+        if (err)
+          return fail(MPDG_ERR_HIP, "ordered vmult schedule: a wait for a neighbouring work item "
+                                    "timed out; results are unreliable");
+      }
+    return MPDG_OK;
+  }
+
   mpdg_status ensure_workspace(mpdg_heat *H, bool mixed)
   {
     const ColourGrid cg = colour_grid(H->desc.n_cells[0], H->desc.n_cells[1], H->desc.n_cells[2],
@@ -1455,7 +1474,11 @@ mpdg_status mpdg_heat_jacobi_vmult(mpdg_heat *h, double *dst, const double *src,
 mpdg_status mpdg_heat_solve(mpdg_heat *h, double *x, const double *b,
                             const mpdg_solver_params *params, mpdg_solver_report *report)
 {
-  return solve_impl(h, x, b, params, report, true);
+  const mpdg_status s = solve_impl(h, x, b, params, report, true);
+  if (s == MPDG_OK || s == MPDG_ERR_NOT_CONVERGED)
+    if (mpdg_status e = check_order_state(h))
+      return e;
+  return s;
 }
 
 static mpdg_status solve_impl(mpdg_heat *h, double *x, const double *b,
@@ -1756,7 +1779,7 @@ mpdg_status mpdg_heat_time_vmult(mpdg_heat *h, double *dst, const double *src, i
   float ms = 0.0f;
   MPDG_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
   *ms_per_vmult = ms / repeats;
-  return MPDG_OK;
+  return check_order_state(h);
 }
 
 } // extern "C"
