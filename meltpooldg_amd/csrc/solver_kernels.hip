@@ -643,17 +643,19 @@ namespace mpdg
             }
           else
             {
+              // after the loop vk holds the last group (g = (k-1)/G): the dot
+              // pass below reuses it instead of loading it again
+              Vec vk[G];
 #pragma unroll
               for (int g = 0; g < kMaxBasis / G; ++g)
                 if (g * G < k)
                   {
-                    Vec vc[G];
-                    load_group(g, j, vc);
+                    load_group(g, j, vk);
 #pragma unroll
                     for (int u = 0; u < G; ++u)
 #pragma unroll
                       for (int e = 0; e < W; ++e)
-                        wd[e] -= hs[g * G + u] * (double)vc[u][e];
+                        wd[e] -= hs[g * G + u] * (double)vk[u][e];
                   }
               Vec r;
 #pragma unroll
@@ -668,12 +670,20 @@ namespace mpdg
                 }
               else
                 {
+                  const int gl = (k - 1) / G;
 #pragma unroll
                   for (int g = 0; g < kMaxBasis / G; ++g)
                     if (g * G < k)
                       {
                         Vec vc[G];
-                        load_group(g, j, vc);
+                        if (g == gl)
+                          {
+#pragma unroll
+                            for (int u = 0; u < G; ++u)
+                              vc[u] = vk[u];
+                          }
+                        else
+                          load_group(g, j, vc);
 #pragma unroll
                         for (int u = 0; u < G; ++u)
                           {
