@@ -22,25 +22,26 @@ namespace mpdg
     }
 
     // p[i0] + p[i0 + stride] + ... (indices < count), added in that order, with
-    // the agent-coherent loads issued eight at a time: a plain loop waits for
+    // the agent-coherent loads issued sixteen at a time: a plain loop waits for
     // every load before the next add, a chain of memory latencies. (Adding 0.0
     // for the missing tail entries changes nothing: the sum starts at +0.0, so
     // it is never -0.0.)
     __device__ __forceinline__ double strided_sum(const double *p, int i0, int count, int stride)
     {
+      constexpr int kBatch = 16;
       double v = 0.0;
-      for (int i = i0; i < count; i += 8 * stride)
+      for (int i = i0; i < count; i += kBatch * stride)
         {
-          double t[8];
+          double t[kBatch];
 #pragma unroll
-          for (int u = 0; u < 8; ++u)
+          for (int u = 0; u < kBatch; ++u)
             {
               const int k = i + u * stride;
               t[u] = k < count ? __hip_atomic_load(p + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
                                : 0.0;
             }
 #pragma unroll
-          for (int u = 0; u < 8; ++u)
+          for (int u = 0; u < kBatch; ++u)
             v += t[u];
         }
       return v;
