@@ -946,6 +946,22 @@ namespace mpdg
         pass_partials<THREADS>(acc, partials, partial_offset, partial_total, finalize, scal);
     }
 
+    // p . Ap of the colour passes: partials[0..total) summed in index order,
+    // scal->pap and scal->alpha = rz / pap (skipped once scal->done is set).
+    __global__ void __launch_bounds__(256) pap_finalize_kernel(const double *__restrict__ partials,
+                                                               int total, SolverScalars *scal)
+    {
+      __shared__ double red[4];
+      if (scal->done)
+        return;
+      const double pap = ordered_sum<256>(partials, total, red);
+      if (threadIdx.x == 0)
+        {
+          scal->pap = pap;
+          scal->alpha = scal->rz / pap;
+        }
+    }
+
     // Ordered schedule: the whole vmult in one launch (see vmult_body). The
     // workgroup that leaves last sums the per-item p . Ap shares in ticket order
     // (with partials and scal: scal->pap, scal->alpha = rz / pap, as the last
@@ -1347,10 +1363,15 @@ namespace mpdg
         {
           if (vg.grid[c] == 0)
             continue;
+          // the passes only store their p . Ap partials; one workgroup sums
+          // them after the last pass (the launch boundary orders the stores),
+          // so no workgroup of the last pass waits for its stores to complete
           kernel<<<vg.grid[c], THREADS, 0, stream>>>(src, dst, heav, prm, c, prm.zlayers,
                                                      vg.items[c], partials, vg.offset[c], vg.total,
-                                                     c == vg.last ? 1 : 0, scal);
+                                                     0, scal);
         }
+      if (partials != nullptr && scal != nullptr && vg.last >= 0)
+        pap_finalize_kernel<<<1, 256, 0, stream>>>(partials, vg.total, scal);
       return hipGetLastError();
     }
 
