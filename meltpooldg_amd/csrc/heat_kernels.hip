@@ -212,7 +212,6 @@ namespace mpdg
     // workgroup publishes an item only after its stores have completed
     // (vmcnt(0)), so neighbours on other XCDs read the sums without whole-cache
     // write-back / invalidate fences (which cost more than the whole vmult).
-    constexpr bool kOrderCoherentDst = true;
 
     template <bool C, typename Number>
     __device__ __forceinline__ Number dst_load(__amdgpu_buffer_rsrc_t r, uint32_t off)
@@ -255,8 +254,8 @@ namespace mpdg
     // colour must have stored. Items touch nodes of their 26 neighbours in the
     // (segment, row, z-chunk) grid; a neighbour's colour is its parity, and its
     // ticket inverts decode_item. Lanes 0..26 of wave 0 poll one neighbour
-    // each; the workgroup then synchronises (and, without coherent dst
-    // accesses, acquires at agent scope) so the neighbours' stores are visible.
+    // each; the workgroup then synchronises, and its coherent dst loads see
+    // the neighbours' completed coherent stores.
     // Every awaited ticket is lower than this one, and the lowest unfinished
     // ticket never waits: its queue serves tickets in order, so it is held or
     // next to be taken by a running workgroup. Hence the waits end.
@@ -292,8 +291,6 @@ namespace mpdg
             }
         }
       __syncthreads();
-      if constexpr (!kOrderCoherentDst)
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
 
     // Gather of one x-line (y = s0, z = s1 of cell layer cz, block cell cb) with
@@ -469,7 +466,7 @@ namespace mpdg
       constexpr int CPB = CW * WPB;
       constexpr int THREADS = 64 * WPB;
       constexpr bool H2 = COEF == kCoefHeaviside;
-      constexpr bool CD = ORD && kOrderCoherentDst; // agent-coherent dst accesses
+      constexpr bool CD = ORD; // agent-coherent dst accesses
       static_assert(LT <= 64 * R && LT > 64 * (R - 1), "lines must fill the last lane slot");
       constexpr ct::Tab<N> T1 = ct::VTab<P>::t;
 
@@ -888,10 +885,7 @@ namespace mpdg
             if (j == it.nl - 1)
               {
                 // the item's stores reach agent scope before its flag does
-                if constexpr (kOrderCoherentDst)
-                  wait_vmem();
-                else
-                  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                wait_vmem();
                 double bs = 0.0;
                 if (ipart != nullptr)
                   bs = block_sum<THREADS>(acc, s_red);
