@@ -44,13 +44,56 @@ def algorithmic_counts(n_cells, degree, bytes_per=8):
             "flops": cells * (12 * nq**4 + 36 * nq**3)}
 
 
+def host_cpu_info():
+    """What the CPU leg ran on: logical CPUs, the ones this process may use
+    (affinity, cgroup quota, OMP_NUM_THREADS), model name and sockets (lscpu)."""
+    import subprocess
+    info = {"nproc": os.cpu_count()}
+    try:
+        info["affinity"] = len(os.sched_getaffinity(0))
+    except AttributeError:
+        info["affinity"] = os.cpu_count()
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()[:2]
+            if q != "max":
+                quota = max(1, int(int(q) / int(per)))
+    except (OSError, ValueError):
+        pass
+    info["cgroup_cpu_quota"] = quota
+    info["omp_num_threads"] = os.environ.get("OMP_NUM_THREADS")
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() == "Model name":
+                info["model"] = v.strip()
+            elif k.strip() == "Socket(s)":
+                info["sockets"] = v.strip()
+            elif k.strip() == "Core(s) per socket":
+                info["cores_per_socket"] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    usable = [info["affinity"] or 1]
+    if quota:
+        usable.append(quota)
+    if info["omp_num_threads"] and info["omp_num_threads"].isdigit():
+        usable.append(int(info["omp_num_threads"]))
+    info["threads_used"] = min(usable)
+    return info
+
+
 def cpu_baseline(threads=None, budget_s=10.0, min_samples=20):
     """Times the oracle's cell-wise sum-factorised FP64 vmult on the host cores
     (the reference cannot be built here: it needs deal.II + p4est + Trilinos +
     MPI): median of at least `min_samples` vmults after 3 warm-ups, sampling
-    for about `budget_s` seconds; also one core, 3 vmults."""
+    for about `budget_s` seconds, on every CPU this process may use (the
+    smallest of its affinity set, its cgroup quota and OMP_NUM_THREADS); also
+    one core, 3 vmults."""
     from oracle import cpu
-    threads = threads or min(16, os.cpu_count() or 1)
+    host = host_cpu_info()
+    threads = threads or host["threads_used"]
     pb = cpu.Problem(N_CELLS, DEGREE, dt=DT, k_amplitude=K_AMP)
     u = pb.random(0)
 
@@ -72,11 +115,13 @@ def cpu_baseline(threads=None, budget_s=10.0, min_samples=20):
     t1, k1 = sample(1, 0, 3, 0.0)
     cpu.set_threads(threads)
     return {"value": round(pb.N / t / 1e9, 4), "unit": "GDoF/s", "cores": threads, "kind": "port",
+            "label": "CPU restatement, not deal.II",
             "sample": f"median of {k} FP64 vmults of the 64^3 Q4 operator after 3 warm-ups "
                       f"(C++ restatement of the reference cell loop, not deal.II; OpenMP {threads} "
                       f"threads), {t*1e3:.1f} ms each; 1 core: {t1*1e3:.0f} ms per vmult "
                       f"({pb.N / t1 / 1e9:.4f} GDoF/s, {k1} vmults)",
-            "one_core_value": round(pb.N / t1 / 1e9, 5)}
+            "one_core_value": round(pb.N / t1 / 1e9, 5),
+            "host": host}
 
 
 def measured_traffic(kernel_name):
@@ -145,7 +190,7 @@ def time_solves(op, b, torch, LinearSolver, LinearSolverData):
 
 def solve_legs(args, torch, dev):
     from meltpooldg_amd import HeatDiffuseOperator, LinearSolver, LinearSolverData
-    from oracle.rng import random_vector
+    from meltpooldg_amd.inputs import random_vector
     op = HeatDiffuseOperator(N_CELLS, DEGREE, conductivity_amplitude=K_AMP)
     op.reset_time_increment(DT)
     b = torch.tensor(random_vector(N_CELLS, DEGREE, 1), dtype=torch.float64, device=dev)
@@ -187,7 +232,7 @@ def main():
 
     import torch
     from meltpooldg_amd import HeatDiffuseOperator, LinearSolver, LinearSolverData
-    from oracle.rng import random_vector
+    from meltpooldg_amd.inputs import random_vector
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)

@@ -8,7 +8,7 @@ config 0: 16^3 Q2, gas (rho_cp = k = 1), dt = 1e-3, Dirichlet all faces: one
 config 2: 96^3 Q3, two-phase sphere (radius 0.3, centre 0.5^3) from a nodal
           smoothed heaviside of width 2 cells, k 1:100 and rho*cp 1:10 (equal
           cp, liquid inside): FP32 cell loop in FP64 defect-correction CG to
-          1e-8 against the FP64 CG solution.
+          1e-8, errors against the oracle's FP64 solution and residual.
 config 3: batched solves (see bench_batch.py).
 config 4: 64^3 Q5 convection-diffusion, vortex velocity, dt = 5e-4,
           k = 1e-2: GMRES(30) + Jacobi, FP64 and FP32-mixed.
@@ -78,6 +78,25 @@ def _solve(op, b, torch, **kw):
     return x, rep
 
 
+def _oracle_reference(pb, b, solver, inv_diag, rel_tol=1e-10):
+    """The oracle's FP64 solution of the same system (Jacobi with inv_diag)."""
+    t0 = time.perf_counter()
+    bn = b.cpu().numpy()
+    ref = pb.solve(bn, solver=solver, restart=30, rel_tol=rel_tol, inv_diag=inv_diag)
+    ref["s"] = time.perf_counter() - t0
+    ref["b"] = bn
+    return ref
+
+
+def _oracle_errors(pb, x, b, ref):
+    """Error of a GPU solution against the oracle's FP64 solution (max norm,
+    relative) and its relative residual in the oracle's FP64 operator."""
+    xn = x.cpu().numpy()
+    r = ref["b"] - pb.vmult(xn)
+    return {"err_vs_oracle_fp64": float(np.abs(xn - ref["x"]).max() / np.abs(ref["x"]).max()),
+            "oracle_rel_residual": float(np.linalg.norm(r) / np.linalg.norm(ref["b"]))}
+
+
 def config0(args, torch):
     from meltpooldg_amd import HeatDiffuseOperator
     from oracle import cpu
@@ -99,29 +118,12 @@ def config0(args, torch):
             "gpu_cg_s": rep["wall_s"], "cpu_cg_s": t_cpu_solve, "solution_rel_err": xerr}
 
 
-def sphere_heaviside(n, p, radius=0.3, centre=0.5, width_cells=2.0):
-    """Nodal smoothed heaviside of 0.3 - |x - c| with eps = width/2 cells
-    (CharacteristicFunctions::smoothed_heaviside,
-    include/meltpooldg/utilities/characteristic_functions.hpp:17-28)."""
-    n1 = n * p + 1
-    x = np.linspace(0.0, 1.0, n1)
-    # GLL support points inside each cell
-    from oracle import tables as T
-    gll = T.gll(p)
-    coords = np.concatenate([(c + gll[:-1]) / n for c in range(n)] + [[1.0]])
-    eps = 0.5 * width_cells / n
-    Z, Y, X = np.meshgrid(coords, coords, coords, indexing="ij")
-    d = radius - np.sqrt((X - centre) ** 2 + (Y - centre) ** 2 + (Z - centre) ** 2)
-    H = np.where(d > eps, 1.0, np.where(d <= -eps, 0.0,
-                 (d + eps) / (2 * eps) + np.sin(np.pi * d / eps) / (2 * np.pi)))
-    return H.reshape(-1), x
-
-
 def config2(args, torch):
     from meltpooldg_amd import HeatDiffuseOperator, MaterialData, PhaseData
     from oracle import cpu
     n, p = 96, 3
-    H, _ = sphere_heaviside(n, p)
+    from meltpooldg_amd.inputs import sphere_heaviside
+    H = sphere_heaviside(n, p)
     mat = MaterialData(type="gas_liquid", gas=PhaseData(1.0, 1.0, 1.0),
                        liquid=PhaseData(10.0, 1.0, 100.0))
     results = {"config": 2, "workload": "96^3 Q3 two-phase sphere, k 1:100, rho*cp 1:10, CG+Jacobi to 1e-8"}
@@ -140,18 +142,19 @@ def config2(args, torch):
     b = _t(pb.random(1), torch)
     x64, rep64 = _solve(op, b, torch, solver_type="CG", rel_tolerance=1e-8)
     xmx, repmx = _solve(op, b, torch, solver_type="CG", rel_tolerance=1e-8, mixed_precision=True)
-    # reference: FP64 solve to 1e-12 (the discrete solution the 1e-8 solves approximate)
-    xref, repref = _solve(op, b, torch, solver_type="CG", rel_tolerance=1e-12)
-    e = lambda x: float((x - xref).abs().max() / xref.abs().max())
+    # reference: the oracle's FP64 CG + Jacobi solution to 1e-10 (the discrete
+    # solution the 1e-8 solves approximate) and the oracle's own residuals
+    ref = _oracle_reference(pb, b, "CG", pb.inverse_diagonal())
+    e = lambda x: _oracle_errors(pb, x, b, ref)
     results.update({"n_dofs": op.n_dofs(), "vmult_rel_err_vs_oracle": err,
                     "cpu_vmult_ms": t_cpu_vmult * 1e3,
                     "gpu_vmult_ms_fp64": _vmult_time(op, torch, torch.float64),
                     "gpu_vmult_ms_fp32": _vmult_time(op32, torch, torch.float32),
+                    "oracle_fp64_cg_1e-10": {"iterations": ref["iterations"], "s": ref["s"]},
                     "fp64": {"s": rep64["wall_s"], "iterations": rep64["iterations"],
-                             "err_vs_fp64_1e-12": e(x64)},
+                             **e(x64)},
                     "fp32_mixed": {"s": repmx["wall_s"], "inner_iterations": repmx["iterations"],
-                                   "outer_iterations": repmx["outer_iterations"],
-                                   "err_vs_fp64_1e-12": e(xmx)},
+                                   "outer_iterations": repmx["outer_iterations"], **e(xmx)},
                     "speedup_mixed_vs_fp64": rep64["wall_s"] / repmx["wall_s"]})
     return results
 
@@ -173,15 +176,18 @@ def config4(args, torch):
     b = _t(pb.random(1), torch)
     x64, rep64 = _solve(op, b, torch, solver_type="GMRES", rel_tolerance=1e-8)
     xmx, repmx = _solve(op, b, torch, solver_type="GMRES", rel_tolerance=1e-8, mixed_precision=True)
-    xref, _ = _solve(op, b, torch, solver_type="GMRES", rel_tolerance=1e-12)
-    e = lambda x: float((x - xref).abs().max() / xref.abs().max())
+    from oracle.kron import KronOperator
+    inv = KronOperator(n, p, dt=5e-4, k=1e-2, velocity=True, w0=0.2).inverse_diagonal()
+    ref = _oracle_reference(pb, b, "GMRES", inv)
+    e = lambda x: _oracle_errors(pb, x, b, ref)
     return {"config": 4, "workload": "64^3 Q5 convection-diffusion (vortex), GMRES(30)+Jacobi to 1e-8",
             "n_dofs": op.n_dofs(), "tuning": tuning, "vmult_rel_err_vs_oracle": err, "cpu_vmult_ms": t_cpu_vmult * 1e3,
             "gpu_vmult_ms_fp64": _vmult_time(op, torch, torch.float64),
             "gpu_vmult_ms_fp32": _vmult_time(op32, torch, torch.float32),
-            "fp64": {"s": rep64["wall_s"], "iterations": rep64["iterations"], "err_vs_fp64_1e-12": e(x64)},
+            "oracle_fp64_gmres_1e-10": {"iterations": ref["iterations"], "s": ref["s"]},
+            "fp64": {"s": rep64["wall_s"], "iterations": rep64["iterations"], **e(x64)},
             "fp32_mixed": {"s": repmx["wall_s"], "inner_iterations": repmx["iterations"],
-                           "outer_iterations": repmx["outer_iterations"], "err_vs_fp64_1e-12": e(xmx)},
+                           "outer_iterations": repmx["outer_iterations"], **e(xmx)},
             "speedup_mixed_vs_fp64": rep64["wall_s"] / repmx["wall_s"]}
 
 

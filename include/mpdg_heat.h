@@ -230,6 +230,29 @@ mpdg_status mpdg_heat_jacobi_vmult(mpdg_heat *h, double *dst, const double *src,
 mpdg_status mpdg_heat_solve(mpdg_heat *h, double *x, const double *b,
                             const mpdg_solver_params *params, mpdg_solver_report *report);
 
+/* LinearSolver::solve(op, x, b, data, preconditioner) exactly
+ * (linear_solver.hpp:26-134): the caller's preconditioner object is applied as
+ * it stands, not updated (its update is the caller's, as
+ * HeatDiffuseOperation::solve calls preconditioner.update() before the solve,
+ * heat_diffuse_operation.cpp:275-276). inv_diag is the inverse diagonal of a
+ * JacobiPreconditioner (preconditioner_jacobi.hpp:51-69; device pointer of
+ * n_dofs() doubles, e.g. from mpdg_heat_inverse_diagonal, read only), or NULL
+ * for PreconditionIdentity; params->preconditioner_type is not read. A
+ * diagonal computed once can serve many solves (a time step's Newton
+ * increments). Otherwise as mpdg_heat_solve. */
+mpdg_status mpdg_heat_solve_preconditioned(mpdg_heat *h, double *x, const double *b,
+                                           const double *inv_diag,
+                                           const mpdg_solver_params *params,
+                                           mpdg_solver_report *report);
+
+/* SolverControl::enable_history_data / get_history_data (deal.II; the
+ * ReductionControl of linear_solver.hpp:36-38): from the next FP64 CG solve
+ * on, history[k] = ||r_k||_2 for k = 0 (the initial residual) .. iterations,
+ * entries past `length` dropped. history is a caller's device buffer, kept
+ * (not copied); length 0 (or NULL) stops recording. GMRES and the inner FP32
+ * solves of the mixed-precision solve do not record. */
+mpdg_status mpdg_heat_set_residual_history(mpdg_heat *h, double *history, int length);
+
 /* Default parameters: LinearSolverData defaults with CG + Diagonal. */
 void mpdg_solver_params_default(mpdg_solver_params *p);
 
@@ -350,9 +373,15 @@ typedef enum {
   MPDG_SCHEDULE_ORDERED = 1  /* one launch per vmult: work items are taken in colour
                                 order and each waits only for the neighbouring items
                                 of earlier colours (line forms; the cell form always
-                                runs colour passes). Each wait is bounded: if one
-                                ever gives up, mpdg_heat_solve and mpdg_heat_time_vmult
-                                return MPDG_ERR_HIP */
+                                runs colour passes). All workgroups of a launch are
+                                resident at once. Each wait is bounded: if one ever
+                                gives up, the handle's failure is sticky and returned
+                                as MPDG_ERR_HIP by the synchronising calls
+                                (mpdg_heat_solve, mpdg_heat_newton_solve,
+                                mpdg_heat_time_vmult) and by every later call on the
+                                handle, mpdg_heat_vmult included (the vmult itself does
+                                not synchronise: it queues a copy of the error bit that
+                                the next call reads) */
 } mpdg_vmult_schedule;
 
 typedef struct {

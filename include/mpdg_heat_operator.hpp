@@ -131,6 +131,7 @@ namespace mpdg
       , inverse_diag_(storage)
     {}
     void update() { op_.compute_inverse_diagonal_from_matrixfree(inverse_diag_); }
+    const DeviceVector &get_inverse_diagonal() const { return inverse_diag_; }
     void vmult(DeviceVector &dst, const DeviceVector &src) const
     {
       check(mpdg_heat_jacobi_vmult(op_.handle(), dst.data, src.data, inverse_diag_.data));
@@ -167,6 +168,31 @@ namespace mpdg
       p.mixed_precision = data.mixed_precision ? 1 : 0;
       mpdg_solver_report rep;
       const mpdg_status s = mpdg_heat_solve(op.handle(), solution.data, rhs.data, &p, &rep);
+      if (report)
+        *report = rep;
+      if (s == MPDG_ERR_NOT_CONVERGED)
+        throw NoConvergence(rep.iterations, rep.final_residual);
+      check(s);
+      return rep.iterations;
+    }
+
+    // LinearSolver::solve(op, x, b, data, preconditioner) (linear_solver.hpp:26-32):
+    // the Jacobi preconditioner is applied as the caller last updated it.
+    static int solve(const HeatOperator &op, DeviceVector &solution, const DeviceVector &rhs,
+                     const LinearSolverData &data, const JacobiPreconditioner &preconditioner,
+                     mpdg_solver_report *report = nullptr)
+    {
+      mpdg_solver_params p;
+      mpdg_solver_params_default(&p);
+      p.solver_type = data.solver_type;
+      p.max_iterations = data.max_iterations;
+      p.rel_tolerance = data.rel_tolerance;
+      p.abs_tolerance = data.abs_tolerance;
+      p.mixed_precision = data.mixed_precision ? 1 : 0;
+      mpdg_solver_report rep;
+      const mpdg_status s =
+        mpdg_heat_solve_preconditioned(op.handle(), solution.data, rhs.data,
+                                       preconditioner.get_inverse_diagonal().data, &p, &rep);
       if (report)
         *report = rep;
       if (s == MPDG_ERR_NOT_CONVERGED)

@@ -158,6 +158,10 @@ SIGNATURES = {
     "mpdg_heat_jacobi_vmult": (ctypes.c_int, [_VP, _VP, _VP, _VP]),
     "mpdg_heat_solve": (ctypes.c_int, [_VP, _VP, _VP, ctypes.POINTER(SolverParams),
                                        ctypes.POINTER(SolverReport)]),
+    "mpdg_heat_solve_preconditioned": (ctypes.c_int, [_VP, _VP, _VP, _VP,
+                                                      ctypes.POINTER(SolverParams),
+                                                      ctypes.POINTER(SolverReport)]),
+    "mpdg_heat_set_residual_history": (ctypes.c_int, [_VP, _VP, ctypes.c_int]),
     "mpdg_solver_params_default": (None, [ctypes.POINTER(SolverParams)]),
     "mpdg_last_error": (ctypes.c_char_p, []),
     "mpdg_heat_batch_create": (ctypes.c_int, [ctypes.POINTER(HeatDesc), ctypes.c_int,

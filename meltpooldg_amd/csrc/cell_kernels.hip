@@ -134,7 +134,7 @@ namespace mpdg
       constexpr ct::Tab<N> T1 = ct::CTab<P>::t;
       constexpr uint32_t SZ = sizeof(Number);
 
-      const uint32_t vbytes = (uint32_t)prm.n1x * prm.n1y * prm.n1z * SZ;
+      const uint32_t vbytes = vector_bytes(prm.n1x, prm.n1y, prm.n1z, (int)SZ);
       const __amdgpu_buffer_rsrc_t rsrc = make_rsrc(src, vbytes);
       const __amdgpu_buffer_rsrc_t rdst = make_rsrc(dst, vbytes);
       const __amdgpu_buffer_rsrc_t rheav = make_rsrc(H2 ? heav : src, vbytes);

@@ -83,6 +83,14 @@ struct mpdg_heat
   // ordered vmult schedule: device state and per-item flags per precision
   DevBuf<OrderState> order[2];
   DevBuf<unsigned> order_flags[2];
+  // sticky report of a timed-out ordered wait for calls that do not
+  // synchronise (the vmult entry points): each such launch is followed by an
+  // asynchronous copy of the error bit into pinned memory and an event; the
+  // next call on the handle reads the copy once its event has completed
+  unsigned *host_order_err = nullptr; // pinned, one per precision
+  hipEvent_t order_ev[2]{};
+  bool order_pending[2]{};
+  bool order_failed = false;
   VmultTables<double> vt64{};
   VmultTables<float> vt32{};
   DiagTables<double> dt64{};
@@ -103,7 +111,11 @@ struct mpdg_heat
   DevBuf<double> robin_rows; // one row of (p+1)^2 face integrals per Robin face cell
 
   // solver workspace
-  DevBuf<double> r, p, q, inv_diag, partials, history;
+  DevBuf<double> r, p, q, inv_diag, partials;
+  // SolverControl::enable_history_data: a caller's device buffer of
+  // history_len entries that the FP64 CG fills with ||r_k|| (k = 0 initial)
+  double *history = nullptr;
+  int history_len = 0;
   DevBuf<float> r32, p32, q32, x32, inv32;
   DevBuf<double> V64, gm;  // GMRES basis (FP64) and small coefficient block
   DevBuf<float> V32;       // GMRES basis of the FP32 inner solves
@@ -139,7 +151,7 @@ struct mpdg_heat
 
   ~mpdg_heat()
   {
-    for (auto *b : {&sin_tab64, &sin_tab32_src, &heav64, &r, &p, &q, &inv_diag, &partials, &history,
+    for (auto *b : {&sin_tab64, &sin_tab32_src, &heav64, &r, &p, &q, &inv_diag, &partials,
                     &V64, &gm, &robin_rows})
       b->release();
     for (auto *b : {&sin_tab32, &heav32, &r32, &p32, &q32, &x32, &inv32, &V32})
@@ -154,6 +166,11 @@ struct mpdg_heat
       }
     if (host_nres)
       (void)hipHostFree(host_nres);
+    if (host_order_err)
+      (void)hipHostFree(host_order_err);
+    for (hipEvent_t ev : order_ev)
+      if (ev)
+        (void)hipEventDestroy(ev);
     scal.release();
     if (host_scal)
       (void)hipHostFree(host_scal);
@@ -436,6 +453,16 @@ namespace
     return MPDG_OK;
   }
 
+  mpdg_status poll_order_state(mpdg_heat *H);
+
+  // check_ready plus the sticky ordered-schedule report of earlier calls
+  mpdg_status check_call(mpdg_heat *H)
+  {
+    if (mpdg_status s = check_ready(H))
+      return s;
+    return poll_order_state(H);
+  }
+
   // The tangent of radiating faces depends on the registered temperature.
   mpdg_status check_tangent(const mpdg_heat *H)
   {
@@ -452,8 +479,45 @@ namespace
   // After a synchronising call: a bounded wait of the ordered vmult schedule
   // that gave up (only a defect can cause it) leaves its error bit set, and the
   // results since then are not to be trusted.
+  mpdg_status order_failure()
+  {
+    return fail(MPDG_ERR_HIP, "ordered vmult schedule: a wait for a neighbouring work item "
+                              "timed out; results are unreliable");
+  }
+
+  // Non-blocking half of the report: a completed copy of the error bit that
+  // shows the bit set makes the handle's failure sticky; every later call on
+  // the handle returns it.
+  mpdg_status poll_order_state(mpdg_heat *H)
+  {
+    for (int k = 0; k < 2; ++k)
+      if (H->order_pending[k] && hipEventQuery(H->order_ev[k]) == hipSuccess)
+        {
+          H->order_pending[k] = false;
+          if (H->host_order_err[k])
+            H->order_failed = true;
+        }
+    return H->order_failed ? order_failure() : MPDG_OK;
+  }
+
+  // After an ordered launch of a non-synchronising call: queue the copy of the
+  // error bit that poll_order_state reads later.
+  mpdg_status queue_order_check(mpdg_heat *H, int k)
+  {
+    const bool active = k == MPDG_FP64 ? H->prm64.order != nullptr : H->prm32.order != nullptr;
+    if (!active)
+      return MPDG_OK;
+    MPDG_HIP(hipMemcpyAsync(H->host_order_err + k, &H->order[k].ptr->error, sizeof(unsigned),
+                            hipMemcpyDeviceToHost, H->stream));
+    MPDG_HIP(hipEventRecord(H->order_ev[k], H->stream));
+    H->order_pending[k] = true;
+    return MPDG_OK;
+  }
+
   mpdg_status check_order_state(mpdg_heat *H)
   {
+    if (H->order_failed)
+      return order_failure();
     for (int k = 0; k < 2; ++k)
       {
         const bool active = k == MPDG_FP64 ? H->prm64.order != nullptr : H->prm32.order != nullptr;
@@ -462,8 +526,10 @@ namespace
         unsigned err = 0;
         MPDG_HIP(hipMemcpy(&err, &H->order[k].ptr->error, sizeof(unsigned), hipMemcpyDeviceToHost));
         if (err)
-          return fail(MPDG_ERR_HIP, "ordered vmult schedule: a wait for a neighbouring work item "
-                                    "timed out; results are unreliable");
+          {
+            H->order_failed = true;
+            return order_failure();
+          }
       }
     return MPDG_OK;
   }
@@ -710,6 +776,10 @@ namespace
   {
     double *part = H->partials.ptr;
     SolverScalars *sc = H->scal.ptr;
+    // the residual history is the FP64 CG's (the inner solves of the mixed
+    // precision solve do not record)
+    double *hist = sizeof(Number) == 8 && H->history_len > 0 ? H->history : nullptr;
+    const int hist_len = hist ? H->history_len : 0;
     if (x_is_zero)
       {
         MPDG_HIP(launch_cg_init<Number>(b, nullptr, r, p, inv, H->N, rel_tol, abs_tol, max_it,
@@ -727,6 +797,9 @@ namespace
         MPDG_HIP(launch_cg_init<Number>(b, q, r, p, inv, H->N, rel_tol, abs_tol, max_it, part, sc,
                                         H->stream));
       }
+    if (hist)
+      MPDG_HIP(hipMemcpyAsync(hist, &sc->res0, sizeof(double), hipMemcpyDeviceToDevice,
+                              H->stream));
     auto chunk = [&]() -> mpdg_status {
       for (int k = 0; k < check; ++k)
         {
@@ -737,14 +810,13 @@ namespace
             st = vmult32(H, q, p, part, sc);
           if (st)
             return st;
-          MPDG_HIP(launch_cg_update<Number>(x, r, p, q, inv, H->N, part, sc, H->history.ptr,
-                                            (int)H->history.n, H->stream));
+          MPDG_HIP(launch_cg_update<Number>(x, r, p, q, inv, H->N, part, sc, hist, hist_len,
+                                            H->stream));
           MPDG_HIP(launch_cg_direction<Number>(p, r, inv, H->N, sc, H->stream));
         }
       return MPDG_OK;
     };
-    const void *key[] = {x, r, p, q, inv, part, sc, H->history.ptr,
-                         (const void *)(uintptr_t)H->history.n};
+    const void *key[] = {x, r, p, q, inv, part, sc, hist, (const void *)(uintptr_t)hist_len};
     for (;;)
       {
         if (mpdg_status st = run_chunk(H, H->pcg_graph[sizeof(Number) == 4], check, key,
@@ -949,8 +1021,16 @@ mpdg_status mpdg_heat_create(const mpdg_heat_desc *desc, mpdg_heat **out)
       const int64_t drop = ((d.dirichlet_faces >> (2 * k)) & 1u) + ((d.dirichlet_faces >> (2 * k + 1)) & 1u);
       Nf *= std::max<int64_t>(0, n1 - drop);
     }
-  if (N >= ((int64_t)1 << 40))
-    return fail(MPDG_ERR_ARGUMENT, "mesh too large");
+  // The cell kernels address each vector through a raw buffer resource whose
+  // byte size and offsets are 32-bit, and the cell form marks a skipped node by
+  // bit 31 of its byte offset: an FP64 vector must stay below 2^31 bytes.
+  if (N * (int64_t)sizeof(double) >= kMaxVectorBytes)
+    return fail(MPDG_ERR_UNSUPPORTED,
+                "mesh too large: n_dofs() = " + std::to_string(N) + " needs " +
+                  std::to_string(N * (int64_t)sizeof(double)) +
+                  " bytes per FP64 vector; the cell kernels address vectors below 2^31 bytes "
+                  "(at most " + std::to_string(kMaxVectorBytes / (int64_t)sizeof(double) - 1) +
+                  " DoFs)");
   if (d.precision != MPDG_FP64 && d.precision != MPDG_FP32)
     return fail(MPDG_ERR_ARGUMENT, "unknown precision");
   if (d.material != MPDG_MATERIAL_GAS && d.material != MPDG_MATERIAL_GAS_LIQUID &&
@@ -1068,6 +1148,12 @@ mpdg_status mpdg_heat_create(const mpdg_heat_desc *desc, mpdg_heat **out)
     e = hipEventCreate(&H->ev0);
   if (e == hipSuccess)
     e = hipEventCreate(&H->ev1);
+  for (int k = 0; k < 2 && e == hipSuccess; ++k)
+    e = hipEventCreateWithFlags(&H->order_ev[k], hipEventDisableTiming);
+  if (e == hipSuccess)
+    e = hipHostMalloc((void **)&H->host_order_err, 2 * sizeof(unsigned), hipHostMallocDefault);
+  if (e == hipSuccess)
+    H->host_order_err[0] = H->host_order_err[1] = 0u;
   // ordered vmult schedule: counters and epoch at 0, one flag per possible work
   // item (an item holds at least one cell)
   for (int k = 0; k < 2 && e == hipSuccess; ++k)
@@ -1241,6 +1327,15 @@ mpdg_status mpdg_heat_set_graphs(mpdg_heat *h, int enable)
   return MPDG_OK;
 }
 
+mpdg_status mpdg_heat_set_residual_history(mpdg_heat *h, double *history, int length)
+{
+  if (!h || length < 0 || (length > 0 && !history))
+    return fail(MPDG_ERR_ARGUMENT, "history needs a device buffer of length >= 0 entries");
+  h->history = length > 0 ? history : nullptr;
+  h->history_len = length;
+  return MPDG_OK;
+}
+
 mpdg_status mpdg_heat_set_temperature_old(mpdg_heat *h, const double *temperature_old)
 {
   if (!h)
@@ -1257,7 +1352,11 @@ mpdg_status mpdg_heat_vmult(mpdg_heat *h, double *dst, const double *src)
     return fail(MPDG_ERR_ARGUMENT, "null vector");
   if (dst == src)
     return fail(MPDG_ERR_ARGUMENT, "vmult cannot run in place");
-  return vmult64(h, dst, src, nullptr, nullptr);
+  if (mpdg_status s = poll_order_state(h))
+    return s;
+  if (mpdg_status s = vmult64(h, dst, src, nullptr, nullptr))
+    return s;
+  return queue_order_check(h, MPDG_FP64);
 }
 
 mpdg_status mpdg_heat_vmult_f32(mpdg_heat *h, float *dst, const float *src)
@@ -1268,12 +1367,16 @@ mpdg_status mpdg_heat_vmult_f32(mpdg_heat *h, float *dst, const float *src)
     return fail(MPDG_ERR_ARGUMENT, "null vector");
   if ((const void *)dst == (const void *)src)
     return fail(MPDG_ERR_ARGUMENT, "vmult cannot run in place");
-  return vmult32(h, dst, src, nullptr, nullptr);
+  if (mpdg_status s = poll_order_state(h))
+    return s;
+  if (mpdg_status s = vmult32(h, dst, src, nullptr, nullptr))
+    return s;
+  return queue_order_check(h, MPDG_FP32);
 }
 
 mpdg_status mpdg_heat_inverse_diagonal(mpdg_heat *h, double *inv_diag)
 {
-  if (mpdg_status s = check_ready(h))
+  if (mpdg_status s = check_call(h))
     return s;
   if (!inv_diag)
     return fail(MPDG_ERR_ARGUMENT, "null vector");
@@ -1283,7 +1386,7 @@ mpdg_status mpdg_heat_inverse_diagonal(mpdg_heat *h, double *inv_diag)
 mpdg_status mpdg_heat_create_rhs(mpdg_heat *h, double *dst, const double *temperature,
                                  const double *temperature_old, const double *heat_source)
 {
-  if (mpdg_status s = check_ready(h))
+  if (mpdg_status s = check_call(h))
     return s;
   if (!dst || !temperature || !temperature_old)
     return fail(MPDG_ERR_ARGUMENT, "null vector");
@@ -1354,18 +1457,19 @@ mpdg_status mpdg_heat_linear_predictor(mpdg_heat *h, double *predictor, const do
 // inverse diagonal is used as it is.
 static mpdg_status solve_impl(mpdg_heat *h, double *x, const double *b,
                               const mpdg_solver_params *params, mpdg_solver_report *report,
-                              bool update_preconditioner);
+                              bool update_preconditioner, const double *user_inv = nullptr,
+                              bool user_identity = false);
 static mpdg_status solve_on_stream(mpdg_heat *h, double *x, const double *b,
                                    const mpdg_solver_params &prm, mpdg_solver_report *report,
                                    bool update_preconditioner, bool mixed, bool use_gmres,
-                                   int check);
+                                   int check, const double *user_inv, bool user_identity);
 
 mpdg_status mpdg_heat_newton_solve(mpdg_heat *h, double *temperature,
                                    const double *temperature_old, const double *heat_source,
                                    const mpdg_newton_params *newton,
                                    const mpdg_solver_params *linear, mpdg_newton_report *report)
 {
-  if (mpdg_status s = check_ready(h))
+  if (mpdg_status s = check_call(h))
     return s;
   if (!temperature || !temperature_old)
     return fail(MPDG_ERR_ARGUMENT, "null vector");
@@ -1449,7 +1553,8 @@ mpdg_status mpdg_heat_newton_solve(mpdg_heat *h, double *temperature,
           rep.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
           if (report)
             *report = rep;
-          return MPDG_OK;
+          // a timed-out ordered wait makes the step's temperatures unreliable
+          return check_order_state(h);
         }
       // solution += solution_update (homogeneous constraints: dT is 0 there)
       MPDG_HIP(launch_axpby(temperature, 1.0, temperature, 1.0, h->ndu.ptr, h->N, h->stream));
@@ -1459,6 +1564,8 @@ mpdg_status mpdg_heat_newton_solve(mpdg_heat *h, double *temperature,
   rep.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   if (report)
     *report = rep;
+  if (mpdg_status s = check_order_state(h))
+    return s;
   return fail(MPDG_ERR_NOT_CONVERGED, "Newton-Raphson solver did not converge");
 }
 
@@ -1481,11 +1588,28 @@ mpdg_status mpdg_heat_solve(mpdg_heat *h, double *x, const double *b,
   return s;
 }
 
+mpdg_status mpdg_heat_solve_preconditioned(mpdg_heat *h, double *x, const double *b,
+                                           const double *inv_diag,
+                                           const mpdg_solver_params *params,
+                                           mpdg_solver_report *report)
+{
+  if (!h)
+    return fail(MPDG_ERR_ARGUMENT, "null handle");
+  if (inv_diag && (inv_diag == x || inv_diag == b))
+    return fail(MPDG_ERR_ARGUMENT, "the inverse diagonal must not alias x or b");
+  const mpdg_status s = solve_impl(h, x, b, params, report, false, inv_diag, inv_diag == nullptr);
+  if (s == MPDG_OK || s == MPDG_ERR_NOT_CONVERGED)
+    if (mpdg_status e = check_order_state(h))
+      return e;
+  return s;
+}
+
 static mpdg_status solve_impl(mpdg_heat *h, double *x, const double *b,
                               const mpdg_solver_params *params, mpdg_solver_report *report,
-                              bool update_preconditioner)
+                              bool update_preconditioner, const double *user_inv,
+                              bool user_identity)
 {
-  if (mpdg_status s = check_ready(h))
+  if (mpdg_status s = check_call(h))
     return s;
   if (!x || !b)
     return fail(MPDG_ERR_ARGUMENT, "null vector");
@@ -1529,7 +1653,7 @@ static mpdg_status solve_impl(mpdg_heat *h, double *x, const double *b,
       h->stream = h->gstream;
     }
   const mpdg_status st = solve_on_stream(h, x, b, prm, report, update_preconditioner, mixed,
-                                         use_gmres, check);
+                                         use_gmres, check, user_inv, user_identity);
   if (h->stream != user)
     {
       h->stream = user;
@@ -1542,13 +1666,20 @@ static mpdg_status solve_impl(mpdg_heat *h, double *x, const double *b,
 static mpdg_status solve_on_stream(mpdg_heat *h, double *x, const double *b,
                                    const mpdg_solver_params &prm, mpdg_solver_report *report,
                                    bool update_preconditioner, bool mixed, bool use_gmres,
-                                   int check)
+                                   int check, const double *user_inv, bool user_identity)
 {
   mpdg_solver_report rep{};
   const auto t0 = std::chrono::steady_clock::now();
 
+  // The inverse diagonal the Krylov loop applies: the caller's (a
+  // JacobiPreconditioner updated by the caller) or the handle's own.
+  const double *inv = user_inv ? user_inv : h->inv_diag.ptr;
   // JacobiPreconditioner::update -> compute_inverse_diagonal_from_matrixfree
-  if (!update_preconditioner)
+  if (user_inv)
+    ;
+  else if (user_identity)
+    MPDG_HIP(launch_fill(h->inv_diag.ptr, 1.0, h->N, h->stream));
+  else if (!update_preconditioner)
     ;
   else if (prm.preconditioner_type == MPDG_PRECONDITIONER_DIAGONAL)
     {
@@ -1565,7 +1696,7 @@ static mpdg_status solve_on_stream(mpdg_heat *h, double *x, const double *b,
       double r0 = 0.0, rf = 0.0;
       bool conv = false;
       if (mpdg_status s = gmres<double>(h, x, b, h->r.ptr, h->p.ptr, h->q.ptr, h->V64.ptr,
-                                        h->inv_diag.ptr, prm.rel_tolerance, prm.abs_tolerance,
+                                        inv, prm.rel_tolerance, prm.abs_tolerance,
                                         prm.max_iterations, prm.gmres_restart, false, &its, &r0,
                                         &rf, &conv))
         return s;
@@ -1577,7 +1708,7 @@ static mpdg_status solve_on_stream(mpdg_heat *h, double *x, const double *b,
     }
   else if (!mixed)
     {
-      if (mpdg_status s = pcg<double>(h, x, b, h->r.ptr, h->p.ptr, h->q.ptr, h->inv_diag.ptr,
+      if (mpdg_status s = pcg<double>(h, x, b, h->r.ptr, h->p.ptr, h->q.ptr, inv,
                                       prm.rel_tolerance, prm.abs_tolerance, prm.max_iterations,
                                       check, false))
         return s;
@@ -1592,7 +1723,7 @@ static mpdg_status solve_on_stream(mpdg_heat *h, double *x, const double *b,
     {
       // Defect correction: r = b - A64 x in FP64, inner FP32 PCG for A d = r,
       // x += d in FP64, until ||r|| <= max(abs, rel ||r0||).
-      MPDG_HIP(launch_to_f32(h->inv32.ptr, h->inv_diag.ptr, h->N, h->stream));
+      MPDG_HIP(launch_to_f32(h->inv32.ptr, inv, h->N, h->stream));
       double tol = 0.0, res0 = 0.0;
       int inner_total = 0, outer = 0;
       for (;;)
@@ -1685,7 +1816,7 @@ mpdg_status mpdg_heat_set_tuning(mpdg_heat *h, int precision, const mpdg_tuning 
 
 mpdg_status mpdg_heat_autotune(mpdg_heat *h, int precision, int repeats, mpdg_tuning *best)
 {
-  if (mpdg_status s = check_ready(h))
+  if (mpdg_status s = check_call(h))
     return s;
   if (precision != MPDG_FP64 && precision != MPDG_FP32)
     return fail(MPDG_ERR_ARGUMENT, "precision must be MPDG_FP64 or MPDG_FP32");
@@ -1761,7 +1892,7 @@ mpdg_status mpdg_heat_autotune(mpdg_heat *h, int precision, int repeats, mpdg_tu
 mpdg_status mpdg_heat_time_vmult(mpdg_heat *h, double *dst, const double *src, int repeats,
                                  int precision, double *ms_per_vmult)
 {
-  if (mpdg_status s = check_ready(h))
+  if (mpdg_status s = check_call(h))
     return s;
   if (!dst || !src || repeats < 1 || !ms_per_vmult)
     return fail(MPDG_ERR_ARGUMENT, "invalid timing arguments");
@@ -2049,6 +2180,18 @@ mpdg_status mpdg_heat_batch_create(const mpdg_heat_desc *descs, int count, mpdg_
   for (int s = 0; s < count; ++s)
     if ((descs[s].convection_faces | descs[s].radiation_faces) & 0x3Fu)
       return fail(MPDG_ERR_UNSUPPORTED, "batches of operators with Robin faces are not built");
+  // each member's own limits are checked by mpdg_heat_create; the vector size
+  // limit is checked here as well, before the batch creates its stream
+  for (int s = 0; s < count; ++s)
+    {
+      int64_t n = 1;
+      for (int k = 0; k < 3; ++k)
+        n *= std::max<int64_t>(1, (int64_t)descs[s].n_cells[k] * descs[s].degree + 1);
+      if (n * (int64_t)sizeof(double) >= kMaxVectorBytes)
+        return fail(MPDG_ERR_UNSUPPORTED,
+                    "batch member " + std::to_string(s) + ": mesh too large (n_dofs() = " +
+                      std::to_string(n) + "); the cell kernels address vectors below 2^31 bytes");
+    }
   mpdg_heat_batch *B = new (std::nothrow) mpdg_heat_batch();
   if (!B)
     return fail(MPDG_ERR_ARGUMENT, "out of host memory");

@@ -477,7 +477,7 @@ namespace mpdg
       __shared__ Number FZ[2][CPB * NL]; // z-face carry [cell][b][a], by layer parity
       __shared__ Number XE[2][CPB * NL]; // right x-face of each cell line [cell][line]
 
-      const uint32_t vbytes = (uint32_t)prm.n1x * prm.n1y * prm.n1z * sizeof(Number);
+      const uint32_t vbytes = vector_bytes(prm.n1x, prm.n1y, prm.n1z, (int)sizeof(Number));
       const __amdgpu_buffer_rsrc_t rsrc = make_rsrc(src, vbytes);
       const __amdgpu_buffer_rsrc_t rdst = make_rsrc(dst, vbytes);
       const __amdgpu_buffer_rsrc_t rheav = make_rsrc(H2 ? heav : src, vbytes);
@@ -1343,7 +1343,10 @@ namespace mpdg
                 nb = 1;
               oresident = nb < kMaxResidentPerCu ? nb : kMaxResidentPerCu;
             }
-          const int opc = prm.blocks_per_cu > 0 ? prm.blocks_per_cu : oresident;
+          // every workgroup of an ordered launch must be resident at once: a
+          // tuned count above the kernel's own occupancy is capped, so the
+          // waits' forward progress does not rest on the dispatch order
+          const int opc = prm.blocks_per_cu > 0 ? std::min(prm.blocks_per_cu, oresident) : oresident;
           const VmultGrid og8 = vmult_grid(prm.nx, prm.ny, prm.nz, OShape::CW * OShape::WPB,
                                            prm.zlayers, opc * device_cu_count());
           OrderGrid og{};

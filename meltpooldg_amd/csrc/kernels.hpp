@@ -9,6 +9,19 @@
 
 namespace mpdg
 {
+  // Largest vector the cell kernels address (bytes, exclusive): their raw
+  // buffer resources carry 32-bit sizes and offsets, and the cell form marks a
+  // skipped node by bit 31 of its byte offset. mpdg_heat_create rejects meshes
+  // whose FP64 vectors reach it.
+  constexpr int64_t kMaxVectorBytes = (int64_t)1 << 31;
+
+  // Byte size of a lattice vector for a raw buffer resource (the host has
+  // checked that it is below kMaxVectorBytes, so the 32-bit result is exact).
+  __host__ __device__ inline uint32_t vector_bytes(int n1x, int n1y, int n1z, int size)
+  {
+    return (uint32_t)((int64_t)n1x * n1y * n1z * size);
+  }
+
   // Per-handle operator constants passed by value to the cell kernels.
   template <typename Number>
   struct CellParams

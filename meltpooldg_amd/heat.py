@@ -293,6 +293,26 @@ class HeatDiffuseOperator:
         captured hipGraphs (default) or launch every kernel eagerly."""
         L.check(self._lib.mpdg_heat_set_graphs(self._h, 1 if enable else 0))
 
+    def enable_history_data(self, length=10001):
+        """SolverControl::enable_history_data: the FP64 CG solves record
+        ||r_k|| for k = 0 .. iterations into a device buffer of `length`
+        entries (get_history_data reads it)."""
+        import torch
+        self._history = torch.zeros(int(length), dtype=torch.float64, device="cuda")
+        L.check(self._lib.mpdg_heat_set_residual_history(self._h, _ptr(self._history),
+                                                         int(length)))
+
+    def get_history_data(self, iterations=None):
+        """The recorded residual norms of the last FP64 CG solve (host numpy),
+        entries 0 .. iterations (default: up to the last nonzero)."""
+        import torch
+        torch.cuda.synchronize()
+        h = self._history.cpu().numpy()
+        if iterations is None:
+            nz = h.nonzero()[0]
+            iterations = int(nz[-1]) if nz.size else 0
+        return h[: iterations + 1].copy()
+
     def set_heaviside(self, heaviside):
         """Register the level-set heaviside field (read plain, interpolated at q)."""
         self._check_vec(heaviside, "float64")
@@ -386,18 +406,20 @@ class HeatDiffuseOperator:
 
 
 class JacobiPreconditioner:
-    """preconditioner_jacobi.hpp:25-89: update() -> inverse diagonal, vmult(dst, src)."""
+    """preconditioner_jacobi.hpp:25-89: update() -> inverse diagonal, vmult(dst, src).
+    The inverse diagonal starts as a zero vector, as reinit()'s
+    initialize_dof_vector leaves it; update() computes it."""
 
     def __init__(self, operator):
         import torch
         self.eq_operator = operator
-        self.inverse_diag = torch.empty(operator.n_dofs(), dtype=torch.float64, device="cuda")
+        self.inverse_diag = torch.zeros(operator.n_dofs(), dtype=torch.float64, device="cuda")
 
     def update(self, *_):
         self.eq_operator.compute_inverse_diagonal_from_matrixfree(self.inverse_diag)
 
     def reinit(self):
-        pass
+        self.inverse_diag.zero_()
 
     def vmult(self, dst, src):
         op = self.eq_operator
@@ -433,20 +455,40 @@ class LinearSolver:
     def solve(system_matrix, solution, rhs, data, preconditioner=None, identifier=""):
         """Solve system_matrix * solution = rhs; returns the iteration count.
 
-        ``preconditioner`` is a JacobiPreconditioner (or None / 'Identity').
-        The engine runs the whole Krylov loop on the device; with a Jacobi
-        preconditioner it recomputes the inverse diagonal first, as
-        HeatDiffuseOperation::solve does before every solve
+        ``preconditioner`` decides when it is given, as the object passed to the
+        reference's LinearSolver::solve does:
+
+        * a JacobiPreconditioner is applied with its inverse diagonal as the
+          caller last computed it (``update()``); it is not recomputed here
+          (mpdg_heat_solve_preconditioned);
+        * the string ``"Identity"`` is PreconditionIdentity.
+
+        Without one, ``data.preconditioner_type`` decides ("Identity" or
+        "Diagonal"; others are not built and raise): "Diagonal" recomputes the
+        inverse diagonal on the device first, as HeatDiffuseOperation::solve
+        updates its preconditioner before every solve
         (source/heat/heat_diffuse_operation.cpp:275-276).
         """
         op = system_matrix
-        use_diag = isinstance(preconditioner, JacobiPreconditioner) or data.preconditioner_type == "Diagonal"
-        prm = _solver_params(op._lib, data, use_diag)
         op._check_vec(solution, "float64")
         op._check_vec(rhs, "float64")
         rep = L.SolverReport()
-        st = op._lib.mpdg_heat_solve(op._h, _ptr(solution), _ptr(rhs), ctypes.byref(prm),
-                                     ctypes.byref(rep))
+        if preconditioner is None:
+            prm = _solver_params(op._lib, data)
+            st = op._lib.mpdg_heat_solve(op._h, _ptr(solution), _ptr(rhs), ctypes.byref(prm),
+                                         ctypes.byref(rep))
+        elif isinstance(preconditioner, JacobiPreconditioner) or preconditioner == "Identity":
+            prm = _solver_params(op._lib, data, use_diag=preconditioner != "Identity")
+            inv = None
+            if isinstance(preconditioner, JacobiPreconditioner):
+                if preconditioner.eq_operator is not op:
+                    raise ValueError("the preconditioner belongs to another operator")
+                inv = _ptr(preconditioner.inverse_diag)
+            st = op._lib.mpdg_heat_solve_preconditioned(op._h, _ptr(solution), _ptr(rhs), inv,
+                                                        ctypes.byref(prm), ctypes.byref(rep))
+        else:
+            raise ValueError(f"preconditioner {preconditioner!r} is not built "
+                             "(a JacobiPreconditioner, 'Identity' or None)")
         LinearSolver.last_report = {"iterations": rep.iterations,
                                     "outer_iterations": rep.outer_iterations,
                                     "initial_residual": rep.initial_residual,

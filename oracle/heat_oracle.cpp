@@ -155,6 +155,10 @@ namespace
     int64_t N;
     double h[3];
     double lo[3];
+    // sin(2 pi x) and sin(pi x) at the Gauss coordinates of each direction,
+    // index c (p + 1) + q: the same std::sin of the same argument the
+    // quadrature-point physics evaluates, computed once per coordinate
+    std::vector<double> sin2pi[3], sinpi[3];
   };
 
   Mesh make_mesh(const oracle_problem *pb)
@@ -169,6 +173,23 @@ namespace
         m.N *= m.N1[d];
         m.h[d] = (pb->hi[d] - pb->lo[d]) / pb->n[d];
         m.lo[d] = pb->lo[d];
+      }
+    if (pb->degree >= 1 && pb->degree <= ORACLE_MAX_DEGREE)
+      {
+        const Tables t = make_tables(pb->degree);
+        const double pi = (double)PI_L, two_pi = 2.0 * (double)PI_L;
+        for (int d = 0; d < 3; ++d)
+          {
+            m.sin2pi[d].resize((size_t)std::max(m.n[d], 0) * t.n);
+            m.sinpi[d].resize(m.sin2pi[d].size());
+            for (int c = 0; c < m.n[d]; ++c)
+              for (int q = 0; q < t.n; ++q)
+                {
+                  const double xq = m.lo[d] + (c + t.X[q]) * m.h[d];
+                  m.sin2pi[d][(size_t)c * t.n + q] = std::sin(two_pi * xq);
+                  m.sinpi[d][(size_t)c * t.n + q] = std::sin(pi * xq);
+                }
+          }
       }
     return m;
   }
@@ -420,7 +441,10 @@ namespace
   // Prescribed velocity at a quadrature point: 1 = the vortex field (analytic at
   // q), 2 = a constant field (what VectorTools::interpolate of a constant
   // prescribed_velocity function gives at any point).
-  inline void velocity_at(const oracle_problem *pb, double xq, double yq, double v[3])
+  // (the sines at the Gauss coordinate c (p + 1) + q of each direction come
+  // from the mesh's tables: ix, iy index them)
+  inline void velocity_at(const oracle_problem *pb, const Mesh &m, size_t ix, size_t iy,
+                          double v[3])
   {
     if (pb->velocity == 2)
       {
@@ -429,18 +453,62 @@ namespace
         v[2] = pb->velocity_value[2];
         return;
       }
-    const double pi = (double)PI_L, two_pi = 2.0 * (double)PI_L;
-    const double sx = std::sin(pi * xq), sy = std::sin(pi * yq);
-    v[0] = -sx * sx * std::sin(two_pi * yq);
-    v[1] = std::sin(two_pi * xq) * sy * sy;
+    const double sx = m.sinpi[0][ix], sy = m.sinpi[1][iy];
+    v[0] = -sx * sx * m.sin2pi[1][iy];
+    v[1] = m.sin2pi[0][ix] * sy * sy;
     v[2] = pb->w0;
   }
 
   // 1D contraction along one tensor direction of a local n^3 array
   // (index (c*n+b)*n+a, a = x fastest). out = M in along `dir`, or M^T.
+  template <bool transpose, int n, int dir>
+  void apply_dir_n(const double (*M)[MAXN], const double *in, double *out)
+  {
+    constexpr int stride = dir == 0 ? 1 : (dir == 1 ? n : n * n);
+    for (int c = 0; c < n; ++c)
+      for (int b = 0; b < n; ++b)
+        for (int a = 0; a < n; ++a)
+          {
+            const int idx[3] = {a, b, c};
+            const int o = idx[dir];
+            const int base = (c * n + b) * n + a - o * stride;
+            double s = 0.0;
+            for (int r = 0; r < n; ++r)
+              s += (transpose ? M[r][o] : M[o][r]) * in[base + r * stride];
+            out[(c * n + b) * n + a] = s;
+          }
+  }
+
+  // the same loops with n and the direction compile-time constants (the
+  // compiler unrolls them; each sum runs over r in the same order)
   template <bool transpose>
   void apply_dir(const double (*M)[MAXN], int n, int dir, const double *in, double *out)
   {
+    switch (n)
+      {
+        case 2: return dir == 0 ? apply_dir_n<transpose, 2, 0>(M, in, out) :
+                       dir == 1 ? apply_dir_n<transpose, 2, 1>(M, in, out) :
+                                  apply_dir_n<transpose, 2, 2>(M, in, out);
+        case 3: return dir == 0 ? apply_dir_n<transpose, 3, 0>(M, in, out) :
+                       dir == 1 ? apply_dir_n<transpose, 3, 1>(M, in, out) :
+                                  apply_dir_n<transpose, 3, 2>(M, in, out);
+        case 4: return dir == 0 ? apply_dir_n<transpose, 4, 0>(M, in, out) :
+                       dir == 1 ? apply_dir_n<transpose, 4, 1>(M, in, out) :
+                                  apply_dir_n<transpose, 4, 2>(M, in, out);
+        case 5: return dir == 0 ? apply_dir_n<transpose, 5, 0>(M, in, out) :
+                       dir == 1 ? apply_dir_n<transpose, 5, 1>(M, in, out) :
+                                  apply_dir_n<transpose, 5, 2>(M, in, out);
+        case 6: return dir == 0 ? apply_dir_n<transpose, 6, 0>(M, in, out) :
+                       dir == 1 ? apply_dir_n<transpose, 6, 1>(M, in, out) :
+                                  apply_dir_n<transpose, 6, 2>(M, in, out);
+        case 7: return dir == 0 ? apply_dir_n<transpose, 7, 0>(M, in, out) :
+                       dir == 1 ? apply_dir_n<transpose, 7, 1>(M, in, out) :
+                                  apply_dir_n<transpose, 7, 2>(M, in, out);
+        case 8: return dir == 0 ? apply_dir_n<transpose, 8, 0>(M, in, out) :
+                       dir == 1 ? apply_dir_n<transpose, 8, 1>(M, in, out) :
+                                  apply_dir_n<transpose, 8, 2>(M, in, out);
+        default: break;
+      }
     const int stride = dir == 0 ? 1 : (dir == 1 ? n : n * n);
     for (int c = 0; c < n; ++c)
       for (int b = 0; b < n; ++b)
@@ -528,7 +596,6 @@ namespace
     const double hx = m.h[0], hy = m.h[1], hz = m.h[2];
     const double jac_det = hx * hy * hz;
     const double dt_inv = 1. / pb->dt;
-    const double two_pi = 2.0 * (double)PI_L;
 
     double V[MAXN * MAXN * MAXN], Fx[MAXN * MAXN * MAXN], Fy[MAXN * MAXN * MAXN],
       Fz[MAXN * MAXN * MAXN];
@@ -537,9 +604,6 @@ namespace
         for (int qx = 0; qx < n; ++qx)
           {
             const int q = (qz * n + qy) * n + qx;
-            const double xq = m.lo[0] + (g.c[0] + t.X[qx]) * hx;
-            const double yq = m.lo[1] + (g.c[1] + t.X[qy]) * hy;
-            const double zq = m.lo[2] + (g.c[2] + t.X[qz]) * hz;
             const double JxW = t.W[qx] * t.W[qy] * t.W[qz] * jac_det;
 
             const MatState ms = material_state(pb, uses_heaviside(pb) ? Hq[q] : 0.0,
@@ -547,8 +611,8 @@ namespace
             const double rho_cp = ms.rho_cp;
             double k = ms.k;
             if (pb->conductivity_field == 1)
-              k += pb->k_amplitude * std::sin(two_pi * xq) * std::sin(two_pi * yq) *
-                   std::sin(two_pi * zq);
+              k += pb->k_amplitude * m.sin2pi[0][(size_t)g.c[0] * n + qx] *
+                   m.sin2pi[1][(size_t)g.c[1] * n + qy] * m.sin2pi[2][(size_t)g.c[2] * n + qz];
 
             const double gx = Gx[q] / hx, gy = Gy[q] / hy, gz = Gz[q] / hz;
             double val = dt_inv * rho_cp * U[q];
@@ -556,7 +620,7 @@ namespace
             double v[3] = {0.0, 0.0, 0.0};
             if (pb->velocity != 0)
               {
-                velocity_at(pb, xq, yq, v);
+                velocity_at(pb, m, (size_t)g.c[0] * n + qx, (size_t)g.c[1] * n + qy, v);
                 val += rho_cp * (gx * v[0] + gy * v[1] + gz * v[2]);
               }
             if (td)
@@ -732,7 +796,6 @@ namespace
     const double hx = m.h[0], hy = m.h[1], hz = m.h[2];
     const double jac_det = hx * hy * hz;
     const double dt_inv = 1. / pb->dt;
-    const double two_pi = 2.0 * (double)PI_L;
 
     double V[MAXN * MAXN * MAXN], Fx[MAXN * MAXN * MAXN], Fy[MAXN * MAXN * MAXN],
       Fz[MAXN * MAXN * MAXN];
@@ -741,9 +804,6 @@ namespace
         for (int qx = 0; qx < n; ++qx)
           {
             const int q = (qz * n + qy) * n + qx;
-            const double xq = m.lo[0] + (g.c[0] + t.X[qx]) * hx;
-            const double yq = m.lo[1] + (g.c[1] + t.X[qy]) * hy;
-            const double zq = m.lo[2] + (g.c[2] + t.X[qz]) * hz;
             const double JxW = t.W[qx] * t.W[qy] * t.W[qz] * jac_det;
 
             // get_material_parameters (heat_diffuse_operator.cpp:1133-1165) at T_new(q)
@@ -751,15 +811,15 @@ namespace
             const double rho_cp = ms.rho_cp;
             double k = ms.k;
             if (pb->conductivity_field == 1)
-              k += pb->k_amplitude * std::sin(two_pi * xq) * std::sin(two_pi * yq) *
-                   std::sin(two_pi * zq);
+              k += pb->k_amplitude * m.sin2pi[0][(size_t)g.c[0] * n + qx] *
+                   m.sin2pi[1][(size_t)g.c[1] * n + qy] * m.sin2pi[2][(size_t)g.c[2] * n + qz];
 
             const double gx = Gx[q] / hx, gy = Gy[q] / hy, gz = Gz[q] / hz;
             double val = dt_inv * rho_cp * (U[q] - Uo[q]) - Fq[q];
             if (pb->velocity != 0)
               {
                 double v[3];
-                velocity_at(pb, xq, yq, v);
+                velocity_at(pb, m, (size_t)g.c[0] * n + qx, (size_t)g.c[1] * n + qy, v);
                 val += rho_cp * (gx * v[0] + gy * v[1] + gz * v[2]);
               }
             // submit_value(-1 val), submit_gradient(-1 val_grad)

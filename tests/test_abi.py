@@ -68,6 +68,40 @@ def test_argument_errors_without_gpu(lib):
     assert lib.mpdg_heat_n_dofs(None) == -1
 
 
+def test_vector_size_limit_is_enforced(lib):
+    """The cell kernels address each vector through a raw buffer with 32-bit
+    byte sizes and offsets: a mesh whose FP64 vector reaches 2^31 bytes is
+    refused before anything is allocated (no GPU needed), also for an FP32 cell
+    loop (its FP64 outer vectors go through the same kernels), and also as a
+    batch member."""
+    from meltpooldg_amd import _lib as L
+    h = ctypes.c_void_p()
+    d = L.HeatDesc()
+    d.box_hi[:] = [1.0, 1.0, 1.0]
+    d.gas.density = d.gas.specific_heat_capacity = d.gas.thermal_conductivity = 1.0
+    d.dirichlet_faces = 0x3F
+    # 128^3 Q6: 769^3 = 454,756,609 DoFs, 3.6 GB per FP64 vector
+    d.n_cells[:] = [128, 128, 128]
+    d.degree = 6
+    for prec in (L.MPDG_FP64, L.MPDG_FP32):
+        d.precision = prec
+        assert lib.mpdg_heat_create(ctypes.byref(d), ctypes.byref(h)) == L.MPDG_ERR_UNSUPPORTED
+        msg = lib.mpdg_last_error().decode()
+        assert "2^31 bytes" in msg and "454756609" in msg, msg
+        assert not h.value
+    # the first size past the limit: N1 = 646 gives 269,586,136 DoFs (> 2^28)
+    d.precision = L.MPDG_FP64
+    d.n_cells[:] = [645, 645, 645]
+    d.degree = 1
+    assert 646 ** 3 * 8 >= 2 ** 31 > 645 ** 3 * 8
+    assert lib.mpdg_heat_create(ctypes.byref(d), ctypes.byref(h)) == L.MPDG_ERR_UNSUPPORTED
+    descs = (L.HeatDesc * 2)(d, d)
+    descs[0].n_cells[:] = [4, 4, 4]
+    b = ctypes.c_void_p()
+    assert lib.mpdg_heat_batch_create(descs, 2, ctypes.byref(b)) != L.MPDG_OK
+    assert not b.value
+
+
 def test_solver_defaults_follow_linear_solver_data(lib):
     from meltpooldg_amd import _lib as L
     p = L.SolverParams()

@@ -150,7 +150,9 @@ def test_solidification_solves_match_oracle(solver):
     assert ref["status"] == 0
     x = torch.zeros(pb.N, dtype=torch.float64, device="cuda")
     data = LinearSolverData(solver_type=solver, preconditioner_type="Diagonal", rel_tolerance=1e-10)
-    its = LinearSolver.solve(op, x, _t(b), data, JacobiPreconditioner(op))
+    prec = JacobiPreconditioner(op)
+    prec.update()
+    its = LinearSolver.solve(op, x, _t(b), data, prec)
     assert its == ref["iterations"], (solver, its, ref["iterations"])
     assert rel_err(_np(x), ref["x"]) < 1e-8
 
