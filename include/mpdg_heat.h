@@ -344,6 +344,20 @@ mpdg_status mpdg_heat_distribute_constraints(mpdg_heat *h, double *v);
  * the FE field v with the operator's Gauss quadrature, entries read plain. */
 mpdg_status mpdg_heat_l2_norm(mpdg_heat *h, const double *v, double *norm);
 
+/* Allocation-free variant of mpdg_heat_l2_norm. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops). It calls no hipMalloc / hipFree and
+ * no allocating library routine; the reduction scratch (one partial per
+ * workgroup and a ticket) is a static device array of 4 MiB per GPU, held
+ * until the process ends. The result is written to `norm`, a caller's DEVICE
+ * double, asynchronously on the handle's stream (no synchronisation); it is
+ * bitwise the value mpdg_heat_l2_norm returns. Meshes needing more than 2^19
+ * reduction workgroups (row segments x rows x layers) are refused with
+ * MPDG_ERR_UNSUPPORTED. All calls on one GPU share the scratch: calls that
+ * overlap on one GPU (other streams, other threads) race. The handle's device
+ * must be current. */
+mpdg_status mpdg_heat_l2_norm_noalloc(mpdg_heat *h, const double *v, double *norm);
+
 /* Krylov iteration chunks as hipGraphs (no counterpart in the reference, whose
  * solvers run on the host): with enable != 0 (the default) a solve runs on a
  * capture stream of the handle's own, ordered after and before the handle's

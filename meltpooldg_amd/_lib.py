@@ -155,6 +155,7 @@ SIGNATURES = {
                                                   ctypes.c_double]),
     "mpdg_heat_l2_norm": (ctypes.c_int, [_VP, _VP, ctypes.POINTER(ctypes.c_double)]),
     "mpdg_heat_distribute_constraints": (ctypes.c_int, [_VP, _VP]),
+    "mpdg_heat_l2_norm_noalloc": (ctypes.c_int, [_VP, _VP, _VP]),
     "mpdg_heat_jacobi_vmult": (ctypes.c_int, [_VP, _VP, _VP, _VP]),
     "mpdg_heat_solve": (ctypes.c_int, [_VP, _VP, _VP, ctypes.POINTER(SolverParams),
                                        ctypes.POINTER(SolverReport)]),

@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstddef>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -2521,3 +2522,81 @@ mpdg_status mpdg_heat_batch_solve(mpdg_heat_batch *b, double *const *x, const do
 }
 
 } // extern "C"
+
+// ---------------------------------------------------------------------------
+// Allocation-free FE L2 norm (mpdg_heat_l2_norm_noalloc)
+// ---------------------------------------------------------------------------
+// The reduction scratch of the L2 norm kernel (one partial per workgroup, the
+// last-workgroup ticket) lives in one file-scope device array per GPU instead
+// of the handle's lazily allocated workspace. Host code never passes the
+// array's name to a kernel (on the host it is only a stand-in for the symbol):
+// each device's address comes from hipGetSymbolAddress, once per device with
+// that device current, and only that address is used.
+namespace
+{
+  constexpr int kNoAllocPartials = 1 << 19; // workgroups of one norm launch (4 MiB of partials)
+  constexpr int kNoAllocTicket = kNoAllocPartials;      // slot of the unsigned ticket
+  constexpr int kNoAllocDoubles = kNoAllocPartials + 32; // + ticket, padded to 256 bytes
+  constexpr int kMaxDevices = 64;
+} // namespace
+
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+namespace
+{
+  std::mutex g_scratch_mutex;
+  double *g_scratch_ptr[kMaxDevices] = {};
+
+  // Device address of this device's scratch array (set-up on first use).
+  mpdg_status scratch_address(int device, double **out)
+  {
+    if (device < 0 || device >= kMaxDevices)
+      return fail(MPDG_ERR_UNSUPPORTED, "allocation-free entry points: device index above 63");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device])
+      {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+          return fail(MPDG_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                      hipGetErrorString(e));
+        g_scratch_ptr[device] = static_cast<double *>(ptr);
+      }
+    *out = g_scratch_ptr[device];
+    return MPDG_OK;
+  }
+
+  // The ticket must start at 0 (the kernel resets it on exit; this clears a
+  // ticket a failed launch may have left). A kernel rather than a memset, so
+  // that the call can be captured into a graph.
+  __global__ void clear_ticket_kernel(unsigned *ticket) { *ticket = 0u; }
+
+  __global__ void sqrt_in_place_kernel(double *v) { *v = sqrt(*v); }
+} // namespace
+
+extern "C" mpdg_status mpdg_heat_l2_norm_noalloc(mpdg_heat *h, const double *v, double *norm)
+{
+  if (!h || !v || !norm)
+    return fail(MPDG_ERR_ARGUMENT, "null argument");
+  const int blocks = l2_norm_blocks(h->desc.degree, h->desc.n_cells[0], h->desc.n_cells[1],
+                                    h->desc.n_cells[2]);
+  if (blocks > kNoAllocPartials)
+    return fail(MPDG_ERR_UNSUPPORTED,
+                "mpdg_heat_l2_norm_noalloc: the mesh needs " + std::to_string(blocks) +
+                  " reduction workgroups; the static scratch holds " +
+                  std::to_string(kNoAllocPartials));
+  int device = 0;
+  MPDG_HIP(hipGetDevice(&device));
+  if (device != h->device)
+    return fail(MPDG_ERR_ARGUMENT, "mpdg_heat_l2_norm_noalloc: the handle's device is not current");
+  double *base = nullptr;
+  if (mpdg_status s = scratch_address(device, &base))
+    return s;
+  unsigned *ticket = reinterpret_cast<unsigned *>(base + kNoAllocTicket);
+  clear_ticket_kernel<<<1, 1, 0, h->stream>>>(ticket);
+  MPDG_HIP(hipGetLastError());
+  MPDG_HIP(launch_l2_norm_sq(h->desc.degree, h->prm64, v, base, ticket, norm, h->stream));
+  sqrt_in_place_kernel<<<1, 1, 0, h->stream>>>(norm);
+  MPDG_HIP(hipGetLastError());
+  return MPDG_OK;
+}
