@@ -29,7 +29,8 @@ DT = 1e-3
 K_AMP = 0.5
 INNER_REL_TOLERANCE = 1e-4  # mixed CG: reduction asked of each inner FP32 solve
 HBM_PEAK_GBS = 8000.0
-FP64_PEAK_TFLOPS = 78.6
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector (VALU) peak; its FP64 matrix peak is the same
+CLOCK_SETTLE_S = 0.5
 
 
 def algorithmic_counts(n_cells, degree, bytes_per=8):
@@ -124,9 +125,25 @@ def cpu_baseline(threads=None, budget_s=10.0, min_samples=20):
             "host": host}
 
 
+def kernel_label(tuning):
+    """The line kernel of the 64^3 Q4 FP64 workload (4 wavefronts x 2 cells,
+    one lane slot, 128-VGPR budget) and its launches per vmult under the
+    installed schedule."""
+    k = "vmult_kernel<4, double, 1, false, 2, 1, 4, 4>"
+    if tuning["variant"] != "line":
+        k = f"vmult ({tuning['variant']} form)"
+    sched = tuning["schedule"]
+    if sched == "slabs":
+        return k + " x 16 (8 colour passes x 2 z-slabs on two streams)", 16
+    if sched == "ordered":
+        return k.replace("vmult_kernel", "vmult_ordered_kernel") + " x 1 (ordered)", 1
+    return k + " x 8 colour passes", 8
+
+
 def measured_traffic(kernel_name):
-    """HBM bytes per launch of the vmult kernel from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary, if it was taken on this kernel."""
+    """HBM bytes per colour pass of the vmult kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary, if it was taken on this kernel and
+    schedule."""
     path = os.path.join(ROOT, "profiles", "vmult_q4_64_fp64_traffic.json")
     try:
         with open(path) as f:
@@ -135,7 +152,7 @@ def measured_traffic(kernel_name):
         return None
     if t.get("kernel") != kernel_name:
         return None
-    return t.get("bytes_per_launch")
+    return t.get("bytes_per_pass")
 
 
 def copy_calibration(args, torch, dev):
@@ -273,6 +290,14 @@ def main():
 
     # the library launches on the handle's stream: torch's current stream here
     stream = torch.cuda.current_stream()
+    # clock settling, untimed and independent of --warmup: the GPU reaches its
+    # steady clocks after some 100 ms of work (with only 5 vmults of warm-up
+    # the first timed ones ran about 10 % slower)
+    t_settle = time.perf_counter() + CLOCK_SETTLE_S
+    while time.perf_counter() < t_settle:
+        for _ in range(20):
+            op.vmult(dst, src)
+        torch.cuda.synchronize()
     for _ in range(args.warmup):
         op.vmult(dst, src)
     torch.cuda.synchronize()
@@ -289,23 +314,26 @@ def main():
     ms_step = wall * 1e3 / args.steps
     gdofs = N / (ms_step * 1e-3) / 1e9
 
-    # one vmult = 8 launches of the cell kernel (one per colour pass): per-launch
-    # algorithmic flops and bytes are 1/8 of the vmult's, the launch time 1/8 of
-    # the event time per vmult
+    # one vmult = 8 colour passes of the line kernel. With the passes schedule
+    # a pass is one launch; with the slab schedule it is two launches, one per
+    # z-slab on its own stream, running side by side. Per pass, algorithmic
+    # flops and bytes are 1/8 of the vmult's and the time 1/8 of the event
+    # time per vmult.
     t_k = ms_event * 1e-3
     ach_flops = cnt["flops"] / t_k / 1e12
     ach_bytes = cnt["bytes"] / t_k / 1e9
-    kernel = "vmult_kernel<4, double, 1, false, 2, 1, 4, 4>"  # line form; the cell form is p <= 2
-    roofline = {"bound": "mfma", "achieved": round(ach_flops, 3), "peak": FP64_PEAK_TFLOPS,
+    kernel, launches = kernel_label(tuning)
+    roofline = {"bound": "valu_fp64", "achieved": round(ach_flops, 3), "peak": FP64_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(ach_flops / FP64_PEAK_TFLOPS, 4),
                 "traffic": measured_traffic(kernel),
-                "kernel": kernel + " x 8 colour passes",
-                "launch_ms": round(ms_event / 8, 5),
-                "algorithmic_bytes_per_launch": cnt["bytes"] // 8,
-                "algorithmic_flops_per_launch": cnt["flops"] // 8,
+                "kernel": kernel, "launches_per_vmult": launches,
+                "pass_ms": round(ms_event / 8, 5),
+                "algorithmic_bytes_per_pass": cnt["bytes"] // 8,
+                "algorithmic_flops_per_pass": cnt["flops"] // 8,
                 "hbm_achieved_GBs": round(ach_bytes, 1),
                 "hbm_frac": round(ach_bytes / HBM_PEAK_GBS, 4),
-                "tuning": tuning}
+                "tuning": tuning,
+                "note": "FP64 vector-FMA (VALU) kernel: peak is MI355X's FP64 vector rate"}
 
     solve = None
     if not args.no_solve:

@@ -395,7 +395,14 @@ typedef enum {
                                 mpdg_heat_time_vmult) and by every later call on the
                                 handle, mpdg_heat_vmult included (the vmult itself does
                                 not synchronise: it queues a copy of the error bit that
-                                the next call reads) */
+                                the next call reads) */,
+  MPDG_SCHEDULE_SLABS = 2    /* the colour passes split into two z-slabs of work items,
+                                each on its own stream (the handle's second stream,
+                                forked from and joined to the launch stream by events),
+                                so one slab's passes fill the other's launch ramps and
+                                tails. The two slabs share one lattice plane, ordered by
+                                one event between them: dst is bitwise the PASSES
+                                result. Line forms; the cell form runs PASSES */
 } mpdg_vmult_schedule;
 
 typedef struct {

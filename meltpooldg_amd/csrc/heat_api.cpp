@@ -88,6 +88,8 @@ struct mpdg_heat
   // synchronise (the vmult entry points): each such launch is followed by an
   // asynchronous copy of the error bit into pinned memory and an event; the
   // next call on the handle reads the copy once its event has completed
+  // slab schedule of the colour passes: the second stream and its events
+  SlabStreams slabs{};
   unsigned *host_order_err = nullptr; // pinned, one per precision
   hipEvent_t order_ev[2]{};
   bool order_pending[2]{};
@@ -169,6 +171,11 @@ struct mpdg_heat
       (void)hipHostFree(host_nres);
     if (host_order_err)
       (void)hipHostFree(host_order_err);
+    for (hipEvent_t ev : {slabs.fork, slabs.join, slabs.mid})
+      if (ev)
+        (void)hipEventDestroy(ev);
+    if (slabs.side)
+      (void)hipStreamDestroy(slabs.side);
     for (hipEvent_t ev : order_ev)
       if (ev)
         (void)hipEventDestroy(ev);
@@ -200,6 +207,21 @@ struct mpdg_heat
 
 namespace
 {
+  // Slab schedule resources, made when a tuning first asks for it: a second
+  // stream (default priority) and its events.
+  hipError_t ensure_slabs(mpdg_heat *H)
+  {
+    if (H->slabs.side)
+      return hipSuccess;
+    hipError_t e = hipSuccess;
+    for (hipEvent_t *ev : {&H->slabs.fork, &H->slabs.join, &H->slabs.mid})
+      if (e == hipSuccess && !*ev)
+        e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
+    if (e == hipSuccess)
+      e = hipStreamCreateWithFlags(&H->slabs.side, hipStreamNonBlocking);
+    return e;
+  }
+
   // Default launch tuning per degree and precision: the fastest setting of a
   // sweep over z-chunk length, workgroups per CU and variant on an MI355X at
   // the sizes of the benchmark workloads (mpdg_heat_autotune refines it for a
@@ -233,13 +255,23 @@ namespace
           break;
         default: t.z_layers = 2; t.blocks_per_cu = 4; break;
       }
+    // FP64 degrees 2..4 on large meshes: the slab schedule (two streams) with
+    // two-layer chunks measured 1.05x (Q4 64^3), 1.09x (Q3 96^3) and 1.15x
+    // (Q2 96^3) the passes; the FP32 forms did not gain
+    if (!fp32 && degree >= 2 && degree <= 4 && cells >= (int64_t(1) << 18))
+      {
+        t.schedule = MPDG_SCHEDULE_SLABS;
+        t.z_layers = 2;
+        t.blocks_per_cu = 0;
+      }
     return t;
   }
 
   bool tuning_valid(const mpdg_tuning &t, int degree, int precision)
   {
     return t.z_layers >= 1 && t.blocks_per_cu >= 0 && t.blocks_per_cu <= kMaxResidentPerCu &&
-           (t.schedule == MPDG_SCHEDULE_PASSES || t.schedule == MPDG_SCHEDULE_ORDERED) &&
+           (t.schedule == MPDG_SCHEDULE_PASSES || t.schedule == MPDG_SCHEDULE_ORDERED ||
+            t.schedule == MPDG_SCHEDULE_SLABS) &&
            (t.variant == MPDG_VMULT_LINE ||
             (t.variant == MPDG_VMULT_CELL && degree <= kCellMaxDegree && t.tile_log2 >= 3 &&
              t.tile_log2 <= 6) ||
@@ -405,6 +437,7 @@ namespace
                          H->order[pi].ptr != nullptr;
     prm.order = ordered ? H->order[pi].ptr : nullptr;
     prm.order_flags = ordered ? H->order_flags[pi].ptr : nullptr;
+    prm.slabs = tu.schedule == MPDG_SCHEDULE_SLABS && prm.variant != kVariantCell ? 2 : 1;
     // per-direction tables (see coefficient_tables): [sin2pi dir 0 | dir 1 | dir 2 |
     // u_x x-factor | u_x y-factor | u_y x-factor | u_y y-factor]
     size_t off = 0;
@@ -665,10 +698,12 @@ namespace
       }
     else if constexpr (sizeof(Number) == 8)
       MPDG_HIP(launch_vmult<double>(H->desc.degree, coef_mode(H), H->desc.velocity != 0, H->prm64,
-                                    H->vt64, src, dst, H->heav64.ptr, partials, scal, H->stream));
+                                    H->vt64, src, dst, H->heav64.ptr, partials, scal, H->stream,
+                                    H->slabs.side ? &H->slabs : nullptr));
     else
       MPDG_HIP(launch_vmult<float>(H->desc.degree, coef_mode(H), H->desc.velocity != 0, H->prm32,
-                                   H->vt32, src, dst, H->heav32.ptr, partials, scal, H->stream));
+                                   H->vt32, src, dst, H->heav32.ptr, partials, scal, H->stream,
+                                   H->slabs.side ? &H->slabs : nullptr));
     MPDG_HIP(launch_robin<Number>(H->desc.degree, kRobinTangent, H->robin, dst, src,
                                   H->temperature, H->robin_rows.ptr, partials, scal, H->stream));
     return MPDG_OK;
@@ -1155,6 +1190,9 @@ mpdg_status mpdg_heat_create(const mpdg_heat_desc *desc, mpdg_heat **out)
     e = hipHostMalloc((void **)&H->host_order_err, 2 * sizeof(unsigned), hipHostMallocDefault);
   if (e == hipSuccess)
     H->host_order_err[0] = H->host_order_err[1] = 0u;
+  if (e == hipSuccess && (H->tune[MPDG_FP64].schedule == MPDG_SCHEDULE_SLABS ||
+                          H->tune[MPDG_FP32].schedule == MPDG_SCHEDULE_SLABS))
+    e = ensure_slabs(H);
   // ordered vmult schedule: counters and epoch at 0, one flag per possible work
   // item (an item holds at least one cell)
   for (int k = 0; k < 2 && e == hipSuccess; ++k)
@@ -1809,6 +1847,8 @@ mpdg_status mpdg_heat_set_tuning(mpdg_heat *h, int precision, const mpdg_tuning 
                 "and tile_log2 in 3..6; the MFMA variant degree >= 3 and MPDG_FP32");
   if (h->stream)
     MPDG_HIP(hipStreamSynchronize(h->stream));
+  if (t->schedule == MPDG_SCHEDULE_SLABS)
+    MPDG_HIP(ensure_slabs(h));
   h->tune[precision] = *t;
   fill_params<double>(h, h->prm64, h->sin_tab64.ptr);
   fill_params<float>(h, h->prm32, h->sin_tab32.ptr);
@@ -1829,6 +1869,7 @@ mpdg_status mpdg_heat_autotune(mpdg_heat *h, int precision, int repeats, mpdg_tu
   const bool fp32 = precision == MPDG_FP32;
   if (mpdg_status s = ensure_workspace(h, fp32))
     return s;
+  MPDG_HIP(ensure_slabs(h));
   // scratch vectors: src = a fill of ones, dst = the PCG's q
   void *src = fp32 ? (void *)h->p32.ptr : (void *)h->p.ptr;
   void *dst = fp32 ? (void *)h->q32.ptr : (void *)h->q.ptr;
@@ -1845,7 +1886,7 @@ mpdg_status mpdg_heat_autotune(mpdg_heat *h, int precision, int repeats, mpdg_tu
         continue;
       for (int bpc : {0, 2, 4, 8})
         {
-          for (int sch : {MPDG_SCHEDULE_PASSES, MPDG_SCHEDULE_ORDERED})
+          for (int sch : {MPDG_SCHEDULE_PASSES, MPDG_SCHEDULE_ORDERED, MPDG_SCHEDULE_SLABS})
             {
               cand.push_back(mpdg_tuning{zl, bpc, MPDG_VMULT_LINE, 4, sch});
               if (fp32 && h->desc.degree >= kMfmaMinDegree)

@@ -926,15 +926,17 @@ namespace mpdg
     __global__ void __launch_bounds__(64 * WPB, MINW)
       vmult_kernel(const Number *__restrict__ src, Number *__restrict__ dst,
                    const Number *__restrict__ heav, const CellParams<Number> prm,
-                   const int colour, const int layers, const int items,
+                   const int colour, const int layers, const int items, const int item_base,
                    double *__restrict__ partials, const int partial_offset,
                    const int partial_total, const int finalize, SolverScalars *__restrict__ scal)
     {
       constexpr int THREADS = 64 * WPB;
       if (scal != nullptr && scal->done)
         return;
+      // items [item_base, item_base + items) of the colour
       const double acc = vmult_body<P, Number, COEF, VEL, CW, R, WPB, MF>(
-        src, dst, heav, prm, colour, layers, items, blockIdx.x, gridDim.x);
+        src, dst, heav, prm, colour, layers, item_base + items, item_base + (int)blockIdx.x,
+        gridDim.x);
       // p . Ap: sum over cells of u_cell . y_cell (constrained entries are 0)
       if (partials != nullptr)
         pass_partials<THREADS>(acc, partials, partial_offset, partial_total, finalize, scal);
@@ -1307,10 +1309,70 @@ namespace mpdg
       static constexpr int MINW = L4 ? 6 : ((P >= 5 && !FP32) || (MF && P == 5)) ? 2 : kMinWavesPerSimd;
     };
 
+    // Slab schedule of the colour passes (prm.slabs == 2): the z-chunks of the
+    // work items are split into slab 0 = [0, ks) and slab 1 = [ks, nkz), and
+    // each slab runs its eight passes on its own stream. Items of the two
+    // slabs share only the lattice plane between chunks ks - 1 and ks. Those
+    // two chunks differ in parity, so the even one's items (colours 0..3)
+    // store into that plane before the odd one's items (colours 4..7)
+    // read-modify-write it, as in the single-stream passes: the odd side's
+    // pass 4 waits for the even side's pass 3 (event mid). Every lattice node
+    // therefore receives its contributions in the same order as with the
+    // single-stream passes and dst is bitwise the same; the p . Ap partials
+    // are grouped per slab launch.
+    template <typename Kernel, typename Number, int CPB, int THREADS>
+    hipError_t vmult_slabs(Kernel kernel, const CellParams<Number> &prm, const Number *src,
+                           Number *dst, const Number *heav, double *partials, SolverScalars *scal,
+                           hipStream_t stream, const SlabStreams &ss, int max_blocks)
+    {
+      const int nsx = (prm.nx + CPB - 1) / CPB;
+      const int nkz = (prm.nz + prm.zlayers - 1) / prm.zlayers;
+      const int ks = nkz / 2;
+      const hipStream_t st[2] = {stream, ss.side};
+      const int even = ((ks - 1) & 1) == 0 ? 0 : 1; // slab whose boundary chunk is even
+      hipError_t e = hipEventRecord(ss.fork, stream);
+      if (e == hipSuccess)
+        e = hipStreamWaitEvent(ss.side, ss.fork, 0);
+      int off = 0;
+      for (int c = 0; c < 8 && e == hipSuccess; ++c)
+        {
+          const int ox = c & 1, oy = (c >> 1) & 1, oz = (c >> 2) & 1;
+          const int per_kz = ((nsx - ox + 1) / 2) * ((prm.ny - oy + 1) / 2);
+          // items of colour c in chunks below kz (kz is the slowest item index)
+          auto below = [&](int kz) { return per_kz * ((kz - oz + 1) / 2); };
+          const int base[2] = {0, below(ks)};
+          const int cnt[2] = {below(ks), below(nkz) - below(ks)};
+          if (c == 4)
+            e = hipStreamWaitEvent(st[1 - even], ss.mid, 0);
+          for (int sl = 0; sl < 2 && e == hipSuccess; ++sl)
+            {
+              if (cnt[sl] == 0)
+                continue;
+              const int grid = std::min(cnt[sl], max_blocks);
+              kernel<<<grid, THREADS, 0, st[sl]>>>(src, dst, heav, prm, c, prm.zlayers, cnt[sl],
+                                                   base[sl], partials, off, 0, 0, scal);
+              off += grid;
+              e = hipGetLastError();
+            }
+          if (c == 3 && e == hipSuccess)
+            e = hipEventRecord(ss.mid, st[even]);
+        }
+      if (e == hipSuccess)
+        e = hipEventRecord(ss.join, ss.side);
+      if (e == hipSuccess)
+        e = hipStreamWaitEvent(stream, ss.join, 0);
+      if (e == hipSuccess && partials != nullptr && scal != nullptr && off > 0)
+        {
+          pap_finalize_kernel<<<1, 256, 0, stream>>>(partials, off, scal);
+          e = hipGetLastError();
+        }
+      return e;
+    }
+
     template <typename Number, int P, int COEF, bool VEL, bool MF>
     hipError_t vmult_passes(const CellParams<Number> &prm, const VmultTables<Number> &tab,
                             const Number *src, Number *dst, const Number *heav, double *partials,
-                            SolverScalars *scal, hipStream_t stream)
+                            SolverScalars *scal, hipStream_t stream, const SlabStreams *slabs)
     {
       using Shape = VShape<P, sizeof(Number) == 4, MF, COEF != kCoefHeaviside && !VEL>;
       constexpr int THREADS = 64 * Shape::WPB;
@@ -1356,6 +1418,9 @@ namespace mpdg
           okernel<<<grid, OTHREADS, 0, stream>>>(src, dst, heav, prm, prm.zlayers, og, partials, scal);
           return hipGetLastError();
         }
+      if (slabs != nullptr && prm.slabs == 2 && (prm.nz + prm.zlayers - 1) / prm.zlayers >= 2)
+        return vmult_slabs<decltype(kernel), Number, CPB, THREADS>(
+          kernel, prm, src, dst, heav, partials, scal, stream, *slabs, per_cu * device_cu_count());
       for (int c = 0; c < 8; ++c)
         {
           if (vg.grid[c] == 0)
@@ -1364,8 +1429,8 @@ namespace mpdg
           // them after the last pass (the launch boundary orders the stores),
           // so no workgroup of the last pass waits for its stores to complete
           kernel<<<vg.grid[c], THREADS, 0, stream>>>(src, dst, heav, prm, c, prm.zlayers,
-                                                     vg.items[c], partials, vg.offset[c], vg.total,
-                                                     0, scal);
+                                                     vg.items[c], 0, partials, vg.offset[c],
+                                                     vg.total, 0, scal);
         }
       if (partials != nullptr && scal != nullptr && vg.last >= 0)
         pap_finalize_kernel<<<1, 256, 0, stream>>>(partials, vg.total, scal);
@@ -1400,16 +1465,17 @@ namespace mpdg
     template <typename Number, int P, bool MF>
     hipError_t vmult_deg_mf(int coef, bool vel, const CellParams<Number> &prm,
                             const VmultTables<Number> &tab, const Number *src, Number *dst,
-                            const Number *heav, double *partials, SolverScalars *scal, hipStream_t s)
+                            const Number *heav, double *partials, SolverScalars *scal, hipStream_t s,
+                            const SlabStreams *slabs)
     {
       switch (coef * 2 + (vel ? 1 : 0))
         {
-          case 0: return vmult_passes<Number, P, 0, false, MF>(prm, tab, src, dst, heav, partials, scal, s);
-          case 1: return vmult_passes<Number, P, 0, true, MF>(prm, tab, src, dst, heav, partials, scal, s);
-          case 2: return vmult_passes<Number, P, 1, false, MF>(prm, tab, src, dst, heav, partials, scal, s);
-          case 3: return vmult_passes<Number, P, 1, true, MF>(prm, tab, src, dst, heav, partials, scal, s);
-          case 4: return vmult_passes<Number, P, 2, false, MF>(prm, tab, src, dst, heav, partials, scal, s);
-          case 5: return vmult_passes<Number, P, 2, true, MF>(prm, tab, src, dst, heav, partials, scal, s);
+          case 0: return vmult_passes<Number, P, 0, false, MF>(prm, tab, src, dst, heav, partials, scal, s, slabs);
+          case 1: return vmult_passes<Number, P, 0, true, MF>(prm, tab, src, dst, heav, partials, scal, s, slabs);
+          case 2: return vmult_passes<Number, P, 1, false, MF>(prm, tab, src, dst, heav, partials, scal, s, slabs);
+          case 3: return vmult_passes<Number, P, 1, true, MF>(prm, tab, src, dst, heav, partials, scal, s, slabs);
+          case 4: return vmult_passes<Number, P, 2, false, MF>(prm, tab, src, dst, heav, partials, scal, s, slabs);
+          case 5: return vmult_passes<Number, P, 2, true, MF>(prm, tab, src, dst, heav, partials, scal, s, slabs);
           default: return hipErrorInvalidValue;
         }
     }
@@ -1417,12 +1483,13 @@ namespace mpdg
     template <typename Number, int P>
     hipError_t vmult_deg(int coef, bool vel, const CellParams<Number> &prm,
                          const VmultTables<Number> &tab, const Number *src, Number *dst,
-                         const Number *heav, double *partials, SolverScalars *scal, hipStream_t s)
+                         const Number *heav, double *partials, SolverScalars *scal, hipStream_t s,
+                         const SlabStreams *slabs)
     {
       if constexpr (has_mfma_form<Number, P>())
         if (prm.variant == kVariantLineMfma)
-          return vmult_deg_mf<Number, P, true>(coef, vel, prm, tab, src, dst, heav, partials, scal, s);
-      return vmult_deg_mf<Number, P, false>(coef, vel, prm, tab, src, dst, heav, partials, scal, s);
+          return vmult_deg_mf<Number, P, true>(coef, vel, prm, tab, src, dst, heav, partials, scal, s, slabs);
+      return vmult_deg_mf<Number, P, false>(coef, vel, prm, tab, src, dst, heav, partials, scal, s, slabs);
     }
 
     template <typename Number, int P>
@@ -1548,19 +1615,19 @@ namespace mpdg
   hipError_t launch_vmult(int degree, int coef, bool vel, const CellParams<Number> &prm,
                           const VmultTables<Number> &tab, const Number *src, Number *dst,
                           const Number *heaviside, double *partials, SolverScalars *scal,
-                          hipStream_t stream)
+                          hipStream_t stream, const SlabStreams *slabs)
   {
     if (prm.variant == kVariantCell && degree <= kCellMaxDegree)
       return launch_vmult_cell<Number>(degree, coef, vel, prm, src, dst, heaviside, partials, scal,
                                        stream);
     switch (degree)
       {
-        case 1: return vmult_deg<Number, 1>(coef, vel, prm, tab, src, dst, heaviside, partials, scal, stream);
-        case 2: return vmult_deg<Number, 2>(coef, vel, prm, tab, src, dst, heaviside, partials, scal, stream);
-        case 3: return vmult_deg<Number, 3>(coef, vel, prm, tab, src, dst, heaviside, partials, scal, stream);
-        case 4: return vmult_deg<Number, 4>(coef, vel, prm, tab, src, dst, heaviside, partials, scal, stream);
-        case 5: return vmult_deg<Number, 5>(coef, vel, prm, tab, src, dst, heaviside, partials, scal, stream);
-        case 6: return vmult_deg<Number, 6>(coef, vel, prm, tab, src, dst, heaviside, partials, scal, stream);
+        case 1: return vmult_deg<Number, 1>(coef, vel, prm, tab, src, dst, heaviside, partials, scal, stream, slabs);
+        case 2: return vmult_deg<Number, 2>(coef, vel, prm, tab, src, dst, heaviside, partials, scal, stream, slabs);
+        case 3: return vmult_deg<Number, 3>(coef, vel, prm, tab, src, dst, heaviside, partials, scal, stream, slabs);
+        case 4: return vmult_deg<Number, 4>(coef, vel, prm, tab, src, dst, heaviside, partials, scal, stream, slabs);
+        case 5: return vmult_deg<Number, 5>(coef, vel, prm, tab, src, dst, heaviside, partials, scal, stream, slabs);
+        case 6: return vmult_deg<Number, 6>(coef, vel, prm, tab, src, dst, heaviside, partials, scal, stream, slabs);
         default: return hipErrorInvalidValue;
       }
   }
@@ -1786,10 +1853,12 @@ namespace mpdg
 
   template hipError_t launch_vmult<double>(int, int, bool, const CellParams<double> &,
                                            const VmultTables<double> &, const double *, double *,
-                                           const double *, double *, SolverScalars *, hipStream_t);
+                                           const double *, double *, SolverScalars *, hipStream_t,
+                                           const SlabStreams *);
   template hipError_t launch_vmult<float>(int, int, bool, const CellParams<float> &,
                                           const VmultTables<float> &, const float *, float *,
-                                          const float *, double *, SolverScalars *, hipStream_t);
+                                          const float *, double *, SolverScalars *, hipStream_t,
+                                          const SlabStreams *);
   template hipError_t launch_diagonal<double>(int, int, bool, const CellParams<double> &,
                                               const DiagTables<double> &, double *, const double *,
                                               hipStream_t);

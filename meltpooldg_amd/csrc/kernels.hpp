@@ -60,6 +60,9 @@ namespace mpdg
     // neighbouring items of earlier colours
     struct OrderState *order;
     unsigned *order_flags;
+    // line forms, slab schedule (slabs == 2, with the handle's SlabStreams):
+    // the colour passes split into two z-slabs of work items on two streams
+    int slabs;
     const Number *sin2pi[3]; // sin(2 pi x) at the quadrature coordinates of each direction
     // in-plane velocity at q, separable: u_x = vel_x[0](x) vel_y[0](y), u_y = vel_x[1](x) vel_y[1](y)
     // (tables over the x resp. y quadrature coordinates); u_z = w0
@@ -148,6 +151,15 @@ namespace mpdg
 
   // Persistent launch shape of the vmult: per colour the number of work items
   // (segment x row x z-chunk) and of workgroups looping over them.
+  // Two-slab schedule of the line forms' colour passes: the handle's second
+  // stream and the events that fork it from, and join it to, the launch
+  // stream, plus the event that orders the two slabs at their shared plane.
+  struct SlabStreams
+  {
+    hipStream_t side;
+    hipEvent_t fork, join, mid;
+  };
+
   struct VmultGrid
   {
     int items[8];
@@ -165,12 +177,13 @@ namespace mpdg
 
   // Cell-loop vmult: dst = A src over all 8 colour passes. With partials (and
   // scal) it also reduces p.Ap into scal->pap and sets scal->alpha = rz / pap;
-  // with scal it does nothing once scal->done is set.
+  // with scal it does nothing once scal->done is set. slabs: the handle's
+  // second stream and events, used when prm.slabs == 2 (line forms).
   template <typename Number>
   hipError_t launch_vmult(int degree, int coef, bool vel, const CellParams<Number> &prm,
                           const VmultTables<Number> &tab, const Number *src, Number *dst,
                           const Number *heaviside, double *partials, SolverScalars *scal,
-                          hipStream_t stream);
+                          hipStream_t stream, const SlabStreams *slabs = nullptr);
 
   // Cell form of the vmult (cell_kernels.hip), same contract as launch_vmult,
   // degree <= kCellMaxDegree; launch_vmult forwards to it when

@@ -347,7 +347,7 @@ class HeatDiffuseOperator:
         names = {v: k for k, v in cls._VARIANTS.items()}
         return {"z_layers": t.z_layers, "blocks_per_cu": t.blocks_per_cu,
                 "variant": names.get(t.variant, t.variant), "tile_log2": t.tile_log2,
-                "schedule": "ordered" if t.schedule == 1 else "passes"}
+                "schedule": {0: "passes", 1: "ordered", 2: "slabs"}[t.schedule]}
 
     def create_rhs(self, dst, src, temperature, heat_source=None):
         """HeatDiffuseMultiPhaseOperator::create_rhs(dst, src = temperature_old)
@@ -376,9 +376,9 @@ class HeatDiffuseOperator:
         t.variant = self._VARIANTS[v] if isinstance(v, str) else int(v)
         t.tile_log2 = int(cur["tile_log2"] if tile_log2 is None else tile_log2)
         sc = cur["schedule"] if schedule is None else schedule
-        if sc not in ("passes", "ordered"):
-            raise ValueError("schedule must be 'passes' or 'ordered'")
-        t.schedule = 1 if sc == "ordered" else 0
+        if sc not in ("passes", "ordered", "slabs"):
+            raise ValueError("schedule must be 'passes', 'ordered' or 'slabs'")
+        t.schedule = {"passes": 0, "ordered": 1, "slabs": 2}[sc]
         L.check(self._lib.mpdg_heat_set_tuning(self._h, self._prec_code(precision), ctypes.byref(t)))
 
     def autotune(self, precision=None, repeats=5):
