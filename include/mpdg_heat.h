@@ -68,8 +68,16 @@ typedef enum {
 typedef enum {
   MPDG_VELOCITY_NONE = 0,
   MPDG_VELOCITY_VORTEX = 1,  /* (-sin^2(pi x) sin(2 pi y), sin(2 pi x) sin^2(pi y), w0) */
-  MPDG_VELOCITY_CONSTANT = 2 /* velocity_value, e.g. a nodally interpolated constant
+  MPDG_VELOCITY_CONSTANT = 2, /* velocity_value, e.g. a nodally interpolated constant
                                 prescribed_velocity field (exact for any FE_Q degree) */
+  MPDG_VELOCITY_NODAL = 3     /* a DoF-vector velocity (the operator's `velocity`,
+                                include/meltpooldg/heat/heat_diffuse_operator.hpp:137-139)
+                                given by mpdg_heat_set_velocity: FE_Q(degree) nodal
+                                values of each component on the temperature's lattice,
+                                read plain and interpolated at q (heat_diffuse_operator.cpp:
+                                975-980). FP64 cell loop only (no FP32 handle, no mixed
+                                precision, no batches), not with the separable-sine
+                                conductivity */
 } mpdg_velocity_field;
 
 typedef struct {
@@ -178,6 +186,14 @@ mpdg_status mpdg_heat_set_dt(mpdg_heat *h, double dt);
  * pointer to n_dofs() doubles, read plain (no constraints) and interpolated at q.
  * The engine copies it (and converts to the cell-loop precision). */
 mpdg_status mpdg_heat_set_heaviside(mpdg_heat *h, const double *heaviside);
+
+/* The nodal velocity of MPDG_VELOCITY_NODAL (the DoF vector the reference operator
+ * reads through vel_eval.read_dof_values_plain, heat_diffuse_operator.cpp:975-980):
+ * device pointer to 3 n_dofs() doubles, component-blocked [v_x | v_y | v_z], each
+ * component in lattice numbering. The engine copies it on the handle's stream:
+ * call again when the field changes. Required before vmult, inverse diagonal,
+ * create_rhs and solve of such a handle. */
+mpdg_status mpdg_heat_set_velocity(mpdg_heat *h, const double *velocity);
 
 /* The temperature the tangent is linearised at: the operator's reference to the
  * current solution (constructor argument temperature_in,

@@ -388,6 +388,77 @@ namespace mpdg
       int sx, cy, cz, cx0, ncell;
     };
 
+    // A nodal vector field at the Gauss points of a row segment's cells (the
+    // block-synchronous layout: one workgroup per segment of CPB cells, thread
+    // = cell x line): component c of node I is v[c * n_dofs + I], read plain
+    // (read_dof_values_plain) and interpolated with S in x, y and z into the
+    // LDS tensors V0, V1, V2 (cell l at l * STR, index (qz * N + qy) * N + qx).
+    // Every thread of the workgroup calls it; it ends with a barrier.
+    template <int P, int CPB>
+    __device__ __forceinline__ void nodal_vector_at_q(const CellParams<double> &prm,
+                                                      const Segment &g, const double *v,
+                                                      double *V0, double *V1, double *V2)
+    {
+      constexpr int N = P + 1;
+      constexpr int NL = N * N;
+      constexpr int STR = lds_stride<N>();
+      constexpr ct::EO<N> TS = ct::VTab<P>::S;
+      const int t = threadIdx.x;
+      const bool active = t < CPB * NL;
+      const int l = t % CPB;
+      const int s = t / CPB;
+      const int s0 = s % N, s1 = s / N;
+      const bool cell_ok = active && l < g.ncell;
+      const int64_t nd = (int64_t)prm.n1x * prm.n1y * prm.n1z;
+      double *W[3] = {V0 + l * STR, V1 + l * STR, V2 + l * STR};
+      if (active)
+        {
+          const int64_t base = (int64_t)(g.cx0 + l) * P +
+                               (int64_t)prm.n1x * (g.cy * P + s0 + (int64_t)prm.n1y * (g.cz * P + s1));
+#pragma unroll
+          for (int c = 0; c < 3; ++c)
+            {
+              double x[N], o[N];
+#pragma unroll
+              for (int a = 0; a < N; ++a)
+                x[a] = cell_ok ? v[c * nd + base + a] : 0.0;
+              eo_apply<N, 1>(TS, x, o);
+#pragma unroll
+              for (int q = 0; q < N; ++q)
+                W[c][(s1 * N + s0) * N + q] = o[q];
+            }
+        }
+      __syncthreads();
+      if (active)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          {
+            double x[N], o[N];
+#pragma unroll
+            for (int b = 0; b < N; ++b)
+              x[b] = W[c][(s1 * N + b) * N + s0];
+            eo_apply<N, 1>(TS, x, o);
+#pragma unroll
+            for (int b = 0; b < N; ++b)
+              W[c][(s1 * N + b) * N + s0] = o[b];
+          }
+      __syncthreads();
+      if (active)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          {
+            double x[N], o[N];
+#pragma unroll
+            for (int z = 0; z < N; ++z)
+              x[z] = W[c][(z * N + s1) * N + s0];
+            eo_apply<N, 1>(TS, x, o);
+#pragma unroll
+            for (int z = 0; z < N; ++z)
+              W[c][(z * N + s1) * N + s0] = o[z];
+          }
+      __syncthreads();
+    }
+
     template <typename Number, int CPB>
     __device__ __forceinline__ Segment segment_of(const CellParams<Number> &prm, int colour)
     {

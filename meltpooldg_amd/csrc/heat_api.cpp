@@ -103,6 +103,8 @@ struct mpdg_heat
   DevBuf<double> heav64;
   DevBuf<float> heav32;
   bool has_heaviside = false;
+  DevBuf<double> vel64; // nodal velocity, 3 N (MPDG_VELOCITY_NODAL)
+  bool has_velocity = false;
 
   // Robin faces and the temperature the tangent is linearised at (a caller's
   // device vector, not a copy: the reference operator holds a reference)
@@ -154,7 +156,7 @@ struct mpdg_heat
 
   ~mpdg_heat()
   {
-    for (auto *b : {&sin_tab64, &sin_tab32_src, &heav64, &r, &p, &q, &inv_diag, &partials,
+    for (auto *b : {&sin_tab64, &sin_tab32_src, &heav64, &vel64, &r, &p, &q, &inv_diag, &partials,
                     &V64, &gm, &robin_rows})
       b->release();
     for (auto *b : {&sin_tab32, &heav32, &r32, &p32, &q32, &x32, &inv32, &V32})
@@ -465,6 +467,18 @@ namespace
     return d.material == MPDG_MATERIAL_GAS_LIQUID || d.material == MPDG_MATERIAL_GAS_LIQUID_SOLID;
   }
 
+  bool nodal_velocity(const mpdg_heat_desc &d) { return d.velocity == MPDG_VELOCITY_NODAL; }
+
+  // The cell loop evaluates its coefficients per quadrature point from nodal
+  // fields (material_kernels.hip, FP64): temperature-dependent materials and
+  // nodal velocity fields.
+  bool generic_cells(const mpdg_heat_desc &d) { return temperature_dependent(d) || nodal_velocity(d); }
+
+  int velocity_mode(const mpdg_heat_desc &d)
+  {
+    return nodal_velocity(d) ? kVelNodal : (d.velocity != MPDG_VELOCITY_NONE ? kVelTables : kVelNone);
+  }
+
   int coef_mode(const mpdg_heat *H)
   {
     if (temperature_dependent(H->desc))
@@ -484,6 +498,8 @@ namespace
                   "time increment not set: call mpdg_heat_set_dt (reset_time_increment) first");
     if (uses_heaviside(H->desc) && !H->has_heaviside)
       return fail(MPDG_ERR_ARGUMENT, "gas_liquid(_solid) material needs mpdg_heat_set_heaviside");
+    if (nodal_velocity(H->desc) && !H->has_velocity)
+      return fail(MPDG_ERR_ARGUMENT, "a nodal velocity field needs mpdg_heat_set_velocity");
     return MPDG_OK;
   }
 
@@ -685,16 +701,18 @@ namespace
   {
     if (mpdg_status s = check_tangent(H))
       return s;
-    if (temperature_dependent(H->desc))
+    if (generic_cells(H->desc))
       {
-        // the solidification tangent (FP64 only; creation rejects FP32 handles)
+        // coefficients per quadrature point from nodal fields (FP64 only;
+        // creation rejects FP32 handles)
         if constexpr (sizeof(Number) == 8)
-          MPDG_HIP(launch_tmat_vmult(H->desc.degree, H->desc.velocity != 0, H->prm64, src, dst,
-                                     H->temperature, H->temperature_old, H->heav64.ptr, partials,
-                                     scal, H->stream));
+          MPDG_HIP(launch_tmat_vmult(H->desc.degree, velocity_mode(H->desc), H->prm64, src, dst,
+                                     H->temperature, H->temperature_old,
+                                     uses_heaviside(H->desc) ? H->heav64.ptr : nullptr,
+                                     H->vel64.ptr, partials, scal, H->stream));
         else
-          return fail(MPDG_ERR_UNSUPPORTED,
-                      "temperature-dependent materials have an FP64 cell loop only");
+          return fail(MPDG_ERR_UNSUPPORTED, "temperature-dependent materials and nodal velocity "
+                                            "fields have an FP64 cell loop only");
       }
     else if constexpr (sizeof(Number) == 8)
       MPDG_HIP(launch_vmult<double>(H->desc.degree, coef_mode(H), H->desc.velocity != 0, H->prm64,
@@ -725,9 +743,11 @@ namespace
   {
     if (mpdg_status s = check_tangent(H))
       return s;
-    if (temperature_dependent(H->desc))
-      MPDG_HIP(launch_tmat_diagonal(H->desc.degree, H->desc.velocity != 0, H->prm64, H->dt64, inv,
-                                    H->temperature, H->temperature_old, H->heav64.ptr, H->stream));
+    if (generic_cells(H->desc))
+      MPDG_HIP(launch_tmat_diagonal(H->desc.degree, velocity_mode(H->desc), H->prm64, H->dt64, inv,
+                                    H->temperature, H->temperature_old,
+                                    uses_heaviside(H->desc) ? H->heav64.ptr : nullptr,
+                                    H->vel64.ptr, H->stream));
     else
       MPDG_HIP(launch_diagonal<double>(H->desc.degree, coef_mode(H), H->desc.velocity != 0,
                                        H->prm64, H->dt64, inv, H->heav64.ptr, H->stream));
@@ -1102,8 +1122,13 @@ mpdg_status mpdg_heat_create(const mpdg_heat_desc *desc, mpdg_heat **out)
                                               "volume specific heat capacity");
         }
     }
+  if (nodal_velocity(d) && d.precision != MPDG_FP64)
+    return fail(MPDG_ERR_UNSUPPORTED, "nodal velocity fields have an FP64 cell loop only");
+  if (nodal_velocity(d) && d.conductivity_field != MPDG_CONDUCTIVITY_MATERIAL)
+    return fail(MPDG_ERR_UNSUPPORTED,
+                "the separable-sine conductivity is not built with a nodal velocity field");
   if (d.velocity != MPDG_VELOCITY_NONE && d.velocity != MPDG_VELOCITY_VORTEX &&
-      d.velocity != MPDG_VELOCITY_CONSTANT)
+      d.velocity != MPDG_VELOCITY_CONSTANT && d.velocity != MPDG_VELOCITY_NODAL)
     return fail(MPDG_ERR_UNSUPPORTED, "velocity field not supported");
   if (d.velocity == MPDG_VELOCITY_CONSTANT &&
       !(std::isfinite(d.velocity_value[0]) && std::isfinite(d.velocity_value[1]) &&
@@ -1253,6 +1278,19 @@ mpdg_status mpdg_heat_set_dt(mpdg_heat *h, double dt)
   h->dt_set = true;
   fill_params<double>(h, h->prm64, h->sin_tab64.ptr);
   fill_params<float>(h, h->prm32, h->sin_tab32.ptr);
+  return MPDG_OK;
+}
+
+mpdg_status mpdg_heat_set_velocity(mpdg_heat *h, const double *velocity)
+{
+  if (!h || !velocity)
+    return fail(MPDG_ERR_ARGUMENT, "null handle or velocity");
+  if (!nodal_velocity(h->desc))
+    return fail(MPDG_ERR_ARGUMENT, "the handle was not created with MPDG_VELOCITY_NODAL");
+  MPDG_HIP(h->vel64.alloc(3 * (size_t)h->N));
+  MPDG_HIP(hipMemcpyAsync(h->vel64.ptr, velocity, 3 * (size_t)h->N * sizeof(double),
+                          hipMemcpyDeviceToDevice, h->stream));
+  h->has_velocity = true;
   return MPDG_OK;
 }
 
@@ -1429,8 +1467,9 @@ mpdg_status mpdg_heat_create_rhs(mpdg_heat *h, double *dst, const double *temper
     return s;
   if (!dst || !temperature || !temperature_old)
     return fail(MPDG_ERR_ARGUMENT, "null vector");
-  MPDG_HIP(launch_create_rhs(h->desc.degree, coef_mode(h), h->desc.velocity != 0, h->prm64, dst,
-                             temperature, temperature_old, heat_source, h->heav64.ptr, h->stream));
+  MPDG_HIP(launch_create_rhs(h->desc.degree, coef_mode(h), velocity_mode(h->desc), h->prm64, dst,
+                             temperature, temperature_old, heat_source, h->heav64.ptr,
+                             h->vel64.ptr, h->stream));
   // rhs_boundary_loop: the Robin faces with T read plain
   MPDG_HIP(launch_robin<double>(h->desc.degree, kRobinRhs, h->robin, dst, nullptr, temperature,
                                 h->robin_rows.ptr, nullptr, nullptr, h->stream));
@@ -1665,9 +1704,9 @@ static mpdg_status solve_impl(mpdg_heat *h, double *x, const double *b,
   if (prm.max_iterations < 1 || !(prm.rel_tolerance >= 0) || !(prm.abs_tolerance >= 0))
     return fail(MPDG_ERR_ARGUMENT, "invalid solver parameters");
   const bool mixed = prm.mixed_precision != 0;
-  if (mixed && temperature_dependent(h->desc))
-    return fail(MPDG_ERR_UNSUPPORTED,
-                "temperature-dependent materials have an FP64 cell loop only (no mixed precision)");
+  if (mixed && generic_cells(h->desc))
+    return fail(MPDG_ERR_UNSUPPORTED, "temperature-dependent materials and nodal velocity fields "
+                                      "have an FP64 cell loop only (no mixed precision)");
   if (mpdg_status s = ensure_workspace(h, mixed))
     return s;
   if (use_gmres)
@@ -1863,9 +1902,9 @@ mpdg_status mpdg_heat_autotune(mpdg_heat *h, int precision, int repeats, mpdg_tu
     return fail(MPDG_ERR_ARGUMENT, "precision must be MPDG_FP64 or MPDG_FP32");
   if (repeats < 1)
     return fail(MPDG_ERR_ARGUMENT, "repeats must be >= 1");
-  if (temperature_dependent(h->desc))
-    return fail(MPDG_ERR_UNSUPPORTED,
-                "the tangent of a temperature-dependent material has one launch form (no tuning)");
+  if (generic_cells(h->desc))
+    return fail(MPDG_ERR_UNSUPPORTED, "the tangent of a temperature-dependent material or of a "
+                                      "nodal velocity field has one launch form (no tuning)");
   const bool fp32 = precision == MPDG_FP32;
   if (mpdg_status s = ensure_workspace(h, fp32))
     return s;
@@ -2219,6 +2258,8 @@ mpdg_status mpdg_heat_batch_create(const mpdg_heat_desc *descs, int count, mpdg_
                   "batch members must share material kind, conductivity and velocity fields and stream");
   if (descs[0].material != MPDG_MATERIAL_GAS)
     return fail(MPDG_ERR_UNSUPPORTED, "batches are built for the gas material only");
+  if (nodal_velocity(descs[0]))
+    return fail(MPDG_ERR_UNSUPPORTED, "batches are not built for nodal velocity fields");
   for (int s = 0; s < count; ++s)
     if ((descs[s].convection_faces | descs[s].radiation_faces) & 0x3Fu)
       return fail(MPDG_ERR_UNSUPPORTED, "batches of operators with Robin faces are not built");

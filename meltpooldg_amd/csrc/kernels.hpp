@@ -109,6 +109,15 @@ namespace mpdg
     unsigned pad[3];
   };
 
+  // Velocity of the advection term: none, the separable tables of CellParams
+  // (vortex or constant), or a nodal DoF-vector field
+  enum VelocityMode
+  {
+    kVelNone = 0,
+    kVelTables = 1,
+    kVelNodal = 2
+  };
+
   enum CoefMode
   {
     kCoefConstant = 0,
@@ -201,28 +210,35 @@ namespace mpdg
   // Residual right-hand side (rhs_kernels.hip): dst += -(rho_cp/dt M (T - T_old)
   // - M_1 f + K T + rho_cp C_u T) on the free rows, T, T_old, f read plain
   // (constrained values included); heat_source may be null.
-  hipError_t launch_create_rhs(int degree, int coef, bool vel, const CellParams<double> &prm,
+  // vmode: kVelNone, kVelTables (prm.vel_* tables and w0) or kVelNodal
+  // (velocity: three component-blocked nodal vectors).
+  hipError_t launch_create_rhs(int degree, int coef, int vmode, const CellParams<double> &prm,
                                double *dst, const double *temperature,
                                const double *temperature_old, const double *heat_source,
-                               const double *heaviside, hipStream_t stream);
+                               const double *heaviside, const double *velocity,
+                               hipStream_t stream);
 
   // *out = sum over cells and Gauss points of JxW u^2 (FE L2 norm squared);
   // partials needs l2_norm_blocks() slots, *ticket must start at 0 (reset on exit).
   int l2_norm_blocks(int degree, int nx, int ny, int nz);
-  // Tangent of a temperature-dependent material (material_kernels.hip), FP64:
-  // dst = J src over 8 colour passes (coloured first-touch scatter), the
-  // material and its temperature derivatives evaluated at T(q) (T and T_old
-  // read plain, heaviside when prm.tmat == 3). Same p.Ap / stop contract as
-  // launch_vmult.
-  hipError_t launch_tmat_vmult(int degree, bool vel, const CellParams<double> &prm,
+  // Tangent with the coefficients evaluated per quadrature point from nodal
+  // fields (material_kernels.hip), FP64: a temperature-dependent material
+  // (prm.tmat >= 2: the material and its temperature derivatives at T(q), T
+  // and T_old read plain) or a constant-density one, the heaviside when given,
+  // and the velocity of vmode (kVelNodal: three component-blocked nodal
+  // vectors). dst = J src over 8 colour passes (coloured first-touch scatter).
+  // Same p.Ap / stop contract as launch_vmult.
+  hipError_t launch_tmat_vmult(int degree, int vmode, const CellParams<double> &prm,
                                const double *src, double *dst, const double *temperature,
                                const double *temperature_old, const double *heaviside,
-                               double *partials, SolverScalars *scal, hipStream_t stream);
+                               const double *velocity, double *partials, SolverScalars *scal,
+                               hipStream_t stream);
   // Diagonal of the same tangent (not inverted), dst overwritten.
-  hipError_t launch_tmat_diagonal(int degree, bool vel, const CellParams<double> &prm,
+  hipError_t launch_tmat_diagonal(int degree, int vmode, const CellParams<double> &prm,
                                   const DiagTables<double> &tab, double *dst,
                                   const double *temperature, const double *temperature_old,
-                                  const double *heaviside, hipStream_t stream);
+                                  const double *heaviside, const double *velocity,
+                                  hipStream_t stream);
   // Workgroup partial slots of launch_tmat_vmult.
   int tmat_blocks(int degree, int nx, int ny, int nz);
 

@@ -18,6 +18,13 @@
 // applied with the derivative of u). Everything is recomputed per launch from
 // the three nodal fields, so nothing per quadrature point is stored in HBM.
 //
+// The same kernels run the constant-density materials (gas, gas_liquid: the
+// material of the heaviside alone, no temperature terms) when the velocity is
+// a nodal DoF-vector field (VM == kVelNodal): its three components are read
+// plain at the cell nodes and interpolated to the Gauss points
+// (vel_eval.read_dof_values_plain + evaluate(values),
+// heat_diffuse_operator.cpp:975-980), and w3 rho_cp v_d is kept per point.
+//
 // Block-synchronous layout, as the residual kernel: one workgroup per row
 // segment of CPB cells of one colour (thread = cell x line), 1D contractions
 // in registers with the compile-time even-odd tables, LDS between the line
@@ -31,24 +38,33 @@ namespace mpdg
 {
   namespace
   {
-    constexpr int tmat_cpb(int P) { return P <= 2 ? 32 : (P == 3 ? 16 : (P <= 5 ? 8 : 4)); }
+    // cells per workgroup: eight cell tensors live in LDS, eleven with a nodal
+    // velocity (then half as many cells from degree 5 on)
+    constexpr int tmat_cpb(int P, int VM = 0)
+    {
+      return (P <= 2 ? 32 : (P == 3 ? 16 : (P <= 5 ? 8 : 4))) / (VM == kVelNodal && P >= 5 ? 2 : 1);
+    }
 
-    template <int P>
+    template <int P, int VM>
     constexpr int tmat_threads()
     {
-      return ((tmat_cpb(P) * (P + 1) * (P + 1) + 63) / 64) * 64;
+      return ((tmat_cpb(P, VM) * (P + 1) * (P + 1) + 63) / 64) * 64;
     }
 
     // Coefficients of the linearised material at the Gauss points of the
     // segment's cells (layout of each cell tensor: (qz * N + qy) * N + qx).
     // Every thread of the workgroup must call it; ends with a barrier.
-    template <int P, bool VEL, int CPB>
+    template <int P, int VM, int CPB>
     __device__ __forceinline__ void tmat_coefficients(const CellParams<double> &prm,
                                                       const Segment &g, const double *T,
                                                       const double *To, const double *heav,
-                                                      double *CM, double *CK, double *CR,
-                                                      double *CB0, double *CB1, double *CB2)
+                                                      const double *vel, double *CM, double *CK,
+                                                      double *CR, double *CB0, double *CB1,
+                                                      double *CB2, double *V0, double *V1,
+                                                      double *V2)
     {
+      if constexpr (VM == kVelNodal)
+        nodal_vector_at_q<P, CPB>(prm, g, vel, V0, V1, V2);
       constexpr int N = P + 1;
       constexpr int NL = N * N;
       constexpr int STR = lds_stride<N>();
@@ -62,7 +78,8 @@ namespace mpdg
       const int s0 = s % N, s1 = s / N;
       const bool cell_ok = active && l < g.ncell;
       const int cx = min(g.cx0 + l, prm.nx - 1);
-      const bool use_h = prm.tmat == 3;
+      const bool use_h = heav != nullptr;        // gas_liquid, gas_liquid_solid
+      const bool use_t = prm.tmat >= 2;           // temperature-dependent material
       // staging: T -> CK, T_old -> CM, H -> CR, grad T -> CB0..2
       double *Tl = CK + l * STR, *Ol = CM + l * STR, *Hl = CR + l * STR;
       double *G0 = CB0 + l * STR, *G1 = CB1 + l * STR, *G2 = CB2 + l * STR;
@@ -76,8 +93,8 @@ namespace mpdg
 #pragma unroll
           for (int a = 0; a < N; ++a)
             {
-              tl[a] = cell_ok ? T[base + a] : 0.0;
-              ol[a] = cell_ok ? To[base + a] : 0.0;
+              tl[a] = (use_t && cell_ok) ? T[base + a] : 0.0;
+              ol[a] = (use_t && cell_ok) ? To[base + a] : 0.0;
               hl[a] = (use_h && cell_ok) ? heav[base + a] : 0.0;
             }
           const int row = (s1 * N + s0) * N;
@@ -181,23 +198,36 @@ namespace mpdg
         {
           const double wxy = T1.w[s0] * T1.w[s1];
           double ux = 0.0, uy = 0.0;
-          if constexpr (VEL)
+          if constexpr (VM == kVelTables)
             {
               ux = prm.vel_x[0][cx * N + s0] * prm.vel_y[0][g.cy * N + s1];
               uy = prm.vel_x[1][cx * N + s0] * prm.vel_y[1][g.cy * N + s1];
             }
+          double *W0 = V0 + l * STR, *W1 = V1 + l * STR, *W2 = V2 + l * STR;
 #pragma unroll
           for (int q = 0; q < N; ++q)
             {
               const int idx = (q * N + s1) * N + s0;
               const double Tq = Tl[idx], Oq = Ol[idx], Hq = use_h ? Hl[idx] : 0.0;
               const double g0 = G0[idx], g1 = G1[idx], g2 = G2[idx];
-              double rc, k, drc, dk;
-              solid_liquid_material(prm, Hq, Tq, rc, k, drc, dk);
+              double rc, k, drc = 0.0, dk = 0.0;
+              if (use_t)
+                solid_liquid_material(prm, Hq, Tq, rc, k, drc, dk);
+              else
+                two_phase(prm, Hq, rc, k);
               const double w3 = T1.w[q] * wxy;
               double m = rc * prm.dtinv_vol + drc * prm.dtinv_vol * (Tq - Oq);
-              if constexpr (VEL)
+              if constexpr (VM == kVelTables)
                 m += drc * (prm.vf[0] * ux * g0 + prm.vf[1] * uy * g1 + prm.vf[2] * prm.w0 * g2);
+              if constexpr (VM == kVelNodal)
+                {
+                  const double vx = W0[idx], vy = W1[idx], vz = W2[idx];
+                  m += drc * (prm.vf[0] * vx * g0 + prm.vf[1] * vy * g1 + prm.vf[2] * vz * g2);
+                  // the advection coefficients w3 rho_cp v_d of the cell operator
+                  W0[idx] = w3 * rc * vx;
+                  W1[idx] = w3 * rc * vy;
+                  W2[idx] = w3 * rc * vz;
+                }
               Ol[idx] = w3 * m;
               Tl[idx] = w3 * k;
               Hl[idx] = w3 * rc;
@@ -212,14 +242,16 @@ namespace mpdg
     // dst = J src, one colour pass; partial p.Ap of the pass when partials is
     // given (then scal receives p.Ap); a given scal alone only stops the launch
     // once scal->done is set.
-    template <int P, bool VEL, int CPB, int THREADS>
+    template <int P, int VM, int CPB, int THREADS>
     __global__ void __launch_bounds__(THREADS)
       tmat_vmult_kernel(const double *__restrict__ src, double *__restrict__ dst,
                         const double *__restrict__ T, const double *__restrict__ To,
-                        const double *__restrict__ heav, const CellParams<double> prm,
-                        const int colour, double *__restrict__ partials, const int offset,
-                        const int total, const int finalize, SolverScalars *scal)
+                        const double *__restrict__ heav, const double *__restrict__ vel,
+                        const CellParams<double> prm, const int colour,
+                        double *__restrict__ partials, const int offset, const int total,
+                        const int finalize, SolverScalars *scal)
     {
+      constexpr bool VEL = VM != kVelNone;
       if (scal != nullptr && scal->done)
         return; // uniform over the grid: done is not written during the launch
       constexpr int N = P + 1;
@@ -234,10 +266,13 @@ namespace mpdg
       __shared__ double B[CPB * STR];  // q-space accumulator of the x and y passes
       __shared__ double CM[CPB * STR], CK[CPB * STR], CR[CPB * STR];
       __shared__ double CB0[CPB * STR], CB1[CPB * STR], CB2[CPB * STR];
+      __shared__ double V0[VM == kVelNodal ? CPB * STR : 1], V1[VM == kVelNodal ? CPB * STR : 1],
+        V2[VM == kVelNodal ? CPB * STR : 1];
       __shared__ double XE[CPB * NL];
 
       const Segment g = segment_of<double, CPB>(prm, colour);
-      tmat_coefficients<P, VEL, CPB>(prm, g, T, To, heav, CM, CK, CR, CB0, CB1, CB2);
+      tmat_coefficients<P, VM, CPB>(prm, g, T, To, heav, vel, CM, CK, CR, CB0, CB1, CB2, V0, V1,
+                                    V2);
 
       const int t = threadIdx.x;
       const bool active = t < CPB * NL;
@@ -246,6 +281,8 @@ namespace mpdg
       const int s0 = s % N, s1 = s / N;
       const bool cell_ok = active && l < g.ncell;
       const int cx = min(g.cx0 + l, prm.nx - 1);
+      const int vo = VM == kVelNodal ? l * STR : 0;
+      const double *W0 = V0 + vo, *W1 = V1 + vo, *W2 = V2 + vo; // w3 rho_cp v_d at q
       double *Al = A + l * STR;
       double *Bl = B + l * STR;
       const double *Ml = CM + l * STR, *Kl = CK + l * STR, *Rl = CR + l * STR;
@@ -314,11 +351,13 @@ namespace mpdg
               const int idx = (s1 * N + s0) * N + q;
               f[q] = prm.kf[0] * (Kl[idx] * gx[q] + E0[idx] * v[q]);
               cv[q] = 0.0;
-              if constexpr (VEL)
+              if constexpr (VM == kVelTables)
                 {
                   const double vx = prm.vel_x[0][cx * N + q] * prm.vel_y[0][g.cy * N + s0];
                   cv[q] = prm.vf[0] * Rl[idx] * vx * gx[q];
                 }
+              if constexpr (VM == kVelNodal)
+                cv[q] = prm.vf[0] * W0[idx] * gx[q];
             }
           eo_apply<N, -1>(TCt, f, o);
 #pragma unroll
@@ -341,11 +380,13 @@ namespace mpdg
               const int idx = (s1 * N + q) * N + s0;
               f[q] = prm.kf[1] * (Kl[idx] * gy[q] + E1[idx] * v[q]);
               cv[q] = 0.0;
-              if constexpr (VEL)
+              if constexpr (VM == kVelTables)
                 {
                   const double vy = prm.vel_x[1][cx * N + s0] * prm.vel_y[1][g.cy * N + q];
                   cv[q] = prm.vf[1] * Rl[idx] * vy * gy[q];
                 }
+              if constexpr (VM == kVelNodal)
+                cv[q] = prm.vf[1] * W1[idx] * gy[q];
             }
           eo_apply<N, -1>(TCt, f, o);
 #pragma unroll
@@ -373,8 +414,10 @@ namespace mpdg
             {
               const int idx = (q * N + s1) * N + s0;
               double val = Ml[idx] * U[q];
-              if constexpr (VEL)
+              if constexpr (VM == kVelTables)
                 val += prm.vf[2] * Rl[idx] * prm.w0 * Gz[q];
+              if constexpr (VM == kVelNodal)
+                val += prm.vf[2] * W2[idx] * Gz[q];
               V[q] = val + o[q] + Bl[idx];
             }
           eo_apply<N, 1>(TSt, V, y1);
@@ -431,21 +474,27 @@ namespace mpdg
     // diag_i = sum_q [ CM phi_i^2 + sum_d kf_d CK (d_d phi_i)^2
     //                  + sum_d (vf_d CR v_d + kf_d CB_d) phi_i d_d phi_i ]
     // contracted direction by direction with S o S, D o D and S o D.
-    template <int P, bool VEL, int CPB, int THREADS>
+    template <int P, int VM, int CPB, int THREADS>
     __global__ void __launch_bounds__(THREADS)
       tmat_diag_kernel(double *__restrict__ dst, const double *__restrict__ T,
                        const double *__restrict__ To, const double *__restrict__ heav,
-                       const DiagTables<double> tab, const CellParams<double> prm, const int colour)
+                       const double *__restrict__ vel, const DiagTables<double> tab,
+                       const CellParams<double> prm, const int colour)
     {
       constexpr int N = P + 1;
       constexpr int NL = N * N;
       constexpr int STR = lds_stride<N>();
       __shared__ double CM[CPB * STR], CK[CPB * STR], CR[CPB * STR];
       __shared__ double CB0[CPB * STR], CB1[CPB * STR], CB2[CPB * STR];
+      __shared__ double V0[VM == kVelNodal ? CPB * STR : 1], V1[VM == kVelNodal ? CPB * STR : 1],
+        V2[VM == kVelNodal ? CPB * STR : 1];
       __shared__ double XE[CPB * NL];
 
       const Segment g = segment_of<double, CPB>(prm, colour);
-      tmat_coefficients<P, VEL, CPB>(prm, g, T, To, heav, CM, CK, CR, CB0, CB1, CB2);
+      tmat_coefficients<P, VM, CPB>(prm, g, T, To, heav, vel, CM, CK, CR, CB0, CB1, CB2, V0, V1,
+                                    V2);
+      const int vo0 = VM == kVelNodal ? (int)(threadIdx.x % CPB) * STR : 0;
+      const double *W0 = V0 + vo0, *W1 = V1 + vo0, *W2 = V2 + vo0; // w3 rho_cp v_d at q
 
       const int t = threadIdx.x;
       const bool active = t < CPB * NL;
@@ -469,19 +518,23 @@ namespace mpdg
               const int idx = (s1 * N + s0) * N + q;
               const double k = Bk[idx];
               double vx = 0.0, vy = 0.0, vz = 0.0;
-              if constexpr (VEL)
+              if constexpr (VM == kVelTables)
                 {
                   vx = prm.vel_x[0][cx * N + q] * prm.vel_y[0][g.cy * N + s0];
                   vy = prm.vel_x[1][cx * N + q] * prm.vel_y[1][g.cy * N + s0];
                   vz = prm.w0;
                 }
+              // w3 rho_cp v_d: the tables' velocity times Cr, or the nodal one
+              const double a0 = VM == kVelNodal ? W0[idx] : Cr[idx] * vx;
+              const double a1 = VM == kVelNodal ? W1[idx] : Cr[idx] * vy;
+              const double a2 = VM == kVelNodal ? W2[idx] : Cr[idx] * vz;
               Fm[q] = Am[idx];
               Fx2[q] = prm.kf[0] * k;
-              Fxs[q] = prm.vf[0] * Cr[idx] * vx + prm.kf[0] * D0[idx];
+              Fxs[q] = prm.vf[0] * a0 + prm.kf[0] * D0[idx];
               Fky[q] = prm.kf[1] * k;
               Fkz[q] = prm.kf[2] * k;
-              Fvy[q] = prm.vf[1] * Cr[idx] * vy + prm.kf[1] * D1[idx];
-              Fvz[q] = prm.vf[2] * Cr[idx] * vz + prm.kf[2] * D2[idx];
+              Fvy[q] = prm.vf[1] * a1 + prm.kf[1] * D1[idx];
+              Fvz[q] = prm.vf[2] * a2 + prm.kf[2] * D2[idx];
             }
           double g1[N], t1[N], g2[N], g3[N], g4[N], g5[N];
           eo_apply<N, 1>(tab.S2t, Fm, g1);
@@ -591,92 +644,113 @@ namespace mpdg
         }
     }
 
-    template <int P, bool VEL>
+    template <int P, int VM>
     hipError_t tmat_vmult_passes(const CellParams<double> &prm, const double *src, double *dst,
                                  const double *T, const double *To, const double *heav,
-                                 double *partials, SolverScalars *scal, hipStream_t s)
+                                 const double *vel, double *partials, SolverScalars *scal,
+                                 hipStream_t s)
     {
-      constexpr int CPB = tmat_cpb(P);
-      constexpr int THREADS = tmat_threads<P>();
+      constexpr int CPB = tmat_cpb(P, VM);
+      constexpr int THREADS = tmat_threads<P, VM>();
       const ColourGrid cg = colour_grid(prm.nx, prm.ny, prm.nz, CPB);
       for (int c = 0; c < 8; ++c)
         {
           if (cg.blocks[c] == 0)
             continue;
-          tmat_vmult_kernel<P, VEL, CPB, THREADS><<<cg.blocks[c], THREADS, 0, s>>>(
-            src, dst, T, To, heav, prm, c, partials, cg.offset[c], cg.total, c == cg.last ? 1 : 0,
-            scal);
+          tmat_vmult_kernel<P, VM, CPB, THREADS><<<cg.blocks[c], THREADS, 0, s>>>(
+            src, dst, T, To, heav, vel, prm, c, partials, cg.offset[c], cg.total,
+            c == cg.last ? 1 : 0, scal);
         }
       return hipGetLastError();
     }
 
-    template <int P, bool VEL>
+    template <int P, int VM>
     hipError_t tmat_diag_passes(const CellParams<double> &prm, const DiagTables<double> &tab,
                                 double *dst, const double *T, const double *To,
-                                const double *heav, hipStream_t s)
+                                const double *heav, const double *vel, hipStream_t s)
     {
-      constexpr int CPB = tmat_cpb(P);
-      constexpr int THREADS = tmat_threads<P>();
+      constexpr int CPB = tmat_cpb(P, VM);
+      constexpr int THREADS = tmat_threads<P, VM>();
       const ColourGrid cg = colour_grid(prm.nx, prm.ny, prm.nz, CPB);
       for (int c = 0; c < 8; ++c)
         {
           if (cg.blocks[c] == 0)
             continue;
-          tmat_diag_kernel<P, VEL, CPB, THREADS><<<cg.blocks[c], THREADS, 0, s>>>(dst, T, To, heav,
-                                                                                 tab, prm, c);
+          tmat_diag_kernel<P, VM, CPB, THREADS><<<cg.blocks[c], THREADS, 0, s>>>(
+            dst, T, To, heav, vel, tab, prm, c);
         }
       return hipGetLastError();
+    }
+
+    template <int P>
+    hipError_t tmat_vmult_vm(int vmode, const CellParams<double> &prm, const double *src,
+                             double *dst, const double *T, const double *To, const double *heav,
+                             const double *vel, double *partials, SolverScalars *scal,
+                             hipStream_t s)
+    {
+      switch (vmode)
+        {
+          case kVelNone: return tmat_vmult_passes<P, kVelNone>(prm, src, dst, T, To, heav, vel, partials, scal, s);
+          case kVelTables: return tmat_vmult_passes<P, kVelTables>(prm, src, dst, T, To, heav, vel, partials, scal, s);
+          case kVelNodal: return tmat_vmult_passes<P, kVelNodal>(prm, src, dst, T, To, heav, vel, partials, scal, s);
+          default: return hipErrorInvalidValue;
+        }
+    }
+
+    template <int P>
+    hipError_t tmat_diag_vm(int vmode, const CellParams<double> &prm,
+                            const DiagTables<double> &tab, double *dst, const double *T,
+                            const double *To, const double *heav, const double *vel, hipStream_t s)
+    {
+      switch (vmode)
+        {
+          case kVelNone: return tmat_diag_passes<P, kVelNone>(prm, tab, dst, T, To, heav, vel, s);
+          case kVelTables: return tmat_diag_passes<P, kVelTables>(prm, tab, dst, T, To, heav, vel, s);
+          case kVelNodal: return tmat_diag_passes<P, kVelNodal>(prm, tab, dst, T, To, heav, vel, s);
+          default: return hipErrorInvalidValue;
+        }
     }
   } // namespace
 
   int tmat_blocks(int degree, int nx, int ny, int nz)
   {
-    return colour_grid(nx, ny, nz, tmat_cpb(degree)).total;
+    // the nodal-velocity kernels' smaller segments give the larger count
+    return colour_grid(nx, ny, nz, tmat_cpb(degree, kVelNodal)).total;
   }
 
-  hipError_t launch_tmat_vmult(int degree, bool vel, const CellParams<double> &prm,
+  hipError_t launch_tmat_vmult(int degree, int vmode, const CellParams<double> &prm,
                                const double *src, double *dst, const double *temperature,
                                const double *temperature_old, const double *heaviside,
-                               double *partials, SolverScalars *scal, hipStream_t stream)
+                               const double *velocity, double *partials, SolverScalars *scal,
+                               hipStream_t stream)
   {
-#define MPDG_TMAT_VMULT(P)                                                                        \
-  return vel ? tmat_vmult_passes<P, true>(prm, src, dst, temperature, temperature_old, heaviside, \
-                                          partials, scal, stream)                                 \
-             : tmat_vmult_passes<P, false>(prm, src, dst, temperature, temperature_old,          \
-                                           heaviside, partials, scal, stream)
     switch (degree)
       {
-        case 1: MPDG_TMAT_VMULT(1);
-        case 2: MPDG_TMAT_VMULT(2);
-        case 3: MPDG_TMAT_VMULT(3);
-        case 4: MPDG_TMAT_VMULT(4);
-        case 5: MPDG_TMAT_VMULT(5);
-        case 6: MPDG_TMAT_VMULT(6);
+        case 1: return tmat_vmult_vm<1>(vmode, prm, src, dst, temperature, temperature_old, heaviside, velocity, partials, scal, stream);
+        case 2: return tmat_vmult_vm<2>(vmode, prm, src, dst, temperature, temperature_old, heaviside, velocity, partials, scal, stream);
+        case 3: return tmat_vmult_vm<3>(vmode, prm, src, dst, temperature, temperature_old, heaviside, velocity, partials, scal, stream);
+        case 4: return tmat_vmult_vm<4>(vmode, prm, src, dst, temperature, temperature_old, heaviside, velocity, partials, scal, stream);
+        case 5: return tmat_vmult_vm<5>(vmode, prm, src, dst, temperature, temperature_old, heaviside, velocity, partials, scal, stream);
+        case 6: return tmat_vmult_vm<6>(vmode, prm, src, dst, temperature, temperature_old, heaviside, velocity, partials, scal, stream);
         default: return hipErrorInvalidValue;
       }
-#undef MPDG_TMAT_VMULT
   }
 
-  hipError_t launch_tmat_diagonal(int degree, bool vel, const CellParams<double> &prm,
+  hipError_t launch_tmat_diagonal(int degree, int vmode, const CellParams<double> &prm,
                                   const DiagTables<double> &tab, double *dst,
                                   const double *temperature, const double *temperature_old,
-                                  const double *heaviside, hipStream_t stream)
+                                  const double *heaviside, const double *velocity,
+                                  hipStream_t stream)
   {
-#define MPDG_TMAT_DIAG(P)                                                                         \
-  return vel ? tmat_diag_passes<P, true>(prm, tab, dst, temperature, temperature_old, heaviside, \
-                                         stream)                                                  \
-             : tmat_diag_passes<P, false>(prm, tab, dst, temperature, temperature_old, heaviside, \
-                                          stream)
     switch (degree)
       {
-        case 1: MPDG_TMAT_DIAG(1);
-        case 2: MPDG_TMAT_DIAG(2);
-        case 3: MPDG_TMAT_DIAG(3);
-        case 4: MPDG_TMAT_DIAG(4);
-        case 5: MPDG_TMAT_DIAG(5);
-        case 6: MPDG_TMAT_DIAG(6);
+        case 1: return tmat_diag_vm<1>(vmode, prm, tab, dst, temperature, temperature_old, heaviside, velocity, stream);
+        case 2: return tmat_diag_vm<2>(vmode, prm, tab, dst, temperature, temperature_old, heaviside, velocity, stream);
+        case 3: return tmat_diag_vm<3>(vmode, prm, tab, dst, temperature, temperature_old, heaviside, velocity, stream);
+        case 4: return tmat_diag_vm<4>(vmode, prm, tab, dst, temperature, temperature_old, heaviside, velocity, stream);
+        case 5: return tmat_diag_vm<5>(vmode, prm, tab, dst, temperature, temperature_old, heaviside, velocity, stream);
+        case 6: return tmat_diag_vm<6>(vmode, prm, tab, dst, temperature, temperature_old, heaviside, velocity, stream);
         default: return hipErrorInvalidValue;
       }
-#undef MPDG_TMAT_DIAG
   }
 } // namespace mpdg

@@ -22,13 +22,16 @@ namespace mpdg
 {
   namespace
   {
-    template <int P, int COEF, bool VEL, int CPB, int THREADS>
+    template <int P, int COEF, int VM, int CPB, int THREADS>
     __global__ void __launch_bounds__(THREADS)
       rhs_kernel(double *__restrict__ dst, const double *__restrict__ tnew,
                  const double *__restrict__ told, const double *__restrict__ fsrc,
-                 const double *__restrict__ heav, const CellParams<double> prm, const int colour)
+                 const double *__restrict__ heav, const double *__restrict__ vel,
+                 const CellParams<double> prm, const int colour)
     {
       using Number = double;
+      constexpr bool VEL = VM != kVelNone;
+      constexpr bool NV = VM == kVelNodal;
       constexpr int N = P + 1;
       constexpr int NL = N * N;
       constexpr int STR = lds_stride<N>();
@@ -46,8 +49,14 @@ namespace mpdg
       __shared__ Number KB[QC ? CPB * STR : 1];
       __shared__ Number RB[QC ? CPB * STR : 1];
       __shared__ Number XE[CPB * NL];
+      // nodal velocity at q (kVelNodal)
+      __shared__ Number VX[NV ? CPB * STR : 1], VY[NV ? CPB * STR : 1], VZ[NV ? CPB * STR : 1];
 
       const Segment g = segment_of<Number, CPB>(prm, colour);
+      if constexpr (NV)
+        nodal_vector_at_q<P, CPB>(prm, g, vel, VX, VY, VZ);
+      const int vo = NV ? (int)(threadIdx.x % CPB) * STR : 0;
+      const Number *VXl = VX + vo, *VYl = VY + vo, *VZl = VZ + vo;
       // the heaviside is read for gas_liquid and gas_liquid_solid
       const bool use_h = COEF == kCoefHeaviside || (COEF == kCoefTemperature && prm.tmat == 3);
       const int t = threadIdx.x;
@@ -197,7 +206,8 @@ namespace mpdg
               if constexpr (VEL)
                 {
                   const Number rc = QC ? RBl[(s1 * N + s0) * N + q] : prm.rho_cp0;
-                  const Number vx = prm.vel_x[0][cx * N + q] * prm.vel_y[0][g.cy * N + s0];
+                  const Number vx = NV ? VXl[(s1 * N + s0) * N + q] :
+                                         prm.vel_x[0][cx * N + q] * prm.vel_y[0][g.cy * N + s0];
                   cv[q] = -rc * w3 * prm.vf[0] * vx * gx[q];
                 }
             }
@@ -226,7 +236,8 @@ namespace mpdg
               if constexpr (VEL)
                 {
                   const Number rc = QC ? RBl[(s1 * N + q) * N + s0] : prm.rho_cp0;
-                  const Number vy = prm.vel_x[1][cx * N + s0] * prm.vel_y[1][g.cy * N + q];
+                  const Number vy = NV ? VYl[(s1 * N + q) * N + s0] :
+                                         prm.vel_x[1][cx * N + s0] * prm.vel_y[1][g.cy * N + q];
                   cv[q] = -rc * w3 * prm.vf[1] * vy * gy[q];
                 }
             }
@@ -261,7 +272,7 @@ namespace mpdg
               const Number mass = QC ? rc * prm.dtinv_vol : prm.mass;
               Number val = -(mass * w3 * dq[q] - prm.vol * w3 * fq[q]);
               if constexpr (VEL)
-                val -= rc * w3 * prm.vf[2] * prm.w0 * Gz[q];
+                val -= rc * w3 * prm.vf[2] * (NV ? VZl[(q * N + s1) * N + s0] : prm.w0) * Gz[q];
               V[q] = val + o[q] + Bl[(q * N + s1) * N + s0];
             }
           eo_apply<N, 1>(TSt, V, y1);
@@ -425,38 +436,52 @@ namespace mpdg
       return hipGetLastError();
     }
 
-    template <int P, int COEF, bool VEL>
-    hipError_t rhs_passes(const CellParams<double> &prm, double *dst, const double *tn,
-                          const double *to, const double *f, const double *heav, hipStream_t s)
+    // cells per workgroup: three more cell tensors in LDS with a nodal
+    // velocity (then half as many cells from degree 4 on)
+    template <int P, int VM>
+    constexpr int rhs_cpb()
     {
-      constexpr int CPB = cells_per_block_for(P, false);
+      return cells_per_block_for(P, false) / (VM == kVelNodal && P >= 4 ? 2 : 1);
+    }
+
+    template <int P, int COEF, int VM>
+    hipError_t rhs_passes(const CellParams<double> &prm, double *dst, const double *tn,
+                          const double *to, const double *f, const double *heav,
+                          const double *vel, hipStream_t s)
+    {
+      constexpr int CPB = rhs_cpb<P, VM>();
       constexpr int THREADS = ((CPB * (P + 1) * (P + 1) + 63) / 64) * 64;
       const ColourGrid cg = colour_grid(prm.nx, prm.ny, prm.nz, CPB);
       for (int c = 0; c < 8; ++c)
         {
           if (cg.blocks[c] == 0)
             continue;
-          rhs_kernel<P, COEF, VEL, CPB, THREADS><<<cg.blocks[c], THREADS, 0, s>>>(dst, tn, to, f,
-                                                                                 heav, prm, c);
+          rhs_kernel<P, COEF, VM, CPB, THREADS><<<cg.blocks[c], THREADS, 0, s>>>(dst, tn, to, f,
+                                                                                heav, vel, prm, c);
         }
       return hipGetLastError();
     }
 
     template <int P>
-    hipError_t rhs_deg(int coef, bool vel, const CellParams<double> &prm, double *dst,
+    hipError_t rhs_deg(int coef, int vmode, const CellParams<double> &prm, double *dst,
                        const double *tn, const double *to, const double *f, const double *heav,
-                       hipStream_t s)
+                       const double *vel, hipStream_t s)
     {
-      switch (coef * 2 + (vel ? 1 : 0))
+      // the separable-sine conductivity has no nodal-velocity form (creation
+      // refuses that combination)
+      switch (coef * 3 + vmode)
         {
-          case 0: return rhs_passes<P, 0, false>(prm, dst, tn, to, f, heav, s);
-          case 1: return rhs_passes<P, 0, true>(prm, dst, tn, to, f, heav, s);
-          case 2: return rhs_passes<P, 1, false>(prm, dst, tn, to, f, heav, s);
-          case 3: return rhs_passes<P, 1, true>(prm, dst, tn, to, f, heav, s);
-          case 4: return rhs_passes<P, 2, false>(prm, dst, tn, to, f, heav, s);
-          case 5: return rhs_passes<P, 2, true>(prm, dst, tn, to, f, heav, s);
-          case 6: return rhs_passes<P, 3, false>(prm, dst, tn, to, f, heav, s);
-          case 7: return rhs_passes<P, 3, true>(prm, dst, tn, to, f, heav, s);
+          case 0: return rhs_passes<P, 0, kVelNone>(prm, dst, tn, to, f, heav, vel, s);
+          case 1: return rhs_passes<P, 0, kVelTables>(prm, dst, tn, to, f, heav, vel, s);
+          case 2: return rhs_passes<P, 0, kVelNodal>(prm, dst, tn, to, f, heav, vel, s);
+          case 3: return rhs_passes<P, 1, kVelNone>(prm, dst, tn, to, f, heav, vel, s);
+          case 4: return rhs_passes<P, 1, kVelTables>(prm, dst, tn, to, f, heav, vel, s);
+          case 6: return rhs_passes<P, 2, kVelNone>(prm, dst, tn, to, f, heav, vel, s);
+          case 7: return rhs_passes<P, 2, kVelTables>(prm, dst, tn, to, f, heav, vel, s);
+          case 8: return rhs_passes<P, 2, kVelNodal>(prm, dst, tn, to, f, heav, vel, s);
+          case 9: return rhs_passes<P, 3, kVelNone>(prm, dst, tn, to, f, heav, vel, s);
+          case 10: return rhs_passes<P, 3, kVelTables>(prm, dst, tn, to, f, heav, vel, s);
+          case 11: return rhs_passes<P, 3, kVelNodal>(prm, dst, tn, to, f, heav, vel, s);
           default: return hipErrorInvalidValue;
         }
     }
@@ -483,19 +508,20 @@ namespace mpdg
       }
   }
 
-  hipError_t launch_create_rhs(int degree, int coef, bool vel, const CellParams<double> &prm,
+  hipError_t launch_create_rhs(int degree, int coef, int vmode, const CellParams<double> &prm,
                                double *dst, const double *temperature,
                                const double *temperature_old, const double *heat_source,
-                               const double *heaviside, hipStream_t stream)
+                               const double *heaviside, const double *velocity,
+                               hipStream_t stream)
   {
     switch (degree)
       {
-        case 1: return rhs_deg<1>(coef, vel, prm, dst, temperature, temperature_old, heat_source, heaviside, stream);
-        case 2: return rhs_deg<2>(coef, vel, prm, dst, temperature, temperature_old, heat_source, heaviside, stream);
-        case 3: return rhs_deg<3>(coef, vel, prm, dst, temperature, temperature_old, heat_source, heaviside, stream);
-        case 4: return rhs_deg<4>(coef, vel, prm, dst, temperature, temperature_old, heat_source, heaviside, stream);
-        case 5: return rhs_deg<5>(coef, vel, prm, dst, temperature, temperature_old, heat_source, heaviside, stream);
-        case 6: return rhs_deg<6>(coef, vel, prm, dst, temperature, temperature_old, heat_source, heaviside, stream);
+        case 1: return rhs_deg<1>(coef, vmode, prm, dst, temperature, temperature_old, heat_source, heaviside, velocity, stream);
+        case 2: return rhs_deg<2>(coef, vmode, prm, dst, temperature, temperature_old, heat_source, heaviside, velocity, stream);
+        case 3: return rhs_deg<3>(coef, vmode, prm, dst, temperature, temperature_old, heat_source, heaviside, velocity, stream);
+        case 4: return rhs_deg<4>(coef, vmode, prm, dst, temperature, temperature_old, heat_source, heaviside, velocity, stream);
+        case 5: return rhs_deg<5>(coef, vmode, prm, dst, temperature, temperature_old, heat_source, heaviside, velocity, stream);
+        case 6: return rhs_deg<6>(coef, vmode, prm, dst, temperature, temperature_old, heat_source, heaviside, velocity, stream);
         default: return hipErrorInvalidValue;
       }
   }

@@ -152,7 +152,7 @@ def _make_desc(n_cells, degree, material, box, precision, conductivity_amplitude
     d.apparent_capacity_type = APPARENT_CAPACITY_TYPES[mat.apparent_capacity_type]
     d.conductivity_field = 1 if conductivity_amplitude else 0
     d.conductivity_amplitude = float(conductivity_amplitude)
-    d.velocity = {None: 0, "none": 0, "vortex": 1, "constant": 2}[velocity]
+    d.velocity = {None: 0, "none": 0, "vortex": 1, "constant": 2, "nodal": 3}[velocity]
     d.velocity_w0 = float(velocity_w0)
     if velocity == "constant":
         v = tuple(float(c) for c in velocity_value)
@@ -197,7 +197,8 @@ class HeatDiffuseOperator:
     parameters ``convection`` (ConvectionBC) and ``radiation`` (RadiationBC);
     radiation makes the tangent depend on the temperature registered with
     ``set_temperature``. ``velocity``: None, "vortex" (analytic at q, z component
-    ``velocity_w0``) or "constant" (``velocity_value``, a prescribed_velocity
+    ``velocity_w0``), "nodal" (a DoF-vector field given by ``set_velocity``,
+    FP64) or "constant" (``velocity_value``, a prescribed_velocity
     field constant in space).
     """
 
@@ -312,6 +313,18 @@ class HeatDiffuseOperator:
             nz = h.nonzero()[0]
             iterations = int(nz[-1]) if nz.size else 0
         return h[: iterations + 1].copy()
+
+    def set_velocity(self, velocity):
+        """The nodal velocity of a ``velocity="nodal"`` operator (the DoF vector
+        the reference operator reads with vel_eval.read_dof_values_plain,
+        heat_diffuse_operator.cpp:975-980): an FP64 device tensor of 3 n_dofs()
+        values, component-blocked [v_x | v_y | v_z] in lattice numbering, or
+        of shape (3, n_dofs()). The engine copies it."""
+        v = velocity.reshape(-1)
+        if not (getattr(v, "is_cuda", False) and str(v.dtype).endswith("float64")
+                and v.is_contiguous() and v.numel() == 3 * self.n_dofs()):
+            raise ValueError("velocity must be a contiguous FP64 GPU tensor of 3 n_dofs() values")
+        L.check(self._lib.mpdg_heat_set_velocity(self._h, _ptr(v)))
 
     def set_heaviside(self, heaviside):
         """Register the level-set heaviside field (read plain, interpolated at q)."""

@@ -48,21 +48,24 @@ def material(H, mat):
 
 
 def assemble(n, p, dt, mat, heaviside=None, k_amplitude=0.0, velocity=False, w0=0.2,
-             lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), dirichlet_faces=0x3F, velocity_value=None):
+             lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), dirichlet_faces=0x3F, velocity_value=None,
+             velocity_field=None):
     """Return (A_full, mask): dense lattice matrix without constraints and the
     constrained-DoF mask. The constrained operator is A_full restricted to free
     rows/columns, zero elsewhere."""
     A, _, _, mask = assemble_parts(n, p, dt, mat, heaviside, k_amplitude, velocity, w0, lo, hi,
-                                   dirichlet_faces, velocity_value)
+                                   dirichlet_faces, velocity_value, velocity_field)
     return A, mask
 
 
 def assemble_parts(n, p, dt, mat, heaviside=None, k_amplitude=0.0, velocity=False, w0=0.2,
                    lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), dirichlet_faces=0x3F,
-                   velocity_value=None):
+                   velocity_value=None, velocity_field=None):
     """(A_full, M_rc, M_1, mask): the full tangent, its mass part rho_cp/dt M and
     the unit mass matrix M_1 (integral of phi_i phi_j), all without constraints.
-    velocity: False, True / "vortex", or "constant" with velocity_value."""
+    velocity: False, True / "vortex", "constant" with velocity_value, or
+    "nodal" with velocity_field (3 N nodal values, component-blocked, evaluated
+    at q with this module's own basis)."""
     n = (n, n, n) if np.isscalar(n) else tuple(n)
     n1 = lattice_shape(n, p)
     N = n1[0] * n1[1] * n1[2]
@@ -104,6 +107,8 @@ def assemble_parts(n, p, dt, mat, heaviside=None, k_amplitude=0.0, velocity=Fals
                     Aloc += G.T @ ((W * k)[:, None] * G)
                 if velocity == "constant":
                     vx, vy, vz = (np.full_like(Xq, c) for c in velocity_value)
+                elif velocity == "nodal":
+                    vx, vy, vz = (Phi @ velocity_field[c * N + dofs] for c in range(3))
                 elif velocity:
                     vx = -np.sin(np.pi * Xq) ** 2 * np.sin(2 * np.pi * Yq)
                     vy = np.sin(2 * np.pi * Xq) * np.sin(np.pi * Yq) ** 2

@@ -50,6 +50,7 @@ class OracleProblem(ctypes.Structure):
         ("latent_heat", ctypes.c_double),
         ("apparent_capacity", ctypes.c_int),
         ("temperature_old", ctypes.POINTER(ctypes.c_double)),
+        ("velocity_field", ctypes.POINTER(ctypes.c_double)),
     ]
 
 APPARENT_CAPACITY_TYPES = {"poly4_bell": 0, "constant": 1, "qlq": 2}
@@ -161,7 +162,7 @@ class Problem:
     def __init__(self, n, degree, dt=1e-3, material=None, k_amplitude=0.0, velocity=False,
                  w0=0.2, heaviside=None, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0),
                  dirichlet_faces=0x3F, robin=None, temperature=None, velocity_value=None,
-                 temperature_old=None):
+                 temperature_old=None, velocity_field=None):
         mat = material or {"type": "gas", "gas_rho": 1.0, "gas_cp": 1.0, "gas_k": 1.0}
         n = (n, n, n) if np.isscalar(n) else tuple(n)
         s = OracleProblem()
@@ -173,8 +174,9 @@ class Problem:
         _fill_material(s, mat)
         s.conductivity_field = 1 if k_amplitude else 0
         s.k_amplitude = k_amplitude
-        # velocity: False/None, True or "vortex", "constant" (velocity_value)
-        s.velocity = 2 if velocity == "constant" else (1 if velocity else 0)
+        # velocity: False/None, True or "vortex", "constant" (velocity_value),
+        # "nodal" (velocity_field: 3 n_dofs values, component-blocked)
+        s.velocity = {"constant": 2, "nodal": 3}.get(velocity, 1 if velocity else 0)
         s.w0 = w0
         if velocity == "constant":
             s.velocity_value[:] = [float(c) for c in velocity_value]
@@ -189,6 +191,10 @@ class Problem:
         self._T = None
         self._To = None
         self._h = None
+        self._v = None
+        if velocity == "nodal":
+            self._v = np.ascontiguousarray(velocity_field, dtype=np.float64)
+            s.velocity_field = _dp(self._v)
         if heaviside is not None:
             self._h = np.ascontiguousarray(heaviside, dtype=np.float64)
             s.heaviside = _dp(self._h)

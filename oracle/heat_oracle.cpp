@@ -207,6 +207,8 @@ namespace
       return false;
     if ((pb->material == 1 || pb->material == 3) && !pb->heaviside)
       return false;
+    if (pb->velocity < 0 || pb->velocity > 3 || (pb->velocity == 3 && !pb->velocity_field))
+      return false;
     if (pb->material >= 2)
       {
         if (pb->consistent_with_evaporation)
@@ -459,6 +461,14 @@ namespace
     v[2] = pb->w0;
   }
 
+  // Nodal velocity (velocity == 3) at the Gauss points of one cell: each
+  // component's nodal values read plain and interpolated with S in x, y, z
+  // (vel_eval.read_dof_values_plain + evaluate(values)); defined after the
+  // cell helpers it uses.
+  struct CellGeom;
+  void nodal_velocity_q(const oracle_problem *pb, const struct Tables &t, const struct Mesh &m,
+                        const CellGeom &g, double (*Vq)[MAXN * MAXN * MAXN]);
+
   // 1D contraction along one tensor direction of a local n^3 array
   // (index (c*n+b)*n+a, a = x fastest). out = M in along `dir`, or M^T.
   template <bool transpose, int n, int dir>
@@ -597,6 +607,10 @@ namespace
     const double jac_det = hx * hy * hz;
     const double dt_inv = 1. / pb->dt;
 
+    double Vq[3][MAXN * MAXN * MAXN];
+    if (pb->velocity == 3)
+      nodal_velocity_q(pb, t, m, g, Vq);
+
     double V[MAXN * MAXN * MAXN], Fx[MAXN * MAXN * MAXN], Fy[MAXN * MAXN * MAXN],
       Fz[MAXN * MAXN * MAXN];
     for (int qz = 0; qz < n; ++qz)
@@ -618,7 +632,14 @@ namespace
             double val = dt_inv * rho_cp * U[q];
             double fx = k * gx, fy = k * gy, fz = k * gz;
             double v[3] = {0.0, 0.0, 0.0};
-            if (pb->velocity != 0)
+            if (pb->velocity == 3)
+              {
+                v[0] = Vq[0][q];
+                v[1] = Vq[1][q];
+                v[2] = Vq[2][q];
+                val += rho_cp * (gx * v[0] + gy * v[1] + gz * v[2]);
+              }
+            else if (pb->velocity != 0)
               {
                 velocity_at(pb, m, (size_t)g.c[0] * n + qx, (size_t)g.c[1] * n + qy, v);
                 val += rho_cp * (gx * v[0] + gy * v[1] + gz * v[2]);
@@ -670,6 +691,24 @@ namespace
     const int64_t iy = (int64_t)g.c[1] * m.p + b;
     const int64_t iz = (int64_t)g.c[2] * m.p + c;
     return ix + m.N1[0] * (iy + m.N1[1] * iz);
+  }
+
+  void nodal_velocity_q(const oracle_problem *pb, const Tables &t, const Mesh &m,
+                        const CellGeom &g, double (*Vq)[MAXN * MAXN * MAXN])
+  {
+    const int n = t.n;
+    double loc[MAXN * MAXN * MAXN], t1[MAXN * MAXN * MAXN], t2[MAXN * MAXN * MAXN];
+    for (int comp = 0; comp < 3; ++comp)
+      {
+        const double *v = pb->velocity_field + comp * m.N;
+        for (int c = 0; c < n; ++c)
+          for (int b = 0; b < n; ++b)
+            for (int a = 0; a < n; ++a)
+              loc[(c * n + b) * n + a] = v[node_index(m, g, a, b, c)];
+        apply_dir<false>(t.S, n, 0, loc, t1);
+        apply_dir<false>(t.S, n, 1, t1, t2);
+        apply_dir<false>(t.S, n, 2, t2, Vq[comp]);
+      }
   }
 
   // read_dof_values_plain of the heaviside and, for a temperature-dependent
@@ -797,6 +836,10 @@ namespace
     const double jac_det = hx * hy * hz;
     const double dt_inv = 1. / pb->dt;
 
+    double Vq[3][MAXN * MAXN * MAXN];
+    if (pb->velocity == 3)
+      nodal_velocity_q(pb, t, m, g, Vq);
+
     double V[MAXN * MAXN * MAXN], Fx[MAXN * MAXN * MAXN], Fy[MAXN * MAXN * MAXN],
       Fz[MAXN * MAXN * MAXN];
     for (int qz = 0; qz < n; ++qz)
@@ -819,7 +862,14 @@ namespace
             if (pb->velocity != 0)
               {
                 double v[3];
-                velocity_at(pb, m, (size_t)g.c[0] * n + qx, (size_t)g.c[1] * n + qy, v);
+                if (pb->velocity == 3)
+                  {
+                    v[0] = Vq[0][q];
+                    v[1] = Vq[1][q];
+                    v[2] = Vq[2][q];
+                  }
+                else
+                  velocity_at(pb, m, (size_t)g.c[0] * n + qx, (size_t)g.c[1] * n + qy, v);
                 val += rho_cp * (gx * v[0] + gy * v[1] + gz * v[2]);
               }
             // submit_value(-1 val), submit_gradient(-1 val_grad)

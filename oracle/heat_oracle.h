@@ -51,7 +51,10 @@ enum { ORACLE_MAX_DEGREE = 7 };
  *           3 = gas_liquid_solid (heaviside and temperature) */
 /* conductivity_field: 0 = from material, 1 = material k + amp*s(x)s(y)s(z), s = sin(2 pi t) */
 /* velocity: 0 = none, 1 = vortex (-sin^2(pi x) sin(2 pi y), sin(2 pi x) sin^2(pi y), w0),
- *           2 = constant velocity_value */
+ *           2 = constant velocity_value,
+ *           3 = nodal: velocity_field holds the FE_Q(degree) nodal values of the three
+ *               components, component-blocked [v_x | v_y | v_z] in lattice order, read
+ *               plain and interpolated at q (vel_eval, heat_diffuse_operator.cpp:975-980) */
 typedef struct {
   int n[3];
   int degree;
@@ -93,6 +96,7 @@ typedef struct {
   /* materials 2, 3: the tangent's cells read `temperature` and this old
    * temperature plain (read_dof_values_plain, heat_diffuse_operator.cpp:1004-1014) */
   const double *temperature_old;
+  const double *velocity_field; /* velocity == 3: 3 n_dofs values */
 } oracle_problem;
 
 #define ORACLE_STEFAN_BOLTZMANN 5.670374419e-8

@@ -94,3 +94,55 @@ def test_constant_z_velocity_equals_vortex_on_fields_of_z():
     dv = pv.vmult(T) - p0.vmult(T)
     assert np.abs(dc - dv).max() <= 1e-13 * np.abs(dc).max()
     assert np.abs(dc).max() > 1e-3
+
+
+@pytest.mark.parametrize("n,p,mat,faces", [
+    ((2, 3, 2), 1, GAS, 0b000000),
+    ((2, 2, 2), 2, TWO, 0b100101),
+    ((2, 1, 2), 3, GAS, 0b000011),
+    ((1, 2, 1), 4, TWO, 0b001000),
+])
+def test_nodal_velocity_matches_dense_assembly(n, p, mat, faces):
+    """A nodal (DoF-vector) velocity read plain and interpolated at q
+    (vel_eval, heat_diffuse_operator.cpp:975-980): the oracle against dense
+    assembly with the brute-force module's own basis, for a random field whose
+    three components differ, for the tangent, its diagonal and the residual."""
+    lo, hi = (0.0, -0.5, 0.25), (1.0, 0.5, 1.0)
+    rng = np.random.default_rng(10 + p)
+    from oracle.rng import lattice_shape
+    N = int(np.prod(lattice_shape(n, p)))
+    H = rng.uniform(0.0, 1.0, N) if mat["type"] == "gas_liquid" else None
+    v = np.concatenate([rng.uniform(-1.0, 1.0, N), 0.5 * rng.uniform(-1.0, 1.0, N),
+                        2.0 + rng.uniform(-1.0, 1.0, N)])
+    pb = cpu.Problem(n, p, dt=0.05, material=mat, heaviside=H, lo=lo, hi=hi,
+                     dirichlet_faces=faces, velocity="nodal", velocity_field=v)
+    A, Mrc, M1, mask = brute.assemble_parts(n, p, 0.05, mat, heaviside=H, velocity="nodal",
+                                            velocity_field=v, lo=lo, hi=hi, dirichlet_faces=faces)
+    src = rng.uniform(-1.0, 1.0, N)
+    ref = brute.vmult(A, mask, np.where(mask, 0.0, src))
+    assert np.abs(pb.vmult(src) - ref).max() <= 1e-12 * np.abs(ref).max()
+    d = brute.diagonal(A, mask)
+    assert np.abs(pb.diagonal() - d).max() <= 1e-12 * np.abs(d).max()
+    T, To = rng.uniform(-1, 1, N), rng.uniform(-1, 1, N)
+    r = brute.create_rhs(A, Mrc, M1, mask, T, To)
+    assert np.abs(pb.create_rhs(T, To) - r).max() <= 1e-12 * np.abs(r).max()
+    # swapping two components changes the operator (the component order is pinned)
+    vs = np.concatenate([v[N:2 * N], v[:N], v[2 * N:]])
+    ps = cpu.Problem(n, p, dt=0.05, material=mat, heaviside=H, lo=lo, hi=hi,
+                     dirichlet_faces=faces, velocity="nodal", velocity_field=vs)
+    assert np.abs(ps.vmult(src) - ref).max() > 1e-6 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("p", [1, 3])
+def test_nodal_constant_velocity_equals_constant_mode(p):
+    """The nodal interpolant of a constant field is the constant: the nodal
+    mode with constant nodal values is the constant mode to rounding."""
+    n = (2, 3, 2)
+    u = (0.3, -1.1, 2.0)
+    pc = cpu.Problem(n, p, dt=0.1, material=GAS, velocity="constant", velocity_value=u)
+    N = pc.N
+    v = np.concatenate([np.full(N, c) for c in u])
+    pn = cpu.Problem(n, p, dt=0.1, material=GAS, velocity="nodal", velocity_field=v)
+    x = pc.random(3)
+    yc = pc.vmult(x)
+    assert np.abs(pn.vmult(x) - yc).max() <= 1e-13 * np.abs(yc).max()
