@@ -127,6 +127,14 @@ typedef struct {
   double emissivity;
   double radiation_temperature_infinity;
   void *stream;                /* hipStream_t the handle launches on; NULL = default stream */
+  /* Neumann faces (bits as for dirichlet_faces): a constant heat flux vector q
+   * per face (neumann_flux[face] = {qx, qy, qz}); create_rhs adds -(q . n)
+   * integrated over the face, n the outward normal (the neumann_bc functions of
+   * rhs_boundary_loop, source/heat/heat_diffuse_operator.cpp:735-767, here
+   * constant in space). Not on a Robin face (the reference's AssertThrow,
+   * :744-747); no effect on the tangent. */
+  unsigned neumann_faces;
+  double neumann_flux[6][3];
 } mpdg_heat_desc;
 
 typedef struct mpdg_heat mpdg_heat;
@@ -231,9 +239,9 @@ mpdg_status mpdg_heat_inverse_diagonal(mpdg_heat *h, double *inv_diag);
  * f by reference from its constructor (heat_diffuse_operator.hpp:186-188);
  * here they are arguments. heat_source may be NULL (f = 0). The Robin faces add
  * -(alpha (T - T_inf) + eps sigma (T^4 - T_inf^4)) integrated over the faces
- * (rhs_boundary_loop :719-781), T read plain. A temperature-dependent
- * material is evaluated at T(q) (get_material_parameters :1133-1165). No
- * Neumann faces and no evaporative cooling term. */
+ * and the Neumann faces -(q . n) (rhs_boundary_loop :719-781), T read plain. A
+ * temperature-dependent material is evaluated at T(q) (get_material_parameters
+ * :1133-1165). No evaporative cooling term. */
 mpdg_status mpdg_heat_create_rhs(mpdg_heat *h, double *dst, const double *temperature,
                                  const double *temperature_old, const double *heat_source);
 /* JacobiPreconditioner::vmult (preconditioner_jacobi.hpp:51-56): dst = inv_diag .* src. */

@@ -56,6 +56,8 @@ class HeatDesc(ctypes.Structure):
         ("emissivity", ctypes.c_double),
         ("radiation_temperature_infinity", ctypes.c_double),
         ("stream", ctypes.c_void_p),
+        ("neumann_faces", ctypes.c_uint),
+        ("neumann_flux", (ctypes.c_double * 3) * 6),
     ]
 
 

@@ -85,7 +85,7 @@ namespace mpdg
     // One face cell per workgroup: scratch[cell][a1 + N a2] =
     //   tangent:  int_F phi_a (alpha + 4 eps sigma T^3) u
     //   diagonal: int_F phi_a^2 (alpha + 4 eps sigma T^3)
-    //   rhs:      -int_F phi_a (eps sigma (T^4 - T_inf^4) + alpha (T - T_inf))
+    //   rhs:      -int_F phi_a (eps sigma (T^4 - T_inf^4) + alpha (T - T_inf) + q . n)
     // with T and u read constraint-resolved (tangent, diagonal) or T plain (rhs).
     template <int P, int MODE, typename Number>
     __global__ void __launch_bounds__(kFaceCellThreads)
@@ -162,6 +162,9 @@ namespace mpdg
           double val = 0.0;
           if constexpr (MODE == kRobinRhs)
             {
+              // Neumann face: -q . n (a constant per face)
+              if ((rp.neumann >> f) & 1u)
+                val -= rp.qn[f];
               const double Ti = rp.t_inf_rad;
               if (rad)
                 val -= rp.eps_sigma * ((Tq * Tq) * (Tq * Tq) - (Ti * Ti) * (Ti * Ti));

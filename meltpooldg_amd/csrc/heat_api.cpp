@@ -109,6 +109,7 @@ struct mpdg_heat
   // Robin faces and the temperature the tangent is linearised at (a caller's
   // device vector, not a copy: the reference operator holds a reference)
   RobinParams robin{};
+  RobinParams robin_rhs{}; // the residual's faces: Robin and Neumann
   const double *temperature = nullptr;
   // old temperature the tangent of a temperature-dependent material reads
   // (the operator's reference to temperature_old), also a caller's vector
@@ -281,15 +282,20 @@ namespace
              precision == MPDG_FP32));
   }
 
-  // Robin faces of a descriptor: faces ascending, each face's node lattice a
-  // contiguous thread range (tangential directions e1 < e2, e1 fastest).
-  RobinParams robin_params(const mpdg_heat_desc &d, const double *h)
+  // Robin faces of a descriptor (the tangent's and diagonal's; with_neumann:
+  // the residual's, the Neumann faces with their -(q . n) included): faces
+  // ascending, each face's node lattice a contiguous thread range (tangential
+  // directions e1 < e2, e1 fastest).
+  RobinParams robin_params(const mpdg_heat_desc &d, const double *h, bool with_neumann = false)
   {
     constexpr double kStefanBoltzmann = 5.670374419e-8; // physical_constants.hpp:6
     RobinParams rp{};
     rp.conv = d.convection_faces & 0x3Fu;
     rp.rad = d.radiation_faces & 0x3Fu;
-    rp.robin = rp.conv | rp.rad;
+    rp.neumann = with_neumann ? d.neumann_faces & 0x3Fu : 0u;
+    rp.robin = rp.conv | rp.rad | rp.neumann;
+    for (int f = 0; f < 6; ++f)
+      rp.qn[f] = (f & 1 ? 1.0 : -1.0) * d.neumann_flux[f][f >> 1]; // q . n, n outward
     rp.dirichlet = d.dirichlet_faces & 0x3Fu;
     for (int k = 0; k < 3; ++k)
       {
@@ -1139,8 +1145,15 @@ mpdg_status mpdg_heat_create(const mpdg_heat_desc *desc, mpdg_heat **out)
     return fail(MPDG_ERR_UNSUPPORTED, "conductivity field not supported");
   if (uses_heaviside(d) && d.conductivity_field != MPDG_CONDUCTIVITY_MATERIAL)
     return fail(MPDG_ERR_UNSUPPORTED, "separable conductivity applies to the gas material only");
-  if ((d.dirichlet_faces | d.convection_faces | d.radiation_faces) & ~0x3Fu)
+  if ((d.dirichlet_faces | d.convection_faces | d.radiation_faces | d.neumann_faces) & ~0x3Fu)
     return fail(MPDG_ERR_ARGUMENT, "face masks have 6 bits (faces 2d + side)");
+  if (d.neumann_faces & (d.convection_faces | d.radiation_faces))
+    return fail(MPDG_ERR_ARGUMENT, "It is not allowed to specify both Neumann and radiation "
+                                   "and/or convection boundary conditions at the same face.");
+  for (int f = 0; f < 6; ++f)
+    for (int c = 0; c < 3; ++c)
+      if (((d.neumann_faces >> f) & 1u) && !std::isfinite(d.neumann_flux[f][c]))
+        return fail(MPDG_ERR_ARGUMENT, "Neumann fluxes must be finite");
   if (!std::isfinite(d.convection_coefficient) || !std::isfinite(d.emissivity) ||
       !std::isfinite(d.convection_temperature_infinity) ||
       !std::isfinite(d.radiation_temperature_infinity))
@@ -1238,9 +1251,11 @@ mpdg_status mpdg_heat_create(const mpdg_heat_desc *desc, mpdg_heat **out)
   fill_params<double>(H, H->prm64, H->sin_tab64.ptr);
   fill_params<float>(H, H->prm32, H->sin_tab32.ptr);
   H->robin = robin_params(d, H->h);
-  if (H->robin.nfaces > 0)
+  H->robin_rhs = robin_params(d, H->h, true);
+  if (H->robin_rhs.nfaces > 0)
     {
-      const hipError_t er = H->robin_rows.alloc((size_t)robin_scratch(H->robin, d.degree));
+      // rows of the residual's face set, which contains the tangent's
+      const hipError_t er = H->robin_rows.alloc((size_t)robin_scratch(H->robin_rhs, d.degree));
       if (er != hipSuccess)
         {
           delete H;
@@ -1471,7 +1486,7 @@ mpdg_status mpdg_heat_create_rhs(mpdg_heat *h, double *dst, const double *temper
                              temperature, temperature_old, heat_source, h->heav64.ptr,
                              h->vel64.ptr, h->stream));
   // rhs_boundary_loop: the Robin faces with T read plain
-  MPDG_HIP(launch_robin<double>(h->desc.degree, kRobinRhs, h->robin, dst, nullptr, temperature,
+  MPDG_HIP(launch_robin<double>(h->desc.degree, kRobinRhs, h->robin_rhs, dst, nullptr, temperature,
                                 h->robin_rows.ptr, nullptr, nullptr, h->stream));
   return MPDG_OK;
 }

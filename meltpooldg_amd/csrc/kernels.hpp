@@ -269,12 +269,15 @@ namespace mpdg
     int n[3], n1[3];      // cells and lattice nodes per direction
     double h[3];          // cell extents
     double alpha, t_inf_conv, eps_sigma, t_inf_rad;
+    // residual only: Neumann faces (in `robin` too) and their -(q . n) per face
+    unsigned neumann;
+    double qn[6];
   };
   enum RobinMode
   {
     kRobinTangent = 0,  // dst += A_face src (src and T constraint-resolved)
     kRobinDiagonal = 1, // dst += diag(A_face)
-    kRobinRhs = 2       // dst += -(alpha (T - T_inf) + eps sigma (T^4 - T_inf^4)) faces, T plain
+    kRobinRhs = 2       // dst += -(alpha (T - T_inf) + eps sigma (T^4 - T_inf^4) + q . n) faces, T plain
   };
   constexpr int kRobinThreads = 256;
   // workgroups of the node launch (partial-dot slots it needs)

@@ -122,7 +122,8 @@ def _stream_handle():
 
 def _make_desc(n_cells, degree, material, box, precision, conductivity_amplitude, velocity,
                velocity_w0, dirichlet_faces, stream, convection=None, radiation=None,
-               convection_boundary_ids=(), radiation_boundary_ids=(), velocity_value=None):
+               convection_boundary_ids=(), radiation_boundary_ids=(), velocity_value=None,
+               neumann_bc=None):
     n = (n_cells,) * 3 if isinstance(n_cells, int) else tuple(n_cells)
     mat = material or MaterialData()
     d = L.HeatDesc()
@@ -166,6 +167,12 @@ def _make_desc(n_cells, degree, material, box, precision, conductivity_amplitude
     d.radiation_faces = _face_mask(radiation_boundary_ids)
     d.emissivity = float(rad.emissivity)
     d.radiation_temperature_infinity = float(rad.temperature_infinity)
+    for face, flux in (neumann_bc or {}).items():
+        q = [float(c) for c in flux]
+        if not 0 <= int(face) < 6 or len(q) != 3:
+            raise ValueError("neumann_bc maps a face id 0..5 to a flux vector (qx, qy, qz)")
+        d.neumann_faces |= 1 << int(face)
+        d.neumann_flux[int(face)][:] = q
     d.stream = stream
     return d
 
@@ -199,20 +206,24 @@ class HeatDiffuseOperator:
     ``set_temperature``. ``velocity``: None, "vortex" (analytic at q, z component
     ``velocity_w0``), "nodal" (a DoF-vector field given by ``set_velocity``,
     FP64) or "constant" (``velocity_value``, a prescribed_velocity
-    field constant in space).
+    field constant in space). ``neumann_bc``: {face id: (qx, qy, qz)}, a heat
+    flux constant on the face, whose -(q . n) create_rhs integrates (the
+    reference's neumann_bc functions, heat_diffuse_operator.cpp:735-767); not on
+    a Robin face.
     """
 
     def __init__(self, n_cells, degree, material=None, box=((0.0, 0.0, 0.0), (1.0, 1.0, 1.0)),
                  precision="fp64", conductivity_amplitude=0.0, velocity=None, velocity_w0=0.2,
                  dirichlet_faces=0x3F, stream=None, convection=None, radiation=None,
-                 convection_boundary_ids=(), radiation_boundary_ids=(), velocity_value=None):
+                 convection_boundary_ids=(), radiation_boundary_ids=(), velocity_value=None,
+                 neumann_bc=None):
         self._lib = L.load()
         n = (n_cells,) * 3 if isinstance(n_cells, int) else tuple(n_cells)
         mat = material or MaterialData()
         d = _make_desc(n_cells, degree, mat, box, precision, conductivity_amplitude, velocity,
                        velocity_w0, dirichlet_faces, stream if stream is not None else _stream_handle(),
                        convection, radiation, convection_boundary_ids, radiation_boundary_ids,
-                       velocity_value)
+                       velocity_value, neumann_bc)
         self.desc = d
         h = ctypes.c_void_p()
         L.check(self._lib.mpdg_heat_create(ctypes.byref(d), ctypes.byref(h)))

@@ -51,6 +51,8 @@ class OracleProblem(ctypes.Structure):
         ("apparent_capacity", ctypes.c_int),
         ("temperature_old", ctypes.POINTER(ctypes.c_double)),
         ("velocity_field", ctypes.POINTER(ctypes.c_double)),
+        ("neumann_faces", ctypes.c_uint),
+        ("neumann_flux", (ctypes.c_double * 3) * 6),
     ]
 
 APPARENT_CAPACITY_TYPES = {"poly4_bell": 0, "constant": 1, "qlq": 2}
@@ -162,7 +164,7 @@ class Problem:
     def __init__(self, n, degree, dt=1e-3, material=None, k_amplitude=0.0, velocity=False,
                  w0=0.2, heaviside=None, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0),
                  dirichlet_faces=0x3F, robin=None, temperature=None, velocity_value=None,
-                 temperature_old=None, velocity_field=None):
+                 temperature_old=None, velocity_field=None, neumann=None):
         mat = material or {"type": "gas", "gas_rho": 1.0, "gas_cp": 1.0, "gas_k": 1.0}
         n = (n, n, n) if np.isscalar(n) else tuple(n)
         s = OracleProblem()
@@ -188,6 +190,10 @@ class Problem:
         s.radiation_faces = int(rb.get("radiation_faces", 0))
         s.emissivity = float(rb.get("emissivity", 0.0))
         s.radiation_t_inf = float(rb.get("radiation_t_inf", 0.0))
+        # neumann: {face f: (qx, qy, qz)}, a constant flux vector per face
+        for f, q in (neumann or {}).items():
+            s.neumann_faces |= 1 << int(f)
+            s.neumann_flux[int(f)][:] = [float(c) for c in q]
         self._T = None
         self._To = None
         self._h = None

@@ -209,6 +209,9 @@ namespace
       return false;
     if (pb->velocity < 0 || pb->velocity > 3 || (pb->velocity == 3 && !pb->velocity_field))
       return false;
+    // no face may be both Neumann and Robin (heat_diffuse_operator.cpp:744-747)
+    if (pb->neumann_faces & (pb->convection_faces | pb->radiation_faces) & 0x3Fu)
+      return false;
     if (pb->material >= 2)
       {
         if (pb->consistent_with_evaporation)
@@ -978,6 +981,9 @@ namespace
             }
           else
             {
+              // Neumann: -q . n, n the outward normal of face f (-e_d or +e_d)
+              if ((pb->neumann_faces >> b.f) & 1u)
+                val -= (b.side ? 1.0 : -1.0) * pb->neumann_flux[b.f][b.d];
               const double Ti = pb->radiation_t_inf;
               if (rad)
                 val -= pb->emissivity * ORACLE_STEFAN_BOLTZMANN *
@@ -1001,9 +1007,10 @@ namespace
   // The boundary-face part of MatrixFree::loop: every face cell of every Robin
   // face, faces ascending, face cells lexicographic; body(b, c1, c2).
   template <typename Body>
-  void for_robin_face_cells(const oracle_problem *pb, const Mesh &m, Body body)
+  void for_robin_face_cells(const oracle_problem *pb, const Mesh &m, Body body,
+                            bool with_neumann = false)
   {
-    const unsigned mask = robin_faces(pb);
+    const unsigned mask = robin_faces(pb) | (with_neumann ? pb->neumann_faces & 0x3Fu : 0u);
     for (int f = 0; f < 6; ++f)
       {
         if (!((mask >> f) & 1u))
@@ -1253,13 +1260,17 @@ int oracle_create_rhs(const oracle_problem *pb, const double *temperature,
               dst[node_index(m, g, a, b, c)] += y[(c * n + b) * n + a];
           }
   });
-  // rhs_boundary_loop (heat_diffuse_operator.cpp:719-781): T read plain
-  for_robin_face_cells(pb, m, [&](const BoxFace &b, int c1, int c2) {
-    double Tl[MAXN * MAXN], y[MAXN * MAXN];
-    face_gather(pb, m, b, c1, c2, temperature, true, nullptr, Tl, nullptr);
-    face_local(pb, t, m, b, kFaceRhs, Tl, nullptr, y);
-    face_scatter(pb, m, b, c1, c2, y, dst);
-  });
+  // rhs_boundary_loop (heat_diffuse_operator.cpp:719-781): T read plain; the
+  // Robin and the Neumann faces
+  for_robin_face_cells(
+    pb, m,
+    [&](const BoxFace &b, int c1, int c2) {
+      double Tl[MAXN * MAXN], y[MAXN * MAXN];
+      face_gather(pb, m, b, c1, c2, temperature, true, nullptr, Tl, nullptr);
+      face_local(pb, t, m, b, kFaceRhs, Tl, nullptr, y);
+      face_scatter(pb, m, b, c1, c2, y, dst);
+    },
+    true);
   return 0;
 }
 

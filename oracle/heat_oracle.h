@@ -97,6 +97,11 @@ typedef struct {
    * temperature plain (read_dof_values_plain, heat_diffuse_operator.cpp:1004-1014) */
   const double *temperature_old;
   const double *velocity_field; /* velocity == 3: 3 n_dofs values */
+  /* Neumann faces (bit f as for dirichlet_faces) with a constant heat flux
+   * vector q per face: the residual gains -(q . n) on the face
+   * (rhs_boundary_loop, heat_diffuse_operator.cpp:761-767); not on a Robin face */
+  unsigned neumann_faces;
+  double neumann_flux[6][3];
 } oracle_problem;
 
 #define ORACLE_STEFAN_BOLTZMANN 5.670374419e-8

@@ -34,7 +34,8 @@ def oracle_problem(case):
                        k_amplitude=case["k_amplitude"], velocity=case["velocity"],
                        heaviside=case["heaviside"], dirichlet_faces=case["faces"],
                        robin=case.get("robin"), lo=case.get("lo", (0.0, 0.0, 0.0)),
-                       hi=case.get("hi", (1.0, 1.0, 1.0)), velocity_value=case.get("velocity_value"))
+                       hi=case.get("hi", (1.0, 1.0, 1.0)), velocity_value=case.get("velocity_value"),
+                       neumann=case.get("neumann"))
 
 
 def _ids(mask):
@@ -71,7 +72,8 @@ def engine_operator(case, precision="fp64"):
                              radiation=RadiationBC(rb.get("emissivity", 0.0),
                                                    rb.get("radiation_t_inf", 0.0)),
                              convection_boundary_ids=_ids(rb.get("convection_faces", 0)),
-                             radiation_boundary_ids=_ids(rb.get("radiation_faces", 0)))
+                             radiation_boundary_ids=_ids(rb.get("radiation_faces", 0)),
+                             neumann_bc=case.get("neumann"))
     op.reset_time_increment(case["dt"])
     if case["heaviside"] is not None:
         op.set_heaviside(torch.tensor(case["heaviside"], dtype=torch.float64, device="cuda"))
