@@ -354,6 +354,9 @@ class CaseSetup:
     temperature0: object       # lattice vector
     coords: tuple              # lattice coordinates per direction
     heaviside_at: object = None  # t -> lattice vector, for a time-dependent heaviside
+    # {face id: (qx, qy, qz)}: a constant heat flux per face, the Neumann
+    # conditions a case attaches (heat_diffuse_operator.cpp:78, 735-767)
+    neumann_bc: dict = None
 
     def norm_scale(self):
         """Factor from the 3D L2 norm to the case's own dimension."""
@@ -510,20 +513,24 @@ def make_operator(case, stream=None):
                              velocity="constant" if case.velocity else None,
                              velocity_value=case.velocity, dirichlet_faces=case.dirichlet_faces,
                              stream=stream, convection=case.convection, radiation=case.radiation,
-                             convection_boundary_ids=conv_ids, radiation_boundary_ids=rad_ids)
+                             convection_boundary_ids=conv_ids, radiation_boundary_ids=rad_ids,
+                             neumann_bc=case.neumann_bc)
     return op
 
 
-def run(prm, log=print, max_steps=None, graphs=True):
+def run(prm, log=print, max_steps=None, graphs=True, case=None):
     """HeatTransferApplication::run on the device. Returns one record per time
     step: step, time, dt, the L2 norm of the solution (in the case's own
     dimension) and the Newton report. ``graphs``: replay the Krylov restart
-    cycles / iteration chunks as captured hipGraphs (False: eager launches)."""
+    cycles / iteration chunks as captured hipGraphs (False: eager launches).
+    ``case``: a CaseSetup to run instead of ``setup(prm)`` (a case with its
+    own boundary conditions, e.g. Neumann faces)."""
     import torch
     from .heat import HeatDiffuseOperation
     if not isinstance(prm, Parameters):
         prm = parse_parameters(prm)
-    case = setup(prm)
+    if case is None:
+        case = setup(prm)
     op = make_operator(case)
     op.set_graphs(graphs)
     if case.heaviside is not None:

@@ -186,7 +186,7 @@ def oracle_problem(cs, dt):
     return cpu.Problem(cs.n_cells, cs.degree, dt=dt, material=mat, heaviside=cs.heaviside,
                        lo=cs.box[0], hi=cs.box[1], dirichlet_faces=cs.dirichlet_faces,
                        robin=robin, velocity="constant" if cs.velocity else None,
-                       velocity_value=cs.velocity)
+                       velocity_value=cs.velocity, neumann=cs.neumann_bc)
 
 
 @pytest.mark.parametrize("name", SUPPORTED)
