@@ -234,14 +234,59 @@ namespace mpdg
     };
 
     // All even-odd tables of the vmult kernel for degree P.
+    // M with its point symmetry made exact: M[N-1-i][N-1-j] = sign M[i][j]
+    // (sign +1 for S on symmetric points, -1 for the derivative C, whose
+    // centre entry is then exactly 0).
+    template <int N>
+    struct Sq
+    {
+      double a[N][N]{};
+    };
+
+    template <int N>
+    constexpr Sq<N> symmetrize(const double (&M)[N][N], int sign)
+    {
+      Sq<N> r{};
+      for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j)
+          r.a[i][j] = 0.5 * (M[i][j] + sign * M[N - 1 - i][N - 1 - j]);
+      return r;
+    }
+
+    // Even-odd form of M^T from that of M (M point-symmetric with `sign`):
+    // the blocks are transposes of M's (swapped for sign -1) and the centre
+    // column and row trade places, so M and M^T share their constants
+    // bitwise (a kernel then holds one set of scalar registers for both).
+    template <int N>
+    constexpr EO<N> transpose_eo(const EO<N> &m, int sign)
+    {
+      EO<N> r{};
+      const int h = N / 2;
+      for (int i = 0; i < h; ++i)
+        for (int j = 0; j < h; ++j)
+          {
+            r.e[i][j] = sign > 0 ? m.e[j][i] : m.o[j][i];
+            r.o[i][j] = sign > 0 ? m.o[j][i] : m.e[j][i];
+          }
+      for (int i = 0; i < h; ++i)
+        {
+          r.mc[i] = m.mr[i];
+          r.mr[i] = m.mc[i];
+        }
+      r.mm = m.mm;
+      return r;
+    }
+
     template <int P>
     struct VTab
     {
       static constexpr Tab<P + 1> t = make_tab<P>();
-      static constexpr EO<P + 1> S = split<P + 1>(t.S, false);
-      static constexpr EO<P + 1> St = split<P + 1>(t.S, true);
-      static constexpr EO<P + 1> C = split<P + 1>(t.C, false);
-      static constexpr EO<P + 1> Ct = split<P + 1>(t.C, true);
+      static constexpr Sq<P + 1> Ssym = symmetrize<P + 1>(t.S, 1);
+      static constexpr Sq<P + 1> Csym = symmetrize<P + 1>(t.C, -1);
+      static constexpr EO<P + 1> S = split<P + 1>(Ssym.a, false);
+      static constexpr EO<P + 1> St = transpose_eo<P + 1>(S, 1);
+      static constexpr EO<P + 1> C = split<P + 1>(Csym.a, false);
+      static constexpr EO<P + 1> Ct = transpose_eo<P + 1>(C, -1);
     };
   } // namespace ct
 } // namespace mpdg

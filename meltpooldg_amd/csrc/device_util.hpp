@@ -46,10 +46,14 @@ namespace mpdg
         }
       if constexpr (N % 2 == 1)
         {
-          Number m = Number(M.mm) * x[H];
+          // an exactly-zero centre entry (the derivative's) is left out: the
+          // table is a compile-time constant, so the test folds away
+          Number m = Number(M.mr[0]) * (SIGN > 0 ? E[0] : O[0]);
 #pragma unroll
-          for (int j = 0; j < H; ++j)
+          for (int j = 1; j < H; ++j)
             m += Number(M.mr[j]) * (SIGN > 0 ? E[j] : O[j]);
+          if (M.mm != 0.0)
+            m += Number(M.mm) * x[H];
           y[H] = m;
         }
     }

@@ -52,6 +52,17 @@ namespace mpdg
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 
+    // Orders the LDS phases of a vmult layer: the wavefront's own (its lines
+    // cover whole cells) or the workgroup's (spread form).
+    template <bool WG>
+    __device__ __forceinline__ void phase_sync()
+    {
+      if constexpr (WG)
+        __syncthreads();
+      else
+        wave_sync();
+    }
+
     // The four 1D matrices of the line contractions.
     enum LineMatrix
     {
@@ -462,12 +473,18 @@ namespace mpdg
       constexpr int N = P + 1;
       constexpr int NL = N * N;
       constexpr int STR = lds_stride<N>();
-      constexpr int LT = CW * NL;
-      constexpr int CPB = CW * WPB;
+      // R = 0: spread form, CW = cells of the workgroup, one line per thread
+      constexpr bool SPREAD = R == 0;
+      constexpr int RR = SPREAD ? 1 : R;
+      constexpr int CPB = SPREAD ? CW : CW * WPB;
+      constexpr int LT = SPREAD ? CPB * NL : CW * NL;
       constexpr int THREADS = 64 * WPB;
       constexpr bool H2 = COEF == kCoefHeaviside;
+      constexpr bool SINE = COEF == kCoefSine;
       constexpr bool CD = ORD; // agent-coherent dst accesses
-      static_assert(LT <= 64 * R && LT > 64 * (R - 1), "lines must fill the last lane slot");
+      static_assert(SPREAD ? (LT <= THREADS && LT > THREADS - NL)
+                           : (LT <= 64 * R && LT > 64 * (R - 1)),
+                    "lines must fill the last lane slot");
       constexpr ct::Tab<N> T1 = ct::VTab<P>::t;
 
       __shared__ Number A[CPB * STR];
@@ -476,6 +493,7 @@ namespace mpdg
       __shared__ Number RB[(H2 && VEL) ? CPB * STR : 1];
       __shared__ Number FZ[2][CPB * NL]; // z-face carry [cell][b][a], by layer parity
       __shared__ Number XE[2][CPB * NL]; // right x-face of each cell line [cell][line]
+      __shared__ Number KW[SINE ? CPB * STR : 1]; // k w3 at the q-points [cell][qz][qy][qx]
 
       const uint32_t vbytes = vector_bytes(prm.n1x, prm.n1y, prm.n1z, (int)sizeof(Number));
       const __amdgpu_buffer_rsrc_t rsrc = make_rsrc(src, vbytes);
@@ -484,25 +502,40 @@ namespace mpdg
       const int w = threadIdx.x >> 6;
       const int lane = threadIdx.x & 63;
       const LineCols<N, MF> LC = line_cols<P, MF>(lane);
-      int cb[R], s0v[R], s1v[R], sv[R];
-      bool lv[R];
+      int cb[RR], s0v[RR], s1v[RR], sv[RR];
+      bool lv[RR];
 #pragma unroll
-      for (int r = 0; r < R; ++r)
+      for (int r = 0; r < RR; ++r)
         {
-          // a spare lane (only in the last slot) repeats a line of that same slot,
-          // so both run in one instruction and see the same LDS contents
-          const int g0 = lane + 64 * r;
-          lv[r] = g0 < LT;
-          const int g = lv[r] ? g0 : 64 * (R - 1) + (g0 - LT) % (LT - 64 * (R - 1));
-          const int l = g % CW;
-          sv[r] = g / CW;
+          int g, l;
+          if constexpr (SPREAD)
+            {
+              // thread t owns line t / CPB of cell t % CPB; the few spare
+              // threads (last wavefront) repeat lines of that wavefront
+              const int g0 = threadIdx.x;
+              lv[r] = g0 < LT;
+              g = lv[r] ? g0 : g0 - (THREADS - LT);
+              l = g % CPB;
+              sv[r] = g / CPB;
+              cb[r] = l;
+            }
+          else
+            {
+              // a spare lane (only in the last slot) repeats a line of that same slot,
+              // so both run in one instruction and see the same LDS contents
+              const int g0 = lane + 64 * r;
+              lv[r] = g0 < LT;
+              g = lv[r] ? g0 : 64 * (R - 1) + (g0 - LT) % (LT - 64 * (R - 1));
+              l = g % CW;
+              sv[r] = g / CW;
+              cb[r] = w * CW + l;
+            }
           s0v[r] = sv[r] % N;
           s1v[r] = sv[r] / N;
-          cb[r] = w * CW + l;
         }
-      LaneConst lc[R];
+      LaneConst lc[RR];
 #pragma unroll
-      for (int r = 0; r < R; ++r)
+      for (int r = 0; r < RR; ++r)
         {
           lc[r].loff = cb[r] * P + prm.n1x * (s0v[r] + prm.n1y * s1v[r]);
           lc[r].bits = (s0v[r] == 0 ? kS0Lo : 0u) | (s0v[r] == P ? kS0Hi : 0u) |
@@ -531,9 +564,9 @@ namespace mpdg
       else
         it = decode_item<Number, CPB>(prm, colour, layers, item);
       LayerU LU = layer_uniforms<Number, P>(prm, it, 0);
-      Number u[R][N], hl[R][N];
+      Number u[RR][N], hl[RR][N];
 #pragma unroll
-      for (int r = 0; r < R; ++r)
+      for (int r = 0; r < RR; ++r)
         {
           gather_line<P>(rsrc, lc[r], LU, cb[r], it.ncell, item < items, u[r]);
           if constexpr (H2)
@@ -545,6 +578,17 @@ namespace mpdg
       while (item < items)
         {
           const int cz = it.cz0 + j;
+          // separable-sine conductivity: sin(2 pi x) sin(2 pi y) at the lane's
+          // z-line of the z-line phase (qx = s0, qy = s1), loaded early
+          Number ssx[RR], ssy[RR];
+          if constexpr (SINE)
+#pragma unroll
+            for (int r = 0; r < RR; ++r)
+              {
+                const int cx = min(it.cx0 + cb[r], prm.nx - 1);
+                ssx[r] = prm.sin2pi[0][cx * N + s0v[r]];
+                ssy[r] = prm.sin2pi[1][it.cy * N + s1v[r]];
+              }
           int item_n = item, j_n = j + 1;
           Item it_n = it;
           if (j_n >= it.nl)
@@ -569,7 +613,7 @@ namespace mpdg
 
           // ---- x-lines (y = s0, z = s1): interpolate in x ------------------------
 #pragma unroll
-          for (int r = 0; r < R; ++r)
+          for (int r = 0; r < RR; ++r)
             {
               Number *Al = A + cb[r] * STR;
               Number t1[N];
@@ -589,9 +633,9 @@ namespace mpdg
 
           // prefetch of the next layer's x-lines
           const LayerU LUn = layer_uniforms<Number, P>(prm, it_n, j_n);
-          Number un[R][N], hn[R][N];
+          Number un[RR][N], hn[RR][N];
 #pragma unroll
-          for (int r = 0; r < R; ++r)
+          for (int r = 0; r < RR; ++r)
             {
               gather_line<P>(rsrc, lc[r], LUn, cb[r], it_n.ncell, item_n < items, un[r]);
               if constexpr (H2)
@@ -607,11 +651,11 @@ namespace mpdg
           // face or shared with the neighbouring segment. Offsets are out of
           // range where nothing is written / read; nodes stored first get 0
           // from the early read-modify-write load.
-          uint32_t wl[R], w0o[R], wPo[R];
-          Number dv[R][N];
-          bool zl[R], z0[R], zP[R];
+          uint32_t wl[RR], w0o[RR], wPo[RR];
+          Number dv[RR][N];
+          bool zl[RR], z0[RR], zP[RR];
 #pragma unroll
-          for (int r = 0; r < R; ++r)
+          for (int r = 0; r < RR; ++r)
             {
               constexpr uint32_t SZ = sizeof(Number);
               const uint32_t obase = (uint32_t)(LU.base + lc[r].loff) * SZ;
@@ -643,11 +687,11 @@ namespace mpdg
               zP[r] = conP;
               dv[r][N - 1] = dst_load<CD, Number>(rdst, (wrP & !firstP & !conP) ? obase + P * SZ : kOutOfRange);
             }
-          wave_sync();
+          phase_sync<SPREAD>();
 
           // ---- y-lines (qx = s0, z = s1): interpolate in y ------------------------
 #pragma unroll
-          for (int r = 0; r < R; ++r)
+          for (int r = 0; r < RR; ++r)
             {
               Number *Al = A + cb[r] * STR;
               Number v[N], o[N];
@@ -670,23 +714,27 @@ namespace mpdg
                     KBl[(s1v[r] * N + b) * N + s0v[r]] = o[b];
                 }
             }
-          wave_sync();
+          phase_sync<SPREAD>();
 
-          // ---- z-lines (qx = s0, qy = s1): values and z-gradient --------------
-          Number U[R][N], Gz[R][N], RC[R][N];
+          // ---- z-lines (qx = s0, qy = s1): values, z-gradient and z-flux ------
+          // The z-line's own part of the q-point residual is completed here,
+          //   Vz = rho_cp/dt w3 U + C^T (k w3 kf_z dU/dz) (+ rho_cp w3 u_z dU/dz),
+          // so only Vz stays in registers until the x- and y-fluxes are added.
+          Number Vz[RR][N];
 #pragma unroll
-          for (int r = 0; r < R; ++r)
+          for (int r = 0; r < RR; ++r)
             {
+              const int s0 = s0v[r], s1 = s1v[r];
               Number *Al = A + cb[r] * STR;
-              Number v[N];
+              Number v[N], U[N], Gz[N], kq[N], rcq[N], f[N], o[N];
 #pragma unroll
               for (int c = 0; c < N; ++c)
-                v[c] = Al[(c * N + s1v[r]) * N + s0v[r]];
-              line_apply<P, kLmS, MF>(LC, v, U[r]);
-              line_apply<P, kLmC, MF>(LC, U[r], Gz[r]);
+                v[c] = Al[(c * N + s1) * N + s0];
+              line_apply<P, kLmS, MF>(LC, v, U);
+              line_apply<P, kLmC, MF>(LC, U, Gz);
 #pragma unroll
               for (int q = 0; q < N; ++q)
-                Al[(q * N + s1v[r]) * N + s0v[r]] = U[r][q];
+                Al[(q * N + s1) * N + s0] = U[q];
               if constexpr (H2)
                 {
                   Number *KBl = KB + cb[r] * STR;
@@ -694,25 +742,53 @@ namespace mpdg
                   Number Hq[N];
 #pragma unroll
                   for (int c = 0; c < N; ++c)
-                    v[c] = KBl[(c * N + s1v[r]) * N + s0v[r]];
+                    v[c] = KBl[(c * N + s1) * N + s0];
                   line_apply<P, kLmS, MF>(LC, v, Hq);
 #pragma unroll
                   for (int q = 0; q < N; ++q)
                     {
-                      Number rc, k;
-                      two_phase(prm, Hq[q], rc, k);
-                      KBl[(q * N + s1v[r]) * N + s0v[r]] = k;
+                      two_phase(prm, Hq[q], rcq[q], kq[q]);
+                      KBl[(q * N + s1) * N + s0] = kq[q];
                       if constexpr (VEL)
-                        RBl[(q * N + s1v[r]) * N + s0v[r]] = rc;
-                      RC[r][q] = rc;
+                        RBl[(q * N + s1) * N + s0] = rcq[q];
                     }
                 }
+              else
+                {
+                  const Number c = SINE ? prm.kamp * ssx[r] * ssy[r] : Number(0);
+#pragma unroll
+                  for (int q = 0; q < N; ++q)
+                    {
+                      kq[q] = SINE ? prm.k0 + c * prm.sin2pi[2][cz * N + q] : prm.k0;
+                      rcq[q] = prm.rho_cp0;
+                    }
+                }
+              const Number wxy = (Number)T1.w[s0] * (Number)T1.w[s1];
+#pragma unroll
+              for (int q = 0; q < N; ++q)
+                {
+                  const Number kw = kq[q] * ((Number)T1.w[q] * wxy);
+                  if constexpr (SINE)
+                    KW[cb[r] * STR + (q * N + s1) * N + s0] = kw;
+                  f[q] = kw * prm.kf[2] * Gz[q];
+                }
+              line_apply<P, kLmCt, MF>(LC, f, o);
+#pragma unroll
+              for (int q = 0; q < N; ++q)
+                {
+                  const Number w3 = (Number)T1.w[q] * wxy;
+                  const Number mass = H2 ? rcq[q] * prm.dtinv_vol : prm.mass;
+                  Number val = mass * w3 * U[q] + o[q];
+                  if constexpr (VEL)
+                    val += rcq[q] * w3 * prm.vf[2] * prm.w0 * Gz[q];
+                  Vz[r][q] = val;
+                }
             }
-          wave_sync();
+          phase_sync<SPREAD>();
 
           // ---- x-lines (qy = s0, qz = s1): x-gradient flux --------------------
 #pragma unroll
-          for (int r = 0; r < R; ++r)
+          for (int r = 0; r < RR; ++r)
             {
               const int s0 = s0v[r], s1 = s1v[r];
               const int cx = min(it.cx0 + cb[r], prm.nx - 1);
@@ -730,8 +806,9 @@ namespace mpdg
               for (int q = 0; q < N; ++q)
                 {
                   const Number w3 = (Number)T1.w[q] * wyz;
-                  const Number k = conductivity<Number, COEF, N>(prm, KBl, cx, it.cy, cz, q, s0, s1);
-                  f[q] = k * w3 * prm.kf[0] * gx[q];
+                  const Number kw = SINE ? KW[cb[r] * STR + (s1 * N + s0) * N + q]
+                                         : conductivity<Number, COEF, N>(prm, KBl, cx, it.cy, cz, q, s0, s1) * w3;
+                  f[q] = kw * prm.kf[0] * gx[q];
                   if constexpr (VEL)
                     {
                       const Number rc = H2 ? RBl[(s1 * N + s0) * N + q] : prm.rho_cp0;
@@ -744,11 +821,11 @@ namespace mpdg
               for (int q = 0; q < N; ++q)
                 Bl[(s1 * N + s0) * N + q] = VEL ? o[q] + cv[q] : o[q];
             }
-          wave_sync();
+          phase_sync<SPREAD>();
 
           // ---- y-lines (qx = s0, qz = s1): y-gradient flux ----------------------
 #pragma unroll
-          for (int r = 0; r < R; ++r)
+          for (int r = 0; r < RR; ++r)
             {
               const int s0 = s0v[r], s1 = s1v[r];
               const int cx = min(it.cx0 + cb[r], prm.nx - 1);
@@ -766,8 +843,9 @@ namespace mpdg
               for (int q = 0; q < N; ++q)
                 {
                   const Number w3 = (Number)T1.w[q] * wxz;
-                  const Number k = conductivity<Number, COEF, N>(prm, KBl, cx, it.cy, cz, s0, q, s1);
-                  f[q] = k * w3 * prm.kf[1] * gy[q];
+                  const Number kw = SINE ? KW[cb[r] * STR + (s1 * N + q) * N + s0]
+                                         : conductivity<Number, COEF, N>(prm, KBl, cx, it.cy, cz, s0, q, s1) * w3;
+                  f[q] = kw * prm.kf[1] * gy[q];
                   if constexpr (VEL)
                     {
                       const Number rc = H2 ? RBl[(s1 * N + q) * N + s0] : prm.rho_cp0;
@@ -783,50 +861,29 @@ namespace mpdg
                   Bl[idx] = Bl[idx] + (VEL ? o[q] + cv[q] : o[q]);
                 }
             }
-          wave_sync();
+          phase_sync<SPREAD>();
 
-          // ---- z-lines (qx = s0, qy = s1): q-point physics, integrate z -------
+          // ---- z-lines (qx = s0, qy = s1): add the x- and y-fluxes, integrate z
 #pragma unroll
-          for (int r = 0; r < R; ++r)
+          for (int r = 0; r < RR; ++r)
             {
               const int s0 = s0v[r], s1 = s1v[r];
-              const int cx = min(it.cx0 + cb[r], prm.nx - 1);
               Number *Al = A + cb[r] * STR;
               const Number *Bl = B + cb[r] * STR;
-              const Number *KBl = KB + (H2 ? cb[r] * STR : 0);
-              Number f[N], o[N], V[N], y1[N];
-              const Number wxy = (Number)T1.w[s0] * (Number)T1.w[s1];
+              Number V[N], y1[N];
 #pragma unroll
               for (int q = 0; q < N; ++q)
-                {
-                  const Number w3 = (Number)T1.w[q] * wxy;
-                  const Number k = conductivity<Number, COEF, N>(prm, KBl, cx, it.cy, cz, s0, s1, q);
-                  f[q] = k * w3 * prm.kf[2] * Gz[r][q];
-                }
-              line_apply<P, kLmCt, MF>(LC, f, o);
-#pragma unroll
-              for (int q = 0; q < N; ++q)
-                {
-                  const Number w3 = (Number)T1.w[q] * wxy;
-                  const Number mass = H2 ? RC[r][q] * prm.dtinv_vol : prm.mass;
-                  Number val = mass * w3 * U[r][q] + o[q] + Bl[(q * N + s1) * N + s0];
-                  if constexpr (VEL)
-                    {
-                      const Number rc = H2 ? RC[r][q] : prm.rho_cp0;
-                      val += rc * w3 * prm.vf[2] * prm.w0 * Gz[r][q];
-                    }
-                  V[q] = val;
-                }
+                V[q] = Vz[r][q] + Bl[(q * N + s1) * N + s0];
               line_apply<P, kLmSt, MF>(LC, V, y1);
 #pragma unroll
               for (int c = 0; c < N; ++c)
                 Al[(c * N + s1) * N + s0] = y1[c];
             }
-          wave_sync();
+          phase_sync<SPREAD>();
 
           // ---- y-lines (qx = s0, c = s1): integrate y ------------------------------
 #pragma unroll
-          for (int r = 0; r < R; ++r)
+          for (int r = 0; r < RR; ++r)
             {
               Number *Al = A + cb[r] * STR;
               Number v[N], o[N];
@@ -838,12 +895,12 @@ namespace mpdg
               for (int b = 0; b < N; ++b)
                 Al[(s1v[r] * N + b) * N + s0v[r]] = o[b];
             }
-          wave_sync();
+          phase_sync<SPREAD>();
 
           // ---- x-lines (b = s0, c = s1): integrate x, publish right faces ---------
-          Number Y[R][N];
+          Number Y[RR][N];
 #pragma unroll
-          for (int r = 0; r < R; ++r)
+          for (int r = 0; r < RR; ++r)
             {
               const Number *Al = A + cb[r] * STR;
               Number v[N];
@@ -861,7 +918,7 @@ namespace mpdg
 
           // ---- sum shared x- and z-faces, scatter -----------------------------------
 #pragma unroll
-          for (int r = 0; r < R; ++r)
+          for (int r = 0; r < RR; ++r)
             {
               const int b = s0v[r], c = s1v[r];
               if (cb[r] > 0)
@@ -903,7 +960,7 @@ namespace mpdg
               }
 
 #pragma unroll
-          for (int r = 0; r < R; ++r)
+          for (int r = 0; r < RR; ++r)
 #pragma unroll
             for (int a = 0; a < N; ++a)
               {
@@ -916,7 +973,7 @@ namespace mpdg
           it = it_n;
           LU = LUn;
           ++step;
-          wave_sync();
+          phase_sync<SPREAD>();
         }
 
       return acc;
@@ -1297,14 +1354,23 @@ namespace mpdg
     // per wavefront under an 80-VGPR budget (6 waves per SIMD), which measured
     // 106.7 against 96.7 GDoF/s for 5 cells at 4 waves (64^3, sine k); with a
     // velocity or a heaviside that budget spills.
+    // Launch shape of the line-form vmult. Wavefront form: CW cells per
+    // wavefront, R line slots per lane. Spread form (FP64 from degree 4, whose
+    // cells do not tile a wavefront: 25, 36, 49 lines): the CPB cells of a
+    // workgroup spread one line per thread over all its wavefronts (250, 252,
+    // 245 of 256 threads), the phases then ordered by workgroup barriers; the
+    // kernel takes CW = CPB and R = 0.
     template <int P, bool FP32, bool MF = false, bool LEAN = false>
     struct VShape
     {
       static constexpr bool L4 = LEAN && FP32 && !MF && P == 4;
-      static constexpr int CW = MF ? (P == 4 ? 2 : P == 5 ? 3 : P == 6 ? 1 : 4)
-                                   : P == 1 ? 16 : P == 2 ? 7 : P == 3 ? 4 : P == 4 ? (FP32 ? (L4 ? 2 : 5) : 2) : P == 5 ? (FP32 ? 3 : 1) : 1;
-      static constexpr int R = (CW * (P + 1) * (P + 1) + 63) / 64;
+      static constexpr bool SPREAD = !FP32 && !MF && P >= 4;
       static constexpr int WPB = 4;
+      static constexpr int CWW = MF ? (P == 4 ? 2 : P == 5 ? 3 : P == 6 ? 1 : 4)
+                                    : P == 1 ? 16 : P == 2 ? 7 : P == 3 ? 4 : P == 4 ? (FP32 ? (L4 ? 2 : 5) : 2) : P == 5 ? (FP32 ? 3 : 1) : 1;
+      static constexpr int CPB = SPREAD ? (64 * WPB) / ((P + 1) * (P + 1)) : CWW * WPB;
+      static constexpr int CW = SPREAD ? CPB : CWW;
+      static constexpr int R = SPREAD ? 0 : (CWW * (P + 1) * (P + 1) + 63) / 64;
       // register budget: 128 VGPRs (4 waves per SIMD), 256 for FP64 from degree 5
       static constexpr int MINW = L4 ? 6 : ((P >= 5 && !FP32) || (MF && P == 5)) ? 2 : kMinWavesPerSimd;
     };
@@ -1376,7 +1442,7 @@ namespace mpdg
     {
       using Shape = VShape<P, sizeof(Number) == 4, MF, COEF != kCoefHeaviside && !VEL>;
       constexpr int THREADS = 64 * Shape::WPB;
-      constexpr int CPB = Shape::CW * Shape::WPB;
+      constexpr int CPB = Shape::CPB;
       auto kernel = vmult_kernel<P, Number, COEF, VEL, Shape::CW, Shape::R, Shape::WPB, Shape::MINW, MF>;
       static int resident = 0; // workgroups per CU, from the occupancy calculator
       if (resident == 0)
@@ -1409,7 +1475,7 @@ namespace mpdg
           // tuned count above the kernel's own occupancy is capped, so the
           // waits' forward progress does not rest on the dispatch order
           const int opc = prm.blocks_per_cu > 0 ? std::min(prm.blocks_per_cu, oresident) : oresident;
-          const VmultGrid og8 = vmult_grid(prm.nx, prm.ny, prm.nz, OShape::CW * OShape::WPB,
+          const VmultGrid og8 = vmult_grid(prm.nx, prm.ny, prm.nz, OShape::CPB,
                                            prm.zlayers, opc * device_cu_count());
           OrderGrid og{};
           for (int c = 0; c < 8; ++c)
@@ -1746,9 +1812,7 @@ namespace mpdg
   {
 #define MPDG_CPB(P)                                                                               \
   case P:                                                                                         \
-    return mf ? VShape<P, true, true>::CW * VShape<P, true, true>::WPB                            \
-              : fp32 ? VShape<P, true>::CW * VShape<P, true>::WPB                                 \
-                     : VShape<P, false>::CW * VShape<P, false>::WPB;
+    return mf ? VShape<P, true, true>::CPB : fp32 ? VShape<P, true>::CPB : VShape<P, false>::CPB;
     switch (degree)
       {
         MPDG_CPB(1)
