@@ -1345,34 +1345,31 @@ namespace mpdg
         }
     }
 
-    // Wavefront shape of the vmult per degree: CW cells per wavefront, R line
-    // slots per lane (CW*(P+1)^2 lines fill the last slot), 4 wavefronts per
-    // workgroup. The MFMA form holds the matrix columns in 4(P+1) more VGPRs,
-    // so it takes fewer cells per wavefront at degree 4 and 6 and a 256-VGPR
-    // budget at degree 5. LEAN (colour-pass launches of an operator without a
-    // velocity or a nodal heaviside): the FP32 degree-4 FMA form takes 2 cells
-    // per wavefront under an 80-VGPR budget (6 waves per SIMD), which measured
-    // 106.7 against 96.7 GDoF/s for 5 cells at 4 waves (64^3, sine k); with a
-    // velocity or a heaviside that budget spills.
-    // Launch shape of the line-form vmult. Wavefront form: CW cells per
-    // wavefront, R line slots per lane. Spread form (FP64 from degree 4, whose
-    // cells do not tile a wavefront: 25, 36, 49 lines): the CPB cells of a
-    // workgroup spread one line per thread over all its wavefronts (250, 252,
-    // 245 of 256 threads), the phases then ordered by workgroup barriers; the
-    // kernel takes CW = CPB and R = 0.
+    // Launch shape of the line-form vmult. Wavefront form (degrees 1-3 and the
+    // MFMA form): CW cells per wavefront, R line slots per lane, 4 wavefronts
+    // per workgroup; the MFMA form holds the matrix columns in 4(P+1) more
+    // VGPRs, so it takes fewer cells per wavefront at degree 4 and 6 and a
+    // 256-VGPR budget at degree 5. Spread form (FMA, from degree 4, whose cells
+    // do not tile a wavefront: 25, 36, 49 lines): the CPB cells of a workgroup
+    // spread one line per thread over all its wavefronts (250, 252, 245 of 256
+    // threads), the phases then ordered by workgroup barriers; the kernel takes
+    // CW = CPB and R = 0. Register budgets: 128 VGPRs (4 waves per SIMD); FP64
+    // from degree 5 256; FP32 degree 4 of an operator without a velocity or a
+    // nodal heaviside (LEAN, the mixed CG's inner operator) 80 (6 waves), which
+    // measured 130.6 against 128.2 GDoF/s at 4 waves (64^3, sine k) and 115
+    // for the wavefront form (2 cells per wavefront, 6 waves).
     template <int P, bool FP32, bool MF = false, bool LEAN = false>
     struct VShape
     {
-      static constexpr bool L4 = LEAN && FP32 && !MF && P == 4;
-      static constexpr bool SPREAD = !FP32 && !MF && P >= 4;
+      static constexpr bool SPREAD = !MF && P >= 4;
       static constexpr int WPB = 4;
       static constexpr int CWW = MF ? (P == 4 ? 2 : P == 5 ? 3 : P == 6 ? 1 : 4)
-                                    : P == 1 ? 16 : P == 2 ? 7 : P == 3 ? 4 : P == 4 ? (FP32 ? (L4 ? 2 : 5) : 2) : P == 5 ? (FP32 ? 3 : 1) : 1;
+                                    : P == 1 ? 16 : P == 2 ? 7 : P == 3 ? 4 : 1;
       static constexpr int CPB = SPREAD ? (64 * WPB) / ((P + 1) * (P + 1)) : CWW * WPB;
       static constexpr int CW = SPREAD ? CPB : CWW;
       static constexpr int R = SPREAD ? 0 : (CWW * (P + 1) * (P + 1) + 63) / 64;
-      // register budget: 128 VGPRs (4 waves per SIMD), 256 for FP64 from degree 5
-      static constexpr int MINW = L4 ? 6 : ((P >= 5 && !FP32) || (MF && P == 5)) ? 2 : kMinWavesPerSimd;
+      static constexpr int MINW = (LEAN && FP32 && !MF && P == 4) ? 6
+                                  : ((P >= 5 && !FP32) || (MF && P == 5)) ? 2 : kMinWavesPerSimd;
     };
 
     // Slab schedule of the colour passes (prm.slabs == 2): the z-chunks of the
