@@ -16,6 +16,23 @@ namespace mpdg
 
   namespace
   {
+    // acc + c x, where a compile-time zero c adds nothing (GLL nodes and Gauss
+    // points share the cell centre at odd point counts, so the interpolation
+    // matrix's middle row and its transpose's middle column are zero off the
+    // diagonal). `set` says whether acc holds a term yet; after unrolling, c
+    // and set are constants and the tests fold away.
+    template <typename Number>
+    __device__ __forceinline__ void eo_term(Number &acc, bool &set, double c, Number x)
+    {
+      if (c == 0.0)
+        return;
+      if (c == 1.0)
+        acc = set ? acc + x : x;
+      else
+        acc = set ? acc + Number(c) * x : Number(c) * x;
+      set = true;
+    }
+
     // y = M x with a compile-time even-odd table (entries folded into the code).
     template <int N, int SIGN, typename Number>
     __device__ __forceinline__ void eo_apply(const ct::EO<N> &M, const Number (&x)[N], Number (&y)[N])
@@ -31,29 +48,27 @@ namespace mpdg
 #pragma unroll
       for (int i = 0; i < H; ++i)
         {
-          Number a = Number(M.e[i][0]) * E[0];
-          Number b = Number(M.o[i][0]) * O[0];
+          Number a = Number(0), b = Number(0);
+          bool sa = false, sb = false;
 #pragma unroll
-          for (int j = 1; j < H; ++j)
+          for (int j = 0; j < H; ++j)
             {
-              a += Number(M.e[i][j]) * E[j];
-              b += Number(M.o[i][j]) * O[j];
+              eo_term(a, sa, M.e[i][j], E[j]);
+              eo_term(b, sb, M.o[i][j], O[j]);
             }
           if constexpr (N % 2 == 1)
-            a += Number(M.mc[i]) * x[H];
+            eo_term(a, sa, M.mc[i], x[H]);
           y[i] = a + b;
           y[N - 1 - i] = SIGN > 0 ? a - b : b - a;
         }
       if constexpr (N % 2 == 1)
         {
-          // an exactly-zero centre entry (the derivative's) is left out: the
-          // table is a compile-time constant, so the test folds away
-          Number m = Number(M.mr[0]) * (SIGN > 0 ? E[0] : O[0]);
+          Number m = Number(0);
+          bool sm = false;
 #pragma unroll
-          for (int j = 1; j < H; ++j)
-            m += Number(M.mr[j]) * (SIGN > 0 ? E[j] : O[j]);
-          if (M.mm != 0.0)
-            m += Number(M.mm) * x[H];
+          for (int j = 0; j < H; ++j)
+            eo_term(m, sm, M.mr[j], SIGN > 0 ? E[j] : O[j]);
+          eo_term(m, sm, M.mm, x[H]);
           y[H] = m;
         }
     }

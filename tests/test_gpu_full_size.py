@@ -14,7 +14,8 @@ against the CPU oracle at its full size, on the same seeded inputs.
 * 96^3 Q3 two-phase sphere (k 1:100, rho_cp 1:10): FP64 vmult and inverse
   diagonal within 1e-12 of the oracle (column-by-column diagonal); FP64 CG to
   1e-8 in the oracle's iteration count (_check_cg_iterations) with its
-  residual history; FP32-mixed at 1e-8 in the oracle's
+  residual history, x within 5e-8 of the oracle's iterate at the same count
+  and at 1e-8 in the oracle's residual; FP32-mixed at 1e-8 in the oracle's
   residual and within 1e-6 of the oracle's solution.
 * the 64-member heterogeneous batch (n ~ U{8..48}, p ~ U{1..4}): every
   member's FP32-mixed batch solution meets 1e-8 in the oracle's residual; the
@@ -61,11 +62,16 @@ def _np(t):
     return t.cpu().numpy()
 
 
+def _solve_into(op, x, b, **kw):
+    from meltpooldg_amd import LinearSolver, LinearSolverData
+    return LinearSolver.solve(op, x, b, LinearSolverData(preconditioner_type="Diagonal", **kw))
+
+
 def _solve(op, b, **kw):
     import torch
-    from meltpooldg_amd import LinearSolver, LinearSolverData
+    from meltpooldg_amd import LinearSolver
     x = torch.zeros_like(b)
-    it = LinearSolver.solve(op, x, b, LinearSolverData(preconditioner_type="Diagonal", **kw))
+    it = _solve_into(op, x, b, **kw)
     return x, it, dict(LinearSolver.last_report)
 
 
@@ -99,6 +105,32 @@ def _check_cg_iterations(op, it, ref, rel_tol, label):
     assert abs(h_gpu[k] / tol - 1.0) < 1e-2 and abs(h_ora[k] / tol - 1.0) < 1e-2
 
 
+def _check_cg_solution(op, pb, b, x, it, ref, inv, rel_tol, x_tol=1e-8):
+    """The GPU's CG solution against the oracle's after the same number of
+    iterations: where the stop moved by one (_check_cg_iterations), the side
+    that stopped later is re-run to the earlier count (the oracle with its
+    iteration cap; the GPU with max_iterations, which then raises
+    NoConvergence and leaves its iterate in x), so both x are the same
+    iterate of the same recurrence. x_tol bounds their relative max-norm
+    difference (rounding drift amplified by CG; see the sphere test for the
+    one case that needs more than 1e-8)."""
+    from meltpooldg_amd import NoConvergence
+    k = min(it, ref["iterations"])
+    ref_x = ref["x"]
+    gpu_x = _np(x)
+    if it < ref["iterations"]:
+        ref_x = pb.solve(b, rel_tol=rel_tol, inv_diag=inv, max_it=k)["x"]
+    elif it > ref["iterations"]:
+        import torch
+        x2 = torch.zeros_like(x)
+        with pytest.raises(NoConvergence):
+            _solve_into(op, x2, _t(b), solver_type="CG", rel_tolerance=rel_tol, max_iterations=k)
+        gpu_x = _np(x2)
+    err = rel_err(gpu_x, ref_x)
+    print("cg solution after", k, "iterations: rel err", err, flush=True)
+    assert err < x_tol
+
+
 def _oracle_rel_residual(pb, x, b):
     r = b - pb.vmult(x)
     return float(np.linalg.norm(r) / np.linalg.norm(b))
@@ -122,7 +154,7 @@ def test_q4_64_sine_cg_matches_oracle(cpu):
     op.enable_history_data(2000)
     x, it, rep = _solve(op, _t(b), solver_type="CG", rel_tolerance=1e-8)
     _check_cg_iterations(op, it, ref, 1e-8, "q4 cg")
-    assert rel_err(_np(x), ref["x"]) < 1e-8
+    _check_cg_solution(op, pb, b, x, it, ref, inv, 1e-8)
     xm, itm, repm = _solve(op, _t(b), solver_type="CG", rel_tolerance=1e-8, mixed_precision=True,
                            inner_rel_tolerance=1e-4)
     xm = _np(xm)
@@ -187,7 +219,15 @@ def test_q3_96_sphere_two_phase_matches_oracle(cpu):
     op.enable_history_data(2000)
     x, it, rep = _solve(op, _t(b), solver_type="CG", rel_tolerance=1e-8)
     _check_cg_iterations(op, it, ref, 1e-8, "q3 sphere cg")
-    assert rel_err(_np(x), ref["x"]) < 1e-8
+    # k 1:100 across the sphere: the two FP64 recurrences' residual histories
+    # drift apart by up to ~1.5 % over 577 iterations, and their iterates at
+    # equal counts by ~1.1e-8 (measured), while the oracle's own 1e-8 solution
+    # is 3.1e-8 from its 1e-12 solution (and its iterates 577 and 578 differ by
+    # 7.5e-9): the drift is a third of the solve's own error. The GPU's x must
+    # stay within 5e-8 of the oracle's iterate and meet 1e-8 in the oracle's
+    # own residual.
+    _check_cg_solution(op, pb, b, x, it, ref, inv, 1e-8, x_tol=5e-8)
+    assert _oracle_rel_residual(pb, _np(x), b) <= 1e-8 * 1.01
     xm, itm, repm = _solve(op, _t(b), solver_type="CG", rel_tolerance=1e-8, mixed_precision=True)
     xm = _np(xm)
     res = _oracle_rel_residual(pb, xm, b)
