@@ -289,9 +289,13 @@ const char *mpdg_last_error(void);
  * follows mpdg_heat_solve (CG + Jacobi, optional FP32 defect correction). All
  * members are solved together: one batched vmult launch per degree and colour,
  * segmented vector kernels, per-solve convergence flags, and the iteration
- * chunk captured as one hipGraph and replayed. Members must share the material
- * kind, conductivity field and velocity field; meshes and degrees may differ.
- * Members with Robin faces are rejected (MPDG_ERR_UNSUPPORTED). */
+ * chunk captured as one hipGraph and replayed. With mixed precision every
+ * member runs its own defect correction on the device (its FP64 residual, stop
+ * test and FP32 inner restart happen when its own inner solve stops), so a
+ * member finishes when it converges, not when the slowest one does. Members
+ * must share the material kind, conductivity field and velocity field; meshes
+ * and degrees may differ. Members with Robin faces are rejected
+ * (MPDG_ERR_UNSUPPORTED). */
 typedef struct mpdg_heat_batch mpdg_heat_batch;
 typedef struct {
   int status;               /* 0 converged, 1 not converged */
@@ -299,7 +303,7 @@ typedef struct {
   int outer_iterations;     /* mixed precision: defect-correction steps */
   double initial_residual;
   double final_residual;    /* FP64 ||b - A x|| at exit (mixed), PCG residual otherwise */
-  double seconds;           /* time from the start of the batch solve to this member's convergence */
+  double seconds;           /* device wall clock from the start of the batch solve to this member's stop */
 } mpdg_batch_report;
 mpdg_status mpdg_heat_batch_create(const mpdg_heat_desc *descs, int count, mpdg_heat_batch **out);
 mpdg_status mpdg_heat_batch_destroy(mpdg_heat_batch *b);

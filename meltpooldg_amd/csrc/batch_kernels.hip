@@ -125,6 +125,7 @@ namespace mpdg
             {
               sc.done = 1;
               sc.status = on ? 0 : 2;
+              sc.t_done = wall_clock64();
             }
         }
     }
@@ -167,11 +168,13 @@ namespace mpdg
             {
               sc.done = 1;
               sc.status = 0;
+              sc.t_done = wall_clock64();
             }
           else if (it >= sc.max_it || isnan(res))
             {
               sc.done = 1;
               sc.status = 1;
+              sc.t_done = wall_clock64();
             }
         }
     }
@@ -212,7 +215,187 @@ namespace mpdg
       if (seg_reduce<1>(v, sg, nblocks, partials, counters + s, tot))
         res[s] = sqrt(tot[0]);
     }
+    // One ticket per workgroup of solve sg: true in the last one to arrive
+    // (every other workgroup of the solve has read its state by then).
+    __device__ __forceinline__ bool seg_ticket(const BatchSeg &sg, unsigned *counter)
+    {
+      __shared__ bool last;
+      __syncthreads();
+      if (threadIdx.x == 0)
+        {
+          wait_vmem();
+          last = atomicAdd(counter, 1u) == (unsigned)sg.nblk - 1;
+          if (last)
+            *counter = 0u;
+        }
+      __syncthreads();
+      return last && threadIdx.x == 0;
+    }
+
+    __global__ void batch_begin_kernel(BatchOuter *st, SolverScalars *scal, SolverScalars *ogate,
+                                       int nsolve)
+    {
+      const int s = blockIdx.x * blockDim.x + threadIdx.x;
+      if (s >= nsolve)
+        return;
+      BatchOuter o{};
+      o.phase = kPhaseOuter;
+      o.status = 1;
+      o.t_start = wall_clock64();
+      st[s] = o;
+      scal[s].done = 1;
+      scal[s].counter = 0u;
+      ogate[s].done = 0;
+    }
+
+    // The phase is read by every workgroup of a solve before its ticket and
+    // changed only by the last one, so all workgroups of a solve take the same
+    // branch.
+    __global__ void __launch_bounds__(T)
+      batch_outer_step_kernel(double *r, const double *b, const double *ax, const BatchSeg *seg,
+                              int nsolve, int nblocks, double *partials, BatchOuter *st,
+                              SolverScalars *ogate, double abs_tol, double rel_tol,
+                              double inner_rel, int max_it)
+    {
+      const int s = find_solve(seg, nsolve, blockIdx.x);
+      if (st[s].phase != kPhaseOuter)
+        return;
+      const BatchSeg sg = seg[s];
+      double v[1] = {0.0};
+      const int64_t end = sg.off + sg.n;
+      for (int64_t i = sg.off + (int64_t)(blockIdx.x - sg.blk0) * T + threadIdx.x; i < end;
+           i += (int64_t)sg.nblk * T)
+        {
+          const double ri = b[i] - ax[i];
+          r[i] = ri;
+          v[0] += ri * ri;
+        }
+      double tot[1];
+      if (seg_reduce<1>(v, sg, nblocks, partials, &st[s].counter, tot))
+        {
+          BatchOuter &o = st[s];
+          const double res = sqrt(tot[0]);
+          if (o.outer == 0)
+            {
+              o.res0 = res;
+              o.tol = fmax(abs_tol, rel_tol * res);
+            }
+          o.res = res;
+          ogate[s].done = 1;
+          if (res <= o.tol || !isfinite(res) || o.inner_total >= max_it)
+            {
+              o.status = res <= o.tol ? 0 : 1;
+              o.phase = kPhaseDone;
+              o.t_done = wall_clock64();
+            }
+          else
+            {
+              // the inner solve need not go below half the outer tolerance
+              o.inner_rel = fmax(inner_rel, 0.5 * o.tol / res);
+              o.outer += 1;
+              o.phase = kPhaseInnerStart;
+            }
+        }
+    }
+
+    __global__ void __launch_bounds__(T)
+      batch_inner_start_kernel(const double *r, float *r32, float *p32, float *x32,
+                               const float *inv, const BatchSeg *seg, int nsolve, int nblocks,
+                               double *partials, BatchOuter *st, SolverScalars *scal, int max_it)
+    {
+      const int s = find_solve(seg, nsolve, blockIdx.x);
+      if (st[s].phase != kPhaseInnerStart)
+        return;
+      const BatchSeg sg = seg[s];
+      double v[2] = {0.0, 0.0};
+      const int64_t end = sg.off + sg.n;
+      for (int64_t i = sg.off + (int64_t)(blockIdx.x - sg.blk0) * T + threadIdx.x; i < end;
+           i += (int64_t)sg.nblk * T)
+        {
+          const float ri = (float)r[i];
+          const float zi = ri * inv[i];
+          r32[i] = ri;
+          p32[i] = zi;
+          x32[i] = 0.0f;
+          v[0] += (double)ri * (double)ri;
+          v[1] += (double)ri * (double)zi;
+        }
+      double tot[2];
+      if (seg_reduce<2>(v, sg, nblocks, partials, &st[s].counter, tot))
+        {
+          BatchOuter &o = st[s];
+          SolverScalars &sc = scal[s];
+          const double res = sqrt(tot[0]);
+          sc.rr = tot[0];
+          sc.rz = tot[1];
+          sc.res = res;
+          sc.res0 = res;
+          sc.tol = o.inner_rel * res;
+          sc.iteration = 0;
+          sc.max_it = max(1, max_it - o.inner_total);
+          sc.status = 1;
+          sc.done = res <= sc.tol ? 1 : 0;
+          o.phase = kPhaseInner;
+        }
+    }
+
+    __global__ void __launch_bounds__(T)
+      batch_inner_finish_kernel(double *x, const float *x32, const BatchSeg *seg, int nsolve,
+                                BatchOuter *st, const SolverScalars *scal, SolverScalars *ogate)
+    {
+      const int s = find_solve(seg, nsolve, blockIdx.x);
+      if (st[s].phase != kPhaseInner || !scal[s].done)
+        return;
+      const BatchSeg sg = seg[s];
+      const int64_t end = sg.off + sg.n;
+      for (int64_t i = sg.off + (int64_t)(blockIdx.x - sg.blk0) * T + threadIdx.x; i < end;
+           i += (int64_t)sg.nblk * T)
+        x[i] += (double)x32[i];
+      if (seg_ticket(sg, &st[s].counter))
+        {
+          st[s].inner_total += scal[s].iteration;
+          st[s].phase = kPhaseOuter;
+          ogate[s].done = 0;
+        }
+    }
   } // namespace
+
+  hipError_t launch_batch_begin(BatchOuter *st, SolverScalars *scal, SolverScalars *ogate,
+                                int nsolve, hipStream_t s)
+  {
+    batch_begin_kernel<<<(nsolve + 63) / 64, 64, 0, s>>>(st, scal, ogate, nsolve);
+    return hipGetLastError();
+  }
+
+  hipError_t launch_batch_outer_step(double *r, const double *b, const double *ax,
+                                     const BatchSeg *seg, int nsolve, int nblocks,
+                                     double *partials, BatchOuter *st, SolverScalars *ogate,
+                                     double abs_tol, double rel_tol, double inner_rel, int max_it,
+                                     hipStream_t s)
+  {
+    batch_outer_step_kernel<<<nblocks, T, 0, s>>>(r, b, ax, seg, nsolve, nblocks, partials, st,
+                                                  ogate, abs_tol, rel_tol, inner_rel, max_it);
+    return hipGetLastError();
+  }
+
+  hipError_t launch_batch_inner_start(const double *r, float *r32, float *p32, float *x32,
+                                      const float *inv, const BatchSeg *seg, int nsolve,
+                                      int nblocks, double *partials, BatchOuter *st,
+                                      SolverScalars *scal, int max_it, hipStream_t s)
+  {
+    batch_inner_start_kernel<<<nblocks, T, 0, s>>>(r, r32, p32, x32, inv, seg, nsolve, nblocks,
+                                                   partials, st, scal, max_it);
+    return hipGetLastError();
+  }
+
+  hipError_t launch_batch_inner_finish(double *x, const float *x32, const BatchSeg *seg,
+                                       int nsolve, int nblocks, BatchOuter *st,
+                                       const SolverScalars *scal, SolverScalars *ogate,
+                                       hipStream_t s)
+  {
+    batch_inner_finish_kernel<<<nblocks, T, 0, s>>>(x, x32, seg, nsolve, st, scal, ogate);
+    return hipGetLastError();
+  }
 
   template <typename Number>
   hipError_t launch_batch_cg_init(const double *b, const Number *ax, Number *r, Number *p,

@@ -2050,8 +2050,16 @@ struct mpdg_heat_batch
   DevBuf<float> R32, P32, Q32, X32, D32;
   DevBuf<int> active;
   DevBuf<SolverScalars> scal;
+  DevBuf<SolverScalars> ogate; // .done = 1: member out of the FP64 residual vmult
+  DevBuf<BatchOuter> outer;    // per-member defect-correction state (mixed solves)
   SolverScalars *host_scal = nullptr;
+  BatchOuter *host_outer = nullptr;
   double *host_res = nullptr;
+  double clock_hz = 0.0;       // device wall clock rate
+  hipGraphExec_t graph_mixed = nullptr;
+  int graph_steps_mixed = 0;
+  double graph_abs = -1.0, graph_rel = -1.0, graph_inner = -1.0;
+  int graph_max_it = -1;
   std::vector<BatchGroup<double>> g64;
   std::vector<BatchGroup<float>> g32;
   int vmult_slots64 = 0, vmult_slots32 = 0;
@@ -2066,6 +2074,8 @@ struct mpdg_heat_batch
       (void)hipGraphExecDestroy(graph64);
     if (graph32)
       (void)hipGraphExecDestroy(graph32);
+    if (graph_mixed)
+      (void)hipGraphExecDestroy(graph_mixed);
     for (auto *m : members)
       delete m;
     for (auto *b : {&X, &B, &R, &P, &Q, &D, &partials, &rel, &res})
@@ -2074,6 +2084,8 @@ struct mpdg_heat_batch
       b->release();
     active.release();
     scal.release();
+    ogate.release();
+    outer.release();
     seg.release();
     for (auto &g : g64)
       {
@@ -2091,6 +2103,8 @@ struct mpdg_heat_batch
       }
     if (host_scal)
       (void)hipHostFree(host_scal);
+    if (host_outer)
+      (void)hipHostFree(host_outer);
     if (host_res)
       (void)hipHostFree(host_res);
     if (own_stream && stream)
@@ -2249,6 +2263,56 @@ namespace
     return MPDG_OK;
   }
 
+  // One chunk of the per-member defect correction of a mixed batch solve:
+  // the FP64 residual vmult of the Outer members (the others are gated off),
+  // their outer step, the FP32 PCG start of the InnerStart members, `steps`
+  // FP32 PCG iterations of the Inner members and the finish (x += d) of those
+  // whose inner solve stopped. Every member moves through its own outer steps,
+  // so a member's convergence does not wait for the others'.
+  mpdg_status batch_mixed_chunk(mpdg_heat_batch *B, int steps, const mpdg_solver_params &prm)
+  {
+    double *vpart = B->partials.ptr + B->vmult_slots32;
+    for (auto &g : B->g64)
+      MPDG_HIP(launch_vmult_batch<double>(g.degree, B->coef, B->vel, g.variant64, g.opsx.ptr,
+                                          g.dev_block_off.ptr, g.nops, g.plan, nullptr,
+                                          B->ogate.ptr, B->stream));
+    MPDG_HIP(launch_batch_outer_step(B->R.ptr, B->B.ptr, B->Q.ptr, B->seg.ptr, B->count,
+                                     B->nblocks, vpart, B->outer.ptr, B->ogate.ptr,
+                                     prm.abs_tolerance, prm.rel_tolerance,
+                                     prm.inner_rel_tolerance, prm.max_iterations, B->stream));
+    MPDG_HIP(launch_batch_inner_start(B->R.ptr, B->R32.ptr, B->P32.ptr, B->X32.ptr, B->D32.ptr,
+                                      B->seg.ptr, B->count, B->nblocks, vpart, B->outer.ptr,
+                                      B->scal.ptr, prm.max_iterations, B->stream));
+    if (mpdg_status st = batch_pcg_steps<float>(B, B->g32, steps, B->vmult_slots32, B->X32.ptr,
+                                                B->R32.ptr, B->P32.ptr, B->Q32.ptr, B->D32.ptr))
+      return st;
+    MPDG_HIP(launch_batch_inner_finish(B->X.ptr, B->X32.ptr, B->seg.ptr, B->count, B->nblocks,
+                                       B->outer.ptr, B->scal.ptr, B->ogate.ptr, B->stream));
+    return MPDG_OK;
+  }
+
+  mpdg_status batch_mixed_graph(mpdg_heat_batch *B, int steps, const mpdg_solver_params &prm,
+                                hipGraphExec_t *exec)
+  {
+    hipGraph_t graph = nullptr;
+    MPDG_HIP(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
+    mpdg_status st = batch_mixed_chunk(B, steps, prm);
+    const hipError_t e = hipStreamEndCapture(B->stream, &graph);
+    if (st)
+      {
+        if (graph)
+          (void)hipGraphDestroy(graph);
+        return st;
+      }
+    if (e != hipSuccess)
+      return fail(MPDG_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    const hipError_t e2 = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e2 != hipSuccess)
+      return fail(MPDG_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e2));
+    return MPDG_OK;
+  }
+
   mpdg_status read_batch_scalars(mpdg_heat_batch *B)
   {
     MPDG_HIP(hipMemcpyAsync(B->host_scal, B->scal.ptr, B->count * sizeof(SolverScalars),
@@ -2351,6 +2415,10 @@ mpdg_status mpdg_heat_batch_create(const mpdg_heat_desc *descs, int count, mpdg_
   if (e == hipSuccess)
     e = B->scal.alloc(count);
   if (e == hipSuccess)
+    e = B->ogate.alloc(count);
+  if (e == hipSuccess)
+    e = B->outer.alloc(count);
+  if (e == hipSuccess)
     e = hipMemcpy(B->seg.ptr, seg.data(), count * sizeof(BatchSeg), hipMemcpyHostToDevice);
   if (e == hipSuccess)
     {
@@ -2367,6 +2435,16 @@ mpdg_status mpdg_heat_batch_create(const mpdg_heat_desc *descs, int count, mpdg_
     e = hipHostMalloc((void **)&B->host_scal, count * sizeof(SolverScalars), hipHostMallocDefault);
   if (e == hipSuccess)
     e = hipHostMalloc((void **)&B->host_res, 2 * count * sizeof(double), hipHostMallocDefault);
+  if (e == hipSuccess)
+    e = hipHostMalloc((void **)&B->host_outer, count * sizeof(BatchOuter), hipHostMallocDefault);
+  if (e == hipSuccess)
+    {
+      int dev = 0, khz = 0;
+      e = hipGetDevice(&dev);
+      if (e == hipSuccess)
+        e = hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev);
+      B->clock_hz = 1e3 * (khz > 0 ? khz : 100000);
+    }
   if (e != hipSuccess)
     {
       delete B;
@@ -2407,7 +2485,9 @@ mpdg_status mpdg_heat_batch_set_dt(mpdg_heat_batch *b, double dt)
     (void)hipGraphExecDestroy(b->graph64);
   if (b->graph32)
     (void)hipGraphExecDestroy(b->graph32);
-  b->graph64 = b->graph32 = nullptr;
+  if (b->graph_mixed)
+    (void)hipGraphExecDestroy(b->graph_mixed);
+  b->graph64 = b->graph32 = b->graph_mixed = nullptr;
   if (mpdg_status st = build_groups<double>(b, b->g64, b->vmult_slots64, b->P.ptr, b->Q.ptr))
     return st;
   if (mpdg_status st = build_groups<float>(b, b->g32, b->vmult_slots32, b->P32.ptr, b->Q32.ptr))
@@ -2462,19 +2542,6 @@ mpdg_status mpdg_heat_batch_solve(mpdg_heat_batch *b, double *const *x, const do
   std::vector<double> relv(S, prm.rel_tolerance);
   MPDG_HIP(hipMemcpyAsync(b->rel.ptr, relv.data(), S * sizeof(double), hipMemcpyHostToDevice, b->stream));
 
-  auto outer_residual = [&]() -> mpdg_status {
-    // R = B - A64 X for every member, ||R|| per member
-    for (auto &g : b->g64)
-      MPDG_HIP(launch_vmult_batch<double>(g.degree, b->coef, b->vel, g.variant64, g.opsx.ptr,
-                                          g.dev_block_off.ptr,
-                                          g.nops, g.plan, nullptr, nullptr, b->stream));
-    MPDG_HIP(launch_batch_residual(b->R.ptr, b->B.ptr, b->Q.ptr, b->seg.ptr, S, b->nblocks,
-                                   b->partials.ptr, b->res.ptr, b->stream));
-    MPDG_HIP(hipMemcpyAsync(b->host_res, b->res.ptr, S * sizeof(double), hipMemcpyDeviceToHost,
-                            b->stream));
-    MPDG_HIP(hipStreamSynchronize(b->stream));
-    return MPDG_OK;
-  };
   // opsx was built for the FP64 group layout; its block offsets equal g64's
   // (same degree and precision), so the same dev_block_off serves both.
 
@@ -2491,6 +2558,7 @@ mpdg_status mpdg_heat_batch_solve(mpdg_heat_batch *b, double *const *x, const do
           b->graph_steps64 = steps;
         }
       // r = b - A x0 for all members, then the PCG start
+      MPDG_HIP(launch_batch_begin(b->outer.ptr, b->scal.ptr, b->ogate.ptr, S, b->stream));
       for (auto &g : b->g64)
         MPDG_HIP(launch_vmult_batch<double>(g.degree, b->coef, b->vel, g.variant64, g.opsx.ptr,
                                           g.dev_block_off.ptr,
@@ -2505,13 +2573,12 @@ mpdg_status mpdg_heat_batch_solve(mpdg_heat_batch *b, double *const *x, const do
           if (mpdg_status st = read_batch_scalars(b))
             return st;
           bool all = true;
-          const double t = now();
           for (int s = 0; s < S; ++s)
             {
               const SolverScalars &sc = b->host_scal[s];
               if (sc.done && rep[s].seconds < 0)
                 {
-                  rep[s].seconds = t;
+                  rep[s].seconds = 0.0; // device time stamps below
                   rep[s].status = sc.status == 0 ? 0 : 1;
                   rep[s].iterations = sc.iteration;
                   rep[s].initial_residual = sc.res0;
@@ -2523,83 +2590,58 @@ mpdg_status mpdg_heat_batch_solve(mpdg_heat_batch *b, double *const *x, const do
             break;
           MPDG_HIP(hipGraphLaunch(b->graph64, b->stream));
         }
+      // per-member completion: device wall clock at the stop, from the start stamp
+      MPDG_HIP(hipMemcpyAsync(b->host_outer, b->outer.ptr, S * sizeof(BatchOuter),
+                              hipMemcpyDeviceToHost, b->stream));
+      MPDG_HIP(hipStreamSynchronize(b->stream));
+      for (int s = 0; s < S; ++s)
+        rep[s].seconds =
+          (double)(b->host_scal[s].t_done - b->host_outer[s].t_start) / b->clock_hz;
     }
   else
     {
-      if (!b->graph32 || b->graph_steps32 != steps)
+      if (!b->graph_mixed || b->graph_steps_mixed != steps || b->graph_abs != prm.abs_tolerance ||
+          b->graph_rel != prm.rel_tolerance || b->graph_inner != prm.inner_rel_tolerance ||
+          b->graph_max_it != prm.max_iterations)
         {
-          if (b->graph32)
-            (void)hipGraphExecDestroy(b->graph32);
-          b->graph32 = nullptr;
-          if (mpdg_status st = batch_graph<float>(b, b->g32, steps, b->vmult_slots32, b->X32.ptr,
-                                                  b->R32.ptr, b->P32.ptr, b->Q32.ptr, b->D32.ptr,
-                                                  &b->graph32))
+          if (b->graph_mixed)
+            (void)hipGraphExecDestroy(b->graph_mixed);
+          b->graph_mixed = nullptr;
+          if (mpdg_status st = batch_mixed_graph(b, steps, prm, &b->graph_mixed))
             return st;
-          b->graph_steps32 = steps;
+          b->graph_steps_mixed = steps;
+          b->graph_abs = prm.abs_tolerance;
+          b->graph_rel = prm.rel_tolerance;
+          b->graph_inner = prm.inner_rel_tolerance;
+          b->graph_max_it = prm.max_iterations;
         }
       MPDG_HIP(launch_to_f32(b->D32.ptr, b->D.ptr, b->total, b->stream));
-      std::vector<double> tol(S, 0.0);
-      std::vector<int> act(S, 1), inner_total(S, 0), outer(S, 0);
-      for (int round = 0;; ++round)
+      MPDG_HIP(launch_batch_begin(b->outer.ptr, b->scal.ptr, b->ogate.ptr, S, b->stream));
+      // every chunk moves each unfinished member by at least one outer step
+      // or `steps` inner iterations; the cap only guards against a stop test
+      // that never fires
+      const long max_chunks = 2L * (prm.max_iterations / steps + 2) + 64;
+      bool all = false;
+      for (long chunk = 0; chunk < max_chunks && !all; ++chunk)
         {
-          if (mpdg_status st = outer_residual())
-            return st;
-          const double t = now();
-          bool any = false;
+          MPDG_HIP(hipGraphLaunch(b->graph_mixed, b->stream));
+          MPDG_HIP(hipMemcpyAsync(b->host_outer, b->outer.ptr, S * sizeof(BatchOuter),
+                                  hipMemcpyDeviceToHost, b->stream));
+          MPDG_HIP(hipStreamSynchronize(b->stream));
+          all = true;
           for (int s = 0; s < S; ++s)
-            {
-              const double r = b->host_res[s];
-              if (round == 0)
-                {
-                  tol[s] = std::max(prm.abs_tolerance, prm.rel_tolerance * r);
-                  rep[s].initial_residual = r;
-                }
-              rep[s].final_residual = r;
-              if (act[s] && (r <= tol[s] || !std::isfinite(r) || inner_total[s] >= prm.max_iterations))
-                {
-                  act[s] = 0;
-                  rep[s].seconds = t;
-                  rep[s].status = r <= tol[s] ? 0 : 1;
-                }
-              if (act[s])
-                {
-                  relv[s] = std::max(prm.inner_rel_tolerance, 0.5 * tol[s] / r);
-                  any = true;
-                  ++outer[s];
-                }
-            }
-          if (!any)
-            break;
-          MPDG_HIP(hipMemcpyAsync(b->rel.ptr, relv.data(), S * sizeof(double), hipMemcpyHostToDevice,
-                                  b->stream));
-          MPDG_HIP(hipMemcpyAsync(b->active.ptr, act.data(), S * sizeof(int), hipMemcpyHostToDevice,
-                                  b->stream));
-          MPDG_HIP(launch_fill_f32(b->X32.ptr, 0.0f, b->total, b->stream));
-          MPDG_HIP(launch_batch_cg_init<float>(b->R.ptr, nullptr, b->R32.ptr, b->P32.ptr, b->D32.ptr,
-                                               b->seg.ptr, S, b->nblocks,
-                                               b->partials.ptr + b->vmult_slots32, b->scal.ptr,
-                                               b->rel.ptr, 0.0, prm.max_iterations, b->active.ptr,
-                                               b->stream));
-          for (;;)
-            {
-              if (mpdg_status st = read_batch_scalars(b))
-                return st;
-              bool all = true;
-              for (int s = 0; s < S; ++s)
-                all = all && b->host_scal[s].done;
-              if (all)
-                break;
-              MPDG_HIP(hipGraphLaunch(b->graph32, b->stream));
-            }
-          for (int s = 0; s < S; ++s)
-            if (act[s])
-              inner_total[s] += b->host_scal[s].iteration;
-          MPDG_HIP(launch_add_f32(b->X.ptr, b->X32.ptr, b->total, b->stream));
+            all = all && b->host_outer[s].phase == kPhaseDone;
         }
       for (int s = 0; s < S; ++s)
         {
-          rep[s].iterations = inner_total[s];
-          rep[s].outer_iterations = outer[s];
+          const BatchOuter &o = b->host_outer[s];
+          rep[s].iterations = o.inner_total;
+          rep[s].outer_iterations = o.outer;
+          rep[s].initial_residual = o.res0;
+          rep[s].final_residual = o.res;
+          rep[s].status = o.phase == kPhaseDone && o.status == 0 ? 0 : 1;
+          if (o.phase == kPhaseDone)
+            rep[s].seconds = (double)(o.t_done - o.t_start) / b->clock_hz;
         }
     }
   for (int s = 0; s < S; ++s)

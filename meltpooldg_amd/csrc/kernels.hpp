@@ -106,7 +106,8 @@ namespace mpdg
     double res0;
     int iteration, done, status, max_it;
     unsigned counter;
-    unsigned pad[3];
+    unsigned pad;
+    unsigned long long t_done; // device wall clock when done was set (batched solves)
   };
 
   // Velocity of the advection term: none, the separable tables of CellParams
@@ -401,6 +402,46 @@ namespace mpdg
   hipError_t launch_batch_residual(double *r, const double *b, const double *ax,
                                    const BatchSeg *seg, int nsolve, int nblocks, double *partials,
                                    double *res, hipStream_t s);
+
+  // Per-member defect correction of the batched mixed-precision solve, run on
+  // the device so that every member moves through its own outer steps:
+  // Outer (r = b - A64 x, stop test) -> InnerStart (r32, d = 0, PCG start) ->
+  // Inner (FP32 PCG chunks until its own stop) -> Outer (x += d) ... -> Done.
+  enum BatchPhase
+  {
+    kPhaseOuter = 0,
+    kPhaseInnerStart = 1,
+    kPhaseInner = 2,
+    kPhaseDone = 3
+  };
+  struct BatchOuter
+  {
+    double tol, res0, res, inner_rel;
+    unsigned long long t_start, t_done; // device wall clock (wall_clock64 ticks)
+    int phase, outer, inner_total, status;
+    unsigned counter;
+    unsigned pad[3];
+  };
+  // Every member to Outer (mixed) with its inner PCG stopped; t_start stamped.
+  // ogate[s].done = 1 keeps a member out of the FP64 residual vmult.
+  hipError_t launch_batch_begin(BatchOuter *st, SolverScalars *scal, SolverScalars *ogate,
+                                int nsolve, hipStream_t s);
+  // Outer members: r = b - ax, ||r||, then Done or InnerStart.
+  hipError_t launch_batch_outer_step(double *r, const double *b, const double *ax,
+                                     const BatchSeg *seg, int nsolve, int nblocks,
+                                     double *partials, BatchOuter *st, SolverScalars *ogate,
+                                     double abs_tol, double rel_tol, double inner_rel, int max_it,
+                                     hipStream_t s);
+  // InnerStart members: r32 = r, d = 0, p = D r32, their FP32 PCG scalars; -> Inner.
+  hipError_t launch_batch_inner_start(const double *r, float *r32, float *p32, float *x32,
+                                      const float *inv, const BatchSeg *seg, int nsolve,
+                                      int nblocks, double *partials, BatchOuter *st,
+                                      SolverScalars *scal, int max_it, hipStream_t s);
+  // Inner members whose PCG stopped: x += d, iterations counted; -> Outer.
+  hipError_t launch_batch_inner_finish(double *x, const float *x32, const BatchSeg *seg,
+                                       int nsolve, int nblocks, BatchOuter *st,
+                                       const SolverScalars *scal, SolverScalars *ogate,
+                                       hipStream_t s);
 
   // GMRES building blocks (classical Gram-Schmidt over a basis of at most
   // kMaxBasis vectors stored with leading dimension ld). A non-null `stop`

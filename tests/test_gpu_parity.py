@@ -288,17 +288,28 @@ def _batch_members(seed, count, nmin, nmax):
 
 
 def test_batch_matches_single_solves():
-    """Batched CG (graph-captured chunks, per-member convergence) gives each
-    member the iteration count and solution of its own eager solve; the batch
-    is solved twice to replay the captured graph again after eager work."""
+    """Batched CG (graph-captured chunks, per-member convergence): each member
+    takes the oracle's PCG iteration count (within one, as CG rounding may move
+    the stop) and its solution is within 1e-8 of the oracle's; it also matches
+    its own eager single solve. The batch is solved twice to replay the
+    captured graph again after eager work. Completion times are the device's
+    own per-member stamps: positive and not all equal."""
     import torch
     from meltpooldg_amd import HeatBatch, HeatDiffuseOperator, LinearSolver, LinearSolverData
+    from oracle import cpu
     from oracle.rng import random_vector
     members = _batch_members(7, 7, 3, 12)
     batch = HeatBatch(members)
     batch.reset_time_increment(1e-3)
     data = LinearSolverData(solver_type="CG", preconditioner_type="Diagonal", rel_tolerance=1e-9)
-    bs = [_t(random_vector(m["n_cells"], m["degree"], 1000 + i)) for i, m in enumerate(members)]
+    bnp = [random_vector(m["n_cells"], m["degree"], 1000 + i) for i, m in enumerate(members)]
+    bs = [_t(b) for b in bnp]
+    refs = []
+    for i, m in enumerate(members):
+        pb = cpu.Problem(m["n_cells"], m["degree"], dt=1e-3,
+                         material={"type": "gas", "gas_rho": 1.0, "gas_cp": 1.0,
+                                   "gas_k": m["material"].gas.thermal_conductivity})
+        refs.append(pb.solve(bnp[i], rel_tol=1e-9, inv_diag=pb.inverse_diagonal()))
     for attempt in range(2):
         xs = [torch.zeros_like(b) for b in bs]
         reps = batch.solve(xs, bs, data)
@@ -307,10 +318,14 @@ def test_batch_matches_single_solves():
             op.reset_time_increment(1e-3)
             x = torch.zeros_like(bs[i])
             it = LinearSolver.solve(op, x, bs[i], data)
-            print("member", i, m["n_cells"], m["degree"], reps[i]["iterations"], it, flush=True)
+            print("member", i, m["n_cells"], m["degree"], reps[i]["iterations"], it,
+                  refs[i]["iterations"], reps[i]["seconds"], flush=True)
+            assert abs(reps[i]["iterations"] - refs[i]["iterations"]) <= 1
+            assert rel_err(_np(xs[i]), refs[i]["x"]) < 1e-8
             assert abs(reps[i]["iterations"] - it) <= 1
             assert float((xs[i] - x).abs().max() / x.abs().max()) < 1e-8
             assert reps[i]["status"] == 0 and reps[i]["seconds"] > 0
+        assert len({r["seconds"] for r in reps}) > 1
 
 
 def test_batch_mixed_precision_vs_oracle():
@@ -326,9 +341,17 @@ def test_batch_mixed_precision_vs_oracle():
     bnp = [pb.random(1000 + i) for i, pb in enumerate(pbs)]
     refs = [pb.solve(b, rel_tol=1e-12, inv_diag=pb.inverse_diagonal()) for pb, b in zip(pbs, bnp)]
     xs = [torch.zeros(pb.N, dtype=torch.float64, device="cuda") for pb in pbs]
-    reps = batch.solve(xs, [_t(b) for b in bnp],
-                       LinearSolverData(solver_type="CG", rel_tolerance=1e-8, mixed_precision=True))
-    for i in range(len(members)):
-        print("mixed member", i, reps[i], flush=True)
-        assert reps[i]["final_residual"] <= 1e-8 * reps[i]["initial_residual"]
-        assert rel_err(_np(xs[i]), refs[i]["x"]) < 1e-6
+    data = LinearSolverData(solver_type="CG", rel_tolerance=1e-8, mixed_precision=True)
+    # twice: the second solve replays the captured chunk graph
+    for attempt in range(2):
+        for x in xs:
+            x.zero_()
+        reps = batch.solve(xs, [_t(b) for b in bnp], data)
+        for i in range(len(members)):
+            print("mixed member", i, reps[i], flush=True)
+            assert reps[i]["status"] == 0
+            assert reps[i]["final_residual"] <= 1e-8 * reps[i]["initial_residual"]
+            assert rel_err(_np(xs[i]), refs[i]["x"]) < 1e-6
+            assert reps[i]["seconds"] > 0 and reps[i]["outer_iterations"] >= 1
+        # each member's own device completion stamp: they differ
+        assert len({r["seconds"] for r in reps}) > 1
