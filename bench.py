@@ -125,11 +125,35 @@ def cpu_baseline(threads=None, budget_s=10.0, min_samples=20):
             "host": host}
 
 
+def line_kernel_name(P, fp32, coef, vel, mf=False):
+    """rocprof name of the line-form vmult kernel instantiated for degree P:
+    the launch shape of the library's VShape (spread form from degree 4 without
+    MFMA: the CPB cells of a 4-wavefront workgroup, one line per thread; else
+    CW cells per wavefront in R lane slots) and its register budget."""
+    spread = not mf and P >= 4
+    nl = (P + 1) ** 2
+    if mf:
+        cww = {4: 2, 5: 3, 6: 1}.get(P, 4)
+    else:
+        cww = {1: 16, 2: 7, 3: 4}.get(P, 1)
+    cw = (64 * 4) // nl if spread else cww
+    r = 0 if spread else (cww * nl + 63) // 64
+    lean = coef != 2 and not vel
+    if lean and fp32 and not mf and P == 4:
+        minw = 6
+    elif (P >= 5 and not fp32) or (mf and P == 5):
+        minw = 2
+    else:
+        minw = 4
+    b = lambda v: "true" if v else "false"
+    return (f"vmult_kernel<{P}, {'float' if fp32 else 'double'}, {coef}, {b(vel)}, {cw}, {r}, 4, "
+            f"{minw}, {b(mf)}>")
+
+
 def kernel_label(tuning):
-    """The line kernel of the 64^3 Q4 FP64 workload (4 wavefronts x 2 cells,
-    one lane slot, 128-VGPR budget) and its launches per vmult under the
-    installed schedule."""
-    k = "vmult_kernel<4, double, 1, false, 2, 1, 4, 4>"
+    """The line kernel of the 64^3 Q4 FP64 workload (separable-sine k, no
+    velocity) and its launches per vmult under the installed schedule."""
+    k = line_kernel_name(DEGREE, False, 1, False)
     if tuning["variant"] != "line":
         k = f"vmult ({tuning['variant']} form)"
     sched = tuning["schedule"]
@@ -333,7 +357,11 @@ def main():
                 "hbm_achieved_GBs": round(ach_bytes, 1),
                 "hbm_frac": round(ach_bytes / HBM_PEAK_GBS, 4),
                 "tuning": tuning,
-                "note": "FP64 vector-FMA (VALU) kernel: peak is MI355X's FP64 vector rate"}
+                "note": "FP64 vector-FMA (VALU) kernel: peak is MI355X's FP64 vector rate. "
+                        "achieved = flops of one vmult / its event time (per pass: 1/8 of both). "
+                        "With the slab schedule a pass is two launches side by side, one per "
+                        "z-half; a rocprof launch duration is one of them, so 16 launch "
+                        "durations add up to more than the vmult's time"}
 
     solve = None
     if not args.no_solve:
