@@ -194,6 +194,17 @@ mpdg_status mpdg_heat_set_dt(mpdg_heat *h, double dt);
  * pointer to n_dofs() doubles, read plain (no constraints) and interpolated at q.
  * The engine copies it (and converts to the cell-loop precision). */
 mpdg_status mpdg_heat_set_heaviside(mpdg_heat *h, const double *heaviside);
+/* The same, for a level set on FE_Q_iso_Q1(p) (p linear sub-elements per cell and
+ * direction): replaces the reference's do_level_set_temperature_gradient_interpolation
+ * path, the interpolation matrix of reinit (heat_diffuse_operator.cpp:834-838,
+ * DoFTools::create_dof_interpolation_matrix, source/utilities/dof_tools.cpp:20-49)
+ * applied per cell before the heaviside is evaluated at q (:982-993,
+ * interpolate_dof_values, dof_tools.cpp:79-103). heaviside_iso_q1: device pointer
+ * to n_dofs() doubles in lattice order, node (i,j,k) at the equidistant sub-grid
+ * point (box_lo + (i, j, k) h / p). The engine interpolates it to the GLL support
+ * points of FE_Q(p) (the iso field's trilinear interpolant in the sub-cell holding
+ * each point) and keeps that copy; degrees 1 and 2 share their support points. */
+mpdg_status mpdg_heat_set_heaviside_iso_q1(mpdg_heat *h, const double *heaviside_iso_q1);
 
 /* The nodal velocity of MPDG_VELOCITY_NODAL (the DoF vector the reference operator
  * reads through vel_eval.read_dof_values_plain, heat_diffuse_operator.cpp:975-980):

@@ -139,6 +139,7 @@ SIGNATURES = {
     "mpdg_heat_n_free_dofs": (ctypes.c_int64, [_VP]),
     "mpdg_heat_set_dt": (ctypes.c_int, [_VP, ctypes.c_double]),
     "mpdg_heat_set_heaviside": (ctypes.c_int, [_VP, _VP]),
+    "mpdg_heat_set_heaviside_iso_q1": (ctypes.c_int, [_VP, _VP]),
     "mpdg_heat_set_velocity": (ctypes.c_int, [_VP, _VP]),
     "mpdg_heat_set_temperature": (ctypes.c_int, [_VP, _VP]),
     "mpdg_heat_set_temperature_old": (ctypes.c_int, [_VP, _VP]),

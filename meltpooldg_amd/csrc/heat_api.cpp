@@ -1322,6 +1322,30 @@ mpdg_status mpdg_heat_set_heaviside(mpdg_heat *h, const double *heaviside)
   return MPDG_OK;
 }
 
+mpdg_status mpdg_heat_set_heaviside_iso_q1(mpdg_heat *h, const double *heaviside_iso_q1)
+{
+  if (!h || !heaviside_iso_q1)
+    return fail(MPDG_ERR_ARGUMENT, "null handle or heaviside");
+  const int p = h->desc.degree;
+  // GLL support point a of a cell in the iso element's sub-interval s = floor(x p)
+  // (the last one for x = 1) at fraction t = x p - s
+  IsoQ1Map m{};
+  for (int a = 0; a <= p; ++a)
+    {
+      const double xp = h->t1d.gll[a] * p;
+      const int sa = std::min(std::max((int)std::floor(xp), 0), p - 1);
+      m.s[a] = sa;
+      m.t[a] = a == 0 ? 0.0 : a == p ? 1.0 : xp - sa;
+    }
+  MPDG_HIP(h->heav64.alloc(h->N));
+  MPDG_HIP(h->heav32.alloc(h->N));
+  MPDG_HIP(launch_iso_q1_to_gll(h->heav64.ptr, heaviside_iso_q1, h->prm64.n1x, h->prm64.n1y,
+                                h->prm64.n1z, p, m, h->stream));
+  MPDG_HIP(launch_to_f32(h->heav32.ptr, h->heav64.ptr, h->N, h->stream));
+  h->has_heaviside = true;
+  return MPDG_OK;
+}
+
 mpdg_status mpdg_heat_set_temperature(mpdg_heat *h, const double *temperature)
 {
   if (!h)

@@ -374,6 +374,17 @@ namespace mpdg
   hipError_t launch_permute(double *dst, const double *src, const uint32_t *index, int64_t n,
                             bool to_lattice, hipStream_t s);
   hipError_t launch_fill_f32(float *x, float v, int64_t n, hipStream_t s);
+  // FE_Q_iso_Q1(p) nodal values (equidistant lattice) -> FE_Q(p) nodal values
+  // (GLL lattice): each GLL support point a of a cell lies in the iso element's
+  // sub-interval s[a] at fraction t[a] per direction; the value there is the
+  // iso field's trilinear interpolant in that sub-cell.
+  struct IsoQ1Map
+  {
+    int s[kMaxN];
+    double t[kMaxN];
+  };
+  hipError_t launch_iso_q1_to_gll(double *dst, const double *src, int n1x, int n1y, int n1z,
+                                  int p, const IsoQ1Map &m, hipStream_t s);
   hipError_t launch_dot(const double *a, const double *b, int64_t n, double *partials,
                         double *out, unsigned *counter, hipStream_t s);
 

@@ -994,6 +994,61 @@ namespace mpdg
     return hipGetLastError();
   }
 
+  namespace
+  {
+    // One direction of the FE_Q_iso_Q1 -> FE_Q interpolation: lattice index i
+    // of that direction, n cells of degree p: the index of the left equidistant
+    // node of the sub-interval holding i's GLL point, and the point's fraction
+    // of the way to the next node.
+    __device__ __forceinline__ int iso_q1_left(const IsoQ1Map &m, int i, int n, int p, double &t)
+    {
+      const int c = min(i / p, n - 1);
+      const int a = i - c * p;
+      t = m.t[a];
+      return c * p + m.s[a];
+    }
+
+    __global__ void __launch_bounds__(T)
+      iso_q1_kernel(double *dst, const double *src, int n1x, int n1y, int n1z, int p,
+                    IsoQ1Map m)
+    {
+      const int64_t n = (int64_t)n1x * n1y * n1z;
+      const int nx = (n1x - 1) / p, ny = (n1y - 1) / p, nz = (n1z - 1) / p;
+      for (int64_t i = blockIdx.x * (int64_t)T + threadIdx.x; i < n; i += (int64_t)gridDim.x * T)
+        {
+          const int ix = (int)(i % n1x);
+          const int iy = (int)((i / n1x) % n1y);
+          const int iz = (int)(i / ((int64_t)n1x * n1y));
+          double tx, ty, tz;
+          const int jx = iso_q1_left(m, ix, nx, p, tx);
+          const int jy = iso_q1_left(m, iy, ny, p, ty);
+          const int jz = iso_q1_left(m, iz, nz, p, tz);
+          // trilinear within the sub-cell [jx, jx+1] x [jy, jy+1] x [jz, jz+1]
+          // (jx + 1 <= nx * p = n1x - 1: always a lattice node)
+          double v = 0.0;
+#pragma unroll
+          for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+              for (int a = 0; a < 2; ++a)
+                {
+                  const double w = (a ? tx : 1.0 - tx) * (b ? ty : 1.0 - ty) * (c ? tz : 1.0 - tz);
+                  v += w * src[(jx + a) + (int64_t)n1x * ((jy + b) + (int64_t)n1y * (jz + c))];
+                }
+          dst[i] = v;
+        }
+    }
+  } // namespace
+
+  hipError_t launch_iso_q1_to_gll(double *dst, const double *src, int n1x, int n1y, int n1z,
+                                  int p, const IsoQ1Map &m, hipStream_t s)
+  {
+    const int64_t n = (int64_t)n1x * n1y * n1z;
+    iso_q1_kernel<<<stream_grid(n), T, 0, s>>>(dst, src, n1x, n1y, n1z, p, m);
+    return hipGetLastError();
+  }
+
   hipError_t launch_fill_f32(float *x, float v, int64_t n, hipStream_t s)
   {
     fill_f32_kernel<<<stream_grid(n), T, 0, s>>>(x, v, n);

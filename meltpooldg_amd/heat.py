@@ -337,10 +337,20 @@ class HeatDiffuseOperator:
             raise ValueError("velocity must be a contiguous FP64 GPU tensor of 3 n_dofs() values")
         L.check(self._lib.mpdg_heat_set_velocity(self._h, _ptr(v)))
 
-    def set_heaviside(self, heaviside):
-        """Register the level-set heaviside field (read plain, interpolated at q)."""
+    def set_heaviside(self, heaviside, fe="FE_Q"):
+        """Register the level-set heaviside field (read plain, interpolated at q).
+        fe: the level set's finite element, "FE_Q" (the heat element's space)
+        or "FE_Q_iso_Q1" (p linear sub-elements per cell, equidistant nodes in
+        lattice order): the latter is first interpolated to the heat element's
+        support points, as the reference does when the level set's element is
+        FE_Q_iso_Q1 (heat_diffuse_operator.cpp:834-838, 982-993)."""
         self._check_vec(heaviside, "float64")
-        L.check(self._lib.mpdg_heat_set_heaviside(self._h, _ptr(heaviside)))
+        if fe == "FE_Q":
+            L.check(self._lib.mpdg_heat_set_heaviside(self._h, _ptr(heaviside)))
+        elif fe == "FE_Q_iso_Q1":
+            L.check(self._lib.mpdg_heat_set_heaviside_iso_q1(self._h, _ptr(heaviside)))
+        else:
+            raise ValueError(f"level-set element {fe!r}: FE_Q or FE_Q_iso_Q1")
         self._heaviside = heaviside
 
     # -- OperatorMatrixFree ---------------------------------------------------
