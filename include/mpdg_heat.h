@@ -414,8 +414,13 @@ mpdg_status mpdg_heat_set_graphs(mpdg_heat *h, int enable);
 typedef enum {
   MPDG_VMULT_LINE = 0,     /* lanes own the 1D lines of a few cells, LDS between directions */
   MPDG_VMULT_CELL = 1,     /* each lane owns a whole cell in registers (degree <= 2) */
-  MPDG_VMULT_LINE_MFMA = 2 /* line form with the 1D contractions on FP32 matrix cores
+  MPDG_VMULT_LINE_MFMA = 2, /* line form with the 1D contractions on FP32 matrix cores
                               (MPDG_FP32 tuning only, degree >= 3) */
+  MPDG_VMULT_CELL_MFMA = 3  /* element-matrix form: the cell matrix, the same in every
+                              cell, applied to 16 cells at a time on the matrix cores
+                              (degree <= 2, gas material with a constant conductivity,
+                              no or a constant velocity; tile_log2 1..4 = log2 of the
+                              16-cell tile's extent in x) */
 } mpdg_vmult_variant;
 
 /* How the line forms order their colour passes. Either way every lattice node

@@ -122,6 +122,7 @@ class NewtonReport(ctypes.Structure):
 MPDG_VMULT_LINE = 0
 MPDG_VMULT_CELL = 1
 MPDG_VMULT_LINE_MFMA = 2
+MPDG_VMULT_CELL_MFMA = 3
 
 
 class Tuning(ctypes.Structure):

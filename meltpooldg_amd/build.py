@@ -15,7 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJDIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmpdg_heat.so")
-SOURCES = ["tables.cpp", "heat_kernels.hip", "cell_kernels.hip", "rhs_kernels.hip", "face_kernels.hip",
+SOURCES = ["tables.cpp", "heat_kernels.hip", "cell_kernels.hip", "em_kernels.hip", "rhs_kernels.hip",
+           "face_kernels.hip",
            "material_kernels.hip", "solver_kernels.hip",
            "batch_kernels.hip", "heat_api.cpp"]
 ARCH = "gfx950"

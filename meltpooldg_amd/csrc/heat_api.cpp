@@ -103,6 +103,10 @@ struct mpdg_heat
   DevBuf<double> heav64;
   DevBuf<float> heav32;
   bool has_heaviside = false;
+  DevBuf<double> em64;  // element-matrix form: the cell matrix packed per lane (EmShape)
+  DevBuf<float> em32;
+  std::vector<double> em_host64; // host copies the device tables are uploaded from
+  std::vector<float> em_host32;
   DevBuf<double> vel64; // nodal velocity, 3 N (MPDG_VELOCITY_NODAL)
   bool has_velocity = false;
 
@@ -279,7 +283,9 @@ namespace
             (t.variant == MPDG_VMULT_CELL && degree <= kCellMaxDegree && t.tile_log2 >= 3 &&
              t.tile_log2 <= 6) ||
             (t.variant == MPDG_VMULT_LINE_MFMA && degree >= kMfmaMinDegree &&
-             precision == MPDG_FP32));
+             precision == MPDG_FP32) ||
+            (t.variant == MPDG_VMULT_CELL_MFMA && degree <= kEmMaxDegree && t.tile_log2 >= 1 &&
+             t.tile_log2 <= 4));
   }
 
   // Robin faces of a descriptor (the tangent's and diagonal's; with_neumann:
@@ -376,6 +382,8 @@ namespace
     return tab;
   }
 
+  bool em_eligible(const mpdg_heat_desc &d);
+
   template <typename Number>
   void fill_params(mpdg_heat *H, CellParams<Number> &prm, const Number *tab)
   {
@@ -438,14 +446,17 @@ namespace
     prm.blocks_per_cu = tu.blocks_per_cu;
     prm.variant = (tu.variant == MPDG_VMULT_CELL && d.degree <= kCellMaxDegree) ? kVariantCell :
                   (tu.variant == MPDG_VMULT_LINE_MFMA && d.degree >= kMfmaMinDegree &&
-                   sizeof(Number) == 4) ? kVariantLineMfma : kVariantLine;
+                   sizeof(Number) == 4) ? kVariantLineMfma :
+                  (tu.variant == MPDG_VMULT_CELL_MFMA && em_eligible(d)) ? kVariantCellMfma :
+                                                                            kVariantLine;
     prm.tile_log2 = tu.tile_log2;
     const int pi = sizeof(Number) == 4 ? MPDG_FP32 : MPDG_FP64;
-    const bool ordered = tu.schedule == MPDG_SCHEDULE_ORDERED && prm.variant != kVariantCell &&
+    const bool cellish = prm.variant == kVariantCell || prm.variant == kVariantCellMfma;
+    const bool ordered = tu.schedule == MPDG_SCHEDULE_ORDERED && !cellish &&
                          H->order[pi].ptr != nullptr;
     prm.order = ordered ? H->order[pi].ptr : nullptr;
     prm.order_flags = ordered ? H->order_flags[pi].ptr : nullptr;
-    prm.slabs = tu.schedule == MPDG_SCHEDULE_SLABS && prm.variant != kVariantCell ? 2 : 1;
+    prm.slabs = tu.schedule == MPDG_SCHEDULE_SLABS && !cellish ? 2 : 1;
     // per-direction tables (see coefficient_tables): [sin2pi dir 0 | dir 1 | dir 2 |
     // u_x x-factor | u_x y-factor | u_y x-factor | u_y y-factor]
     size_t off = 0;
@@ -479,6 +490,105 @@ namespace
   // fields (material_kernels.hip, FP64): temperature-dependent materials and
   // nodal velocity fields.
   bool generic_cells(const mpdg_heat_desc &d) { return temperature_dependent(d) || nodal_velocity(d); }
+
+  // The element-matrix form needs the same cell matrix in every cell: the
+  // constant material without a conductivity field, no or a constant velocity.
+  bool em_eligible(const mpdg_heat_desc &d)
+  {
+    return d.degree <= kEmMaxDegree && d.material == MPDG_MATERIAL_GAS &&
+           d.conductivity_field == MPDG_CONDUCTIVITY_MATERIAL &&
+           (d.velocity == MPDG_VELOCITY_NONE || d.velocity == MPDG_VELOCITY_CONSTANT);
+  }
+
+  // Cell matrix of the tangent of a constant material on an affine box cell,
+  // test node m (rows) x trial node n, nodes lexicographic (x fastest):
+  //   rho_cp/dt |J| M x M x M + k (|J|/hx^2 K x M x M + ...) + rho_cp (vx |J|/hx C x M x M + ...)
+  // with the 1D unit-interval matrices of the operator's QGauss(p+1) rule
+  // M_ij = sum_q w phi_i phi_j, K_ij = sum_q w phi_i' phi_j', C_ij = sum_q w phi_i phi_j'
+  // (the terms of tangent_local_cell_operation, heat_diffuse_operator.cpp:1029-1042;
+  // the rule integrates them exactly). Packed per lane as the 16x16x4 MFMA A
+  // operands: entry (rb * T + ks) * 64 + lane holds A[node_r][node_c], column
+  // node 4 ks + lane / 16 (0 past the last node). The product of A operand row
+  // i = lane % 16 lands (measured on gfx950) in lane group i / 4, register
+  // i % 4 for v_mfma_f32_16x16x4f32, and in lane group i % 4, register i / 4
+  // for v_mfma_f64_16x16x4f64; row i of block rb is the node of that slot
+  // (group g, register r: node g + 4 (4 rb + r)).
+  std::vector<double> em_table(const mpdg_heat *H, bool f64)
+  {
+    const mpdg_heat_desc &d = H->desc;
+    const Tables1D &t = H->t1d;
+    const int n = t.n, nn = n * n * n;
+    double M[kMaxN][kMaxN]{}, K[kMaxN][kMaxN]{}, C[kMaxN][kMaxN]{};
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j)
+        for (int q = 0; q < n; ++q)
+          {
+            M[i][j] += t.wq[q] * t.S[q][i] * t.S[q][j];
+            K[i][j] += t.wq[q] * t.D[q][i] * t.D[q][j];
+            C[i][j] += t.wq[q] * t.S[q][i] * t.D[q][j];
+          }
+    const double vol = H->h[0] * H->h[1] * H->h[2];
+    const double rhocp = d.gas.density * d.gas.specific_heat_capacity;
+    const double mass = (H->dt_set ? 1.0 / H->dt : 0.0) * rhocp * vol;
+    const double k = d.gas.thermal_conductivity;
+    const bool cst = d.velocity == MPDG_VELOCITY_CONSTANT;
+    std::vector<double> A((size_t)nn * nn);
+    for (int m = 0; m < nn; ++m)
+      {
+        const int a = m % n, b = (m / n) % n, c = m / (n * n);
+        for (int q = 0; q < nn; ++q)
+          {
+            const int a2 = q % n, b2 = (q / n) % n, c2 = q / (n * n);
+            const double mx = M[a][a2], my = M[b][b2], mz = M[c][c2];
+            double v = mass * mx * my * mz;
+            v += k * (vol / (H->h[0] * H->h[0]) * K[a][a2] * my * mz +
+                      vol / (H->h[1] * H->h[1]) * mx * K[b][b2] * mz +
+                      vol / (H->h[2] * H->h[2]) * mx * my * K[c][c2]);
+            if (cst)
+              v += rhocp * (d.velocity_value[0] * vol / H->h[0] * C[a][a2] * my * mz +
+                            d.velocity_value[1] * vol / H->h[1] * mx * C[b][b2] * mz +
+                            d.velocity_value[2] * vol / H->h[2] * mx * my * C[c][c2]);
+            A[(size_t)m * nn + q] = v;
+          }
+      }
+    const int T = (nn + 3) / 4, RB = (T + 3) / 4;
+    std::vector<double> packed((size_t)RB * T * 64, 0.0);
+    for (int rb = 0; rb < RB; ++rb)
+      for (int ks = 0; ks < T; ++ks)
+        for (int lane = 0; lane < 64; ++lane)
+          {
+            const int i = lane % 16, kk = lane / 16;
+            const int g = f64 ? i % 4 : i / 4, r = f64 ? i / 4 : i % 4;
+            const int tr = 4 * rb + r;
+            const int mr = g + 4 * tr, mc = 4 * ks + kk;
+            if (tr < T && mr < nn && mc < nn)
+              packed[((size_t)rb * T + ks) * 64 + lane] = A[(size_t)mr * nn + mc];
+          }
+    return packed;
+  }
+
+  // (Re)builds the element-matrix tables of an eligible operator (the mass
+  // term follows dt) and points both precisions' parameters at them.
+  mpdg_status update_em(mpdg_heat *H)
+  {
+    H->prm64.em = nullptr;
+    H->prm32.em = nullptr;
+    if (!em_eligible(H->desc))
+      return MPDG_OK;
+    H->em_host64 = em_table(H, true);
+    const std::vector<double> t32 = em_table(H, false);
+    H->em_host32.assign(t32.begin(), t32.end());
+    const size_t cnt = H->em_host64.size();
+    MPDG_HIP(H->em64.alloc(cnt));
+    MPDG_HIP(H->em32.alloc(cnt));
+    MPDG_HIP(hipMemcpyAsync(H->em64.ptr, H->em_host64.data(), cnt * sizeof(double),
+                            hipMemcpyHostToDevice, H->stream));
+    MPDG_HIP(hipMemcpyAsync(H->em32.ptr, H->em_host32.data(), cnt * sizeof(float),
+                            hipMemcpyHostToDevice, H->stream));
+    H->prm64.em = H->em64.ptr;
+    H->prm32.em = H->em32.ptr;
+    return MPDG_OK;
+  }
 
   int velocity_mode(const mpdg_heat_desc &d)
   {
@@ -1250,6 +1360,11 @@ mpdg_status mpdg_heat_create(const mpdg_heat_desc *desc, mpdg_heat **out)
     }
   fill_params<double>(H, H->prm64, H->sin_tab64.ptr);
   fill_params<float>(H, H->prm32, H->sin_tab32.ptr);
+  if (mpdg_status st = update_em(H))
+    {
+      delete H;
+      return st;
+    }
   H->robin = robin_params(d, H->h);
   H->robin_rhs = robin_params(d, H->h, true);
   if (H->robin_rhs.nfaces > 0)
@@ -1293,7 +1408,7 @@ mpdg_status mpdg_heat_set_dt(mpdg_heat *h, double dt)
   h->dt_set = true;
   fill_params<double>(h, h->prm64, h->sin_tab64.ptr);
   fill_params<float>(h, h->prm32, h->sin_tab32.ptr);
-  return MPDG_OK;
+  return update_em(h); // the element matrix's mass term follows dt
 }
 
 mpdg_status mpdg_heat_set_velocity(mpdg_heat *h, const double *velocity)
@@ -1922,7 +2037,12 @@ mpdg_status mpdg_heat_set_tuning(mpdg_heat *h, int precision, const mpdg_tuning 
   if (!tuning_valid(*t, h->desc.degree, precision))
     return fail(MPDG_ERR_ARGUMENT,
                 "tuning needs z_layers >= 1, blocks_per_cu in 0..16; the cell variant degree <= 2 "
-                "and tile_log2 in 3..6; the MFMA variant degree >= 3 and MPDG_FP32");
+                "and tile_log2 in 3..6; the MFMA variant degree >= 3 and MPDG_FP32; the "
+                "element-matrix variant degree <= 2 and tile_log2 in 1..4");
+  if (t->variant == MPDG_VMULT_CELL_MFMA && !em_eligible(h->desc))
+    return fail(MPDG_ERR_UNSUPPORTED,
+                "the element-matrix variant needs the same cell matrix in every cell: gas "
+                "material, no conductivity field, no or a constant velocity");
   if (h->stream)
     MPDG_HIP(hipStreamSynchronize(h->stream));
   if (t->schedule == MPDG_SCHEDULE_SLABS)
@@ -1973,6 +2093,11 @@ mpdg_status mpdg_heat_autotune(mpdg_heat *h, int precision, int repeats, mpdg_tu
           if (h->desc.degree <= kCellMaxDegree)
             for (int tl : {3, 4, 5, 6})
               cand.push_back(mpdg_tuning{zl, bpc, MPDG_VMULT_CELL, tl, MPDG_SCHEDULE_PASSES});
+          // element-matrix form: the widest tiles (it measured slower than the
+          // line and cell forms on every mesh tried, so only a few shapes)
+          if (em_eligible(h->desc) && bpc == 0)
+            for (int tl : {3, 4})
+              cand.push_back(mpdg_tuning{zl, bpc, MPDG_VMULT_CELL_MFMA, tl, MPDG_SCHEDULE_PASSES});
         }
     }
   const mpdg_tuning saved = h->tune[precision];
@@ -2172,6 +2297,11 @@ namespace
                 tile = prm.tile_log2;
                 variant64 = prm64.variant;
                 tile64 = prm64.tile_log2;
+                // batched launches have no element-matrix form: its members take the line form
+                if (variant == kVariantCellMfma)
+                  variant = kVariantLine;
+                if (variant64 == kVariantCellMfma)
+                  variant64 = kVariantLine;
               }
             // one kernel form per group and precision
             prm.variant = variant;

@@ -1683,6 +1683,8 @@ namespace mpdg
     if (prm.variant == kVariantCell && degree <= kCellMaxDegree)
       return launch_vmult_cell<Number>(degree, coef, vel, prm, src, dst, heaviside, partials, scal,
                                        stream);
+    if (prm.variant == kVariantCellMfma && degree <= kEmMaxDegree)
+      return launch_vmult_em<Number>(degree, prm, src, dst, partials, scal, stream);
     switch (degree)
       {
         case 1: return vmult_deg<Number, 1>(coef, vel, prm, tab, src, dst, heaviside, partials, scal, stream, slabs);

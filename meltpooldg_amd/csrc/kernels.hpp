@@ -68,6 +68,19 @@ namespace mpdg
     // (tables over the x resp. y quadrature coordinates); u_z = w0
     const Number *vel_x[2];
     const Number *vel_y[2];
+    // element-matrix form (kVariantCellMfma): the cell matrix packed per lane
+    // as the MFMA A operands, em[(rb * T + ks) * 64 + lane] (EmShape)
+    const Number *em;
+  };
+
+  // Element-matrix form of degree P: T node slots per lane (k-steps of the
+  // 16x16x4 products, node g + 4t for lane group g), RB blocks of 16 rows.
+  template <int P>
+  struct EmShape
+  {
+    static constexpr int NN = (P + 1) * (P + 1) * (P + 1);
+    static constexpr int T = (NN + 3) / 4;
+    static constexpr int RB = (T + 3) / 4;
   };
 
   // Device state of the ordered vmult schedule, one per operator precision
@@ -135,10 +148,12 @@ namespace mpdg
   {
     kVariantLine = 0,
     kVariantCell = 1,
-    kVariantLineMfma = 2 // line form with the 1D contractions on FP32 MFMA (degree >= 3)
+    kVariantLineMfma = 2, // line form with the 1D contractions on FP32 MFMA (degree >= 3)
+    kVariantCellMfma = 3  // element-matrix form on MFMA (constant coefficients, degree <= 2)
   };
   constexpr int kMfmaMinDegree = 3;
   constexpr int kCellMaxDegree = 2;
+  constexpr int kEmMaxDegree = 2;
 
   // Launch shape of one colour pass.
   struct ColourGrid
@@ -207,6 +222,17 @@ namespace mpdg
   VmultGrid cell_grid(int nx, int ny, int nz, int tile_log2, int layers, int max_blocks);
   int cell_items_per_block();
   int cell_resident_blocks(int degree, bool fp32);
+
+  // Element-matrix form of the vmult (em_kernels.hip), same contract as
+  // launch_vmult for degree <= kEmMaxDegree with prm.em set; launch_vmult
+  // forwards to it when prm.variant == kVariantCellMfma.
+  template <typename Number>
+  hipError_t launch_vmult_em(int degree, const CellParams<Number> &prm, const Number *src,
+                             Number *dst, double *partials, SolverScalars *scal,
+                             hipStream_t stream);
+  // Per colour: tiles of 2^tile_log2 x 2^(4-tile_log2) cells (tile_log2 1..4)
+  // times z-chunks, four tiles in flight per workgroup.
+  VmultGrid em_grid(int nx, int ny, int nz, int tile_log2, int layers, int max_blocks);
 
   // Residual right-hand side (rhs_kernels.hip): dst += -(rho_cp/dt M (T - T_old)
   // - M_1 f + K T + rho_cp C_u T) on the free rows, T, T_old, f read plain

@@ -368,7 +368,8 @@ class HeatDiffuseOperator:
         self._check_vec(diag, "float64")
         L.check(self._lib.mpdg_heat_inverse_diagonal(self._h, _ptr(diag)))
 
-    _VARIANTS = {"line": L.MPDG_VMULT_LINE, "cell": L.MPDG_VMULT_CELL, "mfma": L.MPDG_VMULT_LINE_MFMA}
+    _VARIANTS = {"line": L.MPDG_VMULT_LINE, "cell": L.MPDG_VMULT_CELL, "mfma": L.MPDG_VMULT_LINE_MFMA,
+                 "cell_mfma": L.MPDG_VMULT_CELL_MFMA}
 
     def _prec_code(self, precision):
         precision = self._precision if precision is None else precision
