@@ -256,13 +256,14 @@ def main():
     ap.add_argument("--tuning", default=None,
                     help="fixed FP64 tuning z_layers,blocks_per_cu[,variant,tile_log2,schedule]")
     ap.add_argument("--mode", choices=["vmult", "copy", "solve", "robin", "solidification",
-                                       "timestep"],
+                                       "timestep", "q2"],
                     default="vmult",
                     help="copy: time a plain device copy of one DoF vector (HBM counter calibration); "
                          "solve: only the FP64 and FP32-mixed CG solves (solver profiling); "
                          "robin: the headline operator with a convecting and radiating face; "
                          "solidification: the tangent of a temperature-dependent material; "
-                         "timestep: the reference's regression inputs, graphs vs eager")
+                         "timestep: the reference's regression inputs, graphs vs eager; "
+                         "q2: the degree-2 operators against the HBM and vector-ALU bounds")
     ap.add_argument("--config", type=int, default=1,
                     help="1: the headline line (default); 0, 2, 3, 4: the other workloads")
     args = ap.parse_args()
@@ -290,6 +291,9 @@ def main():
     if args.mode == "timestep":
         import bench_configs
         return print(json.dumps(bench_configs.timestep(args, torch)))
+    if args.mode == "q2":
+        import bench_configs
+        return print(json.dumps(bench_configs.q2(args, torch)))
     if args.config != 1:
         import bench_configs
         return bench_configs.run(args.config, args)

@@ -345,6 +345,48 @@ def timestep(args, torch):
             "runs": out}
 
 
+def q2(args, torch):
+    """Low-degree roofline leg (bench.py --mode q2): the FP32 operator of the
+    mixed-precision solve (and the FP64 one) at degree 2 on 96^3 and 192^3
+    cells, unit cube, separable-sine k and a constant k, Dirichlet on all
+    faces, the default tuning. Each vmult's algorithmic bytes (src read + dst
+    written, b N each) and flops (cells x (12 n_q^4 + 36 n_q^3), n_q = 3) are
+    set against MI355X's 8 TB/s and against the vector ALU's non-packed rate
+    (78.6 TFLOP/s for FP32 as for FP64: packed FP32 is disabled in these
+    kernels). At 96^3 the FP32 vectors (2 x 28.8 MB) stay in the 256 MB
+    last-level cache; at 192^3 (2 x 227 MB) they do not."""
+    from meltpooldg_amd import HeatDiffuseOperator
+    peak_tb, peak_tf = 8.0, 78.6
+    rows = []
+    for n in (96, 192):
+        for coef in ("sine", "constant"):
+            kw = {"conductivity_amplitude": 0.5} if coef == "sine" else {}
+            op = HeatDiffuseOperator(n, 2, **kw)
+            op.reset_time_increment(1e-3)
+            N = op.n_dofs()
+            cells = n ** 3
+            flops = cells * (12 * 3 ** 4 + 36 * 3 ** 3)
+            for prec, dtype, b in (("fp32", torch.float32, 4), ("fp64", torch.float64, 8)):
+                src = torch.rand(N, dtype=dtype, device="cuda")
+                dst = torch.empty_like(src)
+                t_end = time.perf_counter() + 0.3
+                while time.perf_counter() < t_end:
+                    for _ in range(10):
+                        op.vmult(dst, src)
+                    torch.cuda.synchronize()
+                ms = op.time_vmult(dst, src, 100)
+                gbs = 2 * b * N / (ms * 1e-3) / 1e9
+                tfs = flops / (ms * 1e-3) / 1e12
+                rows.append({"cells": n, "k": coef, "precision": prec, "n_dofs": N,
+                             "ms_per_vmult": round(ms, 5), "GDoF/s": round(N / ms / 1e6, 2),
+                             "algorithmic_GB/s": round(gbs, 1), "hbm_frac": round(gbs / 1e3 / peak_tb, 4),
+                             "TFLOP/s": round(tfs, 2), "valu_frac": round(tfs / peak_tf, 4),
+                             "tuning": op.get_tuning(prec)})
+            del op
+    return {"mode": "q2", "workload": "Q2 vmult, unit cube, 96^3 and 192^3 cells, sine and constant k",
+            "peaks": {"hbm_TB/s": peak_tb, "valu_TFLOP/s_nonpacked": peak_tf}, "rows": rows}
+
+
 def run(config, args):
     import torch
     fn = {0: config0, 2: config2, 4: config4}

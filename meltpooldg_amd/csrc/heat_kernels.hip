@@ -1347,7 +1347,10 @@ namespace mpdg
 
     // Launch shape of the line-form vmult. Wavefront form (degrees 1-3 and the
     // MFMA form): CW cells per wavefront, R line slots per lane, 4 wavefronts
-    // per workgroup; the MFMA form holds the matrix columns in 4(P+1) more
+    // per workgroup (two slots per lane for the colour-pass launches of FP64
+    // degree 3 and FP32 degree 1 without a velocity or a nodal heaviside:
+    // 85.3 vs 71.7 GDoF/s at 96^3 Q3, 45.2 vs 26.8 at 128^3 Q1; with those
+    // fields the second slot would cost occupancy); the MFMA form holds the matrix columns in 4(P+1) more
     // VGPRs, so it takes fewer cells per wavefront at degree 4 and 6 and a
     // 256-VGPR budget at degree 5. Spread form (FMA, from degree 4, whose cells
     // do not tile a wavefront: 25, 36, 49 lines): the CPB cells of a workgroup
@@ -1364,7 +1367,7 @@ namespace mpdg
       static constexpr bool SPREAD = !MF && P >= 4;
       static constexpr int WPB = 4;
       static constexpr int CWW = MF ? (P == 4 ? 2 : P == 5 ? 3 : P == 6 ? 1 : 4)
-                                    : P == 1 ? 16 : P == 2 ? 7 : P == 3 ? 4 : 1;
+                                    : P == 1 ? (FP32 && LEAN ? 32 : 16) : P == 2 ? 7 : P == 3 ? (!FP32 && LEAN ? 8 : 4) : 1;
       static constexpr int CPB = SPREAD ? (64 * WPB) / ((P + 1) * (P + 1)) : CWW * WPB;
       static constexpr int CW = SPREAD ? CPB : CWW;
       static constexpr int R = SPREAD ? 0 : (CWW * (P + 1) * (P + 1) + 63) / 64;
