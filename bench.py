@@ -264,6 +264,8 @@ def main():
                          "solidification: the tangent of a temperature-dependent material; "
                          "timestep: the reference's regression inputs, graphs vs eager; "
                          "q2: the degree-2 operators against the HBM and vector-ALU bounds")
+    ap.add_argument("--cells", default="96,192",
+                    help="q2 mode: comma-separated cells per direction (96,192 by default)")
     ap.add_argument("--config", type=int, default=1,
                     help="1: the headline line (default); 0, 2, 3, 4: the other workloads")
     args = ap.parse_args()

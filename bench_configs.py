@@ -358,7 +358,7 @@ def q2(args, torch):
     from meltpooldg_amd import HeatDiffuseOperator
     peak_tb, peak_tf = 8.0, 78.6
     rows = []
-    for n in (96, 192):
+    for n in [int(c) for c in args.cells.split(",")]:
         for coef in ("sine", "constant"):
             kw = {"conductivity_amplitude": 0.5} if coef == "sine" else {}
             op = HeatDiffuseOperator(n, 2, **kw)
@@ -383,7 +383,7 @@ def q2(args, torch):
                              "TFLOP/s": round(tfs, 2), "valu_frac": round(tfs / peak_tf, 4),
                              "tuning": op.get_tuning(prec)})
             del op
-    return {"mode": "q2", "workload": "Q2 vmult, unit cube, 96^3 and 192^3 cells, sine and constant k",
+    return {"mode": "q2", "workload": f"Q2 vmult, unit cube, {args.cells} cells per direction, sine and constant k",
             "peaks": {"hbm_TB/s": peak_tb, "valu_TFLOP/s_nonpacked": peak_tf}, "rows": rows}
 
 

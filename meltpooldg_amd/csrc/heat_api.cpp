@@ -251,7 +251,12 @@ namespace
     switch (degree)
       {
         case 1: t.z_layers = fp32 ? 4 : 2; t.blocks_per_cu = fp32 ? 4 : 0; break;
-        case 2: t.z_layers = fp32 ? 4 : 2; t.blocks_per_cu = fp32 ? 8 : 4; break;
+        case 2:
+          // FP32 (two-slot shape): one-layer chunks measured 72.6 against 68.3
+          // GDoF/s at 96^3, two-layer ones 108.6 against 100.1 at 192^3
+          t.z_layers = fp32 ? (cells >= (int64_t(1) << 22) ? 2 : 1) : 2;
+          t.blocks_per_cu = fp32 ? 0 : 4;
+          break;
         case 3: t.z_layers = fp32 ? 2 : 4; t.blocks_per_cu = fp32 ? 8 : 4; break;
         case 4: t.z_layers = fp32 ? 2 : 4; t.blocks_per_cu = fp32 ? 0 : 4; break;
         case 5:

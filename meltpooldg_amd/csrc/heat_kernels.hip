@@ -1348,9 +1348,12 @@ namespace mpdg
     // Launch shape of the line-form vmult. Wavefront form (degrees 1-3 and the
     // MFMA form): CW cells per wavefront, R line slots per lane, 4 wavefronts
     // per workgroup (two slots per lane for the colour-pass launches of FP64
-    // degree 3 and FP32 degree 1 without a velocity or a nodal heaviside:
-    // 85.3 vs 71.7 GDoF/s at 96^3 Q3, 45.2 vs 26.8 at 128^3 Q1; with those
-    // fields the second slot would cost occupancy); the MFMA form holds the matrix columns in 4(P+1) more
+    // degree 3 and FP32 degrees 1 and 2 without a velocity or a nodal
+    // heaviside: 85.3 vs 71.7 GDoF/s at 96^3 Q3, 45.2 vs 26.8 at 128^3 Q1;
+    // FP32 degree 2 in two-wavefront workgroups, so that a workgroup keeps
+    // its 28-cell row segment: 72.9 vs 66.7 GDoF/s at 96^3 and 108.6 vs 97.4
+    // at 192^3, constant k; with those fields the second slot would cost
+    // occupancy); the MFMA form holds the matrix columns in 4(P+1) more
     // VGPRs, so it takes fewer cells per wavefront at degree 4 and 6 and a
     // 256-VGPR budget at degree 5. Spread form (FMA, from degree 4, whose cells
     // do not tile a wavefront: 25, 36, 49 lines): the CPB cells of a workgroup
@@ -1365,9 +1368,11 @@ namespace mpdg
     struct VShape
     {
       static constexpr bool SPREAD = !MF && P >= 4;
-      static constexpr int WPB = 4;
+      static constexpr int WPB = (FP32 && LEAN && !MF && P == 2) ? 2 : 4;
       static constexpr int CWW = MF ? (P == 4 ? 2 : P == 5 ? 3 : P == 6 ? 1 : 4)
-                                    : P == 1 ? (FP32 && LEAN ? 32 : 16) : P == 2 ? 7 : P == 3 ? (!FP32 && LEAN ? 8 : 4) : 1;
+                                    : P == 1 ? (FP32 && LEAN ? 32 : 16)
+                                    : P == 2 ? (FP32 && LEAN ? 14 : 7)
+                                    : P == 3 ? (!FP32 && LEAN ? 8 : 4) : 1;
       static constexpr int CPB = SPREAD ? (64 * WPB) / ((P + 1) * (P + 1)) : CWW * WPB;
       static constexpr int CW = SPREAD ? CPB : CWW;
       static constexpr int R = SPREAD ? 0 : (CWW * (P + 1) * (P + 1) + 63) / 64;
