@@ -152,7 +152,7 @@ typedef struct {
   int gmres_restart;        /* GMRES basis size (30) */
   int mixed_precision;      /* 1: FP32 cell loop inside FP64 defect correction */
   double inner_rel_tolerance; /* mixed: relative reduction of each inner FP32 solve */
-  int check_interval;       /* iterations between host convergence checks (0 = auto) */
+  int check_interval;       /* iterations between host convergence checks (0 = auto, at most 64) */
 } mpdg_solver_params;
 
 typedef struct {

@@ -1151,6 +1151,13 @@ namespace
     return MPDG_OK;
   }
 
+  // Longest captured Krylov chunk (iterations between host checks). A
+  // 128-iteration PCG chunk of the slab-scheduled FP64 operator (about 2,900
+  // graph nodes) aborted the process with a host-heap corruption inside the
+  // HIP runtime on this ROCm image; 64 ran repeatedly. Longer intervals are
+  // refused rather than captured.
+  constexpr int kMaxCheckInterval = 64;
+
   int auto_check(const mpdg_heat *H)
   {
     // poll less often when an iteration is cheap
@@ -1862,6 +1869,8 @@ static mpdg_status solve_impl(mpdg_heat *h, double *x, const double *b,
     return fail(MPDG_ERR_ARGUMENT, "gmres_restart must be in 1..31");
   if (prm.max_iterations < 1 || !(prm.rel_tolerance >= 0) || !(prm.abs_tolerance >= 0))
     return fail(MPDG_ERR_ARGUMENT, "invalid solver parameters");
+  if (prm.check_interval < 0 || prm.check_interval > kMaxCheckInterval)
+    return fail(MPDG_ERR_ARGUMENT, "check_interval must be in 0..64 (0: automatic)");
   const bool mixed = prm.mixed_precision != 0;
   if (mixed && generic_cells(h->desc))
     return fail(MPDG_ERR_UNSUPPORTED, "temperature-dependent materials and nodal velocity fields "
@@ -2670,6 +2679,8 @@ mpdg_status mpdg_heat_batch_solve(mpdg_heat_batch *b, double *const *x, const do
     mpdg_solver_params_default(&prm);
   if (prm.solver_type != MPDG_SOLVER_CG)
     return fail(MPDG_ERR_UNSUPPORTED, "batched solves use CG");
+  if (prm.check_interval < 0 || prm.check_interval > kMaxCheckInterval)
+    return fail(MPDG_ERR_ARGUMENT, "check_interval must be in 0..64 (0: automatic)");
   const int S = b->count;
   const bool mixed = prm.mixed_precision != 0;
   const int steps = prm.check_interval > 0 ? prm.check_interval : 8;

@@ -122,3 +122,32 @@ def test_graphs_in_newton_time_steps_with_robin_and_solidification():
     assert out[True][1] == out[False][1]
     assert np.array_equal(out[True][0], out[False][0])
     assert sum(li for _, li in out[True][1]) > 40  # several restart cycles replayed
+
+
+def test_check_interval_limit():
+    """Chunks of at most 64 iterations are captured; a longer interval is
+    refused before anything is launched (MPDG_ERR_ARGUMENT), and 64 solves
+    to the eager result."""
+    import torch
+    from meltpooldg_amd import LinearSolver, LinearSolverData
+    from meltpooldg_amd import _lib as L
+    case = make_case((6, 5, 4), 2, dt=1e-2, k_amplitude=0.0)
+    op = engine_operator(case)
+    pb = oracle_problem(case)
+    b = _t(pb.random(3))
+    for bad in (65, 128, -1):
+        x = torch.zeros_like(b)
+        data = LinearSolverData(solver_type="CG", preconditioner_type="Diagonal",
+                                rel_tolerance=1e-10, check_interval=bad)
+        with pytest.raises(L.MpdgError) as e:
+            LinearSolver.solve(op, x, b, data)
+        assert "check_interval" in str(e.value)
+        assert float(torch.abs(x).max()) == 0.0
+    x_e, its_e, _ = _solve(op, b, "CG", False)
+    op.set_graphs(True)
+    for rep in range(3):
+        x = torch.zeros_like(b)
+        its = LinearSolver.solve(op, x, b, LinearSolverData(
+            solver_type="CG", preconditioner_type="Diagonal", rel_tolerance=1e-10, check_interval=64))
+        assert its == its_e
+        assert np.array_equal(_np(x), x_e)
