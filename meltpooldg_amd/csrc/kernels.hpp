@@ -372,17 +372,17 @@ namespace mpdg
   hipError_t launch_invert_diagonal(double *d, int64_t n, hipStream_t s);
   hipError_t launch_jacobi(double *dst, const double *src, const double *inv, int64_t n, hipStream_t s);
 
-  template <typename Number>
+  template <typename Number, typename InvT = Number>
   hipError_t launch_cg_init(const double *b, const Number *ax, Number *r, Number *p,
-                            const Number *inv_diag, int64_t n, double rel_tol, double abs_tol,
+                            const InvT *inv_diag, int64_t n, double rel_tol, double abs_tol,
                             int max_it, double *partials, SolverScalars *scal, hipStream_t s);
-  template <typename Number>
+  template <typename Number, typename InvT = Number>
   hipError_t launch_cg_update(Number *x, Number *r, const Number *p, const Number *q,
-                              const Number *inv_diag, int64_t n, double *partials,
+                              const InvT *inv_diag, int64_t n, double *partials,
                               SolverScalars *scal, double *history, int history_len,
                               hipStream_t s);
-  template <typename Number>
-  hipError_t launch_cg_direction(Number *p, const Number *r, const Number *inv_diag, int64_t n,
+  template <typename Number, typename InvT = Number>
+  hipError_t launch_cg_direction(Number *p, const Number *r, const InvT *inv_diag, int64_t n,
                                  const SolverScalars *scal, hipStream_t s);
 
   // Mixed precision helpers.
@@ -394,6 +394,8 @@ namespace mpdg
   hipError_t launch_axpby(double *dst, double a, const double *x, double b, const double *y,
                           int64_t n, hipStream_t s);
   hipError_t launch_to_f32(float *dst, const double *src, int64_t n, hipStream_t s);
+  // FP64 -> bfloat16 (the FP32 bit pattern's upper half, rounded to nearest even)
+  hipError_t launch_to_bf16(uint16_t *dst, const double *src, int64_t n, hipStream_t s);
   hipError_t launch_add_f32(double *x, const float *d, int64_t n, hipStream_t s);
   hipError_t launch_fill(double *x, double v, int64_t n, hipStream_t s);
   // lattice[index[i]] = dofs[i] (to_lattice) or dofs[i] = lattice[index[i]]

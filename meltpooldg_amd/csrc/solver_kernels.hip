@@ -126,10 +126,20 @@ namespace mpdg
         dst[i] = src[i] * inv[i];
     }
 
+    // Entry i of an inverse diagonal: stored in the solve's precision, or (the
+    // FP32 inner CG of the mixed solve) as bfloat16, the upper half of the
+    // FP32 bit pattern rounded to nearest even (to_bf16_kernel).
+    __device__ __forceinline__ double inv_value(const double *inv, int64_t i) { return inv[i]; }
+    __device__ __forceinline__ float inv_value(const float *inv, int64_t i) { return inv[i]; }
+    __device__ __forceinline__ float inv_value(const uint16_t *inv, int64_t i)
+    {
+      return __uint_as_float((uint32_t)inv[i] << 16);
+    }
+
     // r = b - Ax0, p = D r, rr = r.r, rz = r.p; sets tol, res0, done.
-    template <typename Number>
+    template <typename Number, typename InvT>
     __global__ void __launch_bounds__(T)
-      cg_init_kernel(const double *b, const Number *ax, Number *r, Number *p, const Number *inv,
+      cg_init_kernel(const double *b, const Number *ax, Number *r, Number *p, const InvT *inv,
                      int64_t n, double rel_tol, double abs_tol, int max_it, double *partials,
                      SolverScalars *scal)
     {
@@ -137,7 +147,7 @@ namespace mpdg
       for (int64_t i = blockIdx.x * (int64_t)T + threadIdx.x; i < n; i += (int64_t)gridDim.x * T)
         {
           const Number ri = (Number)(b[i] - (ax ? (double)ax[i] : 0.0));
-          const Number zi = ri * inv[i];
+          const Number zi = ri * (Number)inv_value(inv, i);
           r[i] = ri;
           p[i] = zi;
           v[0] += (double)ri * (double)ri;
@@ -165,9 +175,9 @@ namespace mpdg
     }
 
     // x += alpha p, r -= alpha q, rr = r.r, rz = r.(D r); convergence check.
-    template <typename Number>
+    template <typename Number, typename InvT>
     __global__ void __launch_bounds__(T)
-      cg_update_kernel(Number *x, Number *r, const Number *p, const Number *q, const Number *inv,
+      cg_update_kernel(Number *x, Number *r, const Number *p, const Number *q, const InvT *inv,
                        int64_t n, double *partials, SolverScalars *scal, double *history,
                        int history_len)
     {
@@ -183,7 +193,7 @@ namespace mpdg
           const Number ri = r[i] - a * q[i];
           r[i] = ri;
           v[0] += (double)ri * (double)ri;
-          v[1] += (double)ri * (double)(ri * inv[i]);
+          v[1] += (double)ri * (double)(ri * (Number)inv_value(inv, i));
         }
       double tot[2];
       if (finish_reduction<2>(v, partials, &scal->counter, tot) && threadIdx.x == 0)
@@ -212,16 +222,16 @@ namespace mpdg
     }
 
     // p = D r + beta p
-    template <typename Number>
+    template <typename Number, typename InvT>
     __global__ void __launch_bounds__(T)
-      cg_direction_kernel(Number *p, const Number *r, const Number *inv, int64_t n,
+      cg_direction_kernel(Number *p, const Number *r, const InvT *inv, int64_t n,
                           const SolverScalars *scal)
     {
       if (scal->done)
         return;
       const Number beta = (Number)scal->beta;
       for (int64_t i = blockIdx.x * (int64_t)T + threadIdx.x; i < n; i += (int64_t)gridDim.x * T)
-        p[i] = r[i] * inv[i] + beta * p[i];
+        p[i] = r[i] * (Number)inv_value(inv, i) + beta * p[i];
     }
 
     // 16-byte vector of Number (2 doubles or 4 floats) for the wide forms of
@@ -241,12 +251,30 @@ namespace mpdg
       static constexpr int W = 4;
     };
 
+    // W consecutive inverse-diagonal entries from W-entry group j
+    template <typename Number>
+    __device__ __forceinline__ typename Wide<Number>::type inv_wide(const Number *inv, int64_t j)
+    {
+      return reinterpret_cast<const typename Wide<Number>::type *>(inv)[j];
+    }
+    __device__ __forceinline__ Wide<float>::type inv_wide(const uint16_t *inv, int64_t j)
+    {
+      typedef uint16_t H4 __attribute__((ext_vector_type(4)));
+      const H4 h = reinterpret_cast<const H4 *>(inv)[j];
+      Wide<float>::type v;
+      v[0] = __uint_as_float((uint32_t)h[0] << 16);
+      v[1] = __uint_as_float((uint32_t)h[1] << 16);
+      v[2] = __uint_as_float((uint32_t)h[2] << 16);
+      v[3] = __uint_as_float((uint32_t)h[3] << 16);
+      return v;
+    }
+
     // cg_update_kernel over 16-byte vectors: thread t of the fixed grid owns
     // vectors t, t + grid, ... and (for the tail n % W) entry n - n % W + t.
-    template <typename Number>
+    template <typename Number, typename InvT>
     __global__ void __launch_bounds__(T)
       cg_update_wide_kernel(Number *x, Number *r, const Number *p, const Number *q,
-                            const Number *inv, int64_t n, double *partials, SolverScalars *scal,
+                            const InvT *inv, int64_t n, double *partials, SolverScalars *scal,
                             double *history, int history_len)
     {
       if (scal->done)
@@ -259,13 +287,12 @@ namespace mpdg
       V *Rv = reinterpret_cast<V *>(r);
       const V *Pv = reinterpret_cast<const V *>(p);
       const V *Qv = reinterpret_cast<const V *>(q);
-      const V *Iv = reinterpret_cast<const V *>(inv);
       double v[2] = {0.0, 0.0};
       const int64_t t0 = blockIdx.x * (int64_t)T + threadIdx.x;
       const int64_t stride = (int64_t)gridDim.x * T;
       for (int64_t j = t0; j < nv; j += stride)
         {
-          const V pv = Pv[j], qv = Qv[j], iv = Iv[j];
+          const V pv = Pv[j], qv = Qv[j], iv = inv_wide(inv, j);
           V xv = X[j], rv = Rv[j];
           xv = xv + a * pv;
           rv = rv - a * qv;
@@ -285,7 +312,7 @@ namespace mpdg
           const Number ri = r[i] - a * q[i];
           r[i] = ri;
           v[0] += (double)ri * (double)ri;
-          v[1] += (double)ri * (double)(ri * inv[i]);
+          v[1] += (double)ri * (double)(ri * (Number)inv_value(inv, i));
         }
       double tot[2];
       if (finish_reduction<2>(v, partials, &scal->counter, tot) && threadIdx.x == 0)
@@ -313,9 +340,9 @@ namespace mpdg
         }
     }
 
-    template <typename Number>
+    template <typename Number, typename InvT>
     __global__ void __launch_bounds__(T)
-      cg_direction_wide_kernel(Number *p, const Number *r, const Number *inv, int64_t n,
+      cg_direction_wide_kernel(Number *p, const Number *r, const InvT *inv, int64_t n,
                                const SolverScalars *scal)
     {
       if (scal->done)
@@ -326,13 +353,12 @@ namespace mpdg
       const int64_t nv = n / W;
       V *Pv = reinterpret_cast<V *>(p);
       const V *Rv = reinterpret_cast<const V *>(r);
-      const V *Iv = reinterpret_cast<const V *>(inv);
       const int64_t t0 = blockIdx.x * (int64_t)T + threadIdx.x;
       for (int64_t j = t0; j < nv; j += (int64_t)gridDim.x * T)
-        Pv[j] = Rv[j] * Iv[j] + beta * Pv[j];
+        Pv[j] = Rv[j] * inv_wide(inv, j) + beta * Pv[j];
       const int64_t i = nv * W + t0;
       if (i < n)
-        p[i] = r[i] * inv[i] + beta * p[i];
+        p[i] = r[i] * (Number)inv_value(inv, i) + beta * p[i];
     }
 
     bool aligned16(const void *a, const void *b = nullptr, const void *c = nullptr,
@@ -396,6 +422,17 @@ namespace mpdg
                            ((faces & 16u) && iz == 0) || ((faces & 32u) && iz == n1z - 1);
           if (con)
             v[i] = 0.0;
+        }
+    }
+
+    // FP64 -> bfloat16 by way of FP32, rounded to nearest even (finite values)
+    __global__ void __launch_bounds__(T) to_bf16_kernel(uint16_t *dst, const double *src, int64_t n)
+    {
+      for (int64_t i = blockIdx.x * (int64_t)T + threadIdx.x; i < n; i += (int64_t)gridDim.x * T)
+        {
+          uint32_t u = __float_as_uint((float)src[i]);
+          u += 0x7FFFu + ((u >> 16) & 1u);
+          dst[i] = (uint16_t)(u >> 16);
         }
     }
 
@@ -883,41 +920,43 @@ namespace mpdg
   }
 
   // Reductions always use the full fixed grid so each partial covers a fixed index set.
-  template <typename Number>
+  template <typename Number, typename InvT>
   hipError_t launch_cg_init(const double *b, const Number *ax, Number *r, Number *p,
-                            const Number *inv_diag, int64_t n, double rel_tol, double abs_tol,
+                            const InvT *inv_diag, int64_t n, double rel_tol, double abs_tol,
                             int max_it, double *partials, SolverScalars *scal, hipStream_t s)
   {
-    cg_init_kernel<Number><<<kVecBlocks, T, 0, s>>>(b, ax, r, p, inv_diag, n, rel_tol, abs_tol,
-                                                    max_it, partials, scal);
+    cg_init_kernel<Number, InvT><<<kVecBlocks, T, 0, s>>>(b, ax, r, p, inv_diag, n, rel_tol,
+                                                          abs_tol, max_it, partials, scal);
     return hipGetLastError();
   }
 
-  template <typename Number>
+  template <typename Number, typename InvT>
   hipError_t launch_cg_update(Number *x, Number *r, const Number *p, const Number *q,
-                              const Number *inv_diag, int64_t n, double *partials,
+                              const InvT *inv_diag, int64_t n, double *partials,
                               SolverScalars *scal, double *history, int history_len, hipStream_t s)
   {
     if (aligned16(x, r, p, q, inv_diag))
-      cg_update_wide_kernel<Number><<<kVecBlocks, T, 0, s>>>(x, r, p, q, inv_diag, n, partials,
-                                                             scal, history, history_len);
+      cg_update_wide_kernel<Number, InvT><<<kVecBlocks, T, 0, s>>>(x, r, p, q, inv_diag, n,
+                                                                   partials, scal, history,
+                                                                   history_len);
     else
-      cg_update_kernel<Number><<<kVecBlocks, T, 0, s>>>(x, r, p, q, inv_diag, n, partials, scal,
-                                                        history, history_len);
+      cg_update_kernel<Number, InvT><<<kVecBlocks, T, 0, s>>>(x, r, p, q, inv_diag, n, partials,
+                                                              scal, history, history_len);
     return hipGetLastError();
   }
 
-  template <typename Number>
-  hipError_t launch_cg_direction(Number *p, const Number *r, const Number *inv_diag, int64_t n,
+  template <typename Number, typename InvT>
+  hipError_t launch_cg_direction(Number *p, const Number *r, const InvT *inv_diag, int64_t n,
                                  const SolverScalars *scal, hipStream_t s)
   {
     if (aligned16(p, r, inv_diag))
       {
         const int64_t nv = (n + Wide<Number>::W - 1) / Wide<Number>::W;
-        cg_direction_wide_kernel<Number><<<stream_grid(nv), T, 0, s>>>(p, r, inv_diag, n, scal);
+        cg_direction_wide_kernel<Number, InvT><<<stream_grid(nv), T, 0, s>>>(p, r, inv_diag, n,
+                                                                             scal);
       }
     else
-      cg_direction_kernel<Number><<<stream_grid(n), T, 0, s>>>(p, r, inv_diag, n, scal);
+      cg_direction_kernel<Number, InvT><<<stream_grid(n), T, 0, s>>>(p, r, inv_diag, n, scal);
     return hipGetLastError();
   }
 
@@ -947,6 +986,12 @@ namespace mpdg
                           int64_t n, hipStream_t s)
   {
     axpby_kernel<<<stream_grid(n), T, 0, s>>>(dst, a, x, b, y, n);
+    return hipGetLastError();
+  }
+
+  hipError_t launch_to_bf16(uint16_t *dst, const double *src, int64_t n, hipStream_t s)
+  {
+    to_bf16_kernel<<<stream_grid(n), T, 0, s>>>(dst, src, n);
     return hipGetLastError();
   }
 
@@ -1290,4 +1335,15 @@ namespace mpdg
                                                   const SolverScalars *, hipStream_t);
   template hipError_t launch_cg_direction<float>(float *, const float *, const float *, int64_t,
                                                  const SolverScalars *, hipStream_t);
+  template hipError_t launch_cg_init<float, uint16_t>(const double *, const float *, float *,
+                                                      float *, const uint16_t *, int64_t, double,
+                                                      double, int, double *, SolverScalars *,
+                                                      hipStream_t);
+  template hipError_t launch_cg_update<float, uint16_t>(float *, float *, const float *,
+                                                        const float *, const uint16_t *, int64_t,
+                                                        double *, SolverScalars *, double *, int,
+                                                        hipStream_t);
+  template hipError_t launch_cg_direction<float, uint16_t>(float *, const float *, const uint16_t *,
+                                                           int64_t, const SolverScalars *,
+                                                           hipStream_t);
 } // namespace mpdg
