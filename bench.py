@@ -14,6 +14,7 @@ FP32-mixed solver on the same operator.
 """
 import argparse
 import json
+import statistics
 import os
 import sys
 import time
@@ -200,32 +201,42 @@ def copy_calibration(args, torch, dev):
                       "GBs": 16 * N / (ms * 1e-3) / 1e9}))
 
 
-def time_solves(op, b, torch, LinearSolver, LinearSolverData):
-    """CG + Jacobi to 1e-8 in FP64 and FP32-mixed, each after one untimed warm-up
-    solve (workspace allocation). The mixed solve asks each inner FP32 CG for a
-    1e-4 reduction: two outer FP64 corrections reach 1e-8 (277 inner iterations,
-    0.091 s, against four corrections and 286 iterations, 0.095 s, at the 1e-2
-    default); the stopping test is the FP64 residual either way."""
-    solve = {}
-    for name, mixed in (("fp64", False), ("fp32_mixed", True)):
-        data = LinearSolverData(solver_type="CG", preconditioner_type="Diagonal",
-                                rel_tolerance=1e-8, mixed_precision=mixed,
-                                inner_rel_tolerance=INNER_REL_TOLERANCE)
-        x = torch.zeros_like(b)
-        LinearSolver.solve(op, x, b, data)
+def time_solves(op, b, torch, LinearSolver, LinearSolverData, repeats=5):
+    """CG + Jacobi to 1e-8 in FP64 and FP32-mixed: one untimed warm-up solve of
+    each (workspace allocation), then `repeats` timed solves of each,
+    alternating FP64 and mixed so that both see the same clocks; the median
+    of each is reported (the samples beside it). The mixed solve asks each
+    inner FP32 CG for a 1e-4 reduction: two outer FP64 corrections reach 1e-8
+    (277 inner iterations, against four corrections and 286 iterations at the
+    1e-2 default); the stopping test is the FP64 residual either way."""
+    runs = (("fp64", False), ("fp32_mixed", True))
+    data = {name: LinearSolverData(solver_type="CG", preconditioner_type="Diagonal",
+                                   rel_tolerance=1e-8, mixed_precision=mixed,
+                                   inner_rel_tolerance=INNER_REL_TOLERANCE)
+            for name, mixed in runs}
+    x = torch.zeros_like(b)
+    for name, _ in runs:
         x.zero_()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        LinearSolver.solve(op, x, b, data)
-        torch.cuda.synchronize()
-        t = time.perf_counter() - t0
-        rep = LinearSolver.last_report
-        solve[name] = {"seconds": round(t, 4), "iterations": rep["iterations"],
-                       "outer_iterations": rep["outer_iterations"],
-                       "rel_residual": rep["final_residual"] / rep["initial_residual"]}
-        if mixed:
-            solve[name]["speedup_vs_fp64"] = round(solve["fp64"]["seconds"] / t, 3)
-            solve[name]["inner_rel_tolerance"] = INNER_REL_TOLERANCE
+        LinearSolver.solve(op, x, b, data[name])
+    samples = {name: [] for name, _ in runs}
+    solve = {}
+    for rep in range(repeats):
+        for name, _ in runs:
+            x.zero_()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            LinearSolver.solve(op, x, b, data[name])
+            torch.cuda.synchronize()
+            samples[name].append(time.perf_counter() - t0)
+            r = LinearSolver.last_report
+            solve[name] = {"iterations": r["iterations"], "outer_iterations": r["outer_iterations"],
+                           "rel_residual": r["final_residual"] / r["initial_residual"]}
+    for name, _ in runs:
+        solve[name]["seconds"] = round(statistics.median(samples[name]), 5)
+        solve[name]["samples_s"] = [round(t, 5) for t in samples[name]]
+    solve["fp32_mixed"]["speedup_vs_fp64"] = round(solve["fp64"]["seconds"] /
+                                                   solve["fp32_mixed"]["seconds"], 3)
+    solve["fp32_mixed"]["inner_rel_tolerance"] = INNER_REL_TOLERANCE
     return solve
 
 
