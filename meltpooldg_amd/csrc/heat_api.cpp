@@ -235,14 +235,20 @@ namespace
   // sweep over z-chunk length, workgroups per CU and variant on an MI355X at
   // the sizes of the benchmark workloads (mpdg_heat_autotune refines it for a
   // given operator).
-  mpdg_tuning default_tuning(int degree, bool fp32, int64_t cells)
+  // Default launch tuning per degree and precision: the autotuner's picks on
+  // MI355X at the benchmark-like sizes (unit cube, sine k, default materials;
+  // GDoF/s default-before -> default-now in the comments). `lean`: no
+  // heaviside and no velocity (the operators whose FP32 degree-1..2 and FP64
+  // degree-3 launches take the two-slot shapes).
+  mpdg_tuning default_tuning(int degree, bool fp32, int64_t cells, bool lean)
   {
     mpdg_tuning t{};
     t.variant = MPDG_VMULT_LINE;
     t.tile_log2 = 4;
     // the cell form needs about a million cells to fill the chip in each of
-    // its eight colour passes (64 cells per wavefront)
-    if (degree == 1 && cells >= (int64_t(1) << 20))
+    // its eight colour passes (64 cells per wavefront); the lean FP32 line
+    // form (32 cells per wavefront) beats it (128^3: 41.8 vs 37.7)
+    if (degree == 1 && cells >= (int64_t(1) << 20) && !(fp32 && lean))
       {
         t.variant = MPDG_VMULT_CELL;
         t.tile_log2 = 5;
@@ -252,7 +258,13 @@ namespace
       }
     switch (degree)
       {
-        case 1: t.z_layers = fp32 ? 4 : 2; t.blocks_per_cu = fp32 ? 4 : 0; break;
+        case 1:
+          {
+            const bool big_lean = fp32 && lean && cells >= (int64_t(1) << 20);
+            t.z_layers = fp32 && !big_lean ? 4 : 2;
+            t.blocks_per_cu = fp32 ? (big_lean ? 8 : 4) : 0;
+            break;
+          }
         case 2:
           // FP32 (two-slot shape): one-layer chunks measured 72.6 against 68.3
           // GDoF/s at 96^3, two-layer ones 108.6 against 100.1 at 192^3
@@ -269,14 +281,24 @@ namespace
           break;
         default: t.z_layers = 2; t.blocks_per_cu = 4; break;
       }
-    // FP64 degrees 2..4 on large meshes: the slab schedule (two streams) with
-    // two-layer chunks measured 1.05x (Q4 64^3), 1.09x (Q3 96^3) and 1.15x
-    // (Q2 96^3) the passes; the FP32 forms did not gain
-    if (!fp32 && degree >= 2 && degree <= 4 && cells >= (int64_t(1) << 18))
+    // Slab schedule (two streams, two-layer chunks) on large meshes. FP64
+    // degrees 2..4 from 2^18 cells: 1.05x (Q4 64^3), 1.09x (Q3 96^3), 1.15x
+    // (Q2 96^3) the passes; degrees 5..6 from 2^16 cells: Q5 64^3 89.1 vs
+    // 83.9, Q6 48^3 83.5 vs 70.3 (8 workgroups per CU). FP32 degree 3 (96^3:
+    // 118.2 vs 103.5), 5 (64^3: 172.0 vs 156.2) and 6 (48^3, one-layer chunks:
+    // 158.1 vs 144.3); FP32 degrees 2 and 4 did not gain (Q4 126 vs 132).
+    const bool big = cells >= (int64_t(1) << (degree >= 5 ? 16 : 18));
+    if (big && !fp32 && degree >= 2)
       {
         t.schedule = MPDG_SCHEDULE_SLABS;
         t.z_layers = 2;
-        t.blocks_per_cu = 0;
+        t.blocks_per_cu = degree == 6 ? 8 : 0;
+      }
+    if (big && fp32 && (degree == 3 || degree >= 5))
+      {
+        t.schedule = MPDG_SCHEDULE_SLABS;
+        t.z_layers = degree == 6 ? 1 : 2;
+        t.blocks_per_cu = degree == 3 ? 0 : 8;
       }
     return t;
   }
@@ -1290,8 +1312,9 @@ mpdg_status mpdg_heat_create(const mpdg_heat_desc *desc, mpdg_heat **out)
     return fail(MPDG_ERR_ARGUMENT, "out of host memory");
   H->desc = d;
   const int64_t cells = (int64_t)d.n_cells[0] * d.n_cells[1] * d.n_cells[2];
-  H->tune[MPDG_FP64] = default_tuning(d.degree, false, cells);
-  H->tune[MPDG_FP32] = default_tuning(d.degree, true, cells);
+  const bool lean = !uses_heaviside(d) && d.velocity == MPDG_VELOCITY_NONE;
+  H->tune[MPDG_FP64] = default_tuning(d.degree, false, cells, lean);
+  H->tune[MPDG_FP32] = default_tuning(d.degree, true, cells, lean);
   H->stream = (hipStream_t)d.stream;
   H->N = N;
   H->N_free = Nf;
