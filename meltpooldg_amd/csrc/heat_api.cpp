@@ -231,10 +231,6 @@ namespace
     return e;
   }
 
-  // Default launch tuning per degree and precision: the fastest setting of a
-  // sweep over z-chunk length, workgroups per CU and variant on an MI355X at
-  // the sizes of the benchmark workloads (mpdg_heat_autotune refines it for a
-  // given operator).
   // Default launch tuning per degree and precision: the autotuner's picks on
   // MI355X at the benchmark-like sizes (unit cube, sine k, default materials;
   // GDoF/s default-before -> default-now in the comments). `lean`: no
