@@ -65,6 +65,31 @@ def test_slabs_match_passes(n, p, precision):
         assert rel_err(got, pb.vmult(x.cpu().numpy())) <= 1e-12
 
 
+@pytest.mark.parametrize("n,p", SHAPES[:6])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_slabs_match_passes_lean(n, p, precision):
+    """Operators without a velocity or a heaviside take the two-slot and
+    two-wavefront launch shapes (FP32 degrees 1-2, FP64 degree 3) and the
+    80-VGPR FP32 degree-4 budget: the same bitwise agreement."""
+    case = make_case(n, p, dt=1e-2, k_amplitude=0.4, faces=0b110110)
+    op = engine_operator(case, precision=precision)
+    dtype = "float32" if precision == "fp32" else "float64"
+    x = _t(np.random.default_rng(17).uniform(-1.0, 1.0, op.n_dofs()), dtype)
+    print("slabs lean", n, p, precision, flush=True)
+    for zl in (1, 2, 3):
+        ref = _vmult(op, x, z_layers=zl, blocks_per_cu=0, variant="line", schedule="passes",
+                     precision=precision)
+        got = _vmult(op, x, z_layers=zl, blocks_per_cu=0, variant="line", schedule="slabs",
+                     precision=precision)
+        _check_same(got, ref, n, p, zl, precision)
+    if precision == "fp64":
+        pb = oracle_problem(case)
+        assert rel_err(got, pb.vmult(x.cpu().numpy())) <= 1e-12
+    else:
+        pb = oracle_problem(case)
+        assert rel_err(got, pb.vmult(x.cpu().numpy().astype(np.float64))) < 2e-5
+
+
 @pytest.mark.parametrize("p", [3, 4, 5])
 def test_slabs_mfma_form(p):
     case = make_case((7, 5, 6), p, dt=1e-3, k_amplitude=0.5)
