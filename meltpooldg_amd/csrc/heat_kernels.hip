@@ -1765,26 +1765,32 @@ namespace mpdg
       return nb < kMaxResidentPerCu ? nb : kMaxResidentPerCu;
     }
 
-    __global__ void batch_pap_finalize_kernel(const int *block_off, const int *ops_solve, int nops,
-                                              BatchVmultPlan plan, const double *partials,
-                                              SolverScalars *scal)
+    // One workgroup per operator: thread t adds the operator's blocks t,
+    // t + 256, ... of each colour (colours in order, loads sixteen at a time),
+    // then the workgroup adds the threads' sums in a fixed tree, so the sum has
+    // a fixed order. A member with hundreds of blocks per colour took ~45 us
+    // with one thread walking its blocks; this is one pass of loads.
+    __global__ void __launch_bounds__(256)
+      batch_pap_finalize_kernel(const int *block_off, const int *ops_solve, int nops,
+                                BatchVmultPlan plan, const double *partials, SolverScalars *scal)
     {
-      // one workgroup per operator, one thread: fixed order over colours and blocks
+      __shared__ double red[4];
       const int o = blockIdx.x;
-      if (threadIdx.x != 0)
-        return;
       SolverScalars &sc = scal[ops_solve[o]];
       if (sc.done)
         return;
-      double pap = 0.0;
+      double v = 0.0;
       for (int c = 0; c < 8; ++c)
         {
           const int *bo = block_off + c * (nops + 1);
-          for (int b = bo[o]; b < bo[o + 1]; ++b)
-            pap += partials[plan.pofs[c] + b];
+          v += strided_sum(partials + plan.pofs[c] + bo[o], threadIdx.x, bo[o + 1] - bo[o], 256);
         }
-      sc.pap = pap;
-      sc.alpha = sc.rz / pap;
+      const double pap = block_sum<256>(v, red);
+      if (threadIdx.x == 0)
+        {
+          sc.pap = pap;
+          sc.alpha = sc.rz / pap;
+        }
     }
   } // namespace
 
@@ -1894,7 +1900,7 @@ namespace mpdg
                                        const BatchVmultPlan &plan, const double *partials,
                                        SolverScalars *scal, hipStream_t stream)
   {
-    batch_pap_finalize_kernel<<<nops, 64, 0, stream>>>(block_off, ops_solve, nops, plan, partials, scal);
+    batch_pap_finalize_kernel<<<nops, 256, 0, stream>>>(block_off, ops_solve, nops, plan, partials, scal);
     return hipGetLastError();
   }
 
