@@ -277,10 +277,34 @@ namespace mpdg
       return r;
     }
 
+    // Reference-cell 1D mass and stiffness matrices on the Gauss rule of the
+    // operator: Mhat[i][j] = sum_q w_q phi_i(x_q) phi_j(x_q) (kind 0) and
+    // Khat[i][j] = sum_q w_q phi_i'(x_q) phi_j'(x_q) (kind 1). The rule is exact
+    // for these products, so on an affine box cell with constant coefficients
+    // the cell matrix is the Kronecker sum of these factors.
+    template <int N>
+    constexpr Sq<N> gauss_product(const Tab<N> &t, int kind)
+    {
+      Sq<N> r{};
+      for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j)
+          {
+            double s = 0.0;
+            for (int q = 0; q < N; ++q)
+              s += t.w[q] * (kind == 0 ? t.S[q][i] * t.S[q][j] : t.D[q][i] * t.D[q][j]);
+            r.a[i][j] = s;
+          }
+      return r;
+    }
+
     template <int P>
     struct VTab
     {
       static constexpr Tab<P + 1> t = make_tab<P>();
+      static constexpr Sq<P + 1> Mhat = gauss_product<P + 1>(t, 0);
+      static constexpr Sq<P + 1> Khat = gauss_product<P + 1>(t, 1);
+      static constexpr EO<P + 1> Mk = split<P + 1>(symmetrize<P + 1>(Mhat.a, 1).a, false);
+      static constexpr EO<P + 1> Kk = split<P + 1>(symmetrize<P + 1>(Khat.a, 1).a, false);
       static constexpr Sq<P + 1> Ssym = symmetrize<P + 1>(t.S, 1);
       static constexpr Sq<P + 1> Csym = symmetrize<P + 1>(t.C, -1);
       static constexpr EO<P + 1> S = split<P + 1>(Ssym.a, false);

@@ -63,13 +63,15 @@ namespace mpdg
         wave_sync();
     }
 
-    // The four 1D matrices of the line contractions.
+    // The 1D matrices of the line contractions.
     enum LineMatrix
     {
       kLmS = 0,  // interpolate to Gauss points
       kLmC = 1,  // collocation derivative on Gauss points
       kLmCt = 2, // its transpose (flux back to values)
-      kLmSt = 3  // S^T (integrate)
+      kLmSt = 3, // S^T (integrate)
+      kLmM = 4,  // reference-cell 1D mass matrix (tensor-product form)
+      kLmK = 5   // reference-cell 1D stiffness matrix (tensor-product form)
     };
 
     template <int P, int M>
@@ -85,7 +87,9 @@ namespace mpdg
     {
       return M == kLmS ? ct::VTab<P>::S :
              M == kLmC ? ct::VTab<P>::C :
-             M == kLmCt ? ct::VTab<P>::Ct : ct::VTab<P>::St;
+             M == kLmCt ? ct::VTab<P>::Ct :
+             M == kLmSt ? ct::VTab<P>::St :
+             M == kLmM ? ct::VTab<P>::Mk : ct::VTab<P>::Kk;
     }
 
     // MFMA form of a line contraction (FP32, 4 or more points): lane l owns
@@ -133,7 +137,7 @@ namespace mpdg
                                                const Number (&x)[P + 1], Number (&y)[P + 1])
     {
       constexpr int N = P + 1;
-      if constexpr (MF)
+      if constexpr (MF && M <= kLmSt)
         {
           static_assert(sizeof(Number) == 4 && N >= 4, "MFMA line form: FP32, 4+ points");
           typedef float v4f __attribute__((ext_vector_type(4)));
@@ -158,7 +162,7 @@ namespace mpdg
       else
         {
           constexpr ct::EO<N> T = line_eo<P, M>();
-          eo_apply<N, (M == kLmS || M == kLmSt) ? 1 : -1>(T, x, y);
+          eo_apply<N, (M == kLmC || M == kLmCt) ? -1 : 1>(T, x, y);
         }
     }
 
@@ -481,6 +485,12 @@ namespace mpdg
       constexpr int THREADS = 64 * WPB;
       constexpr bool H2 = COEF == kCoefHeaviside;
       constexpr bool SINE = COEF == kCoefSine;
+      // Constant coefficients and no velocity: the cell matrix is
+      //   mass Mh(x)Mh(x)Mh + k sum_d kf_d (Kh in direction d, Mh in the others)
+      // (Mh, Kh: reference 1D mass and stiffness on the Gauss rule, which is
+      // exact for them), applied as 7 nodal line contractions in 4 line phases
+      // instead of 12 contractions and the quadrature-point work in 9.
+      constexpr bool KRON = COEF == kCoefConstant && !VEL && !MF;
       constexpr bool CD = ORD; // agent-coherent dst accesses
       static_assert(SPREAD ? (LT <= THREADS && LT > THREADS - NL)
                            : (LT <= 64 * R && LT > 64 * (R - 1)),
@@ -617,10 +627,27 @@ namespace mpdg
             {
               Number *Al = A + cb[r] * STR;
               Number t1[N];
-              line_apply<P, kLmS, MF>(LC, u[r], t1);
+              if constexpr (KRON)
+                {
+                  // Mh u into A, Kh u into B (x-lines)
+                  Number *Bl = B + cb[r] * STR;
+                  Number t2[N];
+                  line_apply<P, kLmM, MF>(LC, u[r], t1);
+                  line_apply<P, kLmK, MF>(LC, u[r], t2);
 #pragma unroll
-              for (int q = 0; q < N; ++q)
-                Al[(s1v[r] * N + s0v[r]) * N + q] = t1[q];
+                  for (int q = 0; q < N; ++q)
+                    {
+                      Al[(s1v[r] * N + s0v[r]) * N + q] = t1[q];
+                      Bl[(s1v[r] * N + s0v[r]) * N + q] = t2[q];
+                    }
+                }
+              else
+                {
+                  line_apply<P, kLmS, MF>(LC, u[r], t1);
+#pragma unroll
+                  for (int q = 0; q < N; ++q)
+                    Al[(s1v[r] * N + s0v[r]) * N + q] = t1[q];
+                }
               if constexpr (H2)
                 {
                   Number *KBl = KB + cb[r] * STR;
@@ -690,6 +717,35 @@ namespace mpdg
           phase_sync<SPREAD>();
 
           // ---- y-lines (qx = s0, z = s1): interpolate in y ------------------------
+          if constexpr (KRON)
+            {
+              // y-lines (x = s0, z = s1): A <- mass Mh t + k kf_x Mh s + k kf_y Kh t,
+              // B <- Mh t (t = Mh_x u, s = Kh_x u)
+              const Number c0 = prm.k0 * prm.kf[0], c1 = prm.k0 * prm.kf[1];
+#pragma unroll
+              for (int r = 0; r < RR; ++r)
+                {
+                  Number *Al = A + cb[r] * STR;
+                  Number *Bl = B + cb[r] * STR;
+                  Number tv[N], sv2[N], mt[N], ms[N], kt[N];
+#pragma unroll
+                  for (int b = 0; b < N; ++b)
+                    {
+                      tv[b] = Al[(s1v[r] * N + b) * N + s0v[r]];
+                      sv2[b] = Bl[(s1v[r] * N + b) * N + s0v[r]];
+                    }
+                  line_apply<P, kLmM, MF>(LC, tv, mt);
+                  line_apply<P, kLmM, MF>(LC, sv2, ms);
+                  line_apply<P, kLmK, MF>(LC, tv, kt);
+#pragma unroll
+                  for (int b = 0; b < N; ++b)
+                    {
+                      Al[(s1v[r] * N + b) * N + s0v[r]] = prm.mass * mt[b] + c0 * ms[b] + c1 * kt[b];
+                      Bl[(s1v[r] * N + b) * N + s0v[r]] = mt[b];
+                    }
+                }
+            }
+          else
 #pragma unroll
           for (int r = 0; r < RR; ++r)
             {
@@ -721,6 +777,30 @@ namespace mpdg
           //   Vz = rho_cp/dt w3 U + C^T (k w3 kf_z dU/dz) (+ rho_cp w3 u_z dU/dz),
           // so only Vz stays in registers until the x- and y-fluxes are added.
           Number Vz[RR][N];
+          if constexpr (KRON)
+            {
+              // z-lines (x = s0, y = s1): A <- Mh_z A + k kf_z Kh_z B, the cell's result
+              const Number c2 = prm.k0 * prm.kf[2];
+#pragma unroll
+              for (int r = 0; r < RR; ++r)
+                {
+                  Number *Al = A + cb[r] * STR;
+                  const Number *Bl = B + cb[r] * STR;
+                  Number p1[N], p2[N], m1[N], k2[N];
+#pragma unroll
+                  for (int c = 0; c < N; ++c)
+                    {
+                      p1[c] = Al[(c * N + s1v[r]) * N + s0v[r]];
+                      p2[c] = Bl[(c * N + s1v[r]) * N + s0v[r]];
+                    }
+                  line_apply<P, kLmM, MF>(LC, p1, m1);
+                  line_apply<P, kLmK, MF>(LC, p2, k2);
+#pragma unroll
+                  for (int c = 0; c < N; ++c)
+                    Al[(c * N + s1v[r]) * N + s0v[r]] = m1[c] + c2 * k2[c];
+                }
+            }
+          else
 #pragma unroll
           for (int r = 0; r < RR; ++r)
             {
@@ -786,6 +866,8 @@ namespace mpdg
             }
           phase_sync<SPREAD>();
 
+          if constexpr (!KRON)
+            {
           // ---- x-lines (qy = s0, qz = s1): x-gradient flux --------------------
 #pragma unroll
           for (int r = 0; r < RR; ++r)
@@ -896,6 +978,7 @@ namespace mpdg
                 Al[(s1v[r] * N + b) * N + s0v[r]] = o[b];
             }
           phase_sync<SPREAD>();
+            }
 
           // ---- x-lines (b = s0, c = s1): integrate x, publish right faces ---------
           Number Y[RR][N];
@@ -907,7 +990,14 @@ namespace mpdg
 #pragma unroll
               for (int q = 0; q < N; ++q)
                 v[q] = Al[(s1v[r] * N + s0v[r]) * N + q];
-              line_apply<P, kLmSt, MF>(LC, v, Y[r]);
+              if constexpr (KRON)
+                {
+#pragma unroll
+                  for (int a = 0; a < N; ++a)
+                    Y[r][a] = v[a];
+                }
+              else
+                line_apply<P, kLmSt, MF>(LC, v, Y[r]);
               if (lv[r])
 #pragma unroll
                 for (int a = 0; a < N; ++a)
