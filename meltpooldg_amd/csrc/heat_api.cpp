@@ -233,18 +233,19 @@ namespace
 
   // Default launch tuning per degree and precision: the autotuner's picks on
   // MI355X at the benchmark-like sizes (unit cube, sine k, default materials;
-  // GDoF/s default-before -> default-now in the comments). `lean`: no
-  // heaviside and no velocity (the operators whose FP32 degree-1..2 and FP64
-  // degree-3 launches take the two-slot shapes).
-  mpdg_tuning default_tuning(int degree, bool fp32, int64_t cells, bool lean)
+  // measured GDoF/s in the comments); mpdg_heat_autotune refines it for a
+  // given operator.
+  mpdg_tuning default_tuning(int degree, bool fp32, int64_t cells)
   {
     mpdg_tuning t{};
     t.variant = MPDG_VMULT_LINE;
     t.tile_log2 = 4;
     // the cell form needs about a million cells to fill the chip in each of
-    // its eight colour passes (64 cells per wavefront); the lean FP32 line
-    // form (32 cells per wavefront) beats it (128^3: 41.8 vs 37.7)
-    if (degree == 1 && cells >= (int64_t(1) << 20) && !(fp32 && lean))
+    // its eight colour passes (64 cells per wavefront). The lean FP32 line
+    // form (128-cell row segments) beats it only where the rows fill whole
+    // segments (128^3: 41.8 vs 37.7 GDoF/s; 160^3: 31.5 vs 45.0; 200^3,
+    // constant k: 48.8 vs 56.3), so the autotuner decides that case.
+    if (degree == 1 && cells >= (int64_t(1) << 20))
       {
         t.variant = MPDG_VMULT_CELL;
         t.tile_log2 = 5;
@@ -254,13 +255,7 @@ namespace
       }
     switch (degree)
       {
-        case 1:
-          {
-            const bool big_lean = fp32 && lean && cells >= (int64_t(1) << 20);
-            t.z_layers = fp32 && !big_lean ? 4 : 2;
-            t.blocks_per_cu = fp32 ? (big_lean ? 8 : 4) : 0;
-            break;
-          }
+        case 1: t.z_layers = fp32 ? 4 : 2; t.blocks_per_cu = fp32 ? 4 : 0; break;
         case 2:
           // FP32 (two-slot shape): one-layer chunks measured 72.6 against 68.3
           // GDoF/s at 96^3, two-layer ones 108.6 against 100.1 at 192^3
@@ -1308,9 +1303,8 @@ mpdg_status mpdg_heat_create(const mpdg_heat_desc *desc, mpdg_heat **out)
     return fail(MPDG_ERR_ARGUMENT, "out of host memory");
   H->desc = d;
   const int64_t cells = (int64_t)d.n_cells[0] * d.n_cells[1] * d.n_cells[2];
-  const bool lean = !uses_heaviside(d) && d.velocity == MPDG_VELOCITY_NONE;
-  H->tune[MPDG_FP64] = default_tuning(d.degree, false, cells, lean);
-  H->tune[MPDG_FP32] = default_tuning(d.degree, true, cells, lean);
+  H->tune[MPDG_FP64] = default_tuning(d.degree, false, cells);
+  H->tune[MPDG_FP32] = default_tuning(d.degree, true, cells);
   H->stream = (hipStream_t)d.stream;
   H->N = N;
   H->N_free = Nf;
