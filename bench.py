@@ -387,6 +387,16 @@ def main():
         solve = time_solves(op, b, torch, LinearSolver, LinearSolverData)
         solve["fp32_tuning"] = fp32_tuning
 
+    # the degree-2 FP32 operator of the low-degree HBM target, 96^3 cells
+    # (bench.py --mode q2 has the FP64 rows and 192^3)
+    q2 = None
+    if not args.no_solve:
+        import bench_configs
+        del op
+        q2 = [{k: r[k] for k in ("k", "GDoF/s", "algorithmic_GB/s", "hbm_frac", "valu_frac")}
+              for r in bench_configs.q2(argparse.Namespace(cells="96"), torch)["rows"]
+              if r["precision"] == "fp32"]
+
     cpu = None if args.no_cpu else cpu_baseline()
 
     line = {
@@ -409,6 +419,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "solve_time_to_1e-8": solve,
+        "q2_fp32_96": q2,
     }
     print(json.dumps(line))
 
