@@ -271,7 +271,9 @@ def test_gmres_convection_matches_oracle(restart):
     data.gmres_restart = restart
     it = LinearSolver.solve(op, x, _t(b), data)
     print("gmres", restart, it, "oracle", ref["iterations"], flush=True)
-    assert abs(it - ref["iterations"]) <= 1
+    # both sides are deterministic (fixed reduction orders), so the counts are
+    # compared exactly: 35 and 42 for restart 30 and 4
+    assert it == ref["iterations"]
     assert rel_err(_np(x), ref["x"]) < 1e-8
 
 
