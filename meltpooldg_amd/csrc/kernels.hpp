@@ -117,6 +117,7 @@ namespace mpdg
   {
     double rz, rz_old, pap, alpha, beta, rr, res, tol;
     double res0;
+    double alpha_prev; // alpha of the first iteration of a pair (paired x update)
     int iteration, done, status, max_it;
     unsigned counter;
     unsigned pad;
@@ -376,14 +377,27 @@ namespace mpdg
   hipError_t launch_cg_init(const double *b, const Number *ax, Number *r, Number *p,
                             const InvT *inv_diag, int64_t n, double rel_tol, double abs_tol,
                             int max_it, double *partials, SolverScalars *scal, hipStream_t s);
+  // x update of a CG iteration: every iteration (x += alpha p); or in pairs:
+  // none on the first iteration of a pair (its alpha kept), both on the
+  // second, x = (x + alpha_prev p_prev) + alpha p, bitwise the same x.
+  constexpr int kXEvery = 0, kXDefer = 1, kXPair = 2;
+
   template <typename Number, typename InvT = Number>
   hipError_t launch_cg_update(Number *x, Number *r, const Number *p, const Number *q,
                               const InvT *inv_diag, int64_t n, double *partials,
                               SolverScalars *scal, double *history, int history_len,
-                              hipStream_t s);
+                              hipStream_t s, int x_mode = kXEvery,
+                              const Number *p_prev = nullptr);
+  // p = D r + beta p_in (p_in = nullptr: in place)
   template <typename Number, typename InvT = Number>
   hipError_t launch_cg_direction(Number *p, const Number *r, const InvT *inv_diag, int64_t n,
-                                 const SolverScalars *scal, hipStream_t s);
+                                 const SolverScalars *scal, hipStream_t s,
+                                 const Number *p_in = nullptr);
+  // x += alpha p with scal->alpha, run after a solve that stopped on the
+  // first iteration of a pair
+  template <typename Number>
+  hipError_t launch_cg_flush(Number *x, const Number *p, int64_t n, const SolverScalars *scal,
+                             hipStream_t s);
 
   // Mixed precision helpers.
   hipError_t launch_residual(double *r, const double *b, const double *ax, int64_t n,

@@ -175,21 +175,35 @@ namespace mpdg
     }
 
     // x += alpha p, r -= alpha q, rr = r.r, rz = r.(D r); convergence check.
-    template <typename Number, typename InvT>
+    //
+    // XM selects the x update of a pair of iterations (kXEvery, kXDefer,
+    // kXPair): every iteration (x += alpha p), none with alpha kept in
+    // scal->alpha_prev (the first of the pair), or both directions of the pair,
+    // x = (x + alpha_prev p_prev) + alpha p (the second). The pair form
+    // performs the same two roundings in the same order as two single
+    // updates, so x is bitwise the same, and x is read and written every
+    // other iteration only.
+    template <typename Number, typename InvT, int XM>
     __global__ void __launch_bounds__(T)
-      cg_update_kernel(Number *x, Number *r, const Number *p, const Number *q, const InvT *inv,
-                       int64_t n, double *partials, SolverScalars *scal, double *history,
-                       int history_len)
+      cg_update_kernel(Number *x, Number *r, const Number *p, const Number *p_prev,
+                       const Number *q, const InvT *inv, int64_t n, double *partials,
+                       SolverScalars *scal, double *history, int history_len)
     {
       if (scal->done)
         return;
       const double alpha = scal->alpha;
       const Number a = (Number)alpha;
+      const Number a0 = XM == kXPair ? (Number)scal->alpha_prev : Number(0);
       double v[2] = {0.0, 0.0};
       for (int64_t i = blockIdx.x * (int64_t)T + threadIdx.x; i < n; i += (int64_t)gridDim.x * T)
         {
-          const Number pi = p[i];
-          x[i] = x[i] + a * pi;
+          if constexpr (XM == kXEvery)
+            x[i] = x[i] + a * p[i];
+          else if constexpr (XM == kXPair)
+            {
+              Number xi = x[i] + a0 * p_prev[i];
+              x[i] = xi + a * p[i];
+            }
           const Number ri = r[i] - a * q[i];
           r[i] = ri;
           v[0] += (double)ri * (double)ri;
@@ -208,6 +222,8 @@ namespace mpdg
           scal->beta = tot[1] / scal->rz;
           scal->rz_old = scal->rz;
           scal->rz = tot[1];
+          if (XM == kXDefer)
+            scal->alpha_prev = alpha;
           if (res <= scal->tol)
             {
               scal->done = 1;
@@ -221,17 +237,28 @@ namespace mpdg
         }
     }
 
-    // p = D r + beta p
+    // p = D r + beta p_in (p_in may be p)
     template <typename Number, typename InvT>
     __global__ void __launch_bounds__(T)
-      cg_direction_kernel(Number *p, const Number *r, const InvT *inv, int64_t n,
-                          const SolverScalars *scal)
+      cg_direction_kernel(Number *p, const Number *p_in, const Number *r, const InvT *inv,
+                          int64_t n, const SolverScalars *scal)
     {
       if (scal->done)
         return;
       const Number beta = (Number)scal->beta;
       for (int64_t i = blockIdx.x * (int64_t)T + threadIdx.x; i < n; i += (int64_t)gridDim.x * T)
-        p[i] = r[i] * (Number)inv_value(inv, i) + beta * p[i];
+        p[i] = r[i] * (Number)inv_value(inv, i) + beta * p_in[i];
+    }
+
+    // The x update left over when a solve stops on the first iteration of a
+    // pair: x += alpha p (the expression of the single update)
+    template <typename Number>
+    __global__ void __launch_bounds__(T)
+      cg_flush_kernel(Number *x, const Number *p, int64_t n, const SolverScalars *scal)
+    {
+      const Number a = (Number)scal->alpha;
+      for (int64_t i = blockIdx.x * (int64_t)T + threadIdx.x; i < n; i += (int64_t)gridDim.x * T)
+        x[i] = x[i] + a * p[i];
     }
 
     // 16-byte vector of Number (2 doubles or 4 floats) for the wide forms of
@@ -271,32 +298,48 @@ namespace mpdg
 
     // cg_update_kernel over 16-byte vectors: thread t of the fixed grid owns
     // vectors t, t + grid, ... and (for the tail n % W) entry n - n % W + t.
-    template <typename Number, typename InvT>
+    template <typename Number, typename InvT, int XM>
     __global__ void __launch_bounds__(T)
-      cg_update_wide_kernel(Number *x, Number *r, const Number *p, const Number *q,
-                            const InvT *inv, int64_t n, double *partials, SolverScalars *scal,
-                            double *history, int history_len)
+      cg_update_wide_kernel(Number *x, Number *r, const Number *p, const Number *p_prev,
+                            const Number *q, const InvT *inv, int64_t n, double *partials,
+                            SolverScalars *scal, double *history, int history_len)
     {
       if (scal->done)
         return;
       using V = typename Wide<Number>::type;
       constexpr int W = Wide<Number>::W;
-      const Number a = (Number)scal->alpha;
+      const double alpha = scal->alpha;
+      const Number a = (Number)alpha;
+      const Number a0 = XM == kXPair ? (Number)scal->alpha_prev : Number(0);
       const int64_t nv = n / W;
       V *X = reinterpret_cast<V *>(x);
       V *Rv = reinterpret_cast<V *>(r);
       const V *Pv = reinterpret_cast<const V *>(p);
+      const V *P0 = reinterpret_cast<const V *>(p_prev);
       const V *Qv = reinterpret_cast<const V *>(q);
       double v[2] = {0.0, 0.0};
       const int64_t t0 = blockIdx.x * (int64_t)T + threadIdx.x;
       const int64_t stride = (int64_t)gridDim.x * T;
       for (int64_t j = t0; j < nv; j += stride)
         {
-          const V pv = Pv[j], qv = Qv[j], iv = inv_wide(inv, j);
-          V xv = X[j], rv = Rv[j];
-          xv = xv + a * pv;
+          const V qv = Qv[j], iv = inv_wide(inv, j);
+          V rv = Rv[j];
+          if constexpr (XM == kXEvery)
+            {
+              const V pv = Pv[j];
+              V xv = X[j];
+              xv = xv + a * pv;
+              X[j] = xv;
+            }
+          else if constexpr (XM == kXPair)
+            {
+              const V pv = Pv[j], p0 = P0[j];
+              V xv = X[j];
+              xv = xv + a0 * p0;
+              xv = xv + a * pv;
+              X[j] = xv;
+            }
           rv = rv - a * qv;
-          X[j] = xv;
           Rv[j] = rv;
 #pragma unroll
           for (int k = 0; k < W; ++k)
@@ -308,7 +351,13 @@ namespace mpdg
       const int64_t i = nv * W + t0;
       if (i < n)
         {
-          x[i] = x[i] + a * p[i];
+          if constexpr (XM == kXEvery)
+            x[i] = x[i] + a * p[i];
+          else if constexpr (XM == kXPair)
+            {
+              Number xi = x[i] + a0 * p_prev[i];
+              x[i] = xi + a * p[i];
+            }
           const Number ri = r[i] - a * q[i];
           r[i] = ri;
           v[0] += (double)ri * (double)ri;
@@ -327,6 +376,8 @@ namespace mpdg
           scal->beta = tot[1] / scal->rz;
           scal->rz_old = scal->rz;
           scal->rz = tot[1];
+          if (XM == kXDefer)
+            scal->alpha_prev = alpha;
           if (res <= scal->tol)
             {
               scal->done = 1;
@@ -342,8 +393,8 @@ namespace mpdg
 
     template <typename Number, typename InvT>
     __global__ void __launch_bounds__(T)
-      cg_direction_wide_kernel(Number *p, const Number *r, const InvT *inv, int64_t n,
-                               const SolverScalars *scal)
+      cg_direction_wide_kernel(Number *p, const Number *p_in, const Number *r, const InvT *inv,
+                               int64_t n, const SolverScalars *scal)
     {
       if (scal->done)
         return;
@@ -352,13 +403,14 @@ namespace mpdg
       const Number beta = (Number)scal->beta;
       const int64_t nv = n / W;
       V *Pv = reinterpret_cast<V *>(p);
+      const V *Pi = reinterpret_cast<const V *>(p_in);
       const V *Rv = reinterpret_cast<const V *>(r);
       const int64_t t0 = blockIdx.x * (int64_t)T + threadIdx.x;
       for (int64_t j = t0; j < nv; j += (int64_t)gridDim.x * T)
-        Pv[j] = Rv[j] * inv_wide(inv, j) + beta * Pv[j];
+        Pv[j] = Rv[j] * inv_wide(inv, j) + beta * Pi[j];
       const int64_t i = nv * W + t0;
       if (i < n)
-        p[i] = r[i] * (Number)inv_value(inv, i) + beta * p[i];
+        p[i] = r[i] * (Number)inv_value(inv, i) + beta * p_in[i];
     }
 
     bool aligned16(const void *a, const void *b = nullptr, const void *c = nullptr,
@@ -933,30 +985,53 @@ namespace mpdg
   template <typename Number, typename InvT>
   hipError_t launch_cg_update(Number *x, Number *r, const Number *p, const Number *q,
                               const InvT *inv_diag, int64_t n, double *partials,
-                              SolverScalars *scal, double *history, int history_len, hipStream_t s)
+                              SolverScalars *scal, double *history, int history_len, hipStream_t s,
+                              int x_mode, const Number *p_prev)
   {
-    if (aligned16(x, r, p, q, inv_diag))
-      cg_update_wide_kernel<Number, InvT><<<kVecBlocks, T, 0, s>>>(x, r, p, q, inv_diag, n,
-                                                                   partials, scal, history,
-                                                                   history_len);
+    if (x_mode == kXPair && p_prev == nullptr)
+      return hipErrorInvalidValue;
+    const bool wide = aligned16(x, r, p, q, inv_diag) && aligned16(p_prev);
+    auto go = [&](auto xm) {
+      constexpr int XM = decltype(xm)::value;
+      if (wide)
+        cg_update_wide_kernel<Number, InvT, XM><<<kVecBlocks, T, 0, s>>>(
+          x, r, p, p_prev, q, inv_diag, n, partials, scal, history, history_len);
+      else
+        cg_update_kernel<Number, InvT, XM><<<kVecBlocks, T, 0, s>>>(
+          x, r, p, p_prev, q, inv_diag, n, partials, scal, history, history_len);
+    };
+    if (x_mode == kXDefer)
+      go(std::integral_constant<int, kXDefer>());
+    else if (x_mode == kXPair)
+      go(std::integral_constant<int, kXPair>());
     else
-      cg_update_kernel<Number, InvT><<<kVecBlocks, T, 0, s>>>(x, r, p, q, inv_diag, n, partials,
-                                                              scal, history, history_len);
+      go(std::integral_constant<int, kXEvery>());
+    return hipGetLastError();
+  }
+
+  template <typename Number>
+  hipError_t launch_cg_flush(Number *x, const Number *p, int64_t n, const SolverScalars *scal,
+                             hipStream_t s)
+  {
+    cg_flush_kernel<Number><<<stream_grid(n), T, 0, s>>>(x, p, n, scal);
     return hipGetLastError();
   }
 
   template <typename Number, typename InvT>
   hipError_t launch_cg_direction(Number *p, const Number *r, const InvT *inv_diag, int64_t n,
-                                 const SolverScalars *scal, hipStream_t s)
+                                 const SolverScalars *scal, hipStream_t s, const Number *p_in)
   {
-    if (aligned16(p, r, inv_diag))
+    if (p_in == nullptr)
+      p_in = p;
+    if (aligned16(p, r, inv_diag, p_in))
       {
         const int64_t nv = (n + Wide<Number>::W - 1) / Wide<Number>::W;
-        cg_direction_wide_kernel<Number, InvT><<<stream_grid(nv), T, 0, s>>>(p, r, inv_diag, n,
-                                                                             scal);
+        cg_direction_wide_kernel<Number, InvT><<<stream_grid(nv), T, 0, s>>>(p, p_in, r, inv_diag,
+                                                                             n, scal);
       }
     else
-      cg_direction_kernel<Number, InvT><<<stream_grid(n), T, 0, s>>>(p, r, inv_diag, n, scal);
+      cg_direction_kernel<Number, InvT><<<stream_grid(n), T, 0, s>>>(p, p_in, r, inv_diag, n,
+                                                                     scal);
     return hipGetLastError();
   }
 
@@ -1327,14 +1402,19 @@ namespace mpdg
                                             SolverScalars *, hipStream_t);
   template hipError_t launch_cg_update<double>(double *, double *, const double *, const double *,
                                                const double *, int64_t, double *, SolverScalars *,
-                                               double *, int, hipStream_t);
+                                               double *, int, hipStream_t, int, const double *);
   template hipError_t launch_cg_update<float>(float *, float *, const float *, const float *,
                                               const float *, int64_t, double *, SolverScalars *,
-                                              double *, int, hipStream_t);
+                                              double *, int, hipStream_t, int, const float *);
   template hipError_t launch_cg_direction<double>(double *, const double *, const double *, int64_t,
-                                                  const SolverScalars *, hipStream_t);
+                                                  const SolverScalars *, hipStream_t,
+                                                  const double *);
   template hipError_t launch_cg_direction<float>(float *, const float *, const float *, int64_t,
-                                                 const SolverScalars *, hipStream_t);
+                                                 const SolverScalars *, hipStream_t, const float *);
+  template hipError_t launch_cg_flush<double>(double *, const double *, int64_t,
+                                              const SolverScalars *, hipStream_t);
+  template hipError_t launch_cg_flush<float>(float *, const float *, int64_t,
+                                             const SolverScalars *, hipStream_t);
   template hipError_t launch_cg_init<float, uint16_t>(const double *, const float *, float *,
                                                       float *, const uint16_t *, int64_t, double,
                                                       double, int, double *, SolverScalars *,
@@ -1342,8 +1422,8 @@ namespace mpdg
   template hipError_t launch_cg_update<float, uint16_t>(float *, float *, const float *,
                                                         const float *, const uint16_t *, int64_t,
                                                         double *, SolverScalars *, double *, int,
-                                                        hipStream_t);
+                                                        hipStream_t, int, const float *);
   template hipError_t launch_cg_direction<float, uint16_t>(float *, const float *, const uint16_t *,
                                                            int64_t, const SolverScalars *,
-                                                           hipStream_t);
+                                                           hipStream_t, const float *);
 } // namespace mpdg
